@@ -1,0 +1,1095 @@
+/*
+ * CPU oracle for the DiskANN search-and-quantize path -- TEST INFRASTRUCTURE ONLY.
+ * Plain C restatement of the reference algorithm; see oracle.h for the parity status of
+ * every function. Built with -ffp-contract=off so that the only fused operations are the
+ * explicit fmaf() calls that mirror the reference's mul_add_simd.
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* ------------------------------------------------------------------------------------ */
+/* Distances                                                                            */
+/* ------------------------------------------------------------------------------------ */
+
+/* f32x8::sum_tree, diskann-wide/src/arch/x86_64/v3/f32x8_.rs:187-213:
+ * ((x0+x4)+(x2+x6)) + ((x1+x5)+(x3+x7)). */
+static float sum_tree8(const float *t) {
+    float a0 = t[0] + t[4], a1 = t[1] + t[5], a2 = t[2] + t[6], a3 = t[3] + t[7];
+    float b0 = a0 + a2, b1 = a1 + a3;
+    return b0 + b1;
+}
+
+/* simd_op with Strategy4x1 over f32x8 (diskann-vector/src/distance/simd.rs:686-749,
+ * 321-363; L2 schema :817-845, IP schema :1588-1600). The main loop and the full-width
+ * epilogues together place element e (e < len - len%8) into partial accumulator e % 32,
+ * in increasing e; the accumulators combine as (s0+s1)+(s2+s3) lane-wise; the masked
+ * remainder (zero-filled) is accumulated into the combined vector; then sum_tree. */
+static float simd_4x1(int ip, const float *x, const float *y, int64_t len) {
+    float acc[32];
+    for (int p = 0; p < 32; ++p) acc[p] = 0.0f;
+    int64_t d8 = len - (len % 8);
+    int64_t e = 0;
+    for (; e + 32 <= d8; e += 32) {
+        for (int p = 0; p < 32; ++p) {
+            float a = x[e + p], b = y[e + p];
+            if (ip) {
+                acc[p] = fmaf(a, b, acc[p]);
+            } else {
+                float c = a - b;
+                acc[p] = fmaf(c, c, acc[p]);
+            }
+        }
+    }
+    for (int p = 0; e + p < d8; ++p) {
+        float a = x[e + p], b = y[e + p];
+        if (ip) {
+            acc[p] = fmaf(a, b, acc[p]);
+        } else {
+            float c = a - b;
+            acc[p] = fmaf(c, c, acc[p]);
+        }
+    }
+    float t[8];
+    for (int l = 0; l < 8; ++l) t[l] = (acc[l] + acc[8 + l]) + (acc[16 + l] + acc[24 + l]);
+    int64_t r = len % 8;
+    if (r != 0) {
+        for (int l = 0; l < 8; ++l) {
+            float a = l < r ? x[d8 + l] : 0.0f;
+            float b = l < r ? y[d8 + l] : 0.0f;
+            if (ip) {
+                t[l] = fmaf(a, b, t[l]);
+            } else {
+                float c = a - b;
+                t[l] = fmaf(c, c, t[l]);
+            }
+        }
+    }
+    return sum_tree8(t);
+}
+
+/* CosineStateless over f32x8 with Strategy2x4 (simd.rs:2281-2461, 408-480): element e
+ * (e < len - len%8) goes to partial e % 16 for each of normx, normy, xy; combine s0+s1;
+ * masked remainder; FullCosineAccumulator::sum with subnormal-norm guard and clamp. */
+static float cosine_v3(const float *x, const float *y, int64_t len) {
+    float nx[16], ny[16], xy[16];
+    for (int p = 0; p < 16; ++p) nx[p] = ny[p] = xy[p] = 0.0f;
+    int64_t d8 = len - (len % 8);
+    for (int64_t e = 0; e < d8; ++e) {
+        int p = (int)(e % 16);
+        float a = x[e], b = y[e];
+        nx[p] = fmaf(a, a, nx[p]);
+        ny[p] = fmaf(b, b, ny[p]);
+        xy[p] = fmaf(a, b, xy[p]);
+    }
+    float cx[8], cy[8], cxy[8];
+    for (int l = 0; l < 8; ++l) {
+        cx[l] = nx[l] + nx[8 + l];
+        cy[l] = ny[l] + ny[8 + l];
+        cxy[l] = xy[l] + xy[8 + l];
+    }
+    int64_t r = len % 8;
+    if (r != 0) {
+        for (int l = 0; l < 8; ++l) {
+            float a = l < r ? x[d8 + l] : 0.0f;
+            float b = l < r ? y[d8 + l] : 0.0f;
+            cx[l] = fmaf(a, a, cx[l]);
+            cy[l] = fmaf(b, b, cy[l]);
+            cxy[l] = fmaf(a, b, cxy[l]);
+        }
+    }
+    float normx = sum_tree8(cx);
+    float normy = sum_tree8(cy);
+    float denom = sqrtf(normx) * sqrtf(normy);
+    float prod = sum_tree8(cxy);
+    float sim;
+    if (normx < FLT_MIN || normy < FLT_MIN) {
+        sim = 0.0f;
+    } else {
+        float v = prod / denom;
+        sim = fmaxf(-1.0f, fminf(1.0f, v));
+    }
+    return 1.0f - sim;
+}
+
+/* PostOps: diskann-vector/src/distance/implementations.rs:215-405. For f32 data the
+ * DistanceProvider maps CosineNormalized to the full Cosine functor
+ * (diskann-vector/src/distance/distance_provider.rs:288-291). */
+float dko_distance(int metric, const float *q, const float *x, int64_t dim) {
+    switch (metric) {
+    case DKO_L2: return simd_4x1(0, q, x, dim);
+    case DKO_INNER_PRODUCT: return -simd_4x1(1, q, x, dim);
+    case DKO_COSINE:
+    case DKO_COSINE_NORMALIZED: return cosine_v3(q, x, dim);
+    default: return NAN;
+    }
+}
+
+/* square_norm, diskann-quantization/src/algorithms/kmeans/common.rs:9-65. */
+float dko_square_norm(const float *x, int64_t len) {
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t i = 0;
+    if (i + 32 <= len) {
+        float a[32];
+        for (int p = 0; p < 32; ++p) a[p] = 0.0f;
+        while (i + 32 <= len) {
+            for (int p = 0; p < 32; ++p) a[p] = fmaf(x[i + p], x[i + p], a[p]);
+            i += 32;
+        }
+        for (int l = 0; l < 8; ++l) s[l] = (a[l] + a[8 + l]) + (a[16 + l] + a[24 + l]);
+    }
+    while (i + 8 <= len) {
+        for (int l = 0; l < 8; ++l) s[l] = fmaf(x[i + l], x[i + l], s[l]);
+        i += 8;
+    }
+    int64_t r = len - i;
+    if (r != 0) {
+        for (int l = 0; l < 8; ++l) {
+            float v = l < r ? x[i + l] : 0.0f;
+            s[l] = fmaf(v, v, s[l]);
+        }
+    }
+    return sum_tree8(s);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* PQ                                                                                   */
+/* ------------------------------------------------------------------------------------ */
+
+/* ChunkOffsets::partition, diskann-quantization/src/views.rs:233-302: the first
+ * dim % M chunks get one extra dimension. */
+void dko_chunk_offsets(int64_t dim, int64_t nchunks, int64_t *offsets) {
+    int64_t off = 0;
+    offsets[0] = 0;
+    for (int64_t c = 0; c < nchunks; ++c) {
+        off += dim / nchunks;
+        if (c < dim % nchunks) off += 1;
+        offsets[c + 1] = off;
+    }
+}
+
+/* populate_chunk_distances_impl, diskann-providers/src/model/pq/fixed_chunk_pq_table.rs:
+ * 149-184: LUT[m*K + c] = SquaredL2(q_m, pivot_c,m) (TableL2) or InnerProduct (TableIP,
+ * negated). CosineNormalized uses the L2 table (pq/distance/dynamic.rs:73-81). */
+void dko_pq_lut(int metric, const float *pivots, int64_t K, int64_t dim,
+                const int64_t *offsets, int64_t M, const float *query, float *lut) {
+    int ip = metric == DKO_INNER_PRODUCT;
+    for (int64_t c = 0; c < K; ++c) {
+        for (int64_t m = 0; m < M; ++m) {
+            int64_t s = offsets[m], e = offsets[m + 1];
+            float v = simd_4x1(ip, query + s, pivots + c * dim + s, e - s);
+            lut[m * K + c] = ip ? -v : v;
+        }
+    }
+}
+
+/* pq_dist_lookup_single, fixed_chunk_pq_table.rs:79-95: sequential sum in chunk order. */
+float dko_pq_adc(const uint8_t *code, const float *lut, int64_t M, int64_t K) {
+    float acc = 0.0f;
+    for (int64_t m = 0; m < M; ++m) acc += lut[m * K + code[m]];
+    return acc;
+}
+
+/* BasicTable::compress_into, diskann-quantization/src/product/tables/basic.rs:161-193:
+ * per chunk argmin over pivots of SquaredL2 with strict <, first index wins. An
+ * infinite/NaN minimum is an error (returns the failing row + 1). */
+int dko_pq_encode(const float *pivots, int64_t K, int64_t dim, const int64_t *offsets,
+                  int64_t M, const float *x, int64_t n, uint8_t *codes) {
+    for (int64_t i = 0; i < n; ++i) {
+        const float *v = x + i * dim;
+        for (int64_t m = 0; m < M; ++m) {
+            int64_t s = offsets[m], e = offsets[m + 1];
+            float best = INFINITY;
+            int64_t arg = 0;
+            for (int64_t c = 0; c < K; ++c) {
+                float d = simd_4x1(0, v + s, pivots + c * dim + s, e - s);
+                if (d < best) {
+                    best = d;
+                    arg = c;
+                }
+            }
+            if (!isfinite(best)) return (int)(i + 1);
+            codes[i * M + m] = (uint8_t)arg;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Neighbor priority queue, diskann/src/neighbor/queue.rs:68-475                        */
+/* ------------------------------------------------------------------------------------ */
+
+typedef struct {
+    uint32_t *id;
+    float *d;
+    uint8_t *vis;
+    int64_t size, cap, cursor, search_l;
+} pqueue;
+
+static void q_init(pqueue *q, int64_t cap) {
+    q->id = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(cap + 1));
+    q->d = (float *)malloc(sizeof(float) * (size_t)(cap + 1));
+    q->vis = (uint8_t *)malloc((size_t)(cap + 1));
+    q->size = 0;
+    q->cap = cap;
+    q->cursor = 0;
+    q->search_l = cap;
+}
+static void q_free(pqueue *q) {
+    free(q->id);
+    free(q->d);
+    free(q->vis);
+}
+
+/* insert, queue.rs:130-171; lower bound = first index with d >= new.d (:229-280). */
+static void q_insert(pqueue *q, uint32_t id, float d) {
+    if (isnan(d)) return;
+    if (q->size == q->cap && q->d[q->size - 1] < d) return;
+    int64_t idx = 0;
+    if (q->size > 0) {
+        idx = q->size;
+        for (int64_t i = 0; i < q->size; ++i) {
+            if (q->d[i] >= d) {
+                idx = i;
+                break;
+            }
+        }
+    }
+    if (q->size == q->cap) q->size -= 1;
+    int64_t n = q->size - idx;
+    if (n > 0) {
+        memmove(q->id + idx + 1, q->id + idx, sizeof(uint32_t) * (size_t)n);
+        memmove(q->d + idx + 1, q->d + idx, sizeof(float) * (size_t)n);
+        memmove(q->vis + idx + 1, q->vis + idx, (size_t)n);
+    }
+    q->id[idx] = id;
+    q->d[idx] = d;
+    q->vis[idx] = 0;
+    q->size += 1;
+    if (idx < q->cursor) q->cursor = idx;
+}
+
+static int q_has_notvisited(const pqueue *q) {
+    int64_t lim = q->search_l < q->size ? q->search_l : q->size;
+    return q->cursor < lim;
+}
+
+/* closest_notvisited, queue.rs:297-313. */
+static int64_t q_closest_notvisited(pqueue *q) {
+    if (!q_has_notvisited(q)) return -1;
+    int64_t cur = q->cursor;
+    q->vis[cur] = 1;
+    q->cursor += 1;
+    while (q->cursor < q->size && q->vis[q->cursor]) q->cursor += 1;
+    return cur;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Visited set (exact membership; NotInMut, diskann/src/graph/glue.rs:524-561)          */
+/* ------------------------------------------------------------------------------------ */
+
+typedef struct {
+    uint32_t *slot; /* stores id+1; 0 = empty */
+    int64_t cap, n;
+} idset;
+
+static void set_init(idset *s, int64_t cap) {
+    int64_t c = 1024;
+    while (c < cap) c <<= 1;
+    s->slot = (uint32_t *)calloc((size_t)c, sizeof(uint32_t));
+    s->cap = c;
+    s->n = 0;
+}
+static void set_free(idset *s) { free(s->slot); }
+static inline uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+static int set_insert(idset *s, uint32_t id);
+static void set_grow(idset *s) {
+    uint32_t *old = s->slot;
+    int64_t oc = s->cap;
+    s->cap = oc * 2;
+    s->slot = (uint32_t *)calloc((size_t)s->cap, sizeof(uint32_t));
+    s->n = 0;
+    for (int64_t i = 0; i < oc; ++i)
+        if (old[i]) set_insert(s, old[i] - 1);
+    free(old);
+}
+/* returns 1 if newly inserted */
+static int set_insert(idset *s, uint32_t id) {
+    if (2 * (s->n + 1) > s->cap) set_grow(s);
+    uint64_t mask = (uint64_t)s->cap - 1;
+    uint64_t h = hash32(id) & mask;
+    for (;;) {
+        uint32_t v = s->slot[h];
+        if (v == 0) {
+            s->slot[h] = id + 1;
+            s->n++;
+            return 1;
+        }
+        if (v == id + 1) return 0;
+        h = (h + 1) & mask;
+    }
+}
+static void set_clear(idset *s) {
+    memset(s->slot, 0, sizeof(uint32_t) * (size_t)s->cap);
+    s->n = 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Search                                                                               */
+/* ------------------------------------------------------------------------------------ */
+
+int dko_knn_validate(int64_t k, int64_t L, int64_t bw) {
+    if (k == 0) return DKO_ERR_KZERO;
+    if (L == 0) return DKO_ERR_LZERO;
+    if (k > L) return DKO_ERR_L_LESS_THAN_K;
+    if (bw == 0) return DKO_ERR_BEAM_ZERO;
+    if (k < 0 || L < 0 || bw < 0) return DKO_ERR_ARG;
+    return DKO_OK;
+}
+
+typedef struct {
+    pqueue q;
+    idset vis;
+    uint32_t *beam;
+    uint32_t *nb_id;
+    float *nb_d;
+    float *lut;
+    int64_t nb_cap;
+} scratch;
+
+static void scratch_init(scratch *s, const dko_index *ix, int64_t L, int64_t bw) {
+    q_init(&s->q, L + ix->nstart);
+    set_init(&s->vis, 4096);
+    s->beam = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)bw);
+    s->nb_cap = bw * ix->max_degree + 1;
+    s->nb_id = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)s->nb_cap);
+    s->nb_d = (float *)malloc(sizeof(float) * (size_t)s->nb_cap);
+    s->lut = ix->M > 0 ? (float *)malloc(sizeof(float) * (size_t)(ix->M * ix->K)) : NULL;
+}
+static void scratch_free(scratch *s) {
+    q_free(&s->q);
+    set_free(&s->vis);
+    free(s->beam);
+    free(s->nb_id);
+    free(s->nb_d);
+    free(s->lut);
+}
+
+static inline float node_dist(const dko_index *ix, int pq, const float *query,
+                              const float *lut, uint32_t id) {
+    if (pq) return dko_pq_adc(ix->codes + (int64_t)id * ix->M, lut, ix->M, ix->K);
+    return dko_distance(ix->metric, query, ix->vectors + (int64_t)id * ix->vstride, ix->dim);
+}
+
+/* DiskANNIndex::search_internal, diskann/src/graph/index.rs:1938-2005, with the
+ * FullAccessor / QuantAccessor expand_beam of
+ * diskann-providers/.../inmem/full_precision.rs:274-322 and .../inmem/product.rs:304-335. */
+static void search_core(const dko_index *ix, int pq, const float *query, int64_t bw,
+                        scratch *s, uint32_t *rec_ids, float *rec_d, uint32_t *rec_len,
+                        int64_t rec_cap, dko_stats *st) {
+    pqueue *q = &s->q;
+    q->size = 0;
+    q->cursor = 0;
+    set_clear(&s->vis);
+    uint32_t cmps = 0, hops = 0, nrec = 0;
+    if (pq) {
+        int lut_metric = ix->metric == DKO_INNER_PRODUCT ? DKO_INNER_PRODUCT : DKO_L2;
+        dko_pq_lut(lut_metric, ix->pivots, ix->K, ix->dim, ix->offsets, ix->M, query, s->lut);
+    }
+    for (int64_t i = 0; i < ix->nstart; ++i) {
+        uint32_t sp = ix->starts[i];
+        float d = node_dist(ix, pq, query, s->lut, sp);
+        set_insert(&s->vis, sp);
+        q_insert(q, sp, d);
+        cmps += 1;
+    }
+    while (q_has_notvisited(q)) {
+        int64_t nbeam = 0;
+        while (nbeam < bw) {
+            int64_t c = q_closest_notvisited(q);
+            if (c < 0) break;
+            if (rec_ids && nrec < rec_cap) {
+                rec_ids[nrec] = q->id[c];
+                rec_d[nrec] = q->d[c];
+            }
+            nrec++;
+            s->beam[nbeam++] = q->id[c];
+        }
+        int64_t nn = 0;
+        for (int64_t b = 0; b < nbeam; ++b) {
+            uint32_t node = s->beam[b];
+            const uint32_t *adj = ix->adj + (int64_t)node * ix->max_degree;
+            uint32_t dg = ix->deg[node];
+            for (uint32_t j = 0; j < dg; ++j) {
+                uint32_t nbr = adj[j];
+                if (!set_insert(&s->vis, nbr)) continue;
+                s->nb_id[nn] = nbr;
+                s->nb_d[nn] = node_dist(ix, pq, query, s->lut, nbr);
+                nn++;
+            }
+        }
+        for (int64_t j = 0; j < nn; ++j) q_insert(q, s->nb_id[j], s->nb_d[j]);
+        cmps += (uint32_t)nn;
+        hops += (uint32_t)nbeam;
+    }
+    if (rec_len) *rec_len = nrec;
+    st->cmps = cmps;
+    st->hops = hops;
+}
+
+static int is_start(const dko_index *ix, uint32_t id) {
+    for (int64_t i = 0; i < ix->nstart; ++i)
+        if (ix->starts[i] == id) return 1;
+    return 0;
+}
+
+/* Post-processing: FilterStartPoints -> CopyIds / RemoveDeletedIdsAndCopy
+ * (diskann/src/graph/glue.rs:730-821) or FilterStartPoints -> Rerank
+ * (full_precision.rs:357-404). Rerank's sort_unstable has unspecified tie order; this
+ * restatement uses a stable sort (ties keep queue order). */
+static void post_process(const dko_index *ix, int mode, const float *query, scratch *s,
+                         int64_t k, uint32_t *out_ids, float *out_d, dko_stats *st) {
+    pqueue *q = &s->q;
+    int64_t lim = q->search_l < q->size ? q->search_l : q->size;
+    if (mode != DKO_MODE_HYBRID) {
+        int64_t c = 0;
+        for (int64_t i = 0; i < lim && c < k; ++i) {
+            if (is_start(ix, q->id[i])) continue;
+            out_ids[c] = q->id[i];
+            out_d[c] = q->d[i];
+            c++;
+        }
+        st->count = (uint32_t)c;
+        return;
+    }
+    uint32_t *ids = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(lim + 1));
+    float *ds = (float *)malloc(sizeof(float) * (size_t)(lim + 1));
+    int64_t n = 0;
+    for (int64_t i = 0; i < lim; ++i) {
+        if (is_start(ix, q->id[i])) continue;
+        ids[n] = q->id[i];
+        ds[n] = dko_distance(ix->metric, query, ix->vectors + (int64_t)q->id[i] * ix->vstride,
+                             ix->dim);
+        n++;
+    }
+    /* stable insertion sort by distance (partial_cmp, NaN compares Equal) */
+    for (int64_t i = 1; i < n; ++i) {
+        uint32_t id = ids[i];
+        float d = ds[i];
+        int64_t j = i - 1;
+        while (j >= 0 && ds[j] > d) {
+            ids[j + 1] = ids[j];
+            ds[j + 1] = ds[j];
+            j--;
+        }
+        ids[j + 1] = id;
+        ds[j + 1] = d;
+    }
+    int64_t c = n < k ? n : k;
+    for (int64_t i = 0; i < c; ++i) {
+        out_ids[i] = ids[i];
+        out_d[i] = ds[i];
+    }
+    st->count = (uint32_t)c;
+    free(ids);
+    free(ds);
+}
+
+int dko_search(const dko_index *ix, int mode, const float *query, int64_t k, int64_t L,
+               int64_t bw, uint32_t *out_ids, float *out_d, dko_stats *st) {
+    int rc = dko_knn_validate(k, L, bw);
+    if (rc) return rc;
+    if (mode != DKO_MODE_FULL && ix->M <= 0) return DKO_ERR_ARG;
+    scratch s;
+    scratch_init(&s, ix, L, bw);
+    search_core(ix, mode != DKO_MODE_FULL, query, bw, &s, NULL, NULL, NULL, 0, st);
+    post_process(ix, mode, query, &s, k, out_ids, out_d, st);
+    scratch_free(&s);
+    return 0;
+}
+
+int dko_search_full(const dko_index *ix, int mode, const float *query, int64_t L,
+                    int64_t bw, uint32_t *q_ids, float *q_d, uint32_t *q_len,
+                    uint32_t *rec_ids, float *rec_d, uint32_t *rec_len, int64_t rec_cap,
+                    dko_stats *st) {
+    if (L <= 0 || bw <= 0) return DKO_ERR_ARG;
+    scratch s;
+    scratch_init(&s, ix, L, bw);
+    search_core(ix, mode != DKO_MODE_FULL, query, bw, &s, rec_ids, rec_d, rec_len, rec_cap,
+                st);
+    int64_t lim = s.q.search_l < s.q.size ? s.q.search_l : s.q.size;
+    for (int64_t i = 0; i < lim; ++i) {
+        q_ids[i] = s.q.id[i];
+        q_d[i] = s.q.d[i];
+    }
+    *q_len = (uint32_t)lim;
+    st->count = (uint32_t)lim;
+    scratch_free(&s);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Threaded batch search (CPU baseline harness)                                         */
+/* ------------------------------------------------------------------------------------ */
+
+typedef struct {
+    const dko_index *ix;
+    int mode;
+    const float *queries;
+    int64_t qstride, k, L, bw, lo, hi;
+    uint32_t *out_ids, *out_cmps, *out_hops;
+    float *out_d;
+    double *lat_us;
+} batch_arg;
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static void *batch_worker(void *p) {
+    batch_arg *a = (batch_arg *)p;
+    scratch s;
+    scratch_init(&s, a->ix, a->L, a->bw);
+    for (int64_t i = a->lo; i < a->hi; ++i) {
+        double t0 = now_s();
+        dko_stats st;
+        const float *query = a->queries + i * a->qstride;
+        search_core(a->ix, a->mode != DKO_MODE_FULL, query, a->bw, &s, NULL, NULL, NULL, 0,
+                    &st);
+        post_process(a->ix, a->mode, query, &s, a->k, a->out_ids + i * a->k,
+                     a->out_d + i * a->k, &st);
+        for (int64_t j = st.count; j < a->k; ++j) {
+            a->out_ids[i * a->k + j] = 0xFFFFFFFFu;
+            a->out_d[i * a->k + j] = INFINITY;
+        }
+        if (a->out_cmps) a->out_cmps[i] = st.cmps;
+        if (a->out_hops) a->out_hops[i] = st.hops;
+        if (a->lat_us) a->lat_us[i] = (now_s() - t0) * 1e6;
+    }
+    scratch_free(&s);
+    return NULL;
+}
+
+double dko_search_batch(const dko_index *ix, int mode, const float *queries, int64_t nq,
+                        int64_t qstride, int64_t k, int64_t L, int64_t bw, int nthreads,
+                        uint32_t *out_ids, float *out_d, uint32_t *out_cmps,
+                        uint32_t *out_hops, double *lat_us) {
+    if (dko_knn_validate(k, L, bw)) return -1.0;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > nq) nthreads = (int)(nq > 0 ? nq : 1);
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    batch_arg *args = (batch_arg *)malloc(sizeof(batch_arg) * (size_t)nthreads);
+    double t0 = now_s();
+    for (int t = 0; t < nthreads; ++t) {
+        /* static contiguous partition: the first nq % T ranges get one extra query */
+        int64_t base = nq / nthreads, extra = nq % nthreads;
+        int64_t lo = t * base + (t < extra ? t : extra);
+        int64_t hi = lo + base + (t < extra ? 1 : 0);
+        batch_arg a = {ix, mode, queries, qstride, k, L, bw, lo, hi,
+                       out_ids, out_cmps, out_hops, out_d, lat_us};
+        args[t] = a;
+        pthread_create(&th[t], NULL, batch_worker, &args[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    double wall = now_s() - t0;
+    free(th);
+    free(args);
+    return wall;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* RobustPrune                                                                          */
+/* ------------------------------------------------------------------------------------ */
+
+/* PruneKind::update_occlude_factor, diskann/src/graph/config/mod.rs:80-103. */
+float dko_update_occlude_factor(int metric, float d_ik, float d_jk, float cur, float alpha) {
+    if (metric == DKO_INNER_PRODUCT) {
+        if (d_jk < alpha * d_ik) return alpha + 0.01f;
+        return cur;
+    }
+    if (d_jk == 0.0f) return FLT_MAX;
+    float r = d_ik / d_jk;
+    return fmaxf(cur, r);
+}
+
+/* occlude_list, diskann/src/graph/index.rs:2570-2782. The pool must already be sorted
+ * and truncated to max_occlusion. */
+int64_t dko_occlude_list(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                         int64_t dim, const uint32_t *pool_ids, const float *pool_d,
+                         int64_t pool_len, uint32_t self_id, uint32_t *out) {
+    if (pool_len == 0) return 0;
+    float *occ = (float *)calloc((size_t)pool_len, sizeof(float));
+    uint32_t *last = (uint32_t *)calloc((size_t)pool_len, sizeof(uint32_t));
+    uint32_t *nbr = (uint32_t *)calloc((size_t)pool_len, sizeof(uint32_t));
+    uint8_t *valid = (uint8_t *)malloc((size_t)pool_len);
+    for (int64_t i = 0; i < pool_len; ++i) valid[i] = pool_ids[i] != self_id;
+    float alpha = cfg->alpha;
+    int64_t degree = cfg->R;
+    float cur_alpha = 1.0f;
+    float inc = alpha < 1.2f ? alpha : 1.2f;
+    int64_t found = 0;
+    while (found < degree) {
+        for (int64_t i = 0; i < pool_len; ++i) {
+            if (found >= degree) break;
+            float of = occ[i];
+            uint32_t lc = last[i];
+            if (of > cur_alpha) continue;
+            if (!valid[i]) {
+                occ[i] = FLT_MAX;
+                continue;
+            }
+            const float *vi = vectors + (int64_t)pool_ids[i] * vstride;
+            while ((int64_t)lc != found) {
+                uint32_t rp = nbr[lc];
+                lc += 1;
+                if ((int64_t)rp >= i) {
+                    last[i] = lc;
+                    continue;
+                }
+                float d;
+                if (valid[rp])
+                    d = dko_distance(cfg->metric, vi, vectors + (int64_t)pool_ids[rp] * vstride,
+                                     dim);
+                else
+                    d = FLT_MAX;
+                of = dko_update_occlude_factor(cfg->metric, pool_d[i], d, of, cur_alpha);
+                if (of > cur_alpha) break;
+            }
+            last[i] = lc;
+            if (of > cur_alpha) {
+                occ[i] = of;
+                continue;
+            }
+            occ[i] = FLT_MAX;
+            nbr[found] = (uint32_t)i;
+            found++;
+        }
+        if (cur_alpha == alpha) break;
+        cur_alpha = fminf(cur_alpha * inc, alpha);
+    }
+    for (int64_t n = 0; n < found; ++n) out[n] = pool_ids[nbr[n]];
+    int64_t count = found;
+    if (cfg->saturate && alpha > 1.0f) {
+        for (int64_t i = 0; i < pool_len && count < degree; ++i) {
+            if (!valid[i]) continue;
+            int dup = 0;
+            for (int64_t j = 0; j < count; ++j)
+                if (out[j] == pool_ids[i]) dup = 1;
+            if (!dup) out[count++] = pool_ids[i];
+        }
+    }
+    free(occ);
+    free(last);
+    free(nbr);
+    free(valid);
+    return count;
+}
+
+/* Stable sort of (id, d) by distance (the reference's SortedNeighbors uses
+ * select_nth_unstable + sort_unstable, whose tie order is unspecified,
+ * diskann/src/graph/internal/sorted_neighbors.rs:26-44). */
+static void sort_pool(uint32_t *ids, float *ds, int64_t n) {
+    if (n < 2) return;
+    /* bottom-up merge sort */
+    uint32_t *ti = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+    float *td = (float *)malloc(sizeof(float) * (size_t)n);
+    for (int64_t w = 1; w < n; w *= 2) {
+        for (int64_t lo = 0; lo < n; lo += 2 * w) {
+            int64_t mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int64_t a = lo, b = mid, o = lo;
+            while (a < mid && b < hi) {
+                if (ds[b] < ds[a]) {
+                    ti[o] = ids[b];
+                    td[o++] = ds[b++];
+                } else {
+                    ti[o] = ids[a];
+                    td[o++] = ds[a++];
+                }
+            }
+            while (a < mid) {
+                ti[o] = ids[a];
+                td[o++] = ds[a++];
+            }
+            while (b < hi) {
+                ti[o] = ids[b];
+                td[o++] = ds[b++];
+            }
+        }
+        memcpy(ids, ti, sizeof(uint32_t) * (size_t)n);
+        memcpy(ds, td, sizeof(float) * (size_t)n);
+    }
+    free(ti);
+    free(td);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Build                                                                                */
+/* ------------------------------------------------------------------------------------ */
+
+
+/* robust_prune_list, index.rs:2388-2450: distances from `location` to every other id of
+ * `list` (duplicates and self removed), sorted pool, occlude_list. */
+static int64_t prune_list(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                          int64_t dim, uint32_t location, const uint32_t *list, int64_t n,
+                          uint32_t *out) {
+    uint32_t *ids = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n + 1));
+    float *ds = (float *)malloc(sizeof(float) * (size_t)(n + 1));
+    int64_t m = 0;
+    const float *v = vectors + (int64_t)location * vstride;
+    for (int64_t i = 0; i < n; ++i) {
+        if (list[i] == location) continue;
+        ids[m] = list[i];
+        ds[m] = dko_distance(cfg->metric, v, vectors + (int64_t)list[i] * vstride, dim);
+        m++;
+    }
+    sort_pool(ids, ds, m);
+    if (m > cfg->max_occlusion) m = cfg->max_occlusion;
+    int64_t c = dko_occlude_list(cfg, vectors, vstride, dim, ids, ds, m, location, out);
+    free(ids);
+    free(ds);
+    return c;
+}
+
+/* add_edge_and_prune, index.rs:2269-2346: append unique targets; prune when the list
+ * exceeds the max degree with slack. */
+static void add_edge_and_prune(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                               int64_t dim, dko_graph *g, uint32_t source,
+                               const uint32_t *targets, int64_t nt) {
+    uint32_t *buf = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(g->max_degree + nt + 1));
+    int64_t n = g->deg[source];
+    memcpy(buf, g->adj + (int64_t)source * g->max_degree, sizeof(uint32_t) * (size_t)n);
+    int64_t added = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+        int dup = 0;
+        for (int64_t j = 0; j < n; ++j)
+            if (buf[j] == targets[t]) {
+                dup = 1;
+                break;
+            }
+        if (!dup) {
+            buf[n++] = targets[t];
+            added++;
+        }
+    }
+    if (added == 0) {
+        free(buf);
+        return;
+    }
+    uint32_t *dst = g->adj + (int64_t)source * g->max_degree;
+    if (n <= cfg->R_max) {
+        memcpy(dst, buf, sizeof(uint32_t) * (size_t)n);
+        g->deg[source] = (uint32_t)n;
+    } else {
+        uint32_t *out = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(cfg->R + 1));
+        int64_t c = prune_list(cfg, vectors, vstride, dim, source, buf, n, out);
+        memcpy(dst, out, sizeof(uint32_t) * (size_t)c);
+        g->deg[source] = (uint32_t)c;
+        free(out);
+    }
+    free(buf);
+}
+
+/* search + prune for one point (insert index.rs:226-341, search_and_prune :349-434,
+ * robust_prune_with :2481-2537). extras: intra-batch candidates (distances appended to
+ * the visited record). Returns the number of new neighbors. */
+static int64_t search_and_prune(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                                int64_t dim, const dko_graph *g, const uint32_t *starts,
+                                int64_t nstart, uint32_t id, const uint32_t *extras,
+                                int64_t nextras, uint32_t *out) {
+    dko_index ix;
+    memset(&ix, 0, sizeof(ix));
+    ix.metric = cfg->metric;
+    ix.dim = dim;
+    ix.vstride = vstride;
+    ix.vectors = vectors;
+    ix.max_degree = g->max_degree;
+    ix.adj = g->adj;
+    ix.deg = g->deg;
+    ix.nstart = nstart;
+    ix.starts = starts;
+    const float *v = vectors + (int64_t)id * vstride;
+    int64_t L = cfg->L_build;
+    scratch s;
+    scratch_init(&s, &ix, L, 1);
+    int64_t cap = 16384;
+    uint32_t *rid = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(cap + nextras));
+    float *rd = (float *)malloc(sizeof(float) * (size_t)(cap + nextras));
+    uint32_t rlen = 0;
+    dko_stats st;
+    search_core(&ix, 0, v, 1, &s, rid, rd, &rlen, cap, &st);
+    if (rlen > cap) rlen = (uint32_t)cap;
+    /* fill() takes self, then extras, then the record, capped at max_occlusion: extras
+     * beyond the cap are unavailable in the working set */
+    int64_t avail_extras = nextras;
+    if (1 + avail_extras > cfg->max_occlusion) avail_extras = cfg->max_occlusion - 1;
+    for (int64_t e = 0; e < avail_extras; ++e) {
+        rid[rlen] = extras[e];
+        rd[rlen] = dko_distance(cfg->metric, v, vectors + (int64_t)extras[e] * vstride, dim);
+        rlen++;
+    }
+    sort_pool(rid, rd, rlen);
+    int64_t plen = rlen;
+    if (plen > cfg->max_occlusion) plen = cfg->max_occlusion;
+    int64_t c = dko_occlude_list(cfg, vectors, vstride, dim, rid, rd, plen, id, out);
+    free(rid);
+    free(rd);
+    scratch_free(&s);
+    return c;
+}
+
+int dko_build(const dko_build_cfg *cfg, const float *vectors, int64_t vstride, int64_t dim,
+              dko_graph *g, const uint32_t *starts, int64_t nstart, const uint32_t *ids,
+              int64_t nids, int64_t batch, int64_t intra_batch) {
+    if (batch < 1) return DKO_ERR_ARG;
+    uint32_t *nbrs = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(batch * (cfg->R + 1)));
+    int64_t *cnt = (int64_t *)malloc(sizeof(int64_t) * (size_t)batch);
+    uint32_t *extras = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)batch);
+    for (int64_t b0 = 0; b0 < nids; b0 += batch) {
+        int64_t B = nids - b0 < batch ? nids - b0 : batch;
+        const uint32_t *bid = ids + b0;
+        if (batch == 1) {
+            /* single insert: search, prune, set, then back edges one at a time */
+            uint32_t *out = nbrs;
+            int64_t c = search_and_prune(cfg, vectors, vstride, dim, g, starts, nstart, bid[0],
+                                         NULL, 0, out);
+            memcpy(g->adj + (int64_t)bid[0] * g->max_degree, out, sizeof(uint32_t) * (size_t)c);
+            g->deg[bid[0]] = (uint32_t)c;
+            int64_t nb = c < cfg->max_backedges ? c : cfg->max_backedges;
+            for (int64_t j = 0; j < nb; ++j)
+                add_edge_and_prune(cfg, vectors, vstride, dim, g, out[j], &bid[0], 1);
+            continue;
+        }
+        /* candidate generation on the pre-batch graph */
+        int64_t nc = intra_batch < 0 ? B : intra_batch;
+        for (int64_t p = 0; p < B; ++p) {
+            /* async_tools::around(ids, p, nc): up to min(nc, B-1) ids around p, self
+             * excluded (diskann/src/utils/async_tools.rs:51-120) */
+            int64_t ne = 0;
+            int64_t want = nc < B - 1 ? nc : B - 1;
+            if (want > 0) {
+                int64_t start = p - want / 2;
+                if (start < 0) start = 0;
+                if (start + want + 1 > B) start = B - want - 1;
+                for (int64_t q = start; q < start + want + 1 && ne < want; ++q)
+                    if (q != p) extras[ne++] = bid[q];
+            }
+            cnt[p] = search_and_prune(cfg, vectors, vstride, dim, g, starts, nstart, bid[p],
+                                      extras, ne, nbrs + p * (cfg->R + 1));
+        }
+        for (int64_t p = 0; p < B; ++p) {
+            memcpy(g->adj + (int64_t)bid[p] * g->max_degree, nbrs + p * (cfg->R + 1),
+                   sizeof(uint32_t) * (size_t)cnt[p]);
+            g->deg[bid[p]] = (uint32_t)cnt[p];
+        }
+        /* aggregate back edges: target -> sorted list of batch sources */
+        int64_t total = 0;
+        for (int64_t p = 0; p < B; ++p) total += cnt[p];
+        uint64_t *pairs = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(total + 1));
+        int64_t np = 0;
+        for (int64_t p = 0; p < B; ++p)
+            for (int64_t j = 0; j < cnt[p]; ++j)
+                pairs[np++] = ((uint64_t)nbrs[p * (cfg->R + 1) + j] << 32) | bid[p];
+        /* sort pairs (target, source) */
+        for (int64_t w = 1; w < np; w *= 2) {
+            uint64_t *tmp = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)np);
+            for (int64_t lo = 0; lo < np; lo += 2 * w) {
+                int64_t mid = lo + w < np ? lo + w : np, hi = lo + 2 * w < np ? lo + 2 * w : np;
+                int64_t a = lo, b = mid, o = lo;
+                while (a < mid && b < hi) tmp[o++] = pairs[b] < pairs[a] ? pairs[b++] : pairs[a++];
+                while (a < mid) tmp[o++] = pairs[a++];
+                while (b < hi) tmp[o++] = pairs[b++];
+            }
+            memcpy(pairs, tmp, sizeof(uint64_t) * (size_t)np);
+            free(tmp);
+        }
+        uint32_t *tg = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(np + 1));
+        for (int64_t i = 0; i < np;) {
+            uint32_t target = (uint32_t)(pairs[i] >> 32);
+            int64_t j = i, nt = 0;
+            while (j < np && (uint32_t)(pairs[j] >> 32) == target) tg[nt++] = (uint32_t)pairs[j++];
+            add_edge_and_prune(cfg, vectors, vstride, dim, g, target, tg, nt);
+            i = j;
+        }
+        free(tg);
+        free(pairs);
+    }
+    free(nbrs);
+    free(cnt);
+    free(extras);
+    return 0;
+}
+
+/* compute_medoid, diskann-utils/src/sampling/medoid.rs:15-47: f64 column means cast to
+ * f32, then the row nearest by SquaredL2 (strict <, first wins). */
+int64_t dko_medoid(const float *x, int64_t n, int64_t stride, int64_t dim, float *mean_out) {
+    double *sum = (double *)calloc((size_t)dim, sizeof(double));
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < dim; ++j) sum[j] += (double)x[i * stride + j];
+    float *mean = (float *)malloc(sizeof(float) * (size_t)dim);
+    for (int64_t j = 0; j < dim; ++j) mean[j] = (float)(sum[j] / (double)n);
+    int64_t best = 0;
+    float bd = INFINITY;
+    for (int64_t i = 0; i < n; ++i) {
+        float d = simd_4x1(0, mean, x + i * stride, dim);
+        if (d < bd) {
+            bd = d;
+            best = i;
+        }
+    }
+    if (mean_out) memcpy(mean_out, mean, sizeof(float) * (size_t)dim);
+    free(sum);
+    free(mean);
+    return best;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* k-means (Lloyd)                                                                       */
+/* ------------------------------------------------------------------------------------ */
+
+/* distances_in_place, diskann-quantization/src/algorithms/kmeans/lloyds.rs:28-328:
+ * d = ||c||^2 - <c,x> - <c,x> + ||x||^2 with <c,x> a sequential FMA over dims (c
+ * broadcast times data), argmin strict <. Returns the residual (sum of minimum
+ * distances, accumulated per lane of 8 then sum_tree; here the lane is row % 8 within
+ * blocks of 16 rows split 8+8, so row r goes to lane r % 8). */
+float dko_lloyd_assign(const float *x, int64_t n, int64_t dim, const float *c, int64_t K,
+                       uint32_t *nearest) {
+    float *cn = (float *)malloc(sizeof(float) * (size_t)K);
+    for (int64_t k = 0; k < K; ++k) cn[k] = dko_square_norm(c + k * dim, dim);
+    float res[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = 0; i < n; ++i) {
+        const float *xi = x + i * dim;
+        float xn = dko_square_norm(xi, dim);
+        float best = INFINITY;
+        uint32_t arg = 0;
+        for (int64_t k = 0; k < K; ++k) {
+            float dot = 0.0f;
+            for (int64_t j = 0; j < dim; ++j) dot = fmaf(c[k * dim + j], xi[j], dot);
+            float d = cn[k] - dot - dot + xn;
+            if (d < best) {
+                best = d;
+                arg = (uint32_t)k;
+            }
+        }
+        nearest[i] = arg;
+        res[i % 8] += best;
+    }
+    free(cn);
+    return sum_tree8(res);
+}
+
+/* update_centroids, lloyds.rs:345-366: f64 sums in row order divided by max(count,1),
+ * cast to f32; empty clusters become the zero vector. */
+void dko_lloyd_update(const float *x, int64_t n, int64_t dim, const uint32_t *nearest,
+                      int64_t K, float *c) {
+    double *s = (double *)calloc((size_t)(K * dim), sizeof(double));
+    int64_t *cnt = (int64_t *)calloc((size_t)K, sizeof(int64_t));
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t k = nearest[i];
+        cnt[k]++;
+        for (int64_t j = 0; j < dim; ++j) s[k * dim + j] += (double)x[i * dim + j];
+    }
+    for (int64_t k = 0; k < K; ++k) {
+        double den = (double)(cnt[k] > 1 ? cnt[k] : 1);
+        for (int64_t j = 0; j < dim; ++j) c[k * dim + j] = (float)(s[k * dim + j] / den);
+    }
+    free(s);
+    free(cnt);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Brute force FP64 top-k                                                                */
+/* ------------------------------------------------------------------------------------ */
+
+typedef struct {
+    int metric;
+    const float *base;
+    int64_t n, stride, dim;
+    const float *queries;
+    int64_t k, lo, hi;
+    uint32_t *out_ids;
+    double *out_d;
+} bf_arg;
+
+static double dist_f64(int metric, const float *a, const float *b, int64_t dim) {
+    if (metric == DKO_L2) {
+        double s = 0;
+        for (int64_t j = 0; j < dim; ++j) {
+            double c = (double)a[j] - (double)b[j];
+            s += c * c;
+        }
+        return s;
+    }
+    double xy = 0, xx = 0, yy = 0;
+    for (int64_t j = 0; j < dim; ++j) {
+        xy += (double)a[j] * (double)b[j];
+        xx += (double)a[j] * (double)a[j];
+        yy += (double)b[j] * (double)b[j];
+    }
+    if (metric == DKO_INNER_PRODUCT) return -xy;
+    if (xx == 0 || yy == 0) return 1.0;
+    return 1.0 - xy / (sqrt(xx) * sqrt(yy));
+}
+
+static void *bf_worker(void *p) {
+    bf_arg *a = (bf_arg *)p;
+    double *bd = (double *)malloc(sizeof(double) * (size_t)a->k);
+    uint32_t *bi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)a->k);
+    for (int64_t q = a->lo; q < a->hi; ++q) {
+        int64_t cnt = 0;
+        const float *qv = a->queries + q * a->dim;
+        for (int64_t i = 0; i < a->n; ++i) {
+            double d = dist_f64(a->metric, qv, a->base + i * a->stride, a->dim);
+            if (cnt == a->k && !(d < bd[cnt - 1])) continue;
+            int64_t pos = cnt < a->k ? cnt : a->k - 1;
+            while (pos > 0 && bd[pos - 1] > d) {
+                bd[pos] = bd[pos - 1];
+                bi[pos] = bi[pos - 1];
+                pos--;
+            }
+            bd[pos] = d;
+            bi[pos] = (uint32_t)i;
+            if (cnt < a->k) cnt++;
+        }
+        for (int64_t j = 0; j < a->k; ++j) {
+            a->out_ids[q * a->k + j] = j < cnt ? bi[j] : 0xFFFFFFFFu;
+            a->out_d[q * a->k + j] = j < cnt ? bd[j] : INFINITY;
+        }
+    }
+    free(bd);
+    free(bi);
+    return NULL;
+}
+
+void dko_bruteforce_f64(int metric, const float *base, int64_t n, int64_t stride,
+                        int64_t dim, const float *queries, int64_t nq, int64_t k,
+                        int nthreads, uint32_t *out_ids, double *out_d) {
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    bf_arg *args = (bf_arg *)malloc(sizeof(bf_arg) * (size_t)nthreads);
+    for (int t = 0; t < nthreads; ++t) {
+        int64_t base_n = nq / nthreads, extra = nq % nthreads;
+        int64_t lo = t * base_n + (t < extra ? t : extra);
+        int64_t hi = lo + base_n + (t < extra ? 1 : 0);
+        bf_arg a = {metric, base, n, stride, dim, queries, k, lo, hi, out_ids, out_d};
+        args[t] = a;
+        pthread_create(&th[t], NULL, bf_worker, &args[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th);
+    free(args);
+}

@@ -1,0 +1,152 @@
+/*
+ * CPU oracle: a plain-C restatement of the reference (SeliMeli/DiskANN v0.55.0)
+ * search-and-quantize path. TEST INFRASTRUCTURE ONLY: tests, the smoke check and the
+ * CPU-baseline leg of the benchmark call it as the checker; the product path
+ * (diskann_amd/, libdiskann_mi355x.so) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.
+ *
+ * Parity status per function (how each one is pinned):
+ *   dko_distance (L2/IP)      pinned: grid_search golden distances, numpy FP64 bound,
+ *                             hand-derived 32-partial order on adversarial inputs.
+ *   dko_distance (cosine)     pinned: closed forms (parallel / orthogonal / zero-norm)
+ *                             and FP64 bound; V3 (f32x8, Strategy2x4) order chosen.
+ *   queue + dko_search        pinned: grid_search_{1_100,3_5,4_4} golden ids, distances,
+ *                             cmps and hops for beam widths 1/2/4 (reference JSON).
+ *   dko_pq_lut / dko_pq_adc   pinned: test_compute_pq_distance KAT and FP64 bound.
+ *   dko_pq_encode             pinned: brute-force argmin over pivots (FP64 check).
+ *   dko_chunk_offsets         pinned: partition_happy_path KAT.
+ *   dko_occlude_list          pinned: update_occlude_factor KATs + invariants;
+ *                             grid_insert golden (post-build searches) where the
+ *                             reference's unstable sort ties do not matter.
+ *   dko_lloyd_*               pinned: closed form on separated clusters, monotone residual.
+ *   dko_kmeanspp              parity unpinned (reference uses rand 0.9.4 / ChaCha12,
+ *                             not reproducible without the Rust crates).
+ */
+#ifndef DKO_ORACLE_H
+#define DKO_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Metric numbering follows diskann-vector/src/distance/metric.rs:11-20 (repr(C)). */
+enum { DKO_COSINE = 0, DKO_INNER_PRODUCT = 1, DKO_L2 = 2, DKO_COSINE_NORMALIZED = 3 };
+
+/* Search modes: FullPrecision (CopyIds), Hybrid (PQ traversal + FP32 Rerank),
+ * Quantized (PQ traversal, PQ distances returned). */
+enum { DKO_MODE_FULL = 0, DKO_MODE_HYBRID = 1, DKO_MODE_PQ_ONLY = 2 };
+
+float dko_distance(int metric, const float *q, const float *x, int64_t dim);
+float dko_square_norm(const float *x, int64_t len);
+
+/* PQ ---------------------------------------------------------------------------- */
+void dko_chunk_offsets(int64_t dim, int64_t nchunks, int64_t *offsets /* nchunks+1 */);
+void dko_pq_lut(int metric, const float *pivots /* K x dim */, int64_t K, int64_t dim,
+                const int64_t *offsets, int64_t M, const float *query, float *lut /* M*K */);
+float dko_pq_adc(const uint8_t *code, const float *lut, int64_t M, int64_t K);
+int dko_pq_encode(const float *pivots, int64_t K, int64_t dim, const int64_t *offsets,
+                  int64_t M, const float *x, int64_t n, uint8_t *codes);
+
+/* Graph + search -------------------------------------------------------------------- */
+typedef struct {
+    int metric;
+    int64_t dim;
+    int64_t vstride;           /* floats per vector row */
+    const float *vectors;      /* rows indexed by internal id */
+    int64_t max_degree;        /* adjacency row stride */
+    const uint32_t *adj;       /* [rows * max_degree] */
+    const uint32_t *deg;       /* [rows] */
+    int64_t nstart;
+    const uint32_t *starts;    /* start point internal ids */
+    /* PQ (optional) */
+    int64_t M, K;
+    const uint8_t *codes;      /* [rows * M] */
+    const float *pivots;       /* [K * dim] */
+    const int64_t *offsets;    /* [M+1] */
+} dko_index;
+
+typedef struct {
+    uint32_t cmps, hops, count;
+} dko_stats;
+
+/* Knn validation codes (diskann/src/graph/search/knn_search.rs:94-130). */
+enum { DKO_OK = 0, DKO_ERR_KZERO = 1, DKO_ERR_LZERO = 2, DKO_ERR_L_LESS_THAN_K = 3,
+       DKO_ERR_BEAM_ZERO = 4, DKO_ERR_ARG = 5 };
+int dko_knn_validate(int64_t k, int64_t L, int64_t bw);
+
+int dko_search(const dko_index *ix, int mode, const float *query, int64_t k, int64_t L,
+               int64_t bw, uint32_t *out_ids, float *out_d, dko_stats *st);
+
+/* Same search but also returns the whole final queue (first min(L+nstart,size)) and
+ * the visited record (expanded nodes, in expansion order). */
+int dko_search_full(const dko_index *ix, int mode, const float *query, int64_t L,
+                    int64_t bw, uint32_t *q_ids, float *q_d, uint32_t *q_len,
+                    uint32_t *rec_ids, float *rec_d, uint32_t *rec_len, int64_t rec_cap,
+                    dko_stats *st);
+
+/* Threaded batch search: static contiguous partition across nthreads (the reference
+ * benchmark harness, diskann/src/utils/async_tools.rs:311-340). Per-query latency in
+ * microseconds is written to lat_us when non-NULL. Returns wall seconds. */
+double dko_search_batch(const dko_index *ix, int mode, const float *queries, int64_t nq,
+                        int64_t qstride, int64_t k, int64_t L, int64_t bw, int nthreads,
+                        uint32_t *out_ids, float *out_d, uint32_t *out_cmps,
+                        uint32_t *out_hops, double *lat_us);
+
+/* Build (RobustPrune / insert / multi_insert) ---------------------------------------- */
+typedef struct {
+    int metric;
+    int64_t R;                 /* pruned degree */
+    int64_t R_max;             /* max degree with slack */
+    int64_t L_build;
+    float alpha;
+    int64_t max_occlusion;     /* 750 */
+    int saturate;              /* saturate_after_prune */
+    int64_t max_backedges;     /* insert(): number of back edges (default R_max) */
+} dko_build_cfg;
+
+/* occlude_list on a pool sorted by distance (diskann/src/graph/index.rs:2570-2782).
+ * pool_ids/pool_d: candidates; `self_id` excluded. Distances between candidates are
+ * computed with the metric on `vectors`. Returns the number of neighbors written. */
+int64_t dko_occlude_list(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                         int64_t dim, const uint32_t *pool_ids, const float *pool_d,
+                         int64_t pool_len, uint32_t self_id, uint32_t *out);
+
+float dko_update_occlude_factor(int metric, float d_ik, float d_jk, float cur, float alpha);
+
+/* Mutable graph used by the build. */
+typedef struct {
+    int64_t rows, max_degree;
+    uint32_t *adj;
+    uint32_t *deg;
+} dko_graph;
+
+/* Insert `ids` in batches of `batch` with multi_insert semantics
+ * (diskann/src/graph/index.rs:820-1035); batch==1 uses single insert semantics
+ * (index.rs:226-341). intra_batch: -1 = All, 0 = None, n = Max(n).
+ * Vectors for every id (and the start points) must already be present. */
+int dko_build(const dko_build_cfg *cfg, const float *vectors, int64_t vstride, int64_t dim,
+              dko_graph *g, const uint32_t *starts, int64_t nstart, const uint32_t *ids,
+              int64_t nids, int64_t batch, int64_t intra_batch);
+
+/* Medoid start point (diskann-utils/src/sampling/medoid.rs:15-47). */
+int64_t dko_medoid(const float *x, int64_t n, int64_t stride, int64_t dim, float *mean_out);
+
+/* k-means (diskann-quantization/src/algorithms/kmeans/lloyds.rs) -------------------- */
+float dko_lloyd_assign(const float *x, int64_t n, int64_t dim, const float *c, int64_t K,
+                       uint32_t *nearest);
+void dko_lloyd_update(const float *x, int64_t n, int64_t dim, const uint32_t *nearest,
+                      int64_t K, float *c);
+
+/* Exact FP64 brute-force top-k (recall oracle). */
+void dko_bruteforce_f64(int metric, const float *base, int64_t n, int64_t stride,
+                        int64_t dim, const float *queries, int64_t nq, int64_t k,
+                        int nthreads, uint32_t *out_ids, double *out_d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

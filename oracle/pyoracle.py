@@ -1,0 +1,284 @@
+"""ctypes binding for oracle/liboracle.so (test infrastructure only)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+COSINE, INNER_PRODUCT, L2, COSINE_NORMALIZED = 0, 1, 2, 3
+MODE_FULL, MODE_HYBRID, MODE_PQ_ONLY = 0, 1, 2
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _setup(_lib)
+    return _lib
+
+
+class Index(C.Structure):
+    _fields_ = [
+        ("metric", C.c_int), ("dim", C.c_int64), ("vstride", C.c_int64),
+        ("vectors", C.c_void_p), ("max_degree", C.c_int64), ("adj", C.c_void_p),
+        ("deg", C.c_void_p), ("nstart", C.c_int64), ("starts", C.c_void_p),
+        ("M", C.c_int64), ("K", C.c_int64), ("codes", C.c_void_p),
+        ("pivots", C.c_void_p), ("offsets", C.c_void_p),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [("cmps", C.c_uint32), ("hops", C.c_uint32), ("count", C.c_uint32)]
+
+
+class BuildCfg(C.Structure):
+    _fields_ = [
+        ("metric", C.c_int), ("R", C.c_int64), ("R_max", C.c_int64),
+        ("L_build", C.c_int64), ("alpha", C.c_float), ("max_occlusion", C.c_int64),
+        ("saturate", C.c_int), ("max_backedges", C.c_int64),
+    ]
+
+
+class Graph(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("max_degree", C.c_int64), ("adj", C.c_void_p),
+                ("deg", C.c_void_p)]
+
+
+def _setup(L):
+    P, I64, F = C.c_void_p, C.c_int64, C.c_float
+    L.dko_distance.argtypes = [C.c_int, P, P, I64]
+    L.dko_distance.restype = F
+    L.dko_square_norm.argtypes = [P, I64]
+    L.dko_square_norm.restype = F
+    L.dko_chunk_offsets.argtypes = [I64, I64, P]
+    L.dko_pq_lut.argtypes = [C.c_int, P, I64, I64, P, I64, P, P]
+    L.dko_pq_adc.argtypes = [P, P, I64, I64]
+    L.dko_pq_adc.restype = F
+    L.dko_pq_encode.argtypes = [P, I64, I64, P, I64, P, I64, P]
+    L.dko_pq_encode.restype = C.c_int
+    L.dko_knn_validate.argtypes = [I64, I64, I64]
+    L.dko_search.argtypes = [C.POINTER(Index), C.c_int, P, I64, I64, I64, P, P,
+                             C.POINTER(Stats)]
+    L.dko_search_full.argtypes = [C.POINTER(Index), C.c_int, P, I64, I64, P, P, P, P, P, P,
+                                  I64, C.POINTER(Stats)]
+    L.dko_search_batch.argtypes = [C.POINTER(Index), C.c_int, P, I64, I64, I64, I64, I64,
+                                   C.c_int, P, P, P, P, P]
+    L.dko_search_batch.restype = C.c_double
+    L.dko_occlude_list.argtypes = [C.POINTER(BuildCfg), P, I64, I64, P, P, I64, C.c_uint32, P]
+    L.dko_occlude_list.restype = I64
+    L.dko_update_occlude_factor.argtypes = [C.c_int, F, F, F, F]
+    L.dko_update_occlude_factor.restype = F
+    L.dko_build.argtypes = [C.POINTER(BuildCfg), P, I64, I64, C.POINTER(Graph), P, I64, P,
+                            I64, I64, I64]
+    L.dko_medoid.argtypes = [P, I64, I64, I64, P]
+    L.dko_medoid.restype = I64
+    L.dko_lloyd_assign.argtypes = [P, I64, I64, P, I64, P]
+    L.dko_lloyd_assign.restype = F
+    L.dko_lloyd_update.argtypes = [P, I64, I64, P, I64, P]
+    L.dko_bruteforce_f64.argtypes = [C.c_int, P, I64, I64, I64, P, I64, I64, C.c_int, P, P]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def distance(metric, q, x):
+    q = np.ascontiguousarray(q, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    return lib().dko_distance(metric, _p(q), _p(x), q.shape[0])
+
+
+def square_norm(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    return lib().dko_square_norm(_p(x), x.shape[0])
+
+
+def chunk_offsets(dim, m):
+    out = np.zeros(m + 1, dtype=np.int64)
+    lib().dko_chunk_offsets(dim, m, _p(out))
+    return out
+
+
+def pq_lut(metric, pivots, offsets, query):
+    pivots = np.ascontiguousarray(pivots, dtype=np.float32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    query = np.ascontiguousarray(query, dtype=np.float32)
+    K, dim = pivots.shape
+    M = offsets.shape[0] - 1
+    lut = np.zeros(M * K, dtype=np.float32)
+    lib().dko_pq_lut(metric, _p(pivots), K, dim, _p(offsets), M, _p(query), _p(lut))
+    return lut.reshape(M, K)
+
+
+def pq_adc(code, lut):
+    code = np.ascontiguousarray(code, dtype=np.uint8)
+    lut = np.ascontiguousarray(lut, dtype=np.float32)
+    return lib().dko_pq_adc(_p(code), _p(lut), lut.shape[0], lut.shape[1])
+
+
+def pq_encode(pivots, offsets, x):
+    pivots = np.ascontiguousarray(pivots, dtype=np.float32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    K, dim = pivots.shape
+    M = offsets.shape[0] - 1
+    codes = np.zeros((x.shape[0], M), dtype=np.uint8)
+    rc = lib().dko_pq_encode(_p(pivots), K, dim, _p(offsets), M, _p(x), x.shape[0], _p(codes))
+    if rc:
+        raise ValueError("PQ encode: infinite or NaN distance at row %d" % (rc - 1))
+    return codes
+
+
+class OracleIndex:
+    """Holds numpy arrays alive behind an Index struct."""
+
+    def __init__(self, metric, vectors, adj, deg, starts, codes=None, pivots=None,
+                 offsets=None):
+        self.vectors = np.ascontiguousarray(vectors, dtype=np.float32)
+        self.adj = np.ascontiguousarray(adj, dtype=np.uint32)
+        self.deg = np.ascontiguousarray(deg, dtype=np.uint32)
+        self.starts = np.ascontiguousarray(starts, dtype=np.uint32)
+        s = Index()
+        s.metric = metric
+        s.dim = self.vectors.shape[1]
+        s.vstride = self.vectors.shape[1]
+        s.vectors = _p(self.vectors)
+        s.max_degree = self.adj.shape[1]
+        s.adj = _p(self.adj)
+        s.deg = _p(self.deg)
+        s.nstart = self.starts.shape[0]
+        s.starts = _p(self.starts)
+        if codes is not None:
+            self.codes = np.ascontiguousarray(codes, dtype=np.uint8)
+            self.pivots = np.ascontiguousarray(pivots, dtype=np.float32)
+            self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+            s.M = self.codes.shape[1]
+            s.K = self.pivots.shape[0]
+            s.codes = _p(self.codes)
+            s.pivots = _p(self.pivots)
+            s.offsets = _p(self.offsets)
+        self.s = s
+
+    def search(self, query, k, L, bw=1, mode=MODE_FULL):
+        query = np.ascontiguousarray(query, dtype=np.float32)
+        ids = np.zeros(k, dtype=np.uint32)
+        ds = np.zeros(k, dtype=np.float32)
+        st = Stats()
+        rc = lib().dko_search(C.byref(self.s), mode, _p(query), k, L, bw, _p(ids), _p(ds),
+                              C.byref(st))
+        if rc:
+            raise ValueError("knn validation error %d" % rc)
+        n = st.count
+        return ids[:n], ds[:n], st.cmps, st.hops
+
+    def search_full(self, query, L, bw=1, mode=MODE_FULL, rec_cap=1 << 16):
+        query = np.ascontiguousarray(query, dtype=np.float32)
+        qi = np.zeros(L + self.starts.shape[0], dtype=np.uint32)
+        qd = np.zeros(L + self.starts.shape[0], dtype=np.float32)
+        ri = np.zeros(rec_cap, dtype=np.uint32)
+        rd = np.zeros(rec_cap, dtype=np.float32)
+        ql = C.c_uint32()
+        rl = C.c_uint32()
+        st = Stats()
+        lib().dko_search_full(C.byref(self.s), mode, _p(query), L, bw, _p(qi), _p(qd),
+                              C.byref(ql), _p(ri), _p(rd), C.byref(rl), rec_cap, C.byref(st))
+        n = min(rl.value, rec_cap)
+        return qi[:ql.value], qd[:ql.value], ri[:n], rd[:n], st.cmps, st.hops
+
+    def search_batch(self, queries, k, L, bw=1, mode=MODE_FULL, nthreads=1, latency=False):
+        queries = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = queries.shape[0]
+        ids = np.zeros((nq, k), dtype=np.uint32)
+        ds = np.zeros((nq, k), dtype=np.float32)
+        cm = np.zeros(nq, dtype=np.uint32)
+        hp = np.zeros(nq, dtype=np.uint32)
+        lat = np.zeros(nq, dtype=np.float64) if latency else None
+        wall = lib().dko_search_batch(C.byref(self.s), mode, _p(queries), nq,
+                                      queries.shape[1], k, L, bw, nthreads, _p(ids), _p(ds),
+                                      _p(cm), _p(hp), _p(lat))
+        if wall < 0:
+            raise ValueError("knn validation error")
+        return ids, ds, cm, hp, wall, lat
+
+
+def build_graph(metric, vectors, R, R_max, L_build, alpha, starts, ids, batch=1,
+                intra_batch=-1, start_adj=None, max_occlusion=750, saturate=False,
+                max_backedges=None):
+    """Vamana build on the oracle. Returns (adj, deg) for all rows of `vectors`."""
+    vectors = np.ascontiguousarray(vectors, dtype=np.float32)
+    rows = vectors.shape[0]
+    adj = np.zeros((rows, R_max), dtype=np.uint32)
+    deg = np.zeros(rows, dtype=np.uint32)
+    if start_adj is not None:
+        for s, lst in start_adj.items():
+            adj[s, :len(lst)] = lst
+            deg[s] = len(lst)
+    g = Graph(rows, R_max, _p(adj), _p(deg))
+    cfg = BuildCfg(metric, R, R_max, L_build, alpha, max_occlusion, int(saturate),
+                   R_max if max_backedges is None else max_backedges)
+    starts = np.ascontiguousarray(starts, dtype=np.uint32)
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
+    lib().dko_build(C.byref(cfg), _p(vectors), vectors.shape[1], vectors.shape[1],
+                    C.byref(g), _p(starts), starts.shape[0], _p(ids), ids.shape[0], batch,
+                    intra_batch)
+    return adj, deg
+
+
+def occlude_list(metric, vectors, pool_ids, pool_d, self_id, R, alpha, saturate=False):
+    vectors = np.ascontiguousarray(vectors, dtype=np.float32)
+    pool_ids = np.ascontiguousarray(pool_ids, dtype=np.uint32)
+    pool_d = np.ascontiguousarray(pool_d, dtype=np.float32)
+    cfg = BuildCfg(metric, R, R, 0, alpha, 750, int(saturate), R)
+    out = np.zeros(max(R, 1) + pool_ids.shape[0], dtype=np.uint32)
+    n = lib().dko_occlude_list(C.byref(cfg), _p(vectors), vectors.shape[1], vectors.shape[1],
+                               _p(pool_ids), _p(pool_d), pool_ids.shape[0], self_id, _p(out))
+    return out[:n]
+
+
+def update_occlude_factor(metric, d_ik, d_jk, cur, alpha):
+    return lib().dko_update_occlude_factor(metric, d_ik, d_jk, cur, alpha)
+
+
+def medoid(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    mean = np.zeros(x.shape[1], dtype=np.float32)
+    i = lib().dko_medoid(_p(x), x.shape[0], x.shape[1], x.shape[1], _p(mean))
+    return i, mean
+
+
+def lloyd_assign(x, c):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    c = np.ascontiguousarray(c, dtype=np.float32)
+    nearest = np.zeros(x.shape[0], dtype=np.uint32)
+    r = lib().dko_lloyd_assign(_p(x), x.shape[0], x.shape[1], _p(c), c.shape[0], _p(nearest))
+    return nearest, r
+
+
+def lloyd_update(x, nearest, K):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    nearest = np.ascontiguousarray(nearest, dtype=np.uint32)
+    c = np.zeros((K, x.shape[1]), dtype=np.float32)
+    lib().dko_lloyd_update(_p(x), x.shape[0], x.shape[1], _p(nearest), K, _p(c))
+    return c
+
+
+def bruteforce_f64(metric, base, queries, k, nthreads=8):
+    base = np.ascontiguousarray(base, dtype=np.float32)
+    queries = np.ascontiguousarray(queries, dtype=np.float32)
+    nq = queries.shape[0]
+    ids = np.zeros((nq, k), dtype=np.uint32)
+    ds = np.zeros((nq, k), dtype=np.float64)
+    lib().dko_bruteforce_f64(metric, _p(base), base.shape[0], base.shape[1], base.shape[1],
+                             _p(queries), nq, k, nthreads, _p(ids), _p(ds))
+    return ids, ds

@@ -1,0 +1,16 @@
+"""diskann_amd: MI355X-native (gfx950) search-and-quantize path for DiskANN.
+
+The compute path is the HIP library libdiskann_mi355x.so behind the C ABI in
+include/diskann_mi355x.h; this package mirrors the reference's search, PQ and distance
+interfaces on top of it.
+"""
+from .errors import ANNError, CapacityError, DimensionMismatchError, KnnSearchError, PQError
+from .index import (DiskANNIndex, FullPrecision, Hybrid, Knn, Metric, Quantized, SearchResult,
+                    Strategy)
+from . import quantization
+
+__all__ = [
+    "ANNError", "CapacityError", "DimensionMismatchError", "KnnSearchError", "PQError",
+    "DiskANNIndex", "FullPrecision", "Hybrid", "Knn", "Metric", "Quantized", "SearchResult",
+    "Strategy", "quantization",
+]

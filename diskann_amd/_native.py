@@ -1,0 +1,77 @@
+"""ctypes binding of libdiskann_mi355x.so (the C ABI in include/diskann_mi355x.h).
+
+The library is loaded from this package directory; a missing library is an error (there
+is no fallback path).
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdiskann_mi355x.so")
+
+# Every symbol declared in include/diskann_mi355x.h.
+EXPORTS = [
+    "dkm_last_error", "dkm_version", "dkm_init", "dkm_device_alloc", "dkm_device_free",
+    "dkm_memcpy_h2d", "dkm_memcpy_d2h", "dkm_synchronize", "dkm_distance_batch",
+    "dkm_pq_chunk_offsets", "dkm_pq_lut", "dkm_pq_encode", "dkm_pq_train",
+    "dkm_index_create", "dkm_index_destroy", "dkm_set_vectors", "dkm_set_start_points",
+    "dkm_set_graph", "dkm_get_graph", "dkm_set_pq", "dkm_set_codes", "dkm_encode_index",
+    "dkm_get_codes", "dkm_search_batch", "dkm_search_batch_device", "dkm_index_synchronize",
+    "dkm_profile", "dkm_profile_read", "dkm_set_tuning",
+]
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libdiskann_mi355x.so is not built; run `python -m diskann_amd.build` (no CPU "
+            "fallback exists)")
+    lib = C.CDLL(LIB_PATH)
+    _declare(lib)
+    _lib = lib
+    return lib
+
+
+def _declare(L):
+    P, I64, U32, I = C.c_void_p, C.c_int64, C.c_uint32, C.c_int
+    L.dkm_last_error.restype = C.c_char_p
+    L.dkm_version.restype = C.c_char_p
+    sig = {
+        "dkm_init": [I],
+        "dkm_device_alloc": [C.c_size_t, C.POINTER(C.c_void_p)],
+        "dkm_device_free": [P],
+        "dkm_memcpy_h2d": [P, P, C.c_size_t],
+        "dkm_memcpy_d2h": [P, P, C.c_size_t],
+        "dkm_synchronize": [],
+        "dkm_distance_batch": [I, I64, P, I64, P, I64, P, I64, P],
+        "dkm_pq_chunk_offsets": [I64, I64, P],
+        "dkm_pq_lut": [I, P, I64, I64, P, I64, P, I64, P],
+        "dkm_pq_encode": [P, I64, I64, P, I64, P, I64, P],
+        "dkm_pq_train": [P, I64, I64, I64, I64, I64, C.c_uint64, P, P],
+        "dkm_index_create": [I, I64, I64, I64, I64, I64, C.POINTER(C.c_void_p)],
+        "dkm_set_vectors": [P, P, I64, I64],
+        "dkm_set_start_points": [P, P],
+        "dkm_set_graph": [P, P, P, I64, I64, I64],
+        "dkm_get_graph": [P, P, P, I64, I64, I64],
+        "dkm_set_pq": [P, P, I64, P, I64],
+        "dkm_set_codes": [P, P, I64, I64],
+        "dkm_encode_index": [P],
+        "dkm_get_codes": [P, P, I64, I64],
+        "dkm_search_batch": [P, I, P, I64, P, P, U32, I64, P, P, P, P, P],
+        "dkm_search_batch_device": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P],
+        "dkm_index_synchronize": [P],
+        "dkm_profile": [P, I],
+        "dkm_profile_read": [P, C.POINTER(C.c_int64), C.POINTER(C.c_double)],
+        "dkm_set_tuning": [P, I64],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.dkm_index_destroy.argtypes = [P]
+    L.dkm_index_destroy.restype = None

@@ -1,0 +1,717 @@
+// Host runtime behind the C ABI in include/diskann_mi355x.h.
+//
+// Owns the device-resident index (vectors, fixed-stride adjacency, PQ schema and
+// codes), validates requests the way the reference does (Knn::new,
+// diskann/src/graph/search/knn_search.rs:94-130; dimension checks of QueryComputer::new,
+// diskann-providers/src/model/pq/distance/dynamic.rs:62-70) and launches the kernels on
+// the handle's HIP stream. Calls on one handle are serialised by a mutex, so the handle
+// is safe to share across threads (the reference's DiskANNIndex is Sync).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/diskann_mi355x.h"
+#include "dkm_kernels.h"
+#include "dkm_search.h"
+
+namespace {
+
+thread_local std::string g_err;
+int g_device = 0;
+
+dkm_status fail(dkm_status s, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return s;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(e_ == hipErrorOutOfMemory ? DKM_ERR_OUT_OF_MEMORY : DKM_ERR_DEVICE, \
+                        "%s failed: %s", #expr, hipGetErrorString(e_));                 \
+    } while (0)
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void **)&p, sizeof(T) * std::max<size_t>(count, 1));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct dkm_index {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    int metric = DKM_L2;
+    int64_t dim = 0, vstride = 0;
+    int64_t max_points = 0, num_start = 0, rows = 0;
+    int64_t R = 0, R_max = 0, gstride = 0;
+    float *d_vec = nullptr;
+    uint32_t *d_graph = nullptr;
+    uint32_t *d_starts = nullptr;
+    bool starts_set = false;
+    // PQ
+    int64_t M = 0, K = 0;
+    float *d_pivots = nullptr;
+    int32_t *d_offsets = nullptr;
+    uint8_t *d_codes = nullptr;
+    bool codes_set = false;
+    std::vector<int64_t> h_offsets;
+    // visited-set spill
+    uint64_t *d_spill = nullptr;
+    uint32_t *d_spill_counter = nullptr;
+    uint32_t spill_regions = 256;
+    uint64_t spill_slots = 1u << 16;
+    uint32_t epoch = 0;
+    int64_t tune_hslots = 0;
+    // staging for the host-buffer API
+    DevBuf<float> s_q, s_d;
+    DevBuf<uint32_t> s_k, s_L, s_ids, s_cnt, s_cmps, s_hops, s_status;
+    // profiling
+    bool profile = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    int64_t prof_launches = 0;
+    double prof_ms = 0.0;
+};
+
+extern "C" {
+
+const char *dkm_last_error(void) { return g_err.c_str(); }
+const char *dkm_version(void) { return "diskann_mi355x 0.1.0 (gfx950)"; }
+
+dkm_status dkm_init(int device) {
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(DKM_ERR_INVALID_ARG, "device %d of %d", device, n);
+    g_device = device;
+    HIP_TRY(hipSetDevice(device));
+    return DKM_OK;
+}
+
+dkm_status dkm_device_alloc(size_t bytes, void **ptr) {
+    if (!ptr) return fail(DKM_ERR_INVALID_ARG, "null output pointer");
+    HIP_TRY(hipSetDevice(g_device));
+    HIP_TRY(hipMalloc(ptr, std::max<size_t>(bytes, 1)));
+    return DKM_OK;
+}
+dkm_status dkm_device_free(void *ptr) {
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return DKM_OK;
+}
+dkm_status dkm_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return DKM_OK;
+}
+dkm_status dkm_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return DKM_OK;
+}
+dkm_status dkm_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return DKM_OK;
+}
+
+static bool metric_ok(int m) { return m >= 0 && m <= 3; }
+
+dkm_status dkm_distance_batch(int metric, int64_t dim, const float *queries, int64_t nq,
+                              const float *base, int64_t nbase, const uint32_t *ids, int64_t m,
+                              float *out) {
+    if (!metric_ok(metric)) return fail(DKM_ERR_INVALID_ARG, "unknown metric %d", metric);
+    if (dim <= 0 || nq < 0 || nbase <= 0 || m < 0) return fail(DKM_ERR_INVALID_ARG, "bad sizes");
+    if (nq == 0 || m == 0) return DKM_OK;
+    for (int64_t i = 0; i < nq * m; ++i)
+        if (ids[i] >= (uint64_t)nbase)
+            return fail(DKM_ERR_INDEX, "id %u out of range (%lld rows)", ids[i], (long long)nbase);
+    HIP_TRY(hipSetDevice(g_device));
+    float *dq, *db, *dout;
+    uint32_t *di;
+    HIP_TRY(hipMalloc(&dq, sizeof(float) * nq * dim));
+    HIP_TRY(hipMalloc(&db, sizeof(float) * nbase * dim));
+    HIP_TRY(hipMalloc(&di, sizeof(uint32_t) * nq * m));
+    HIP_TRY(hipMalloc(&dout, sizeof(float) * nq * m));
+    HIP_TRY(hipMemcpy(dq, queries, sizeof(float) * nq * dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(db, base, sizeof(float) * nbase * dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(di, ids, sizeof(uint32_t) * nq * m, hipMemcpyHostToDevice));
+    HIP_TRY(dkm::launch_distance_batch(metric, (int)dim, dq, nq, db, dim, di, m, dout, nullptr));
+    HIP_TRY(hipMemcpy(out, dout, sizeof(float) * nq * m, hipMemcpyDeviceToHost));
+    (void)hipFree(dq);
+    (void)hipFree(db);
+    (void)hipFree(di);
+    (void)hipFree(dout);
+    return DKM_OK;
+}
+
+dkm_status dkm_pq_chunk_offsets(int64_t dim, int64_t num_chunks, int64_t *offsets) {
+    if (dim <= 0 || num_chunks <= 0 || num_chunks > dim)
+        return fail(DKM_ERR_PQ, "cannot partition dim %lld into %lld chunks", (long long)dim,
+                    (long long)num_chunks);
+    int64_t off = 0;
+    offsets[0] = 0;
+    for (int64_t c = 0; c < num_chunks; ++c) {
+        off += dim / num_chunks;
+        if (c < dim % num_chunks) off += 1;
+        offsets[c + 1] = off;
+    }
+    return DKM_OK;
+}
+
+static dkm_status check_offsets(const int64_t *offsets, int64_t M, int64_t dim) {
+    if (M <= 0) return fail(DKM_ERR_PQ, "no PQ chunks");
+    if (offsets[0] != 0) return fail(DKM_ERR_PQ, "chunk offsets must start at 0");
+    for (int64_t i = 0; i < M; ++i)
+        if (offsets[i + 1] <= offsets[i]) return fail(DKM_ERR_PQ, "chunk offsets must increase");
+    if (offsets[M] != dim) return fail(DKM_ERR_PQ, "last chunk offset %lld != dim %lld",
+                                       (long long)offsets[M], (long long)dim);
+    return DKM_OK;
+}
+
+dkm_status dkm_pq_lut(int metric, const float *pivots, int64_t K, int64_t dim,
+                      const int64_t *offsets, int64_t M, const float *queries, int64_t nq,
+                      float *lut) {
+    if (K != 256) return fail(DKM_ERR_PQ, "PQ tables need 256 centroids (got %lld)", (long long)K);
+    if (metric == DKM_COSINE)
+        return fail(DKM_ERR_PQ, "cosine PQ uses direct distances, not tables");
+    dkm_status st = check_offsets(offsets, M, dim);
+    if (st) return st;
+    if (nq == 0) return DKM_OK;
+    HIP_TRY(hipSetDevice(g_device));
+    std::vector<int32_t> o32(offsets, offsets + M + 1);
+    float *dp, *dq, *dl;
+    int32_t *dof;
+    HIP_TRY(hipMalloc(&dp, sizeof(float) * K * dim));
+    HIP_TRY(hipMalloc(&dq, sizeof(float) * nq * dim));
+    HIP_TRY(hipMalloc(&dl, sizeof(float) * nq * M * K));
+    HIP_TRY(hipMalloc(&dof, sizeof(int32_t) * (M + 1)));
+    HIP_TRY(hipMemcpy(dp, pivots, sizeof(float) * K * dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dq, queries, sizeof(float) * nq * dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dof, o32.data(), sizeof(int32_t) * (M + 1), hipMemcpyHostToDevice));
+    HIP_TRY(dkm::launch_pq_lut(metric == DKM_INNER_PRODUCT, dp, (int)dim, dof, (int)M, dq, nq, dl,
+                               nullptr));
+    HIP_TRY(hipMemcpy(lut, dl, sizeof(float) * nq * M * K, hipMemcpyDeviceToHost));
+    (void)hipFree(dp);
+    (void)hipFree(dq);
+    (void)hipFree(dl);
+    (void)hipFree(dof);
+    return DKM_OK;
+}
+
+dkm_status dkm_pq_encode(const float *pivots, int64_t K, int64_t dim, const int64_t *offsets,
+                         int64_t M, const float *x, int64_t n, uint8_t *codes) {
+    if (K <= 0 || K > 256) return fail(DKM_ERR_PQ, "1..256 centroids supported (got %lld)", (long long)K);
+    dkm_status st = check_offsets(offsets, M, dim);
+    if (st) return st;
+    if (n == 0) return DKM_OK;
+    HIP_TRY(hipSetDevice(g_device));
+    std::vector<int32_t> o32(offsets, offsets + M + 1);
+    float *dp, *dx;
+    int32_t *dof;
+    uint8_t *dc;
+    uint32_t *dbad;
+    HIP_TRY(hipMalloc(&dp, sizeof(float) * K * dim));
+    HIP_TRY(hipMalloc(&dx, sizeof(float) * n * dim));
+    HIP_TRY(hipMalloc(&dc, n * M));
+    HIP_TRY(hipMalloc(&dof, sizeof(int32_t) * (M + 1)));
+    HIP_TRY(hipMalloc(&dbad, sizeof(uint32_t)));
+    HIP_TRY(hipMemcpy(dp, pivots, sizeof(float) * K * dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dx, x, sizeof(float) * n * dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dof, o32.data(), sizeof(int32_t) * (M + 1), hipMemcpyHostToDevice));
+    uint32_t bad = 0xFFFFFFFFu;
+    HIP_TRY(hipMemcpy(dbad, &bad, sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(dkm::launch_pq_encode(dp, (int)K, (int)dim, dof, (int)M, dx, dim, n, dc, M, dbad,
+                                  nullptr));
+    HIP_TRY(hipMemcpy(codes, dc, n * M, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&bad, dbad, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    (void)hipFree(dp);
+    (void)hipFree(dx);
+    (void)hipFree(dc);
+    (void)hipFree(dof);
+    (void)hipFree(dbad);
+    if (bad != 0xFFFFFFFFu)
+        return fail(DKM_ERR_PQ, "infinite or NaN distance while encoding row %u", bad);
+    return DKM_OK;
+}
+
+dkm_status dkm_pq_train(const float *data, int64_t n, int64_t dim, int64_t M, int64_t K,
+                        int64_t iters, uint64_t seed, float *pivots, int64_t *offsets) {
+    if (n <= 0 || K <= 0 || K > 256) return fail(DKM_ERR_KMEANS, "bad k-means sizes");
+    if (n < K) return fail(DKM_ERR_KMEANS, "need at least K=%lld training rows (got %lld)",
+                           (long long)K, (long long)n);
+    dkm_status st = dkm_pq_chunk_offsets(dim, M, offsets);
+    if (st) return st;
+    HIP_TRY(hipSetDevice(g_device));
+    std::vector<int32_t> o32(offsets, offsets + M + 1);
+    hipError_t e = dkm::pq_train_device(data, n, (int)dim, o32.data(), (int)M, (int)K, (int)iters,
+                                        seed, pivots);
+    if (e != hipSuccess) return fail(DKM_ERR_DEVICE, "PQ training: %s", hipGetErrorString(e));
+    return DKM_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Index
+// ------------------------------------------------------------------------------------
+
+dkm_status dkm_index_create(int metric, int64_t dim, int64_t max_points, int64_t num_start,
+                            int64_t R, int64_t R_max, dkm_index **out) {
+    if (!out) return fail(DKM_ERR_INVALID_ARG, "null output");
+    *out = nullptr;
+    if (!metric_ok(metric)) return fail(DKM_ERR_INVALID_ARG, "unknown metric %d", metric);
+    if (dim <= 0 || max_points <= 0 || num_start <= 0 || num_start > 64 || R <= 0 || R_max < R)
+        return fail(DKM_ERR_INVALID_ARG, "bad index shape");
+    if (max_points + num_start >= (int64_t)0x7FFFFFFF)
+        return fail(DKM_ERR_INVALID_ARG, "too many points for 31-bit ids");
+    HIP_TRY(hipSetDevice(g_device));
+    dkm_index *ix = new dkm_index();
+    ix->device = g_device;
+    ix->metric = metric;
+    ix->dim = dim;
+    ix->vstride = round_up(dim, 16);
+    ix->max_points = max_points;
+    ix->num_start = num_start;
+    ix->rows = max_points + num_start;
+    ix->R = R;
+    ix->R_max = R_max;
+    ix->gstride = round_up(R_max + 1, 16);
+    hipError_t e = hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&ix->d_vec, sizeof(float) * ix->rows * ix->vstride);
+    if (e == hipSuccess) e = hipMemset(ix->d_vec, 0, sizeof(float) * ix->rows * ix->vstride);
+    if (e == hipSuccess) e = hipMalloc(&ix->d_graph, sizeof(uint32_t) * ix->rows * ix->gstride);
+    if (e == hipSuccess) e = hipMemset(ix->d_graph, 0, sizeof(uint32_t) * ix->rows * ix->gstride);
+    if (e == hipSuccess) e = hipMalloc(&ix->d_starts, sizeof(uint32_t) * num_start);
+    if (e == hipSuccess) {
+        std::vector<uint32_t> s(num_start);
+        for (int64_t i = 0; i < num_start; ++i) s[i] = (uint32_t)(max_points + i);
+        e = hipMemcpy(ix->d_starts, s.data(), sizeof(uint32_t) * num_start, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        dkm_index_destroy(ix);
+        return fail(e == hipErrorOutOfMemory ? DKM_ERR_OUT_OF_MEMORY : DKM_ERR_DEVICE,
+                    "index allocation: %s", hipGetErrorString(e));
+    }
+    *out = ix;
+    return DKM_OK;
+}
+
+void dkm_index_destroy(dkm_index *ix) {
+    if (!ix) return;
+    (void)hipSetDevice(ix->device);
+    if (ix->stream) (void)hipStreamSynchronize(ix->stream);
+    for (auto &ev : ix->events) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    void *ptrs[] = {ix->d_vec, ix->d_graph, ix->d_starts, ix->d_pivots, ix->d_offsets,
+                    ix->d_codes, ix->d_spill, ix->d_spill_counter};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    ix->s_q.release();
+    ix->s_d.release();
+    ix->s_k.release();
+    ix->s_L.release();
+    ix->s_ids.release();
+    ix->s_cnt.release();
+    ix->s_cmps.release();
+    ix->s_hops.release();
+    ix->s_status.release();
+    if (ix->stream) (void)hipStreamDestroy(ix->stream);
+    delete ix;
+}
+
+dkm_status dkm_set_vectors(dkm_index *ix, const float *x, int64_t row0, int64_t n) {
+    if (!ix || (!x && n > 0)) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (row0 < 0 || n < 0 || row0 + n > ix->max_points)
+        return fail(DKM_ERR_INDEX, "rows [%lld, %lld) outside [0, %lld)", (long long)row0,
+                    (long long)(row0 + n), (long long)ix->max_points);
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    if (n == 0) return DKM_OK;
+    HIP_TRY(hipMemcpy2D(ix->d_vec + row0 * ix->vstride, sizeof(float) * ix->vstride, x,
+                        sizeof(float) * ix->dim, sizeof(float) * ix->dim, n, hipMemcpyHostToDevice));
+    ix->codes_set = false;
+    return DKM_OK;
+}
+
+dkm_status dkm_set_start_points(dkm_index *ix, const float *vectors) {
+    if (!ix || !vectors) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    HIP_TRY(hipMemcpy2D(ix->d_vec + ix->max_points * ix->vstride, sizeof(float) * ix->vstride,
+                        vectors, sizeof(float) * ix->dim, sizeof(float) * ix->dim, ix->num_start,
+                        hipMemcpyHostToDevice));
+    ix->starts_set = true;
+    ix->codes_set = false;
+    return DKM_OK;
+}
+
+dkm_status dkm_set_graph(dkm_index *ix, const uint32_t *lists, const uint32_t *counts,
+                         int64_t stride, int64_t row0, int64_t n) {
+    if (!ix || (n > 0 && (!lists || !counts))) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (row0 < 0 || n < 0 || row0 + n > ix->rows)
+        return fail(DKM_ERR_INDEX, "graph rows out of range");
+    std::vector<uint32_t> buf(std::max<int64_t>(n, 1) * ix->gstride, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t c = counts[i];
+        if (c > ix->R_max || c > stride)
+            return fail(DKM_ERR_INDEX, "row %lld has %u neighbors (max degree %lld)",
+                        (long long)(row0 + i), c, (long long)ix->R_max);
+        uint32_t *dst = buf.data() + i * ix->gstride;
+        for (uint32_t j = 0; j < c; ++j) {
+            uint32_t id = lists[i * stride + j];
+            if (id >= ix->rows)
+                return fail(DKM_ERR_INDEX, "row %lld: neighbor id %u out of range",
+                            (long long)(row0 + i), id);
+            for (uint32_t t = 0; t < j; ++t)
+                if (dst[t] == id)
+                    return fail(DKM_ERR_INDEX, "row %lld: duplicate neighbor %u",
+                                (long long)(row0 + i), id);
+            dst[j] = id;
+        }
+        dst[ix->gstride - 1] = c;
+    }
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    if (n > 0)
+        HIP_TRY(hipMemcpy(ix->d_graph + row0 * ix->gstride, buf.data(),
+                          sizeof(uint32_t) * n * ix->gstride, hipMemcpyHostToDevice));
+    return DKM_OK;
+}
+
+dkm_status dkm_get_graph(dkm_index *ix, uint32_t *lists, uint32_t *counts, int64_t stride,
+                         int64_t row0, int64_t n) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    if (row0 < 0 || n < 0 || row0 + n > ix->rows) return fail(DKM_ERR_INDEX, "rows out of range");
+    if (n == 0) return DKM_OK;
+    std::vector<uint32_t> buf(n * ix->gstride);
+    {
+        std::lock_guard<std::mutex> g(ix->mu);
+        HIP_TRY(hipSetDevice(ix->device));
+        HIP_TRY(hipStreamSynchronize(ix->stream));
+        HIP_TRY(hipMemcpy(buf.data(), ix->d_graph + row0 * ix->gstride,
+                          sizeof(uint32_t) * n * ix->gstride, hipMemcpyDeviceToHost));
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t c = buf[i * ix->gstride + ix->gstride - 1];
+        counts[i] = c;
+        for (uint32_t j = 0; j < c && j < stride; ++j) lists[i * stride + j] = buf[i * ix->gstride + j];
+    }
+    return DKM_OK;
+}
+
+dkm_status dkm_set_pq(dkm_index *ix, const float *pivots, int64_t K, const int64_t *offsets,
+                      int64_t M) {
+    if (!ix || !pivots || !offsets) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (K != 256) return fail(DKM_ERR_PQ, "the device tables need 256 centroids (got %lld)", (long long)K);
+    dkm_status st = check_offsets(offsets, M, ix->dim);
+    if (st) return st;
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    if (ix->d_pivots) (void)hipFree(ix->d_pivots);
+    if (ix->d_offsets) (void)hipFree(ix->d_offsets);
+    if (ix->d_codes) (void)hipFree(ix->d_codes);
+    ix->d_pivots = nullptr;
+    ix->d_offsets = nullptr;
+    ix->d_codes = nullptr;
+    ix->M = M;
+    ix->K = K;
+    ix->h_offsets.assign(offsets, offsets + M + 1);
+    std::vector<int32_t> o32(offsets, offsets + M + 1);
+    HIP_TRY(hipMalloc(&ix->d_pivots, sizeof(float) * K * ix->dim));
+    HIP_TRY(hipMalloc(&ix->d_offsets, sizeof(int32_t) * (M + 1)));
+    HIP_TRY(hipMalloc(&ix->d_codes, ix->rows * M + 16));
+    HIP_TRY(hipMemset(ix->d_codes, 0, ix->rows * M + 16));
+    HIP_TRY(hipMemcpy(ix->d_pivots, pivots, sizeof(float) * K * ix->dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ix->d_offsets, o32.data(), sizeof(int32_t) * (M + 1), hipMemcpyHostToDevice));
+    ix->codes_set = false;
+    return DKM_OK;
+}
+
+dkm_status dkm_set_codes(dkm_index *ix, const uint8_t *codes, int64_t row0, int64_t n) {
+    if (!ix || (!codes && n > 0)) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (!ix->d_codes) return fail(DKM_ERR_NOT_READY, "set the PQ schema first");
+    if (row0 < 0 || n < 0 || row0 + n > ix->rows) return fail(DKM_ERR_INDEX, "rows out of range");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    if (n) HIP_TRY(hipMemcpy(ix->d_codes + row0 * ix->M, codes, n * ix->M, hipMemcpyHostToDevice));
+    ix->codes_set = true;
+    return DKM_OK;
+}
+
+dkm_status dkm_encode_index(dkm_index *ix) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    if (!ix->d_codes) return fail(DKM_ERR_NOT_READY, "set the PQ schema first");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    uint32_t *dbad;
+    HIP_TRY(hipMalloc(&dbad, sizeof(uint32_t)));
+    uint32_t bad = 0xFFFFFFFFu;
+    HIP_TRY(hipMemcpy(dbad, &bad, sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(dkm::launch_pq_encode(ix->d_pivots, (int)ix->K, (int)ix->dim, ix->d_offsets, (int)ix->M,
+                                  ix->d_vec, ix->vstride, ix->rows, ix->d_codes, ix->M, dbad,
+                                  ix->stream));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    HIP_TRY(hipMemcpy(&bad, dbad, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    (void)hipFree(dbad);
+    if (bad != 0xFFFFFFFFu)
+        return fail(DKM_ERR_PQ, "infinite or NaN distance while encoding row %u", bad);
+    ix->codes_set = true;
+    return DKM_OK;
+}
+
+dkm_status dkm_get_codes(dkm_index *ix, uint8_t *codes, int64_t row0, int64_t n) {
+    if (!ix || !ix->d_codes) return fail(DKM_ERR_NOT_READY, "no PQ codes");
+    if (row0 < 0 || n < 0 || row0 + n > ix->rows) return fail(DKM_ERR_INDEX, "rows out of range");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    if (n) HIP_TRY(hipMemcpy(codes, ix->d_codes + row0 * ix->M, n * ix->M, hipMemcpyDeviceToHost));
+    return DKM_OK;
+}
+
+static uint32_t next_pow2(uint64_t x) {
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// Launch on the handle's stream; caller holds ix->mu.
+static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_queries, int64_t nq,
+                                const uint32_t *d_k, const uint32_t *d_L, uint32_t max_k,
+                                uint32_t max_L, uint32_t bw, int64_t k_stride, uint32_t *d_ids,
+                                float *d_d, uint32_t *d_cnt, uint32_t *d_cmps, uint32_t *d_hops,
+                                uint32_t *d_status) {
+    if (strategy < 0 || strategy > 2) return fail(DKM_ERR_INVALID_ARG, "unknown strategy %d", strategy);
+    if (bw == 0) return fail(DKM_ERR_BEAM_WIDTH_ZERO, "beam width must be >= 1");
+    if (bw > 64) return fail(DKM_ERR_INVALID_ARG, "beam width above 64 is not supported");
+    if (max_L == 0) return fail(DKM_ERR_L_ZERO, "L must be >= 1");
+    if (max_L > 4096) return fail(DKM_ERR_INVALID_ARG, "L above 4096 is not supported");
+    if ((int64_t)max_k > k_stride) return fail(DKM_ERR_INVALID_ARG, "k exceeds output stride");
+    if (!ix->starts_set) return fail(DKM_ERR_NOT_READY, "start points not set");
+    bool pq = strategy != DKM_STRATEGY_FULL;
+    if (pq && (!ix->d_codes || !ix->codes_set))
+        return fail(DKM_ERR_NOT_READY, "PQ strategy needs a PQ schema and codes");
+    if (pq && ix->metric == DKM_COSINE)
+        return fail(DKM_ERR_PQ, "cosine PQ traversal (direct distances) is not supported");
+    if (nq == 0) return DKM_OK;
+    if (!ix->d_spill) {
+        HIP_TRY(hipMalloc(&ix->d_spill, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions));
+        HIP_TRY(hipMemsetAsync(ix->d_spill, 0, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions,
+                               ix->stream));
+        HIP_TRY(hipMalloc(&ix->d_spill_counter, sizeof(uint32_t)));
+    }
+    ix->epoch += 1;
+    if (ix->epoch == 0) {  // wrapped: stale tags could collide, clear the table
+        HIP_TRY(hipMemsetAsync(ix->d_spill, 0, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions,
+                               ix->stream));
+        ix->epoch = 1;
+    }
+    dkm::SearchParams p{};
+    p.metric = ix->metric;
+    p.pq = pq;
+    p.rerank = strategy == DKM_STRATEGY_HYBRID;
+    p.dim = (int)ix->dim;
+    p.vstride = ix->vstride;
+    p.vectors = ix->d_vec;
+    p.graph = ix->d_graph;
+    p.gstride = (int)ix->gstride;
+    p.starts = ix->d_starts;
+    p.nstart = (int)ix->num_start;
+    p.codes = ix->d_codes;
+    p.code_stride = (int)ix->M;
+    p.M = (int)ix->M;
+    p.pivots = ix->d_pivots;
+    p.offsets = ix->d_offsets;
+    p.queries = d_queries;
+    p.qstride = ix->dim;
+    p.nq = (int)nq;
+    p.k = d_k;
+    p.L = d_L;
+    p.beam_width = bw;
+    p.k_stride = k_stride;
+    p.out_ids = d_ids;
+    p.out_d = d_d;
+    p.out_count = d_cnt;
+    p.out_cmps = d_cmps;
+    p.out_hops = d_hops;
+    p.status = d_status;
+    p.qcap = (int)(max_L + ix->num_start);
+    p.nlcap = (int)std::max<int64_t>(bw * ix->R_max, p.qcap) + 64;
+    p.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots
+                                   : std::min<uint32_t>(16384, std::max<uint32_t>(2048, next_pow2(64ull * max_L)));
+    p.spill_table = ix->d_spill;
+    p.spill_slots = ix->spill_slots;
+    p.spill_regions = ix->spill_regions;
+    p.spill_counter = ix->d_spill_counter;
+    p.epoch = ix->epoch;
+    p.max_iters = (uint32_t)std::min<int64_t>(ix->rows + 16, 0x7FFFFFFF);
+    size_t lds = dkm::search_lds_bytes(p);
+    if (lds > 160 * 1024)
+        return fail(DKM_ERR_CAPACITY, "per-query LDS need %zu bytes exceeds 160 KiB", lds);
+    HIP_TRY(dkm::launch_reset_counter(ix->d_spill_counter, ix->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ix->profile) {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, ix->stream));
+    }
+    hipError_t e = dkm::launch_search(p, ix->stream);
+    if (e != hipSuccess) return fail(DKM_ERR_DEVICE, "search launch: %s", hipGetErrorString(e));
+    if (ix->profile) {
+        HIP_TRY(hipEventRecord(e1, ix->stream));
+        ix->events.push_back({e0, e1});
+    }
+    return DKM_OK;
+}
+
+dkm_status dkm_search_batch_device(dkm_index *ix, int strategy, const float *d_queries,
+                                   int64_t nq, const uint32_t *d_k, const uint32_t *d_L,
+                                   uint32_t max_k, uint32_t max_L, uint32_t beam_width,
+                                   int64_t k_stride, uint32_t *d_out_ids, float *d_out_dists,
+                                   uint32_t *d_out_counts, uint32_t *d_out_cmps,
+                                   uint32_t *d_out_hops) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    HIP_TRY(ix->s_status.ensure(std::max<int64_t>(nq, 1)));
+    return launch_locked(ix, strategy, d_queries, nq, d_k, d_L, max_k, max_L, beam_width, k_stride,
+                         d_out_ids, d_out_dists, d_out_counts, d_out_cmps, d_out_hops,
+                         ix->s_status.p);
+}
+
+dkm_status dkm_search_batch(dkm_index *ix, int strategy, const float *queries, int64_t nq,
+                            const uint32_t *k, const uint32_t *L, uint32_t beam_width,
+                            int64_t k_stride, uint32_t *out_ids, float *out_dists,
+                            uint32_t *out_counts, uint32_t *out_cmps, uint32_t *out_hops) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    if (nq < 0) return fail(DKM_ERR_INVALID_ARG, "negative query count");
+    if (nq == 0) return DKM_OK;
+    if (!queries || !k || !L || !out_ids || !out_dists || !out_counts)
+        return fail(DKM_ERR_INVALID_ARG, "null argument");
+    uint32_t max_k = 0, max_L = 0;
+    for (int64_t i = 0; i < nq; ++i) {
+        // Knn::new validation order: k, L, k <= L, beam width
+        if (k[i] == 0) return fail(DKM_ERR_K_ZERO, "query %lld: k must be >= 1", (long long)i);
+        if (L[i] == 0) return fail(DKM_ERR_L_ZERO, "query %lld: L must be >= 1", (long long)i);
+        if (k[i] > L[i])
+            return fail(DKM_ERR_L_LESS_THAN_K, "query %lld: L (%u) < k (%u)", (long long)i, L[i], k[i]);
+        max_k = std::max(max_k, k[i]);
+        max_L = std::max(max_L, L[i]);
+    }
+    if (beam_width == 0) return fail(DKM_ERR_BEAM_WIDTH_ZERO, "beam width must be >= 1");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    HIP_TRY(ix->s_q.ensure(nq * ix->dim));
+    HIP_TRY(ix->s_k.ensure(nq));
+    HIP_TRY(ix->s_L.ensure(nq));
+    HIP_TRY(ix->s_ids.ensure(nq * k_stride));
+    HIP_TRY(ix->s_d.ensure(nq * k_stride));
+    HIP_TRY(ix->s_cnt.ensure(nq));
+    HIP_TRY(ix->s_cmps.ensure(nq));
+    HIP_TRY(ix->s_hops.ensure(nq));
+    HIP_TRY(ix->s_status.ensure(nq));
+    HIP_TRY(hipMemcpyAsync(ix->s_q.p, queries, sizeof(float) * nq * ix->dim, hipMemcpyHostToDevice, ix->stream));
+    HIP_TRY(hipMemcpyAsync(ix->s_k.p, k, sizeof(uint32_t) * nq, hipMemcpyHostToDevice, ix->stream));
+    HIP_TRY(hipMemcpyAsync(ix->s_L.p, L, sizeof(uint32_t) * nq, hipMemcpyHostToDevice, ix->stream));
+    dkm_status st = launch_locked(ix, strategy, ix->s_q.p, nq, ix->s_k.p, ix->s_L.p, max_k, max_L,
+                                  beam_width, k_stride, ix->s_ids.p, ix->s_d.p, ix->s_cnt.p,
+                                  ix->s_cmps.p, ix->s_hops.p, ix->s_status.p);
+    if (st) return st;
+    std::vector<uint32_t> status(nq);
+    HIP_TRY(hipMemcpyAsync(out_ids, ix->s_ids.p, sizeof(uint32_t) * nq * k_stride, hipMemcpyDeviceToHost, ix->stream));
+    HIP_TRY(hipMemcpyAsync(out_dists, ix->s_d.p, sizeof(float) * nq * k_stride, hipMemcpyDeviceToHost, ix->stream));
+    HIP_TRY(hipMemcpyAsync(out_counts, ix->s_cnt.p, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, ix->stream));
+    if (out_cmps)
+        HIP_TRY(hipMemcpyAsync(out_cmps, ix->s_cmps.p, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, ix->stream));
+    if (out_hops)
+        HIP_TRY(hipMemcpyAsync(out_hops, ix->s_hops.p, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, ix->stream));
+    HIP_TRY(hipMemcpyAsync(status.data(), ix->s_status.p, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, ix->stream));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    for (int64_t i = 0; i < nq; ++i) {
+        if (status[i] & dkm::ERR_VISITED_OVERFLOW)
+            return fail(DKM_ERR_CAPACITY, "query %lld: visited set overflowed its LDS and spill capacity",
+                        (long long)i);
+        if (status[i] & dkm::ERR_ITER_LIMIT)
+            return fail(DKM_ERR_INDEX, "query %lld: traversal exceeded the iteration bound", (long long)i);
+    }
+    return DKM_OK;
+}
+
+dkm_status dkm_index_synchronize(dkm_index *ix) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    HIP_TRY(hipSetDevice(ix->device));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    return DKM_OK;
+}
+
+dkm_status dkm_profile(dkm_index *ix, int enable) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    for (auto &ev : ix->events) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    ix->events.clear();
+    ix->prof_launches = 0;
+    ix->prof_ms = 0.0;
+    ix->profile = enable != 0;
+    return DKM_OK;
+}
+
+dkm_status dkm_profile_read(dkm_index *ix, int64_t *launches, double *total_ms) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    for (auto &ev : ix->events) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev.first, ev.second));
+        ix->prof_ms += ms;
+        ix->prof_launches += 1;
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    ix->events.clear();
+    if (launches) *launches = ix->prof_launches;
+    if (total_ms) *total_ms = ix->prof_ms;
+    return DKM_OK;
+}
+
+dkm_status dkm_set_tuning(dkm_index *ix, int64_t visited_lds_slots) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    if (visited_lds_slots < 0 || (visited_lds_slots & (visited_lds_slots - 1)) ||
+        (visited_lds_slots > 0 && visited_lds_slots < 256))
+        return fail(DKM_ERR_INVALID_ARG, "visited slots must be 0 or a power of two >= 256");
+    std::lock_guard<std::mutex> g(ix->mu);
+    ix->tune_hslots = visited_lds_slots;
+    return DKM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,28 @@
+// Launch helpers for the distance / PQ / k-means kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dkm {
+
+hipError_t launch_reset_counter(uint32_t *counter, hipStream_t stream);
+
+// out[i*m + j] = distance(q_i, base[ids[i*m + j]])
+hipError_t launch_distance_batch(int metric, int dim, const float *queries, int64_t nq,
+                                 const float *base, int64_t base_stride, const uint32_t *ids,
+                                 int64_t m, float *out, hipStream_t stream);
+
+// lut[q][m*256 + c]; ip -> negated inner-product table (TableIP), else SquaredL2.
+hipError_t launch_pq_lut(bool ip, const float *pivots, int dim, const int32_t *offsets, int M,
+                         const float *queries, int64_t nq, float *lut, hipStream_t stream);
+
+// codes[i*code_stride + m]; *bad_row receives the smallest failing row (init 0xFFFFFFFF).
+hipError_t launch_pq_encode(const float *pivots, int K, int dim, const int32_t *offsets, int M,
+                            const float *x, int64_t x_stride, int64_t n, uint8_t *codes,
+                            int64_t code_stride, uint32_t *bad_row, hipStream_t stream);
+
+// Full PQ training on the device (host data in, host pivots out).
+hipError_t pq_train_device(const float *data, int64_t n, int dim, const int32_t *offsets, int M,
+                           int K, int iters, uint64_t seed, float *pivots);
+
+}  // namespace dkm

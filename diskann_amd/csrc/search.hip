@@ -1,0 +1,476 @@
+// Batched greedy beam search (Vamana) on gfx950: one wavefront per query.
+//
+// Restates DiskANNIndex::search_internal (diskann/src/graph/index.rs:1938-2005) with
+// the in-memory FullAccessor / QuantAccessor expand_beam
+// (diskann-providers/src/model/graph/provider/async_/inmem/full_precision.rs:274-322,
+//  .../inmem/product.rs:304-335), the NeighborPriorityQueue insert rule
+// (diskann/src/neighbor/queue.rs:130-171) and the Knn post-processing pipelines
+// (diskann/src/graph/glue.rs:730-821, full_precision.rs:357-404).
+//
+// Per-query state lives in LDS:
+//   query vector, PQ lookup table (M x 256 f32), candidate queue (double-buffered,
+//   capacity L + #start points, visited flag in bit 31 of the id), expansion list, and
+//   an open-addressing visited set that spills to a per-query region of a global
+//   (HBM) table when it fills.
+// Queue merge: the sequential "insert before equal distances, drop the tail" rule is
+// the top-C of (queue U new) under the key (distance asc, insertion order desc), so a
+// hop's expansion list is ranked in parallel and merged with two binary searches.
+#include "dkm_device.h"
+#include "dkm_search.h"
+
+namespace dkm {
+
+struct Lds {
+    float *q;        // dim
+    float *lut;      // M*256 (PQ)
+    float *qa_d, *qb_d;
+    uint32_t *qa_i, *qb_i;
+    float *nl_d, *sn_d;
+    uint32_t *nl_i, *sn_i;
+    uint32_t *hash;  // H
+    int32_t *offs;   // M+1
+    uint32_t *starts;
+    uint32_t *beam;
+    int32_t *misc;   // [0] spill region, [1] lds count
+};
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+__host__ __device__ inline size_t lds_bytes(const SearchParams &p) {
+    size_t b = 0;
+    b += align16(sizeof(float) * p.dim);
+    if (p.pq) b += align16(sizeof(float) * p.M * 256);
+    b += 4 * align16(sizeof(float) * p.qcap);            // qa_d qa_i qb_d qb_i
+    b += 4 * align16(sizeof(float) * p.nlcap);           // nl_d nl_i sn_d sn_i
+    b += align16(sizeof(uint32_t) * p.hslots);
+    if (p.pq) b += align16(sizeof(int32_t) * (p.M + 1));
+    b += align16(sizeof(uint32_t) * p.nstart);
+    b += align16(sizeof(uint32_t) * p.beam_width);
+    b += 16;
+    return b;
+}
+
+__device__ inline Lds carve(const SearchParams &p, char *base) {
+    Lds s;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        char *r = base + o;
+        o += align16(bytes);
+        return r;
+    };
+    s.q = (float *)take(sizeof(float) * p.dim);
+    s.lut = p.pq ? (float *)take(sizeof(float) * p.M * 256) : nullptr;
+    s.qa_d = (float *)take(sizeof(float) * p.qcap);
+    s.qa_i = (uint32_t *)take(sizeof(uint32_t) * p.qcap);
+    s.qb_d = (float *)take(sizeof(float) * p.qcap);
+    s.qb_i = (uint32_t *)take(sizeof(uint32_t) * p.qcap);
+    s.nl_d = (float *)take(sizeof(float) * p.nlcap);
+    s.nl_i = (uint32_t *)take(sizeof(uint32_t) * p.nlcap);
+    s.sn_d = (float *)take(sizeof(float) * p.nlcap);
+    s.sn_i = (uint32_t *)take(sizeof(uint32_t) * p.nlcap);
+    s.hash = (uint32_t *)take(sizeof(uint32_t) * p.hslots);
+    s.offs = p.pq ? (int32_t *)take(sizeof(int32_t) * (p.M + 1)) : nullptr;
+    s.starts = (uint32_t *)take(sizeof(uint32_t) * p.nstart);
+    s.beam = (uint32_t *)take(sizeof(uint32_t) * p.beam_width);
+    s.misc = (int32_t *)take(16);
+    return s;
+}
+
+// Visited-set insert for one id per lane (ids within one call are distinct).
+// Returns true if the id was not present (and is now recorded).
+__device__ inline bool visit_insert(const SearchParams &p, Lds &s, uint32_t id, bool active,
+                                    bool spill, uint64_t *gtab, uint32_t *err) {
+    if (!active) return false;
+    const uint32_t H = p.hslots;
+    uint32_t h = hash_id(id) & (H - 1);
+    if (!spill) {
+        for (uint32_t probe = 0; probe < H; ++probe) {
+            uint32_t old = atomicCAS(&s.hash[h], kEmpty, id);
+            if (old == kEmpty) return true;
+            if (old == id) return false;
+            h = (h + 1) & (H - 1);
+        }
+        atomicOr(err, ERR_VISITED_OVERFLOW);
+        return false;
+    }
+    // Spill mode: the LDS table is read-only for this chunk.
+    for (uint32_t probe = 0; probe < H; ++probe) {
+        uint32_t v = __hip_atomic_load(&s.hash[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (v == id) return false;
+        if (v == kEmpty) break;
+        h = (h + 1) & (H - 1);
+    }
+    if (gtab == nullptr) {
+        atomicOr(err, ERR_VISITED_OVERFLOW);
+        return false;
+    }
+    const uint64_t GH = p.spill_slots;
+    const uint64_t key = ((uint64_t)p.epoch << 32) | id;
+    uint64_t g = hash_id(id ^ 0x9e3779b9u) & (GH - 1);
+    for (uint64_t probe = 0; probe < GH; ++probe) {
+        uint64_t cur = __hip_atomic_load(&gtab[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (;;) {
+            if (cur == key) return false;
+            if ((uint32_t)(cur >> 32) == p.epoch) break;  // live entry of another id
+            uint64_t prev = atomicCAS((unsigned long long *)&gtab[g], (unsigned long long)cur,
+                                      (unsigned long long)key);
+            if (prev == cur) return true;
+            cur = prev;
+        }
+        g = (g + 1) & (GH - 1);
+    }
+    atomicOr(err, ERR_VISITED_OVERFLOW);
+    return false;
+}
+
+// Lane-parallel rank of new items among themselves (key: d asc, index desc), NaNs
+// excluded; writes the sorted list. Returns the number of valid (non-NaN) items.
+__device__ inline int rank_new(Lds &s, int nn) {
+    const int lane = lane_id();
+    int nvalid = 0;
+    for (int base = 0; base < nn; base += kWave) {
+        int i = base + lane;
+        bool ok = i < nn && !__builtin_isnan(s.nl_d[i < nn ? i : 0]);
+        nvalid += __builtin_popcountll(ballot(ok));
+    }
+    for (int base = 0; base < nn; base += kWave) {
+        int i = base + lane;
+        if (i < nn) {
+            float di = s.nl_d[i];
+            if (!__builtin_isnan(di)) {
+                int r = 0;
+                for (int j = 0; j < nn; ++j) {
+                    float dj = s.nl_d[j];
+                    r += (dj < di) || (dj == di && j > i);
+                }
+                s.sn_d[r] = di;
+                s.sn_i[r] = s.nl_i[i];
+            }
+        }
+    }
+    return nvalid;
+}
+
+// Merge sorted new items into the queue (qa -> qb). Returns the new size.
+__device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *&qd,
+                                  uint32_t *&qi, float *&od, uint32_t *&oi) {
+    const int lane = lane_id();
+    // new items: position = rank + #old with d_old < d  (lower bound in old queue)
+    for (int r = lane; r < nvalid; r += kWave) {
+        float d = s.sn_d[r];
+        int lo = 0, hi = size;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (qd[mid] < d) lo = mid + 1; else hi = mid;
+        }
+        int pos = r + lo;
+        if (pos < cap) {
+            od[pos] = d;
+            oi[pos] = s.sn_i[r];
+        }
+    }
+    // old items: position = j + #new with d_new <= d_old
+    for (int j = lane; j < size; j += kWave) {
+        float d = qd[j];
+        int lo = 0, hi = nvalid;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (s.sn_d[mid] <= d) lo = mid + 1; else hi = mid;
+        }
+        int pos = j + lo;
+        if (pos < cap) {
+            od[pos] = d;
+            oi[pos] = qi[j];
+        }
+    }
+    __syncthreads();
+    float *td = qd; qd = od; od = td;
+    uint32_t *ti = qi; qi = oi; oi = ti;
+    int ns = size + nvalid;
+    return ns < cap ? ns : cap;
+}
+
+template <int METRIC, bool PQ>
+__device__ inline void compute_distances(const SearchParams &p, Lds &s, int n) {
+    const int lane = lane_id();
+    if (!PQ) {
+        const int half = lane >> 5, hp = lane & 31;
+        for (int c0 = 0; c0 < n; c0 += 2) {
+            int c = c0 + half;
+            bool ok = c < n;
+            uint32_t id = ok ? s.nl_i[c] : p.starts[0];
+            const float *x = p.vectors + (size_t)id * p.vstride;
+            float d = half_distance<METRIC>(s.q, x, p.dim, hp);
+            if (ok && hp == 0) s.nl_d[c] = d;
+        }
+    } else {
+        const int M = p.M;
+        for (int c = lane; c < n; c += kWave) {
+            uint32_t id = s.nl_i[c];
+            const uint8_t *row = p.codes + (size_t)id * p.code_stride;
+            float acc = 0.0f;
+            if ((M & 15) == 0) {
+                for (int m0 = 0; m0 < M; m0 += 16) {
+                    uint4 v = *(const uint4 *)(row + m0);
+                    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int b = 0; b < 16; ++b) {
+                        uint32_t code = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+                        acc += s.lut[(m0 + b) * 256 + code];
+                    }
+                }
+            } else {
+                for (int m = 0; m < M; ++m) acc += s.lut[m * 256 + row[m]];
+            }
+            s.nl_d[c] = acc;
+        }
+    }
+    __syncthreads();
+}
+
+template <int METRIC, bool PQ>
+__global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
+    extern __shared__ __align__(16) char smem[];
+    const int qidx = blockIdx.x;
+    if (qidx >= p.nq) return;
+    const int lane = threadIdx.x;
+    Lds s = carve(p, smem);
+    const int L = (int)p.L[qidx];
+    const int k = (int)p.k[qidx];
+    const int cap = L + p.nstart;
+    uint32_t *err = p.status + qidx;
+
+    // ---- init --------------------------------------------------------------------
+    const float *qg = p.queries + (size_t)qidx * p.qstride;
+    for (int e = lane; e < p.dim; e += kWave) s.q[e] = qg[e];
+    for (uint32_t h = lane; h < p.hslots; h += kWave) s.hash[h] = kEmpty;
+    for (int i = lane; i < p.nstart; i += kWave) s.starts[i] = p.starts[i];
+    if (lane == 0) {
+        s.misc[0] = -1;
+        s.misc[1] = 0;
+    }
+    if (PQ) {
+        for (int i = lane; i <= p.M; i += kWave) s.offs[i] = p.offsets[i];
+    }
+    __syncthreads();
+    if (PQ) {
+        const bool ip = METRIC == METRIC_IP;
+        for (int e = lane; e < p.M * 256; e += kWave) {
+            int m = e >> 8, c = e & 255;
+            int o0 = s.offs[m], o1 = s.offs[m + 1];
+            const float *pv = p.pivots + (size_t)c * p.dim + o0;
+            float v = ip ? -scalar_simd_4x1<true>(s.q + o0, pv, o1 - o0)
+                         : scalar_simd_4x1<false>(s.q + o0, pv, o1 - o0);
+            s.lut[m * 256 + c] = v;
+        }
+        __syncthreads();
+    }
+
+    float *qd = s.qa_d, *od = s.qb_d;
+    uint32_t *qi = s.qa_i, *oi = s.qb_i;
+    int size = 0;
+    uint32_t cmps = 0, hops = 0, nrec = 0;
+    int lds_count = 0;
+    uint64_t *gtab = nullptr;
+
+    // start points: visited + queue insert in order
+    for (int i = lane; i < p.nstart; i += kWave) s.nl_i[i] = s.starts[i];
+    if (lane == 0) *err = 0;
+    __syncthreads();
+    for (int i = 0; i < p.nstart; ++i) {
+        // sequential inserts (start ids may repeat only in malformed input)
+        bool ins = visit_insert(p, s, s.starts[i], lane == 0, false, nullptr, err);
+        (void)ins;
+        __syncthreads();
+    }
+    lds_count = p.nstart;
+    compute_distances<METRIC, PQ>(p, s, p.nstart);
+    {
+        int nvalid = rank_new(s, p.nstart);
+        __syncthreads();
+        size = merge_queue(s, size, nvalid, cap, qd, qi, od, oi);
+    }
+    cmps = p.nstart;
+
+    // ---- main loop -----------------------------------------------------------------
+    for (uint32_t iter = 0;; ++iter) {
+        if (iter > p.max_iters) {
+            if (lane == 0) atomicOr(err, ERR_ITER_LIMIT);
+            break;
+        }
+        // beam: first beam_width unvisited entries
+        int nbeam = 0;
+        for (int base = 0; base < size && nbeam < (int)p.beam_width; base += kWave) {
+            int i = base + lane;
+            bool unv = i < size && !(qi[i < size ? i : 0] & kVisitedBit);
+            uint64_t m = ballot(unv);
+            int before = prefix_count(m);
+            int take = (int)p.beam_width - nbeam;
+            if (unv && before < take) {
+                s.beam[nbeam + before] = qi[i];
+                uint32_t rid = qi[i];
+                qi[i] = rid | kVisitedBit;
+                if (p.rec_ids && nrec + before < p.rec_cap) {
+                    p.rec_ids[(size_t)qidx * p.rec_cap + nrec + before] = rid;
+                    p.rec_d[(size_t)qidx * p.rec_cap + nrec + before] = qd[i];
+                }
+            }
+            int got = __builtin_popcountll(m);
+            nbeam += got < take ? got : take;
+        }
+        __syncthreads();
+        if (nbeam == 0) break;
+        nrec += nbeam;
+
+        // expand: neighbours in adjacency order, filtered by the visited set
+        int nn = 0;
+        for (int b = 0; b < nbeam; ++b) {
+            const uint32_t node = s.beam[b];
+            const uint32_t *row = p.graph + (size_t)node * p.gstride;
+            const int cnt = (int)row[p.gstride - 1];
+            for (int j0 = 0; j0 < cnt; j0 += kWave) {
+                int j = j0 + lane;
+                bool act = j < cnt;
+                uint32_t id = act ? row[j] : 0;
+                bool spill = lds_count + kWave > (int)(p.hslots - (p.hslots >> 3));
+                if (spill && gtab == nullptr) {
+                    if (lane == 0) {
+                        int r = -1;
+                        if (p.spill_table) {
+                            uint32_t reg = atomicAdd(p.spill_counter, 1u);
+                            r = reg < p.spill_regions ? (int)reg : -1;
+                        }
+                        s.misc[0] = r;
+                    }
+                    __syncthreads();
+                    int r = s.misc[0];
+                    if (r >= 0) gtab = p.spill_table + (size_t)r * p.spill_slots;
+                }
+                bool fresh = visit_insert(p, s, id, act, spill, gtab, err);
+                uint64_t m = ballot(fresh);
+                if (fresh) s.nl_i[nn + prefix_count(m)] = id;
+                int got = __builtin_popcountll(m);
+                nn += got;
+                if (!spill) lds_count += got;
+                __syncthreads();
+            }
+        }
+        compute_distances<METRIC, PQ>(p, s, nn);
+        int nvalid = rank_new(s, nn);
+        __syncthreads();
+        size = merge_queue(s, size, nvalid, cap, qd, qi, od, oi);
+        cmps += (uint32_t)nn;
+        hops += (uint32_t)nbeam;
+        if (*((volatile uint32_t *)err) & ERR_VISITED_OVERFLOW) break;
+    }
+
+    // ---- post-processing -----------------------------------------------------------
+    uint32_t *oids = p.out_ids + (size_t)qidx * p.k_stride;
+    float *ods = p.out_d + (size_t)qidx * p.k_stride;
+    int count = 0;
+    if (!p.rerank) {
+        // FilterStartPoints -> CopyIds: first k non-start entries in queue order
+        for (int base = 0; base < size && count < k; base += kWave) {
+            int i = base + lane;
+            bool ok = false;
+            uint32_t id = 0;
+            if (i < size) {
+                id = qi[i] & ~kVisitedBit;
+                ok = true;
+                for (int t = 0; t < p.nstart; ++t) ok = ok && (s.starts[t] != id);
+            }
+            uint64_t m = ballot(ok);
+            int pos = count + prefix_count(m);
+            if (ok && pos < k) {
+                oids[pos] = id;
+                ods[pos] = qd[i];
+            }
+            int got = __builtin_popcountll(m);
+            count = count + got < k ? count + got : k;
+        }
+    } else {
+        // FilterStartPoints -> Rerank: exact FP32 distances, stable sort, first k
+        int n = 0;
+        for (int base = 0; base < size; base += kWave) {
+            int i = base + lane;
+            bool ok = false;
+            uint32_t id = 0;
+            if (i < size) {
+                id = qi[i] & ~kVisitedBit;
+                ok = true;
+                for (int t = 0; t < p.nstart; ++t) ok = ok && (s.starts[t] != id);
+            }
+            uint64_t m = ballot(ok);
+            if (ok) s.nl_i[n + prefix_count(m)] = id;
+            n += __builtin_popcountll(m);
+        }
+        __syncthreads();
+        {
+            const int half = lane >> 5, hp = lane & 31;
+            for (int c0 = 0; c0 < n; c0 += 2) {
+                int c = c0 + half;
+                bool ok = c < n;
+                uint32_t id = ok ? s.nl_i[c] : s.starts[0];
+                const float *x = p.vectors + (size_t)id * p.vstride;
+                float d;
+                if (p.metric == METRIC_L2) d = half_distance<METRIC_L2>(s.q, x, p.dim, hp);
+                else if (p.metric == METRIC_IP) d = half_distance<METRIC_IP>(s.q, x, p.dim, hp);
+                else d = half_distance<METRIC_COSINE>(s.q, x, p.dim, hp);
+                if (ok && hp == 0) s.nl_d[c] = d;
+            }
+        }
+        __syncthreads();
+        for (int i = lane; i < n; i += kWave) {
+            float di = s.nl_d[i];
+            int r = 0;
+            for (int j = 0; j < n; ++j) {
+                float dj = s.nl_d[j];
+                r += (dj < di) || (dj == di && j < i);
+            }
+            if (r < k) {
+                oids[r] = s.nl_i[i];
+                ods[r] = di;
+            }
+        }
+        count = n < k ? n : k;
+    }
+    for (int i = count + lane; i < k; i += kWave) {
+        oids[i] = kEmpty;
+        ods[i] = __builtin_inff();
+    }
+    if (lane == 0) {
+        p.out_count[qidx] = (uint32_t)count;
+        if (p.out_cmps) p.out_cmps[qidx] = cmps;
+        if (p.out_hops) p.out_hops[qidx] = hops;
+        if (p.rec_len) p.rec_len[qidx] = nrec;
+    }
+}
+
+template <int METRIC, bool PQ>
+static hipError_t launch_t(const SearchParams &p, hipStream_t stream) {
+    size_t lds = lds_bytes(p);
+    auto kern = beam_search_kernel<METRIC, PQ>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3((unsigned)p.nq), dim3(64), lds, stream, p);
+    return hipGetLastError();
+}
+
+size_t search_lds_bytes(const SearchParams &p) { return lds_bytes(p); }
+
+hipError_t launch_search(const SearchParams &p, hipStream_t stream) {
+    if (p.pq) {
+        switch (p.metric) {
+        case METRIC_IP: return launch_t<METRIC_IP, true>(p, stream);
+        default: return launch_t<METRIC_L2, true>(p, stream);  // L2 and CosineNormalized tables
+        }
+    }
+    switch (p.metric) {
+    case METRIC_L2: return launch_t<METRIC_L2, false>(p, stream);
+    case METRIC_IP: return launch_t<METRIC_IP, false>(p, stream);
+    default: return launch_t<METRIC_COSINE, false>(p, stream);
+    }
+}
+
+}  // namespace dkm

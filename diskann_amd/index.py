@@ -1,0 +1,172 @@
+"""Host-side mirror of the reference's search interface, driving the HIP kernels.
+
+    Knn::new(k, L, beam_width)          diskann/src/graph/search/knn_search.rs:94-130
+    DiskANNIndex::search / search_with  diskann/src/graph/index.rs:2034-2070
+    FullPrecision / Hybrid / Quantized  diskann-providers/src/model/graph/provider/async_/inmem/
+    DefaultProvider storage             .../inmem/provider.rs:227-248 (frozen start rows after
+                                        the data rows, set_start_points :393-421)
+"""
+import ctypes as C
+import enum
+
+import numpy as np
+
+from . import _native
+from .errors import (BEAM_WIDTH_ZERO, K_ZERO, L_LESS_THAN_K, L_ZERO, KnnSearchError, check)
+
+
+class Metric(enum.IntEnum):
+    """diskann-vector/src/distance/metric.rs:11-20 (repr(C) order)."""
+    COSINE = 0
+    INNER_PRODUCT = 1
+    L2 = 2
+    COSINE_NORMALIZED = 3
+
+
+class Strategy(enum.IntEnum):
+    FULL_PRECISION = 0   # FullPrecision: FP32 traversal, FilterStartPoints + CopyIds
+    HYBRID = 1           # Hybrid: PQ traversal, FilterStartPoints + Rerank (exact FP32)
+    QUANTIZED = 2        # Quantized: PQ traversal, FilterStartPoints + copy PQ distances
+
+
+FullPrecision = Strategy.FULL_PRECISION
+Hybrid = Strategy.HYBRID
+Quantized = Strategy.QUANTIZED
+
+
+class Knn:
+    """k-NN search parameters with the reference's validation and error kinds."""
+
+    def __init__(self, k, L, beam_width=None):
+        if k == 0:
+            raise KnnSearchError(K_ZERO, "k must be non-zero")
+        if L == 0:
+            raise KnnSearchError(L_ZERO, "L must be non-zero")
+        if k > L:
+            raise KnnSearchError(L_LESS_THAN_K, "L (%d) is less than k (%d)" % (L, k))
+        bw = 1 if beam_width is None else beam_width
+        if bw == 0:
+            raise KnnSearchError(BEAM_WIDTH_ZERO, "beam width must be non-zero")
+        self.k, self.L, self.beam_width = int(k), int(L), int(bw)
+
+    def __repr__(self):
+        return "Knn(k=%d, L=%d, beam_width=%d)" % (self.k, self.L, self.beam_width)
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class SearchResult:
+    def __init__(self, ids, distances, counts, cmps, hops):
+        self.ids, self.distances, self.counts, self.cmps, self.hops = (
+            ids, distances, counts, cmps, hops)
+
+
+class DiskANNIndex:
+    """Device-resident in-memory index: N data rows + `num_start` frozen start rows
+    (ids N, N+1, ...), fixed-stride adjacency, optional PQ schema and codes."""
+
+    def __init__(self, metric, dim, max_points, R, R_max=None, num_start=1, device=0):
+        self.lib = _native.load()
+        check(self.lib.dkm_init(device), self.lib)
+        self.metric = Metric(metric)
+        self.dim, self.max_points, self.num_start = int(dim), int(max_points), int(num_start)
+        self.R = int(R)
+        self.R_max = int(R_max if R_max is not None else int(1.3 * R))
+        h = C.c_void_p()
+        check(self.lib.dkm_index_create(int(self.metric), self.dim, self.max_points,
+                                        self.num_start, self.R, self.R_max, C.byref(h)),
+              self.lib)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.dkm_index_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def start_ids(self):
+        return np.arange(self.max_points, self.max_points + self.num_start, dtype=np.uint32)
+
+    def set_vectors(self, x, row0=0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if x.ndim != 2 or x.shape[1] != self.dim:
+            raise ValueError("expected (n, %d) float32 vectors" % self.dim)
+        check(self.lib.dkm_set_vectors(self._h, _p(x), row0, x.shape[0]), self.lib)
+
+    def set_start_points(self, vectors):
+        v = np.ascontiguousarray(np.asarray(vectors, dtype=np.float32).reshape(self.num_start,
+                                                                               self.dim))
+        check(self.lib.dkm_set_start_points(self._h, _p(v)), self.lib)
+
+    def set_graph(self, lists, counts, row0=0):
+        lists = np.ascontiguousarray(lists, dtype=np.uint32)
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        check(self.lib.dkm_set_graph(self._h, _p(lists), _p(counts), lists.shape[1], row0,
+                                     lists.shape[0]), self.lib)
+
+    def get_graph(self, row0=0, n=None):
+        n = self.max_points + self.num_start - row0 if n is None else n
+        lists = np.zeros((n, self.R_max), dtype=np.uint32)
+        counts = np.zeros(n, dtype=np.uint32)
+        check(self.lib.dkm_get_graph(self._h, _p(lists), _p(counts), self.R_max, row0, n),
+              self.lib)
+        return lists, counts
+
+    def set_pq(self, pivots, offsets):
+        pivots = np.ascontiguousarray(pivots, dtype=np.float32)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        check(self.lib.dkm_set_pq(self._h, _p(pivots), pivots.shape[0], _p(offsets),
+                                  offsets.shape[0] - 1), self.lib)
+        self.M = offsets.shape[0] - 1
+
+    def set_codes(self, codes, row0=0):
+        codes = np.ascontiguousarray(codes, dtype=np.uint8)
+        check(self.lib.dkm_set_codes(self._h, _p(codes), row0, codes.shape[0]), self.lib)
+
+    def encode(self):
+        check(self.lib.dkm_encode_index(self._h), self.lib)
+
+    def get_codes(self, row0=0, n=None):
+        n = self.max_points + self.num_start - row0 if n is None else n
+        out = np.zeros((n, self.M), dtype=np.uint8)
+        check(self.lib.dkm_get_codes(self._h, _p(out), row0, n), self.lib)
+        return out
+
+    def set_tuning(self, visited_lds_slots=0):
+        check(self.lib.dkm_set_tuning(self._h, int(visited_lds_slots)), self.lib)
+
+    def search_batch(self, queries, k, L, beam_width=1, strategy=Strategy.FULL_PRECISION):
+        """Batched search. `k` and `L` are ints or per-query arrays (validated per query
+        exactly like Knn::new)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim == 1:
+            q = q[None, :]
+        if q.shape[1] != self.dim:
+            raise ValueError("query dim %d != index dim %d" % (q.shape[1], self.dim))
+        nq = q.shape[0]
+        ks = np.ascontiguousarray(np.broadcast_to(np.asarray(k, dtype=np.uint32), (nq,)))
+        Ls = np.ascontiguousarray(np.broadcast_to(np.asarray(L, dtype=np.uint32), (nq,)))
+        kstride = int(ks.max()) if nq else 1
+        ids = np.zeros((nq, kstride), dtype=np.uint32)
+        ds = np.zeros((nq, kstride), dtype=np.float32)
+        cnt = np.zeros(nq, dtype=np.uint32)
+        cmps = np.zeros(nq, dtype=np.uint32)
+        hops = np.zeros(nq, dtype=np.uint32)
+        check(self.lib.dkm_search_batch(self._h, int(strategy), _p(q), nq, _p(ks), _p(Ls),
+                                        int(beam_width), kstride, _p(ids), _p(ds), _p(cnt),
+                                        _p(cmps), _p(hops)), self.lib)
+        return SearchResult(ids, ds, cnt, cmps, hops)
+
+    def search(self, knn, strategy, query):
+        """Single-query form of DiskANNIndex::search(Knn, strategy, query)."""
+        r = self.search_batch(query, knn.k, knn.L, knn.beam_width, strategy)
+        n = int(r.counts[0])
+        return r.ids[0, :n], r.distances[0, :n], int(r.cmps[0]), int(r.hops[0])

@@ -1,0 +1,76 @@
+"""Product quantization and distance entry points on the device.
+
+    ChunkOffsets::partition        diskann-quantization/src/views.rs:233-302
+    populate_chunk_distances       diskann-providers/src/model/pq/fixed_chunk_pq_table.rs:149-200
+    BasicTable::compress_into      diskann-quantization/src/product/tables/basic.rs:161-193
+    LightPQTrainingParameters      diskann-quantization/src/product/train.rs:22-231
+    DistanceProvider               diskann-vector/src/distance/distance_provider.rs:44-46
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native
+from .errors import check
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def chunk_offsets(dim, num_chunks):
+    lib = _native.load()
+    out = np.zeros(num_chunks + 1, dtype=np.int64)
+    check(lib.dkm_pq_chunk_offsets(dim, num_chunks, _p(out)), lib)
+    return out
+
+
+def train_pq(data, num_chunks, num_centers=256, lloyd_iters=5, seed=0):
+    """Returns (pivots[num_centers, dim], offsets[num_chunks + 1])."""
+    lib = _native.load()
+    x = np.ascontiguousarray(data, dtype=np.float32)
+    n, dim = x.shape
+    pivots = np.zeros((num_centers, dim), dtype=np.float32)
+    offsets = np.zeros(num_chunks + 1, dtype=np.int64)
+    check(lib.dkm_pq_train(_p(x), n, dim, num_chunks, num_centers, lloyd_iters,
+                           C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF), _p(pivots), _p(offsets)), lib)
+    return pivots, offsets
+
+
+def pq_encode(pivots, offsets, x):
+    lib = _native.load()
+    pivots = np.ascontiguousarray(pivots, dtype=np.float32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    M = offsets.shape[0] - 1
+    codes = np.zeros((x.shape[0], M), dtype=np.uint8)
+    check(lib.dkm_pq_encode(_p(pivots), pivots.shape[0], pivots.shape[1], _p(offsets), M,
+                            _p(x), x.shape[0], _p(codes)), lib)
+    return codes
+
+
+def pq_lut(metric, pivots, offsets, queries):
+    lib = _native.load()
+    pivots = np.ascontiguousarray(pivots, dtype=np.float32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    q = np.ascontiguousarray(queries, dtype=np.float32)
+    if q.ndim == 1:
+        q = q[None, :]
+    M = offsets.shape[0] - 1
+    K = pivots.shape[0]
+    lut = np.zeros((q.shape[0], M, K), dtype=np.float32)
+    check(lib.dkm_pq_lut(int(metric), _p(pivots), K, pivots.shape[1], _p(offsets), M, _p(q),
+                         q.shape[0], _p(lut)), lib)
+    return lut
+
+
+def distance_batch(metric, queries, base, ids):
+    """out[i, j] = distance(queries[i], base[ids[i, j]]) in the reference's f32 order."""
+    lib = _native.load()
+    q = np.ascontiguousarray(queries, dtype=np.float32)
+    b = np.ascontiguousarray(base, dtype=np.float32)
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
+    out = np.zeros(ids.shape, dtype=np.float32)
+    check(lib.dkm_distance_batch(int(metric), q.shape[1], _p(q), q.shape[0], _p(b), b.shape[0],
+                                 _p(ids), ids.shape[1], _p(out)), lib)
+    return out

@@ -1,0 +1,160 @@
+/*
+ * diskann_mi355x: MI355X-native (gfx950) search-and-quantize path for DiskANN.
+ *
+ * C ABI. Plain pointers and sizes; the library owns all device state behind opaque
+ * handles. Every entry point returns a dkm_status (0 = success) and leaves a
+ * human-readable message in dkm_last_error() (thread-local) on failure.
+ *
+ * Reference interfaces replaced (paths relative to the SeliMeli/DiskANN v0.55.0 tree):
+ *   - DiskANNIndex::search / search_with + Knn                diskann/src/graph/index.rs:2034-2070,
+ *                                                             diskann/src/graph/search/knn_search.rs:76-217
+ *   - SearchAccessor::{start_point_distances, expand_beam}    diskann/src/graph/glue.rs:125-254
+ *   - FullPrecision / Hybrid / Quantized search strategies    diskann-providers/src/model/graph/provider/
+ *                                                             async_/inmem/{full_precision,product}.rs
+ *   - DistanceProvider::distance_comparer                     diskann-vector/src/distance/distance_provider.rs:44-46
+ *   - FixedChunkPQTable::{populate_chunk_distances,
+ *     compress_into}, QueryComputer                           diskann-providers/src/model/pq/fixed_chunk_pq_table.rs:149-455
+ *   - TrainQuantizer::train (LightPQTrainingParameters)       diskann-quantization/src/product/train.rs:74-231
+ *   - occlude_list / multi_insert                             diskann/src/graph/index.rs:2570-2782, 820-1035
+ */
+#ifndef DISKANN_MI355X_H
+#define DISKANN_MI355X_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes mirror ANNErrorKind / KnnSearchError
+ * (diskann/src/error/ann_error.rs:770-830, diskann/src/graph/search/knn_search.rs). */
+typedef enum {
+    DKM_OK = 0,
+    DKM_ERR_INVALID_ARG = 1,
+    DKM_ERR_K_ZERO = 2,             /* KnnSearchError::KZero */
+    DKM_ERR_L_ZERO = 3,             /* KnnSearchError::LZero */
+    DKM_ERR_L_LESS_THAN_K = 4,      /* KnnSearchError::LLessThanK */
+    DKM_ERR_BEAM_WIDTH_ZERO = 5,    /* KnnSearchError::BeamWidthZero */
+    DKM_ERR_DIM_MISMATCH = 6,       /* ANNErrorKind::DimensionMismatchError */
+    DKM_ERR_OUT_OF_MEMORY = 7,
+    DKM_ERR_DEVICE = 8,             /* HIP runtime failure */
+    DKM_ERR_PQ = 9,                 /* ANNErrorKind::PQError */
+    DKM_ERR_KMEANS = 10,            /* ANNErrorKind::KMeansError */
+    DKM_ERR_INDEX = 11,             /* ANNErrorKind::IndexError (bad id / graph) */
+    DKM_ERR_CAPACITY = 12,          /* a per-query device structure overflowed */
+    DKM_ERR_NOT_READY = 13          /* index lacks data the request needs */
+} dkm_status;
+
+/* Metric numbering of diskann-vector/src/distance/metric.rs:11-20 (repr(C)). */
+typedef enum { DKM_COSINE = 0, DKM_INNER_PRODUCT = 1, DKM_L2 = 2, DKM_COSINE_NORMALIZED = 3 } dkm_metric;
+
+/* Search strategies: FullPrecision (CopyIds post-processing), Hybrid (PQ traversal +
+ * FP32 Rerank), Quantized (PQ only). */
+typedef enum { DKM_STRATEGY_FULL = 0, DKM_STRATEGY_HYBRID = 1, DKM_STRATEGY_QUANTIZED = 2 } dkm_strategy;
+
+typedef struct dkm_index dkm_index;
+
+/* -------------------------------------------------------------------------------- */
+/* Library                                                                          */
+/* -------------------------------------------------------------------------------- */
+const char *dkm_last_error(void);
+const char *dkm_version(void);
+/* Select the HIP device used by subsequently created handles (default 0). */
+dkm_status dkm_init(int device);
+
+/* Device memory helpers for callers that keep inputs resident in HBM. */
+dkm_status dkm_device_alloc(size_t bytes, void **ptr);
+dkm_status dkm_device_free(void *ptr);
+dkm_status dkm_memcpy_h2d(void *dst, const void *src, size_t bytes);
+dkm_status dkm_memcpy_d2h(void *dst, const void *src, size_t bytes);
+dkm_status dkm_synchronize(void);
+
+/* -------------------------------------------------------------------------------- */
+/* Distances (DistanceProvider::distance_comparer + evaluate_similarity)             */
+/* -------------------------------------------------------------------------------- */
+/* out[i*m + j] = distance(queries[i], base[ids[i*m + j]]) with the reference's exact
+ * f32 reduction order. Host buffers. */
+dkm_status dkm_distance_batch(int metric, int64_t dim, const float *queries, int64_t nq,
+                              const float *base, int64_t nbase, const uint32_t *ids,
+                              int64_t m, float *out);
+
+/* -------------------------------------------------------------------------------- */
+/* Product quantization                                                             */
+/* -------------------------------------------------------------------------------- */
+/* ChunkOffsets::partition (diskann-quantization/src/views.rs:233-302). */
+dkm_status dkm_pq_chunk_offsets(int64_t dim, int64_t num_chunks, int64_t *offsets);
+/* populate_chunk_distances: lut[q][m*K + c] for nq queries. Host buffers. */
+dkm_status dkm_pq_lut(int metric, const float *pivots, int64_t K, int64_t dim,
+                      const int64_t *offsets, int64_t M, const float *queries, int64_t nq,
+                      float *lut);
+/* compress_into: codes[i*M + m] = argmin_c SquaredL2(x_i,m , pivot_c,m). Host buffers. */
+dkm_status dkm_pq_encode(const float *pivots, int64_t K, int64_t dim, const int64_t *offsets,
+                         int64_t M, const float *x, int64_t n, uint8_t *codes);
+/* PQ training: per-chunk k-means++ seeding (own RNG, seeded by `seed`) then `iters`
+ * Lloyd iterations. pivots: K x dim (reference layout), offsets: M+1. */
+dkm_status dkm_pq_train(const float *data, int64_t n, int64_t dim, int64_t M, int64_t K,
+                        int64_t iters, uint64_t seed, float *pivots, int64_t *offsets);
+
+/* -------------------------------------------------------------------------------- */
+/* Index (DefaultProvider + DiskANNIndex)                                           */
+/* -------------------------------------------------------------------------------- */
+/* max_points data rows plus `num_start` frozen start rows at ids max_points.. .
+ * R = pruned degree, R_max = max degree with slack (floor(1.3 R) in the reference
+ * benchmark, diskann-benchmark/src/inputs/graph_index.rs:690-692). */
+dkm_status dkm_index_create(int metric, int64_t dim, int64_t max_points, int64_t num_start,
+                            int64_t R, int64_t R_max, dkm_index **out);
+void dkm_index_destroy(dkm_index *ix);
+
+/* Rows [row0, row0+n) of the base vectors (n x dim, host). */
+dkm_status dkm_set_vectors(dkm_index *ix, const float *x, int64_t row0, int64_t n);
+/* Frozen start points: copies `num_start` vectors to rows max_points.. and makes them
+ * the search entry points (set_start_points, .../inmem/provider.rs:393-421). */
+dkm_status dkm_set_start_points(dkm_index *ix, const float *vectors);
+/* Adjacency rows [row0, row0+n): lists[i*stride + j] for j < counts[i]. Lists must be
+ * duplicate-free, ids < max_points + num_start, counts <= R_max. */
+dkm_status dkm_set_graph(dkm_index *ix, const uint32_t *lists, const uint32_t *counts,
+                         int64_t stride, int64_t row0, int64_t n);
+dkm_status dkm_get_graph(dkm_index *ix, uint32_t *lists, uint32_t *counts, int64_t stride,
+                         int64_t row0, int64_t n);
+
+/* PQ schema for the index (pivots K x dim, offsets M+1), then codes for rows. */
+dkm_status dkm_set_pq(dkm_index *ix, const float *pivots, int64_t K, const int64_t *offsets,
+                      int64_t M);
+dkm_status dkm_set_codes(dkm_index *ix, const uint8_t *codes, int64_t row0, int64_t n);
+/* Encode every stored row (data + start points) on the device with the index's PQ. */
+dkm_status dkm_encode_index(dkm_index *ix);
+dkm_status dkm_get_codes(dkm_index *ix, uint8_t *codes, int64_t row0, int64_t n);
+
+/* Batched k-NN search (Knn::new(k, L, bw) per query; k <= L, k, L, bw >= 1).
+ * k and L are per-query arrays (length nq); out_ids/out_dists have k_stride entries per
+ * query, of which out_counts[i] are valid. out_cmps/out_hops may be NULL.
+ * Host buffers; the call blocks until the results are copied back. */
+dkm_status dkm_search_batch(dkm_index *ix, int strategy, const float *queries, int64_t nq,
+                            const uint32_t *k, const uint32_t *L, uint32_t beam_width,
+                            int64_t k_stride, uint32_t *out_ids, float *out_dists,
+                            uint32_t *out_counts, uint32_t *out_cmps, uint32_t *out_hops);
+
+/* Same, with every buffer in device memory (dkm_device_alloc); asynchronous on the
+ * handle's stream. Validation of k/L is the caller's duty (see dkm_search_batch). */
+dkm_status dkm_search_batch_device(dkm_index *ix, int strategy, const float *d_queries,
+                                   int64_t nq, const uint32_t *d_k, const uint32_t *d_L,
+                                   uint32_t max_k, uint32_t max_L, uint32_t beam_width,
+                                   int64_t k_stride, uint32_t *d_out_ids, float *d_out_dists,
+                                   uint32_t *d_out_counts, uint32_t *d_out_cmps,
+                                   uint32_t *d_out_hops);
+dkm_status dkm_index_synchronize(dkm_index *ix);
+
+/* Kernel timing with HIP events on the handle's stream: enable, then read the count and
+ * total milliseconds of the search kernel launches since the last reset. */
+dkm_status dkm_profile(dkm_index *ix, int enable);
+dkm_status dkm_profile_read(dkm_index *ix, int64_t *launches, double *total_ms);
+
+/* Tuning knobs (autotuner surface): visited-set slots in LDS per query (power of two,
+ * 0 = automatic). */
+dkm_status dkm_set_tuning(dkm_index *ix, int64_t visited_lds_slots);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

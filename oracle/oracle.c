@@ -684,7 +684,7 @@ int64_t dko_occlude_list(const dko_build_cfg *cfg, const float *vectors, int64_t
     }
     for (int64_t n = 0; n < found; ++n) out[n] = pool_ids[nbr[n]];
     int64_t count = found;
-    if (cfg->saturate && alpha > 1.0f) {
+    if (cfg->saturate == 2 || (cfg->saturate && alpha > 1.0f)) {
         for (int64_t i = 0; i < pool_len && count < degree; ++i) {
             if (!valid[i]) continue;
             int dup = 0;
@@ -744,9 +744,45 @@ static void sort_pool(uint32_t *ids, float *ds, int64_t n) {
 
 /* robust_prune_list, index.rs:2388-2450: distances from `location` to every other id of
  * `list` (duplicates and self removed), sorted pool, occlude_list. */
+static int64_t prune_list_opt(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                              int64_t dim, uint32_t location, const uint32_t *list, int64_t n,
+                              uint32_t *out, int force_saturate);
 static int64_t prune_list(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
                           int64_t dim, uint32_t location, const uint32_t *list, int64_t n,
                           uint32_t *out) {
+    return prune_list_opt(cfg, vectors, vstride, dim, location, list, n, out, 0);
+}
+static int64_t push_unique(uint32_t *a, int64_t n, uint32_t v) {
+    for (int64_t i = 0; i < n; ++i)
+        if (a[i] == v) return n;
+    a[n] = v;
+    return n + 1;
+}
+static int64_t count_unique_targets(const uint32_t *nbrs, const int64_t *cnt, int64_t B,
+                                    int64_t stride) {
+    int64_t total = 0;
+    for (int64_t p = 0; p < B; ++p) total += cnt[p];
+    uint32_t *t = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(total + 1));
+    int64_t n = 0;
+    for (int64_t p = 0; p < B; ++p)
+        for (int64_t j = 0; j < cnt[p]; ++j) t[n++] = nbrs[p * stride + j];
+    /* insertion into a small sorted set is fine at oracle sizes */
+    int64_t u = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int dup = 0;
+        for (int64_t j = 0; j < u; ++j)
+            if (t[j] == t[i]) {
+                dup = 1;
+                break;
+            }
+        if (!dup) t[u++] = t[i];
+    }
+    free(t);
+    return u;
+}
+static int64_t prune_list_opt(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                              int64_t dim, uint32_t location, const uint32_t *list, int64_t n,
+                              uint32_t *out, int force_saturate) {
     uint32_t *ids = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n + 1));
     float *ds = (float *)malloc(sizeof(float) * (size_t)(n + 1));
     int64_t m = 0;
@@ -759,7 +795,9 @@ static int64_t prune_list(const dko_build_cfg *cfg, const float *vectors, int64_
     }
     sort_pool(ids, ds, m);
     if (m > cfg->max_occlusion) m = cfg->max_occlusion;
-    int64_t c = dko_occlude_list(cfg, vectors, vstride, dim, ids, ds, m, location, out);
+    dko_build_cfg c2 = *cfg;
+    if (force_saturate) c2.saturate = 2; /* forced: saturate regardless of alpha */
+    int64_t c = dko_occlude_list(&c2, vectors, vstride, dim, ids, ds, m, location, out);
     free(ids);
     free(ds);
     return c;
@@ -833,11 +871,9 @@ static int64_t search_and_prune(const dko_build_cfg *cfg, const float *vectors, 
     dko_stats st;
     search_core(&ix, 0, v, 1, &s, rid, rd, &rlen, cap, &st);
     if (rlen > cap) rlen = (uint32_t)cap;
-    /* fill() takes self, then extras, then the record, capped at max_occlusion: extras
-     * beyond the cap are unavailable in the working set */
-    int64_t avail_extras = nextras;
-    if (1 + avail_extras > cfg->max_occlusion) avail_extras = cfg->max_occlusion - 1;
-    for (int64_t e = 0; e < avail_extras; ++e) {
+    /* the in-memory providers' prune working set is a pass-through view: every id is
+     * available, so all extras join the record before sorting */
+    for (int64_t e = 0; e < nextras; ++e) {
         rid[rlen] = extras[e];
         rd[rlen] = dko_distance(cfg->metric, v, vectors + (int64_t)extras[e] * vstride, dim);
         rlen++;
@@ -877,19 +913,55 @@ int dko_build(const dko_build_cfg *cfg, const float *vectors, int64_t vstride, i
         /* candidate generation on the pre-batch graph */
         int64_t nc = intra_batch < 0 ? B : intra_batch;
         for (int64_t p = 0; p < B; ++p) {
-            /* async_tools::around(ids, p, nc): up to min(nc, B-1) ids around p, self
-             * excluded (diskann/src/utils/async_tools.rs:51-120) */
+            /* async_tools::around(ids, p, nc): a circular window of min(nc, B-1) ids
+             * starting ceil(len/2) before p, skipping p
+             * (diskann/src/utils/async_tools.rs:20-130) */
             int64_t ne = 0;
             int64_t want = nc < B - 1 ? nc : B - 1;
             if (want > 0) {
-                int64_t start = p - want / 2;
-                if (start < 0) start = 0;
-                if (start + want + 1 > B) start = B - want - 1;
-                for (int64_t q = start; q < start + want + 1 && ne < want; ++q)
-                    if (q != p) extras[ne++] = bid[q];
+                int64_t half = (want + 1) / 2;
+                int64_t pos = p >= half ? p - half : B - (half - p);
+                for (int64_t r = 0; r < want; ++r) {
+                    int64_t q = pos;
+                    pos = pos + 1 == B ? 0 : pos + 1;
+                    if (q == p) {
+                        q = pos;
+                        pos = pos + 1 == B ? 0 : pos + 1;
+                    }
+                    extras[ne++] = bid[q];
+                }
             }
             cnt[p] = search_and_prune(cfg, vectors, vstride, dim, g, starts, nstart, bid[p],
                                       extras, ne, nbrs + p * (cfg->R + 1));
+        }
+        /* bootstrap (index.rs:595-760, trigger :948-960): when the batch saw only part of
+         * itself and the back edges reach few distinct nodes, every item re-prunes its
+         * edges together with the other batch items, saturated */
+        {
+            int64_t resolved = nc < B ? nc : B;
+            if (resolved < 1) resolved = 1;
+            int64_t uniq = count_unique_targets(nbrs, cnt, B, cfg->R + 1);
+            if (resolved < B && (uniq + 7) / 8 <= B) {
+                uint32_t *cand = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(cfg->R + B + 1));
+                uint32_t *nb2 = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(B * (cfg->R + 1)));
+                int64_t *cnt2 = (int64_t *)malloc(sizeof(int64_t) * (size_t)B);
+                dko_build_cfg sat = *cfg;
+                sat.saturate = 1;
+                for (int64_t p = 0; p < B; ++p) {
+                    int64_t nc2 = 0;
+                    for (int64_t j = 0; j < cnt[p]; ++j) nc2 = push_unique(cand, nc2, nbrs[p * (cfg->R + 1) + j]);
+                    for (int64_t q = 0; q < B; ++q)
+                        if (q != p) nc2 = push_unique(cand, nc2, bid[q]);
+                    cnt2[p] = nc2 ? prune_list_opt(&sat, vectors, vstride, dim, bid[p], cand, nc2,
+                                                   nb2 + p * (cfg->R + 1), 1)
+                                  : 0;
+                }
+                memcpy(nbrs, nb2, sizeof(uint32_t) * (size_t)(B * (cfg->R + 1)));
+                memcpy(cnt, cnt2, sizeof(int64_t) * (size_t)B);
+                free(cand);
+                free(nb2);
+                free(cnt2);
+            }
         }
         for (int64_t p = 0; p < B; ++p) {
             memcpy(g->adj + (int64_t)bid[p] * g->max_degree, nbrs + p * (cfg->R + 1),

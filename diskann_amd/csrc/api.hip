@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/diskann_mi355x.h"
+#include "dkm_build.h"
 #include "dkm_kernels.h"
 #include "dkm_search.h"
 
@@ -661,6 +662,207 @@ dkm_status dkm_search_batch(dkm_index *ix, int strategy, const float *queries, i
         if (status[i] & dkm::ERR_ITER_LIMIT)
             return fail(DKM_ERR_INDEX, "query %lld: traversal exceeded the iteration bound", (long long)i);
     }
+    return DKM_OK;
+}
+
+
+// ------------------------------------------------------------------------------------
+// Graph construction (multi_insert batches)
+// ------------------------------------------------------------------------------------
+
+dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batch,
+                     int64_t intra_batch, uint32_t L_build, float alpha, int64_t max_occlusion,
+                     int saturate) {
+    if (!ix || (!ids && n > 0)) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (batch < 1 || batch > 2048) return fail(DKM_ERR_INVALID_ARG, "batch must be in [1, 2048]");
+    if (L_build == 0 || L_build > 1024) return fail(DKM_ERR_INVALID_ARG, "L_build must be in [1, 1024]");
+    if (!(alpha >= 1.0f)) return fail(DKM_ERR_INVALID_ARG, "alpha must be >= 1");
+    if (max_occlusion < 1 || max_occlusion > 4096) return fail(DKM_ERR_INVALID_ARG, "bad max_occlusion");
+    if (ix->R > 256) return fail(DKM_ERR_INVALID_ARG, "pruned degree above 256 is not supported");
+    if (!ix->starts_set) return fail(DKM_ERR_NOT_READY, "start points not set");
+    for (int64_t i = 0; i < n; ++i)
+        if (ids[i] >= ix->max_points) return fail(DKM_ERR_INDEX, "id %u is not a data row", ids[i]);
+    if (n == 0) return DKM_OK;
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    const int B = (int)batch;
+    const uint32_t rec_cap = std::max<uint32_t>(1024, 4 * L_build);
+    // device work buffers
+    DevBuf<uint32_t> bid, k1, Lb, oid, ocnt, ostat, rid, rlen, pend, pcnt, cnt, bstat, lists, lens;
+    DevBuf<float> bq, od, rd;
+    DevBuf<uint64_t> pairs, pairs2;
+    DevBuf<char> stmp;
+    HIP_TRY(bid.ensure(B));
+    HIP_TRY(k1.ensure(B));
+    HIP_TRY(Lb.ensure(B));
+    HIP_TRY(oid.ensure(B));
+    HIP_TRY(od.ensure(B));
+    HIP_TRY(ocnt.ensure(B));
+    HIP_TRY(ostat.ensure(B));
+    HIP_TRY(bq.ensure((size_t)B * ix->dim));
+    HIP_TRY(rid.ensure((size_t)B * rec_cap));
+    HIP_TRY(rd.ensure((size_t)B * rec_cap));
+    HIP_TRY(rlen.ensure(B));
+    HIP_TRY(pend.ensure((size_t)B * ix->R));
+    HIP_TRY(pcnt.ensure(B));
+    HIP_TRY(cnt.ensure(2));
+    HIP_TRY(bstat.ensure(1));
+    HIP_TRY(pairs.ensure((size_t)B * ix->R));
+    HIP_TRY(pairs2.ensure((size_t)B * ix->R));
+    size_t stmp_bytes = dkm::radix_sort_tmp_bytes((uint32_t)(B * ix->R));
+    HIP_TRY(stmp.ensure(stmp_bytes));
+    {
+        std::vector<uint32_t> ones(B, 1u), Ls(B, L_build);
+        HIP_TRY(hipMemcpy(k1.p, ones.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(Lb.p, Ls.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice));
+        uint32_t z = 0;
+        HIP_TRY(hipMemcpy(bstat.p, &z, sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    if (!ix->d_spill) {
+        HIP_TRY(hipMalloc(&ix->d_spill, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions));
+        HIP_TRY(hipMemsetAsync(ix->d_spill, 0, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions,
+                               ix->stream));
+        HIP_TRY(hipMalloc(&ix->d_spill_counter, sizeof(uint32_t)));
+    }
+    auto next_pow2_i = [](int64_t x) {
+        int64_t q = 1;
+        while (q < x) q <<= 1;
+        return q;
+    };
+    dkm::BuildParams bp{};
+    bp.metric = ix->metric;
+    bp.dim = (int)ix->dim;
+    bp.vstride = ix->vstride;
+    bp.vectors = ix->d_vec;
+    bp.graph = ix->d_graph;
+    bp.gstride = (int)ix->gstride;
+    bp.R = (int)ix->R;
+    bp.R_max = (int)ix->R_max;
+    bp.alpha = alpha;
+    bp.max_occlusion = (int)max_occlusion;
+    bp.saturate = saturate;
+    bp.intra_batch = (int)intra_batch;
+    bp.rec_ids = rid.p;
+    bp.rec_d = rd.p;
+    bp.rec_len = rlen.p;
+    bp.rec_cap = rec_cap;
+    bp.pend = pend.p;
+    bp.pend_cnt = pcnt.p;
+    bp.status = bstat.p;
+    bp.cache_accepted = (size_t)ix->R * ix->dim * sizeof(float) <= 48 * 1024;
+    std::vector<uint32_t> h_pend, h_cnt;
+    for (int64_t b0 = 0; b0 < n; b0 += B) {
+        const int nb = (int)std::min<int64_t>(B, n - b0);
+        HIP_TRY(hipMemcpyAsync(bid.p, ids + b0, sizeof(uint32_t) * nb, hipMemcpyHostToDevice, ix->stream));
+        HIP_TRY(dkm::launch_gather_rows(ix->d_vec, ix->vstride, (int)ix->dim, bid.p, nb, bq.p, ix->stream));
+        // 1. candidate generation: search the pre-batch graph, record expanded nodes
+        ix->epoch += 1;
+        if (ix->epoch == 0) {
+            HIP_TRY(hipMemsetAsync(ix->d_spill, 0, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions,
+                                   ix->stream));
+            ix->epoch = 1;
+        }
+        dkm::SearchParams sp{};
+        sp.metric = ix->metric;
+        sp.pq = 0;
+        sp.rerank = 0;
+        sp.dim = (int)ix->dim;
+        sp.vstride = ix->vstride;
+        sp.vectors = ix->d_vec;
+        sp.graph = ix->d_graph;
+        sp.gstride = (int)ix->gstride;
+        sp.starts = ix->d_starts;
+        sp.nstart = (int)ix->num_start;
+        sp.queries = bq.p;
+        sp.qstride = ix->dim;
+        sp.nq = nb;
+        sp.k = k1.p;
+        sp.L = Lb.p;
+        sp.beam_width = 1;
+        sp.k_stride = 1;
+        sp.out_ids = oid.p;
+        sp.out_d = od.p;
+        sp.out_count = ocnt.p;
+        sp.status = ostat.p;
+        sp.qcap = (int)(L_build + ix->num_start);
+        sp.nlcap = (int)std::max<int64_t>(ix->R_max, sp.qcap) + 64;
+        sp.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots
+                                        : std::min<uint32_t>(16384, std::max<uint32_t>(2048, next_pow2(64ull * L_build)));
+        sp.spill_table = ix->d_spill;
+        sp.spill_slots = ix->spill_slots;
+        sp.spill_regions = ix->spill_regions;
+        sp.spill_counter = ix->d_spill_counter;
+        sp.epoch = ix->epoch;
+        sp.max_iters = (uint32_t)std::min<int64_t>(ix->rows + 16, 0x7FFFFFFF);
+        sp.rec_ids = rid.p;
+        sp.rec_d = rd.p;
+        sp.rec_cap = rec_cap;
+        sp.rec_len = rlen.p;
+        HIP_TRY(dkm::launch_reset_counter(ix->d_spill_counter, ix->stream));
+        HIP_TRY(dkm::launch_search(sp, ix->stream));
+        // 2. prune: record + intra-batch extras
+        bp.batch = nb;
+        bp.batch_ids = bid.p;
+        int64_t want = intra_batch < 0 ? nb - 1 : std::min<int64_t>(intra_batch, nb - 1);
+        bp.sort_cap = (int)next_pow2_i(std::max<int64_t>(rec_cap + std::max<int64_t>(want, 0), 2));
+        if (dkm::build_prune_lds_bytes(bp) > 160 * 1024)
+            return fail(DKM_ERR_CAPACITY, "prune pool for batch %d does not fit in LDS", nb);
+        HIP_TRY(dkm::launch_prune_search(bp, ix->stream));
+        // 3. back edges (needed for the bootstrap decision), bootstrap, set lists
+        uint32_t npairs = 0, nheads = 0;
+        HIP_TRY(dkm::launch_backedges(bp, pairs.p, pairs2.p, cnt.p, stmp.p, stmp_bytes, &npairs,
+                                      &nheads, ix->stream));
+        int64_t resolved = std::max<int64_t>(intra_batch < 0 ? nb : std::min<int64_t>(intra_batch, nb), 1);
+        if (resolved < nb && ((int64_t)nheads + 7) / 8 <= nb) {
+            // bootstrap: candidates = own edges + every other batch item, saturated prune
+            h_pend.resize((size_t)nb * ix->R);
+            h_cnt.resize(nb);
+            HIP_TRY(hipMemcpyAsync(h_pend.data(), pend.p, sizeof(uint32_t) * nb * ix->R, hipMemcpyDeviceToHost, ix->stream));
+            HIP_TRY(hipMemcpyAsync(h_cnt.data(), pcnt.p, sizeof(uint32_t) * nb, hipMemcpyDeviceToHost, ix->stream));
+            HIP_TRY(hipStreamSynchronize(ix->stream));
+            const int stride = (int)(ix->R + nb);
+            std::vector<uint32_t> hl((size_t)nb * stride), hn(nb);
+            for (int p = 0; p < nb; ++p) {
+                uint32_t *dst = hl.data() + (size_t)p * stride;
+                int c = 0;
+                auto push = [&](uint32_t v) {
+                    for (int j = 0; j < c; ++j)
+                        if (dst[j] == v) return;
+                    dst[c++] = v;
+                };
+                for (uint32_t j = 0; j < h_cnt[p]; ++j) push(h_pend[(size_t)p * ix->R + j]);
+                for (int q2 = 0; q2 < nb; ++q2)
+                    if (q2 != p) push(ids[b0 + q2]);
+                hn[p] = (uint32_t)c;
+            }
+            HIP_TRY(lists.ensure(hl.size()));
+            HIP_TRY(lens.ensure(nb));
+            HIP_TRY(hipMemcpyAsync(lists.p, hl.data(), sizeof(uint32_t) * hl.size(), hipMemcpyHostToDevice, ix->stream));
+            HIP_TRY(hipMemcpyAsync(lens.p, hn.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, ix->stream));
+            dkm::BuildParams lp = bp;
+            lp.sort_cap = (int)next_pow2_i(stride);
+            if (dkm::build_prune_lds_bytes(lp) > 160 * 1024)
+                return fail(DKM_ERR_CAPACITY, "bootstrap pool does not fit in LDS");
+            HIP_TRY(dkm::launch_prune_lists(lp, lists.p, lens.p, stride, 1, ix->stream));
+            HIP_TRY(dkm::launch_backedges(bp, pairs.p, pairs2.p, cnt.p, stmp.p, stmp_bytes, &npairs,
+                                          &nheads, ix->stream));
+        }
+        HIP_TRY(dkm::launch_set_lists(bp, ix->stream));
+        // 4. add_edge_and_prune per target
+        if (npairs > 0) {
+            dkm::BuildParams ep = bp;
+            ep.sort_cap = (int)next_pow2_i(ix->R_max + nb + 1);
+            if (dkm::build_prune_lds_bytes(ep) > 160 * 1024)
+                return fail(DKM_ERR_CAPACITY, "back-edge pool does not fit in LDS");
+            HIP_TRY(dkm::launch_apply_backedges(ep, pairs2.p, npairs, ix->stream));
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    uint32_t st = 0;
+    HIP_TRY(hipMemcpy(&st, bstat.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (st & dkm::BUILD_ERR_RECORD) return fail(DKM_ERR_CAPACITY, "search record exceeded %u entries", rec_cap);
+    if (st & dkm::BUILD_ERR_POOL) return fail(DKM_ERR_CAPACITY, "candidate pool exceeded its capacity");
+    ix->codes_set = false;
     return DKM_OK;
 }
 

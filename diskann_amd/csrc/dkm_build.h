@@ -1,0 +1,49 @@
+// Launch interface of the graph-construction kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dkm {
+
+enum : uint32_t { BUILD_ERR_RECORD = 1u, BUILD_ERR_POOL = 2u };
+
+struct BuildParams {
+    int metric;
+    int dim;
+    int64_t vstride;
+    const float *vectors;
+    uint32_t *graph;
+    int gstride;
+    int R, R_max;
+    float alpha;
+    int max_occlusion;
+    int saturate;
+    int sort_cap;          // power of two >= candidate pool before truncation
+    int cache_accepted;    // stage accepted vectors in LDS
+    int batch;
+    const uint32_t *batch_ids;
+    int intra_batch;       // -1 = All, 0 = None, n = Max(n)
+    const uint32_t *rec_ids;
+    const float *rec_d;
+    const uint32_t *rec_len;
+    uint32_t rec_cap;
+    uint32_t *pend;        // batch x R
+    uint32_t *pend_cnt;
+    uint32_t *status;
+};
+
+size_t build_prune_lds_bytes(const BuildParams &p);
+hipError_t launch_gather_rows(const float *vectors, int64_t vstride, int dim, const uint32_t *ids,
+                              int n, float *out, hipStream_t stream);
+hipError_t launch_prune_search(const BuildParams &p, hipStream_t stream);
+hipError_t launch_prune_lists(const BuildParams &p, const uint32_t *lists, const uint32_t *lens,
+                              int stride, int force_saturate, hipStream_t stream);
+hipError_t launch_set_lists(const BuildParams &p, hipStream_t stream);
+hipError_t launch_backedges(const BuildParams &p, uint64_t *pairs, uint64_t *pairs_tmp,
+                            uint32_t *counter, void *sort_tmp, size_t sort_tmp_bytes,
+                            uint32_t *out_npairs, uint32_t *out_nheads, hipStream_t stream);
+hipError_t launch_apply_backedges(const BuildParams &p, const uint64_t *sorted_pairs,
+                                  uint32_t npairs, hipStream_t stream);
+size_t radix_sort_tmp_bytes(uint32_t n);
+
+}  // namespace dkm

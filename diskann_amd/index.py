@@ -140,6 +140,17 @@ class DiskANNIndex:
         check(self.lib.dkm_get_codes(self._h, _p(out), row0, n), self.lib)
         return out
 
+    def build(self, ids=None, batch=1, intra_batch=-1, L_build=100, alpha=1.2,
+              max_occlusion=750, saturate=False):
+        """Insert rows `ids` (default: every data row, in order) with multi_insert
+        semantics in batches of `batch` (batch 1 = insert()); intra_batch -1 = All,
+        0 = None, n = Max(n)."""
+        ids = np.arange(self.max_points, dtype=np.uint32) if ids is None else \
+            np.ascontiguousarray(ids, dtype=np.uint32)
+        check(self.lib.dkm_build(self._h, _p(ids), ids.shape[0], int(batch), int(intra_batch),
+                                 int(L_build), float(alpha), int(max_occlusion), int(saturate)),
+              self.lib)
+
     def set_tuning(self, visited_lds_slots=0):
         check(self.lib.dkm_set_tuning(self._h, int(visited_lds_slots)), self.lib)
 

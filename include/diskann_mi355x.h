@@ -150,6 +150,15 @@ dkm_status dkm_index_synchronize(dkm_index *ix);
 dkm_status dkm_profile(dkm_index *ix, int enable);
 dkm_status dkm_profile_read(dkm_index *ix, int64_t *launches, double *total_ms);
 
+/* Graph construction with multi_insert semantics (diskann/src/graph/index.rs:820-1035):
+ * the rows `ids` (vectors already set) are inserted in consecutive batches of `batch`;
+ * each batch searches the pre-batch graph with L_build, prunes (RobustPrune, alpha,
+ * max_occlusion), sets its lists and applies back edges. intra_batch: -1 = All,
+ * 0 = None, n = Max(n) intra-batch candidates. batch == 1 is insert() semantics. */
+dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batch,
+                     int64_t intra_batch, uint32_t L_build, float alpha, int64_t max_occlusion,
+                     int saturate);
+
 /* Tuning knobs (autotuner surface): visited-set slots in LDS per query (power of two,
  * 0 = automatic). */
 dkm_status dkm_set_tuning(dkm_index *ix, int64_t visited_lds_slots);

@@ -17,7 +17,8 @@ EXPORTS = [
     "dkm_index_create", "dkm_index_destroy", "dkm_set_vectors", "dkm_set_start_points",
     "dkm_set_graph", "dkm_get_graph", "dkm_set_pq", "dkm_set_codes", "dkm_encode_index",
     "dkm_get_codes", "dkm_search_batch", "dkm_search_batch_device", "dkm_index_synchronize",
-    "dkm_profile", "dkm_profile_read", "dkm_set_tuning", "dkm_build",
+    "dkm_profile", "dkm_profile_read", "dkm_set_tuning", "dkm_build", "dkm_medoid",
+    "dkm_bruteforce", "dkm_search_batch_device_stats",
 ]
 
 _lib = None
@@ -69,6 +70,10 @@ def _declare(L):
         "dkm_profile_read": [P, C.POINTER(C.c_int64), C.POINTER(C.c_double)],
         "dkm_set_tuning": [P, I64],
         "dkm_build": [P, P, I64, I64, I64, U32, C.c_float, I64, I],
+        "dkm_medoid": [P, P, C.POINTER(C.c_int64)],
+        "dkm_bruteforce": [P, P, I64, I64, P, P],
+        "dkm_search_batch_device_stats": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P,
+                                          P],
     }
     for name, args in sig.items():
         f = getattr(L, name)

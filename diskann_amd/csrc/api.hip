@@ -19,6 +19,7 @@
 
 #include "../../include/diskann_mi355x.h"
 #include "dkm_build.h"
+#include "dkm_exact.h"
 #include "dkm_kernels.h"
 #include "dkm_search.h"
 
@@ -508,7 +509,7 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
                                 const uint32_t *d_k, const uint32_t *d_L, uint32_t max_k,
                                 uint32_t max_L, uint32_t bw, int64_t k_stride, uint32_t *d_ids,
                                 float *d_d, uint32_t *d_cnt, uint32_t *d_cmps, uint32_t *d_hops,
-                                uint32_t *d_status) {
+                                uint32_t *d_status, uint32_t *d_degsum = nullptr) {
     if (strategy < 0 || strategy > 2) return fail(DKM_ERR_INVALID_ARG, "unknown strategy %d", strategy);
     if (bw == 0) return fail(DKM_ERR_BEAM_WIDTH_ZERO, "beam width must be >= 1");
     if (bw > 64) return fail(DKM_ERR_INVALID_ARG, "beam width above 64 is not supported");
@@ -562,6 +563,7 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     p.out_count = d_cnt;
     p.out_cmps = d_cmps;
     p.out_hops = d_hops;
+    p.out_degsum = d_degsum;
     p.status = d_status;
     p.qcap = (int)(max_L + ix->num_start);
     p.nlcap = (int)std::max<int64_t>(bw * ix->R_max, p.qcap) + 64;
@@ -605,6 +607,76 @@ dkm_status dkm_search_batch_device(dkm_index *ix, int strategy, const float *d_q
     return launch_locked(ix, strategy, d_queries, nq, d_k, d_L, max_k, max_L, beam_width, k_stride,
                          d_out_ids, d_out_dists, d_out_counts, d_out_cmps, d_out_hops,
                          ix->s_status.p);
+}
+
+dkm_status dkm_search_batch_device_stats(dkm_index *ix, int strategy, const float *d_queries,
+                                         int64_t nq, const uint32_t *d_k, const uint32_t *d_L,
+                                         uint32_t max_k, uint32_t max_L, uint32_t beam_width,
+                                         int64_t k_stride, uint32_t *d_out_ids,
+                                         float *d_out_dists, uint32_t *d_out_counts,
+                                         uint32_t *d_out_cmps, uint32_t *d_out_hops,
+                                         uint32_t *d_out_degsum) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    HIP_TRY(ix->s_status.ensure(std::max<int64_t>(nq, 1)));
+    return launch_locked(ix, strategy, d_queries, nq, d_k, d_L, max_k, max_L, beam_width, k_stride,
+                         d_out_ids, d_out_dists, d_out_counts, d_out_cmps, d_out_hops,
+                         ix->s_status.p, d_out_degsum);
+}
+
+dkm_status dkm_medoid(dkm_index *ix, float *out_vector, int64_t *out_row) {
+    if (!ix || !out_vector || !out_row) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    float *dmean;
+    unsigned long long *dbest;
+    HIP_TRY(hipMalloc(&dmean, sizeof(float) * ix->dim));
+    HIP_TRY(hipMalloc(&dbest, sizeof(unsigned long long)));
+    unsigned long long init = ~0ull;
+    HIP_TRY(hipMemcpy(dbest, &init, sizeof(init), hipMemcpyHostToDevice));
+    HIP_TRY(dkm::medoid_device(ix->d_vec, ix->max_points, ix->vstride, (int)ix->dim, dmean, dbest,
+                               ix->stream));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    unsigned long long best = 0;
+    HIP_TRY(hipMemcpy(&best, dbest, sizeof(best), hipMemcpyDeviceToHost));
+    (void)hipFree(dmean);
+    (void)hipFree(dbest);
+    if (best == ~0ull) {
+        *out_row = -1;
+        for (int64_t j = 0; j < ix->dim; ++j) out_vector[j] = 0.0f;
+        return DKM_OK;
+    }
+    *out_row = (int64_t)(best & 0xFFFFFFFFull);
+    HIP_TRY(hipMemcpy(out_vector, ix->d_vec + *out_row * ix->vstride, sizeof(float) * ix->dim,
+                      hipMemcpyDeviceToHost));
+    return DKM_OK;
+}
+
+dkm_status dkm_bruteforce(dkm_index *ix, const float *queries, int64_t nq, int64_t k,
+                          uint32_t *out_ids, double *out_dists) {
+    if (!ix || (nq > 0 && (!queries || !out_ids || !out_dists)))
+        return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (k <= 0 || k > 1024) return fail(DKM_ERR_INVALID_ARG, "k must be in [1, 1024]");
+    if (nq == 0) return DKM_OK;
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    float *dq;
+    uint32_t *di;
+    double *dd;
+    HIP_TRY(hipMalloc(&dq, sizeof(float) * nq * ix->dim));
+    HIP_TRY(hipMalloc(&di, sizeof(uint32_t) * nq * k));
+    HIP_TRY(hipMalloc(&dd, sizeof(double) * nq * k));
+    HIP_TRY(hipMemcpy(dq, queries, sizeof(float) * nq * ix->dim, hipMemcpyHostToDevice));
+    HIP_TRY(dkm::bruteforce_device(ix->metric, ix->d_vec, ix->max_points, ix->vstride, (int)ix->dim,
+                                   dq, nq, (int)k, di, dd, ix->stream));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    HIP_TRY(hipMemcpy(out_ids, di, sizeof(uint32_t) * nq * k, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(out_dists, dd, sizeof(double) * nq * k, hipMemcpyDeviceToHost));
+    (void)hipFree(dq);
+    (void)hipFree(di);
+    (void)hipFree(dd);
+    return DKM_OK;
 }
 
 dkm_status dkm_search_batch(dkm_index *ix, int strategy, const float *queries, int64_t nq,

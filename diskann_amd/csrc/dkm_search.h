@@ -35,6 +35,7 @@ struct SearchParams {
     uint32_t *out_count;
     uint32_t *out_cmps;    // optional
     uint32_t *out_hops;    // optional
+    uint32_t *out_degsum;  // optional: sum of the stored counts of expanded lists
     uint32_t *status;      // per query error bits
     int qcap;              // max L + nstart in the launch
     int nlcap;             // expansion list capacity

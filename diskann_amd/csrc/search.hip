@@ -269,7 +269,7 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     float *qd = s.qa_d, *od = s.qb_d;
     uint32_t *qi = s.qa_i, *oi = s.qb_i;
     int size = 0;
-    uint32_t cmps = 0, hops = 0, nrec = 0;
+    uint32_t cmps = 0, hops = 0, nrec = 0, degsum = 0;
     int lds_count = 0;
     uint64_t *gtab = nullptr;
 
@@ -328,6 +328,7 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
             const uint32_t node = s.beam[b];
             const uint32_t *row = p.graph + (size_t)node * p.gstride;
             const int cnt = (int)row[p.gstride - 1];
+            degsum += (uint32_t)cnt;
             for (int j0 = 0; j0 < cnt; j0 += kWave) {
                 int j = j0 + lane;
                 bool act = j < cnt;
@@ -443,6 +444,7 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
         if (p.out_cmps) p.out_cmps[qidx] = cmps;
         if (p.out_hops) p.out_hops[qidx] = hops;
         if (p.rec_len) p.rec_len[qidx] = nrec;
+        if (p.out_degsum) p.out_degsum[qidx] = degsum;
     }
 }
 

@@ -151,6 +151,22 @@ class DiskANNIndex:
                                  int(L_build), float(alpha), int(max_occlusion), int(saturate)),
               self.lib)
 
+    def medoid(self):
+        """(row, vector) of compute_medoid over the data rows."""
+        v = np.zeros(self.dim, dtype=np.float32)
+        row = C.c_int64()
+        check(self.lib.dkm_medoid(self._h, _p(v), C.byref(row)), self.lib)
+        return int(row.value), v
+
+    def bruteforce(self, queries, k):
+        """Exact FP64 top-k over the data rows: (ids[nq, k], dists[nq, k] float64)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        ids = np.zeros((q.shape[0], k), dtype=np.uint32)
+        ds = np.zeros((q.shape[0], k), dtype=np.float64)
+        check(self.lib.dkm_bruteforce(self._h, _p(q), q.shape[0], int(k), _p(ids), _p(ds)),
+              self.lib)
+        return ids, ds
+
     def set_tuning(self, visited_lds_slots=0):
         check(self.lib.dkm_set_tuning(self._h, int(visited_lds_slots)), self.lib)
 

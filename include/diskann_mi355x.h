@@ -159,6 +159,25 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
                      int64_t intra_batch, uint32_t L_build, float alpha, int64_t max_occlusion,
                      int saturate);
 
+/* compute_medoid over the data rows (diskann-utils/src/sampling/medoid.rs:15-47):
+ * writes the medoid row's vector (dim floats) and its row id (-1 if none qualifies). */
+dkm_status dkm_medoid(dkm_index *ix, float *out_vector, int64_t *out_row);
+
+/* Exact FP64 brute-force top-k over the data rows (recall ground truth): out_ids and
+ * out_dists hold k entries per query, ties broken by smaller id. Host buffers. */
+dkm_status dkm_bruteforce(dkm_index *ix, const float *queries, int64_t nq, int64_t k,
+                          uint32_t *out_ids, double *out_dists);
+
+/* Same as dkm_search_batch_device plus the per-query sum of the stored neighbor counts
+ * of every expanded node (the deg_read term of the traversal's byte count). */
+dkm_status dkm_search_batch_device_stats(dkm_index *ix, int strategy, const float *d_queries,
+                                         int64_t nq, const uint32_t *d_k, const uint32_t *d_L,
+                                         uint32_t max_k, uint32_t max_L, uint32_t beam_width,
+                                         int64_t k_stride, uint32_t *d_out_ids,
+                                         float *d_out_dists, uint32_t *d_out_counts,
+                                         uint32_t *d_out_cmps, uint32_t *d_out_hops,
+                                         uint32_t *d_out_degsum);
+
 /* Tuning knobs (autotuner surface): visited-set slots in LDS per query (power of two,
  * 0 = automatic). */
 dkm_status dkm_set_tuning(dkm_index *ix, int64_t visited_lds_slots);

@@ -1014,8 +1014,8 @@ int64_t dko_medoid(const float *x, int64_t n, int64_t stride, int64_t dim, float
         for (int64_t j = 0; j < dim; ++j) sum[j] += (double)x[i * stride + j];
     float *mean = (float *)malloc(sizeof(float) * (size_t)dim);
     for (int64_t j = 0; j < dim; ++j) mean[j] = (float)(sum[j] / (double)n);
-    int64_t best = 0;
-    float bd = INFINITY;
+    int64_t best = -1;
+    float bd = FLT_MAX;
     for (int64_t i = 0; i < n; ++i) {
         float d = simd_4x1(0, mean, x + i * stride, dim);
         if (d < bd) {

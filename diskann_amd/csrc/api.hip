@@ -91,9 +91,8 @@ struct dkm_index {
     std::vector<int64_t> h_offsets;
     // visited-set spill
     uint64_t *d_spill = nullptr;
-    uint32_t *d_spill_counter = nullptr;
-    uint32_t spill_regions = 256;
-    uint64_t spill_slots = 1u << 16;
+    int64_t spill_regions = 0;          // allocated regions (one per query of a launch)
+    uint64_t spill_slots = 1u << 14;
     uint32_t epoch = 0;
     int64_t tune_hslots = 0;
     // staging for the host-buffer API
@@ -332,7 +331,7 @@ void dkm_index_destroy(dkm_index *ix) {
         (void)hipEventDestroy(ev.second);
     }
     void *ptrs[] = {ix->d_vec, ix->d_graph, ix->d_starts, ix->d_pivots, ix->d_offsets,
-                    ix->d_codes, ix->d_spill, ix->d_spill_counter};
+                    ix->d_codes, ix->d_spill};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     ix->s_q.release();
@@ -504,6 +503,27 @@ static uint32_t next_pow2(uint64_t x) {
     return p;
 }
 
+// Per-query visited-set spill regions (HBM) for `nq` queries; entries are tagged with a
+// launch epoch, so regions never need clearing except when the epoch wraps.
+static dkm_status ensure_spill(dkm_index *ix, int64_t nq) {
+    if (ix->spill_regions < nq) {
+        if (ix->d_spill) HIP_TRY(hipFree(ix->d_spill));
+        ix->d_spill = nullptr;
+        ix->spill_regions = 0;
+        size_t bytes = sizeof(uint64_t) * ix->spill_slots * (size_t)nq;
+        HIP_TRY(hipMalloc(&ix->d_spill, bytes));
+        HIP_TRY(hipMemsetAsync(ix->d_spill, 0, bytes, ix->stream));
+        ix->spill_regions = nq;
+    }
+    ix->epoch += 1;
+    if (ix->epoch == 0) {  // wrapped: stale tags could collide, clear the table
+        HIP_TRY(hipMemsetAsync(ix->d_spill, 0, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions,
+                               ix->stream));
+        ix->epoch = 1;
+    }
+    return DKM_OK;
+}
+
 // Launch on the handle's stream; caller holds ix->mu.
 static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_queries, int64_t nq,
                                 const uint32_t *d_k, const uint32_t *d_L, uint32_t max_k,
@@ -523,17 +543,9 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     if (pq && ix->metric == DKM_COSINE)
         return fail(DKM_ERR_PQ, "cosine PQ traversal (direct distances) is not supported");
     if (nq == 0) return DKM_OK;
-    if (!ix->d_spill) {
-        HIP_TRY(hipMalloc(&ix->d_spill, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions));
-        HIP_TRY(hipMemsetAsync(ix->d_spill, 0, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions,
-                               ix->stream));
-        HIP_TRY(hipMalloc(&ix->d_spill_counter, sizeof(uint32_t)));
-    }
-    ix->epoch += 1;
-    if (ix->epoch == 0) {  // wrapped: stale tags could collide, clear the table
-        HIP_TRY(hipMemsetAsync(ix->d_spill, 0, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions,
-                               ix->stream));
-        ix->epoch = 1;
+    {
+        dkm_status st = ensure_spill(ix, nq);
+        if (st) return st;
     }
     dkm::SearchParams p{};
     p.metric = ix->metric;
@@ -571,14 +583,11 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
                                    : std::min<uint32_t>(16384, std::max<uint32_t>(2048, next_pow2(64ull * max_L)));
     p.spill_table = ix->d_spill;
     p.spill_slots = ix->spill_slots;
-    p.spill_regions = ix->spill_regions;
-    p.spill_counter = ix->d_spill_counter;
     p.epoch = ix->epoch;
     p.max_iters = (uint32_t)std::min<int64_t>(ix->rows + 16, 0x7FFFFFFF);
     size_t lds = dkm::search_lds_bytes(p);
     if (lds > 160 * 1024)
         return fail(DKM_ERR_CAPACITY, "per-query LDS need %zu bytes exceeds 160 KiB", lds);
-    HIP_TRY(dkm::launch_reset_counter(ix->d_spill_counter, ix->stream));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ix->profile) {
         HIP_TRY(hipEventCreate(&e0));
@@ -790,12 +799,6 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
         uint32_t z = 0;
         HIP_TRY(hipMemcpy(bstat.p, &z, sizeof(uint32_t), hipMemcpyHostToDevice));
     }
-    if (!ix->d_spill) {
-        HIP_TRY(hipMalloc(&ix->d_spill, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions));
-        HIP_TRY(hipMemsetAsync(ix->d_spill, 0, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions,
-                               ix->stream));
-        HIP_TRY(hipMalloc(&ix->d_spill_counter, sizeof(uint32_t)));
-    }
     auto next_pow2_i = [](int64_t x) {
         int64_t q = 1;
         while (q < x) q <<= 1;
@@ -828,11 +831,9 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
         HIP_TRY(hipMemcpyAsync(bid.p, ids + b0, sizeof(uint32_t) * nb, hipMemcpyHostToDevice, ix->stream));
         HIP_TRY(dkm::launch_gather_rows(ix->d_vec, ix->vstride, (int)ix->dim, bid.p, nb, bq.p, ix->stream));
         // 1. candidate generation: search the pre-batch graph, record expanded nodes
-        ix->epoch += 1;
-        if (ix->epoch == 0) {
-            HIP_TRY(hipMemsetAsync(ix->d_spill, 0, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions,
-                                   ix->stream));
-            ix->epoch = 1;
+        {
+            dkm_status st = ensure_spill(ix, nb);
+            if (st) return st;
         }
         dkm::SearchParams sp{};
         sp.metric = ix->metric;
@@ -862,15 +863,12 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
                                         : std::min<uint32_t>(16384, std::max<uint32_t>(2048, next_pow2(64ull * L_build)));
         sp.spill_table = ix->d_spill;
         sp.spill_slots = ix->spill_slots;
-        sp.spill_regions = ix->spill_regions;
-        sp.spill_counter = ix->d_spill_counter;
         sp.epoch = ix->epoch;
         sp.max_iters = (uint32_t)std::min<int64_t>(ix->rows + 16, 0x7FFFFFFF);
         sp.rec_ids = rid.p;
         sp.rec_d = rd.p;
         sp.rec_cap = rec_cap;
         sp.rec_len = rlen.p;
-        HIP_TRY(dkm::launch_reset_counter(ix->d_spill_counter, ix->stream));
         HIP_TRY(dkm::launch_search(sp, ix->stream));
         // 2. prune: record + intra-batch extras
         bp.batch = nb;
@@ -934,7 +932,6 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
     HIP_TRY(hipMemcpy(&st, bstat.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (st & dkm::BUILD_ERR_RECORD) return fail(DKM_ERR_CAPACITY, "search record exceeded %u entries", rec_cap);
     if (st & dkm::BUILD_ERR_POOL) return fail(DKM_ERR_CAPACITY, "candidate pool exceeded its capacity");
-    ix->codes_set = false;
     return DKM_OK;
 }
 

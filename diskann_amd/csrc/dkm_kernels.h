@@ -5,8 +5,6 @@
 
 namespace dkm {
 
-hipError_t launch_reset_counter(uint32_t *counter, hipStream_t stream);
-
 // out[i*m + j] = distance(q_i, base[ids[i*m + j]])
 hipError_t launch_distance_batch(int metric, int dim, const float *queries, int64_t nq,
                                  const float *base, int64_t base_stride, const uint32_t *ids,

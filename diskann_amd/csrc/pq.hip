@@ -18,13 +18,6 @@
 
 namespace dkm {
 
-__global__ void reset_counter_kernel(uint32_t *c) { *c = 0; }
-
-hipError_t launch_reset_counter(uint32_t *counter, hipStream_t stream) {
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
-    return hipGetLastError();
-}
-
 // ---------------------------------------------------------------------------------
 // distance batch: one half-wave per (query, candidate) pair
 // ---------------------------------------------------------------------------------

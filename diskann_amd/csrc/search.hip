@@ -334,19 +334,8 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
                 bool act = j < cnt;
                 uint32_t id = act ? row[j] : 0;
                 bool spill = lds_count + kWave > (int)(p.hslots - (p.hslots >> 3));
-                if (spill && gtab == nullptr) {
-                    if (lane == 0) {
-                        int r = -1;
-                        if (p.spill_table) {
-                            uint32_t reg = atomicAdd(p.spill_counter, 1u);
-                            r = reg < p.spill_regions ? (int)reg : -1;
-                        }
-                        s.misc[0] = r;
-                    }
-                    __syncthreads();
-                    int r = s.misc[0];
-                    if (r >= 0) gtab = p.spill_table + (size_t)r * p.spill_slots;
-                }
+                if (spill && gtab == nullptr && p.spill_table)
+                    gtab = p.spill_table + (size_t)qidx * p.spill_slots;  // this query's region
                 bool fresh = visit_insert(p, s, id, act, spill, gtab, err);
                 uint64_t m = ballot(fresh);
                 if (fresh) s.nl_i[nn + prefix_count(m)] = id;

@@ -126,20 +126,22 @@ def main():
     ddeg = DeviceBuf(lib, nbytes=4 * NQ)
     h = ix._h
     strat = int(dk.Hybrid)
+    BWH = [1]
 
     def set_L(L):
         Ls = np.full(NQ, L, np.uint32)
         check(lib.dkm_memcpy_h2d(dL.ptr, C.c_void_p(Ls.ctypes.data), Ls.nbytes), lib)
 
-    def launch(L, nq=NQ, qoff=0):
+    def launch(L, nq=NQ, qoff=0, bw=None):
+        bw = BWH[0] if bw is None else bw
         check(lib.dkm_search_batch_device_stats(
-            h, strat, C.c_void_p(dq.ptr.value + 4 * D * qoff), nq, dk_.ptr, dL.ptr, K, L, 1, K,
+            h, strat, C.c_void_p(dq.ptr.value + 4 * D * qoff), nq, dk_.ptr, dL.ptr, K, L, bw, K,
             dids.ptr, dd.ptr, dcnt.ptr, dcm.ptr, dhp.ptr, ddeg.ptr), lib)
 
     def sync():
         check(lib.dkm_index_synchronize(h), lib)
 
-    # recall sweep
+    # recall sweep at beam width 1: smallest L reaching recall@10 >= 0.95
     sweep = {}
     L_star = None
     for L in (32, 64, 128, 192, 256, 384, 512, 768):
@@ -155,6 +157,27 @@ def main():
     if L_star is None:
         L_star = max(sweep, key=lambda l: sweep[l])
     set_L(L_star)
+
+    # beam width and result-preserving kernel knobs: autotuned on a 2k-query sample
+    tune_q = qs[:2000]
+    best = None
+    for bw in (1, 2, 4, 8):
+        hs, am = ix.autotune(tune_q, K, L_star, bw, dk.Hybrid)
+        launch(L_star, bw=bw)
+        sync()
+        rec_bw = recall_at_10(dids.get(np.uint32, (NQ, K)), gt)
+        t0 = time.perf_counter()
+        for _ in range(2):
+            launch(L_star, bw=bw)
+        sync()
+        dt = (time.perf_counter() - t0) / 2
+        log("bw=%d visited_slots=%d adc_mode=%d recall=%.4f qps=%.0f" % (bw, hs, am, rec_bw,
+                                                                      NQ / dt))
+        if rec_bw >= min(0.95, sweep[L_star]) and (best is None or dt < best[0]):
+            best = (dt, bw, hs, am)
+    _, BW, HSLOTS, ADC = best
+    ix.set_tuning(HSLOTS, ADC)
+    BWH[0] = BW
 
     # timed batched steps
     for _ in range(args.warmup):
@@ -205,9 +228,9 @@ def main():
     oix = po.OracleIndex(po.L2, vec, lists, counts, [N], codes, pivots, offsets)
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     probe = 200
-    _, _, _, _, w, _ = oix.search_batch(qs[:probe], K, L_star, 1, po.MODE_HYBRID, threads)
+    _, _, _, _, w, _ = oix.search_batch(qs[:probe], K, L_star, BW, po.MODE_HYBRID, threads)
     ncpu = int(min(NQ, max(probe, probe * args.cpu_seconds / max(w, 1e-6))))
-    cids, _, ccm, _, cwall, _ = oix.search_batch(qs[:ncpu], K, L_star, 1, po.MODE_HYBRID,
+    cids, _, ccm, _, cwall, _ = oix.search_batch(qs[:ncpu], K, L_star, BW, po.MODE_HYBRID,
                                                  threads)
     cpu_qps = ncpu / cwall
     parity = bool(np.array_equal(cids, ids[:ncpu]) and np.array_equal(ccm, cmps[:ncpu]))
@@ -236,7 +259,8 @@ def main():
                         "Vamana R=64 alpha=1.2, 10k queries/launch, k=10, Hybrid "
                         "(ADC traversal + FP32 rerank)",
             "n": N, "dim": D, "queries_per_launch": NQ, "k": K, "L": L_star,
-            "beam_width": 1, "L_build": L_BUILD, "build_batch": 1024,
+            "beam_width": BW, "visited_lds_slots": HSLOTS, "adc_mode": ADC,
+            "L_build": L_BUILD, "build_batch": 1024,
             "recall_at_10": rec, "recall_by_L": {str(k_): v for k_, v in sweep.items()},
             "p50_ms": float(np.percentile(step_ms, 50)),
             "p99_ms": float(np.percentile(step_ms, 99)),
@@ -256,8 +280,8 @@ def main():
         },
         "cpu_baseline": {
             "value": cpu_qps, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": "%d of the 10k queries, same index, Hybrid L=%d, static partition over "
-                      "%d threads (oracle/oracle.c)" % (ncpu, L_star, threads),
+            "sample": "%d of the 10k queries, same index, Hybrid L=%d beam %d, static "
+                      "partition over %d threads (oracle/oracle.c)" % (ncpu, L_star, BW, threads),
         },
     }
     print(json.dumps(out), flush=True)

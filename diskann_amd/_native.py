@@ -18,7 +18,7 @@ EXPORTS = [
     "dkm_set_graph", "dkm_get_graph", "dkm_set_pq", "dkm_set_codes", "dkm_encode_index",
     "dkm_get_codes", "dkm_search_batch", "dkm_search_batch_device", "dkm_index_synchronize",
     "dkm_profile", "dkm_profile_read", "dkm_set_tuning", "dkm_build", "dkm_medoid",
-    "dkm_bruteforce", "dkm_search_batch_device_stats",
+    "dkm_bruteforce", "dkm_search_batch_device_stats", "dkm_autotune",
 ]
 
 _lib = None
@@ -68,10 +68,11 @@ def _declare(L):
         "dkm_index_synchronize": [P],
         "dkm_profile": [P, I],
         "dkm_profile_read": [P, C.POINTER(C.c_int64), C.POINTER(C.c_double)],
-        "dkm_set_tuning": [P, I64],
+        "dkm_set_tuning": [P, I64, I],
         "dkm_build": [P, P, I64, I64, I64, U32, C.c_float, I64, I],
         "dkm_medoid": [P, P, C.POINTER(C.c_int64)],
         "dkm_bruteforce": [P, P, I64, I64, P, P],
+        "dkm_autotune": [P, I, P, I64, U32, U32, U32, C.POINTER(C.c_int64), C.POINTER(C.c_int)],
         "dkm_search_batch_device_stats": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P,
                                           P],
     }

@@ -95,6 +95,8 @@ struct dkm_index {
     uint64_t spill_slots = 1u << 14;
     uint32_t epoch = 0;
     int64_t tune_hslots = 0;
+    int tune_adc_mode = 1;              // PQ distances: 0 = LDS table, 1 = direct
+    int pq_len = 0;                     // uniform PQ chunk width usable by the fast path
     // staging for the host-buffer API
     DevBuf<float> s_q, s_d;
     DevBuf<uint32_t> s_k, s_L, s_ids, s_cnt, s_cmps, s_hops, s_status;
@@ -444,6 +446,13 @@ dkm_status dkm_set_pq(dkm_index *ix, const float *pivots, int64_t K, const int64
     ix->M = M;
     ix->K = K;
     ix->h_offsets.assign(offsets, offsets + M + 1);
+    ix->pq_len = 0;
+    if (M % 16 == 0) {
+        int64_t w = offsets[1] - offsets[0];
+        bool uniform = true;
+        for (int64_t i = 0; i < M; ++i) uniform = uniform && (offsets[i + 1] - offsets[i] == w);
+        if (uniform && (w == 2 || w == 4)) ix->pq_len = (int)w;
+    }
     std::vector<int32_t> o32(offsets, offsets + M + 1);
     HIP_TRY(hipMalloc(&ix->d_pivots, sizeof(float) * K * ix->dim));
     HIP_TRY(hipMalloc(&ix->d_offsets, sizeof(int32_t) * (M + 1)));
@@ -551,6 +560,8 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     p.metric = ix->metric;
     p.pq = pq;
     p.rerank = strategy == DKM_STRATEGY_HYBRID;
+    p.adc_mode = ix->tune_adc_mode;
+    p.pq_len = pq ? ix->pq_len : 0;
     p.dim = (int)ix->dim;
     p.vstride = ix->vstride;
     p.vectors = ix->d_vec;
@@ -578,7 +589,7 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     p.out_degsum = d_degsum;
     p.status = d_status;
     p.qcap = (int)(max_L + ix->num_start);
-    p.nlcap = (int)std::max<int64_t>(bw * ix->R_max, p.qcap) + 64;
+    p.nlcap = (int)(bw * ix->R_max + 64);
     p.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots
                                    : std::min<uint32_t>(16384, std::max<uint32_t>(2048, next_pow2(64ull * max_L)));
     p.spill_table = ix->d_spill;
@@ -858,7 +869,7 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
         sp.out_count = ocnt.p;
         sp.status = ostat.p;
         sp.qcap = (int)(L_build + ix->num_start);
-        sp.nlcap = (int)std::max<int64_t>(ix->R_max, sp.qcap) + 64;
+        sp.nlcap = (int)(ix->R_max + 64);
         sp.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots
                                         : std::min<uint32_t>(16384, std::max<uint32_t>(2048, next_pow2(64ull * L_build)));
         sp.spill_table = ix->d_spill;
@@ -975,13 +986,107 @@ dkm_status dkm_profile_read(dkm_index *ix, int64_t *launches, double *total_ms) 
     return DKM_OK;
 }
 
-dkm_status dkm_set_tuning(dkm_index *ix, int64_t visited_lds_slots) {
+dkm_status dkm_autotune(dkm_index *ix, int strategy, const float *d_queries, int64_t nq,
+                        uint32_t k, uint32_t L, uint32_t beam_width,
+                        int64_t *out_visited_lds_slots, int *out_adc_mode) {
+    if (!ix || !d_queries || nq <= 0) return fail(DKM_ERR_INVALID_ARG, "bad autotune arguments");
+    if (k == 0 || L == 0 || k > L) return fail(DKM_ERR_INVALID_ARG, "bad k / L");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    DevBuf<uint32_t> dk, dl, ids, cnt, cm, st;
+    DevBuf<float> dd;
+    HIP_TRY(dk.ensure(nq));
+    HIP_TRY(dl.ensure(nq));
+    HIP_TRY(ids.ensure(nq * k));
+    HIP_TRY(dd.ensure(nq * k));
+    HIP_TRY(cnt.ensure(nq));
+    HIP_TRY(cm.ensure(nq));
+    HIP_TRY(st.ensure(nq));
+    {
+        std::vector<uint32_t> hk(nq, k), hl(nq, L);
+        HIP_TRY(hipMemcpy(dk.p, hk.data(), 4 * nq, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dl.p, hl.data(), 4 * nq, hipMemcpyHostToDevice));
+    }
+    const int64_t save_h = ix->tune_hslots;
+    const int save_a = ix->tune_adc_mode;
+    auto timed = [&](int64_t hs, int adc, double *ms) -> dkm_status {
+        ix->tune_hslots = hs;
+        ix->tune_adc_mode = adc;
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        dkm_status s2 = launch_locked(ix, strategy, d_queries, nq, dk.p, dl.p, k, L, beam_width, k,
+                                      ids.p, dd.p, cnt.p, cm.p, nullptr, st.p);  // warm
+        if (s2) return s2;
+        HIP_TRY(hipEventRecord(e0, ix->stream));
+        s2 = launch_locked(ix, strategy, d_queries, nq, dk.p, dl.p, k, L, beam_width, k, ids.p,
+                           dd.p, cnt.p, cm.p, nullptr, st.p);
+        if (s2) return s2;
+        HIP_TRY(hipEventRecord(e1, ix->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float f = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = f;
+        return DKM_OK;
+    };
+    // 1. measure comparisons with a generous table
+    double ms = 0;
+    dkm_status s1 = timed(16384, 1, &ms);
+    if (s1) {
+        ix->tune_hslots = save_h;
+        ix->tune_adc_mode = save_a;
+        return s1;
+    }
+    std::vector<uint32_t> hc(nq);
+    HIP_TRY(hipMemcpy(hc.data(), cm.p, 4 * nq, hipMemcpyDeviceToHost));
+    std::sort(hc.begin(), hc.end());
+    const double p90 = hc[(size_t)((nq - 1) * 0.9)];
+    // 2. candidates: tables holding the p90 visited count at ~60/75/87% load, plus auto
+    std::vector<int64_t> sizes;
+    for (double f : {1.0 / 0.6, 1.0 / 0.75, 1.0 / 0.85}) {
+        int64_t hs = ((int64_t)(p90 * f) + 64 + ix->num_start + 63) / 64 * 64;
+        sizes.push_back(std::min<int64_t>(32768, std::max<int64_t>(256, hs)));
+    }
+    sizes.push_back(0);
+    const bool pq = strategy != DKM_STRATEGY_FULL;
+    double best = 1e30;
+    int64_t best_h = 0;
+    int best_a = 1;
+    for (int adc : pq ? std::vector<int>{0, 1} : std::vector<int>{1}) {
+        for (int64_t hs : sizes) {
+            double t = 0;
+            dkm_status s2 = timed(hs, adc, &t);
+            if (s2 == DKM_ERR_CAPACITY) continue;  // configuration does not fit in LDS
+            if (s2) {
+                ix->tune_hslots = save_h;
+                ix->tune_adc_mode = save_a;
+                return s2;
+            }
+            if (t < best) {
+                best = t;
+                best_h = hs;
+                best_a = adc;
+            }
+        }
+    }
+    ix->tune_hslots = best_h;
+    ix->tune_adc_mode = best_a;
+    if (out_visited_lds_slots) *out_visited_lds_slots = best_h;
+    if (out_adc_mode) *out_adc_mode = best_a;
+    return DKM_OK;
+}
+
+dkm_status dkm_set_tuning(dkm_index *ix, int64_t visited_lds_slots, int adc_mode) {
     if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
-    if (visited_lds_slots < 0 || (visited_lds_slots & (visited_lds_slots - 1)) ||
-        (visited_lds_slots > 0 && visited_lds_slots < 256))
-        return fail(DKM_ERR_INVALID_ARG, "visited slots must be 0 or a power of two >= 256");
+    if (visited_lds_slots < 0 || (visited_lds_slots > 0 && visited_lds_slots < 256) ||
+        visited_lds_slots > 32768)
+        return fail(DKM_ERR_INVALID_ARG, "visited slots must be 0 or in [256, 32768]");
+    if (adc_mode != 0 && adc_mode != 1) return fail(DKM_ERR_INVALID_ARG, "adc_mode must be 0 or 1");
     std::lock_guard<std::mutex> g(ix->mu);
     ix->tune_hslots = visited_lds_slots;
+    ix->tune_adc_mode = adc_mode;
     return DKM_OK;
 }
 

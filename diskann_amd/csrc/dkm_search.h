@@ -11,6 +11,8 @@ struct SearchParams {
     int metric;            // dkm_metric
     int pq;                // traverse with PQ ADC distances
     int rerank;            // Hybrid: FP32 rerank of the final queue
+    int adc_mode;          // PQ: 0 = lookup table in LDS, 1 = direct (no table)
+    int pq_len;            // PQ: uniform chunk width with M % 16 == 0, else 0
     int dim;
     int64_t vstride;       // floats per vector row
     const float *vectors;  // rows x vstride

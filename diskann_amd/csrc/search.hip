@@ -31,7 +31,8 @@ struct Lds {
     int32_t *offs;   // M+1
     uint32_t *starts;
     uint32_t *beam;
-    int32_t *misc;   // [0] spill region, [1] lds count
+    uint32_t *rows;  // beam_width x gstride adjacency rows staged per iteration
+    int32_t *misc;
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
@@ -39,13 +40,14 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(
 __host__ __device__ inline size_t lds_bytes(const SearchParams &p) {
     size_t b = 0;
     b += align16(sizeof(float) * p.dim);
-    if (p.pq) b += align16(sizeof(float) * p.M * 256);
+    if (p.pq && p.adc_mode == 0) b += align16(sizeof(float) * p.M * 256);
     b += 4 * align16(sizeof(float) * p.qcap);            // qa_d qa_i qb_d qb_i
     b += 4 * align16(sizeof(float) * p.nlcap);           // nl_d nl_i sn_d sn_i
     b += align16(sizeof(uint32_t) * p.hslots);
     if (p.pq) b += align16(sizeof(int32_t) * (p.M + 1));
     b += align16(sizeof(uint32_t) * p.nstart);
     b += align16(sizeof(uint32_t) * p.beam_width);
+    b += align16(sizeof(uint32_t) * p.beam_width * p.gstride);
     b += 16;
     return b;
 }
@@ -59,7 +61,7 @@ __device__ inline Lds carve(const SearchParams &p, char *base) {
         return r;
     };
     s.q = (float *)take(sizeof(float) * p.dim);
-    s.lut = p.pq ? (float *)take(sizeof(float) * p.M * 256) : nullptr;
+    s.lut = (p.pq && p.adc_mode == 0) ? (float *)take(sizeof(float) * p.M * 256) : nullptr;
     s.qa_d = (float *)take(sizeof(float) * p.qcap);
     s.qa_i = (uint32_t *)take(sizeof(uint32_t) * p.qcap);
     s.qb_d = (float *)take(sizeof(float) * p.qcap);
@@ -72,6 +74,7 @@ __device__ inline Lds carve(const SearchParams &p, char *base) {
     s.offs = p.pq ? (int32_t *)take(sizeof(int32_t) * (p.M + 1)) : nullptr;
     s.starts = (uint32_t *)take(sizeof(uint32_t) * p.nstart);
     s.beam = (uint32_t *)take(sizeof(uint32_t) * p.beam_width);
+    s.rows = (uint32_t *)take(sizeof(uint32_t) * p.beam_width * p.gstride);
     s.misc = (int32_t *)take(16);
     return s;
 }
@@ -81,14 +84,14 @@ __device__ inline Lds carve(const SearchParams &p, char *base) {
 __device__ inline bool visit_insert(const SearchParams &p, Lds &s, uint32_t id, bool active,
                                     bool spill, uint64_t *gtab, uint32_t *err) {
     if (!active) return false;
-    const uint32_t H = p.hslots;
-    uint32_t h = hash_id(id) & (H - 1);
+    const uint32_t H = p.hslots;  // any size; slot = high half of hash * H
+    uint32_t h = (uint32_t)(((uint64_t)hash_id(id) * H) >> 32);
     if (!spill) {
         for (uint32_t probe = 0; probe < H; ++probe) {
             uint32_t old = atomicCAS(&s.hash[h], kEmpty, id);
             if (old == kEmpty) return true;
             if (old == id) return false;
-            h = (h + 1) & (H - 1);
+            h = h + 1 == H ? 0 : h + 1;
         }
         atomicOr(err, ERR_VISITED_OVERFLOW);
         return false;
@@ -98,7 +101,7 @@ __device__ inline bool visit_insert(const SearchParams &p, Lds &s, uint32_t id, 
         uint32_t v = __hip_atomic_load(&s.hash[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (v == id) return false;
         if (v == kEmpty) break;
-        h = (h + 1) & (H - 1);
+        h = h + 1 == H ? 0 : h + 1;
     }
     if (gtab == nullptr) {
         atomicOr(err, ERR_VISITED_OVERFLOW);
@@ -139,6 +142,7 @@ __device__ inline int rank_new(Lds &s, int nn) {
             float di = s.nl_d[i];
             if (!__builtin_isnan(di)) {
                 int r = 0;
+#pragma unroll 8
                 for (int j = 0; j < nn; ++j) {
                     float dj = s.nl_d[j];
                     r += (dj < di) || (dj == di && j > i);
@@ -190,46 +194,148 @@ __device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *
     return ns < cap ? ns : cap;
 }
 
-template <int METRIC, bool PQ>
-__device__ inline void compute_distances(const SearchParams &p, Lds &s, int n) {
-    const int lane = lane_id();
-    if (!PQ) {
-        const int half = lane >> 5, hp = lane & 31;
-        for (int c0 = 0; c0 < n; c0 += 2) {
-            int c = c0 + half;
-            bool ok = c < n;
-            uint32_t id = ok ? s.nl_i[c] : p.starts[0];
-            const float *x = p.vectors + (size_t)id * p.vstride;
-            float d = half_distance<METRIC>(s.q, x, p.dim, hp);
-            if (ok && hp == 0) s.nl_d[c] = d;
-        }
-    } else {
-        const int M = p.M;
-        for (int c = lane; c < n; c += kWave) {
-            uint32_t id = s.nl_i[c];
-            const uint8_t *row = p.codes + (size_t)id * p.code_stride;
-            float acc = 0.0f;
-            if ((M & 15) == 0) {
-                for (int m0 = 0; m0 < M; m0 += 16) {
-                    uint4 v = *(const uint4 *)(row + m0);
-                    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+// Full-precision distances for n candidates (ids in LDS) into out (LDS): one half-wave per
+// candidate; for DIM a multiple of 32 up to 128, eight candidates per half-wave have their
+// rows in flight at once.
+template <int METRIC, int DIM>
+__device__ inline void fp_distances(const SearchParams &p, const float *q, const uint32_t *ids,
+                                    float *out, int n, uint32_t fallback) {
+    const int lane = lane_id(), half = lane >> 5, hp = lane & 31;
+    if constexpr (METRIC != METRIC_COSINE && DIM > 0 && DIM % 32 == 0 && DIM <= 128) {
+        constexpr int NI = DIM / 32;
+        constexpr int CPB = 8;
+        constexpr bool IP = METRIC == METRIC_IP;
+        float qv[NI];
 #pragma unroll
-                    for (int b = 0; b < 16; ++b) {
-                        uint32_t code = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
-                        acc += s.lut[(m0 + b) * 256 + code];
+        for (int i = 0; i < NI; ++i) qv[i] = q[hp + 32 * i];
+        for (int c0 = 0; c0 < n; c0 += 2 * CPB) {
+            float xv[CPB][NI];
+#pragma unroll
+            for (int j = 0; j < CPB; ++j) {
+                const int c = c0 + 2 * j + half;
+                const uint32_t id = c < n ? ids[c] : fallback;
+                const float *x = p.vectors + (size_t)id * p.vstride;
+#pragma unroll
+                for (int i = 0; i < NI; ++i) xv[j][i] = x[hp + 32 * i];
+            }
+#pragma unroll
+            for (int j = 0; j < CPB; ++j) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int i = 0; i < NI; ++i) {
+                    if (IP) {
+                        acc = fma_(qv[i], xv[j][i], acc);
+                    } else {
+                        float c = qv[i] - xv[j][i];
+                        acc = fma_(c, c, acc);
                     }
                 }
-            } else {
-                for (int m = 0; m < M; ++m) acc += s.lut[m * 256 + row[m]];
+                float d = finish32<IP>(acc, q, q, DIM, hp);  // DIM % 8 == 0: no remainder
+                if (IP) d = -d;
+                const int c = c0 + 2 * j + half;
+                if (c < n && hp == 0) out[c] = d;
             }
-            s.nl_d[c] = acc;
         }
+    } else {
+        for (int c0 = 0; c0 < n; c0 += 2) {
+            const int c = c0 + half;
+            const uint32_t id = c < n ? ids[c] : fallback;
+            const float *x = p.vectors + (size_t)id * p.vstride;
+            float d = half_distance<METRIC>(q, x, p.dim, hp);
+            if (c < n && hp == 0) out[c] = d;
+        }
+    }
+}
+
+// ADC from the LDS lookup table: acc = sum_m LUT[m][code_m], sequential in m.
+__device__ inline void adc_lut(const SearchParams &p, Lds &s, const uint32_t *ids, float *out,
+                               int n) {
+    const int lane = lane_id();
+    const int M = p.M;
+    for (int c = lane; c < n; c += kWave) {
+        const uint8_t *row = p.codes + (size_t)ids[c] * p.code_stride;
+        float acc = 0.0f;
+        if ((M & 15) == 0) {
+            for (int m0 = 0; m0 < M; m0 += 16) {
+                uint4 v = *(const uint4 *)(row + m0);
+                uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int b = 0; b < 16; ++b) {
+                    uint32_t code = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+                    acc += s.lut[(m0 + b) * 256 + code];
+                }
+            }
+        } else {
+            for (int m = 0; m < M; ++m) acc += s.lut[m * 256 + row[m]];
+        }
+        out[c] = acc;
+    }
+}
+
+// Direct ADC without a table: every chunk term is recomputed from the pivots (held in
+// L2) with exactly the arithmetic that fills the table, then summed in chunk order, so
+// the result is bit-identical to the table path while the query keeps no 32-48 KiB
+// table in LDS.
+template <bool IP, int LEN>
+__device__ inline void adc_direct(const SearchParams &p, Lds &s, const uint32_t *ids, float *out,
+                                  int n) {
+    const int lane = lane_id();
+    const int M = p.M;
+    for (int c = lane; c < n; c += kWave) {
+        const uint8_t *row = p.codes + (size_t)ids[c] * p.code_stride;
+        float acc = 0.0f;
+        if constexpr (LEN > 0) {
+            for (int m0 = 0; m0 < M; m0 += 16) {
+                uint4 v = *(const uint4 *)(row + m0);
+                uint32_t w[4] = {v.x, v.y, v.z, v.w};
+                float e[16];
+#pragma unroll
+                for (int b = 0; b < 16; ++b) {
+                    const uint32_t code = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+                    const float *pv = p.pivots + (size_t)code * p.dim + (m0 + b) * LEN;
+                    float pc[LEN];
+                    if constexpr (LEN == 4) {
+                        float4 t4 = *(const float4 *)pv;
+                        pc[0] = t4.x; pc[1] = t4.y; pc[2] = t4.z; pc[3] = t4.w;
+                    } else if constexpr (LEN == 2) {
+                        float2 t2 = *(const float2 *)pv;
+                        pc[0] = t2.x; pc[1] = t2.y;
+                    } else {
+#pragma unroll
+                        for (int l = 0; l < LEN; ++l) pc[l] = pv[l];
+                    }
+                    e[b] = scalar_simd_4x1<IP>(s.q + (m0 + b) * LEN, pc, LEN);
+                }
+#pragma unroll
+                for (int b = 0; b < 16; ++b) acc += IP ? -e[b] : e[b];
+            }
+        } else {
+            for (int m = 0; m < M; ++m) {
+                const int o0 = s.offs[m], o1 = s.offs[m + 1];
+                const float *pv = p.pivots + (size_t)row[m] * p.dim + o0;
+                float v = scalar_simd_4x1<IP>(s.q + o0, pv, o1 - o0);
+                acc += IP ? -v : v;
+            }
+        }
+        out[c] = acc;
+    }
+}
+
+template <int METRIC, int MODE, int DIM, int PQLEN>
+__device__ inline void compute_distances(const SearchParams &p, Lds &s, int n) {
+    if constexpr (MODE == 0) {
+        fp_distances<METRIC, DIM>(p, s.q, s.nl_i, s.nl_d, n, s.starts[0]);
+    } else if constexpr (MODE == 1) {
+        adc_lut(p, s, s.nl_i, s.nl_d, n);
+    } else {
+        adc_direct<METRIC == METRIC_IP, PQLEN>(p, s, s.nl_i, s.nl_d, n);
     }
     __syncthreads();
 }
 
-template <int METRIC, bool PQ>
+template <int METRIC, int MODE, int DIM, int PQLEN>
 __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
+    constexpr bool PQ = MODE != 0;
     extern __shared__ __align__(16) char smem[];
     const int qidx = blockIdx.x;
     if (qidx >= p.nq) return;
@@ -253,7 +359,7 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
         for (int i = lane; i <= p.M; i += kWave) s.offs[i] = p.offsets[i];
     }
     __syncthreads();
-    if (PQ) {
+    if (MODE == 1) {
         const bool ip = METRIC == METRIC_IP;
         for (int e = lane; e < p.M * 256; e += kWave) {
             int m = e >> 8, c = e & 255;
@@ -284,7 +390,7 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
         __syncthreads();
     }
     lds_count = p.nstart;
-    compute_distances<METRIC, PQ>(p, s, p.nstart);
+    compute_distances<METRIC, MODE, DIM, PQLEN>(p, s, p.nstart);
     {
         int nvalid = rank_new(s, p.nstart);
         __syncthreads();
@@ -322,18 +428,24 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
         if (nbeam == 0) break;
         nrec += nbeam;
 
-        // expand: neighbours in adjacency order, filtered by the visited set
-        int nn = 0;
+        // expand: stage the beam nodes' adjacency rows (ids + count slot) with one round
+        // of loads, then filter neighbours in adjacency order through the visited set
         for (int b = 0; b < nbeam; ++b) {
-            const uint32_t node = s.beam[b];
-            const uint32_t *row = p.graph + (size_t)node * p.gstride;
+            const uint32_t *row = p.graph + (size_t)s.beam[b] * p.gstride;
+            for (int j = lane; j < p.gstride; j += kWave) s.rows[b * p.gstride + j] = row[j];
+        }
+        __syncthreads();
+        int nn = 0;
+        const int spill_at = (int)(p.hslots - (p.hslots >> 3)) - kWave;
+        for (int b = 0; b < nbeam; ++b) {
+            const uint32_t *row = s.rows + b * p.gstride;
             const int cnt = (int)row[p.gstride - 1];
             degsum += (uint32_t)cnt;
             for (int j0 = 0; j0 < cnt; j0 += kWave) {
                 int j = j0 + lane;
                 bool act = j < cnt;
                 uint32_t id = act ? row[j] : 0;
-                bool spill = lds_count + kWave > (int)(p.hslots - (p.hslots >> 3));
+                bool spill = lds_count > spill_at;
                 if (spill && gtab == nullptr && p.spill_table)
                     gtab = p.spill_table + (size_t)qidx * p.spill_slots;  // this query's region
                 bool fresh = visit_insert(p, s, id, act, spill, gtab, err);
@@ -342,16 +454,15 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
                 int got = __builtin_popcountll(m);
                 nn += got;
                 if (!spill) lds_count += got;
-                __syncthreads();
             }
         }
-        compute_distances<METRIC, PQ>(p, s, nn);
+        __syncthreads();
+        compute_distances<METRIC, MODE, DIM, PQLEN>(p, s, nn);
         int nvalid = rank_new(s, nn);
         __syncthreads();
         size = merge_queue(s, size, nvalid, cap, qd, qi, od, oi);
         cmps += (uint32_t)nn;
         hops += (uint32_t)nbeam;
-        if (*((volatile uint32_t *)err) & ERR_VISITED_OVERFLOW) break;
     }
 
     // ---- post-processing -----------------------------------------------------------
@@ -379,7 +490,8 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
             count = count + got < k ? count + got : k;
         }
     } else {
-        // FilterStartPoints -> Rerank: exact FP32 distances, stable sort, first k
+        // FilterStartPoints -> Rerank: exact FP32 distances, stable sort, first k.
+        // The spare queue buffer (od, oi) holds the candidates.
         int n = 0;
         for (int base = 0; base < size; base += kWave) {
             int i = base + lane;
@@ -391,34 +503,23 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
                 for (int t = 0; t < p.nstart; ++t) ok = ok && (s.starts[t] != id);
             }
             uint64_t m = ballot(ok);
-            if (ok) s.nl_i[n + prefix_count(m)] = id;
+            if (ok) oi[n + prefix_count(m)] = id;
             n += __builtin_popcountll(m);
         }
         __syncthreads();
-        {
-            const int half = lane >> 5, hp = lane & 31;
-            for (int c0 = 0; c0 < n; c0 += 2) {
-                int c = c0 + half;
-                bool ok = c < n;
-                uint32_t id = ok ? s.nl_i[c] : s.starts[0];
-                const float *x = p.vectors + (size_t)id * p.vstride;
-                float d;
-                if (p.metric == METRIC_L2) d = half_distance<METRIC_L2>(s.q, x, p.dim, hp);
-                else if (p.metric == METRIC_IP) d = half_distance<METRIC_IP>(s.q, x, p.dim, hp);
-                else d = half_distance<METRIC_COSINE>(s.q, x, p.dim, hp);
-                if (ok && hp == 0) s.nl_d[c] = d;
-            }
-        }
+        if (p.metric == METRIC_L2) fp_distances<METRIC_L2, DIM>(p, s.q, oi, od, n, s.starts[0]);
+        else if (p.metric == METRIC_IP) fp_distances<METRIC_IP, DIM>(p, s.q, oi, od, n, s.starts[0]);
+        else fp_distances<METRIC_COSINE, 0>(p, s.q, oi, od, n, s.starts[0]);
         __syncthreads();
         for (int i = lane; i < n; i += kWave) {
-            float di = s.nl_d[i];
+            float di = od[i];
             int r = 0;
             for (int j = 0; j < n; ++j) {
-                float dj = s.nl_d[j];
+                float dj = od[j];
                 r += (dj < di) || (dj == di && j < i);
             }
             if (r < k) {
-                oids[r] = s.nl_i[i];
+                oids[r] = oi[i];
                 ods[r] = di;
             }
         }
@@ -437,10 +538,10 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     }
 }
 
-template <int METRIC, bool PQ>
+template <int METRIC, int MODE, int DIM, int PQLEN>
 static hipError_t launch_t(const SearchParams &p, hipStream_t stream) {
     size_t lds = lds_bytes(p);
-    auto kern = beam_search_kernel<METRIC, PQ>;
+    auto kern = beam_search_kernel<METRIC, MODE, DIM, PQLEN>;
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -452,15 +553,27 @@ size_t search_lds_bytes(const SearchParams &p) { return lds_bytes(p); }
 
 hipError_t launch_search(const SearchParams &p, hipStream_t stream) {
     if (p.pq) {
-        switch (p.metric) {
-        case METRIC_IP: return launch_t<METRIC_IP, true>(p, stream);
-        default: return launch_t<METRIC_L2, true>(p, stream);  // L2 and CosineNormalized tables
+        const bool ip = p.metric == METRIC_IP;  // L2 and CosineNormalized use the L2 table
+        if (p.adc_mode == 0) {
+            if (ip) return launch_t<METRIC_IP, 1, 0, 0>(p, stream);
+            if (p.dim == 128) return launch_t<METRIC_L2, 1, 128, 0>(p, stream);
+            if (p.dim == 96) return launch_t<METRIC_L2, 1, 96, 0>(p, stream);
+            return launch_t<METRIC_L2, 1, 0, 0>(p, stream);
         }
+        if (ip) return launch_t<METRIC_IP, 2, 0, 0>(p, stream);
+        if (p.dim == 128 && p.pq_len == 4) return launch_t<METRIC_L2, 2, 128, 4>(p, stream);
+        if (p.dim == 96 && p.pq_len == 2) return launch_t<METRIC_L2, 2, 96, 2>(p, stream);
+        return launch_t<METRIC_L2, 2, 0, 0>(p, stream);
     }
     switch (p.metric) {
-    case METRIC_L2: return launch_t<METRIC_L2, false>(p, stream);
-    case METRIC_IP: return launch_t<METRIC_IP, false>(p, stream);
-    default: return launch_t<METRIC_COSINE, false>(p, stream);
+    case METRIC_L2:
+        if (p.dim == 128) return launch_t<METRIC_L2, 0, 128, 0>(p, stream);
+        if (p.dim == 96) return launch_t<METRIC_L2, 0, 96, 0>(p, stream);
+        return launch_t<METRIC_L2, 0, 0, 0>(p, stream);
+    case METRIC_IP:
+        if (p.dim == 128) return launch_t<METRIC_IP, 0, 128, 0>(p, stream);
+        return launch_t<METRIC_IP, 0, 0, 0>(p, stream);
+    default: return launch_t<METRIC_COSINE, 0, 0, 0>(p, stream);
     }
 }
 

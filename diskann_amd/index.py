@@ -167,8 +167,28 @@ class DiskANNIndex:
               self.lib)
         return ids, ds
 
-    def set_tuning(self, visited_lds_slots=0):
-        check(self.lib.dkm_set_tuning(self._h, int(visited_lds_slots)), self.lib)
+    def autotune(self, queries, k, L, beam_width=1, strategy=Strategy.HYBRID):
+        """Time the search of `queries` under the result-preserving knobs and keep the
+        fastest; returns (visited_lds_slots, adc_mode)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        d = C.c_void_p()
+        check(self.lib.dkm_device_alloc(q.nbytes, C.byref(d)), self.lib)
+        try:
+            check(self.lib.dkm_memcpy_h2d(d, _p(q), q.nbytes), self.lib)
+            hs = C.c_int64()
+            am = C.c_int()
+            check(self.lib.dkm_autotune(self._h, int(strategy), d, q.shape[0], int(k), int(L),
+                                        int(beam_width), C.byref(hs), C.byref(am)), self.lib)
+        finally:
+            self.lib.dkm_device_free(d)
+        self.tuning = (int(hs.value), int(am.value))
+        return self.tuning
+
+    def set_tuning(self, visited_lds_slots=0, adc_mode=1):
+        """Performance knobs that never change results: LDS visited-set slots per query
+        (0 = automatic) and PQ distance mode (0 = LDS lookup table, 1 = direct)."""
+        self.tuning = (int(visited_lds_slots), int(adc_mode))
+        check(self.lib.dkm_set_tuning(self._h, int(visited_lds_slots), int(adc_mode)), self.lib)
 
     def search_batch(self, queries, k, L, beam_width=1, strategy=Strategy.FULL_PRECISION):
         """Batched search. `k` and `L` are ints or per-query arrays (validated per query

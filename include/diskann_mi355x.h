@@ -178,9 +178,18 @@ dkm_status dkm_search_batch_device_stats(dkm_index *ix, int strategy, const floa
                                          uint32_t *d_out_cmps, uint32_t *d_out_hops,
                                          uint32_t *d_out_degsum);
 
-/* Tuning knobs (autotuner surface): visited-set slots in LDS per query (power of two,
- * 0 = automatic). */
-dkm_status dkm_set_tuning(dkm_index *ix, int64_t visited_lds_slots);
+/* Tuning knobs (autotuner surface; none changes results): visited-set slots in LDS per
+ * query (256..32768, 0 = automatic) and the PQ distance mode (0 = lookup table in LDS,
+ * 1 = direct recomputation from the pivots, the default). */
+dkm_status dkm_set_tuning(dkm_index *ix, int64_t visited_lds_slots, int adc_mode);
+
+/* Autotuner: times the search of `nq` device-resident queries (k, L, beam_width fixed)
+ * over the PQ distance modes and a few visited-set sizes derived from the measured
+ * per-query comparison counts, keeps the fastest setting on the handle and reports it.
+ * Results are identical under every setting. */
+dkm_status dkm_autotune(dkm_index *ix, int strategy, const float *d_queries, int64_t nq,
+                        uint32_t k, uint32_t L, uint32_t beam_width,
+                        int64_t *out_visited_lds_slots, int *out_adc_mode);
 
 #ifdef __cplusplus
 }

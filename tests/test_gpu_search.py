@@ -112,15 +112,18 @@ def test_pq_encode_lut_match_oracle(metric, dim, M):
         assert lut_g[i].tobytes() == lut_o.tobytes()
 
 
-@pytest.mark.parametrize("metric,dim,M", [(po.L2, 128, 32), (po.L2, 96, 48),
+@pytest.mark.parametrize("metric,dim,M", [(po.L2, 128, 32), (po.L2, 96, 48), (po.L2, 40, 7),
                                           (po.INNER_PRODUCT, 64, 16)])
 @pytest.mark.parametrize("strategy", [dk.Hybrid, dk.Quantized])
 @pytest.mark.parametrize("L,bw", [(32, 1), (64, 2)])
-def test_pq_search_matches_oracle(metric, dim, M, strategy, L, bw):
-    log("pq search metric=%d dim=%d M=%d strat=%d L=%d" % (metric, dim, M, strategy, L))
+@pytest.mark.parametrize("adc_mode", [0, 1])
+def test_pq_search_matches_oracle(metric, dim, M, strategy, L, bw, adc_mode):
+    log("pq search metric=%d dim=%d M=%d strat=%d L=%d adc=%d" % (metric, dim, M, strategy, L,
+                                                                  adc_mode))
     vec, adj, deg, qs = dataset(metric, dim=dim)
     piv, off = pq_setup(vec, M)
     ix = make_index(metric, vec, adj, deg)
+    ix.set_tuning(0, adc_mode)
     ix.set_pq(piv, off)
     ix.encode()
     codes = ix.get_codes()
@@ -166,6 +169,7 @@ def test_visited_spill_path_matches_oracle():
     vec, adj, deg, qs = dataset(po.L2)
     ix = make_index(po.L2, vec, adj, deg)
     ix.set_tuning(visited_lds_slots=256)
+    assert ix.tuning == (256, 1)
     oix = po.OracleIndex(po.L2, vec, adj, deg, [vec.shape[0] - 1])
     r = ix.search_batch(qs, 10, 100, 1, dk.FullPrecision)
     for i in range(qs.shape[0]):

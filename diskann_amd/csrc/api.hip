@@ -506,10 +506,13 @@ dkm_status dkm_get_codes(dkm_index *ix, uint8_t *codes, int64_t row0, int64_t n)
     return DKM_OK;
 }
 
-static uint32_t next_pow2(uint64_t x) {
-    uint32_t p = 1;
-    while (p < x) p <<= 1;
-    return p;
+
+// Default LDS visited-set size for a search list of L: roughly the comparisons such a
+// search makes on degree-64 graphs, rounded to 64 slots (the autotuner refines it).
+static uint32_t default_visited_slots(uint64_t L) {
+    uint64_t v = 48 * L + 2048;
+    v = (v + 63) / 64 * 64;
+    return (uint32_t)std::min<uint64_t>(16384, v);
 }
 
 // Per-query visited-set spill regions (HBM) for `nq` queries; entries are tagged with a
@@ -590,8 +593,7 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     p.status = d_status;
     p.qcap = (int)(max_L + ix->num_start);
     p.nlcap = (int)(bw * ix->R_max + 64);
-    p.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots
-                                   : std::min<uint32_t>(16384, std::max<uint32_t>(2048, next_pow2(64ull * max_L)));
+    p.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots : default_visited_slots(max_L);
     p.spill_table = ix->d_spill;
     p.spill_slots = ix->spill_slots;
     p.epoch = ix->epoch;
@@ -870,8 +872,7 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
         sp.status = ostat.p;
         sp.qcap = (int)(L_build + ix->num_start);
         sp.nlcap = (int)(ix->R_max + 64);
-        sp.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots
-                                        : std::min<uint32_t>(16384, std::max<uint32_t>(2048, next_pow2(64ull * L_build)));
+        sp.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots : default_visited_slots(L_build);
         sp.spill_table = ix->d_spill;
         sp.spill_slots = ix->spill_slots;
         sp.epoch = ix->epoch;

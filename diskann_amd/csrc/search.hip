@@ -126,10 +126,30 @@ __device__ inline bool visit_insert(const SearchParams &p, Lds &s, uint32_t id, 
     return false;
 }
 
-// Lane-parallel rank of new items among themselves (key: d asc, index desc), NaNs
-// excluded; writes the sorted list. Returns the number of valid (non-NaN) items.
+__device__ inline float readlane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Rank of each new item among the new items (key: d asc, insertion index desc), NaNs
+// excluded; writes the sorted list sn_d / sn_i. Returns the number of valid items.
 __device__ inline int rank_new(Lds &s, int nn) {
     const int lane = lane_id();
+    if (nn <= kWave) {
+        // one item per lane; the others' distances are broadcast with v_readlane
+        const bool has = lane < nn;
+        const float di = has ? s.nl_d[lane] : 0.0f;
+        const bool ok = has && !__builtin_isnan(di);
+        int r = 0;
+        for (int j = 0; j < nn; ++j) {
+            const float dj = readlane_f(di, j);
+            r += (dj < di) || (dj == di && j > lane);
+        }
+        if (ok) {
+            s.sn_d[r] = di;
+            s.sn_i[r] = s.nl_i[lane];
+        }
+        return __builtin_popcountll(ballot(ok));
+    }
     int nvalid = 0;
     for (int base = 0; base < nn; base += kWave) {
         int i = base + lane;
@@ -155,11 +175,22 @@ __device__ inline int rank_new(Lds &s, int nn) {
     return nvalid;
 }
 
-// Merge sorted new items into the queue (qa -> qb). Returns the new size.
+__device__ inline int wave_min_i(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return v;
+}
+
+// Merge the sorted new items into the queue (qd/qi -> od/oi, then swap). New items go
+// before old items of equal distance; positions >= cap are dropped. cursor_old is the
+// first unvisited index of the old queue (or size); *cursor receives the first unvisited
+// index of the merged queue (or its size). Returns the new size.
 __device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *&qd,
-                                  uint32_t *&qi, float *&od, uint32_t *&oi) {
+                                  uint32_t *&qi, float *&od, uint32_t *&oi, int cursor_old,
+                                  int *cursor) {
     const int lane = lane_id();
-    // new items: position = rank + #old with d_old < d  (lower bound in old queue)
+    int best = 0x7FFFFFFF;
+    // new items: position = rank + #old with d_old < d  (lower bound in the old queue)
     for (int r = lane; r < nvalid; r += kWave) {
         float d = s.sn_d[r];
         int lo = 0, hi = size;
@@ -171,31 +202,57 @@ __device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *
         if (pos < cap) {
             od[pos] = d;
             oi[pos] = s.sn_i[r];
+            if (r == 0) best = pos;
         }
     }
-    // old items: position = j + #new with d_new <= d_old
-    for (int j = lane; j < size; j += kWave) {
-        float d = qd[j];
-        int lo = 0, hi = nvalid;
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (s.sn_d[mid] <= d) lo = mid + 1; else hi = mid;
+    // old items j = lane + 64 e: position = j + #new with d_new <= d_old, the new distances
+    // broadcast lane by lane
+    constexpr int E = 8;
+    for (int j0 = 0; j0 < size; j0 += E * kWave) {
+        float dj[E];
+        uint32_t ij[E];
+        int cnt[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int j = j0 + lane + kWave * e;
+            dj[e] = j < size ? qd[j] : 0.0f;
+            ij[e] = j < size ? qi[j] : 0u;
+            cnt[e] = 0;
         }
-        int pos = j + lo;
-        if (pos < cap) {
-            od[pos] = d;
-            oi[pos] = qi[j];
+        for (int t0 = 0; t0 < nvalid; t0 += kWave) {
+            const float vn = t0 + lane < nvalid ? s.sn_d[t0 + lane] : 0.0f;
+            const int lim = nvalid - t0 < kWave ? nvalid - t0 : kWave;
+            for (int t = 0; t < lim; ++t) {
+                const float v = readlane_f(vn, t);
+#pragma unroll
+                for (int e = 0; e < E; ++e) cnt[e] += v <= dj[e];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int j = j0 + lane + kWave * e;
+            if (j < size) {
+                const int pos = j + cnt[e];
+                if (pos < cap) {
+                    od[pos] = dj[e];
+                    oi[pos] = ij[e];
+                    if (j == cursor_old) best = min(best, pos);
+                }
+            }
         }
     }
+    best = wave_min_i(best);
     __syncthreads();
     float *td = qd; qd = od; od = td;
     uint32_t *ti = qi; qi = oi; oi = ti;
     int ns = size + nvalid;
-    return ns < cap ? ns : cap;
+    ns = ns < cap ? ns : cap;
+    *cursor = best < ns ? best : ns;
+    return ns;
 }
 
 // Full-precision distances for n candidates (ids in LDS) into out (LDS): one half-wave per
-// candidate; for DIM a multiple of 32 up to 128, eight candidates per half-wave have their
+// candidate; for DIM a multiple of 32 up to 128, sixteen candidates per half-wave have their
 // rows in flight at once.
 template <int METRIC, int DIM>
 __device__ inline void fp_distances(const SearchParams &p, const float *q, const uint32_t *ids,
@@ -203,7 +260,7 @@ __device__ inline void fp_distances(const SearchParams &p, const float *q, const
     const int lane = lane_id(), half = lane >> 5, hp = lane & 31;
     if constexpr (METRIC != METRIC_COSINE && DIM > 0 && DIM % 32 == 0 && DIM <= 128) {
         constexpr int NI = DIM / 32;
-        constexpr int CPB = 8;
+        constexpr int CPB = 16;
         constexpr bool IP = METRIC == METRIC_IP;
         float qv[NI];
 #pragma unroll
@@ -273,49 +330,104 @@ __device__ inline void adc_lut(const SearchParams &p, Lds &s, const uint32_t *id
 }
 
 // Direct ADC without a table: every chunk term is recomputed from the pivots (held in
-// L2) with exactly the arithmetic that fills the table, then summed in chunk order, so
-// the result is bit-identical to the table path while the query keeps no 32-48 KiB
-// table in LDS.
+// L2) with exactly the arithmetic that fills the table, so the result is bit-identical to
+// the table path while the query keeps no 32-48 KiB table in LDS. A group of G lanes
+// shares one candidate: each lane computes M/G consecutive chunk terms, then the running
+// sum is handed from lane to lane so the terms are added in chunk order.
+template <typename T>
+__device__ inline T load_as(const uint8_t *p) { return *(const T *)p; }
+
+template <bool IP, int LEN, int M, int G>
+__device__ inline void adc_direct_group(const SearchParams &p, Lds &s, const uint32_t *ids,
+                                        float *out, int n) {
+    constexpr int MPER = M / G;
+    const int lane = lane_id();
+    const int g = lane % G;
+    for (int c0 = 0; c0 < n; c0 += kWave / G) {
+        const int c = c0 + lane / G;
+        const bool ok = c < n;
+        const uint32_t id = ids[ok ? c : 0];
+        const uint8_t *row = p.codes + (size_t)id * p.code_stride + g * MPER;
+        uint8_t code[MPER];
+        if constexpr (MPER % 4 == 0) {
+#pragma unroll
+            for (int q = 0; q < MPER; q += 4) {
+                uint32_t w = load_as<uint32_t>(row + q);
+#pragma unroll
+                for (int b = 0; b < 4; ++b) code[q + b] = (uint8_t)(w >> (8 * b));
+            }
+        } else if constexpr (MPER % 2 == 0) {
+#pragma unroll
+            for (int q = 0; q < MPER; q += 2) {
+                uint16_t w = load_as<uint16_t>(row + q);
+                code[q] = (uint8_t)w;
+                code[q + 1] = (uint8_t)(w >> 8);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < MPER; ++q) code[q] = row[q];
+        }
+        float ent[MPER];
+#pragma unroll
+        for (int q = 0; q < MPER; ++q) {
+            const int m = g * MPER + q;
+            const float *pv = p.pivots + (size_t)code[q] * p.dim + m * LEN;
+            float pc[LEN];
+            if constexpr (LEN == 4) {
+                float4 t4 = *(const float4 *)pv;
+                pc[0] = t4.x; pc[1] = t4.y; pc[2] = t4.z; pc[3] = t4.w;
+            } else if constexpr (LEN == 2) {
+                float2 t2 = *(const float2 *)pv;
+                pc[0] = t2.x; pc[1] = t2.y;
+            } else {
+#pragma unroll
+                for (int l = 0; l < LEN; ++l) pc[l] = pv[l];
+            }
+            const float v = scalar_simd_4x1<IP>(s.q + m * LEN, pc, LEN);
+            ent[q] = IP ? -v : v;
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int step = 0; step < G; ++step) {
+            const float prev = G > 1 ? __shfl_up(acc, 1, G) : acc;
+            if (g == step) {
+                acc = step == 0 ? 0.0f : prev;
+#pragma unroll
+                for (int q = 0; q < MPER; ++q) acc += ent[q];
+            }
+        }
+        if (ok && g == G - 1) out[c] = acc;
+    }
+}
+
 template <bool IP, int LEN>
 __device__ inline void adc_direct(const SearchParams &p, Lds &s, const uint32_t *ids, float *out,
                                   int n) {
+    if constexpr (LEN == 4) {
+        if (p.M == 32) {
+            if (n <= 8) return adc_direct_group<IP, 4, 32, 8>(p, s, ids, out, n);
+            if (n <= 16) return adc_direct_group<IP, 4, 32, 4>(p, s, ids, out, n);
+            if (n <= 32) return adc_direct_group<IP, 4, 32, 2>(p, s, ids, out, n);
+            return adc_direct_group<IP, 4, 32, 1>(p, s, ids, out, n);
+        }
+    } else if constexpr (LEN == 2) {
+        if (p.M == 48) {
+            if (n <= 8) return adc_direct_group<IP, 2, 48, 8>(p, s, ids, out, n);
+            if (n <= 16) return adc_direct_group<IP, 2, 48, 4>(p, s, ids, out, n);
+            if (n <= 32) return adc_direct_group<IP, 2, 48, 2>(p, s, ids, out, n);
+            return adc_direct_group<IP, 2, 48, 1>(p, s, ids, out, n);
+        }
+    }
     const int lane = lane_id();
     const int M = p.M;
     for (int c = lane; c < n; c += kWave) {
         const uint8_t *row = p.codes + (size_t)ids[c] * p.code_stride;
         float acc = 0.0f;
-        if constexpr (LEN > 0) {
-            for (int m0 = 0; m0 < M; m0 += 16) {
-                uint4 v = *(const uint4 *)(row + m0);
-                uint32_t w[4] = {v.x, v.y, v.z, v.w};
-                float e[16];
-#pragma unroll
-                for (int b = 0; b < 16; ++b) {
-                    const uint32_t code = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
-                    const float *pv = p.pivots + (size_t)code * p.dim + (m0 + b) * LEN;
-                    float pc[LEN];
-                    if constexpr (LEN == 4) {
-                        float4 t4 = *(const float4 *)pv;
-                        pc[0] = t4.x; pc[1] = t4.y; pc[2] = t4.z; pc[3] = t4.w;
-                    } else if constexpr (LEN == 2) {
-                        float2 t2 = *(const float2 *)pv;
-                        pc[0] = t2.x; pc[1] = t2.y;
-                    } else {
-#pragma unroll
-                        for (int l = 0; l < LEN; ++l) pc[l] = pv[l];
-                    }
-                    e[b] = scalar_simd_4x1<IP>(s.q + (m0 + b) * LEN, pc, LEN);
-                }
-#pragma unroll
-                for (int b = 0; b < 16; ++b) acc += IP ? -e[b] : e[b];
-            }
-        } else {
-            for (int m = 0; m < M; ++m) {
-                const int o0 = s.offs[m], o1 = s.offs[m + 1];
-                const float *pv = p.pivots + (size_t)row[m] * p.dim + o0;
-                float v = scalar_simd_4x1<IP>(s.q + o0, pv, o1 - o0);
-                acc += IP ? -v : v;
-            }
+        for (int m = 0; m < M; ++m) {
+            const int o0 = s.offs[m], o1 = s.offs[m + 1];
+            const float *pv = p.pivots + (size_t)row[m] * p.dim + o0;
+            float v = scalar_simd_4x1<IP>(s.q + o0, pv, o1 - o0);
+            acc += IP ? -v : v;
         }
         out[c] = acc;
     }
@@ -391,10 +503,11 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     }
     lds_count = p.nstart;
     compute_distances<METRIC, MODE, DIM, PQLEN>(p, s, p.nstart);
+    int cursor = 0;
     {
         int nvalid = rank_new(s, p.nstart);
         __syncthreads();
-        size = merge_queue(s, size, nvalid, cap, qd, qi, od, oi);
+        size = merge_queue(s, size, nvalid, cap, qd, qi, od, oi, 0, &cursor);
     }
     cmps = p.nstart;
 
@@ -404,25 +517,33 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
             if (lane == 0) atomicOr(err, ERR_ITER_LIMIT);
             break;
         }
-        // beam: first beam_width unvisited entries
+        // beam: the first beam_width unvisited entries from the cursor (the first unvisited
+        // index); the cursor then moves to the next unvisited entry after them
         int nbeam = 0;
-        for (int base = 0; base < size && nbeam < (int)p.beam_width; base += kWave) {
-            int i = base + lane;
-            bool unv = i < size && !(qi[i < size ? i : 0] & kVisitedBit);
-            uint64_t m = ballot(unv);
-            int before = prefix_count(m);
-            int take = (int)p.beam_width - nbeam;
+        int next = size;
+        for (int base = cursor; base < size; base += kWave) {
+            const int i = base + lane;
+            const bool unv = i < size && !(qi[i < size ? i : 0] & kVisitedBit);
+            const uint64_t m = ballot(unv);
+            const int before = prefix_count(m);
+            const int take = (int)p.beam_width - nbeam;
             if (unv && before < take) {
-                s.beam[nbeam + before] = qi[i];
-                uint32_t rid = qi[i];
+                const uint32_t rid = qi[i];
+                s.beam[nbeam + before] = rid;
                 qi[i] = rid | kVisitedBit;
                 if (p.rec_ids && nrec + before < p.rec_cap) {
                     p.rec_ids[(size_t)qidx * p.rec_cap + nrec + before] = rid;
                     p.rec_d[(size_t)qidx * p.rec_cap + nrec + before] = qd[i];
                 }
             }
-            int got = __builtin_popcountll(m);
-            nbeam += got < take ? got : take;
+            const int got = __builtin_popcountll(m);
+            if (got > take) {
+                const uint64_t mn = ballot(unv && before == take);
+                next = base + __builtin_ctzll(mn);
+                nbeam += take;
+                break;
+            }
+            nbeam += got;
         }
         __syncthreads();
         if (nbeam == 0) break;
@@ -460,7 +581,7 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
         compute_distances<METRIC, MODE, DIM, PQLEN>(p, s, nn);
         int nvalid = rank_new(s, nn);
         __syncthreads();
-        size = merge_queue(s, size, nvalid, cap, qd, qi, od, oi);
+        size = merge_queue(s, size, nvalid, cap, qd, qi, od, oi, next, &cursor);
         cmps += (uint32_t)nn;
         hops += (uint32_t)nbeam;
     }
@@ -511,17 +632,32 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
         else if (p.metric == METRIC_IP) fp_distances<METRIC_IP, DIM>(p, s.q, oi, od, n, s.starts[0]);
         else fp_distances<METRIC_COSINE, 0>(p, s.q, oi, od, n, s.starts[0]);
         __syncthreads();
-        for (int i = lane; i < n; i += kWave) {
-            float di = od[i];
-            int r = 0;
-            for (int j = 0; j < n; ++j) {
-                float dj = od[j];
-                r += (dj < di) || (dj == di && j < i);
+        // stable selection of the k smallest (distance, position) keys: k rounds of a
+        // wave-wide arg-min; a chosen entry is marked through bit 31 of its id
+        const int kk = n < k ? n : k;
+        for (int r = 0; r < kk; ++r) {
+            unsigned long long bk = ~0ull;
+            for (int i = lane; i < n; i += kWave) {
+                if (!(oi[i] & kVisitedBit)) {
+                    const float d = od[i];
+                    uint32_t u = __float_as_uint(d == 0.0f ? 0.0f : d);
+                    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+                    const unsigned long long key = ((unsigned long long)u << 32) | (uint32_t)i;
+                    bk = key < bk ? key : bk;
+                }
             }
-            if (r < k) {
-                oids[r] = oi[i];
-                ods[r] = di;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const unsigned long long o = __shfl_xor(bk, off);
+                bk = o < bk ? o : bk;
             }
+            const int win = (int)(bk & 0xFFFFFFFFull);
+            if (lane == 0) {
+                oids[r] = oi[win];
+                ods[r] = od[win];
+                oi[win] |= kVisitedBit;
+            }
+            __syncthreads();
         }
         count = n < k ? n : k;
     }

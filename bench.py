@@ -158,8 +158,8 @@ def main():
         L_star = max(sweep, key=lambda l: sweep[l])
     set_L(L_star)
 
-    # beam width and result-preserving kernel knobs: autotuned on a 2k-query sample
-    tune_q = qs[:2000]
+    # beam width and result-preserving kernel knobs: autotuned on the query batch
+    tune_q = qs
     best = None
     for bw in (1, 2, 4, 8):
         hs, am = ix.autotune(tune_q, K, L_star, bw, dk.Hybrid)

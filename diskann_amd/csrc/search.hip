@@ -79,31 +79,36 @@ __device__ inline Lds carve(const SearchParams &p, char *base) {
     return s;
 }
 
-// Visited-set insert for one id per lane (ids within one call are distinct).
-// Returns true if the id was not present (and is now recorded).
+// Visited-set insert for one id per lane (ids within one call are distinct). Returns
+// true if the id was not present (and is now recorded).
+// The LDS table is an array of 4-slot buckets probed in sequence from a hashed start;
+// one 16-byte read checks a whole bucket, an id goes to the first free slot of the first
+// bucket with room, and since slots are never freed a lookup may stop at the first
+// bucket that still has a free slot. Once the table passes its fill threshold (decided
+// per chunk, uniformly) it is only read, and new ids go to this query's region of the
+// HBM spill table (64-bit entries tagged with the launch epoch, linear probing).
 __device__ inline bool visit_insert(const SearchParams &p, Lds &s, uint32_t id, bool active,
                                     bool spill, uint64_t *gtab, uint32_t *err) {
     if (!active) return false;
-    const uint32_t H = p.hslots;  // any size; slot = high half of hash * H
-    uint32_t h = (uint32_t)(((uint64_t)hash_id(id) * H) >> 32);
-    if (!spill) {
-        for (uint32_t probe = 0; probe < H; ++probe) {
-            uint32_t old = atomicCAS(&s.hash[h], kEmpty, id);
-            if (old == kEmpty) return true;
-            if (old == id) return false;
-            h = h + 1 == H ? 0 : h + 1;
+    const uint32_t HB = p.hslots >> 2;  // buckets
+    uint32_t b = (uint32_t)(((uint64_t)hash_id(id) * HB) >> 32);
+    for (uint32_t probe = 0; probe < HB;) {
+        const uint4 v = *(const uint4 *)&s.hash[4 * b];
+        if (v.x == id || v.y == id || v.z == id || v.w == id) return false;
+        const int free_slot = v.x == kEmpty ? 0 : v.y == kEmpty ? 1 : v.z == kEmpty ? 2
+                              : v.w == kEmpty ? 3 : -1;
+        if (free_slot < 0) {
+            b = b + 1 == HB ? 0 : b + 1;
+            ++probe;
+            continue;
         }
-        atomicOr(err, ERR_VISITED_OVERFLOW);
-        return false;
+        if (spill) break;  // not present in LDS
+        const uint32_t old = atomicCAS(&s.hash[4 * b + free_slot], kEmpty, id);
+        if (old == kEmpty) return true;
+        if (old == id) return false;
+        // another lane took the slot: re-read the same bucket
     }
-    // Spill mode: the LDS table is read-only for this chunk.
-    for (uint32_t probe = 0; probe < H; ++probe) {
-        uint32_t v = __hip_atomic_load(&s.hash[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (v == id) return false;
-        if (v == kEmpty) break;
-        h = h + 1 == H ? 0 : h + 1;
-    }
-    if (gtab == nullptr) {
+    if (!spill || gtab == nullptr) {
         atomicOr(err, ERR_VISITED_OVERFLOW);
         return false;
     }
@@ -181,33 +186,11 @@ __device__ inline int wave_min_i(int v) {
     return v;
 }
 
-// Merge the sorted new items into the queue (qd/qi -> od/oi, then swap). New items go
-// before old items of equal distance; positions >= cap are dropped. cursor_old is the
-// first unvisited index of the old queue (or size); *cursor receives the first unvisited
-// index of the merged queue (or its size). Returns the new size.
-__device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *&qd,
-                                  uint32_t *&qi, float *&od, uint32_t *&oi, int cursor_old,
-                                  int *cursor) {
+template <int E>
+__device__ inline void merge_old(Lds &s, int size, int nvalid, int cap, const float *qd,
+                                 const uint32_t *qi, float *od, uint32_t *oi, int cursor_old,
+                                 int &best) {
     const int lane = lane_id();
-    int best = 0x7FFFFFFF;
-    // new items: position = rank + #old with d_old < d  (lower bound in the old queue)
-    for (int r = lane; r < nvalid; r += kWave) {
-        float d = s.sn_d[r];
-        int lo = 0, hi = size;
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (qd[mid] < d) lo = mid + 1; else hi = mid;
-        }
-        int pos = r + lo;
-        if (pos < cap) {
-            od[pos] = d;
-            oi[pos] = s.sn_i[r];
-            if (r == 0) best = pos;
-        }
-    }
-    // old items j = lane + 64 e: position = j + #new with d_new <= d_old, the new distances
-    // broadcast lane by lane
-    constexpr int E = 8;
     for (int j0 = 0; j0 < size; j0 += E * kWave) {
         float dj[E];
         uint32_t ij[E];
@@ -241,6 +224,38 @@ __device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *
             }
         }
     }
+}
+
+// Merge the sorted new items into the queue (qd/qi -> od/oi, then swap). New items go
+// before old items of equal distance; positions >= cap are dropped. cursor_old is the
+// first unvisited index of the old queue (or size); *cursor receives the first unvisited
+// index of the merged queue (or its size). Returns the new size.
+__device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *&qd,
+                                  uint32_t *&qi, float *&od, uint32_t *&oi, int cursor_old,
+                                  int *cursor) {
+    const int lane = lane_id();
+    int best = 0x7FFFFFFF;
+    // new items: position = rank + #old with d_old < d  (lower bound in the old queue)
+    for (int r = lane; r < nvalid; r += kWave) {
+        float d = s.sn_d[r];
+        int lo = 0, hi = size;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (qd[mid] < d) lo = mid + 1; else hi = mid;
+        }
+        int pos = r + lo;
+        if (pos < cap) {
+            od[pos] = d;
+            oi[pos] = s.sn_i[r];
+            if (r == 0) best = pos;
+        }
+    }
+    // old items j = lane + 64 e: position = j + #new with d_new <= d_old, the new distances
+    // broadcast lane by lane; the register tile E follows the queue size
+    if (size <= kWave) merge_old<1>(s, size, nvalid, cap, qd, qi, od, oi, cursor_old, best);
+    else if (size <= 2 * kWave) merge_old<2>(s, size, nvalid, cap, qd, qi, od, oi, cursor_old, best);
+    else if (size <= 4 * kWave) merge_old<4>(s, size, nvalid, cap, qd, qi, od, oi, cursor_old, best);
+    else merge_old<8>(s, size, nvalid, cap, qd, qi, od, oi, cursor_old, best);
     best = wave_min_i(best);
     __syncthreads();
     float *td = qd; qd = od; od = td;
@@ -252,7 +267,7 @@ __device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *
 }
 
 // Full-precision distances for n candidates (ids in LDS) into out (LDS): one half-wave per
-// candidate; for DIM a multiple of 32 up to 128, sixteen candidates per half-wave have their
+// candidate; for DIM a multiple of 32 up to 128, eight candidates per half-wave have their
 // rows in flight at once.
 template <int METRIC, int DIM>
 __device__ inline void fp_distances(const SearchParams &p, const float *q, const uint32_t *ids,
@@ -260,7 +275,7 @@ __device__ inline void fp_distances(const SearchParams &p, const float *q, const
     const int lane = lane_id(), half = lane >> 5, hp = lane & 31;
     if constexpr (METRIC != METRIC_COSINE && DIM > 0 && DIM % 32 == 0 && DIM <= 128) {
         constexpr int NI = DIM / 32;
-        constexpr int CPB = 16;
+        constexpr int CPB = 8;
         constexpr bool IP = METRIC == METRIC_IP;
         float qv[NI];
 #pragma unroll

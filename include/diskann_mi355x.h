@@ -179,7 +179,7 @@ dkm_status dkm_search_batch_device_stats(dkm_index *ix, int strategy, const floa
                                          uint32_t *d_out_degsum);
 
 /* Tuning knobs (autotuner surface; none changes results): visited-set slots in LDS per
- * query (256..32768, 0 = automatic) and the PQ distance mode (0 = lookup table in LDS,
+ * query (a multiple of 64 in 256..32768, 0 = automatic) and the PQ distance mode (0 = lookup table in LDS,
  * 1 = direct recomputation from the pivots, the default). */
 dkm_status dkm_set_tuning(dkm_index *ix, int64_t visited_lds_slots, int adc_mode);
 

@@ -6,11 +6,11 @@ interfaces on top of it.
 """
 from .errors import ANNError, CapacityError, DimensionMismatchError, KnnSearchError, PQError
 from .index import (DiskANNIndex, FullPrecision, Hybrid, Knn, Metric, Quantized, SearchResult,
-                    Strategy)
+                    Server, Strategy)
 from . import quantization
 
 __all__ = [
     "ANNError", "CapacityError", "DimensionMismatchError", "KnnSearchError", "PQError",
     "DiskANNIndex", "FullPrecision", "Hybrid", "Knn", "Metric", "Quantized", "SearchResult",
-    "Strategy", "quantization",
+    "Server", "Strategy", "quantization",
 ]

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -944,6 +945,267 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
     HIP_TRY(hipMemcpy(&st, bstat.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (st & dkm::BUILD_ERR_RECORD) return fail(DKM_ERR_CAPACITY, "search record exceeded %u entries", rec_cap);
     if (st & dkm::BUILD_ERR_POOL) return fail(DKM_ERR_CAPACITY, "candidate pool exceeded its capacity");
+    return DKM_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Serving
+// ------------------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct dkm_server {
+    dkm_index *ix = nullptr;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    int strategy = 0;
+    uint32_t max_k = 0, max_L = 0, bw = 1;
+    int64_t capacity = 0, slots = 0;
+    dkm::ServeCtrl *ctrl = nullptr;  // pinned host memory
+    dkm::ServeReq *req = nullptr;
+    float *queries = nullptr;
+    uint32_t *res_ids = nullptr;
+    float *res_d = nullptr;
+    uint32_t *res_meta = nullptr;
+    uint64_t *claim_seq = nullptr;
+    uint64_t *done_seq = nullptr;
+    dkm::ServeDev *d_dev = nullptr;  // device memory
+    uint32_t *d_slot_status = nullptr;
+    uint64_t *d_spill = nullptr;
+    dkm::SearchParams p{};
+    dkm::ServeParams sp{};
+    uint64_t tail = 0;
+    std::vector<uint8_t> busy;
+    std::vector<double> t_submit;
+    std::vector<uint64_t> open;
+};
+
+static double now_us() {
+    return std::chrono::duration<double, std::micro>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+template <typename T>
+static hipError_t pinned(T **host, size_t count) {
+    void *h = nullptr;
+    hipError_t e = hipHostMalloc(&h, sizeof(T) * std::max<size_t>(count, 1),
+                                 hipHostMallocCoherent | hipHostMallocMapped);
+    if (e != hipSuccess) return e;
+    memset(h, 0, sizeof(T) * std::max<size_t>(count, 1));
+    *host = (T *)h;
+    return hipSuccess;
+}
+
+template <typename T>
+static T *devptr(T *host) {
+    void *d = nullptr;
+    if (hipHostGetDevicePointer(&d, host, 0) != hipSuccess) return host;
+    return (T *)d;
+}
+
+extern "C" {
+
+static void server_free(dkm_server *sv) {
+    if (!sv) return;
+    void *hp[] = {sv->ctrl,  sv->req,       sv->queries, sv->res_ids,
+                  sv->res_d, sv->res_meta, sv->claim_seq, sv->done_seq};
+    for (void *q : hp)
+        if (q) (void)hipHostFree(q);
+    void *dp[] = {sv->d_dev, sv->d_slot_status, sv->d_spill};
+    for (void *q : dp)
+        if (q) (void)hipFree(q);
+    if (sv->stream) (void)hipStreamDestroy(sv->stream);
+    delete sv;
+}
+
+// (Re)launch the persistent loop if requests wait unclaimed and it is not running.
+static dkm_status server_kick(dkm_server *sv) {
+    // claims are taken in sequence order, so the newest request tells whether any waits
+    if (sv->tail == 0) return DKM_OK;
+    const uint64_t last = sv->tail - 1;
+    if (__atomic_load_n(&sv->claim_seq[last % (uint64_t)sv->capacity], __ATOMIC_ACQUIRE) == last + 1)
+        return DKM_OK;
+    hipError_t q = hipStreamQuery(sv->stream);
+    if (q == hipErrorNotReady) return DKM_OK;
+    if (q != hipSuccess) return fail(DKM_ERR_DEVICE, "serving stream: %s", hipGetErrorString(q));
+    HIP_TRY(dkm::launch_serve(sv->p, sv->sp, (int)sv->slots, sv->stream));
+    return DKM_OK;
+}
+
+dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_t max_L,
+                            uint32_t beam_width, int64_t capacity, int64_t slots,
+                            dkm_server **out) {
+    if (!ix || !out) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (strategy < 0 || strategy > 2) return fail(DKM_ERR_INVALID_ARG, "unknown strategy %d", strategy);
+    if (max_k == 0 || max_L == 0 || max_k > max_L || max_L > 4096)
+        return fail(DKM_ERR_INVALID_ARG, "bad max_k / max_L");
+    if (beam_width == 0 || beam_width > 64) return fail(DKM_ERR_BEAM_WIDTH_ZERO, "bad beam width");
+    if (capacity < 1 || slots < 1 || slots > 65536) return fail(DKM_ERR_INVALID_ARG, "bad capacity / slots");
+    if (!ix->starts_set) return fail(DKM_ERR_NOT_READY, "start points not set");
+    const bool pq = strategy != DKM_STRATEGY_FULL;
+    if (pq && (!ix->d_codes || !ix->codes_set))
+        return fail(DKM_ERR_NOT_READY, "PQ strategy needs a PQ schema and codes");
+    if (pq && ix->metric == DKM_COSINE) return fail(DKM_ERR_PQ, "cosine PQ traversal is not supported");
+    HIP_TRY(hipSetDevice(ix->device));
+    dkm_server *sv = new dkm_server();
+    sv->ix = ix;
+    sv->strategy = strategy;
+    sv->max_k = max_k;
+    sv->max_L = max_L;
+    sv->bw = beam_width;
+    sv->capacity = capacity;
+    sv->slots = slots;
+    hipError_t e = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = pinned(&sv->ctrl, 1);
+    if (e == hipSuccess) e = pinned(&sv->req, capacity);
+    if (e == hipSuccess) e = pinned(&sv->queries, (size_t)capacity * ix->dim);
+    if (e == hipSuccess) e = pinned(&sv->res_ids, (size_t)capacity * max_k);
+    if (e == hipSuccess) e = pinned(&sv->res_d, (size_t)capacity * max_k);
+    if (e == hipSuccess) e = pinned(&sv->res_meta, (size_t)capacity * 4);
+    if (e == hipSuccess) e = pinned(&sv->claim_seq, capacity);
+    if (e == hipSuccess) e = pinned(&sv->done_seq, capacity);
+    if (e == hipSuccess) e = hipMalloc(&sv->d_dev, sizeof(dkm::ServeDev));
+    if (e == hipSuccess) e = hipMemset(sv->d_dev, 0, sizeof(dkm::ServeDev));
+    if (e == hipSuccess) e = hipMalloc(&sv->d_slot_status, sizeof(uint32_t) * slots);
+    if (e == hipSuccess)
+        e = hipMalloc(&sv->d_spill, sizeof(uint64_t) * ix->spill_slots * (size_t)slots);
+    if (e == hipSuccess)
+        e = hipMemset(sv->d_spill, 0, sizeof(uint64_t) * ix->spill_slots * (size_t)slots);
+    if (e != hipSuccess) {
+        server_free(sv);
+        return fail(e == hipErrorOutOfMemory ? DKM_ERR_OUT_OF_MEMORY : DKM_ERR_DEVICE,
+                    "server allocation: %s", hipGetErrorString(e));
+    }
+    sv->busy.assign(capacity, 0);
+    sv->t_submit.assign(capacity, 0.0);
+    dkm::SearchParams &p = sv->p;
+    p.metric = ix->metric;
+    p.pq = pq;
+    p.rerank = strategy == DKM_STRATEGY_HYBRID;
+    p.adc_mode = 1;  // the serving kernel always uses the table-free ADC
+    p.pq_len = pq ? ix->pq_len : 0;
+    p.dim = (int)ix->dim;
+    p.vstride = ix->vstride;
+    p.vectors = ix->d_vec;
+    p.graph = ix->d_graph;
+    p.gstride = (int)ix->gstride;
+    p.starts = ix->d_starts;
+    p.nstart = (int)ix->num_start;
+    p.codes = ix->d_codes;
+    p.code_stride = (int)ix->M;
+    p.M = (int)ix->M;
+    p.pivots = ix->d_pivots;
+    p.offsets = ix->d_offsets;
+    p.beam_width = beam_width;
+    p.k_stride = max_k;
+    p.qcap = (int)(max_L + ix->num_start);
+    p.nlcap = (int)(beam_width * ix->R_max + 64);
+    p.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots : default_visited_slots(max_L);
+    p.spill_table = sv->d_spill;
+    p.spill_slots = ix->spill_slots;
+    p.max_iters = (uint32_t)std::min<int64_t>(ix->rows + 16, 0x7FFFFFFF);
+    if (dkm::search_lds_bytes(p) > 160 * 1024) {
+        server_free(sv);
+        return fail(DKM_ERR_CAPACITY, "per-query LDS for max_L %u exceeds 160 KiB", max_L);
+    }
+    dkm::ServeParams &q = sv->sp;
+    q.ctrl = devptr(sv->ctrl);
+    q.dev = sv->d_dev;
+    q.req = devptr(sv->req);
+    q.queries = devptr(sv->queries);
+    q.res_ids = devptr(sv->res_ids);
+    q.res_d = devptr(sv->res_d);
+    q.res_meta = devptr(sv->res_meta);
+    q.claim_seq = devptr(sv->claim_seq);
+    q.done_seq = devptr(sv->done_seq);
+    q.capacity = (uint64_t)capacity;
+    q.max_k = max_k;
+    q.idle_ticks = 2000000;   // 20 ms at 100 MHz
+    q.run_ticks = 500000000;  // 5 s per launch, then the host relaunches
+    q.slot_status = sv->d_slot_status;
+    *out = sv;
+    return DKM_OK;
+}
+
+dkm_status dkm_server_submit(dkm_server *sv, const float *query, uint32_t k, uint32_t L,
+                             uint64_t tag) {
+    if (!sv || !query) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (k == 0) return fail(DKM_ERR_K_ZERO, "k must be >= 1");
+    if (L == 0) return fail(DKM_ERR_L_ZERO, "L must be >= 1");
+    if (k > L) return fail(DKM_ERR_L_LESS_THAN_K, "L (%u) < k (%u)", L, k);
+    if (k > sv->max_k || L > sv->max_L)
+        return fail(DKM_ERR_INVALID_ARG, "k or L above the server's maximum");
+    std::lock_guard<std::mutex> g(sv->mu);
+    HIP_TRY(hipSetDevice(sv->ix->device));
+    const uint64_t seq = sv->tail;
+    const uint64_t slot = seq % (uint64_t)sv->capacity;
+    if (sv->busy[slot]) return fail(DKM_ERR_CAPACITY, "request ring full");
+    if ((uint32_t)(seq + 1) == 0) return fail(DKM_ERR_CAPACITY, "request sequence exhausted");
+    sv->req[slot].k = k;
+    sv->req[slot].L = L;
+    sv->req[slot].tag = tag;
+    memcpy(sv->queries + slot * sv->ix->dim, query, sizeof(float) * sv->ix->dim);
+    sv->busy[slot] = 1;
+    sv->t_submit[slot] = now_us();
+    sv->open.push_back(seq);
+    sv->tail = seq + 1;
+    __atomic_store_n(&sv->ctrl->tail, sv->tail, __ATOMIC_RELEASE);
+    return server_kick(sv);
+}
+
+dkm_status dkm_server_poll(dkm_server *sv, int64_t max_results, uint64_t *tags, uint32_t *ids,
+                           float *dists, uint32_t *counts, uint32_t *cmps, uint32_t *hops,
+                           double *latency_us, int64_t *n_out) {
+    if (!sv || !n_out) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    HIP_TRY(hipSetDevice(sv->ix->device));
+    int64_t n = 0;
+    const double t = now_us();
+    for (size_t i = 0; i < sv->open.size() && n < max_results;) {
+        const uint64_t seq = sv->open[i];
+        const uint64_t slot = seq % (uint64_t)sv->capacity;
+        if (__atomic_load_n(&sv->done_seq[slot], __ATOMIC_ACQUIRE) != seq + 1) {
+            ++i;
+            continue;
+        }
+        const uint32_t *meta = sv->res_meta + slot * 4;
+        if (meta[3] & dkm::ERR_VISITED_OVERFLOW)
+            return fail(DKM_ERR_CAPACITY, "request %llu: visited set overflowed", (unsigned long long)seq);
+        if (meta[3] & dkm::ERR_ITER_LIMIT)
+            return fail(DKM_ERR_INDEX, "request %llu: traversal exceeded the iteration bound",
+                        (unsigned long long)seq);
+        if (tags) tags[n] = sv->req[slot].tag;
+        if (ids) memcpy(ids + n * sv->max_k, sv->res_ids + slot * sv->max_k, sizeof(uint32_t) * sv->max_k);
+        if (dists) memcpy(dists + n * sv->max_k, sv->res_d + slot * sv->max_k, sizeof(float) * sv->max_k);
+        if (counts) counts[n] = meta[0];
+        if (cmps) cmps[n] = meta[1];
+        if (hops) hops[n] = meta[2];
+        if (latency_us) latency_us[n] = t - sv->t_submit[slot];
+        sv->busy[slot] = 0;
+        sv->open[i] = sv->open.back();
+        sv->open.pop_back();
+        ++n;
+    }
+    *n_out = n;
+    return server_kick(sv);
+}
+
+dkm_status dkm_server_pending(dkm_server *sv, int64_t *pending) {
+    if (!sv || !pending) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(sv->mu);
+    *pending = (int64_t)sv->open.size();
+    return DKM_OK;
+}
+
+dkm_status dkm_server_stop(dkm_server *sv) {
+    if (!sv) return DKM_OK;
+    (void)hipSetDevice(sv->ix->device);
+    __atomic_store_n(&sv->ctrl->stop, 1u, __ATOMIC_RELEASE);
+    hipError_t e = hipStreamSynchronize(sv->stream);
+    server_free(sv);
+    if (e != hipSuccess) return fail(DKM_ERR_DEVICE, "serving stream: %s", hipGetErrorString(e));
     return DKM_OK;
 }
 

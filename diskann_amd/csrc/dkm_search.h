@@ -56,3 +56,51 @@ size_t search_lds_bytes(const SearchParams &p);
 hipError_t launch_search(const SearchParams &p, hipStream_t stream);
 
 }  // namespace dkm
+
+namespace dkm {
+
+// Serving: request ring and control block in pinned host memory (fine-grained), claim
+// counter in device memory. A request is published by storing tail = seq + 1 (release);
+// a wave claims seq by CAS on `next`, searches, writes its results into the ring entry
+// and stores done_seq = seq + 1 (release).
+struct ServeCtrl {
+    uint64_t tail;     // published requests (host writes)
+    uint32_t stop;     // host asks idle waves to exit
+    uint32_t pad;
+};
+
+// Device-side mirror of the control block: idle waves poll this (L2) instead of host
+// memory; one wave at a time (holder of `lock`) refreshes it from host memory.
+struct ServeDev {
+    uint64_t next;  // claim counter
+    uint64_t tail;  // last tail read from host memory
+    uint32_t stop;
+    uint32_t lock;
+};
+
+struct ServeReq {
+    uint32_t k, L;
+    uint64_t tag;
+};
+
+struct ServeParams {
+    const ServeCtrl *ctrl;      // pinned
+    ServeDev *dev;              // device
+    const ServeReq *req;        // pinned, capacity entries
+    const float *queries;       // pinned, capacity x dim
+    uint32_t *res_ids;          // pinned, capacity x max_k
+    float *res_d;
+    uint32_t *res_meta;         // pinned, capacity x 4: count, cmps, hops, status
+    uint64_t *claim_seq;        // pinned, capacity: seq + 1 once claimed
+    uint64_t *done_seq;         // pinned, capacity: seq + 1 once the result is written
+    uint64_t capacity;
+    uint32_t max_k;
+    uint64_t idle_ticks;        // exit after this many 100 MHz ticks without work
+    uint64_t run_ticks;         // no new claims once a block has run this long (100 MHz)
+    uint32_t *slot_status;      // device, per block
+};
+
+hipError_t launch_serve(const SearchParams &p, const ServeParams &sp, int blocks,
+                        hipStream_t stream);
+
+}  // namespace dkm

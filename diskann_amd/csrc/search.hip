@@ -88,7 +88,7 @@ __device__ inline Lds carve(const SearchParams &p, char *base) {
 // per chunk, uniformly) it is only read, and new ids go to this query's region of the
 // HBM spill table (64-bit entries tagged with the launch epoch, linear probing).
 __device__ inline bool visit_insert(const SearchParams &p, Lds &s, uint32_t id, bool active,
-                                    bool spill, uint64_t *gtab, uint32_t *err) {
+                                    bool spill, uint64_t *gtab, uint32_t *err, uint32_t epoch) {
     if (!active) return false;
     const uint32_t HB = p.hslots >> 2;  // buckets
     uint32_t b = (uint32_t)(((uint64_t)hash_id(id) * HB) >> 32);
@@ -113,13 +113,13 @@ __device__ inline bool visit_insert(const SearchParams &p, Lds &s, uint32_t id, 
         return false;
     }
     const uint64_t GH = p.spill_slots;
-    const uint64_t key = ((uint64_t)p.epoch << 32) | id;
+    const uint64_t key = ((uint64_t)epoch << 32) | id;
     uint64_t g = hash_id(id ^ 0x9e3779b9u) & (GH - 1);
     for (uint64_t probe = 0; probe < GH; ++probe) {
         uint64_t cur = __hip_atomic_load(&gtab[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         for (;;) {
             if (cur == key) return false;
-            if ((uint32_t)(cur >> 32) == p.epoch) break;  // live entry of another id
+            if ((uint32_t)(cur >> 32) == epoch) break;  // live entry of another id
             uint64_t prev = atomicCAS((unsigned long long *)&gtab[g], (unsigned long long)cur,
                                       (unsigned long long)key);
             if (prev == cur) return true;
@@ -460,21 +460,32 @@ __device__ inline void compute_distances(const SearchParams &p, Lds &s, int n) {
     __syncthreads();
 }
 
+// One query's search with the wave's LDS state; used by the batched kernel (one block
+// per query) and by the persistent serving kernel (a block loops over requests).
+struct QueryJob {
+    const float *query;
+    int L, k;
+    uint32_t *err;
+    uint64_t *spill;        // this query's HBM visited region
+    uint32_t epoch;         // tag of this query's entries in the region
+    uint32_t *rec_ids;      // optional visited record (rec_cap entries)
+    float *rec_d;
+    uint32_t *out_ids;      // k_stride entries
+    float *out_d;
+    uint32_t *out_count, *out_cmps, *out_hops, *rec_len, *out_degsum;
+};
+
 template <int METRIC, int MODE, int DIM, int PQLEN>
-__global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
+__device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJob &job) {
     constexpr bool PQ = MODE != 0;
-    extern __shared__ __align__(16) char smem[];
-    const int qidx = blockIdx.x;
-    if (qidx >= p.nq) return;
     const int lane = threadIdx.x;
-    Lds s = carve(p, smem);
-    const int L = (int)p.L[qidx];
-    const int k = (int)p.k[qidx];
+    const int L = job.L;
+    const int k = job.k;
     const int cap = L + p.nstart;
-    uint32_t *err = p.status + qidx;
+    uint32_t *err = job.err;
 
     // ---- init --------------------------------------------------------------------
-    const float *qg = p.queries + (size_t)qidx * p.qstride;
+    const float *qg = job.query;
     for (int e = lane; e < p.dim; e += kWave) s.q[e] = qg[e];
     for (uint32_t h = lane; h < p.hslots; h += kWave) s.hash[h] = kEmpty;
     for (int i = lane; i < p.nstart; i += kWave) s.starts[i] = p.starts[i];
@@ -512,7 +523,7 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     __syncthreads();
     for (int i = 0; i < p.nstart; ++i) {
         // sequential inserts (start ids may repeat only in malformed input)
-        bool ins = visit_insert(p, s, s.starts[i], lane == 0, false, nullptr, err);
+        bool ins = visit_insert(p, s, s.starts[i], lane == 0, false, nullptr, err, job.epoch);
         (void)ins;
         __syncthreads();
     }
@@ -546,9 +557,9 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
                 const uint32_t rid = qi[i];
                 s.beam[nbeam + before] = rid;
                 qi[i] = rid | kVisitedBit;
-                if (p.rec_ids && nrec + before < p.rec_cap) {
-                    p.rec_ids[(size_t)qidx * p.rec_cap + nrec + before] = rid;
-                    p.rec_d[(size_t)qidx * p.rec_cap + nrec + before] = qd[i];
+                if (job.rec_ids && nrec + before < p.rec_cap) {
+                    job.rec_ids[nrec + before] = rid;
+                    job.rec_d[nrec + before] = qd[i];
                 }
             }
             const int got = __builtin_popcountll(m);
@@ -582,9 +593,8 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
                 bool act = j < cnt;
                 uint32_t id = act ? row[j] : 0;
                 bool spill = lds_count > spill_at;
-                if (spill && gtab == nullptr && p.spill_table)
-                    gtab = p.spill_table + (size_t)qidx * p.spill_slots;  // this query's region
-                bool fresh = visit_insert(p, s, id, act, spill, gtab, err);
+                if (spill && gtab == nullptr) gtab = job.spill;  // this query's HBM region
+                bool fresh = visit_insert(p, s, id, act, spill, gtab, err, job.epoch);
                 uint64_t m = ballot(fresh);
                 if (fresh) s.nl_i[nn + prefix_count(m)] = id;
                 int got = __builtin_popcountll(m);
@@ -602,8 +612,8 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     }
 
     // ---- post-processing -----------------------------------------------------------
-    uint32_t *oids = p.out_ids + (size_t)qidx * p.k_stride;
-    float *ods = p.out_d + (size_t)qidx * p.k_stride;
+    uint32_t *oids = job.out_ids;
+    float *ods = job.out_d;
     int count = 0;
     if (!p.rerank) {
         // FilterStartPoints -> CopyIds: first k non-start entries in queue order
@@ -681,11 +691,152 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
         ods[i] = __builtin_inff();
     }
     if (lane == 0) {
-        p.out_count[qidx] = (uint32_t)count;
-        if (p.out_cmps) p.out_cmps[qidx] = cmps;
-        if (p.out_hops) p.out_hops[qidx] = hops;
-        if (p.rec_len) p.rec_len[qidx] = nrec;
-        if (p.out_degsum) p.out_degsum[qidx] = degsum;
+        *job.out_count = (uint32_t)count;
+        if (job.out_cmps) *job.out_cmps = cmps;
+        if (job.out_hops) *job.out_hops = hops;
+        if (job.rec_len) *job.rec_len = nrec;
+        if (job.out_degsum) *job.out_degsum = degsum;
+    }
+}
+
+template <int METRIC, int MODE, int DIM, int PQLEN>
+__global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
+    extern __shared__ __align__(16) char smem[];
+    const int qidx = blockIdx.x;
+    if (qidx >= p.nq) return;
+    Lds s = carve(p, smem);
+    QueryJob j;
+    j.query = p.queries + (size_t)qidx * p.qstride;
+    j.L = (int)p.L[qidx];
+    j.k = (int)p.k[qidx];
+    j.err = p.status + qidx;
+    j.spill = p.spill_table ? p.spill_table + (size_t)qidx * p.spill_slots : nullptr;
+    j.epoch = p.epoch;
+    j.rec_ids = p.rec_ids ? p.rec_ids + (size_t)qidx * p.rec_cap : nullptr;
+    j.rec_d = p.rec_d ? p.rec_d + (size_t)qidx * p.rec_cap : nullptr;
+    j.out_ids = p.out_ids + (size_t)qidx * p.k_stride;
+    j.out_d = p.out_d + (size_t)qidx * p.k_stride;
+    j.out_count = p.out_count + qidx;
+    j.out_cmps = p.out_cmps ? p.out_cmps + qidx : nullptr;
+    j.out_hops = p.out_hops ? p.out_hops + qidx : nullptr;
+    j.rec_len = p.rec_len ? p.rec_len + qidx : nullptr;
+    j.out_degsum = p.out_degsum ? p.out_degsum + qidx : nullptr;
+    search_query<METRIC, MODE, DIM, PQLEN>(p, s, j);
+}
+
+// Persistent serving kernel: each block (one wave) claims requests from the ring one at
+// a time and runs search_query on them, so requests that arrive while others are being
+// searched start as soon as any block frees up. Exit conditions every block reaches:
+// no work for idle_ticks, the stop flag while idle, or run_ticks elapsed before a claim.
+template <int METRIC, int MODE, int DIM, int PQLEN>
+__global__ __launch_bounds__(64) void serve_kernel(SearchParams p, ServeParams sp) {
+    extern __shared__ __align__(16) char smem[];
+    Lds s = carve(p, smem);
+    const int lane = threadIdx.x;
+    uint64_t *region = p.spill_table + (size_t)blockIdx.x * p.spill_slots;
+    const uint64_t started = wall_clock64();
+    uint64_t idle_start = started;
+    for (;;) {
+        // lane 0 claims the next published request or decides to exit
+        if (lane == 0) {
+            int64_t got = -1;
+            ServeDev *dv = sp.dev;
+            for (;;) {
+                const uint64_t now = wall_clock64();
+                if (now - started >= sp.run_ticks) break;
+                uint64_t t = __hip_atomic_load(&dv->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t n = __hip_atomic_load(&dv->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (n >= t && atomicCAS(&dv->lock, 0u, 1u) == 0u) {
+                    // this wave refreshes the mirror from host memory
+                    t = __hip_atomic_load(&sp.ctrl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                    const uint32_t st = __hip_atomic_load(&sp.ctrl->stop, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_SYSTEM);
+                    __hip_atomic_fetch_max(&dv->tail, t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    if (st) __hip_atomic_store(&dv->stop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&dv->lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (n < t) {
+                    if (atomicCAS((unsigned long long *)&dv->next, (unsigned long long)n,
+                                  (unsigned long long)(n + 1)) == n) {
+                        got = (int64_t)n;
+                        __hip_atomic_store(&sp.claim_seq[n % sp.capacity], n + 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_SYSTEM);
+                        break;
+                    }
+                    continue;
+                }
+                if (__hip_atomic_load(&dv->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (now - idle_start > sp.idle_ticks) break;
+                __builtin_amdgcn_s_sleep(32);
+            }
+            s.misc[2] = (int32_t)(got & 0x7FFFFFFF);
+            s.misc[3] = got < 0 ? 1 : 0;
+            s.misc[1] = (int32_t)((uint64_t)got >> 31);
+        }
+        __syncthreads();
+        if (s.misc[3]) break;
+        const uint64_t seq = ((uint64_t)(uint32_t)s.misc[1] << 31) | (uint32_t)s.misc[2];
+        __syncthreads();
+        const uint64_t slot = seq % sp.capacity;
+        const ServeReq rq = sp.req[slot];
+        QueryJob job;
+        job.query = sp.queries + slot * (uint64_t)p.dim;
+        job.L = (int)rq.L;
+        job.k = (int)rq.k;
+        job.err = sp.slot_status + blockIdx.x;
+        job.spill = region;
+        job.epoch = (uint32_t)(seq + 1);
+        job.rec_ids = nullptr;
+        job.rec_d = nullptr;
+        job.out_ids = sp.res_ids + slot * sp.max_k;
+        job.out_d = sp.res_d + slot * sp.max_k;
+        uint32_t *meta = sp.res_meta + slot * 4;
+        job.out_count = meta;
+        job.out_cmps = meta + 1;
+        job.out_hops = meta + 2;
+        job.rec_len = nullptr;
+        job.out_degsum = nullptr;
+        search_query<METRIC, MODE, DIM, PQLEN>(p, s, job);
+        __threadfence_system();
+        __syncthreads();
+        if (lane == 0) {
+            meta[3] = *job.err;
+            __threadfence_system();
+            __hip_atomic_store(&sp.done_seq[slot], seq + 1, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        idle_start = wall_clock64();
+    }
+}
+
+template <int METRIC, int MODE, int DIM, int PQLEN>
+static hipError_t launch_serve_t(const SearchParams &p, const ServeParams &sp, int blocks,
+                                 hipStream_t stream) {
+    size_t lds = lds_bytes(p);
+    auto kern = serve_kernel<METRIC, MODE, DIM, PQLEN>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64), lds, stream, p, sp);
+    return hipGetLastError();
+}
+
+hipError_t launch_serve(const SearchParams &p, const ServeParams &sp, int blocks,
+                        hipStream_t stream) {
+    if (p.pq) {
+        const bool ip = p.metric == METRIC_IP;
+        if (ip) return launch_serve_t<METRIC_IP, 2, 0, 0>(p, sp, blocks, stream);
+        if (p.dim == 128 && p.pq_len == 4) return launch_serve_t<METRIC_L2, 2, 128, 4>(p, sp, blocks, stream);
+        if (p.dim == 96 && p.pq_len == 2) return launch_serve_t<METRIC_L2, 2, 96, 2>(p, sp, blocks, stream);
+        return launch_serve_t<METRIC_L2, 2, 0, 0>(p, sp, blocks, stream);
+    }
+    switch (p.metric) {
+    case METRIC_L2:
+        if (p.dim == 128) return launch_serve_t<METRIC_L2, 0, 128, 0>(p, sp, blocks, stream);
+        if (p.dim == 96) return launch_serve_t<METRIC_L2, 0, 96, 0>(p, sp, blocks, stream);
+        return launch_serve_t<METRIC_L2, 0, 0, 0>(p, sp, blocks, stream);
+    case METRIC_IP: return launch_serve_t<METRIC_IP, 0, 0, 0>(p, sp, blocks, stream);
+    default: return launch_serve_t<METRIC_COSINE, 0, 0, 0>(p, sp, blocks, stream);
     }
 }
 

@@ -217,3 +217,69 @@ class DiskANNIndex:
         r = self.search_batch(query, knn.k, knn.L, knn.beam_width, strategy)
         n = int(r.counts[0])
         return r.ids[0, :n], r.distances[0, :n], int(r.cmps[0]), int(r.hops[0])
+
+
+class Server:
+    """Continuous-batching request loop over one index.
+
+    A persistent kernel keeps `slots` queries in flight; requests are placed in a ring of
+    `capacity` entries in pinned host memory and each is claimed by the first free wave,
+    so a request never waits for a batch to fill. Each result equals what `search_batch`
+    returns for that query alone (same k, L, beam width and strategy). Submitting to a
+    full ring raises CapacityError; `poll` returns finished requests in completion order.
+    """
+
+    def __init__(self, index, strategy=Strategy.HYBRID, max_k=10, max_L=512, beam_width=1,
+                 capacity=4096, slots=2048):
+        self.index, self.lib = index, index.lib
+        self.dim, self.max_k = index.dim, int(max_k)
+        h = C.c_void_p()
+        check(self.lib.dkm_server_start(index._h, int(strategy), int(max_k), int(max_L),
+                                        int(beam_width), int(capacity), int(slots), C.byref(h)),
+              self.lib)
+        self._h = h
+
+    def submit(self, query, k, L, tag):
+        q = np.ascontiguousarray(query, dtype=np.float32).reshape(-1)
+        if q.shape[0] != self.dim:
+            raise ValueError("query dim %d != index dim %d" % (q.shape[0], self.dim))
+        check(self.lib.dkm_server_submit(self._h, _p(q), int(k), int(L), int(tag)), self.lib)
+
+    def poll(self, max_results=4096):
+        """Finished requests: (tags, ids[n, max_k], dists[n, max_k], counts, cmps, hops,
+        latency_us); latency is measured from submit to the poll that returned it."""
+        m = int(max_results)
+        tags = np.zeros(m, dtype=np.uint64)
+        ids = np.zeros((m, self.max_k), dtype=np.uint32)
+        ds = np.zeros((m, self.max_k), dtype=np.float32)
+        cnt = np.zeros(m, dtype=np.uint32)
+        cmps = np.zeros(m, dtype=np.uint32)
+        hops = np.zeros(m, dtype=np.uint32)
+        lat = np.zeros(m, dtype=np.float64)
+        n = C.c_int64()
+        check(self.lib.dkm_server_poll(self._h, m, _p(tags), _p(ids), _p(ds), _p(cnt), _p(cmps),
+                                       _p(hops), _p(lat), C.byref(n)), self.lib)
+        k = n.value
+        return tags[:k], ids[:k], ds[:k], cnt[:k], cmps[:k], hops[:k], lat[:k]
+
+    def pending(self):
+        n = C.c_int64()
+        check(self.lib.dkm_server_pending(self._h, C.byref(n)), self.lib)
+        return n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            h, self._h = self._h, None
+            check(self.lib.dkm_server_stop(h), self.lib)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

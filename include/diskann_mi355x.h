@@ -178,6 +178,33 @@ dkm_status dkm_search_batch_device_stats(dkm_index *ix, int strategy, const floa
                                          uint32_t *d_out_cmps, uint32_t *d_out_hops,
                                          uint32_t *d_out_degsum);
 
+/* -------------------------------------------------------------------------------- */
+/* Serving: persistent request loop with continuous batching                        */
+/* -------------------------------------------------------------------------------- */
+/* A persistent kernel keeps `slots` queries resident (one wave each) and claims
+ * requests from a ring of `capacity` entries in pinned host memory as soon as a wave
+ * frees up, so requests that arrive while others run join the work already on the GPU.
+ * Admission is by device memory: each slot reserves its visited-set region in HBM and
+ * its LDS; requests beyond the free ring entries are refused with DKM_ERR_CAPACITY
+ * (back-pressure). Requests may mix k and L (k <= max_k, k <= L <= max_L); results equal
+ * dkm_search_batch for the same query, k, L and beam width. Idle waves exit after a
+ * short timeout and the loop is relaunched on demand. Thread-safe per server. */
+typedef struct dkm_server dkm_server;
+dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_t max_L,
+                            uint32_t beam_width, int64_t capacity, int64_t slots,
+                            dkm_server **out);
+dkm_status dkm_server_submit(dkm_server *sv, const float *query, uint32_t k, uint32_t L,
+                             uint64_t tag);
+/* Returns up to max_results completed requests (any order): tag, max_k ids / distances
+ * per result (counts[i] valid), comparisons, hops and the request's latency in
+ * microseconds from submit to the poll that saw it complete. */
+dkm_status dkm_server_poll(dkm_server *sv, int64_t max_results, uint64_t *tags,
+                           uint32_t *ids, float *dists, uint32_t *counts, uint32_t *cmps,
+                           uint32_t *hops, double *latency_us, int64_t *n_out);
+/* Requests submitted and not yet returned by dkm_server_poll. */
+dkm_status dkm_server_pending(dkm_server *sv, int64_t *pending);
+dkm_status dkm_server_stop(dkm_server *sv);
+
 /* Tuning knobs (autotuner surface; none changes results): visited-set slots in LDS per
  * query (a multiple of 64 in 256..32768, 0 = automatic) and the PQ distance mode (0 = lookup table in LDS,
  * 1 = direct recomputation from the pivots, the default). */

@@ -71,6 +71,82 @@ def recall_at_10(ids, gt):
     return hit / (10.0 * ids.shape[0])
 
 
+def serve_drain(sv, want, arrival, t0, lat_ms, ids_out, limit_s=60.0):
+    """Polls until `want` more results arrive; latency is measured from each request's
+    arrival time (not its submit time)."""
+    got = 0
+    tl = time.perf_counter()
+    while got < want:
+        tags, ids, _, _, _, _, _ = sv.poll(4096)
+        if len(tags):
+            now = (time.perf_counter() - t0) * 1e3
+            tg = tags.astype(np.int64)
+            lat_ms[tg] = now - arrival[tg]
+            if ids_out is not None:
+                ids_out[tg % ids_out.shape[0]] = ids
+            got += len(tags)
+        elif time.perf_counter() - tl > limit_s:
+            raise RuntimeError("serving stalled: %d of %d results" % (got, want))
+    return got
+
+
+def serve_measure(dk, ix, qs, K, L, BW, batch_ids):
+    """Served QPS at saturation and p50/p99 latency under Poisson arrivals.
+
+    The server keeps as many queries resident as the device holds (one wave each) and
+    claims requests from the ring as waves free up."""
+    NQ = qs.shape[0]
+    out = {}
+    with dk.Server(ix, dk.Hybrid, max_k=K, max_L=L, beam_width=BW, capacity=1 << 15,
+                   slots=0) as sv:
+        out["slots"] = sv.slots
+        tags = np.arange(NQ, dtype=np.uint64)
+        # warm-up pass
+        arr = np.zeros(NQ)
+        lat = np.zeros(NQ)
+        t0 = time.perf_counter()
+        assert sv.submit_batch(qs[:2000], K, L, tags[:2000]) == 2000
+        serve_drain(sv, 2000, arr, t0, lat, None)
+        # saturation: all queries arrive at t=0
+        served_ids = np.zeros((NQ, K), np.uint32)
+        t0 = time.perf_counter()
+        assert sv.submit_batch(qs, K, L, tags) == NQ
+        serve_drain(sv, NQ, arr, t0, lat, served_ids)
+        wall = time.perf_counter() - t0
+        sat = NQ / wall
+        out["saturation_qps"] = sat
+        out["identical_to_batched"] = bool(np.array_equal(served_ids, batch_ids))
+        # open loop: Poisson arrivals at fractions of the saturation rate
+        rng = np.random.default_rng(0xD15C4002)
+        loads = []
+        nreq = 2 * NQ
+        for frac in (0.25, 0.5, 0.8):
+            rate = frac * sat
+            arrival = np.cumsum(rng.exponential(1e3 / rate, nreq))   # ms
+            lat = np.full(nreq, np.nan)
+            qidx = np.arange(nreq) % NQ
+            t0 = time.perf_counter()
+            i = 0
+            done = 0
+            while i < nreq:
+                now = (time.perf_counter() - t0) * 1e3
+                j = int(np.searchsorted(arrival, now, side="right"))
+                if j > i:
+                    i += sv.submit_batch(qs[qidx[i:j]], K, L, np.arange(i, j, dtype=np.uint64))
+                tg, _, _, _, _, _, _ = sv.poll(4096)
+                if len(tg):
+                    lat[tg.astype(np.int64)] = (time.perf_counter() - t0) * 1e3 - \
+                        arrival[tg.astype(np.int64)]
+                    done += len(tg)
+            done += serve_drain(sv, nreq - done, arrival, t0, lat, None)
+            wall = time.perf_counter() - t0
+            loads.append({"offered_qps": rate, "achieved_qps": nreq / wall,
+                          "p50_ms": float(np.percentile(lat, 50)),
+                          "p99_ms": float(np.percentile(lat, 99))})
+        out["poisson"] = loads
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +304,13 @@ def main():
     plain_qps, lat, plain_hs = plain
     ix.set_tuning(HSLOTS, ADC)
 
+    # served: the persistent request loop (continuous batching) on the same index
+    try:
+        served = serve_measure(dk, ix, qs, K, L_star, BW, ids)
+    except dk.ANNError as e:      # the serving loop refuses to start when switched off
+        served = {"skipped": str(e)}
+    log("served: %s" % json.dumps(served))
+
     # CPU baseline: the oracle (C restatement of the reference search) on this host
     from oracle import pyoracle as po
     lists, counts = ix.get_graph()
@@ -277,6 +360,9 @@ def main():
             "plain_p99_ms": float(np.percentile(lat, 99)),
             "plain_visited_lds_slots": plain_hs,
             "batched_over_plain": qps / plain_qps,
+            "served": served,
+            "served_over_plain": (served["saturation_qps"] / plain_qps
+                                  if "saturation_qps" in served else None),
             "gpu_cpu_parity_on_sample": parity,
             "build_seconds": t_build, "pq_train_encode_seconds": t_pq,
             "mean_cmps": float(cmps.mean()), "mean_hops": float(hops.mean()),

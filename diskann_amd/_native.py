@@ -19,8 +19,8 @@ EXPORTS = [
     "dkm_get_codes", "dkm_search_batch", "dkm_search_batch_device", "dkm_index_synchronize",
     "dkm_profile", "dkm_profile_read", "dkm_set_tuning", "dkm_build", "dkm_medoid",
     "dkm_bruteforce", "dkm_search_batch_device_stats", "dkm_autotune",
-    "dkm_server_start", "dkm_server_submit", "dkm_server_poll", "dkm_server_pending",
-    "dkm_server_stop",
+    "dkm_server_start", "dkm_server_slots", "dkm_server_submit",
+    "dkm_server_submit_batch", "dkm_server_poll", "dkm_server_pending", "dkm_server_stop",
 ]
 
 _lib = None
@@ -76,7 +76,9 @@ def _declare(L):
         "dkm_bruteforce": [P, P, I64, I64, P, P],
         "dkm_autotune": [P, I, P, I64, U32, U32, U32, C.POINTER(C.c_int64), C.POINTER(C.c_int)],
         "dkm_server_start": [P, I, U32, U32, U32, I64, I64, C.POINTER(C.c_void_p)],
+        "dkm_server_slots": [P, C.POINTER(C.c_int64)],
         "dkm_server_submit": [P, P, U32, U32, C.c_uint64],
+        "dkm_server_submit_batch": [P, P, I64, P, P, P, C.POINTER(C.c_int64)],
         "dkm_server_poll": [P, I64, P, P, P, P, P, P, P, C.POINTER(C.c_int64)],
         "dkm_server_pending": [P, C.POINTER(C.c_int64)],
         "dkm_server_stop": [P],

@@ -949,87 +949,160 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
 }
 
 // ------------------------------------------------------------------------------------
-// Serving
+// Serving: continuous batching over the batch search kernel
 // ------------------------------------------------------------------------------------
 
 }  // extern "C"
 
-struct dkm_server {
-    dkm_index *ix = nullptr;
+namespace {
+
+constexpr int kServeStreams = 4;
+
+// One in-flight launch: requests [seq0, seq0 + n) of the ring.
+struct ServeLane {
     hipStream_t stream = nullptr;
-    std::mutex mu;
-    int strategy = 0;
-    uint32_t max_k = 0, max_L = 0, bw = 1;
-    int64_t capacity = 0, slots = 0;
-    dkm::ServeCtrl *ctrl = nullptr;  // pinned host memory
-    dkm::ServeReq *req = nullptr;
-    float *queries = nullptr;
-    uint32_t *res_ids = nullptr;
-    float *res_d = nullptr;
-    uint32_t *res_meta = nullptr;
-    uint64_t *claim_seq = nullptr;
-    uint64_t *done_seq = nullptr;
-    dkm::ServeDev *d_dev = nullptr;  // device memory
-    uint32_t *d_slot_status = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    uint64_t seq0 = 0;
+    int64_t n = 0;
+    uint32_t epoch = 0;
+    float *d_q = nullptr;  // device buffers sized for one launch
+    uint32_t *d_k = nullptr, *d_L = nullptr, *d_ids = nullptr, *d_cnt = nullptr;
+    uint32_t *d_cmps = nullptr, *d_hops = nullptr, *d_status = nullptr;
+    float *d_d = nullptr;
     uint64_t *d_spill = nullptr;
-    dkm::SearchParams p{};
-    dkm::ServeParams sp{};
-    uint64_t tail = 0;
-    std::vector<uint8_t> busy;
-    std::vector<double> t_submit;
-    std::vector<uint64_t> open;
 };
 
-static double now_us() {
+double now_us() {
     return std::chrono::duration<double, std::micro>(
                std::chrono::steady_clock::now().time_since_epoch())
         .count();
 }
 
 template <typename T>
-static hipError_t pinned(T **host, size_t count) {
+hipError_t pinned(T **host, size_t count) {
     void *h = nullptr;
-    hipError_t e = hipHostMalloc(&h, sizeof(T) * std::max<size_t>(count, 1),
-                                 hipHostMallocCoherent | hipHostMallocMapped);
+    hipError_t e = hipHostMalloc(&h, sizeof(T) * std::max<size_t>(count, 1), hipHostMallocDefault);
     if (e != hipSuccess) return e;
     memset(h, 0, sizeof(T) * std::max<size_t>(count, 1));
     *host = (T *)h;
     return hipSuccess;
 }
 
+// Copies ring entries [seq0, seq0 + n) (at most two contiguous pieces) host -> device or
+// device -> host; `elems` values of `T` per entry.
 template <typename T>
-static T *devptr(T *host) {
-    void *d = nullptr;
-    if (hipHostGetDevicePointer(&d, host, 0) != hipSuccess) return host;
-    return (T *)d;
+hipError_t ring_copy(T *ring, T *dev, uint64_t seq0, int64_t n, int64_t elems,
+                            int64_t capacity, bool to_device, hipStream_t st) {
+    const int64_t a = (int64_t)(seq0 % (uint64_t)capacity);
+    const int64_t n1 = std::min(n, capacity - a);
+    const size_t b1 = sizeof(T) * elems * n1, b2 = sizeof(T) * elems * (n - n1);
+    hipError_t e = to_device ? hipMemcpyAsync(dev, ring + a * elems, b1, hipMemcpyHostToDevice, st)
+                             : hipMemcpyAsync(ring + a * elems, dev, b1, hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess || n == n1) return e;
+    return to_device ? hipMemcpyAsync(dev + n1 * elems, ring, b2, hipMemcpyHostToDevice, st)
+                     : hipMemcpyAsync(ring, dev + n1 * elems, b2, hipMemcpyDeviceToHost, st);
 }
+
+}  // namespace
+
+struct dkm_server {
+    dkm_index *ix = nullptr;
+    std::mutex mu;
+    uint32_t max_k = 0, max_L = 0;
+    int64_t capacity = 0, slots = 0, per_launch = 0;
+    dkm::SearchParams base{};  // everything but the per-launch fields
+    // ring in pinned host memory, indexed by seq % capacity
+    float *queries = nullptr;
+    uint32_t *req_k = nullptr, *req_L = nullptr;
+    uint32_t *res_ids = nullptr, *res_cnt = nullptr, *res_cmps = nullptr, *res_hops = nullptr,
+             *res_status = nullptr;
+    float *res_d = nullptr;
+    std::vector<uint64_t> tags;
+    std::vector<double> t_submit;
+    std::vector<uint8_t> busy;      // entry holds a request not yet returned by poll
+    std::vector<uint64_t> finished; // seqs whose results are in the ring, not yet returned
+    uint64_t tail = 0;              // next seq to assign
+    uint64_t launched = 0;          // next seq to launch
+    int64_t inflight = 0;           // requests launched and not finished
+    ServeLane lane[kServeStreams];
+};
 
 extern "C" {
 
 static void server_free(dkm_server *sv) {
     if (!sv) return;
-    void *hp[] = {sv->ctrl,  sv->req,       sv->queries, sv->res_ids,
-                  sv->res_d, sv->res_meta, sv->claim_seq, sv->done_seq};
+    for (ServeLane &l : sv->lane) {
+        if (l.stream) (void)hipStreamSynchronize(l.stream);
+        void *dp[] = {l.d_q, l.d_k, l.d_L, l.d_ids, l.d_cnt, l.d_cmps, l.d_hops, l.d_status, l.d_d,
+                      l.d_spill};
+        for (void *q : dp)
+            if (q) (void)hipFree(q);
+        if (l.done) (void)hipEventDestroy(l.done);
+        if (l.stream) (void)hipStreamDestroy(l.stream);
+    }
+    void *hp[] = {sv->queries, sv->req_k,    sv->req_L,    sv->res_ids,   sv->res_cnt,
+                  sv->res_cmps, sv->res_hops, sv->res_status, sv->res_d};
     for (void *q : hp)
         if (q) (void)hipHostFree(q);
-    void *dp[] = {sv->d_dev, sv->d_slot_status, sv->d_spill};
-    for (void *q : dp)
-        if (q) (void)hipFree(q);
-    if (sv->stream) (void)hipStreamDestroy(sv->stream);
     delete sv;
 }
 
-// (Re)launch the persistent loop if requests wait unclaimed and it is not running.
-static dkm_status server_kick(dkm_server *sv) {
-    // claims are taken in sequence order, so the newest request tells whether any waits
-    if (sv->tail == 0) return DKM_OK;
-    const uint64_t last = sv->tail - 1;
-    if (__atomic_load_n(&sv->claim_seq[last % (uint64_t)sv->capacity], __ATOMIC_ACQUIRE) == last + 1)
-        return DKM_OK;
-    hipError_t q = hipStreamQuery(sv->stream);
-    if (q == hipErrorNotReady) return DKM_OK;
-    if (q != hipSuccess) return fail(DKM_ERR_DEVICE, "serving stream: %s", hipGetErrorString(q));
-    HIP_TRY(dkm::launch_serve(sv->p, sv->sp, (int)sv->slots, sv->stream));
+// Collects finished launches, then launches waiting requests on free streams while the
+// in-flight count stays within `slots`. Caller holds sv->mu.
+static dkm_status server_pump(dkm_server *sv) {
+    for (ServeLane &l : sv->lane) {
+        if (!l.busy) continue;
+        hipError_t q = hipEventQuery(l.done);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) return fail(DKM_ERR_DEVICE, "serving launch: %s", hipGetErrorString(q));
+        for (int64_t i = 0; i < l.n; ++i) sv->finished.push_back(l.seq0 + (uint64_t)i);
+        sv->inflight -= l.n;
+        l.busy = false;
+    }
+    for (ServeLane &l : sv->lane) {
+        if (sv->launched >= sv->tail || sv->inflight >= sv->slots) break;
+        if (l.busy) continue;
+        const int64_t n = std::min<int64_t>(
+            {(int64_t)(sv->tail - sv->launched), sv->per_launch, sv->slots - sv->inflight});
+        const uint64_t seq0 = sv->launched;
+        const int64_t dim = sv->ix->dim, cap = sv->capacity;
+        HIP_TRY(ring_copy(sv->queries, l.d_q, seq0, n, dim, cap, true, l.stream));
+        HIP_TRY(ring_copy(sv->req_k, l.d_k, seq0, n, 1, cap, true, l.stream));
+        HIP_TRY(ring_copy(sv->req_L, l.d_L, seq0, n, 1, cap, true, l.stream));
+        if (++l.epoch == 0) {  // wrapped: stale tags could collide, clear the table
+            HIP_TRY(hipMemsetAsync(l.d_spill, 0,
+                                   sizeof(uint64_t) * sv->ix->spill_slots * sv->per_launch, l.stream));
+            l.epoch = 1;
+        }
+        dkm::SearchParams p = sv->base;
+        p.queries = l.d_q;
+        p.nq = (int)n;
+        p.k = l.d_k;
+        p.L = l.d_L;
+        p.out_ids = l.d_ids;
+        p.out_d = l.d_d;
+        p.out_count = l.d_cnt;
+        p.out_cmps = l.d_cmps;
+        p.out_hops = l.d_hops;
+        p.status = l.d_status;
+        p.spill_table = l.d_spill;
+        p.epoch = l.epoch;
+        hipError_t e = dkm::launch_search(p, l.stream);
+        if (e != hipSuccess) return fail(DKM_ERR_DEVICE, "serving launch: %s", hipGetErrorString(e));
+        HIP_TRY(ring_copy(sv->res_ids, l.d_ids, seq0, n, sv->max_k, cap, false, l.stream));
+        HIP_TRY(ring_copy(sv->res_d, l.d_d, seq0, n, sv->max_k, cap, false, l.stream));
+        HIP_TRY(ring_copy(sv->res_cnt, l.d_cnt, seq0, n, 1, cap, false, l.stream));
+        HIP_TRY(ring_copy(sv->res_cmps, l.d_cmps, seq0, n, 1, cap, false, l.stream));
+        HIP_TRY(ring_copy(sv->res_hops, l.d_hops, seq0, n, 1, cap, false, l.stream));
+        HIP_TRY(ring_copy(sv->res_status, l.d_status, seq0, n, 1, cap, false, l.stream));
+        HIP_TRY(hipEventRecord(l.done, l.stream));
+        l.busy = true;
+        l.seq0 = seq0;
+        l.n = n;
+        sv->launched += (uint64_t)n;
+        sv->inflight += n;
+    }
     return DKM_OK;
 }
 
@@ -1041,50 +1114,23 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
     if (strategy < 0 || strategy > 2) return fail(DKM_ERR_INVALID_ARG, "unknown strategy %d", strategy);
     if (max_k == 0 || max_L == 0 || max_k > max_L || max_L > 4096)
         return fail(DKM_ERR_INVALID_ARG, "bad max_k / max_L");
-    if (beam_width == 0 || beam_width > 64) return fail(DKM_ERR_BEAM_WIDTH_ZERO, "bad beam width");
-    if (capacity < 1 || slots < 1 || slots > 65536) return fail(DKM_ERR_INVALID_ARG, "bad capacity / slots");
+    if (beam_width == 0) return fail(DKM_ERR_BEAM_WIDTH_ZERO, "beam width must be >= 1");
+    if (beam_width > 64) return fail(DKM_ERR_INVALID_ARG, "beam width above 64 is not supported");
+    if (capacity < 1 || capacity > (int64_t(1) << 24) || slots < 0 || slots > (int64_t(1) << 20))
+        return fail(DKM_ERR_INVALID_ARG, "bad capacity / slots");
     if (!ix->starts_set) return fail(DKM_ERR_NOT_READY, "start points not set");
     const bool pq = strategy != DKM_STRATEGY_FULL;
     if (pq && (!ix->d_codes || !ix->codes_set))
         return fail(DKM_ERR_NOT_READY, "PQ strategy needs a PQ schema and codes");
     if (pq && ix->metric == DKM_COSINE) return fail(DKM_ERR_PQ, "cosine PQ traversal is not supported");
+    std::lock_guard<std::mutex> ig(ix->mu);
     HIP_TRY(hipSetDevice(ix->device));
-    dkm_server *sv = new dkm_server();
-    sv->ix = ix;
-    sv->strategy = strategy;
-    sv->max_k = max_k;
-    sv->max_L = max_L;
-    sv->bw = beam_width;
-    sv->capacity = capacity;
-    sv->slots = slots;
-    hipError_t e = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = pinned(&sv->ctrl, 1);
-    if (e == hipSuccess) e = pinned(&sv->req, capacity);
-    if (e == hipSuccess) e = pinned(&sv->queries, (size_t)capacity * ix->dim);
-    if (e == hipSuccess) e = pinned(&sv->res_ids, (size_t)capacity * max_k);
-    if (e == hipSuccess) e = pinned(&sv->res_d, (size_t)capacity * max_k);
-    if (e == hipSuccess) e = pinned(&sv->res_meta, (size_t)capacity * 4);
-    if (e == hipSuccess) e = pinned(&sv->claim_seq, capacity);
-    if (e == hipSuccess) e = pinned(&sv->done_seq, capacity);
-    if (e == hipSuccess) e = hipMalloc(&sv->d_dev, sizeof(dkm::ServeDev));
-    if (e == hipSuccess) e = hipMemset(sv->d_dev, 0, sizeof(dkm::ServeDev));
-    if (e == hipSuccess) e = hipMalloc(&sv->d_slot_status, sizeof(uint32_t) * slots);
-    if (e == hipSuccess)
-        e = hipMalloc(&sv->d_spill, sizeof(uint64_t) * ix->spill_slots * (size_t)slots);
-    if (e == hipSuccess)
-        e = hipMemset(sv->d_spill, 0, sizeof(uint64_t) * ix->spill_slots * (size_t)slots);
-    if (e != hipSuccess) {
-        server_free(sv);
-        return fail(e == hipErrorOutOfMemory ? DKM_ERR_OUT_OF_MEMORY : DKM_ERR_DEVICE,
-                    "server allocation: %s", hipGetErrorString(e));
-    }
-    sv->busy.assign(capacity, 0);
-    sv->t_submit.assign(capacity, 0.0);
-    dkm::SearchParams &p = sv->p;
+    HIP_TRY(hipStreamSynchronize(ix->stream));  // index contents complete before serving
+    dkm::SearchParams p{};
     p.metric = ix->metric;
     p.pq = pq;
     p.rerank = strategy == DKM_STRATEGY_HYBRID;
-    p.adc_mode = 1;  // the serving kernel always uses the table-free ADC
+    p.adc_mode = ix->tune_adc_mode;
     p.pq_len = pq ? ix->pq_len : 0;
     p.dim = (int)ix->dim;
     p.vstride = ix->vstride;
@@ -1098,112 +1144,188 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
     p.M = (int)ix->M;
     p.pivots = ix->d_pivots;
     p.offsets = ix->d_offsets;
+    p.qstride = ix->dim;
     p.beam_width = beam_width;
     p.k_stride = max_k;
     p.qcap = (int)(max_L + ix->num_start);
     p.nlcap = (int)(beam_width * ix->R_max + 64);
     p.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots : default_visited_slots(max_L);
-    p.spill_table = sv->d_spill;
     p.spill_slots = ix->spill_slots;
     p.max_iters = (uint32_t)std::min<int64_t>(ix->rows + 16, 0x7FFFFFFF);
-    if (dkm::search_lds_bytes(p) > 160 * 1024) {
-        server_free(sv);
+    if (dkm::search_lds_bytes(p) > 160 * 1024)
         return fail(DKM_ERR_CAPACITY, "per-query LDS for max_L %u exceeds 160 KiB", max_L);
+    if (slots == 0) {
+        // two device-fulls in flight: queued blocks refill slots as earlier queries finish
+        int resident = 0;
+        HIP_TRY(dkm::search_resident_blocks(p, &resident));
+        if (resident < 1) return fail(DKM_ERR_CAPACITY, "search kernel does not fit on the device");
+        slots = 2 * (int64_t)resident;
     }
-    dkm::ServeParams &q = sv->sp;
-    q.ctrl = devptr(sv->ctrl);
-    q.dev = sv->d_dev;
-    q.req = devptr(sv->req);
-    q.queries = devptr(sv->queries);
-    q.res_ids = devptr(sv->res_ids);
-    q.res_d = devptr(sv->res_d);
-    q.res_meta = devptr(sv->res_meta);
-    q.claim_seq = devptr(sv->claim_seq);
-    q.done_seq = devptr(sv->done_seq);
-    q.capacity = (uint64_t)capacity;
-    q.max_k = max_k;
-    q.idle_ticks = 2000000;   // 20 ms at 100 MHz
-    q.run_ticks = 500000000;  // 5 s per launch, then the host relaunches
-    q.slot_status = sv->d_slot_status;
+    dkm_server *sv = new dkm_server();
+    sv->ix = ix;
+    sv->max_k = max_k;
+    sv->max_L = max_L;
+    sv->capacity = capacity;
+    sv->slots = slots;
+    sv->per_launch = std::max<int64_t>(1, (slots + kServeStreams - 1) / kServeStreams);
+    sv->base = p;
+    const int64_t P = sv->per_launch;
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = pinned(&sv->queries, (size_t)capacity * ix->dim);
+    if (e == hipSuccess) e = pinned(&sv->req_k, capacity);
+    if (e == hipSuccess) e = pinned(&sv->req_L, capacity);
+    if (e == hipSuccess) e = pinned(&sv->res_ids, (size_t)capacity * max_k);
+    if (e == hipSuccess) e = pinned(&sv->res_d, (size_t)capacity * max_k);
+    if (e == hipSuccess) e = pinned(&sv->res_cnt, capacity);
+    if (e == hipSuccess) e = pinned(&sv->res_cmps, capacity);
+    if (e == hipSuccess) e = pinned(&sv->res_hops, capacity);
+    if (e == hipSuccess) e = pinned(&sv->res_status, capacity);
+    for (ServeLane &l : sv->lane) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&l.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipMalloc(&l.d_q, sizeof(float) * P * ix->dim);
+        if (e == hipSuccess) e = hipMalloc(&l.d_k, sizeof(uint32_t) * P);
+        if (e == hipSuccess) e = hipMalloc(&l.d_L, sizeof(uint32_t) * P);
+        if (e == hipSuccess) e = hipMalloc(&l.d_ids, sizeof(uint32_t) * P * max_k);
+        if (e == hipSuccess) e = hipMalloc(&l.d_d, sizeof(float) * P * max_k);
+        if (e == hipSuccess) e = hipMalloc(&l.d_cnt, sizeof(uint32_t) * P);
+        if (e == hipSuccess) e = hipMalloc(&l.d_cmps, sizeof(uint32_t) * P);
+        if (e == hipSuccess) e = hipMalloc(&l.d_hops, sizeof(uint32_t) * P);
+        if (e == hipSuccess) e = hipMalloc(&l.d_status, sizeof(uint32_t) * P);
+        if (e == hipSuccess) e = hipMalloc(&l.d_spill, sizeof(uint64_t) * ix->spill_slots * P);
+        if (e == hipSuccess)
+            e = hipMemsetAsync(l.d_spill, 0, sizeof(uint64_t) * ix->spill_slots * P, l.stream);
+    }
+    for (ServeLane &l : sv->lane)
+        if (e == hipSuccess && l.stream) e = hipStreamSynchronize(l.stream);
+    if (e != hipSuccess) {
+        server_free(sv);
+        return fail(e == hipErrorOutOfMemory ? DKM_ERR_OUT_OF_MEMORY : DKM_ERR_DEVICE,
+                    "server allocation: %s", hipGetErrorString(e));
+    }
+    sv->tags.assign(capacity, 0);
+    sv->t_submit.assign(capacity, 0.0);
+    sv->busy.assign(capacity, 0);
     *out = sv;
+    return DKM_OK;
+}
+
+dkm_status dkm_server_slots(dkm_server *sv, int64_t *slots) {
+    if (!sv || !slots) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    *slots = sv->slots;
+    return DKM_OK;
+}
+
+// Places one validated request in its ring entry; the caller holds sv->mu.
+static bool server_put(dkm_server *sv, const float *query, uint32_t k, uint32_t L, uint64_t tag,
+                       double t) {
+    const uint64_t seq = sv->tail;
+    const uint64_t slot = seq % (uint64_t)sv->capacity;
+    if (sv->busy[slot]) return false;
+    sv->req_k[slot] = k;
+    sv->req_L[slot] = L;
+    sv->tags[slot] = tag;
+    memcpy(sv->queries + slot * sv->ix->dim, query, sizeof(float) * sv->ix->dim);
+    sv->busy[slot] = 1;
+    sv->t_submit[slot] = t;
+    sv->tail = seq + 1;
+    return true;
+}
+
+static dkm_status server_check(const dkm_server *sv, uint32_t k, uint32_t L) {
+    if (k == 0) return fail(DKM_ERR_K_ZERO, "k must be >= 1");
+    if (L == 0) return fail(DKM_ERR_L_ZERO, "L must be >= 1");
+    if (k > L) return fail(DKM_ERR_L_LESS_THAN_K, "L (%u) < k (%u)", L, k);
+    if (k > sv->max_k || L > sv->max_L)
+        return fail(DKM_ERR_INVALID_ARG, "k or L above the server's maximum");
     return DKM_OK;
 }
 
 dkm_status dkm_server_submit(dkm_server *sv, const float *query, uint32_t k, uint32_t L,
                              uint64_t tag) {
     if (!sv || !query) return fail(DKM_ERR_INVALID_ARG, "null argument");
-    if (k == 0) return fail(DKM_ERR_K_ZERO, "k must be >= 1");
-    if (L == 0) return fail(DKM_ERR_L_ZERO, "L must be >= 1");
-    if (k > L) return fail(DKM_ERR_L_LESS_THAN_K, "L (%u) < k (%u)", L, k);
-    if (k > sv->max_k || L > sv->max_L)
-        return fail(DKM_ERR_INVALID_ARG, "k or L above the server's maximum");
+    dkm_status st = server_check(sv, k, L);
+    if (st != DKM_OK) return st;
     std::lock_guard<std::mutex> g(sv->mu);
     HIP_TRY(hipSetDevice(sv->ix->device));
-    const uint64_t seq = sv->tail;
-    const uint64_t slot = seq % (uint64_t)sv->capacity;
-    if (sv->busy[slot]) return fail(DKM_ERR_CAPACITY, "request ring full");
-    if ((uint32_t)(seq + 1) == 0) return fail(DKM_ERR_CAPACITY, "request sequence exhausted");
-    sv->req[slot].k = k;
-    sv->req[slot].L = L;
-    sv->req[slot].tag = tag;
-    memcpy(sv->queries + slot * sv->ix->dim, query, sizeof(float) * sv->ix->dim);
-    sv->busy[slot] = 1;
-    sv->t_submit[slot] = now_us();
-    sv->open.push_back(seq);
-    sv->tail = seq + 1;
-    __atomic_store_n(&sv->ctrl->tail, sv->tail, __ATOMIC_RELEASE);
-    return server_kick(sv);
+    if (!server_put(sv, query, k, L, tag, now_us())) return fail(DKM_ERR_CAPACITY, "request ring full");
+    return server_pump(sv);
+}
+
+dkm_status dkm_server_submit_batch(dkm_server *sv, const float *queries, int64_t n,
+                                   const uint32_t *k, const uint32_t *L, const uint64_t *tags,
+                                   int64_t *accepted) {
+    if (!sv || !accepted || (n > 0 && (!queries || !k || !L || !tags)))
+        return fail(DKM_ERR_INVALID_ARG, "null argument");
+    *accepted = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        dkm_status st = server_check(sv, k[i], L[i]);
+        if (st != DKM_OK) return st;
+    }
+    std::lock_guard<std::mutex> g(sv->mu);
+    HIP_TRY(hipSetDevice(sv->ix->device));
+    const double t = now_us();
+    int64_t i = 0;
+    for (; i < n; ++i)
+        if (!server_put(sv, queries + i * sv->ix->dim, k[i], L[i], tags[i], t)) break;
+    *accepted = i;
+    return server_pump(sv);
 }
 
 dkm_status dkm_server_poll(dkm_server *sv, int64_t max_results, uint64_t *tags, uint32_t *ids,
                            float *dists, uint32_t *counts, uint32_t *cmps, uint32_t *hops,
                            double *latency_us, int64_t *n_out) {
     if (!sv || !n_out) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    *n_out = 0;
     std::lock_guard<std::mutex> g(sv->mu);
     HIP_TRY(hipSetDevice(sv->ix->device));
-    int64_t n = 0;
+    dkm_status st = server_pump(sv);
+    if (st != DKM_OK) return st;
     const double t = now_us();
-    for (size_t i = 0; i < sv->open.size() && n < max_results;) {
-        const uint64_t seq = sv->open[i];
+    int64_t n = 0;
+    while (n < max_results && !sv->finished.empty()) {
+        const uint64_t seq = sv->finished.back();
+        sv->finished.pop_back();
         const uint64_t slot = seq % (uint64_t)sv->capacity;
-        if (__atomic_load_n(&sv->done_seq[slot], __ATOMIC_ACQUIRE) != seq + 1) {
-            ++i;
-            continue;
-        }
-        const uint32_t *meta = sv->res_meta + slot * 4;
-        if (meta[3] & dkm::ERR_VISITED_OVERFLOW)
-            return fail(DKM_ERR_CAPACITY, "request %llu: visited set overflowed", (unsigned long long)seq);
-        if (meta[3] & dkm::ERR_ITER_LIMIT)
+        sv->busy[slot] = 0;
+        const uint32_t status = sv->res_status[slot];
+        if (status) {
+            // the failing request is consumed; the n results before it are valid
+            *n_out = n;
+            if (status & dkm::ERR_VISITED_OVERFLOW)
+                return fail(DKM_ERR_CAPACITY, "request %llu: visited set overflowed",
+                            (unsigned long long)seq);
             return fail(DKM_ERR_INDEX, "request %llu: traversal exceeded the iteration bound",
                         (unsigned long long)seq);
-        if (tags) tags[n] = sv->req[slot].tag;
+        }
+        if (tags) tags[n] = sv->tags[slot];
         if (ids) memcpy(ids + n * sv->max_k, sv->res_ids + slot * sv->max_k, sizeof(uint32_t) * sv->max_k);
         if (dists) memcpy(dists + n * sv->max_k, sv->res_d + slot * sv->max_k, sizeof(float) * sv->max_k);
-        if (counts) counts[n] = meta[0];
-        if (cmps) cmps[n] = meta[1];
-        if (hops) hops[n] = meta[2];
+        if (counts) counts[n] = sv->res_cnt[slot];
+        if (cmps) cmps[n] = sv->res_cmps[slot];
+        if (hops) hops[n] = sv->res_hops[slot];
         if (latency_us) latency_us[n] = t - sv->t_submit[slot];
-        sv->busy[slot] = 0;
-        sv->open[i] = sv->open.back();
-        sv->open.pop_back();
         ++n;
     }
     *n_out = n;
-    return server_kick(sv);
+    return server_pump(sv);  // entries freed above may admit waiting requests
 }
 
 dkm_status dkm_server_pending(dkm_server *sv, int64_t *pending) {
     if (!sv || !pending) return fail(DKM_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> g(sv->mu);
-    *pending = (int64_t)sv->open.size();
+    *pending = (int64_t)(sv->tail - sv->launched) + sv->inflight + (int64_t)sv->finished.size();
     return DKM_OK;
 }
 
 dkm_status dkm_server_stop(dkm_server *sv) {
     if (!sv) return DKM_OK;
     (void)hipSetDevice(sv->ix->device);
-    __atomic_store_n(&sv->ctrl->stop, 1u, __ATOMIC_RELEASE);
-    hipError_t e = hipStreamSynchronize(sv->stream);
+    hipError_t e = hipSuccess;
+    for (ServeLane &l : sv->lane) {
+        hipError_t e1 = hipStreamSynchronize(l.stream);
+        if (e == hipSuccess) e = e1;
+    }
     server_free(sv);
     if (e != hipSuccess) return fail(DKM_ERR_DEVICE, "serving stream: %s", hipGetErrorString(e));
     return DKM_OK;

@@ -54,7 +54,17 @@ struct SearchParams {
 
 size_t search_lds_bytes(const SearchParams &p);
 hipError_t launch_search(const SearchParams &p, hipStream_t stream);
+// Search blocks (one query each) of this configuration resident on the current device.
+hipError_t search_resident_blocks(const SearchParams &p, int *blocks);
 
+}  // namespace dkm
+
+namespace dkm {
+
+// Serving: request ring and control block in pinned host memory (fine-grained), claim
+// counter in device memory. A request is published by storing tail = seq + 1 (release);
+// a wave claims seq by CAS on `next`, searches, writes its results into the ring entry
+// and stores done_seq = seq + 1 (release).
 }  // namespace dkm
 
 namespace dkm {
@@ -65,17 +75,18 @@ namespace dkm {
 // and stores done_seq = seq + 1 (release).
 struct ServeCtrl {
     uint64_t tail;     // published requests (host writes)
-    uint32_t stop;     // host asks idle waves to exit
+    uint32_t stop;     // host asks the waves to exit
     uint32_t pad;
 };
 
-// Device-side mirror of the control block: idle waves poll this (L2) instead of host
-// memory; one wave at a time (holder of `lock`) refreshes it from host memory.
+// Device-side mirror of the control block: idle waves poll this instead of host memory;
+// at most one wave per poll interval (the one that moves last_poll) reads host memory.
 struct ServeDev {
-    uint64_t next;  // claim counter
-    uint64_t tail;  // last tail read from host memory
+    uint64_t next;       // claim counter
+    uint64_t tail;       // largest tail read from host memory
+    uint64_t last_poll;  // wall clock (100 MHz) of the latest host read
     uint32_t stop;
-    uint32_t lock;
+    uint32_t pad;
 };
 
 struct ServeReq {
@@ -93,14 +104,20 @@ struct ServeParams {
     uint32_t *res_meta;         // pinned, capacity x 4: count, cmps, hops, status
     uint64_t *claim_seq;        // pinned, capacity: seq + 1 once claimed
     uint64_t *done_seq;         // pinned, capacity: seq + 1 once the result is written
+    uint32_t *stage_ids;        // device, 2 x grid x max_k: search output before copy-out
+    float *stage_d;
+    uint32_t *stage_meta;       // device, 2 x grid x 4
+    uint32_t *slot_status;      // device, 2 x grid
     uint64_t capacity;
-    uint32_t max_k;
-    uint64_t idle_ticks;        // exit after this many 100 MHz ticks without work
-    uint64_t run_ticks;         // no new claims once a block has run this long (100 MHz)
-    uint32_t *slot_status;      // device, per block
+    uint32_t max_k, max_L;
+    uint64_t run_ticks;         // a wave takes no new request after running this long
+    uint64_t poll_ticks;        // minimum interval between reads of the host's tail
+    uint32_t part;              // 0/1: which half of the per-wave buffers this launch uses
 };
 
 hipError_t launch_serve(const SearchParams &p, const ServeParams &sp, int blocks,
                         hipStream_t stream);
+// Serving blocks of this configuration that are resident on the current device at once.
+hipError_t serve_resident_blocks(const SearchParams &p, int *blocks);
 
 }  // namespace dkm

@@ -724,159 +724,61 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     search_query<METRIC, MODE, DIM, PQLEN>(p, s, j);
 }
 
-// Persistent serving kernel: each block (one wave) claims requests from the ring one at
-// a time and runs search_query on them, so requests that arrive while others are being
-// searched start as soon as any block frees up. Exit conditions every block reaches:
-// no work for idle_ticks, the stop flag while idle, or run_ticks elapsed before a claim.
+// resident == nullptr: launch one block per query; otherwise report in *resident how many
+// blocks of this configuration the current device holds at once.
 template <int METRIC, int MODE, int DIM, int PQLEN>
-__global__ __launch_bounds__(64) void serve_kernel(SearchParams p, ServeParams sp) {
-    extern __shared__ __align__(16) char smem[];
-    Lds s = carve(p, smem);
-    const int lane = threadIdx.x;
-    uint64_t *region = p.spill_table + (size_t)blockIdx.x * p.spill_slots;
-    const uint64_t started = wall_clock64();
-    uint64_t idle_start = started;
-    for (;;) {
-        // lane 0 claims the next published request or decides to exit
-        if (lane == 0) {
-            int64_t got = -1;
-            ServeDev *dv = sp.dev;
-            for (;;) {
-                const uint64_t now = wall_clock64();
-                if (now - started >= sp.run_ticks) break;
-                uint64_t t = __hip_atomic_load(&dv->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t n = __hip_atomic_load(&dv->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (n >= t && atomicCAS(&dv->lock, 0u, 1u) == 0u) {
-                    // this wave refreshes the mirror from host memory
-                    t = __hip_atomic_load(&sp.ctrl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-                    const uint32_t st = __hip_atomic_load(&sp.ctrl->stop, __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_SYSTEM);
-                    __hip_atomic_fetch_max(&dv->tail, t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                    if (st) __hip_atomic_store(&dv->stop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(&dv->lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                if (n < t) {
-                    if (atomicCAS((unsigned long long *)&dv->next, (unsigned long long)n,
-                                  (unsigned long long)(n + 1)) == n) {
-                        got = (int64_t)n;
-                        __hip_atomic_store(&sp.claim_seq[n % sp.capacity], n + 1, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_SYSTEM);
-                        break;
-                    }
-                    continue;
-                }
-                if (__hip_atomic_load(&dv->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                if (now - idle_start > sp.idle_ticks) break;
-                __builtin_amdgcn_s_sleep(32);
-            }
-            s.misc[2] = (int32_t)(got & 0x7FFFFFFF);
-            s.misc[3] = got < 0 ? 1 : 0;
-            s.misc[1] = (int32_t)((uint64_t)got >> 31);
-        }
-        __syncthreads();
-        if (s.misc[3]) break;
-        const uint64_t seq = ((uint64_t)(uint32_t)s.misc[1] << 31) | (uint32_t)s.misc[2];
-        __syncthreads();
-        const uint64_t slot = seq % sp.capacity;
-        const ServeReq rq = sp.req[slot];
-        QueryJob job;
-        job.query = sp.queries + slot * (uint64_t)p.dim;
-        job.L = (int)rq.L;
-        job.k = (int)rq.k;
-        job.err = sp.slot_status + blockIdx.x;
-        job.spill = region;
-        job.epoch = (uint32_t)(seq + 1);
-        job.rec_ids = nullptr;
-        job.rec_d = nullptr;
-        job.out_ids = sp.res_ids + slot * sp.max_k;
-        job.out_d = sp.res_d + slot * sp.max_k;
-        uint32_t *meta = sp.res_meta + slot * 4;
-        job.out_count = meta;
-        job.out_cmps = meta + 1;
-        job.out_hops = meta + 2;
-        job.rec_len = nullptr;
-        job.out_degsum = nullptr;
-        search_query<METRIC, MODE, DIM, PQLEN>(p, s, job);
-        __threadfence_system();
-        __syncthreads();
-        if (lane == 0) {
-            meta[3] = *job.err;
-            __threadfence_system();
-            __hip_atomic_store(&sp.done_seq[slot], seq + 1, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_SYSTEM);
-        }
-        idle_start = wall_clock64();
-    }
-}
-
-template <int METRIC, int MODE, int DIM, int PQLEN>
-static hipError_t launch_serve_t(const SearchParams &p, const ServeParams &sp, int blocks,
-                                 hipStream_t stream) {
-    size_t lds = lds_bytes(p);
-    auto kern = serve_kernel<METRIC, MODE, DIM, PQLEN>;
-    hipError_t e = hipFuncSetAttribute((const void *)kern,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64), lds, stream, p, sp);
-    return hipGetLastError();
-}
-
-hipError_t launch_serve(const SearchParams &p, const ServeParams &sp, int blocks,
-                        hipStream_t stream) {
-    if (p.pq) {
-        const bool ip = p.metric == METRIC_IP;
-        if (ip) return launch_serve_t<METRIC_IP, 2, 0, 0>(p, sp, blocks, stream);
-        if (p.dim == 128 && p.pq_len == 4) return launch_serve_t<METRIC_L2, 2, 128, 4>(p, sp, blocks, stream);
-        if (p.dim == 96 && p.pq_len == 2) return launch_serve_t<METRIC_L2, 2, 96, 2>(p, sp, blocks, stream);
-        return launch_serve_t<METRIC_L2, 2, 0, 0>(p, sp, blocks, stream);
-    }
-    switch (p.metric) {
-    case METRIC_L2:
-        if (p.dim == 128) return launch_serve_t<METRIC_L2, 0, 128, 0>(p, sp, blocks, stream);
-        if (p.dim == 96) return launch_serve_t<METRIC_L2, 0, 96, 0>(p, sp, blocks, stream);
-        return launch_serve_t<METRIC_L2, 0, 0, 0>(p, sp, blocks, stream);
-    case METRIC_IP: return launch_serve_t<METRIC_IP, 0, 0, 0>(p, sp, blocks, stream);
-    default: return launch_serve_t<METRIC_COSINE, 0, 0, 0>(p, sp, blocks, stream);
-    }
-}
-
-template <int METRIC, int MODE, int DIM, int PQLEN>
-static hipError_t launch_t(const SearchParams &p, hipStream_t stream) {
+static hipError_t launch_t(const SearchParams &p, hipStream_t stream, int *resident) {
     size_t lds = lds_bytes(p);
     auto kern = beam_search_kernel<METRIC, MODE, DIM, PQLEN>;
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
+    if (resident) {
+        int per_cu = 0, dev = 0, cus = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kern, 64, lds);
+        if (e == hipSuccess) e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        *resident = per_cu * cus;
+        return e;
+    }
     hipLaunchKernelGGL(kern, dim3((unsigned)p.nq), dim3(64), lds, stream, p);
     return hipGetLastError();
+}
+
+static hipError_t dispatch(const SearchParams &p, hipStream_t stream, int *resident) {
+    if (p.pq) {
+        const bool ip = p.metric == METRIC_IP;  // L2 and CosineNormalized use the L2 table
+        if (p.adc_mode == 0) {
+            if (ip) return launch_t<METRIC_IP, 1, 0, 0>(p, stream, resident);
+            if (p.dim == 128) return launch_t<METRIC_L2, 1, 128, 0>(p, stream, resident);
+            if (p.dim == 96) return launch_t<METRIC_L2, 1, 96, 0>(p, stream, resident);
+            return launch_t<METRIC_L2, 1, 0, 0>(p, stream, resident);
+        }
+        if (ip) return launch_t<METRIC_IP, 2, 0, 0>(p, stream, resident);
+        if (p.dim == 128 && p.pq_len == 4) return launch_t<METRIC_L2, 2, 128, 4>(p, stream, resident);
+        if (p.dim == 96 && p.pq_len == 2) return launch_t<METRIC_L2, 2, 96, 2>(p, stream, resident);
+        return launch_t<METRIC_L2, 2, 0, 0>(p, stream, resident);
+    }
+    switch (p.metric) {
+    case METRIC_L2:
+        if (p.dim == 128) return launch_t<METRIC_L2, 0, 128, 0>(p, stream, resident);
+        if (p.dim == 96) return launch_t<METRIC_L2, 0, 96, 0>(p, stream, resident);
+        return launch_t<METRIC_L2, 0, 0, 0>(p, stream, resident);
+    case METRIC_IP:
+        if (p.dim == 128) return launch_t<METRIC_IP, 0, 128, 0>(p, stream, resident);
+        return launch_t<METRIC_IP, 0, 0, 0>(p, stream, resident);
+    default: return launch_t<METRIC_COSINE, 0, 0, 0>(p, stream, resident);
+    }
 }
 
 size_t search_lds_bytes(const SearchParams &p) { return lds_bytes(p); }
 
 hipError_t launch_search(const SearchParams &p, hipStream_t stream) {
-    if (p.pq) {
-        const bool ip = p.metric == METRIC_IP;  // L2 and CosineNormalized use the L2 table
-        if (p.adc_mode == 0) {
-            if (ip) return launch_t<METRIC_IP, 1, 0, 0>(p, stream);
-            if (p.dim == 128) return launch_t<METRIC_L2, 1, 128, 0>(p, stream);
-            if (p.dim == 96) return launch_t<METRIC_L2, 1, 96, 0>(p, stream);
-            return launch_t<METRIC_L2, 1, 0, 0>(p, stream);
-        }
-        if (ip) return launch_t<METRIC_IP, 2, 0, 0>(p, stream);
-        if (p.dim == 128 && p.pq_len == 4) return launch_t<METRIC_L2, 2, 128, 4>(p, stream);
-        if (p.dim == 96 && p.pq_len == 2) return launch_t<METRIC_L2, 2, 96, 2>(p, stream);
-        return launch_t<METRIC_L2, 2, 0, 0>(p, stream);
-    }
-    switch (p.metric) {
-    case METRIC_L2:
-        if (p.dim == 128) return launch_t<METRIC_L2, 0, 128, 0>(p, stream);
-        if (p.dim == 96) return launch_t<METRIC_L2, 0, 96, 0>(p, stream);
-        return launch_t<METRIC_L2, 0, 0, 0>(p, stream);
-    case METRIC_IP:
-        if (p.dim == 128) return launch_t<METRIC_IP, 0, 128, 0>(p, stream);
-        return launch_t<METRIC_IP, 0, 0, 0>(p, stream);
-    default: return launch_t<METRIC_COSINE, 0, 0, 0>(p, stream);
-    }
+    return dispatch(p, stream, nullptr);
+}
+
+hipError_t search_resident_blocks(const SearchParams &p, int *blocks) {
+    return dispatch(p, nullptr, blocks);
 }
 
 }  // namespace dkm

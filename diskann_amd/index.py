@@ -220,17 +220,19 @@ class DiskANNIndex:
 
 
 class Server:
-    """Continuous-batching request loop over one index.
+    """Continuous batching over one index.
 
-    A persistent kernel keeps `slots` queries in flight; requests are placed in a ring of
-    `capacity` entries in pinned host memory and each is claimed by the first free wave,
-    so a request never waits for a batch to fill. Each result equals what `search_batch`
-    returns for that query alone (same k, L, beam width and strategy). Submitting to a
-    full ring raises CapacityError; `poll` returns finished requests in completion order.
+    Requests go into a ring of `capacity` entries in pinned host memory; every submit and
+    poll launches the requests waiting at that moment as one batch on one of four
+    streams, while at most `slots` requests are in flight (0: twice what the device holds
+    at once), so requests arriving while others run overlap with them on the device.
+    Each result equals what `search_batch` returns for that query alone (same k, L, beam
+    width and strategy). Submitting to a full ring raises CapacityError; `poll` returns
+    finished requests. Progress needs the caller to keep calling submit or poll.
     """
 
     def __init__(self, index, strategy=Strategy.HYBRID, max_k=10, max_L=512, beam_width=1,
-                 capacity=4096, slots=2048):
+                 capacity=4096, slots=0):
         self.index, self.lib = index, index.lib
         self.dim, self.max_k = index.dim, int(max_k)
         h = C.c_void_p()
@@ -238,6 +240,9 @@ class Server:
                                         int(beam_width), int(capacity), int(slots), C.byref(h)),
               self.lib)
         self._h = h
+        n = C.c_int64()
+        check(self.lib.dkm_server_slots(h, C.byref(n)), self.lib)
+        self.slots = n.value
 
     def submit(self, query, k, L, tag):
         q = np.ascontiguousarray(query, dtype=np.float32).reshape(-1)
@@ -245,22 +250,35 @@ class Server:
             raise ValueError("query dim %d != index dim %d" % (q.shape[0], self.dim))
         check(self.lib.dkm_server_submit(self._h, _p(q), int(k), int(L), int(tag)), self.lib)
 
+    def submit_batch(self, queries, k, L, tags):
+        """Submits in order until the ring is full; returns how many were accepted."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim != 2 or q.shape[1] != self.dim:
+            raise ValueError("queries must be (n, %d)" % self.dim)
+        n = q.shape[0]
+        ks = np.ascontiguousarray(np.broadcast_to(np.asarray(k, dtype=np.uint32), (n,)))
+        Ls = np.ascontiguousarray(np.broadcast_to(np.asarray(L, dtype=np.uint32), (n,)))
+        tg = np.ascontiguousarray(np.broadcast_to(np.asarray(tags, dtype=np.uint64), (n,)))
+        acc = C.c_int64()
+        check(self.lib.dkm_server_submit_batch(self._h, _p(q), n, _p(ks), _p(Ls), _p(tg),
+                                               C.byref(acc)), self.lib)
+        return acc.value
+
     def poll(self, max_results=4096):
         """Finished requests: (tags, ids[n, max_k], dists[n, max_k], counts, cmps, hops,
         latency_us); latency is measured from submit to the poll that returned it."""
         m = int(max_results)
-        tags = np.zeros(m, dtype=np.uint64)
-        ids = np.zeros((m, self.max_k), dtype=np.uint32)
-        ds = np.zeros((m, self.max_k), dtype=np.float32)
-        cnt = np.zeros(m, dtype=np.uint32)
-        cmps = np.zeros(m, dtype=np.uint32)
-        hops = np.zeros(m, dtype=np.uint32)
-        lat = np.zeros(m, dtype=np.float64)
+        if getattr(self, "_pbuf", None) is None or self._pbuf[0].shape[0] < m:
+            self._pbuf = (np.empty(m, np.uint64), np.empty((m, self.max_k), np.uint32),
+                          np.empty((m, self.max_k), np.float32), np.empty(m, np.uint32),
+                          np.empty(m, np.uint32), np.empty(m, np.uint32),
+                          np.empty(m, np.float64))
+        tags, ids, ds, cnt, cmps, hops, lat = self._pbuf
         n = C.c_int64()
         check(self.lib.dkm_server_poll(self._h, m, _p(tags), _p(ids), _p(ds), _p(cnt), _p(cmps),
                                        _p(hops), _p(lat), C.byref(n)), self.lib)
         k = n.value
-        return tags[:k], ids[:k], ds[:k], cnt[:k], cmps[:k], hops[:k], lat[:k]
+        return tuple(a[:k].copy() for a in self._pbuf)
 
     def pending(self):
         n = C.c_int64()

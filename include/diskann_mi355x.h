@@ -179,30 +179,50 @@ dkm_status dkm_search_batch_device_stats(dkm_index *ix, int strategy, const floa
                                          uint32_t *d_out_degsum);
 
 /* -------------------------------------------------------------------------------- */
-/* Serving: persistent request loop with continuous batching                        */
+/* Serving: continuous batching                                                        */
 /* -------------------------------------------------------------------------------- */
-/* A persistent kernel keeps `slots` queries resident (one wave each) and claims
- * requests from a ring of `capacity` entries in pinned host memory as soon as a wave
- * frees up, so requests that arrive while others run join the work already on the GPU.
- * Admission is by device memory: each slot reserves its visited-set region in HBM and
- * its LDS; requests beyond the free ring entries are refused with DKM_ERR_CAPACITY
- * (back-pressure). Requests may mix k and L (k <= max_k, k <= L <= max_L); results equal
- * dkm_search_batch for the same query, k, L and beam width. Idle waves exit after a
- * short timeout and the loop is relaunched on demand. Thread-safe per server. */
+/* The reference serves queries by calling DiskANNIndex::search concurrently from many
+ * async workers over one shared index (diskann-benchmark-core/src/search/api.rs:400-430,
+ * diskann/src/graph/index.rs:2034-2070); a server is the device-side form of that loop.
+ * Requests go into a ring of `capacity` entries in pinned host memory. Every submit and
+ * poll call collects finished launches and launches the requests waiting at that moment
+ * as one batch (uneven k and L in one launch) on one of four streams, as long as at most
+ * `slots` requests are in flight: requests that arrive while others run start in the
+ * next launch and overlap with them on the device, filling compute units as earlier
+ * queries finish. Admission is by device memory: each stream owns the visited-set
+ * regions (HBM) of slots / 4 requests, allocated at start; submits beyond the free ring
+ * entries are refused with DKM_ERR_CAPACITY (back-pressure). Requests need
+ * k <= max_k and k <= L <= max_L (L < k is rejected as Knn::new does,
+ * diskann/src/graph/search/knn_search.rs:101-106). Each result equals
+ * dkm_search_batch for the same query, k, L, beam width and strategy. slots = 0 picks
+ * twice the number of queries the device holds at once. Progress needs the caller to
+ * keep calling submit or poll. The index must not be modified while a server runs.
+ * Thread-safe per server. */
 typedef struct dkm_server dkm_server;
 dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_t max_L,
                             uint32_t beam_width, int64_t capacity, int64_t slots,
                             dkm_server **out);
+dkm_status dkm_server_slots(dkm_server *sv, int64_t *slots);
 dkm_status dkm_server_submit(dkm_server *sv, const float *query, uint32_t k, uint32_t L,
                              uint64_t tag);
+/* Submits requests in order until the ring is full: *accepted of the n were taken (the
+ * rest may be resubmitted later). All n are validated first; an invalid one rejects the
+ * whole call with its status. */
+dkm_status dkm_server_submit_batch(dkm_server *sv, const float *queries, int64_t n,
+                                   const uint32_t *k, const uint32_t *L, const uint64_t *tags,
+                                   int64_t *accepted);
 /* Returns up to max_results completed requests (any order): tag, max_k ids / distances
  * per result (counts[i] valid), comparisons, hops and the request's latency in
- * microseconds from submit to the poll that saw it complete. */
+ * microseconds from submit to the poll that returned it. A request whose search failed
+ * (visited-set overflow, iteration bound) is consumed and its error returned; the
+ * *n_out results before it are valid. */
 dkm_status dkm_server_poll(dkm_server *sv, int64_t max_results, uint64_t *tags,
                            uint32_t *ids, float *dists, uint32_t *counts, uint32_t *cmps,
                            uint32_t *hops, double *latency_us, int64_t *n_out);
 /* Requests submitted and not yet returned by dkm_server_poll. */
 dkm_status dkm_server_pending(dkm_server *sv, int64_t *pending);
+/* Waits for the launches in flight, then releases the server and its memory; results
+ * not yet returned by dkm_server_poll are dropped. */
 dkm_status dkm_server_stop(dkm_server *sv);
 
 /* Tuning knobs (autotuner surface; none changes results): visited-set slots in LDS per

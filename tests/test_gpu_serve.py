@@ -58,8 +58,8 @@ def test_served_equals_oracle_mixed_k_L(strategy):
         assert (g[2], g[3]) == (cm, hp), i
 
 
-def test_served_equals_batch_across_relaunch():
-    # requests separated by more than the idle timeout: the loop exits and is relaunched
+def test_served_equals_batch_across_idle_gaps():
+    # bursts separated by idle time: every burst is launched and served the same way
     vec, adj, deg, qs = dataset(po.L2)
     ix = make_index(po.L2, vec, adj, deg)
     ref = ix.search_batch(qs, 10, 64, 1, dk.FullPrecision)
@@ -74,6 +74,34 @@ def test_served_equals_batch_across_relaunch():
                 assert (g[2], g[3]) == (ref.cmps[i], ref.hops[i])
                 assert g[4] > 0
             time.sleep(0.1)
+
+
+def test_served_equals_batch_under_continuous_traffic():
+    # small bursts with polls in between keep several launches in flight on different
+    # streams (slots 16 -> 4 per launch); every result must equal the batched search
+    log("serve continuous traffic")
+    vec, adj, deg, qs = dataset(po.L2)
+    ix = make_index(po.L2, vec, adj, deg)
+    ref = ix.search_batch(qs, 10, 64, 1, dk.FullPrecision)
+    nreq = 10 * qs.shape[0]
+    got = {}
+    with dk.Server(ix, dk.FullPrecision, max_k=10, max_L=64, capacity=64, slots=16) as sv:
+        i = 0
+        t0 = time.time()
+        while len(got) < nreq:
+            if i < nreq:
+                i += sv.submit_batch(qs[[j % qs.shape[0] for j in range(i, min(nreq, i + 8))]],
+                                     10, 64, np.arange(i, min(nreq, i + 8)))
+            tags, ids, ds, cnt, cmps, hops, lat = sv.poll()
+            for j in range(len(tags)):
+                got[int(tags[j])] = (ids[j, :cnt[j]].copy(), ds[j, :cnt[j]].copy(), int(cmps[j]),
+                                     int(hops[j]))
+            assert time.time() - t0 < 60, "served %d of %d" % (len(got), nreq)
+    for tag, g in got.items():
+        q = tag % qs.shape[0]
+        assert list(g[0]) == list(ref.ids[q, :ref.counts[q]]), tag
+        assert g[1].tobytes() == ref.distances[q, :ref.counts[q]].tobytes(), tag
+        assert (g[2], g[3]) == (ref.cmps[q], ref.hops[q]), tag
 
 
 def test_server_admission_and_validation():
@@ -96,3 +124,20 @@ def test_server_admission_and_validation():
         assert len(drain(sv, 1)) == 1
     with pytest.raises(dk.ANNError):
         dk.Server(ix, dk.Hybrid, max_k=10, max_L=64)    # no PQ codes on this index
+
+
+def test_submit_batch_with_device_sized_slots():
+    vec, adj, deg, qs = dataset(po.L2)
+    ix = make_index(po.L2, vec, adj, deg)
+    ref = ix.search_batch(qs, 10, 48, 1, dk.FullPrecision)
+    with dk.Server(ix, dk.FullPrecision, max_k=10, max_L=48, capacity=32, slots=0) as sv:
+        assert sv.slots >= 256
+        acc = sv.submit_batch(qs, 10, 48, np.arange(qs.shape[0]))
+        assert acc == 32                        # ring of 32 entries
+        got = drain(sv, 32)
+        assert sv.submit_batch(qs[32:], 10, 48, np.arange(32, qs.shape[0])) == 32
+        got.update(drain(sv, 32))
+    assert sorted(got) == list(range(qs.shape[0]))
+    for i, g in got.items():
+        assert list(g[0]) == list(ref.ids[i, :ref.counts[i]])
+        assert (g[2], g[3]) == (ref.cmps[i], ref.hops[i])

@@ -8,10 +8,11 @@
 // (diskann/src/graph/glue.rs:730-821, full_precision.rs:357-404).
 //
 // Per-query state lives in LDS:
-//   query vector, PQ lookup table (M x 256 f32), candidate queue (double-buffered,
-//   capacity L + #start points, visited flag in bit 31 of the id), expansion list, and
-//   an open-addressing visited set that spills to a per-query region of a global
-//   (HBM) table when it fills.
+//   query vector, PQ lookup table (M x 256 f32), candidate queue (capacity L + #start
+//   points, visited flag in bit 31 of the id, merged in place), expansion list (ranked
+//   in place), and an open-addressing visited set that spills to a per-query region of
+//   a global (HBM) table when it fills. Keeping this small keeps more queries resident
+//   per CU, which is what hides the latency of the gathers.
 // Queue merge: the sequential "insert before equal distances, drop the tail" rule is
 // the top-C of (queue U new) under the key (distance asc, insertion order desc), so a
 // hop's expansion list is ranked in parallel and merged with two binary searches.
@@ -23,17 +24,30 @@ namespace dkm {
 struct Lds {
     float *q;        // dim
     float *lut;      // M*256 (PQ)
-    float *qa_d, *qb_d;
-    uint32_t *qa_i, *qb_i;
-    float *nl_d, *sn_d;
+    float *qa_d;
+    uint32_t *qa_i;
+    float *nl_d, *sn_d;     // sn aliases nl unless the expansion list can exceed 256
     uint32_t *nl_i, *sn_i;
     uint32_t *hash;  // H
     int32_t *offs;   // M+1
     uint32_t *starts;
     uint32_t *beam;
-    uint32_t *rows;  // beam_width x gstride adjacency rows staged per iteration
+    uint32_t *rows;  // beam_width x gstride adjacency rows staged per iteration; the
+                     // merge reuses it for the new items' target positions
     int32_t *misc;
 };
+
+// Expansion lists longer than this are ranked through a separate sorted buffer.
+constexpr int kInPlaceRank = 4 * 64;
+
+__host__ __device__ inline bool separate_sorted(const SearchParams &p) {
+    return p.nlcap > kInPlaceRank;
+}
+
+__host__ __device__ inline int rows_entries(const SearchParams &p) {
+    const int r = (int)(p.beam_width * p.gstride);
+    return r > p.nlcap ? r : p.nlcap;
+}
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
@@ -41,13 +55,13 @@ __host__ __device__ inline size_t lds_bytes(const SearchParams &p) {
     size_t b = 0;
     b += align16(sizeof(float) * p.dim);
     if (p.pq && p.adc_mode == 0) b += align16(sizeof(float) * p.M * 256);
-    b += 4 * align16(sizeof(float) * p.qcap);            // qa_d qa_i qb_d qb_i
-    b += 4 * align16(sizeof(float) * p.nlcap);           // nl_d nl_i sn_d sn_i
+    b += 2 * align16(sizeof(float) * p.qcap);                               // qa_d qa_i
+    b += (separate_sorted(p) ? 4 : 2) * align16(sizeof(float) * p.nlcap);  // nl (sn)
     b += align16(sizeof(uint32_t) * p.hslots);
     if (p.pq) b += align16(sizeof(int32_t) * (p.M + 1));
     b += align16(sizeof(uint32_t) * p.nstart);
     b += align16(sizeof(uint32_t) * p.beam_width);
-    b += align16(sizeof(uint32_t) * p.beam_width * p.gstride);
+    b += align16(sizeof(uint32_t) * rows_entries(p));
     b += 16;
     return b;
 }
@@ -64,54 +78,29 @@ __device__ inline Lds carve(const SearchParams &p, char *base) {
     s.lut = (p.pq && p.adc_mode == 0) ? (float *)take(sizeof(float) * p.M * 256) : nullptr;
     s.qa_d = (float *)take(sizeof(float) * p.qcap);
     s.qa_i = (uint32_t *)take(sizeof(uint32_t) * p.qcap);
-    s.qb_d = (float *)take(sizeof(float) * p.qcap);
-    s.qb_i = (uint32_t *)take(sizeof(uint32_t) * p.qcap);
     s.nl_d = (float *)take(sizeof(float) * p.nlcap);
     s.nl_i = (uint32_t *)take(sizeof(uint32_t) * p.nlcap);
-    s.sn_d = (float *)take(sizeof(float) * p.nlcap);
-    s.sn_i = (uint32_t *)take(sizeof(uint32_t) * p.nlcap);
+    if (separate_sorted(p)) {
+        s.sn_d = (float *)take(sizeof(float) * p.nlcap);
+        s.sn_i = (uint32_t *)take(sizeof(uint32_t) * p.nlcap);
+    } else {
+        s.sn_d = s.nl_d;
+        s.sn_i = s.nl_i;
+    }
     s.hash = (uint32_t *)take(sizeof(uint32_t) * p.hslots);
     s.offs = p.pq ? (int32_t *)take(sizeof(int32_t) * (p.M + 1)) : nullptr;
     s.starts = (uint32_t *)take(sizeof(uint32_t) * p.nstart);
     s.beam = (uint32_t *)take(sizeof(uint32_t) * p.beam_width);
-    s.rows = (uint32_t *)take(sizeof(uint32_t) * p.beam_width * p.gstride);
+    s.rows = (uint32_t *)take(sizeof(uint32_t) * rows_entries(p));
     s.misc = (int32_t *)take(16);
     return s;
 }
 
-// Visited-set insert for one id per lane (ids within one call are distinct). Returns
-// true if the id was not present (and is now recorded).
-// The LDS table is an array of 4-slot buckets probed in sequence from a hashed start;
-// one 16-byte read checks a whole bucket, an id goes to the first free slot of the first
-// bucket with room, and since slots are never freed a lookup may stop at the first
-// bucket that still has a free slot. Once the table passes its fill threshold (decided
-// per chunk, uniformly) it is only read, and new ids go to this query's region of the
-// HBM spill table (64-bit entries tagged with the launch epoch, linear probing).
-__device__ inline bool visit_insert(const SearchParams &p, Lds &s, uint32_t id, bool active,
-                                    bool spill, uint64_t *gtab, uint32_t *err, uint32_t epoch) {
-    if (!active) return false;
-    const uint32_t HB = p.hslots >> 2;  // buckets
-    uint32_t b = (uint32_t)(((uint64_t)hash_id(id) * HB) >> 32);
-    for (uint32_t probe = 0; probe < HB;) {
-        const uint4 v = *(const uint4 *)&s.hash[4 * b];
-        if (v.x == id || v.y == id || v.z == id || v.w == id) return false;
-        const int free_slot = v.x == kEmpty ? 0 : v.y == kEmpty ? 1 : v.z == kEmpty ? 2
-                              : v.w == kEmpty ? 3 : -1;
-        if (free_slot < 0) {
-            b = b + 1 == HB ? 0 : b + 1;
-            ++probe;
-            continue;
-        }
-        if (spill) break;  // not present in LDS
-        const uint32_t old = atomicCAS(&s.hash[4 * b + free_slot], kEmpty, id);
-        if (old == kEmpty) return true;
-        if (old == id) return false;
-        // another lane took the slot: re-read the same bucket
-    }
-    if (!spill || gtab == nullptr) {
-        atomicOr(err, ERR_VISITED_OVERFLOW);
-        return false;
-    }
+// Insert-or-find in this query's region of the HBM visited table (64-bit entries tagged
+// with the launch epoch, linear probing; entries of other epochs read as empty). Returns
+// true if the id was not present.
+__device__ inline bool hbm_visit(const SearchParams &p, uint64_t *gtab, uint32_t id,
+                                 uint32_t *err, uint32_t epoch) {
     const uint64_t GH = p.spill_slots;
     const uint64_t key = ((uint64_t)epoch << 32) | id;
     uint64_t g = hash_id(id ^ 0x9e3779b9u) & (GH - 1);
@@ -131,18 +120,71 @@ __device__ inline bool visit_insert(const SearchParams &p, Lds &s, uint32_t id, 
     return false;
 }
 
+// Visited-set insert for one id per lane (ids within one call are distinct). Returns
+// true if the id was not present (and is now recorded).
+// Until the LDS table passes its fill threshold it is the whole set: an array of 4-slot
+// buckets probed in sequence from a hashed start; one 16-byte read checks a whole bucket,
+// an id goes to the first free slot of the first bucket with room, and since slots are
+// never freed a lookup may stop at the first bucket that still has a free slot. When it
+// passes the threshold (decided per chunk, uniformly) its ids are copied to the query's
+// HBM table (flush_visited), which from then on is the set; the LDS array becomes a
+// 4-way cache of recently seen ids (home bucket only, random replacement): a hit there
+// proves the id visited without an HBM round trip, a miss asks the HBM table.
+__device__ inline bool visit_insert(const SearchParams &p, Lds &s, uint32_t id, bool active,
+                                    bool spill, uint64_t *gtab, uint32_t *err, uint32_t epoch) {
+    if (!active) return false;
+    const uint32_t HB = p.hslots >> 2;  // buckets
+    uint32_t b = (uint32_t)(((uint64_t)hash_id(id) * HB) >> 32);
+    if (spill) {
+        const uint4 v = *(const uint4 *)&s.hash[4 * b];
+        if (v.x == id || v.y == id || v.z == id || v.w == id) return false;
+        const bool fresh = hbm_visit(p, gtab, id, err, epoch);
+        s.hash[4 * b + (hash_id(id ^ 0x5bd1e995u) >> 30)] = id;
+        return fresh;
+    }
+    for (uint32_t probe = 0; probe < HB;) {
+        const uint4 v = *(const uint4 *)&s.hash[4 * b];
+        if (v.x == id || v.y == id || v.z == id || v.w == id) return false;
+        const int free_slot = v.x == kEmpty ? 0 : v.y == kEmpty ? 1 : v.z == kEmpty ? 2
+                              : v.w == kEmpty ? 3 : -1;
+        if (free_slot < 0) {
+            b = b + 1 == HB ? 0 : b + 1;
+            ++probe;
+            continue;
+        }
+        const uint32_t old = atomicCAS(&s.hash[4 * b + free_slot], kEmpty, id);
+        if (old == kEmpty) return true;
+        if (old == id) return false;
+        // another lane took the slot: re-read the same bucket
+    }
+    atomicOr(err, ERR_VISITED_OVERFLOW);
+    return false;
+}
+
+// Copies every id of the LDS table into the HBM table (once, when the LDS table fills).
+__device__ inline void flush_visited(const SearchParams &p, Lds &s, uint64_t *gtab, uint32_t *err,
+                                     uint32_t epoch) {
+    for (uint32_t h = lane_id(); h < p.hslots; h += kWave) {
+        const uint32_t v = s.hash[h];
+        if (v != kEmpty) (void)hbm_visit(p, gtab, v, err, epoch);
+    }
+}
+
 __device__ inline float readlane_f(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
 // Rank of each new item among the new items (key: d asc, insertion index desc), NaNs
-// excluded; writes the sorted list sn_d / sn_i. Returns the number of valid items.
+// excluded; writes the sorted list sn_d / sn_i and returns the number of valid items.
+// Up to kInPlaceRank items every lane holds its items in registers until all ranks are
+// known, so the sorted list may overwrite the expansion list (sn aliases nl).
 __device__ inline int rank_new(Lds &s, int nn) {
     const int lane = lane_id();
     if (nn <= kWave) {
         // one item per lane; the others' distances are broadcast with v_readlane
         const bool has = lane < nn;
         const float di = has ? s.nl_d[lane] : 0.0f;
+        const uint32_t ii = has ? s.nl_i[lane] : 0u;
         const bool ok = has && !__builtin_isnan(di);
         int r = 0;
         for (int j = 0; j < nn; ++j) {
@@ -151,10 +193,43 @@ __device__ inline int rank_new(Lds &s, int nn) {
         }
         if (ok) {
             s.sn_d[r] = di;
-            s.sn_i[r] = s.nl_i[lane];
+            s.sn_i[r] = ii;
         }
         return __builtin_popcountll(ballot(ok));
     }
+    if (nn <= kInPlaceRank) {
+        constexpr int E = kInPlaceRank / kWave;
+        float d[E];
+        uint32_t id[E];
+        int r[E];
+        bool ok[E];
+        int nvalid = 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = lane + kWave * e;
+            const bool has = i < nn;
+            d[e] = has ? s.nl_d[i] : 0.0f;
+            id[e] = has ? s.nl_i[i] : 0u;
+            ok[e] = has && !__builtin_isnan(d[e]);
+            r[e] = 0;
+            nvalid += __builtin_popcountll(ballot(ok[e]));
+        }
+        for (int j = 0; j < nn; ++j) {
+            const float dj = s.nl_d[j];
+#pragma unroll
+            for (int e = 0; e < E; ++e) r[e] += (dj < d[e]) || (dj == d[e] && j > lane + kWave * e);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (ok[e]) {
+                s.sn_d[r[e]] = d[e];
+                s.sn_i[r[e]] = id[e];
+            }
+        }
+        return nvalid;
+    }
+    // longer lists (sn is a separate buffer then)
     int nvalid = 0;
     for (int base = 0; base < nn; base += kWave) {
         int i = base + lane;
@@ -186,12 +261,16 @@ __device__ inline int wave_min_i(int v) {
     return v;
 }
 
+// Old queue items j = j0 + lane + 64 e move to j + #new with d_new <= d_old (the new
+// distances broadcast lane by lane). Blocks go from the end of the queue to its start
+// and every item of a block is read before any is written, so moving right never
+// overwrites an item not yet read.
 template <int E>
-__device__ inline void merge_old(Lds &s, int size, int nvalid, int cap, const float *qd,
-                                 const uint32_t *qi, float *od, uint32_t *oi, int cursor_old,
-                                 int &best) {
+__device__ inline void merge_old(Lds &s, int size, int nvalid, int cap, float *qd, uint32_t *qi,
+                                 int cursor_old, int &best) {
     const int lane = lane_id();
-    for (int j0 = 0; j0 < size; j0 += E * kWave) {
+    constexpr int B = E * kWave;
+    for (int j0 = size > 0 ? ((size - 1) / B) * B : 0; j0 >= 0; j0 -= B) {
         float dj[E];
         uint32_t ij[E];
         int cnt[E];
@@ -217,8 +296,8 @@ __device__ inline void merge_old(Lds &s, int size, int nvalid, int cap, const fl
             if (j < size) {
                 const int pos = j + cnt[e];
                 if (pos < cap) {
-                    od[pos] = dj[e];
-                    oi[pos] = ij[e];
+                    qd[pos] = dj[e];
+                    qi[pos] = ij[e];
                     if (j == cursor_old) best = min(best, pos);
                 }
             }
@@ -226,16 +305,17 @@ __device__ inline void merge_old(Lds &s, int size, int nvalid, int cap, const fl
     }
 }
 
-// Merge the sorted new items into the queue (qd/qi -> od/oi, then swap). New items go
-// before old items of equal distance; positions >= cap are dropped. cursor_old is the
-// first unvisited index of the old queue (or size); *cursor receives the first unvisited
-// index of the merged queue (or its size). Returns the new size.
-__device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *&qd,
-                                  uint32_t *&qi, float *&od, uint32_t *&oi, int cursor_old,
-                                  int *cursor) {
+// Merge the sorted new items into the queue in place. New items go before old items of
+// equal distance; positions >= cap are dropped. cursor_old is the first unvisited index
+// of the old queue (or size); *cursor receives the first unvisited index of the merged
+// queue (or its size). Returns the new size.
+__device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *qd, uint32_t *qi,
+                                  int cursor_old, int *cursor) {
     const int lane = lane_id();
     int best = 0x7FFFFFFF;
-    // new items: position = rank + #old with d_old < d  (lower bound in the old queue)
+    // new items: position = rank + #old with d_old < d (lower bound in the old queue),
+    // computed before the old items move
+    int32_t *npos = (int32_t *)s.rows;
     for (int r = lane; r < nvalid; r += kWave) {
         float d = s.sn_d[r];
         int lo = 0, hi = size;
@@ -243,23 +323,25 @@ __device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *
             int mid = (lo + hi) >> 1;
             if (qd[mid] < d) lo = mid + 1; else hi = mid;
         }
-        int pos = r + lo;
+        const int pos = r + lo;
+        npos[r] = pos;
+        if (r == 0 && pos < cap) best = pos;
+    }
+    __syncthreads();
+    if (size <= kWave) merge_old<1>(s, size, nvalid, cap, qd, qi, cursor_old, best);
+    else if (size <= 2 * kWave) merge_old<2>(s, size, nvalid, cap, qd, qi, cursor_old, best);
+    else if (size <= 4 * kWave) merge_old<4>(s, size, nvalid, cap, qd, qi, cursor_old, best);
+    else merge_old<8>(s, size, nvalid, cap, qd, qi, cursor_old, best);
+    __syncthreads();
+    for (int r = lane; r < nvalid; r += kWave) {
+        const int pos = npos[r];
         if (pos < cap) {
-            od[pos] = d;
-            oi[pos] = s.sn_i[r];
-            if (r == 0) best = pos;
+            qd[pos] = s.sn_d[r];
+            qi[pos] = s.sn_i[r];
         }
     }
-    // old items j = lane + 64 e: position = j + #new with d_new <= d_old, the new distances
-    // broadcast lane by lane; the register tile E follows the queue size
-    if (size <= kWave) merge_old<1>(s, size, nvalid, cap, qd, qi, od, oi, cursor_old, best);
-    else if (size <= 2 * kWave) merge_old<2>(s, size, nvalid, cap, qd, qi, od, oi, cursor_old, best);
-    else if (size <= 4 * kWave) merge_old<4>(s, size, nvalid, cap, qd, qi, od, oi, cursor_old, best);
-    else merge_old<8>(s, size, nvalid, cap, qd, qi, od, oi, cursor_old, best);
     best = wave_min_i(best);
     __syncthreads();
-    float *td = qd; qd = od; od = td;
-    uint32_t *ti = qi; qi = oi; oi = ti;
     int ns = size + nvalid;
     ns = ns < cap ? ns : cap;
     *cursor = best < ns ? best : ns;
@@ -510,8 +592,8 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
         __syncthreads();
     }
 
-    float *qd = s.qa_d, *od = s.qb_d;
-    uint32_t *qi = s.qa_i, *oi = s.qb_i;
+    float *qd = s.qa_d;
+    uint32_t *qi = s.qa_i;
     int size = 0;
     uint32_t cmps = 0, hops = 0, nrec = 0, degsum = 0;
     int lds_count = 0;
@@ -533,7 +615,7 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
     {
         int nvalid = rank_new(s, p.nstart);
         __syncthreads();
-        size = merge_queue(s, size, nvalid, cap, qd, qi, od, oi, 0, &cursor);
+        size = merge_queue(s, size, nvalid, cap, qd, qi, 0, &cursor);
     }
     cmps = p.nstart;
 
@@ -593,7 +675,10 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
                 bool act = j < cnt;
                 uint32_t id = act ? row[j] : 0;
                 bool spill = lds_count > spill_at;
-                if (spill && gtab == nullptr) gtab = job.spill;  // this query's HBM region
+                if (spill && gtab == nullptr) {
+                    gtab = job.spill;  // this query's HBM region becomes the set
+                    flush_visited(p, s, gtab, err, job.epoch);
+                }
                 bool fresh = visit_insert(p, s, id, act, spill, gtab, err, job.epoch);
                 uint64_t m = ballot(fresh);
                 if (fresh) s.nl_i[nn + prefix_count(m)] = id;
@@ -606,7 +691,7 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
         compute_distances<METRIC, MODE, DIM, PQLEN>(p, s, nn);
         int nvalid = rank_new(s, nn);
         __syncthreads();
-        size = merge_queue(s, size, nvalid, cap, qd, qi, od, oi, next, &cursor);
+        size = merge_queue(s, size, nvalid, cap, qd, qi, next, &cursor);
         cmps += (uint32_t)nn;
         hops += (uint32_t)nbeam;
     }
@@ -637,7 +722,9 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
         }
     } else {
         // FilterStartPoints -> Rerank: exact FP32 distances, stable sort, first k.
-        // The spare queue buffer (od, oi) holds the candidates.
+        // The candidates are compacted to the front of the queue (in place: an entry
+        // only moves left, and every entry of a chunk is read before any is written) and
+        // their FP32 distances replace the queue distances.
         int n = 0;
         for (int base = 0; base < size; base += kWave) {
             int i = base + lane;
@@ -649,13 +736,13 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
                 for (int t = 0; t < p.nstart; ++t) ok = ok && (s.starts[t] != id);
             }
             uint64_t m = ballot(ok);
-            if (ok) oi[n + prefix_count(m)] = id;
+            if (ok) qi[n + prefix_count(m)] = id;
             n += __builtin_popcountll(m);
         }
         __syncthreads();
-        if (p.metric == METRIC_L2) fp_distances<METRIC_L2, DIM>(p, s.q, oi, od, n, s.starts[0]);
-        else if (p.metric == METRIC_IP) fp_distances<METRIC_IP, DIM>(p, s.q, oi, od, n, s.starts[0]);
-        else fp_distances<METRIC_COSINE, 0>(p, s.q, oi, od, n, s.starts[0]);
+        if (p.metric == METRIC_L2) fp_distances<METRIC_L2, DIM>(p, s.q, qi, qd, n, s.starts[0]);
+        else if (p.metric == METRIC_IP) fp_distances<METRIC_IP, DIM>(p, s.q, qi, qd, n, s.starts[0]);
+        else fp_distances<METRIC_COSINE, 0>(p, s.q, qi, qd, n, s.starts[0]);
         __syncthreads();
         // stable selection of the k smallest (distance, position) keys: k rounds of a
         // wave-wide arg-min; a chosen entry is marked through bit 31 of its id
@@ -663,8 +750,8 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
         for (int r = 0; r < kk; ++r) {
             unsigned long long bk = ~0ull;
             for (int i = lane; i < n; i += kWave) {
-                if (!(oi[i] & kVisitedBit)) {
-                    const float d = od[i];
+                if (!(qi[i] & kVisitedBit)) {
+                    const float d = qd[i];
                     uint32_t u = __float_as_uint(d == 0.0f ? 0.0f : d);
                     u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
                     const unsigned long long key = ((unsigned long long)u << 32) | (uint32_t)i;
@@ -678,9 +765,9 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
             }
             const int win = (int)(bk & 0xFFFFFFFFull);
             if (lane == 0) {
-                oids[r] = oi[win];
-                ods[r] = od[win];
-                oi[win] |= kVisitedBit;
+                oids[r] = qi[win];
+                ods[r] = qd[win];
+                qi[win] |= kVisitedBit;
             }
             __syncthreads();
         }

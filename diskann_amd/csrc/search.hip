@@ -174,6 +174,155 @@ __device__ inline float readlane_f(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
+__device__ inline uint32_t readlane_u(uint32_t v, int l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
+// True if the LDS level proves `id` visited: the exact table (probe until a bucket with a
+// free slot) before the HBM table takes over, the home bucket of the cache after.
+__device__ inline bool lds_has(const SearchParams &p, const Lds &s, uint32_t id, bool spill) {
+    const uint32_t HB = p.hslots >> 2;
+    uint32_t b = (uint32_t)(((uint64_t)hash_id(id) * HB) >> 32);
+    for (uint32_t probe = 0; probe < HB; ++probe) {
+        const uint4 v = *(const uint4 *)&s.hash[4 * b];
+        if (v.x == id || v.y == id || v.z == id || v.w == id) return true;
+        if (spill || v.x == kEmpty || v.y == kEmpty || v.z == kEmpty || v.w == kEmpty) return false;
+        b = b + 1 == HB ? 0 : b + 1;
+    }
+    return false;
+}
+
+// Visited-set step for up to E distinct ids per lane, none of which the LDS level knows
+// (fresh[e] = the id was not visited; it is recorded now). With the HBM table every
+// round issues the pending loads / compare-and-swaps of all E ids before using any
+// result, so one hop costs about two memory round trips instead of two per row.
+template <int E>
+__device__ inline void visit_group(const SearchParams &p, Lds &s, const uint32_t (&id)[E],
+                                   const bool (&act)[E], bool spill, uint64_t *gtab,
+                                   uint32_t *err, uint32_t epoch, bool (&fresh)[E]) {
+    if (!spill) {
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            fresh[e] = visit_insert(p, s, id[e], act[e], false, nullptr, err, epoch);
+        return;
+    }
+    const uint64_t GH = p.spill_slots;
+    uint64_t key[E], g[E], cur[E];
+    int st[E];  // 0 pending, 1 fresh, 2 present / inactive
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        key[e] = ((uint64_t)epoch << 32) | id[e];
+        g[e] = hash_id(id[e] ^ 0x9e3779b9u) & (GH - 1);
+        st[e] = act[e] ? 0 : 2;
+        cur[e] = 0;
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        if (st[e] == 0) cur[e] = __hip_atomic_load(&gtab[g[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint64_t round = 0; round < GH + kWave; ++round) {
+        bool need_load[E], need_cas[E];
+        bool pending = false;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            need_load[e] = need_cas[e] = false;
+            if (st[e] != 0) continue;
+            if (cur[e] == key[e]) {
+                st[e] = 2;
+            } else if ((uint32_t)(cur[e] >> 32) == epoch) {  // live entry of another id
+                g[e] = (g[e] + 1) & (GH - 1);
+                need_load[e] = true;
+            } else {
+                need_cas[e] = true;
+            }
+            pending = pending || st[e] == 0;
+        }
+        if (!__builtin_amdgcn_ballot_w64(pending)) break;
+        uint64_t got[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            got[e] = cur[e];
+            if (need_load[e])
+                got[e] = __hip_atomic_load(&gtab[g[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else if (need_cas[e])
+                got[e] = atomicCAS((unsigned long long *)&gtab[g[e]], (unsigned long long)cur[e],
+                                   (unsigned long long)key[e]);
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (need_cas[e] && got[e] == cur[e]) st[e] = 1;
+            cur[e] = got[e];
+        }
+    }
+    const uint32_t HB = p.hslots >> 2;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if (st[e] == 0) atomicOr(err, ERR_VISITED_OVERFLOW);
+        fresh[e] = st[e] == 1;
+        if (act[e]) {  // remember it in the cache
+            const uint32_t b = (uint32_t)(((uint64_t)hash_id(id[e]) * HB) >> 32);
+            s.hash[4 * b + (hash_id(id[e] ^ 0x5bd1e995u) >> 30)] = id[e];
+        }
+    }
+}
+
+// Runs the n distinct ids c[0..n) through the visited set, E ids per lane at a time, and
+// compacts the fresh ones to the front of c in order; returns their number.
+template <int E>
+__device__ inline int visit_chunk(const SearchParams &p, Lds &s, uint32_t *c, int n, bool spill,
+                                  uint64_t *gtab, uint32_t *err, uint32_t epoch) {
+    const int lane = lane_id();
+    int nn = 0;
+    for (int t0 = 0; t0 < n; t0 += E * kWave) {
+        uint32_t id[E];
+        bool act[E], fresh[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int t = t0 + lane + kWave * e;
+            act[e] = t < n;
+            id[e] = act[e] ? c[t] : 0u;
+        }
+        visit_group<E>(p, s, id, act, spill, gtab, err, epoch, fresh);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint64_t mk = ballot(fresh[e]);
+            if (fresh[e]) c[nn + prefix_count(mk)] = id[e];
+            nn += __builtin_popcountll(mk);
+        }
+    }
+    return nn;
+}
+
+// One hop's visited filtering, row by row in beam order: a row's ids the LDS level
+// proves visited are dropped, the rest go through the visited set together (one row's
+// ids are distinct), and the fresh ones are appended to s.nl_i in adjacency order. A
+// row sees everything the rows before it recorded, as sequential processing would.
+// Returns the number of fresh ids.
+__device__ inline int expand_hop(const SearchParams &p, Lds &s, int nbeam, bool spill,
+                                 uint64_t *gtab, uint32_t *err, uint32_t epoch, uint32_t &degsum) {
+    const int lane = lane_id();
+    uint32_t *c = s.nl_i;
+    int nn = 0;
+    for (int b = 0; b < nbeam; ++b) {
+        const uint32_t *row = s.rows + b * p.gstride;
+        const int cnt = (int)row[p.gstride - 1];
+        degsum += (uint32_t)cnt;
+        int m = 0;
+        for (int j0 = 0; j0 < cnt; j0 += kWave) {
+            const int j = j0 + lane;
+            const uint32_t id = j < cnt ? row[j] : kEmpty;
+            const bool keep = j < cnt && !lds_has(p, s, id, spill);
+            const uint64_t mk = ballot(keep);
+            if (keep) c[nn + m + prefix_count(mk)] = id;
+            m += __builtin_popcountll(mk);
+        }
+        __syncthreads();
+        if (m <= kWave) nn += visit_chunk<1>(p, s, c + nn, m, spill, gtab, err, epoch);
+        else nn += visit_chunk<2>(p, s, c + nn, m, spill, gtab, err, epoch);
+        __syncthreads();
+    }
+    return nn;
+}
+
 // Rank of each new item among the new items (key: d asc, insertion index desc), NaNs
 // excluded; writes the sorted list sn_d / sn_i and returns the number of valid items.
 // Up to kInPlaceRank items every lane holds its items in registers until all ranks are
@@ -597,6 +746,7 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
     int size = 0;
     uint32_t cmps = 0, hops = 0, nrec = 0, degsum = 0;
     int lds_count = 0;
+    const int spill_at = (int)(p.hslots - (p.hslots >> 3)) - (p.nlcap - kWave);
     uint64_t *gtab = nullptr;
 
     // start points: visited + queue insert in order
@@ -658,35 +808,21 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
         nrec += nbeam;
 
         // expand: stage the beam nodes' adjacency rows (ids + count slot) with one round
-        // of loads, then filter neighbours in adjacency order through the visited set
+        // of loads, then filter the neighbours through the visited set
         for (int b = 0; b < nbeam; ++b) {
             const uint32_t *row = p.graph + (size_t)s.beam[b] * p.gstride;
             for (int j = lane; j < p.gstride; j += kWave) s.rows[b * p.gstride + j] = row[j];
         }
         __syncthreads();
-        int nn = 0;
-        const int spill_at = (int)(p.hslots - (p.hslots >> 3)) - kWave;
-        for (int b = 0; b < nbeam; ++b) {
-            const uint32_t *row = s.rows + b * p.gstride;
-            const int cnt = (int)row[p.gstride - 1];
-            degsum += (uint32_t)cnt;
-            for (int j0 = 0; j0 < cnt; j0 += kWave) {
-                int j = j0 + lane;
-                bool act = j < cnt;
-                uint32_t id = act ? row[j] : 0;
-                bool spill = lds_count > spill_at;
-                if (spill && gtab == nullptr) {
-                    gtab = job.spill;  // this query's HBM region becomes the set
-                    flush_visited(p, s, gtab, err, job.epoch);
-                }
-                bool fresh = visit_insert(p, s, id, act, spill, gtab, err, job.epoch);
-                uint64_t m = ballot(fresh);
-                if (fresh) s.nl_i[nn + prefix_count(m)] = id;
-                int got = __builtin_popcountll(m);
-                nn += got;
-                if (!spill) lds_count += got;
-            }
+        // the exact LDS table keeps room for one more hop's ids; past that the HBM table
+        // takes over (decided per hop, uniformly)
+        const bool spill = lds_count > spill_at;
+        if (spill && gtab == nullptr) {
+            gtab = job.spill;
+            flush_visited(p, s, gtab, err, job.epoch);
         }
+        const int nn = expand_hop(p, s, nbeam, spill, gtab, err, job.epoch, degsum);
+        if (!spill) lds_count += nn;
         __syncthreads();
         compute_distances<METRIC, MODE, DIM, PQLEN>(p, s, nn);
         int nvalid = rank_new(s, nn);

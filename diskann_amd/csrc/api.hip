@@ -91,10 +91,9 @@ struct dkm_index {
     bool codes_set = false;
     std::vector<int64_t> h_offsets;
     // visited-set spill
-    uint64_t *d_spill = nullptr;
+    uint32_t *d_spill = nullptr;
     int64_t spill_regions = 0;          // allocated regions (one per query of a launch)
     uint64_t spill_slots = 1u << 14;
-    uint32_t epoch = 0;
     int64_t tune_hslots = 0;
     int tune_adc_mode = 1;              // PQ distances: 0 = LDS table, 1 = direct
     int pq_len = 0;                     // uniform PQ chunk width usable by the fast path
@@ -516,23 +515,15 @@ static uint32_t default_visited_slots(uint64_t L) {
     return 256;
 }
 
-// Per-query visited-set spill regions (HBM) for `nq` queries; entries are tagged with a
-// launch epoch, so regions never need clearing except when the epoch wraps.
+// Per-query visited-set spill regions (HBM) for `nq` queries; a query clears its own
+// region when it starts using it.
 static dkm_status ensure_spill(dkm_index *ix, int64_t nq) {
     if (ix->spill_regions < nq) {
         if (ix->d_spill) HIP_TRY(hipFree(ix->d_spill));
         ix->d_spill = nullptr;
         ix->spill_regions = 0;
-        size_t bytes = sizeof(uint64_t) * ix->spill_slots * (size_t)nq;
-        HIP_TRY(hipMalloc(&ix->d_spill, bytes));
-        HIP_TRY(hipMemsetAsync(ix->d_spill, 0, bytes, ix->stream));
+        HIP_TRY(hipMalloc(&ix->d_spill, sizeof(uint32_t) * ix->spill_slots * (size_t)nq));
         ix->spill_regions = nq;
-    }
-    ix->epoch += 1;
-    if (ix->epoch == 0) {  // wrapped: stale tags could collide, clear the table
-        HIP_TRY(hipMemsetAsync(ix->d_spill, 0, sizeof(uint64_t) * ix->spill_slots * ix->spill_regions,
-                               ix->stream));
-        ix->epoch = 1;
     }
     return DKM_OK;
 }
@@ -597,7 +588,6 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     p.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots : default_visited_slots(max_L);
     p.spill_table = ix->d_spill;
     p.spill_slots = ix->spill_slots;
-    p.epoch = ix->epoch;
     p.max_iters = (uint32_t)std::min<int64_t>(ix->rows + 16, 0x7FFFFFFF);
     size_t lds = dkm::search_lds_bytes(p);
     if (lds > 160 * 1024)
@@ -876,7 +866,6 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
         sp.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots : default_visited_slots(L_build);
         sp.spill_table = ix->d_spill;
         sp.spill_slots = ix->spill_slots;
-        sp.epoch = ix->epoch;
         sp.max_iters = (uint32_t)std::min<int64_t>(ix->rows + 16, 0x7FFFFFFF);
         sp.rec_ids = rid.p;
         sp.rec_d = rd.p;
@@ -965,12 +954,11 @@ struct ServeLane {
     bool busy = false;
     uint64_t seq0 = 0;
     int64_t n = 0;
-    uint32_t epoch = 0;
     float *d_q = nullptr;  // device buffers sized for one launch
     uint32_t *d_k = nullptr, *d_L = nullptr, *d_ids = nullptr, *d_cnt = nullptr;
     uint32_t *d_cmps = nullptr, *d_hops = nullptr, *d_status = nullptr;
     float *d_d = nullptr;
-    uint64_t *d_spill = nullptr;
+    uint32_t *d_spill = nullptr;
 };
 
 double now_us() {
@@ -1070,11 +1058,6 @@ static dkm_status server_pump(dkm_server *sv) {
         HIP_TRY(ring_copy(sv->queries, l.d_q, seq0, n, dim, cap, true, l.stream));
         HIP_TRY(ring_copy(sv->req_k, l.d_k, seq0, n, 1, cap, true, l.stream));
         HIP_TRY(ring_copy(sv->req_L, l.d_L, seq0, n, 1, cap, true, l.stream));
-        if (++l.epoch == 0) {  // wrapped: stale tags could collide, clear the table
-            HIP_TRY(hipMemsetAsync(l.d_spill, 0,
-                                   sizeof(uint64_t) * sv->ix->spill_slots * sv->per_launch, l.stream));
-            l.epoch = 1;
-        }
         dkm::SearchParams p = sv->base;
         p.queries = l.d_q;
         p.nq = (int)n;
@@ -1087,7 +1070,6 @@ static dkm_status server_pump(dkm_server *sv) {
         p.out_hops = l.d_hops;
         p.status = l.d_status;
         p.spill_table = l.d_spill;
-        p.epoch = l.epoch;
         hipError_t e = dkm::launch_search(p, l.stream);
         if (e != hipSuccess) return fail(DKM_ERR_DEVICE, "serving launch: %s", hipGetErrorString(e));
         HIP_TRY(ring_copy(sv->res_ids, l.d_ids, seq0, n, sv->max_k, cap, false, l.stream));
@@ -1192,9 +1174,7 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
         if (e == hipSuccess) e = hipMalloc(&l.d_cmps, sizeof(uint32_t) * P);
         if (e == hipSuccess) e = hipMalloc(&l.d_hops, sizeof(uint32_t) * P);
         if (e == hipSuccess) e = hipMalloc(&l.d_status, sizeof(uint32_t) * P);
-        if (e == hipSuccess) e = hipMalloc(&l.d_spill, sizeof(uint64_t) * ix->spill_slots * P);
-        if (e == hipSuccess)
-            e = hipMemsetAsync(l.d_spill, 0, sizeof(uint64_t) * ix->spill_slots * P, l.stream);
+        if (e == hipSuccess) e = hipMalloc(&l.d_spill, sizeof(uint32_t) * ix->spill_slots * P);
     }
     for (ServeLane &l : sv->lane)
         if (e == hipSuccess && l.stream) e = hipStreamSynchronize(l.stream);

@@ -42,9 +42,9 @@ struct SearchParams {
     int qcap;              // max L + nstart in the launch
     int nlcap;             // expansion list capacity
     uint32_t hslots;       // LDS visited slots (power of two)
-    uint64_t *spill_table; // nq x spill_slots (one region per query of the launch)
+    uint32_t *spill_table; // nq x spill_slots (one region per query of the launch); a
+                           // query clears its region when it starts using it
     uint64_t spill_slots;  // power of two
-    uint32_t epoch;
     uint32_t max_iters;
     uint32_t *rec_ids;     // optional visited record (expanded nodes), rec_cap per query
     float *rec_d;

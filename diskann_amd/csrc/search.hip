@@ -96,24 +96,16 @@ __device__ inline Lds carve(const SearchParams &p, char *base) {
     return s;
 }
 
-// Insert-or-find in this query's region of the HBM visited table (64-bit entries tagged
-// with the launch epoch, linear probing; entries of other epochs read as empty). Returns
-// true if the id was not present.
-__device__ inline bool hbm_visit(const SearchParams &p, uint64_t *gtab, uint32_t id,
-                                 uint32_t *err, uint32_t epoch) {
+// Insert-or-find in this query's region of the HBM visited table (u32 ids, kEmpty when
+// free, linear probing, cleared by the query before first use): one compare-and-swap per
+// probe. Returns true if the id was not present.
+__device__ inline bool hbm_visit(const SearchParams &p, uint32_t *gtab, uint32_t id, uint32_t *err) {
     const uint64_t GH = p.spill_slots;
-    const uint64_t key = ((uint64_t)epoch << 32) | id;
     uint64_t g = hash_id(id ^ 0x9e3779b9u) & (GH - 1);
     for (uint64_t probe = 0; probe < GH; ++probe) {
-        uint64_t cur = __hip_atomic_load(&gtab[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (;;) {
-            if (cur == key) return false;
-            if ((uint32_t)(cur >> 32) == epoch) break;  // live entry of another id
-            uint64_t prev = atomicCAS((unsigned long long *)&gtab[g], (unsigned long long)cur,
-                                      (unsigned long long)key);
-            if (prev == cur) return true;
-            cur = prev;
-        }
+        const uint32_t prev = atomicCAS(&gtab[g], kEmpty, id);
+        if (prev == kEmpty) return true;
+        if (prev == id) return false;
         g = (g + 1) & (GH - 1);
     }
     atomicOr(err, ERR_VISITED_OVERFLOW);
@@ -131,14 +123,14 @@ __device__ inline bool hbm_visit(const SearchParams &p, uint64_t *gtab, uint32_t
 // 4-way cache of recently seen ids (home bucket only, random replacement): a hit there
 // proves the id visited without an HBM round trip, a miss asks the HBM table.
 __device__ inline bool visit_insert(const SearchParams &p, Lds &s, uint32_t id, bool active,
-                                    bool spill, uint64_t *gtab, uint32_t *err, uint32_t epoch) {
+                                    bool spill, uint32_t *gtab, uint32_t *err) {
     if (!active) return false;
     const uint32_t HB = p.hslots >> 2;  // buckets
     uint32_t b = (uint32_t)(((uint64_t)hash_id(id) * HB) >> 32);
     if (spill) {
         const uint4 v = *(const uint4 *)&s.hash[4 * b];
         if (v.x == id || v.y == id || v.z == id || v.w == id) return false;
-        const bool fresh = hbm_visit(p, gtab, id, err, epoch);
+        const bool fresh = hbm_visit(p, gtab, id, err);
         s.hash[4 * b + (hash_id(id ^ 0x5bd1e995u) >> 30)] = id;
         return fresh;
     }
@@ -161,12 +153,19 @@ __device__ inline bool visit_insert(const SearchParams &p, Lds &s, uint32_t id, 
     return false;
 }
 
-// Copies every id of the LDS table into the HBM table (once, when the LDS table fills).
-__device__ inline void flush_visited(const SearchParams &p, Lds &s, uint64_t *gtab, uint32_t *err,
-                                     uint32_t epoch) {
-    for (uint32_t h = lane_id(); h < p.hslots; h += kWave) {
+// Clears this query's HBM region, then copies every id of the LDS table into it (once,
+// when the LDS table fills). The wave waits for its clearing stores before the first
+// compare-and-swap on the region.
+__device__ inline void flush_visited(const SearchParams &p, Lds &s, uint32_t *gtab, uint32_t *err) {
+    const int lane = lane_id();
+    const uint4 empty = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+    for (uint64_t i = 4 * (uint64_t)lane; i < p.spill_slots; i += 4 * kWave)
+        *(uint4 *)&gtab[i] = empty;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (uint32_t h = lane; h < p.hslots; h += kWave) {
         const uint32_t v = s.hash[h];
-        if (v != kEmpty) (void)hbm_visit(p, gtab, v, err, epoch);
+        if (v != kEmpty) (void)hbm_visit(p, gtab, v, err);
     }
 }
 
@@ -198,59 +197,43 @@ __device__ inline bool lds_has(const SearchParams &p, const Lds &s, uint32_t id,
 // result, so one hop costs about two memory round trips instead of two per row.
 template <int E>
 __device__ inline void visit_group(const SearchParams &p, Lds &s, const uint32_t (&id)[E],
-                                   const bool (&act)[E], bool spill, uint64_t *gtab,
-                                   uint32_t *err, uint32_t epoch, bool (&fresh)[E]) {
+                                   const bool (&act)[E], bool spill, uint32_t *gtab,
+                                   uint32_t *err, bool (&fresh)[E]) {
     if (!spill) {
 #pragma unroll
-        for (int e = 0; e < E; ++e)
-            fresh[e] = visit_insert(p, s, id[e], act[e], false, nullptr, err, epoch);
+        for (int e = 0; e < E; ++e) fresh[e] = visit_insert(p, s, id[e], act[e], false, nullptr, err);
         return;
     }
     const uint64_t GH = p.spill_slots;
-    uint64_t key[E], g[E], cur[E];
+    uint64_t g[E];
     int st[E];  // 0 pending, 1 fresh, 2 present / inactive
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        key[e] = ((uint64_t)epoch << 32) | id[e];
         g[e] = hash_id(id[e] ^ 0x9e3779b9u) & (GH - 1);
         st[e] = act[e] ? 0 : 2;
-        cur[e] = 0;
     }
-#pragma unroll
-    for (int e = 0; e < E; ++e)
-        if (st[e] == 0) cur[e] = __hip_atomic_load(&gtab[g[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (uint64_t round = 0; round < GH + kWave; ++round) {
-        bool need_load[E], need_cas[E];
+    // round: a load per pending id; an id found empty is claimed with a compare-and-swap
+    // in the same round (dups, the common case, cost one load)
+    for (uint64_t round = 0; round < GH; ++round) {
         bool pending = false;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            need_load[e] = need_cas[e] = false;
-            if (st[e] != 0) continue;
-            if (cur[e] == key[e]) {
-                st[e] = 2;
-            } else if ((uint32_t)(cur[e] >> 32) == epoch) {  // live entry of another id
-                g[e] = (g[e] + 1) & (GH - 1);
-                need_load[e] = true;
-            } else {
-                need_cas[e] = true;
-            }
-            pending = pending || st[e] == 0;
-        }
+        for (int e = 0; e < E; ++e) pending = pending || st[e] == 0;
         if (!__builtin_amdgcn_ballot_w64(pending)) break;
-        uint64_t got[E];
+        uint32_t cur[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            cur[e] = st[e] == 0 ? __hip_atomic_load(&gtab[g[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                : 0u;
+        uint32_t prev[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            prev[e] = (st[e] == 0 && cur[e] == kEmpty) ? atomicCAS(&gtab[g[e]], kEmpty, id[e]) : cur[e];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            got[e] = cur[e];
-            if (need_load[e])
-                got[e] = __hip_atomic_load(&gtab[g[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            else if (need_cas[e])
-                got[e] = atomicCAS((unsigned long long *)&gtab[g[e]], (unsigned long long)cur[e],
-                                   (unsigned long long)key[e]);
-        }
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            if (need_cas[e] && got[e] == cur[e]) st[e] = 1;
-            cur[e] = got[e];
+            if (st[e] != 0) continue;
+            if (cur[e] == kEmpty && prev[e] == kEmpty) st[e] = 1;
+            else if (prev[e] == id[e]) st[e] = 2;
+            else if (prev[e] != kEmpty) g[e] = (g[e] + 1) & (GH - 1);
         }
     }
     const uint32_t HB = p.hslots >> 2;
@@ -269,7 +252,7 @@ __device__ inline void visit_group(const SearchParams &p, Lds &s, const uint32_t
 // compacts the fresh ones to the front of c in order; returns their number.
 template <int E>
 __device__ inline int visit_chunk(const SearchParams &p, Lds &s, uint32_t *c, int n, bool spill,
-                                  uint64_t *gtab, uint32_t *err, uint32_t epoch) {
+                                  uint32_t *gtab, uint32_t *err) {
     const int lane = lane_id();
     int nn = 0;
     for (int t0 = 0; t0 < n; t0 += E * kWave) {
@@ -281,7 +264,7 @@ __device__ inline int visit_chunk(const SearchParams &p, Lds &s, uint32_t *c, in
             act[e] = t < n;
             id[e] = act[e] ? c[t] : 0u;
         }
-        visit_group<E>(p, s, id, act, spill, gtab, err, epoch, fresh);
+        visit_group<E>(p, s, id, act, spill, gtab, err, fresh);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const uint64_t mk = ballot(fresh[e]);
@@ -298,7 +281,7 @@ __device__ inline int visit_chunk(const SearchParams &p, Lds &s, uint32_t *c, in
 // row sees everything the rows before it recorded, as sequential processing would.
 // Returns the number of fresh ids.
 __device__ inline int expand_hop(const SearchParams &p, Lds &s, int nbeam, bool spill,
-                                 uint64_t *gtab, uint32_t *err, uint32_t epoch, uint32_t &degsum) {
+                                 uint32_t *gtab, uint32_t *err, uint32_t &degsum) {
     const int lane = lane_id();
     uint32_t *c = s.nl_i;
     int nn = 0;
@@ -316,8 +299,8 @@ __device__ inline int expand_hop(const SearchParams &p, Lds &s, int nbeam, bool 
             m += __builtin_popcountll(mk);
         }
         __syncthreads();
-        if (m <= kWave) nn += visit_chunk<1>(p, s, c + nn, m, spill, gtab, err, epoch);
-        else nn += visit_chunk<2>(p, s, c + nn, m, spill, gtab, err, epoch);
+        if (m <= kWave) nn += visit_chunk<1>(p, s, c + nn, m, spill, gtab, err);
+        else nn += visit_chunk<2>(p, s, c + nn, m, spill, gtab, err);
         __syncthreads();
     }
     return nn;
@@ -697,8 +680,7 @@ struct QueryJob {
     const float *query;
     int L, k;
     uint32_t *err;
-    uint64_t *spill;        // this query's HBM visited region
-    uint32_t epoch;         // tag of this query's entries in the region
+    uint32_t *spill;        // this query's HBM visited region
     uint32_t *rec_ids;      // optional visited record (rec_cap entries)
     float *rec_d;
     uint32_t *out_ids;      // k_stride entries
@@ -747,7 +729,7 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
     uint32_t cmps = 0, hops = 0, nrec = 0, degsum = 0;
     int lds_count = 0;
     const int spill_at = (int)(p.hslots - (p.hslots >> 3)) - (p.nlcap - kWave);
-    uint64_t *gtab = nullptr;
+    uint32_t *gtab = nullptr;
 
     // start points: visited + queue insert in order
     for (int i = lane; i < p.nstart; i += kWave) s.nl_i[i] = s.starts[i];
@@ -755,7 +737,7 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
     __syncthreads();
     for (int i = 0; i < p.nstart; ++i) {
         // sequential inserts (start ids may repeat only in malformed input)
-        bool ins = visit_insert(p, s, s.starts[i], lane == 0, false, nullptr, err, job.epoch);
+        bool ins = visit_insert(p, s, s.starts[i], lane == 0, false, nullptr, err);
         (void)ins;
         __syncthreads();
     }
@@ -819,9 +801,9 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
         const bool spill = lds_count > spill_at;
         if (spill && gtab == nullptr) {
             gtab = job.spill;
-            flush_visited(p, s, gtab, err, job.epoch);
+            flush_visited(p, s, gtab, err);
         }
-        const int nn = expand_hop(p, s, nbeam, spill, gtab, err, job.epoch, degsum);
+        const int nn = expand_hop(p, s, nbeam, spill, gtab, err, degsum);
         if (!spill) lds_count += nn;
         __syncthreads();
         compute_distances<METRIC, MODE, DIM, PQLEN>(p, s, nn);
@@ -934,7 +916,6 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     j.k = (int)p.k[qidx];
     j.err = p.status + qidx;
     j.spill = p.spill_table ? p.spill_table + (size_t)qidx * p.spill_slots : nullptr;
-    j.epoch = p.epoch;
     j.rec_ids = p.rec_ids ? p.rec_ids + (size_t)qidx * p.rec_cap : nullptr;
     j.rec_d = p.rec_d ? p.rec_d + (size_t)qidx * p.rec_cap : nullptr;
     j.out_ids = p.out_ids + (size_t)qidx * p.k_stride;

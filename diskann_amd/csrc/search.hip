@@ -393,16 +393,18 @@ __device__ inline int wave_min_i(int v) {
     return v;
 }
 
-// Old queue items j = j0 + lane + 64 e move to j + #new with d_new <= d_old (the new
-// distances broadcast lane by lane). Blocks go from the end of the queue to its start
-// and every item of a block is read before any is written, so moving right never
-// overwrites an item not yet read.
+// Old queue items j = j0 + lane + 64 e (j >= j_first) move to j + #new with
+// d_new <= d_old (the new distances broadcast lane by lane). Blocks go from the end of
+// the queue down to j_first and every item of a block is read before any is written, so
+// moving right never overwrites an item not yet read. Items before j_first do not move.
 template <int E>
 __device__ inline void merge_old(Lds &s, int size, int nvalid, int cap, float *qd, uint32_t *qi,
-                                 int cursor_old, int &best) {
+                                 int cursor_old, int &best, int j_first) {
     const int lane = lane_id();
     constexpr int B = E * kWave;
-    for (int j0 = size > 0 ? ((size - 1) / B) * B : 0; j0 >= 0; j0 -= B) {
+    const int nblk = (size - j_first + B - 1) / B;
+    for (int k = nblk - 1; k >= 0; --k) {
+        const int j0 = j_first + k * B;
         float dj[E];
         uint32_t ij[E];
         int cnt[E];
@@ -460,10 +462,17 @@ __device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *
         if (r == 0 && pos < cap) best = pos;
     }
     __syncthreads();
-    if (size <= kWave) merge_old<1>(s, size, nvalid, cap, qd, qi, cursor_old, best);
-    else if (size <= 2 * kWave) merge_old<2>(s, size, nvalid, cap, qd, qi, cursor_old, best);
-    else if (size <= 4 * kWave) merge_old<4>(s, size, nvalid, cap, qd, qi, cursor_old, best);
-    else merge_old<8>(s, size, nvalid, cap, qd, qi, cursor_old, best);
+    // old items in front of the smallest new item keep their place
+    const int first = nvalid > 0 ? npos[0] : size;
+    const int j_first = (first < size ? first : size) & ~(kWave - 1);
+    if (cursor_old < j_first) best = min(best, cursor_old);
+    const int range = size - j_first;
+    if (range > 0) {
+        if (range <= kWave) merge_old<1>(s, size, nvalid, cap, qd, qi, cursor_old, best, j_first);
+        else if (range <= 2 * kWave) merge_old<2>(s, size, nvalid, cap, qd, qi, cursor_old, best, j_first);
+        else if (range <= 4 * kWave) merge_old<4>(s, size, nvalid, cap, qd, qi, cursor_old, best, j_first);
+        else merge_old<8>(s, size, nvalid, cap, qd, qi, cursor_old, best, j_first);
+    }
     __syncthreads();
     for (int r = lane; r < nvalid; r += kWave) {
         const int pos = npos[r];

@@ -398,8 +398,8 @@ __device__ inline int wave_min_i(int v) {
 // the queue down to j_first and every item of a block is read before any is written, so
 // moving right never overwrites an item not yet read. Items before j_first do not move.
 template <int E>
-__device__ inline void merge_old(Lds &s, int size, int nvalid, int cap, float *qd, uint32_t *qi,
-                                 int cursor_old, int &best, int j_first) {
+__device__ inline void merge_old(Lds &s, const int32_t *lo, int size, int nvalid, int cap, float *qd,
+                                 uint32_t *qi, int cursor_old, int &best, int j_first) {
     const int lane = lane_id();
     constexpr int B = E * kWave;
     const int nblk = (size - j_first + B - 1) / B;
@@ -415,13 +415,31 @@ __device__ inline void merge_old(Lds &s, int size, int nvalid, int cap, float *q
             ij[e] = j < size ? qi[j] : 0u;
             cnt[e] = 0;
         }
-        for (int t0 = 0; t0 < nvalid; t0 += kWave) {
-            const float vn = t0 + lane < nvalid ? s.sn_d[t0 + lane] : 0.0f;
-            const int lim = nvalid - t0 < kWave ? nvalid - t0 : kWave;
-            for (int t = 0; t < lim; ++t) {
+        if (nvalid <= 8) {
+            // few new items: count them directly (d_new <= d_old)
+            const float vn = lane < nvalid ? s.sn_d[lane] : 0.0f;
+            for (int t = 0; t < nvalid; ++t) {
                 const float v = readlane_f(vn, t);
 #pragma unroll
                 for (int e = 0; e < E; ++e) cnt[e] += v <= dj[e];
+            }
+        } else {
+            // d_new[r] <= d_old[j]  <=>  lo[r] <= j, where lo[r] (non-decreasing in r) is
+            // the number of old items smaller than new item r: binary search in lo
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int j = j0 + lane + kWave * e;
+                int base = 0, len = nvalid;
+                while (len > 0) {
+                    const int half = len >> 1;
+                    if (lo[base + half] <= j) {
+                        base += half + 1;
+                        len -= half + 1;
+                    } else {
+                        len = half;
+                    }
+                }
+                cnt[e] = base;
             }
         }
 #pragma unroll
@@ -449,33 +467,32 @@ __device__ inline int merge_queue(Lds &s, int size, int nvalid, int cap, float *
     int best = 0x7FFFFFFF;
     // new items: position = rank + #old with d_old < d (lower bound in the old queue),
     // computed before the old items move
-    int32_t *npos = (int32_t *)s.rows;
+    int32_t *lo = (int32_t *)s.rows;
     for (int r = lane; r < nvalid; r += kWave) {
         float d = s.sn_d[r];
-        int lo = 0, hi = size;
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (qd[mid] < d) lo = mid + 1; else hi = mid;
+        int a = 0, b = size;
+        while (a < b) {
+            int mid = (a + b) >> 1;
+            if (qd[mid] < d) a = mid + 1; else b = mid;
         }
-        const int pos = r + lo;
-        npos[r] = pos;
-        if (r == 0 && pos < cap) best = pos;
+        lo[r] = a;
+        if (r == 0 && a < cap) best = a;
     }
     __syncthreads();
     // old items in front of the smallest new item keep their place
-    const int first = nvalid > 0 ? npos[0] : size;
+    const int first = nvalid > 0 ? lo[0] : size;
     const int j_first = (first < size ? first : size) & ~(kWave - 1);
     if (cursor_old < j_first) best = min(best, cursor_old);
     const int range = size - j_first;
     if (range > 0) {
-        if (range <= kWave) merge_old<1>(s, size, nvalid, cap, qd, qi, cursor_old, best, j_first);
-        else if (range <= 2 * kWave) merge_old<2>(s, size, nvalid, cap, qd, qi, cursor_old, best, j_first);
-        else if (range <= 4 * kWave) merge_old<4>(s, size, nvalid, cap, qd, qi, cursor_old, best, j_first);
-        else merge_old<8>(s, size, nvalid, cap, qd, qi, cursor_old, best, j_first);
+        if (range <= kWave) merge_old<1>(s, lo, size, nvalid, cap, qd, qi, cursor_old, best, j_first);
+        else if (range <= 2 * kWave) merge_old<2>(s, lo, size, nvalid, cap, qd, qi, cursor_old, best, j_first);
+        else if (range <= 4 * kWave) merge_old<4>(s, lo, size, nvalid, cap, qd, qi, cursor_old, best, j_first);
+        else merge_old<8>(s, lo, size, nvalid, cap, qd, qi, cursor_old, best, j_first);
     }
     __syncthreads();
     for (int r = lane; r < nvalid; r += kWave) {
-        const int pos = npos[r];
+        const int pos = lo[r] + r;
         if (pos < cap) {
             qd[pos] = s.sn_d[r];
             qi[pos] = s.sn_i[r];

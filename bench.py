@@ -156,6 +156,9 @@ def main():
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--cpu-seconds", type=float, default=1.5,
                     help="wall-clock budget of the CPU baseline sample")
+    ap.add_argument("--profile-copy-gib", type=float, default=0.0,
+                    help="after the run, copy this many GiB with the library's plain copy "
+                         "kernel (reference point for memory-traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -380,6 +383,12 @@ def main():
         },
     }
     print(json.dumps(out), flush=True)
+    if args.profile_copy_gib > 0:
+        nb = int(args.profile_copy_gib * (1 << 30)) // 16 * 16
+        a, b = DeviceBuf(lib, nbytes=nb), DeviceBuf(lib, nbytes=nb)
+        for _ in range(2):
+            check(lib.dkm_device_copy(b.ptr, a.ptr, nb), lib)
+        log("copied %d bytes twice" % nb)
 
 
 if __name__ == "__main__":

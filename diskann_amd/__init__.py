@@ -4,13 +4,15 @@ The compute path is the HIP library libdiskann_mi355x.so behind the C ABI in
 include/diskann_mi355x.h; this package mirrors the reference's search, PQ and distance
 interfaces on top of it.
 """
-from .errors import ANNError, CapacityError, DimensionMismatchError, KnnSearchError, PQError
+from .errors import (ANNError, CapacityError, DimensionMismatchError, KnnSearchError, PQError,
+                     StorageError)
 from .index import (DiskANNIndex, FullPrecision, Hybrid, Knn, Metric, Quantized, SearchResult,
                     Server, Strategy)
-from . import quantization
+from . import quantization, storage
 
 __all__ = [
     "ANNError", "CapacityError", "DimensionMismatchError", "KnnSearchError", "PQError",
+    "StorageError", "storage",
     "DiskANNIndex", "FullPrecision", "Hybrid", "Knn", "Metric", "Quantized", "SearchResult",
     "Server", "Strategy", "quantization",
 ]

@@ -12,7 +12,8 @@ LIB_PATH = os.path.join(HERE, "libdiskann_mi355x.so")
 # Every symbol declared in include/diskann_mi355x.h.
 EXPORTS = [
     "dkm_last_error", "dkm_version", "dkm_init", "dkm_device_alloc", "dkm_device_free",
-    "dkm_memcpy_h2d", "dkm_memcpy_d2h", "dkm_synchronize", "dkm_distance_batch",
+    "dkm_memcpy_h2d", "dkm_memcpy_d2h", "dkm_device_copy", "dkm_synchronize",
+    "dkm_distance_batch",
     "dkm_pq_chunk_offsets", "dkm_pq_lut", "dkm_pq_encode", "dkm_pq_train",
     "dkm_index_create", "dkm_index_destroy", "dkm_set_vectors", "dkm_set_start_points",
     "dkm_set_graph", "dkm_get_graph", "dkm_set_pq", "dkm_set_codes", "dkm_encode_index",
@@ -21,6 +22,9 @@ EXPORTS = [
     "dkm_bruteforce", "dkm_search_batch_device_stats", "dkm_autotune",
     "dkm_server_start", "dkm_server_slots", "dkm_server_submit",
     "dkm_server_submit_batch", "dkm_server_poll", "dkm_server_pending", "dkm_server_stop",
+    "dkm_index_shape", "dkm_bin_write", "dkm_bin_read_shape", "dkm_bin_read",
+    "dkm_set_vectors_from_bin", "dkm_save_graph", "dkm_save_graph_disk", "dkm_load_graph",
+    "dkm_pq_pivots_write", "dkm_pq_pivots_read_shape", "dkm_pq_pivots_read",
 ]
 
 _lib = None
@@ -42,6 +46,7 @@ def load():
 
 def _declare(L):
     P, I64, U32, I = C.c_void_p, C.c_int64, C.c_uint32, C.c_int
+    P64, S = C.POINTER(C.c_int64), C.c_char_p
     L.dkm_last_error.restype = C.c_char_p
     L.dkm_version.restype = C.c_char_p
     sig = {
@@ -50,6 +55,7 @@ def _declare(L):
         "dkm_device_free": [P],
         "dkm_memcpy_h2d": [P, P, C.c_size_t],
         "dkm_memcpy_d2h": [P, P, C.c_size_t],
+        "dkm_device_copy": [P, P, C.c_size_t],
         "dkm_synchronize": [],
         "dkm_distance_batch": [I, I64, P, I64, P, I64, P, I64, P],
         "dkm_pq_chunk_offsets": [I64, I64, P],
@@ -82,6 +88,17 @@ def _declare(L):
         "dkm_server_poll": [P, I64, P, P, P, P, P, P, P, C.POINTER(C.c_int64)],
         "dkm_server_pending": [P, C.POINTER(C.c_int64)],
         "dkm_server_stop": [P],
+        "dkm_index_shape": [P, P64, P64, P64, P64, P64],
+        "dkm_bin_write": [S, P, I64, I64, I64],
+        "dkm_bin_read_shape": [S, P64, P64],
+        "dkm_bin_read": [S, P, I64, I64, I64],
+        "dkm_set_vectors_from_bin": [P, S],
+        "dkm_save_graph": [P, S, U32],
+        "dkm_save_graph_disk": [P, S, U32],
+        "dkm_load_graph": [P, S, C.POINTER(C.c_uint32)],
+        "dkm_pq_pivots_write": [S, P, I64, I64, P, P, I64],
+        "dkm_pq_pivots_read_shape": [S, P64, P64, P64],
+        "dkm_pq_pivots_read": [S, P, I64, I64, P, I64],
         "dkm_search_batch_device_stats": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P,
                                           P],
     }

@@ -71,6 +71,14 @@ struct DevBuf {
 
 }  // namespace
 
+namespace dkm {
+// Records the thread's error message for dkm_last_error (other source files of the library).
+__attribute__((visibility("hidden"))) dkm_status set_error(dkm_status s, const char *msg) {
+    g_err = msg;
+    return s;
+}
+}  // namespace dkm
+
 struct dkm_index {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -135,6 +143,16 @@ dkm_status dkm_memcpy_h2d(void *dst, const void *src, size_t bytes) {
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return DKM_OK;
 }
+dkm_status dkm_device_copy(void *dst, const void *src, size_t bytes) {
+    if ((!dst || !src) && bytes) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return fail(DKM_ERR_INVALID_ARG, "copy needs 16-byte multiples and alignment");
+    HIP_TRY(hipSetDevice(g_device));
+    HIP_TRY(dkm::copy_device(dst, src, bytes, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    return DKM_OK;
+}
+
 dkm_status dkm_memcpy_d2h(void *dst, const void *src, size_t bytes) {
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return DKM_OK;
@@ -347,6 +365,17 @@ void dkm_index_destroy(dkm_index *ix) {
     ix->s_status.release();
     if (ix->stream) (void)hipStreamDestroy(ix->stream);
     delete ix;
+}
+
+dkm_status dkm_index_shape(dkm_index *ix, int64_t *dim, int64_t *max_points, int64_t *num_start,
+                           int64_t *R, int64_t *R_max) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    if (dim) *dim = ix->dim;
+    if (max_points) *max_points = ix->max_points;
+    if (num_start) *num_start = ix->num_start;
+    if (R) *R = ix->R;
+    if (R_max) *R_max = ix->R_max;
+    return DKM_OK;
 }
 
 dkm_status dkm_set_vectors(dkm_index *ix, const float *x, int64_t row0, int64_t n) {

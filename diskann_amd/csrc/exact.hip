@@ -4,6 +4,7 @@
 //                    order, mean cast to f32, first row with the smallest SquaredL2,
 //                    strict <, starting from f32::MAX)
 //   brute force      exact top-k in f64 (ties: smaller id first)
+#include <algorithm>
 #include "dkm_device.h"
 #include "dkm_exact.h"
 
@@ -157,6 +158,21 @@ hipError_t bruteforce_device(int metric, const float *x, int64_t n, int64_t stri
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(bruteforce_kernel, dim3((unsigned)nq), dim3(256), lds, stream, metric, x, n,
                        stride, dim, queries, k, out_ids, out_d);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void copy_kernel(uint4 *__restrict__ dst,
+                                                   const uint4 *__restrict__ src, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t copy_device(void *dst, const void *src, size_t bytes, hipStream_t stream) {
+    const size_t n = bytes / 16;
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, stream, (uint4 *)dst,
+                       (const uint4 *)src, n);
     return hipGetLastError();
 }
 

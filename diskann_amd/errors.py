@@ -16,6 +16,7 @@ KMEANS = 10
 INDEX = 11
 CAPACITY = 12
 NOT_READY = 13
+IO = 14
 
 
 class ANNError(RuntimeError):
@@ -40,10 +41,14 @@ class CapacityError(ANNError):
     pass
 
 
+class StorageError(ANNError, OSError):
+    """ANNErrorKind::IOError: a file is missing, short or malformed."""
+
+
 _KIND = {
     K_ZERO: KnnSearchError, L_ZERO: KnnSearchError, L_LESS_THAN_K: KnnSearchError,
     BEAM_WIDTH_ZERO: KnnSearchError, DIM_MISMATCH: DimensionMismatchError, PQ: PQError,
-    KMEANS: PQError, CAPACITY: CapacityError,
+    KMEANS: PQError, CAPACITY: CapacityError, IO: StorageError,
 }
 
 

@@ -101,6 +101,26 @@ class DiskANNIndex:
             raise ValueError("expected (n, %d) float32 vectors" % self.dim)
         check(self.lib.dkm_set_vectors(self._h, _p(x), row0, x.shape[0]), self.lib)
 
+    def set_vectors_from_bin(self, path):
+        """Streams an f32 .bin file into the data rows."""
+        check(self.lib.dkm_set_vectors_from_bin(self._h, str(path).encode()), self.lib)
+
+    def save_graph(self, path, start_point=None):
+        """The in-memory layout (every row, start points last; save_direct)."""
+        sp = self.max_points if start_point is None else int(start_point)
+        check(self.lib.dkm_save_graph(self._h, str(path).encode(), sp), self.lib)
+
+    def save_graph_disk(self, path, actual_start_point):
+        """The disk-index layout (data rows only, start id remapped; DiskAdaptor)."""
+        check(self.lib.dkm_save_graph_disk(self._h, str(path).encode(), int(actual_start_point)),
+              self.lib)
+
+    def load_graph(self, path):
+        """Loads a graph file in the in-memory layout; returns the header's start id."""
+        sp = C.c_uint32()
+        check(self.lib.dkm_load_graph(self._h, str(path).encode(), C.byref(sp)), self.lib)
+        return sp.value
+
     def set_start_points(self, vectors):
         v = np.ascontiguousarray(np.asarray(vectors, dtype=np.float32).reshape(self.num_start,
                                                                                self.dim))

@@ -43,7 +43,8 @@ typedef enum {
     DKM_ERR_KMEANS = 10,            /* ANNErrorKind::KMeansError */
     DKM_ERR_INDEX = 11,             /* ANNErrorKind::IndexError (bad id / graph) */
     DKM_ERR_CAPACITY = 12,          /* a per-query device structure overflowed */
-    DKM_ERR_NOT_READY = 13          /* index lacks data the request needs */
+    DKM_ERR_NOT_READY = 13,         /* index lacks data the request needs */
+    DKM_ERR_IO = 14                 /* ANNErrorKind::IOError (file missing, short, malformed) */
 } dkm_status;
 
 /* Metric numbering of diskann-vector/src/distance/metric.rs:11-20 (repr(C)). */
@@ -68,6 +69,9 @@ dkm_status dkm_device_alloc(size_t bytes, void **ptr);
 dkm_status dkm_device_free(void *ptr);
 dkm_status dkm_memcpy_h2d(void *dst, const void *src, size_t bytes);
 dkm_status dkm_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy through a plain streaming kernel (synchronous); bytes a multiple of
+ * 16, pointers 16-byte aligned. Also the reference point for memory-traffic counters. */
+dkm_status dkm_device_copy(void *dst, const void *src, size_t bytes);
 dkm_status dkm_synchronize(void);
 
 /* -------------------------------------------------------------------------------- */
@@ -224,6 +228,40 @@ dkm_status dkm_server_pending(dkm_server *sv, int64_t *pending);
 /* Waits for the launches in flight, then releases the server and its memory; results
  * not yet returned by dkm_server_poll are dropped. */
 dkm_status dkm_server_stop(dkm_server *sv);
+
+/* Index shape as created (any output may be NULL). */
+dkm_status dkm_index_shape(dkm_index *ix, int64_t *dim, int64_t *max_points, int64_t *num_start,
+                           int64_t *R, int64_t *R_max);
+
+/* -------------------------------------------------------------------------------- */
+/* On-disk formats (host side; little-endian)                                       */
+/* -------------------------------------------------------------------------------- */
+/* .bin matrix: u32 npoints, u32 ndims, then row-major values of elem_size bytes
+ * (diskann-utils/src/io.rs:60-100 write_bin / read_bin). */
+dkm_status dkm_bin_write(const char *path, const void *data, int64_t elem_size, int64_t nrows,
+                         int64_t ncols);
+dkm_status dkm_bin_read_shape(const char *path, int64_t *nrows, int64_t *ncols);
+dkm_status dkm_bin_read(const char *path, void *out, int64_t elem_size, int64_t nrows,
+                        int64_t ncols);
+/* Streams an f32 .bin (<= max_points rows of the index's dim) into the data rows. */
+dkm_status dkm_set_vectors_from_bin(dkm_index *ix, const char *path);
+/* Graph file (diskann-providers/src/storage/bin.rs:310-380 save_graph, :230-308
+ * load_graph). dkm_save_graph / dkm_load_graph are the in-memory index's save_direct /
+ * load_direct (simple_neighbor_provider.rs:143-185): every row, start points last, the
+ * header's max degree = R_max. dkm_save_graph_disk is the disk-index adaptor
+ * (simple_neighbor_provider.rs:313-356): data rows only, the in-memory start id replaced
+ * by actual_start_point, no start points in the header, observed max degree. */
+dkm_status dkm_save_graph(dkm_index *ix, const char *path, uint32_t start_point);
+dkm_status dkm_save_graph_disk(dkm_index *ix, const char *path, uint32_t actual_start_point);
+dkm_status dkm_load_graph(dkm_index *ix, const char *path, uint32_t *start_point);
+/* PQ pivots file (diskann-providers/src/storage/pq_storage.rs:93-149 write_pivot_data,
+ * :278-349 load_pq_pivots_bin): centroid may be NULL (written as zeros); on read a
+ * non-zero centroid is added to every pivot row. PQ codes files are u8 .bin matrices. */
+dkm_status dkm_pq_pivots_write(const char *path, const float *pivots, int64_t K, int64_t dim,
+                               const float *centroid, const int64_t *offsets, int64_t M);
+dkm_status dkm_pq_pivots_read_shape(const char *path, int64_t *K, int64_t *dim, int64_t *M);
+dkm_status dkm_pq_pivots_read(const char *path, float *pivots, int64_t K, int64_t dim,
+                              int64_t *offsets, int64_t M);
 
 /* Tuning knobs (autotuner surface; none changes results): visited-set slots in LDS per
  * query (a multiple of 64 in 256..32768, 0 = automatic) and the PQ distance mode (0 = lookup table in LDS,

@@ -71,6 +71,19 @@ def recall_at_10(ids, gt):
     return hit / (10.0 * ids.shape[0])
 
 
+def profile_copy(lib, gib):
+    """Copies `gib` GiB twice with the library's plain copy kernel: the reference point that
+    scales the memory-traffic counters of a profiled run."""
+    if gib <= 0:
+        return
+    from diskann_amd.errors import check
+    nb = int(gib * (1 << 30)) // 16 * 16
+    a, b = DeviceBuf(lib, nbytes=nb), DeviceBuf(lib, nbytes=nb)
+    for _ in range(2):
+        check(lib.dkm_device_copy(b.ptr, a.ptr, nb), lib)
+    log("copied %d bytes twice" % nb)
+
+
 def serve_drain(sv, want, arrival, t0, lat_ms, ids_out, limit_s=60.0):
     """Polls until `want` more results arrive; latency is measured from each request's
     arrival time (not its submit time)."""
@@ -156,6 +169,10 @@ def main():
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--cpu-seconds", type=float, default=1.5,
                     help="wall-clock budget of the CPU baseline sample")
+    ap.add_argument("--fixed", default="",
+                    help="L,beam_width,visited_slots: skip the L sweep and the autotuner")
+    ap.add_argument("--timed-only", action="store_true",
+                    help="stop after the timed steps (profiling runs)")
     ap.add_argument("--profile-copy-gib", type=float, default=0.0,
                     help="after the run, copy this many GiB with the library's plain copy "
                          "kernel (reference point for memory-traffic counters)")
@@ -220,43 +237,51 @@ def main():
     def sync():
         check(lib.dkm_index_synchronize(h), lib)
 
-    # recall sweep at beam width 1: smallest L reaching recall@10 >= 0.95
-    sweep = {}
-    L_star = None
-    for L in (32, 64, 128, 192, 256, 384, 512, 768):
-        set_L(L)
-        launch(L)
-        sync()
-        rec = recall_at_10(dids.get(np.uint32, (NQ, K)), gt)
-        sweep[L] = rec
-        log("L=%d recall@10=%.4f" % (L, rec))
-        if L >= 128 and rec >= 0.95:
-            L_star = L
-            break
-    if L_star is None:
-        L_star = max(sweep, key=lambda l: sweep[l])
-    set_L(L_star)
+    if args.fixed:
+        L_star, BW, HSLOTS = (int(v) for v in args.fixed.split(","))
+        ADC = 1
+        sweep = {}
+        set_L(L_star)
+        ix.set_tuning(HSLOTS, ADC)
+        BWH[0] = BW
+    else:
+        # recall sweep at beam width 1: smallest L reaching recall@10 >= 0.95
+        sweep = {}
+        L_star = None
+        for L in (32, 64, 128, 192, 256, 384, 512, 768):
+            set_L(L)
+            launch(L)
+            sync()
+            rec = recall_at_10(dids.get(np.uint32, (NQ, K)), gt)
+            sweep[L] = rec
+            log("L=%d recall@10=%.4f" % (L, rec))
+            if L >= 128 and rec >= 0.95:
+                L_star = L
+                break
+        if L_star is None:
+            L_star = max(sweep, key=lambda l: sweep[l])
+        set_L(L_star)
 
-    # beam width and result-preserving kernel knobs: autotuned on the query batch
-    tune_q = qs
-    best = None
-    for bw in (1, 2, 4, 8):
-        hs, am = ix.autotune(tune_q, K, L_star, bw, dk.Hybrid)
-        launch(L_star, bw=bw)
-        sync()
-        rec_bw = recall_at_10(dids.get(np.uint32, (NQ, K)), gt)
-        t0 = time.perf_counter()
-        for _ in range(2):
+        # beam width and result-preserving kernel knobs: autotuned on the query batch
+        tune_q = qs
+        best = None
+        for bw in (1, 2, 4, 8):
+            hs, am = ix.autotune(tune_q, K, L_star, bw, dk.Hybrid)
             launch(L_star, bw=bw)
-        sync()
-        dt = (time.perf_counter() - t0) / 2
-        log("bw=%d visited_slots=%d adc_mode=%d recall=%.4f qps=%.0f" % (bw, hs, am, rec_bw,
-                                                                      NQ / dt))
-        if rec_bw >= min(0.95, sweep[L_star]) and (best is None or dt < best[0]):
-            best = (dt, bw, hs, am)
-    _, BW, HSLOTS, ADC = best
-    ix.set_tuning(HSLOTS, ADC)
-    BWH[0] = BW
+            sync()
+            rec_bw = recall_at_10(dids.get(np.uint32, (NQ, K)), gt)
+            t0 = time.perf_counter()
+            for _ in range(2):
+                launch(L_star, bw=bw)
+            sync()
+            dt = (time.perf_counter() - t0) / 2
+            log("bw=%d visited_slots=%d adc_mode=%d recall=%.4f qps=%.0f" % (bw, hs, am, rec_bw,
+                                                                          NQ / dt))
+            if rec_bw >= min(0.95, sweep[L_star]) and (best is None or dt < best[0]):
+                best = (dt, bw, hs, am)
+        _, BW, HSLOTS, ADC = best
+        ix.set_tuning(HSLOTS, ADC)
+        BWH[0] = BW
 
     # timed batched steps
     for _ in range(args.warmup):
@@ -287,6 +312,14 @@ def main():
     bytes_per_q = cmps * (M + 4) + hops * 4 + degs * 4 + 256 * D * 4 + (L_star + 1) * D * 4
     alg_bytes = float(bytes_per_q.sum())
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+
+    if args.timed_only:
+        print(json.dumps({"timed_only": True, "queries_per_s": qps, "kernel_ms": kernel_ms,
+                          "L": L_star, "beam_width": BW, "visited_lds_slots": HSLOTS,
+                          "recall_at_10": rec, "algorithmic_bytes_per_launch": alg_bytes,
+                          "achieved_gbs": achieved}), flush=True)
+        profile_copy(lib, args.profile_copy_gib)
+        return
 
     # plain per-query GPU path: one query per launch, synchronised; it gets its own
     # latency-oriented setting (a full LDS visited table) when that is faster
@@ -329,11 +362,15 @@ def main():
     cpu_qps = ncpu / cwall
     parity = bool(np.array_equal(cids, ids[:ncpu]) and np.array_equal(ccm, cmps[:ncpu]))
 
+    # HBM bytes per launch from the rocprofv3 counter runs (profiles/), when they were
+    # taken at the settings chosen here
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_hybrid_1M.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tr = json.load(f)
+        if (tr.get("L"), tr.get("beam_width"), tr.get("visited_lds_slots")) == (L_star, BW, HSLOTS):
+            traffic = tr.get("hbm_bytes_per_launch")
 
     out = {
         "metric": "QPS at recall@10=0.95 + p99 latency, served batched vs plain per-query GPU",
@@ -383,12 +420,7 @@ def main():
         },
     }
     print(json.dumps(out), flush=True)
-    if args.profile_copy_gib > 0:
-        nb = int(args.profile_copy_gib * (1 << 30)) // 16 * 16
-        a, b = DeviceBuf(lib, nbytes=nb), DeviceBuf(lib, nbytes=nb)
-        for _ in range(2):
-            check(lib.dkm_device_copy(b.ptr, a.ptr, nb), lib)
-        log("copied %d bytes twice" % nb)
+    profile_copy(lib, args.profile_copy_gib)
 
 
 if __name__ == "__main__":

@@ -101,7 +101,7 @@ struct dkm_index {
     // visited-set spill
     uint32_t *d_spill = nullptr;
     int64_t spill_regions = 0;          // allocated regions (one per query of a launch)
-    uint64_t spill_slots = 1u << 14;
+    uint64_t spill_slots = 0;           // slots per region (power of two)
     int64_t tune_hslots = 0;
     int tune_adc_mode = 1;              // PQ distances: 0 = LDS table, 1 = direct
     int pq_len = 0;                     // uniform PQ chunk width usable by the fast path
@@ -541,18 +541,30 @@ dkm_status dkm_get_codes(dkm_index *ix, uint8_t *codes, int64_t row0, int64_t n)
 // hides the gather latency better than a large LDS table does (measured on MI355X).
 static uint32_t default_visited_slots(uint64_t L) {
     (void)L;
-    return 256;
+    return 512;
 }
 
-// Per-query visited-set spill regions (HBM) for `nq` queries; a query clears its own
-// region when it starts using it.
-static dkm_status ensure_spill(dkm_index *ix, int64_t nq) {
-    if (ix->spill_regions < nq) {
+// Slots of one query's HBM visited region for searches up to list size L: a power of two
+// of at least 16 L (a query visits about 8 L ids at these graph degrees; the table stays
+// near half full) and at least 16384. A query that still fills it reports
+// DKM_ERR_CAPACITY.
+static uint64_t region_slots(uint64_t L) {
+    uint64_t s = 16384;
+    while (s < 16 * L) s <<= 1;
+    return s;
+}
+
+// Per-query visited-set regions (HBM) for `nq` queries of `slots` entries each; a query
+// clears its own region when it starts using it.
+static dkm_status ensure_spill(dkm_index *ix, int64_t nq, uint64_t slots) {
+    if (ix->spill_regions < nq || ix->spill_slots < slots) {
         if (ix->d_spill) HIP_TRY(hipFree(ix->d_spill));
         ix->d_spill = nullptr;
         ix->spill_regions = 0;
-        HIP_TRY(hipMalloc(&ix->d_spill, sizeof(uint32_t) * ix->spill_slots * (size_t)nq));
+        const uint64_t s = std::max(slots, ix->spill_slots);
+        HIP_TRY(hipMalloc(&ix->d_spill, sizeof(uint32_t) * s * (size_t)nq));
         ix->spill_regions = nq;
+        ix->spill_slots = s;
     }
     return DKM_OK;
 }
@@ -577,7 +589,7 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
         return fail(DKM_ERR_PQ, "cosine PQ traversal (direct distances) is not supported");
     if (nq == 0) return DKM_OK;
     {
-        dkm_status st = ensure_spill(ix, nq);
+        dkm_status st = ensure_spill(ix, nq, region_slots(max_L));
         if (st) return st;
     }
     dkm::SearchParams p{};
@@ -865,7 +877,7 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
         HIP_TRY(dkm::launch_gather_rows(ix->d_vec, ix->vstride, (int)ix->dim, bid.p, nb, bq.p, ix->stream));
         // 1. candidate generation: search the pre-batch graph, record expanded nodes
         {
-            dkm_status st = ensure_spill(ix, nb);
+            dkm_status st = ensure_spill(ix, nb, region_slots(L_build));
             if (st) return st;
         }
         dkm::SearchParams sp{};
@@ -1161,7 +1173,7 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
     p.qcap = (int)(max_L + ix->num_start);
     p.nlcap = (int)(beam_width * ix->R_max + 64);
     p.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots : default_visited_slots(max_L);
-    p.spill_slots = ix->spill_slots;
+    p.spill_slots = region_slots(max_L);
     p.max_iters = (uint32_t)std::min<int64_t>(ix->rows + 16, 0x7FFFFFFF);
     if (dkm::search_lds_bytes(p) > 160 * 1024)
         return fail(DKM_ERR_CAPACITY, "per-query LDS for max_L %u exceeds 160 KiB", max_L);
@@ -1203,7 +1215,7 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
         if (e == hipSuccess) e = hipMalloc(&l.d_cmps, sizeof(uint32_t) * P);
         if (e == hipSuccess) e = hipMalloc(&l.d_hops, sizeof(uint32_t) * P);
         if (e == hipSuccess) e = hipMalloc(&l.d_status, sizeof(uint32_t) * P);
-        if (e == hipSuccess) e = hipMalloc(&l.d_spill, sizeof(uint32_t) * ix->spill_slots * P);
+        if (e == hipSuccess) e = hipMalloc(&l.d_spill, sizeof(uint32_t) * p.spill_slots * P);
     }
     for (ServeLane &l : sv->lane)
         if (e == hipSuccess && l.stream) e = hipStreamSynchronize(l.stream);

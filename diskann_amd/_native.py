@@ -25,6 +25,7 @@ EXPORTS = [
     "dkm_index_shape", "dkm_bin_write", "dkm_bin_read_shape", "dkm_bin_read",
     "dkm_set_vectors_from_bin", "dkm_save_graph", "dkm_save_graph_disk", "dkm_load_graph",
     "dkm_pq_pivots_write", "dkm_pq_pivots_read_shape", "dkm_pq_pivots_read",
+    "dkm_kmeans_device", "dkm_kmeans_assign_device",
 ]
 
 _lib = None
@@ -99,6 +100,9 @@ def _declare(L):
         "dkm_pq_pivots_write": [S, P, I64, I64, P, P, I64],
         "dkm_pq_pivots_read_shape": [S, P64, P64, P64],
         "dkm_pq_pivots_read": [S, P, I64, I64, P, I64],
+        "dkm_kmeans_device": [P, I64, I64, I64, I64, I, P, P, C.POINTER(C.c_double),
+                              C.POINTER(C.c_double)],
+        "dkm_kmeans_assign_device": [P, I64, I64, P, I64, I, P, P],
         "dkm_search_batch_device_stats": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P,
                                           P],
     }

@@ -22,6 +22,7 @@
 #include "dkm_build.h"
 #include "dkm_exact.h"
 #include "dkm_kernels.h"
+#include "dkm_kmeans.h"
 #include "dkm_search.h"
 
 namespace {
@@ -1349,6 +1350,103 @@ dkm_status dkm_server_stop(dkm_server *sv) {
     }
     server_free(sv);
     if (e != hipSuccess) return fail(DKM_ERR_DEVICE, "serving stream: %s", hipGetErrorString(e));
+    return DKM_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Large-dimension k-means
+// ------------------------------------------------------------------------------------
+
+static dkm_status kmeans_check(const float *d_x, int64_t n, int64_t dim, const float *d_c,
+                               int64_t K, int precision) {
+    if (!d_x || !d_c) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (n <= 0 || dim <= 0 || K <= 0 || n > 0x7FFFFFFF || K > 0x7FFFFFFF || dim > 4096)
+        return fail(DKM_ERR_INVALID_ARG, "bad k-means shape");
+    if (precision != DKM_PRECISION_F32 && precision != DKM_PRECISION_BF16)
+        return fail(DKM_ERR_INVALID_ARG, "unknown precision %d", precision);
+    if (precision == DKM_PRECISION_BF16 && dkm::kmeans_mfma_lds((int)((dim + 15) / 16 * 16)) > 160 * 1024)
+        return fail(DKM_ERR_INVALID_ARG, "dimension %lld too large for the BF16 tile", (long long)dim);
+    return DKM_OK;
+}
+
+dkm_status dkm_kmeans_assign_device(const float *d_x, int64_t n, int64_t dim, const float *d_c,
+                                    int64_t K, int precision, uint32_t *d_assign,
+                                    float *d_dist) {
+    dkm_status st = kmeans_check(d_x, n, dim, d_c, K, precision);
+    if (st) return st;
+    if (!d_assign) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    HIP_TRY(hipSetDevice(g_device));
+    DevBuf<float> cn;
+    HIP_TRY(cn.ensure(K));
+    HIP_TRY(dkm::kmeans_norms(d_c, K, (int)dim, dim, cn.p, nullptr));
+    if (precision == DKM_PRECISION_F32) {
+        HIP_TRY(dkm::kmeans_assign_f32(d_x, n, (int)dim, dim, d_c, (int)K, cn.p, d_assign, d_dist,
+                                       nullptr));
+    } else {
+        const int dp = (int)((dim + 15) / 16 * 16);
+        DevBuf<uint16_t> xb, cb;
+        HIP_TRY(xb.ensure(n * dp));
+        HIP_TRY(cb.ensure(K * dp));
+        HIP_TRY(dkm::kmeans_to_bf16(d_x, n, (int)dim, dim, dp, xb.p, nullptr));
+        HIP_TRY(dkm::kmeans_to_bf16(d_c, K, (int)dim, dim, dp, cb.p, nullptr));
+        HIP_TRY(dkm::kmeans_assign_mfma(xb.p, n, dp, cb.p, cn.p, (int)K, d_assign, d_dist, nullptr));
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    return DKM_OK;
+}
+
+dkm_status dkm_kmeans_device(const float *d_x, int64_t n, int64_t dim, int64_t K, int64_t iters,
+                             int precision, float *d_centroids, uint32_t *d_assign,
+                             double *ms_assign, double *ms_update) {
+    dkm_status st = kmeans_check(d_x, n, dim, d_centroids, K, precision);
+    if (st) return st;
+    if (!d_assign || iters < 0) return fail(DKM_ERR_INVALID_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(g_device));
+    const int dp = (int)((dim + 15) / 16 * 16);
+    const bool bf = precision == DKM_PRECISION_BF16;
+    DevBuf<float> cn;
+    DevBuf<uint16_t> xb, cb;
+    DevBuf<uint64_t> keys, keys_tmp;
+    DevBuf<int64_t> start;
+    DevBuf<uint8_t> sort_tmp;
+    const size_t sort_bytes = dkm::kmeans_sort_tmp_bytes(n);
+    HIP_TRY(cn.ensure(K));
+    HIP_TRY(keys.ensure(n));
+    HIP_TRY(keys_tmp.ensure(n));
+    HIP_TRY(start.ensure(K + 1));
+    HIP_TRY(sort_tmp.ensure(std::max<size_t>(sort_bytes, 1)));
+    if (bf) {
+        HIP_TRY(xb.ensure(n * dp));
+        HIP_TRY(cb.ensure(K * dp));
+        HIP_TRY(dkm::kmeans_to_bf16(d_x, n, (int)dim, dim, dp, xb.p, nullptr));
+    }
+    hipEvent_t ev[3];
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    double ta = 0.0, tu = 0.0;
+    for (int64_t it = 0; it < iters; ++it) {
+        HIP_TRY(hipEventRecord(ev[0], nullptr));
+        HIP_TRY(dkm::kmeans_norms(d_centroids, K, (int)dim, dim, cn.p, nullptr));
+        if (bf) {
+            HIP_TRY(dkm::kmeans_to_bf16(d_centroids, K, (int)dim, dim, dp, cb.p, nullptr));
+            HIP_TRY(dkm::kmeans_assign_mfma(xb.p, n, dp, cb.p, cn.p, (int)K, d_assign, nullptr, nullptr));
+        } else {
+            HIP_TRY(dkm::kmeans_assign_f32(d_x, n, (int)dim, dim, d_centroids, (int)K, cn.p, d_assign,
+                                           nullptr, nullptr));
+        }
+        HIP_TRY(hipEventRecord(ev[1], nullptr));
+        HIP_TRY(dkm::kmeans_update(d_x, n, (int)dim, dim, d_assign, (int)K, keys.p, keys_tmp.p,
+                                   sort_tmp.p, sort_bytes, start.p, d_centroids, nullptr));
+        HIP_TRY(hipEventRecord(ev[2], nullptr));
+        HIP_TRY(hipEventSynchronize(ev[2]));
+        float a = 0.0f, u = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
+        HIP_TRY(hipEventElapsedTime(&u, ev[1], ev[2]));
+        ta += a;
+        tu += u;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (ms_assign) *ms_assign = ta;
+    if (ms_update) *ms_update = tu;
     return DKM_OK;
 }
 

@@ -215,4 +215,38 @@ __device__ __forceinline__ int prefix_count(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
 }
 
+// square_norm (common.rs:9-65) of a strided span of `len` floats.
+__device__ inline float square_norm_ref(const float *x, int len) {
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int i = 0;
+    if (i + 32 <= len) {
+        float a[32];
+#pragma unroll
+        for (int p = 0; p < 32; ++p) a[p] = 0.0f;
+        while (i + 32 <= len) {
+#pragma unroll
+            for (int p = 0; p < 32; ++p) a[p] = fma_(x[i + p], x[i + p], a[p]);
+            i += 32;
+        }
+#pragma unroll
+        for (int l = 0; l < 8; ++l) s[l] = (a[l] + a[8 + l]) + (a[16 + l] + a[24 + l]);
+    }
+    while (i + 8 <= len) {
+#pragma unroll
+        for (int l = 0; l < 8; ++l) s[l] = fma_(x[i + l], x[i + l], s[l]);
+        i += 8;
+    }
+    const int r = len - i;
+    if (r != 0) {
+#pragma unroll
+        for (int l = 0; l < 8; ++l) {
+            float v = l < r ? x[i + l] : 0.0f;
+            s[l] = fma_(v, v, s[l]);
+        }
+    }
+    float a0 = s[0] + s[4], a1 = s[1] + s[5], a2 = s[2] + s[6], a3 = s[3] + s[7];
+    return (a0 + a2) + (a1 + a3);
+}
+
+
 }  // namespace dkm

@@ -143,39 +143,6 @@ hipError_t launch_pq_encode(const float *pivots, int K, int dim, const int32_t *
 // PQ training
 // ---------------------------------------------------------------------------------
 
-// square_norm (common.rs:9-65) of a strided span of `len` floats.
-__device__ inline float square_norm_dev(const float *x, int len) {
-    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int i = 0;
-    if (i + 32 <= len) {
-        float a[32];
-#pragma unroll
-        for (int p = 0; p < 32; ++p) a[p] = 0.0f;
-        while (i + 32 <= len) {
-#pragma unroll
-            for (int p = 0; p < 32; ++p) a[p] = fma_(x[i + p], x[i + p], a[p]);
-            i += 32;
-        }
-#pragma unroll
-        for (int l = 0; l < 8; ++l) s[l] = (a[l] + a[8 + l]) + (a[16 + l] + a[24 + l]);
-    }
-    while (i + 8 <= len) {
-#pragma unroll
-        for (int l = 0; l < 8; ++l) s[l] = fma_(x[i + l], x[i + l], s[l]);
-        i += 8;
-    }
-    const int r = len - i;
-    if (r != 0) {
-#pragma unroll
-        for (int l = 0; l < 8; ++l) {
-            float v = l < r ? x[i + l] : 0.0f;
-            s[l] = fma_(v, v, s[l]);
-        }
-    }
-    float a0 = s[0] + s[4], a1 = s[1] + s[5], a2 = s[2] + s[6], a3 = s[3] + s[7];
-    return (a0 + a2) + (a1 + a3);
-}
-
 __device__ __host__ inline uint64_t splitmix64(uint64_t x) {
     x += 0x9e3779b97f4a7c15ull;
     x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -196,7 +163,7 @@ __global__ void chunk_norms_kernel(const float *x, int64_t n, int dim, const int
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int o0 = offsets[m], o1 = offsets[m + 1];
-    norms[(int64_t)m * n + r] = square_norm_dev(x + r * dim + o0, o1 - o0);
+    norms[(int64_t)m * n + r] = square_norm_ref(x + r * dim + o0, o1 - o0);
 }
 
 // k-means++ seeding, one 256-thread block per chunk; rows split in 256 contiguous
@@ -317,7 +284,7 @@ __global__ __launch_bounds__(256) void lloyd_assign_kernel(const float *x, int64
         cs[e] = cent[c * maxds + d];
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < K; c += blockDim.x) cn[c] = square_norm_dev(cs + c * ds, ds);
+    for (int c = threadIdx.x; c < K; c += blockDim.x) cn[c] = square_norm_ref(cs + c * ds, ds);
     __syncthreads();
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (r >= n) return;

@@ -74,3 +74,59 @@ def distance_batch(metric, queries, base, ids):
     check(lib.dkm_distance_batch(int(metric), q.shape[1], _p(q), q.shape[0], _p(b), b.shape[0],
                                  _p(ids), ids.shape[1], _p(out)), lib)
     return out
+
+
+F32 = 0   # k-means assignment in the reference's FP32 arithmetic
+BF16 = 1  # k-means assignment as a BF16 MFMA GEMM with FP32 accumulation
+
+
+class _Dev:
+    def __init__(self, lib, arr=None, nbytes=None):
+        self.lib = lib
+        self.nbytes = arr.nbytes if arr is not None else nbytes
+        self.ptr = C.c_void_p()
+        check(lib.dkm_device_alloc(max(self.nbytes, 1), C.byref(self.ptr)), lib)
+        if arr is not None:
+            check(lib.dkm_memcpy_h2d(self.ptr, C.c_void_p(arr.ctypes.data), self.nbytes), lib)
+
+    def get(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        check(self.lib.dkm_memcpy_d2h(C.c_void_p(out.ctypes.data), self.ptr, out.nbytes), self.lib)
+        return out
+
+    def __del__(self):
+        try:
+            self.lib.dkm_device_free(self.ptr)
+        except Exception:
+            pass
+
+
+def kmeans(data, init, iters, precision=F32):
+    """Lloyd iterations from the given initial centres (lloyds.rs): returns (centres,
+    assignment, assign_ms, update_ms). precision F32 reproduces the reference arithmetic;
+    BF16 computes the assignment's dot products with bf16 MFMA."""
+    lib = _native.load()
+    x = np.ascontiguousarray(data, dtype=np.float32)
+    c = np.ascontiguousarray(init, dtype=np.float32)
+    n, dim = x.shape
+    K = c.shape[0]
+    dx, dc = _Dev(lib, x), _Dev(lib, c)
+    da = _Dev(lib, nbytes=4 * n)
+    ta, tu = C.c_double(), C.c_double()
+    check(lib.dkm_kmeans_device(dx.ptr, n, dim, K, int(iters), int(precision), dc.ptr, da.ptr,
+                                C.byref(ta), C.byref(tu)), lib)
+    return dc.get(np.float32, (K, dim)), da.get(np.uint32, n), ta.value, tu.value
+
+
+def kmeans_assign(data, centres, precision=F32):
+    """Nearest centre per row and its score (the FP32 score includes x.x; the BF16 score
+    is c.c - 2 x.c)."""
+    lib = _native.load()
+    x = np.ascontiguousarray(data, dtype=np.float32)
+    c = np.ascontiguousarray(centres, dtype=np.float32)
+    n, dim = x.shape
+    dx, dc = _Dev(lib, x), _Dev(lib, c)
+    da, dd = _Dev(lib, nbytes=4 * n), _Dev(lib, nbytes=4 * n)
+    check(lib.dkm_kmeans_assign_device(dx.ptr, n, dim, dc.ptr, c.shape[0], int(precision), da.ptr,
+                                       dd.ptr), lib)
+    return da.get(np.uint32, n), dd.get(np.float32, n)

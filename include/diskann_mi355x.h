@@ -263,6 +263,28 @@ dkm_status dkm_pq_pivots_read_shape(const char *path, int64_t *K, int64_t *dim, 
 dkm_status dkm_pq_pivots_read(const char *path, float *pivots, int64_t K, int64_t dim,
                               int64_t *offsets, int64_t M);
 
+/* -------------------------------------------------------------------------------- */
+/* Large-dimension k-means (FP32 reference assignment or BF16 MFMA assignment)       */
+/* -------------------------------------------------------------------------------- */
+/* Lloyd iterations of diskann-quantization/src/algorithms/kmeans/lloyds.rs:28-426 on
+ * device-resident f32 data (n x dim, row-major): assignment by c.c - x.c - x.c + x.x,
+ * strict < in centre order; update = f64 sums in row order / max(count, 1) (empty
+ * clusters become zero). DKM_PRECISION_F32 computes the assignment in the reference's
+ * FP32 arithmetic (sequential FMA over the dimensions); DKM_PRECISION_BF16 rounds data
+ * and centres to bf16 and computes x.c as an MFMA GEMM with FP32 accumulation (c.c stays
+ * FP32), ties to the smaller centre index. d_centroids holds the K initial centres on
+ * entry and the final ones on return; ms_* receive the summed device time of each step
+ * (may be NULL). Synchronous. */
+typedef enum { DKM_PRECISION_F32 = 0, DKM_PRECISION_BF16 = 1 } dkm_precision;
+dkm_status dkm_kmeans_device(const float *d_x, int64_t n, int64_t dim, int64_t K, int64_t iters,
+                             int precision, float *d_centroids, uint32_t *d_assign,
+                             double *ms_assign, double *ms_update);
+/* One assignment pass (the nearest centre and its score c.c - 2 x.c + x.x, or without the
+ * x.x term for BF16) for the error report against FP32 / FP64. d_dist may be NULL. */
+dkm_status dkm_kmeans_assign_device(const float *d_x, int64_t n, int64_t dim, const float *d_c,
+                                    int64_t K, int precision, uint32_t *d_assign,
+                                    float *d_dist);
+
 /* Tuning knobs (autotuner surface; none changes results): visited-set slots in LDS per
  * query (a multiple of 64 in 256..32768, 0 = automatic) and the PQ distance mode (0 = lookup table in LDS,
  * 1 = direct recomputation from the pivots, the default). */

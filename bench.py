@@ -160,6 +160,85 @@ def serve_measure(dk, ix, qs, K, L, BW, batch_ids):
     return out
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def kmeans_workload(args):
+    """1M x 768 unit-norm vectors (1,024-component mixture), K = 1,024, 20 Lloyd iterations:
+    assignment as a BF16 MFMA GEMM with FP32 accumulation, FP64 centroid update. Reports
+    MFMA throughput against the dense BF16 peak and the error of the BF16 assignment
+    against the FP32 reference arithmetic and FP64 distances on a sample."""
+    import diskann_amd as dk  # noqa: F401
+    from diskann_amd import quantization as qz
+    from diskann_amd.errors import check
+    from diskann_amd import _native
+    lib = _native.load()
+    N, D, K, IT = args.n, 768, 1024, 20
+    rng = np.random.default_rng(0xD15C0003)
+    t0 = time.time()
+    centers = rng.standard_normal((1024, D)).astype(np.float32)
+    x = np.empty((N, D), np.float32)
+    for r0 in range(0, N, 100_000):
+        r1 = min(N, r0 + 100_000)
+        blk = centers[rng.integers(0, 1024, r1 - r0)] + \
+            (0.5 * rng.standard_normal((r1 - r0, D))).astype(np.float32)
+        blk /= np.linalg.norm(blk, axis=1, keepdims=True)
+        x[r0:r1] = blk
+    log("data %.1fs" % (time.time() - t0))
+    init = x[rng.choice(N, K, replace=False)].copy()
+    dx = DeviceBuf(lib, x)
+    dc = DeviceBuf(lib, init)
+    da = DeviceBuf(lib, nbytes=4 * N)
+    ta, tu = C.c_double(), C.c_double()
+    t0 = time.perf_counter()
+    check(lib.dkm_kmeans_device(dx.ptr, N, D, K, IT, qz.BF16, dc.ptr, da.ptr, C.byref(ta),
+                                C.byref(tu)), lib)
+    wall = time.perf_counter() - t0
+    assign_ms = ta.value / IT
+    update_ms = tu.value / IT
+    flops = 2.0 * N * K * D
+    tflops = flops / (assign_ms * 1e-3) / 1e12
+    cen = dc.get(np.float32, (K, D))
+    a16 = da.get(np.uint32, N)
+    # one FP32 (reference arithmetic) assignment pass with the same centres, for timing
+    # and agreement
+    da32 = DeviceBuf(lib, nbytes=4 * N)
+    dd = DeviceBuf(lib, nbytes=4 * N)
+    dcf = DeviceBuf(lib, cen)
+    t1 = time.perf_counter()
+    check(lib.dkm_kmeans_assign_device(dx.ptr, N, D, dcf.ptr, K, qz.F32, da32.ptr, dd.ptr), lib)
+    f32_ms = (time.perf_counter() - t1) * 1e3
+    a32 = da32.get(np.uint32, N)
+    # FP64 distances on a sample: gap between the BF16 choice and the exact optimum
+    samp = rng.choice(N, 2000, replace=False)
+    xs = x[samp].astype(np.float64)
+    d64 = (xs * xs).sum(1)[:, None] - 2.0 * xs @ cen.T.astype(np.float64) + \
+        (cen.astype(np.float64) ** 2).sum(1)[None, :]
+    opt = d64.min(1)
+    got16 = d64[np.arange(len(samp)), a16[samp]]
+    got32 = d64[np.arange(len(samp)), a32[samp]]
+    out = {
+        "metric": "k-means assignment throughput (BF16 MFMA, FP32 accumulate)",
+        "value": tflops, "unit": "TFLOP/s", "n_gpus": 1, "steps": IT, "warmup": 0,
+        "ms_per_step": assign_ms + update_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {
+            "workload": "k-means 1M x 768 unit-norm (1,024-component mixture), K = 1024, "
+                        "20 Lloyd iterations, BF16 MFMA assignment, FP64 update",
+            "assign_ms_per_iter": assign_ms, "update_ms_per_iter": update_ms,
+            "total_seconds": wall, "fp32_reference_assign_ms": f32_ms,
+            "bf16_vs_fp32_assignment_agreement": float(np.mean(a16 == a32)),
+            "bf16_excess_distance_rel_mean_fp64": float(np.mean((got16 - opt) / np.maximum(opt, 1e-30))),
+            "bf16_excess_distance_rel_max_fp64": float(np.max((got16 - opt) / np.maximum(opt, 1e-30))),
+            "fp32_excess_distance_rel_max_fp64": float(np.max((got32 - opt) / np.maximum(opt, 1e-30))),
+        },
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "assign_mfma_kernel<2>", "flops_per_launch": flops},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +248,9 @@ def main():
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--cpu-seconds", type=float, default=1.5,
                     help="wall-clock budget of the CPU baseline sample")
+    ap.add_argument("--workload", default="search1m",
+                    help="search1m (default: batched Hybrid search, 1M x 128) or kmeans768 "
+                         "(BF16 MFMA k-means, 1M x 768, K = 1024, 20 iterations)")
     ap.add_argument("--fixed", default="",
                     help="L,beam_width,visited_slots: skip the L sweep and the autotuner")
     ap.add_argument("--timed-only", action="store_true",
@@ -181,6 +263,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(search shards by replicating the index; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "kmeans768":
+        return kmeans_workload(args)
 
     import diskann_amd as dk
     from diskann_amd import quantization as qz

@@ -234,7 +234,7 @@ def kmeans_workload(args):
         },
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "assign_mfma_kernel<2>", "flops_per_launch": flops},
+                     "kernel": "assign_mfma_res_kernel<48>", "flops_per_launch": flops},
     }
     print(json.dumps(out), flush=True)
 

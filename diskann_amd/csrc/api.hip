@@ -1364,9 +1364,19 @@ static dkm_status kmeans_check(const float *d_x, int64_t n, int64_t dim, const f
         return fail(DKM_ERR_INVALID_ARG, "bad k-means shape");
     if (precision != DKM_PRECISION_F32 && precision != DKM_PRECISION_BF16)
         return fail(DKM_ERR_INVALID_ARG, "unknown precision %d", precision);
-    if (precision == DKM_PRECISION_BF16 && dkm::kmeans_mfma_lds((int)((dim + 15) / 16 * 16)) > 160 * 1024)
+    if (precision == DKM_PRECISION_BF16 &&
+        dkm::kmeans_mfma_lds(dkm::kmeans_bf16_dp((int)dim), (int)K) > 160 * 1024)
         return fail(DKM_ERR_INVALID_ARG, "dimension %lld too large for the BF16 tile", (long long)dim);
     return DKM_OK;
+}
+
+// BF16 copy of rows (or centres) in the layout the assignment kernel for (dp, K) reads.
+static hipError_t bf16_copy(const float *x, int64_t rows, int64_t dim, int dp, int K, bool centres,
+                            uint16_t *out) {
+    if (dkm::kmeans_bf16_frag(dp, K))
+        return dkm::kmeans_to_bf16_frag(x, rows, (int)dim, dim, dp,
+                                        dkm::kmeans_bf16_rows(rows, dp, K, centres), out, nullptr);
+    return dkm::kmeans_to_bf16(x, rows, (int)dim, dim, dp, out, nullptr);
 }
 
 dkm_status dkm_kmeans_assign_device(const float *d_x, int64_t n, int64_t dim, const float *d_c,
@@ -1383,12 +1393,12 @@ dkm_status dkm_kmeans_assign_device(const float *d_x, int64_t n, int64_t dim, co
         HIP_TRY(dkm::kmeans_assign_f32(d_x, n, (int)dim, dim, d_c, (int)K, cn.p, d_assign, d_dist,
                                        nullptr));
     } else {
-        const int dp = (int)((dim + 15) / 16 * 16);
+        const int dp = dkm::kmeans_bf16_dp((int)dim);
         DevBuf<uint16_t> xb, cb;
-        HIP_TRY(xb.ensure(n * dp));
-        HIP_TRY(cb.ensure(K * dp));
-        HIP_TRY(dkm::kmeans_to_bf16(d_x, n, (int)dim, dim, dp, xb.p, nullptr));
-        HIP_TRY(dkm::kmeans_to_bf16(d_c, K, (int)dim, dim, dp, cb.p, nullptr));
+        HIP_TRY(xb.ensure(dkm::kmeans_bf16_rows(n, dp, (int)K, false) * dp));
+        HIP_TRY(cb.ensure(dkm::kmeans_bf16_rows(K, dp, (int)K, true) * dp));
+        HIP_TRY(bf16_copy(d_x, n, dim, dp, (int)K, false, xb.p));
+        HIP_TRY(bf16_copy(d_c, K, dim, dp, (int)K, true, cb.p));
         HIP_TRY(dkm::kmeans_assign_mfma(xb.p, n, dp, cb.p, cn.p, (int)K, d_assign, d_dist, nullptr));
     }
     HIP_TRY(hipDeviceSynchronize());
@@ -1402,23 +1412,23 @@ dkm_status dkm_kmeans_device(const float *d_x, int64_t n, int64_t dim, int64_t K
     if (st) return st;
     if (!d_assign || iters < 0) return fail(DKM_ERR_INVALID_ARG, "bad argument");
     HIP_TRY(hipSetDevice(g_device));
-    const int dp = (int)((dim + 15) / 16 * 16);
+    const int dp = dkm::kmeans_bf16_dp((int)dim);
     const bool bf = precision == DKM_PRECISION_BF16;
     DevBuf<float> cn;
     DevBuf<uint16_t> xb, cb;
     DevBuf<uint64_t> keys, keys_tmp;
     DevBuf<int64_t> start;
     DevBuf<uint8_t> sort_tmp;
-    const size_t sort_bytes = dkm::kmeans_sort_tmp_bytes(n);
+    const size_t sort_bytes = dkm::kmeans_sort_tmp_bytes(n, (int)K);
     HIP_TRY(cn.ensure(K));
     HIP_TRY(keys.ensure(n));
     HIP_TRY(keys_tmp.ensure(n));
     HIP_TRY(start.ensure(K + 1));
     HIP_TRY(sort_tmp.ensure(std::max<size_t>(sort_bytes, 1)));
     if (bf) {
-        HIP_TRY(xb.ensure(n * dp));
-        HIP_TRY(cb.ensure(K * dp));
-        HIP_TRY(dkm::kmeans_to_bf16(d_x, n, (int)dim, dim, dp, xb.p, nullptr));
+        HIP_TRY(xb.ensure(dkm::kmeans_bf16_rows(n, dp, (int)K, false) * dp));
+        HIP_TRY(cb.ensure(dkm::kmeans_bf16_rows(K, dp, (int)K, true) * dp));
+        HIP_TRY(bf16_copy(d_x, n, dim, dp, (int)K, false, xb.p));
     }
     hipEvent_t ev[3];
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
@@ -1427,7 +1437,7 @@ dkm_status dkm_kmeans_device(const float *d_x, int64_t n, int64_t dim, int64_t K
         HIP_TRY(hipEventRecord(ev[0], nullptr));
         HIP_TRY(dkm::kmeans_norms(d_centroids, K, (int)dim, dim, cn.p, nullptr));
         if (bf) {
-            HIP_TRY(dkm::kmeans_to_bf16(d_centroids, K, (int)dim, dim, dp, cb.p, nullptr));
+            HIP_TRY(bf16_copy(d_centroids, K, dim, dp, (int)K, true, cb.p));
             HIP_TRY(dkm::kmeans_assign_mfma(xb.p, n, dp, cb.p, cn.p, (int)K, d_assign, nullptr, nullptr));
         } else {
             HIP_TRY(dkm::kmeans_assign_f32(d_x, n, (int)dim, dim, d_centroids, (int)K, cn.p, d_assign,

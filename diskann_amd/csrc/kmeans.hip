@@ -38,6 +38,32 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float *x, int64_t ro
     }
 }
 
+// rows x dim f32 -> the MFMA fragment layout of the register-resident kernel: 16-byte chunk
+// ((t * ks + s) * 64 + l) holds row 32 t + l % 32, dims 16 s + 8 (l / 32) .. + 7 (zero past
+// `rows` and `dim`), so every 1 KB block is one k-step's operand fragment of 32 rows.
+__global__ __launch_bounds__(256) void to_bf16_frag_kernel(const float *x, int64_t rows, int dim,
+                                                           int64_t stride, int ks,
+                                                           int64_t chunks, uint16_t *out) {
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < chunks;
+         c += (int64_t)gridDim.x * 256) {
+        const int64_t t = c / (ks * 64);
+        const int rem = (int)(c - t * ks * 64);
+        const int s = rem >> 6, l = rem & 63;
+        const int64_t r = 32 * t + (l & 31);
+        const int d0 = 16 * s + 8 * (l >> 5);
+        uint16_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            v[j] = (r < rows && d0 + j < dim) ? to_bf16(x[r * stride + d0 + j]) : (uint16_t)0;
+        uint4 w;
+        w.x = v[0] | ((uint32_t)v[1] << 16);
+        w.y = v[2] | ((uint32_t)v[3] << 16);
+        w.z = v[4] | ((uint32_t)v[5] << 16);
+        w.w = v[6] | ((uint32_t)v[7] << 16);
+        *(uint4 *)(out + 8 * c) = w;
+    }
+}
+
 // Squared norm in the reference order (f32x8 partials, as square_norm).
 __global__ __launch_bounds__(256) void row_norms_kernel(const float *x, int64_t rows, int dim,
                                                         int64_t stride, float *out) {
@@ -199,6 +225,166 @@ __global__ __launch_bounds__(256) void assign_mfma_kernel(const uint16_t *xb, in
     }
 }
 
+// BF16 MFMA assignment with the rows resident in registers (dims up to 768). Rows and centres
+// are stored in the fragment layout (to_bf16_frag_kernel), so each operand load is one
+// contiguous 1 KB block. A workgroup of four waves owns 256 rows; wave w keeps the B-operand
+// fragments of two 32-row tiles for the whole padded dimension (KS k-steps of 16: lane l
+// holds row l % 32, dims 16 s + 8 (l / 32) .. + 7), so X is read from memory exactly once,
+// and every centre fragment read from LDS feeds two MFMAs. The centres stream through LDS in
+// tiles of 32 (KS blocks of 1 KB, block s being the A-operand fragment of k-step s), loaded
+// straight into LDS by global_load_lds (16 bytes a lane) and double-buffered so the next tile
+// loads while the current one multiplies. v_mfma_f32_32x32x16_bf16(centres, rows) leaves
+// accumulator r of lane l = x.c of row l % 32 and centre 8 (r / 4) + 4 (l / 32) + r % 4 of
+// the tile; the accumulators start at -c.c / 2, so they end at -score / 2, and each lane
+// tracks its row's largest value over its centres in increasing index order (strict >); the
+// two half-waves are merged at the end, ties to the smaller centre.
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void global_void_t;
+
+// Wave w loads blocks w, w + 4, ... of a tile; `src` is the tile's first block + 8 lane. The
+// loads are kept together, ahead of the tile's LDS reads (each sets M0, and the compiler
+// drains outstanding LDS reads before an M0 write).
+template <int KS>
+__device__ __forceinline__ void res_prefetch(const uint16_t *src, uint16_t *buf, int wave) {
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int s = 0; s < KS / 4; ++s)
+        __builtin_amdgcn_global_load_lds((global_void_t *)(src + (4 * s + wave) * 512),
+                                         (lds_void_t *)(buf + (4 * s + wave) * 512), 16, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// The centre fragments are read from LDS in batches of B k-steps: batch j + 1 is issued right
+// after the first MFMA pair of batch j, so the one wait before each batch finds its reads
+// (issued a batch earlier) complete. The wait is a full LDS drain: the compiler counts the
+// in-flight global_load_lds of the next tile as outstanding LDS work and cannot wait for a
+// partial count. Scheduling group barriers pin the order.
+template <int KS>
+__device__ __forceinline__ void res_tile(const bf16x8_t *a0, const bf16x8_t *a1,
+                                         const uint16_t *buf, floatx16_t &acc0, floatx16_t &acc1) {
+    constexpr int B = KS % 4 == 0 && KS < 48 ? 4 : 3;
+    static_assert(KS % B == 0, "k-steps per tile");
+    bf16x8_t b[2 * B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) b[i] = *(const bf16x8_t *)(buf + i * 512);
+    __builtin_amdgcn_sched_group_barrier(0x100, B, 0);
+#pragma unroll
+    for (int j = 0; j < KS / B; ++j) {
+        bf16x8_t *cur = b + (j & 1) * B, *nxt = b + ((j + 1) & 1) * B;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int s = j * B + i;
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur[i], a0[s], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur[i], a1[s], acc1, 0, 0, 0);
+            if (i == 0 && j + 1 < KS / B) {
+#pragma unroll
+                for (int q = 0; q < B; ++q) nxt[q] = *(const bf16x8_t *)(buf + (s + B + q) * 512);
+            }
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        if (j + 1 < KS / B) __builtin_amdgcn_sched_group_barrier(0x100, B, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 2 * B - 2, 0);
+    }
+}
+
+// Both row tiles' accumulators start at -c.c / 2 of the lane's 16 centres of the tile at c0.
+__device__ __forceinline__ void res_init(const float *cn, int c0, floatx16_t &acc0,
+                                         floatx16_t &acc1) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const float4 c4 = *(const float4 *)(cn + c0 + 8 * g);
+        acc0[4 * g + 0] = -0.5f * c4.x;
+        acc0[4 * g + 1] = -0.5f * c4.y;
+        acc0[4 * g + 2] = -0.5f * c4.z;
+        acc0[4 * g + 3] = -0.5f * c4.w;
+    }
+    acc1 = acc0;
+}
+
+// Running best of one row: the largest x.c - c.c / 2 (strict >, centres in increasing order,
+// so the first of equal scores is kept).
+__device__ __forceinline__ void res_epilogue(const floatx16_t &acc, int c0, float &bs,
+                                             uint32_t &bc) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float v = acc[4 * g + q];
+            if (v > bs) {
+                bs = v;
+                bc = (uint32_t)(c0 + 8 * g + q);
+            }
+        }
+    }
+}
+
+// xf: rows in fragment layout (padded to a multiple of 256 rows), cf: centres in fragment
+// layout (padded to Kp, a multiple of 64). LDS: two centre tiles (2 x KS KB) and the centre
+// norms padded to Kp with +inf.
+template <int KS>
+__global__ __launch_bounds__(256, 1) void assign_mfma_res_kernel(const uint16_t *xf, int64_t n,
+                                                                 const uint16_t *cf,
+                                                                 const float *cnorm, int K, int Kp,
+                                                                 uint32_t *assign,
+                                                                 float *best_out) {
+    extern __shared__ __align__(16) uint16_t sm[];
+    float *cn_s = (float *)(sm + 2 * KS * 512);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int64_t rtile = (int64_t)blockIdx.x * 8 + 2 * wave;
+    const uint16_t *ap = xf + rtile * KS * 512 + lane * 8;
+    bf16x8_t a0[KS], a1[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) a0[s] = *(const bf16x8_t *)(ap + s * 512);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) a1[s] = *(const bf16x8_t *)(ap + (KS + s) * 512);
+    // Where the row fragments live: the first tile's and eight of the second's in AGPRs (MFMA
+    // reads its B operand from there directly), the rest in VGPRs beside the centre-fragment
+    // batches. The empty asm statements only constrain the register class.
+#pragma unroll
+    for (int s = 0; s < KS; ++s) asm volatile("" : "+a"(a0[s]));
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        if (s < 8) asm volatile("" : "+a"(a1[s]));
+        else asm volatile("" : "+v"(a1[s]));
+    }
+    for (int c = tid; c < Kp; c += 256) cn_s[c] = c < K ? cnorm[c] : __builtin_inff();
+    float bs0 = -__builtin_inff(), bs1 = -__builtin_inff();
+    uint32_t bc0 = 0xFFFFFFFFu, bc1 = 0xFFFFFFFFu;
+    const int ntiles = Kp / 32;
+    auto src = [&](int t) { return cf + (int64_t)t * KS * 512 + lane * 8; };
+    floatx16_t pa, pb;
+    res_prefetch<KS>(src(0), sm, wave);
+    for (int t = 0; t < ntiles; ++t) {
+        __builtin_amdgcn_s_waitcnt(0);  // own loads of tile t landed
+        __syncthreads();                // everyone's, and tile t - 1 fully read
+        uint16_t *cur = sm + (t & 1) * KS * 512;
+        res_init(cn_s, 32 * t + 4 * half, pa, pb);
+        if (t + 1 < ntiles) res_prefetch<KS>(src(t + 1), sm + ((t + 1) & 1) * KS * 512, wave);
+        res_tile<KS>(a0, a1, cur + lane * 8, pa, pb);
+        res_epilogue(pa, 32 * t + 4 * half, bs0, bc0);
+        res_epilogue(pb, 32 * t + 4 * half, bs1, bc1);
+    }
+    const float bsv[2] = {bs0, bs1};
+    const uint32_t bcv[2] = {bc0, bc1};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        float bs = bsv[h];
+        uint32_t bc = bcv[h];
+        const float os = __shfl_xor(bs, 32);
+        const uint32_t oc = __shfl_xor(bc, 32);
+        if (os > bs || (os == bs && oc < bc)) {
+            bs = os;
+            bc = oc;
+        }
+        const int64_t row = 32 * (rtile + h) + l32;
+        if (half == 0 && row < n) {
+            assign[row] = bc;
+            if (best_out) best_out[row] = -2.0f * bs;
+        }
+    }
+}
+
 // (cluster << 32 | row) keys, sorted, give every cluster's rows in increasing order.
 __global__ __launch_bounds__(256) void cluster_keys_kernel(const uint32_t *assign, int64_t n,
                                                            uint64_t *keys) {
@@ -220,29 +406,29 @@ __global__ __launch_bounds__(256) void cluster_starts_kernel(const uint64_t *key
     start[c] = lo;
 }
 
-// One workgroup per cluster: f64 sums of its rows in row order, divided by max(count, 1).
+// Centroid update: workgroup (k, y) sums dims 256 y .. + 255 of cluster k's rows in f64, in
+// increasing row order (one dim per thread; eight rows' loads are issued before their adds so
+// the order of the sums is unchanged), and divides by max(count, 1).
 __global__ __launch_bounds__(256) void update_kernel(const float *x, int dim, int64_t stride,
                                                     const uint64_t *keys, const int64_t *start,
                                                     float *c) {
     const int k = blockIdx.x;
+    const int d = blockIdx.y * 256 + threadIdx.x;
     const int64_t b = start[k], e = start[k + 1];
     const double den = (double)(e - b > 1 ? e - b : 1);
-    for (int d0 = 0; d0 < dim; d0 += 256 * 4) {
-        double s[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int64_t i = b; i < e; ++i) {
-            const float *xr = x + (int64_t)(keys[i] & 0xFFFFFFFFull) * stride;
+    const int dd = d < dim ? d : dim - 1;
+    double s = 0.0;
+    int64_t i = b;
+    for (; i + 8 <= e; i += 8) {
+        float v[8];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int d = d0 + threadIdx.x + 256 * q;
-                if (d < dim) s[q] += (double)xr[d];
-            }
-        }
+        for (int q = 0; q < 8; ++q)
+            v[q] = x[(int64_t)(keys[i + q] & 0xFFFFFFFFull) * stride + dd];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int d = d0 + threadIdx.x + 256 * q;
-            if (d < dim) c[(int64_t)k * dim + d] = (float)(s[q] / den);
-        }
+        for (int q = 0; q < 8; ++q) s += (double)v[q];
     }
+    for (; i < e; ++i) s += (double)x[(int64_t)(keys[i] & 0xFFFFFFFFull) * stride + dd];
+    if (d < dim) c[(int64_t)k * dim + d] = (float)(s / den);
 }
 
 static unsigned grid_for(int64_t n, int64_t per) { return (unsigned)((n + per - 1) / per); }
@@ -252,6 +438,15 @@ hipError_t kmeans_to_bf16(const float *x, int64_t rows, int dim, int64_t stride,
     const int64_t total = rows * dp;
     const unsigned g = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
     hipLaunchKernelGGL(to_bf16_kernel, dim3(std::max(g, 1u)), dim3(256), 0, st, x, rows, dim, stride, dp, out);
+    return hipGetLastError();
+}
+
+hipError_t kmeans_to_bf16_frag(const float *x, int64_t rows, int dim, int64_t stride, int dp,
+                               int64_t rows_pad, uint16_t *out, hipStream_t st) {
+    const int64_t chunks = rows_pad * dp / 8;
+    const unsigned g = (unsigned)std::min<int64_t>((chunks + 255) / 256, 65536);
+    hipLaunchKernelGGL(to_bf16_frag_kernel, dim3(std::max(g, 1u)), dim3(256), 0, st, x, rows, dim,
+                       stride, dp / 16, chunks, out);
     return hipGetLastError();
 }
 
@@ -272,16 +467,61 @@ hipError_t kmeans_assign_f32(const float *x, int64_t n, int dim, int64_t stride,
     return hipGetLastError();
 }
 
-size_t kmeans_mfma_lds(int dp) {
+static constexpr int kResSteps[] = {4, 8, 16, 24, 32, 48};
+
+int kmeans_bf16_dp(int dim) {
+    for (int ks : kResSteps)
+        if (16 * ks >= dim) return 16 * ks;
+    return (dim + 15) / 16 * 16;
+}
+
+static int64_t res_kp(int K) { return ((int64_t)K + 63) / 64 * 64; }
+
+static bool use_resident(int dp, int K) {
+    return dp <= 768 && dp % 64 == 0 && (size_t)2 * dp * 64 + 4 * (size_t)res_kp(K) <= 160 * 1024;
+}
+
+bool kmeans_bf16_frag(int dp, int K) { return use_resident(dp, K); }
+
+int64_t kmeans_bf16_rows(int64_t n, int dp, int K, bool centres) {
+    if (!use_resident(dp, K)) return n;
+    return centres ? res_kp((int)n) : (n + 255) / 256 * 256;
+}
+
+size_t kmeans_mfma_lds(int dp, int K) {
+    if (use_resident(dp, K)) return (size_t)2 * dp * 64 + 4 * (size_t)res_kp(K);
     const int rb = (size_t)64 * (dp + 8) * 2 <= 128 * 1024 ? 2 : 1;
     return std::max<size_t>((size_t)rb * 32 * (dp + 8) * 2, (size_t)8 * rb * 32 * 4);
+}
+
+template <int KS>
+static hipError_t launch_res(const uint16_t *xb, int64_t n, const uint16_t *cb, const float *cnorm,
+                             int K, uint32_t *assign, float *best, hipStream_t st) {
+    const size_t lds = kmeans_mfma_lds(KS * 16, K);
+    hipError_t e = hipFuncSetAttribute((const void *)assign_mfma_res_kernel<KS>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(assign_mfma_res_kernel<KS>, dim3(grid_for(n, 256)), dim3(256), lds, st, xb,
+                       n, cb, cnorm, K, (int)res_kp(K), assign, best);
+    return hipGetLastError();
 }
 
 hipError_t kmeans_assign_mfma(const uint16_t *xb, int64_t n, int dp, const uint16_t *cb,
                               const float *cnorm, int K, uint32_t *assign, float *best,
                               hipStream_t st) {
     if (n == 0) return hipSuccess;
-    const size_t lds = kmeans_mfma_lds(dp);
+    if (use_resident(dp, K)) {
+        switch (dp / 16) {
+        case 4: return launch_res<4>(xb, n, cb, cnorm, K, assign, best, st);
+        case 8: return launch_res<8>(xb, n, cb, cnorm, K, assign, best, st);
+        case 16: return launch_res<16>(xb, n, cb, cnorm, K, assign, best, st);
+        case 24: return launch_res<24>(xb, n, cb, cnorm, K, assign, best, st);
+        case 32: return launch_res<32>(xb, n, cb, cnorm, K, assign, best, st);
+        case 48: return launch_res<48>(xb, n, cb, cnorm, K, assign, best, st);
+        default: break;
+        }
+    }
+    const size_t lds = kmeans_mfma_lds(dp, K);
     if ((size_t)64 * (dp + 8) * 2 <= 128 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)assign_mfma_kernel<2>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -298,10 +538,17 @@ hipError_t kmeans_assign_mfma(const uint16_t *xb, int64_t n, int dp, const uint1
     return hipGetLastError();
 }
 
-size_t kmeans_sort_tmp_bytes(int64_t n) {
+// Keys (cluster << 32 | row) only carry 32 + ceil(log2 K) significant bits.
+static unsigned key_bits(int K) {
+    unsigned b = 1;
+    while ((1ull << b) < (uint64_t)K) ++b;
+    return 32 + b;
+}
+
+size_t kmeans_sort_tmp_bytes(int64_t n, int K) {
     size_t bytes = 0;
     (void)rocprim::radix_sort_keys(nullptr, bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
-                                   (unsigned)n, 0, 64);
+                                   (unsigned)n, 0, key_bits(K));
     return bytes;
 }
 
@@ -314,14 +561,16 @@ hipError_t kmeans_update(const float *x, int64_t n, int dim, int64_t stride,
                            keys_tmp);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        e = rocprim::radix_sort_keys(sort_tmp, sort_bytes, keys_tmp, keys, (unsigned)n, 0, 64, st);
+        e = rocprim::radix_sort_keys(sort_tmp, sort_bytes, keys_tmp, keys, (unsigned)n, 0,
+                                     key_bits(K), st);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(cluster_starts_kernel, dim3(grid_for(K + 1, 256)), dim3(256), 0, st, keys, n,
                        K, start);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(update_kernel, dim3(K), dim3(256), 0, st, x, dim, stride, keys, start, c);
+    hipLaunchKernelGGL(update_kernel, dim3(K, (dim + 255) / 256), dim3(256), 0, st, x, dim, stride,
+                       keys, start, c);
     return hipGetLastError();
 }
 

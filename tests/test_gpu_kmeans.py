@@ -29,16 +29,25 @@ def test_f32_assign_matches_oracle(n, dim, K):
     assert np.array_equal(a, ao)
 
 
-@pytest.mark.parametrize("n,dim,K", [(2000, 64, 40), (500, 50, 70), (300, 768, 130)])
+@pytest.mark.parametrize("n,dim,K", [(2000, 64, 40), (500, 50, 70), (300, 768, 130),
+                                     (1000, 256, 300), (129, 384, 33), (70, 512, 64),
+                                     (257, 100, 1), (150, 800, 40)])
 def test_bf16_assign_exact_on_integer_data(n, dim, K):
     # small integers: every bf16 input, product and partial sum is exact, so the MFMA
-    # scores equal the FP32 ones and the arg-min (ties to the first centre) must agree
+    # scores equal the FP32 ones and the arg-min (ties to the first centre) must agree.
+    # Dims up to 768 take the register-resident kernel (padded to 64 .. 768), 800 the
+    # LDS-tiled one; K = 1, 33, 300 leave partial centre tiles.
+    log("kmeans bf16 exact n=%d dim=%d K=%d" % (n, dim, K))
     rng = np.random.default_rng(2)
     x = rng.integers(-4, 5, (n, dim)).astype(np.float32)
     c = rng.integers(-4, 5, (K, dim)).astype(np.float32)
-    a16, _ = qz.kmeans_assign(x, c, qz.BF16)
+    a16, d16 = qz.kmeans_assign(x, c, qz.BF16)
     ao, _ = po.lloyd_assign(x, c)
     assert np.array_equal(a16, ao)
+    # the reported score is c.c - 2 x.c of the chosen centre (exact here)
+    cx = c.astype(np.float64)[a16]
+    want = (cx * cx).sum(1) - 2.0 * (x.astype(np.float64) * cx).sum(1)
+    assert np.array_equal(d16.astype(np.float64), want)
 
 
 def test_bf16_assign_close_on_clustered_data():

@@ -262,7 +262,7 @@ __device__ __forceinline__ void res_prefetch(const uint16_t *src, uint16_t *buf,
 template <int KS>
 __device__ __forceinline__ void res_tile(const bf16x8_t *a0, const bf16x8_t *a1,
                                          const uint16_t *buf, floatx16_t &acc0, floatx16_t &acc1) {
-    constexpr int B = KS % 4 == 0 && KS < 48 ? 4 : 3;
+    constexpr int B = KS % 8 == 0 ? 8 : 4;
     static_assert(KS % B == 0, "k-steps per tile");
     bf16x8_t b[2 * B];
 #pragma unroll

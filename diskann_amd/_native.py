@@ -25,7 +25,7 @@ EXPORTS = [
     "dkm_index_shape", "dkm_bin_write", "dkm_bin_read_shape", "dkm_bin_read",
     "dkm_set_vectors_from_bin", "dkm_save_graph", "dkm_save_graph_disk", "dkm_load_graph",
     "dkm_pq_pivots_write", "dkm_pq_pivots_read_shape", "dkm_pq_pivots_read",
-    "dkm_kmeans_device", "dkm_kmeans_assign_device",
+    "dkm_kmeans_device", "dkm_kmeans_assign_device", "dkm_pq_distance_batch",
 ]
 
 _lib = None
@@ -63,6 +63,7 @@ def _declare(L):
         "dkm_pq_lut": [I, P, I64, I64, P, I64, P, I64, P],
         "dkm_pq_encode": [P, I64, I64, P, I64, P, I64, P],
         "dkm_pq_train": [P, I64, I64, I64, I64, I64, C.c_uint64, P, P],
+        "dkm_pq_distance_batch": [I, P, I64, I64, P, I64, P, P, I64, P, I64, P, I64, P],
         "dkm_index_create": [I, I64, I64, I64, I64, I64, C.POINTER(C.c_void_p)],
         "dkm_set_vectors": [P, P, I64, I64],
         "dkm_set_start_points": [P, P],

@@ -54,6 +54,10 @@ template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
     hipError_t ensure(size_t count) {
         if (count <= n && p) return hipSuccess;
         if (p) (void)hipFree(p);
@@ -244,6 +248,57 @@ dkm_status dkm_pq_lut(int metric, const float *pivots, int64_t K, int64_t dim,
     (void)hipFree(dq);
     (void)hipFree(dl);
     (void)hipFree(dof);
+    return DKM_OK;
+}
+
+dkm_status dkm_pq_distance_batch(int metric, const float *pivots, int64_t K, int64_t dim,
+                                 const int64_t *offsets, int64_t M, const float *queries,
+                                 const uint8_t *query_codes, int64_t nq, const uint8_t *codes,
+                                 int64_t n, const uint32_t *ids, int64_t m, float *out) {
+    if (!metric_ok(metric)) return fail(DKM_ERR_INVALID_ARG, "unknown metric %d", metric);
+    if (K <= 0 || K > 256) return fail(DKM_ERR_PQ, "1..256 centroids supported (got %lld)", (long long)K);
+    dkm_status st = check_offsets(offsets, M, dim);
+    if (st) return st;
+    if ((queries == nullptr) == (query_codes == nullptr))
+        return fail(DKM_ERR_INVALID_ARG, "give exactly one of queries / query_codes");
+    if (nq < 0 || n <= 0 || m < 0 || !pivots || !codes || !out || (nq * m > 0 && !ids))
+        return fail(DKM_ERR_INVALID_ARG, "bad sizes");
+    if (nq == 0 || m == 0) return DKM_OK;
+    for (int64_t i = 0; i < nq * m; ++i)
+        if (ids[i] >= (uint64_t)n)
+            return fail(DKM_ERR_INDEX, "id %u out of range (%lld codes)", ids[i], (long long)n);
+    for (int64_t i = 0; i < n * M; ++i)
+        if (codes[i] >= K) return fail(DKM_ERR_PQ, "code %u >= %lld pivots", codes[i], (long long)K);
+    if (query_codes)
+        for (int64_t i = 0; i < nq * M; ++i)
+            if (query_codes[i] >= K)
+                return fail(DKM_ERR_PQ, "code %u >= %lld pivots", query_codes[i], (long long)K);
+    HIP_TRY(hipSetDevice(g_device));
+    std::vector<int32_t> o32(offsets, offsets + M + 1);
+    DevBuf<float> dp, dq, dout;
+    DevBuf<int32_t> dof;
+    DevBuf<uint8_t> dc, dqc;
+    DevBuf<uint32_t> di;
+    HIP_TRY(dp.ensure(K * dim));
+    HIP_TRY(dof.ensure(M + 1));
+    HIP_TRY(dc.ensure(n * M));
+    HIP_TRY(di.ensure(nq * m));
+    HIP_TRY(dout.ensure(nq * m));
+    HIP_TRY(hipMemcpy(dp.p, pivots, sizeof(float) * K * dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dof.p, o32.data(), sizeof(int32_t) * (M + 1), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dc.p, codes, (size_t)n * M, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(di.p, ids, sizeof(uint32_t) * nq * m, hipMemcpyHostToDevice));
+    if (queries) {
+        HIP_TRY(dq.ensure(nq * dim));
+        HIP_TRY(hipMemcpy(dq.p, queries, sizeof(float) * nq * dim, hipMemcpyHostToDevice));
+    } else {
+        HIP_TRY(dqc.ensure(nq * M));
+        HIP_TRY(hipMemcpy(dqc.p, query_codes, (size_t)nq * M, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(dkm::launch_pq_direct(metric, dp.p, (int)dim, dof.p, (int)M, queries ? dq.p : nullptr,
+                                  query_codes ? dqc.p : nullptr, nq, dc.p, di.p, m, dout.p,
+                                  nullptr));
+    HIP_TRY(hipMemcpy(out, dout.p, sizeof(float) * nq * m, hipMemcpyDeviceToHost));
     return DKM_OK;
 }
 

@@ -14,6 +14,14 @@ hipError_t launch_distance_batch(int metric, int dim, const float *queries, int6
 hipError_t launch_pq_lut(bool ip, const float *pivots, int dim, const int32_t *offsets, int M,
                          const float *queries, int64_t nq, float *lut, hipStream_t stream);
 
+// Direct PQ distances, out[i*m + j] = d(left_i, pivots selected by codes[ids[i*m + j]]):
+// left_i is the full-precision row queries[i] (query_codes == nullptr) or the pivots of
+// query_codes[i] (queries == nullptr). Resumable f32x8 accumulation over the chunks.
+hipError_t launch_pq_direct(int metric, const float *pivots, int dim, const int32_t *offsets,
+                            int M, const float *queries, const uint8_t *query_codes, int64_t nq,
+                            const uint8_t *codes, const uint32_t *ids, int64_t m, float *out,
+                            hipStream_t stream);
+
 // codes[i*code_stride + m]; *bad_row receives the smallest failing row (init 0xFFFFFFFF).
 hipError_t launch_pq_encode(const float *pivots, int K, int dim, const int32_t *offsets, int M,
                             const float *x, int64_t x_stride, int64_t n, uint8_t *codes,

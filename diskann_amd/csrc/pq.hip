@@ -62,6 +62,138 @@ hipError_t launch_distance_batch(int metric, int dim, const float *queries, int6
 }
 
 // ---------------------------------------------------------------------------------
+// Direct PQ distances (FixedChunkPQTable::{l2_distance, inner_product, cosine_distance,
+// qq_*}, diskann-providers/src/model/pq/fixed_chunk_pq_table.rs:32-56, 220-365): eight
+// lanes per pair, lane l holding lane l of the f32x8 accumulator. For each chunk the lane
+// computes its lane of the simd_op vector over the chunk's slices (chunk-local element o
+// into partial o % 32 for L2/IP with Strategy4x1, o % 16 for cosine with Strategy2x4, then
+// the masked remainder FMA), adds it to the running accumulator, and the accumulator is
+// reduced by sum_tree (shuffles by 4, 2, 1) after the last chunk.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ float down8(float v, int d) { return __shfl_down(v, d, 8); }
+
+__device__ __forceinline__ float tree8(float v) {
+    float u = v + down8(v, 4);
+    float w = u + down8(u, 2);
+    return w + down8(w, 1);
+}
+
+template <int METRIC, bool QQ>
+__global__ __launch_bounds__(256) void pq_direct_kernel(const float *pivots, int dim,
+                                                        const int32_t *offsets, int M,
+                                                        const float *queries,
+                                                        const uint8_t *query_codes, int64_t nq,
+                                                        const uint8_t *codes,
+                                                        const uint32_t *ids, int64_t m,
+                                                        float *out) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = nq * m;
+    const int64_t pair = gid >> 3;
+    const int l = (int)(gid & 7);
+    const int64_t pr = pair < total ? pair : total - 1;  // all eight lanes stay in the shuffles
+    const int64_t qi = pr / m;
+    const uint8_t *code = codes + (int64_t)ids[pr] * M;
+    float acc = 0.0f, ax = 0.0f, ay = 0.0f, axy = 0.0f;
+    for (int c = 0; c < M; ++c) {
+        const int b = offsets[c], len = offsets[c + 1] - b;
+        const float *L = QQ ? pivots + (int64_t)query_codes[qi * M + c] * dim + b
+                            : queries + qi * dim + b;
+        const float *R = pivots + (int64_t)code[c] * dim + b;
+        const int d8 = len & ~7, rem = len & 7;
+        if (METRIC == METRIC_COSINE) {
+            float nx[2] = {0.0f, 0.0f}, ny[2] = {0.0f, 0.0f}, xy[2] = {0.0f, 0.0f};
+            for (int v = 0; 8 * v < d8; ++v) {
+                const float a = L[8 * v + l], r = R[8 * v + l];
+                nx[v & 1] = fma_(a, a, nx[v & 1]);
+                ny[v & 1] = fma_(r, r, ny[v & 1]);
+                xy[v & 1] = fma_(a, r, xy[v & 1]);
+            }
+            float cx = nx[0] + nx[1], cy = ny[0] + ny[1], cxy = xy[0] + xy[1];
+            if (rem) {
+                const float a = l < rem ? L[d8 + l] : 0.0f, r = l < rem ? R[d8 + l] : 0.0f;
+                cx = fma_(a, a, cx);
+                cy = fma_(r, r, cy);
+                cxy = fma_(a, r, cxy);
+            }
+            ax = ax + cx;
+            ay = ay + cy;
+            axy = axy + cxy;
+        } else {
+            float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int v = 0; 8 * v < d8; ++v) {
+                const float a = L[8 * v + l], r = R[8 * v + l];
+                if (METRIC == METRIC_IP) {
+                    s[v & 3] = fma_(a, r, s[v & 3]);
+                } else {
+                    const float d = a - r;
+                    s[v & 3] = fma_(d, d, s[v & 3]);
+                }
+            }
+            float t = (s[0] + s[1]) + (s[2] + s[3]);
+            if (rem) {
+                const float a = l < rem ? L[d8 + l] : 0.0f, r = l < rem ? R[d8 + l] : 0.0f;
+                if (METRIC == METRIC_IP) {
+                    t = fma_(a, r, t);
+                } else {
+                    const float d = a - r;
+                    t = fma_(d, d, t);
+                }
+            }
+            acc = acc + t;
+        }
+    }
+    float res;
+    if (METRIC == METRIC_COSINE) {
+        const float normx = tree8(ax), normy = tree8(ay), prod = tree8(axy);
+        const float denom = __builtin_sqrtf(normx) * __builtin_sqrtf(normy);
+        float sim;
+        if (normx < 1.17549435e-38f || normy < 1.17549435e-38f) {
+            sim = 0.0f;
+        } else {
+            const float v = prod / denom;
+            sim = __builtin_fmaxf(-1.0f, __builtin_fminf(1.0f, v));
+        }
+        res = 1.0f - sim;
+    } else {
+        const float sum = tree8(acc);
+        res = METRIC == METRIC_IP ? -sum : sum;
+    }
+    if (pair < total && l == 0) out[pair] = res;
+}
+
+template <int METRIC>
+static void launch_direct_t(dim3 grid, hipStream_t st, const float *pivots, int dim,
+                            const int32_t *offsets, int M, const float *queries,
+                            const uint8_t *query_codes, int64_t nq, const uint8_t *codes,
+                            const uint32_t *ids, int64_t m, float *out) {
+    if (query_codes)
+        hipLaunchKernelGGL((pq_direct_kernel<METRIC, true>), grid, dim3(256), 0, st, pivots, dim,
+                           offsets, M, queries, query_codes, nq, codes, ids, m, out);
+    else
+        hipLaunchKernelGGL((pq_direct_kernel<METRIC, false>), grid, dim3(256), 0, st, pivots, dim,
+                           offsets, M, queries, query_codes, nq, codes, ids, m, out);
+}
+
+hipError_t launch_pq_direct(int metric, const float *pivots, int dim, const int32_t *offsets,
+                            int M, const float *queries, const uint8_t *query_codes, int64_t nq,
+                            const uint8_t *codes, const uint32_t *ids, int64_t m, float *out,
+                            hipStream_t stream) {
+    const int64_t pairs = nq * m;
+    if (pairs == 0) return hipSuccess;
+    dim3 grid((unsigned)((pairs * 8 + 255) / 256));
+    if (metric == METRIC_L2)
+        launch_direct_t<METRIC_L2>(grid, stream, pivots, dim, offsets, M, queries, query_codes,
+                                   nq, codes, ids, m, out);
+    else if (metric == METRIC_IP)
+        launch_direct_t<METRIC_IP>(grid, stream, pivots, dim, offsets, M, queries, query_codes,
+                                   nq, codes, ids, m, out);
+    else
+        launch_direct_t<METRIC_COSINE>(grid, stream, pivots, dim, offsets, M, queries,
+                                       query_codes, nq, codes, ids, m, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
 // PQ lookup tables
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void pq_lut_kernel(bool ip, const float *pivots, int dim,

@@ -15,7 +15,7 @@ from .errors import check
 
 
 def _p(a):
-    return C.c_void_p(a.ctypes.data)
+    return None if a is None else C.c_void_p(a.ctypes.data)
 
 
 def chunk_offsets(dim, num_chunks):
@@ -62,6 +62,28 @@ def pq_lut(metric, pivots, offsets, queries):
     check(lib.dkm_pq_lut(int(metric), _p(pivots), K, pivots.shape[1], _p(offsets), M, _p(q),
                          q.shape[0], _p(lut)), lib)
     return lut
+
+
+def pq_distance(metric, pivots, offsets, codes, ids, queries=None, query_codes=None):
+    """Direct PQ distances (pq::distance::DistanceComputer, dynamic.rs:170-205): row i of
+    the result holds d(left_i, codes[ids[i, j]]) for the full-precision query rows
+    (`queries`) or the code rows `query_codes` (code vs code), exactly one of the two."""
+    lib = _native.load()
+    pivots = np.ascontiguousarray(pivots, dtype=np.float32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    codes = np.ascontiguousarray(codes, dtype=np.uint8)
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
+    if ids.ndim == 1:
+        ids = ids[None, :]
+    nq, m = ids.shape
+    M = offsets.shape[0] - 1
+    q = None if queries is None else np.ascontiguousarray(queries, dtype=np.float32)
+    qc = None if query_codes is None else np.ascontiguousarray(query_codes, dtype=np.uint8)
+    out = np.zeros((nq, m), dtype=np.float32)
+    check(lib.dkm_pq_distance_batch(int(metric), _p(pivots), pivots.shape[0], pivots.shape[1],
+                                    _p(offsets), M, _p(q), _p(qc), nq, _p(codes), codes.shape[0],
+                                    _p(ids), m, _p(out)), lib)
+    return out
 
 
 def distance_batch(metric, queries, base, ids):

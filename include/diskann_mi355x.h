@@ -93,6 +93,19 @@ dkm_status dkm_pq_lut(int metric, const float *pivots, int64_t K, int64_t dim,
                       const int64_t *offsets, int64_t M, const float *queries, int64_t nq,
                       float *lut);
 /* compress_into: codes[i*M + m] = argmin_c SquaredL2(x_i,m , pivot_c,m). Host buffers. */
+/* Direct PQ distances, replacing pq::distance::DistanceComputer::evaluate_similarity
+ * (diskann-providers/src/model/pq/distance/dynamic.rs:170-205) over
+ * FixedChunkPQTable::{l2_distance, inner_product, cosine_distance, qq_l2_distance,
+ * qq_ip_distance, qq_cosine_distance} (fixed_chunk_pq_table.rs:32-56, 220-365), the
+ * distances of the Hybrid prune (HybridComputer, .../async_/distances.rs:118-165).
+ * out[i*m + j] = d(left_i, codes[ids[i*m + j]]) where left_i is the full-precision row
+ * queries[i] (query_codes NULL) or the pivots of query_codes[i] (queries NULL); pivots
+ * K x dim, codes n x M, chunk offsets M + 1. Cosine-normalized uses the cosine functions,
+ * as the reference's table does. Same f32 operation order as the reference. */
+dkm_status dkm_pq_distance_batch(int metric, const float *pivots, int64_t K, int64_t dim,
+                                 const int64_t *offsets, int64_t M, const float *queries,
+                                 const uint8_t *query_codes, int64_t nq, const uint8_t *codes,
+                                 int64_t n, const uint32_t *ids, int64_t m, float *out);
 dkm_status dkm_pq_encode(const float *pivots, int64_t K, int64_t dim, const int64_t *offsets,
                          int64_t M, const float *x, int64_t n, uint8_t *codes);
 /* PQ training: per-chunk k-means++ seeding (own RNG, seeded by `seed`) then `iters`

@@ -118,6 +118,128 @@ static float cosine_v3(const float *x, const float *y, int64_t len) {
     return 1.0f - sim;
 }
 
+/* The f32x8 vector a simd_op leaves before its final reduction, for one pair of slices
+ * (the per-chunk step of the resumable schemas, simd.rs:609-666): L2/IP with Strategy4x1
+ * (slice-local element o < len - len%8 into partial o % 32, (s0+s1)+(s2+s3), masked
+ * remainder FMA), cosine with Strategy2x4 (partial o % 16, s0+s1, masked remainder) for
+ * the three accumulators normx, normy, xy. */
+static void chunk_vec_4x1(int ip, const float *x, const float *y, int64_t len, float t[8]) {
+    float acc[32];
+    for (int p = 0; p < 32; ++p) acc[p] = 0.0f;
+    int64_t d8 = len - (len % 8);
+    for (int64_t e = 0; e < d8; ++e) {
+        int p = (int)(e % 32);
+        float a = x[e], b = y[e];
+        if (ip) {
+            acc[p] = fmaf(a, b, acc[p]);
+        } else {
+            float c = a - b;
+            acc[p] = fmaf(c, c, acc[p]);
+        }
+    }
+    for (int l = 0; l < 8; ++l) t[l] = (acc[l] + acc[8 + l]) + (acc[16 + l] + acc[24 + l]);
+    int64_t r = len % 8;
+    if (r != 0) {
+        for (int l = 0; l < 8; ++l) {
+            float a = l < r ? x[d8 + l] : 0.0f;
+            float b = l < r ? y[d8 + l] : 0.0f;
+            if (ip) {
+                t[l] = fmaf(a, b, t[l]);
+            } else {
+                float c = a - b;
+                t[l] = fmaf(c, c, t[l]);
+            }
+        }
+    }
+}
+
+static void chunk_vec_cos(const float *x, const float *y, int64_t len, float cx[8], float cy[8],
+                          float cxy[8]) {
+    float nx[16], ny[16], xy[16];
+    for (int p = 0; p < 16; ++p) nx[p] = ny[p] = xy[p] = 0.0f;
+    int64_t d8 = len - (len % 8);
+    for (int64_t e = 0; e < d8; ++e) {
+        int p = (int)(e % 16);
+        float a = x[e], b = y[e];
+        nx[p] = fmaf(a, a, nx[p]);
+        ny[p] = fmaf(b, b, ny[p]);
+        xy[p] = fmaf(a, b, xy[p]);
+    }
+    for (int l = 0; l < 8; ++l) {
+        cx[l] = nx[l] + nx[8 + l];
+        cy[l] = ny[l] + ny[8 + l];
+        cxy[l] = xy[l] + xy[8 + l];
+    }
+    int64_t r = len % 8;
+    if (r != 0) {
+        for (int l = 0; l < 8; ++l) {
+            float a = l < r ? x[d8 + l] : 0.0f;
+            float b = l < r ? y[d8 + l] : 0.0f;
+            cx[l] = fmaf(a, a, cx[l]);
+            cy[l] = fmaf(b, b, cy[l]);
+            cxy[l] = fmaf(a, b, cxy[l]);
+        }
+    }
+}
+
+/* FullCosineAccumulator::sum: similarity with the subnormal-norm guard and clamp. */
+static float cos_sim(const float cx[8], const float cy[8], const float cxy[8]) {
+    float normx = sum_tree8(cx);
+    float normy = sum_tree8(cy);
+    float denom = sqrtf(normx) * sqrtf(normy);
+    float prod = sum_tree8(cxy);
+    if (normx < FLT_MIN || normy < FLT_MIN) return 0.0f;
+    float v = prod / denom;
+    return fmaxf(-1.0f, fminf(1.0f, v));
+}
+
+/* Direct PQ distances, diskann-providers/src/model/pq/fixed_chunk_pq_table.rs:32-56
+ * (direct_distance_impl: full-precision vector against the pivots a code selects) and
+ * :290-365 (self_distance: the pivots of two codes), metric dispatch as in
+ * .../pq/distance/dynamic.rs:111-135. Per chunk the slices are reduced to an f32x8 vector
+ * (chunk_vec_*), which is added lane-wise into the running accumulator (zero at the
+ * start, ResumableL2/IP/Cosine::combine_with); the accumulator is reduced once at the end.
+ * L2 -> sum; inner product -> -sum; cosine and cosine-normalized -> 1 - similarity. Left
+ * operand: the full-precision vector (or the first code's pivots), right: the code's. */
+static float pq_direct_core(int metric, const float *pivots, int64_t dim, const int64_t *offsets,
+                            int64_t M, const float *q, const uint8_t *qcode,
+                            const uint8_t *code) {
+    float acc[8], ax[8], ay[8], axy[8];
+    for (int l = 0; l < 8; ++l) acc[l] = ax[l] = ay[l] = axy[l] = 0.0f;
+    const int cos = metric == DKO_COSINE || metric == DKO_COSINE_NORMALIZED;
+    for (int64_t c = 0; c < M; ++c) {
+        int64_t b = offsets[c], e = offsets[c + 1];
+        const float *left = q ? q + b : pivots + (int64_t)qcode[c] * dim + b;
+        const float *right = pivots + (int64_t)code[c] * dim + b;
+        if (cos) {
+            float cx[8], cy[8], cxy[8];
+            chunk_vec_cos(left, right, e - b, cx, cy, cxy);
+            for (int l = 0; l < 8; ++l) {
+                ax[l] = ax[l] + cx[l];
+                ay[l] = ay[l] + cy[l];
+                axy[l] = axy[l] + cxy[l];
+            }
+        } else {
+            float t[8];
+            chunk_vec_4x1(metric == DKO_INNER_PRODUCT, left, right, e - b, t);
+            for (int l = 0; l < 8; ++l) acc[l] = acc[l] + t[l];
+        }
+    }
+    if (cos) return 1.0f - cos_sim(ax, ay, axy);
+    float s = sum_tree8(acc);
+    return metric == DKO_INNER_PRODUCT ? -s : s;
+}
+
+float dko_pq_direct(int metric, const float *pivots, int64_t dim, const int64_t *offsets,
+                    int64_t M, const float *query, const uint8_t *code) {
+    return pq_direct_core(metric, pivots, dim, offsets, M, query, NULL, code);
+}
+
+float dko_pq_qq(int metric, const float *pivots, int64_t dim, const int64_t *offsets, int64_t M,
+                const uint8_t *a, const uint8_t *b) {
+    return pq_direct_core(metric, pivots, dim, offsets, M, NULL, a, b);
+}
+
 /* PostOps: diskann-vector/src/distance/implementations.rs:215-405. For f32 data the
  * DistanceProvider maps CosineNormalized to the full Cosine functor
  * (diskann-vector/src/distance/distance_provider.rs:288-291). */

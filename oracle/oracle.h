@@ -15,6 +15,10 @@
  *                             cmps and hops for beam widths 1/2/4 (reference JSON).
  *   dko_pq_lut / dko_pq_adc   pinned: test_compute_pq_distance KAT and FP64 bound.
  *   dko_pq_encode             pinned: brute-force argmin over pivots (FP64 check).
+ *   dko_pq_direct / dko_pq_qq pinned: test_quant_quant_distances KAT (seeded integer
+ *                             pivot tables: equal to the distance of the reconstructed
+ *                             vectors, L2 / IP / cosine exact), integer-query KAT for the
+ *                             full-precision side, reference tolerances on float queries.
  *   dko_chunk_offsets         pinned: partition_happy_path KAT.
  *   dko_occlude_list          pinned: update_occlude_factor KATs + invariants;
  *                             grid_insert golden (post-build searches) where the
@@ -48,6 +52,12 @@ void dko_chunk_offsets(int64_t dim, int64_t nchunks, int64_t *offsets /* nchunks
 void dko_pq_lut(int metric, const float *pivots /* K x dim */, int64_t K, int64_t dim,
                 const int64_t *offsets, int64_t M, const float *query, float *lut /* M*K */);
 float dko_pq_adc(const uint8_t *code, const float *lut, int64_t M, int64_t K);
+/* Direct PQ distances (fixed_chunk_pq_table.rs:32-56, :290-365): full-precision query vs
+ * the pivots a code selects, and code vs code, with the resumable f32x8 accumulation. */
+float dko_pq_direct(int metric, const float *pivots, int64_t dim, const int64_t *offsets,
+                    int64_t M, const float *query, const uint8_t *code);
+float dko_pq_qq(int metric, const float *pivots, int64_t dim, const int64_t *offsets, int64_t M,
+                const uint8_t *a, const uint8_t *b);
 int dko_pq_encode(const float *pivots, int64_t K, int64_t dim, const int64_t *offsets,
                   int64_t M, const float *x, int64_t n, uint8_t *codes);
 

@@ -66,6 +66,10 @@ def _setup(L):
     L.dko_pq_adc.argtypes = [P, P, I64, I64]
     L.dko_pq_adc.restype = F
     L.dko_pq_encode.argtypes = [P, I64, I64, P, I64, P, I64, P]
+    L.dko_pq_direct.argtypes = [C.c_int, P, I64, P, I64, P, P]
+    L.dko_pq_direct.restype = C.c_float
+    L.dko_pq_qq.argtypes = [C.c_int, P, I64, P, I64, P, P]
+    L.dko_pq_qq.restype = C.c_float
     L.dko_pq_encode.restype = C.c_int
     L.dko_knn_validate.argtypes = [I64, I64, I64]
     L.dko_search.argtypes = [C.POINTER(Index), C.c_int, P, I64, I64, I64, P, P,
@@ -125,6 +129,28 @@ def pq_adc(code, lut):
     code = np.ascontiguousarray(code, dtype=np.uint8)
     lut = np.ascontiguousarray(lut, dtype=np.float32)
     return lib().dko_pq_adc(_p(code), _p(lut), lut.shape[0], lut.shape[1])
+
+
+def pq_direct(metric, pivots, offsets, query, code):
+    """Full-precision query vs the pivots a code selects (fixed_chunk_pq_table.rs:32-56)."""
+    L = lib()
+    pv = np.ascontiguousarray(pivots, dtype=np.float32)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    q = np.ascontiguousarray(query, dtype=np.float32)
+    c = np.ascontiguousarray(code, dtype=np.uint8)
+    return float(L.dko_pq_direct(int(metric), _p(pv), pv.shape[1], _p(off), len(off) - 1, _p(q),
+                                 _p(c)))
+
+
+def pq_qq(metric, pivots, offsets, a, b):
+    """Code vs code through the pivot table (fixed_chunk_pq_table.rs:290-365)."""
+    L = lib()
+    pv = np.ascontiguousarray(pivots, dtype=np.float32)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    ca = np.ascontiguousarray(a, dtype=np.uint8)
+    cb = np.ascontiguousarray(b, dtype=np.uint8)
+    return float(L.dko_pq_qq(int(metric), _p(pv), pv.shape[1], _p(off), len(off) - 1, _p(ca),
+                             _p(cb)))
 
 
 def pq_encode(pivots, offsets, x):

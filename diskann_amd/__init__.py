@@ -8,11 +8,11 @@ from .errors import (ANNError, CapacityError, DimensionMismatchError, KnnSearchE
                      StorageError)
 from .index import (DiskANNIndex, FullPrecision, Hybrid, Knn, Metric, Quantized, SearchResult,
                     Server, Strategy)
-from . import quantization, storage
+from . import prescore, quantization, storage
 
 __all__ = [
     "ANNError", "CapacityError", "DimensionMismatchError", "KnnSearchError", "PQError",
-    "StorageError", "storage",
+    "StorageError", "storage", "prescore",
     "DiskANNIndex", "FullPrecision", "Hybrid", "Knn", "Metric", "Quantized", "SearchResult",
     "Server", "Strategy", "quantization",
 ]

@@ -26,6 +26,7 @@ EXPORTS = [
     "dkm_set_vectors_from_bin", "dkm_save_graph", "dkm_save_graph_disk", "dkm_load_graph",
     "dkm_pq_pivots_write", "dkm_pq_pivots_read_shape", "dkm_pq_pivots_read",
     "dkm_kmeans_device", "dkm_kmeans_assign_device", "dkm_pq_distance_batch",
+    "dkm_prescore_create", "dkm_prescore_search", "dkm_prescore_destroy",
 ]
 
 _lib = None
@@ -104,6 +105,8 @@ def _declare(L):
         "dkm_kmeans_device": [P, I64, I64, I64, I64, I, P, P, C.POINTER(C.c_double),
                               C.POINTER(C.c_double)],
         "dkm_kmeans_assign_device": [P, I64, I64, P, I64, I, P, P],
+        "dkm_prescore_create": [P, I64, I64, I, I, C.POINTER(C.c_void_p)],
+        "dkm_prescore_search": [P, P, I64, I64, I64, P, P, C.POINTER(C.c_double)],
         "dkm_search_batch_device_stats": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P,
                                           P],
     }
@@ -113,3 +116,5 @@ def _declare(L):
         f.restype = C.c_int
     L.dkm_index_destroy.argtypes = [P]
     L.dkm_index_destroy.restype = None
+    L.dkm_prescore_destroy.argtypes = [P]
+    L.dkm_prescore_destroy.restype = None

@@ -23,6 +23,7 @@
 #include "dkm_exact.h"
 #include "dkm_kernels.h"
 #include "dkm_kmeans.h"
+#include "dkm_prescore.h"
 #include "dkm_search.h"
 
 namespace {
@@ -1658,6 +1659,231 @@ dkm_status dkm_set_tuning(dkm_index *ix, int64_t visited_lds_slots, int adc_mode
     std::lock_guard<std::mutex> g(ix->mu);
     ix->tune_hslots = visited_lds_slots;
     ix->tune_adc_mode = adc_mode;
+    return DKM_OK;
+}
+
+
+// -------------------------------------------------------------------------------------
+// Candidate pre-scoring (BF16 MFMA approximate scores, exact FP32 rescore of the best C)
+// -------------------------------------------------------------------------------------
+struct dkm_prescorer {
+    int metric = 0;
+    int64_t n = 0, dim = 0, ns = 0;
+    int dp = 0;
+    const float *x = nullptr;        // caller's f32 rows (rescore)
+    DevBuf<uint16_t> xf, sf;         // bf16 fragment layout: rows, sample rows
+    DevBuf<float> xn, sn;            // f32 squared norms (L2 only)
+    std::mutex mu;
+};
+
+namespace {
+constexpr int64_t kPsBatch = 1024;  // queries per filter launch (bounds the LDS stage load)
+}
+
+dkm_status dkm_prescore_create(const float *d_x, int64_t n, int64_t dim, int metric,
+                               int precision, dkm_prescorer **out) {
+    if (!d_x || !out) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (n <= 0 || n > 0x7FFFFFFF || dim <= 0) return fail(DKM_ERR_INVALID_ARG, "bad shape");
+    if (metric != DKM_L2 && metric != DKM_INNER_PRODUCT && metric != DKM_COSINE_NORMALIZED)
+        return fail(DKM_ERR_INVALID_ARG,
+                    "pre-scoring supports L2, inner product and cosine-normalized (got %d)", metric);
+    if (precision != DKM_PRECISION_BF16)
+        return fail(DKM_ERR_INVALID_ARG, "unknown pre-scoring precision %d", precision);
+    const int dp = dkm::kmeans_bf16_dp((int)dim);
+    if (!dkm::ps_supported_dp(dp))
+        return fail(DKM_ERR_INVALID_ARG, "pre-scoring supports dims up to 768 (got %lld)",
+                    (long long)dim);
+    HIP_TRY(hipSetDevice(g_device));
+    auto *ps = new dkm_prescorer();
+    ps->metric = metric;
+    ps->n = n;
+    ps->dim = dim;
+    ps->dp = dp;
+    ps->x = d_x;
+    ps->ns = std::min<int64_t>(n, PS_SAMPLE_MAX);
+    const bool l2 = metric == DKM_L2;
+    auto bad = [&](hipError_t e) {
+        delete ps;
+        return fail(DKM_ERR_DEVICE, "%s", hipGetErrorString(e));
+    };
+    hipError_t e;
+    const int64_t npad = (n + 255) / 256 * 256, spad = (ps->ns + 255) / 256 * 256;
+    if ((e = ps->xf.ensure(npad * dp)) != hipSuccess) return bad(e);
+    if ((e = dkm::kmeans_to_bf16_frag(d_x, n, (int)dim, dim, dp, npad, ps->xf.p, nullptr)) !=
+        hipSuccess)
+        return bad(e);
+    // the sample: rows floor(i * n / ns), i < ns
+    std::vector<int64_t> rows(ps->ns);
+    for (int64_t i = 0; i < ps->ns; ++i) rows[i] = (int64_t)((__int128)i * n / ps->ns);
+    DevBuf<int64_t> drows;
+    DevBuf<float> srows;
+    if ((e = drows.ensure(ps->ns)) != hipSuccess) return bad(e);
+    if ((e = srows.ensure(ps->ns * dim)) != hipSuccess) return bad(e);
+    if ((e = hipMemcpy(drows.p, rows.data(), sizeof(int64_t) * ps->ns, hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return bad(e);
+    if ((e = dkm::ps_gather_rows(d_x, (int)dim, drows.p, ps->ns, srows.p, nullptr)) != hipSuccess)
+        return bad(e);
+    if ((e = ps->sf.ensure(spad * dp)) != hipSuccess) return bad(e);
+    if ((e = dkm::kmeans_to_bf16_frag(srows.p, ps->ns, (int)dim, dim, dp, spad, ps->sf.p,
+                                      nullptr)) != hipSuccess)
+        return bad(e);
+    if (l2) {
+        if ((e = ps->xn.ensure(n)) != hipSuccess) return bad(e);
+        if ((e = ps->sn.ensure(ps->ns)) != hipSuccess) return bad(e);
+        if ((e = dkm::ps_norms(d_x, n, (int)dim, nullptr, ps->xn.p, nullptr)) != hipSuccess)
+            return bad(e);
+        if ((e = dkm::ps_norms(d_x, ps->ns, (int)dim, drows.p, ps->sn.p, nullptr)) != hipSuccess)
+            return bad(e);
+    }
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return bad(e);
+    *out = ps;
+    return DKM_OK;
+}
+
+void dkm_prescore_destroy(dkm_prescorer *ps) {
+    if (!ps) return;
+    (void)hipSetDevice(g_device);
+    (void)hipDeviceSynchronize();
+    delete ps;
+}
+
+dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, int64_t k,
+                               int64_t candidates, uint32_t *d_ids, float *d_dists,
+                               double *ms_stages) {
+    if (!ps || !d_q || !d_ids || !d_dists) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (k <= 0) return fail(DKM_ERR_K_ZERO, "k must be positive");
+    if (nq < 0) return fail(DKM_ERR_INVALID_ARG, "bad query count");
+    const int64_t C = std::min<int64_t>(std::max<int64_t>(candidates, k), ps->n);
+    if (k > ps->n) return fail(DKM_ERR_INVALID_ARG, "k = %lld exceeds %lld rows", (long long)k,
+                               (long long)ps->n);
+    if (nq == 0) return DKM_OK;
+    std::lock_guard<std::mutex> g(ps->mu);
+    HIP_TRY(hipSetDevice(g_device));
+    const int dp = ps->dp;
+    const bool l2 = ps->metric == DKM_L2;
+    const int64_t B = std::min<int64_t>(nq, kPsBatch);
+    const int64_t Bp = (B + 63) / 64 * 64;
+    // candidate list capacity: the thresholds aim at ~1.5 C, retries at ~4.5 C / ~1.1 C
+    const int64_t cap = std::min<int64_t>(ps->n, 8 * C);
+    if (dkm::ps_filter_lds(dp, (int)Bp) > 160 * 1024)
+        return fail(DKM_ERR_INVALID_ARG, "pre-scoring tile does not fit LDS");
+    const double want = 1.5 * (double)C * (double)ps->ns / (double)ps->n;
+    const int j0 = (int)std::min<int64_t>(ps->ns - 1, std::max<int64_t>(0, (int64_t)std::ceil(want) - 1));
+    const int j_loose = (int)std::min<int64_t>(ps->ns - 1, 3 * ((int64_t)j0 + 1) - 1);
+    const int j_tight = (int)std::max<int64_t>(0, ((int64_t)j0 + 1) * 3 / 4 - 1);
+    const bool all_pass = 1.5 * (double)C >= (double)ps->n;  // tiny bases: every row is a candidate
+    DevBuf<uint16_t> qf;
+    DevBuf<float> dense, tau, dexact;
+    DevBuf<uint32_t> cnt, bnd_b, bnd_e, cids;
+    DevBuf<uint64_t> lists, sorted, fkeys, fsorted;
+    DevBuf<uint8_t> tmp;
+    size_t tmp1 = 0, tmp2 = 0;
+    HIP_TRY(dkm::ps_sort_tmp_bytes(B * cap, (int)B, &tmp1));
+    HIP_TRY(dkm::ps_sort_tmp_bytes(B * C, (int)B, &tmp2));
+    HIP_TRY(qf.ensure(Bp * dp));
+    HIP_TRY(dense.ensure(B * ps->ns));
+    HIP_TRY(tau.ensure(3 * B));
+    HIP_TRY(cnt.ensure(B));
+    HIP_TRY(bnd_b.ensure(B));
+    HIP_TRY(bnd_e.ensure(B));
+    HIP_TRY(lists.ensure(B * cap));
+    HIP_TRY(sorted.ensure(B * cap));
+    HIP_TRY(cids.ensure(B * C));
+    HIP_TRY(dexact.ensure(B * C));
+    HIP_TRY(fkeys.ensure(B * C));
+    HIP_TRY(fsorted.ensure(B * C));
+    HIP_TRY(tmp.ensure(std::max<size_t>(std::max(tmp1, tmp2), 1)));
+    hipEvent_t ev[5];
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    struct EvGuard {
+        hipEvent_t *e;
+        ~EvGuard() {
+            for (int i = 0; i < 5; ++i) (void)hipEventDestroy(e[i]);
+        }
+    } evg{ev};
+    double acc_ms[4] = {0, 0, 0, 0};
+    std::vector<uint32_t> hcnt(B);
+    std::vector<float> htau(3 * B), hretry(B);
+    for (int64_t q0 = 0; q0 < nq; q0 += B) {
+        const int b = (int)std::min<int64_t>(B, nq - q0);
+        const int bp = (int)((b + 63) / 64 * 64);
+        const float *q = d_q + q0 * ps->dim;
+        HIP_TRY(hipEventRecord(ev[0], nullptr));
+        HIP_TRY(dkm::kmeans_to_bf16_frag(q, b, (int)ps->dim, ps->dim, dp, bp, qf.p, nullptr));
+        if (!all_pass) {
+            HIP_TRY(dkm::ps_dense(dp, ps->sf.p, ps->ns, ps->sn.p, qf.p, b, bp, l2, dense.p, ps->ns,
+                                  nullptr));
+            HIP_TRY(dkm::ps_select(dense.p, ps->ns, (int)ps->ns, j0, j_loose, j_tight, tau.p, b,
+                                   nullptr));
+            HIP_TRY(hipMemcpy(htau.data(), tau.p, sizeof(float) * 3 * b, hipMemcpyDeviceToHost));
+        } else {
+            for (int i = 0; i < b; ++i) htau[i] = __builtin_inff();
+            HIP_TRY(hipMemcpy(tau.p, htau.data(), sizeof(float) * b, hipMemcpyHostToDevice));
+        }
+        HIP_TRY(hipMemset(cnt.p, 0, sizeof(uint32_t) * b));
+        HIP_TRY(dkm::ps_filter(dp, ps->xf.p, ps->n, ps->xn.p, qf.p, b, bp, l2, tau.p, cnt.p,
+                               lists.p, cap, nullptr));
+        // redo queries whose list is short (looser threshold, then everything) or overflowed
+        // (tighter threshold)
+        for (int round = 0;; ++round) {
+            HIP_TRY(hipMemcpy(hcnt.data(), cnt.p, sizeof(uint32_t) * b, hipMemcpyDeviceToHost));
+            bool redo = false;
+            for (int i = 0; i < b; ++i) {
+                hretry[i] = -__builtin_inff();
+                if (hcnt[i] < (uint32_t)C) {
+                    hretry[i] = round == 0 ? htau[b + i] : __builtin_inff();
+                    hcnt[i] = 0;
+                    redo = true;
+                } else if (hcnt[i] > (uint64_t)cap) {
+                    hretry[i] = round == 0 ? htau[2 * b + i] : -__builtin_inff();
+                    if (round > 0)
+                        return fail(DKM_ERR_CAPACITY, "query %lld: %u candidates exceed %lld",
+                                    (long long)(q0 + i), hcnt[i], (long long)cap);
+                    hcnt[i] = 0;
+                    redo = true;
+                }
+            }
+            if (!redo) break;
+            if (round == 2)
+                return fail(DKM_ERR_CAPACITY, "pre-scoring threshold did not converge");
+            HIP_TRY(hipMemcpy(cnt.p, hcnt.data(), sizeof(uint32_t) * b, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(tau.p, hretry.data(), sizeof(float) * b, hipMemcpyHostToDevice));
+            HIP_TRY(dkm::ps_filter(dp, ps->xf.p, ps->n, ps->xn.p, qf.p, b, bp, l2, tau.p, cnt.p,
+                                   lists.p, cap, nullptr));
+        }
+        HIP_TRY(hipEventRecord(ev[1], nullptr));
+        HIP_TRY(dkm::ps_bounds(cnt.p, b, cap, bnd_b.p, bnd_e.p, nullptr));
+        HIP_TRY(dkm::ps_sort(tmp.p, tmp1, lists.p, sorted.p, (int64_t)b * cap, b, bnd_b.p,
+                             bnd_e.p, nullptr));
+        HIP_TRY(dkm::ps_take(sorted.p, b, cap, (int)C, cids.p, nullptr));
+        HIP_TRY(hipEventRecord(ev[2], nullptr));
+        HIP_TRY(dkm::launch_distance_batch(ps->metric, (int)ps->dim, q, b, ps->x, ps->dim, cids.p,
+                                           C, dexact.p, nullptr));
+        HIP_TRY(hipEventRecord(ev[3], nullptr));
+        HIP_TRY(dkm::ps_final_keys(dexact.p, cids.p, (int64_t)b * C, fkeys.p, nullptr));
+        std::vector<uint32_t> fb(b), fe(b);
+        for (int i = 0; i < b; ++i) {
+            fb[i] = (uint32_t)(i * C);
+            fe[i] = (uint32_t)((i + 1) * C);
+        }
+        HIP_TRY(hipMemcpy(bnd_b.p, fb.data(), sizeof(uint32_t) * b, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(bnd_e.p, fe.data(), sizeof(uint32_t) * b, hipMemcpyHostToDevice));
+        HIP_TRY(dkm::ps_sort(tmp.p, tmp2, fkeys.p, fsorted.p, (int64_t)b * C, b, bnd_b.p, bnd_e.p,
+                             nullptr));
+        HIP_TRY(dkm::ps_output(fsorted.p, b, (int)C, (int)k, d_ids + q0 * k, d_dists + q0 * k,
+                               nullptr));
+        HIP_TRY(hipEventRecord(ev[4], nullptr));
+        HIP_TRY(hipEventSynchronize(ev[4]));
+        for (int i = 0; i < 4; ++i) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+            acc_ms[i] += ms;
+        }
+    }
+    if (ms_stages)
+        for (int i = 0; i < 4; ++i) ms_stages[i] = acc_ms[i];
     return DKM_OK;
 }
 

@@ -1,0 +1,118 @@
+// Register-resident BF16 MFMA GEMM pieces shared by the k-means assignment and the
+// candidate pre-scoring kernels (gfx950, v_mfma_f32_32x32x16_bf16).
+//
+// Operands are stored in the "fragment layout": 16-byte chunk ((t * KS + s) * 64 + l) holds
+// row 32 t + l % 32, dims 16 s + 8 (l / 32) .. + 7, so each 1 KB block is one k-step's
+// operand fragment of 32 rows and every operand load is contiguous. A workgroup of four
+// waves owns 256 "rows"; wave w keeps the B-operand fragments of two 32-row tiles for the
+// whole padded dimension (KS k-steps) in registers, so rows are read from memory once. The
+// "columns" stream through LDS in tiles of 32 (KS blocks of 1 KB), loaded by global_load_lds
+// and double-buffered. v_mfma_f32_32x32x16_bf16(column fragment, row fragment) leaves
+// accumulator r of lane l = <row l % 32, column 8 (r / 4) + 4 (l / 32) + r % 4 of the tile>.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dkm {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16_t;
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void global_void_t;
+
+// f32 -> bf16, round to nearest even (NaN kept quiet).
+__device__ inline uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Wave w loads blocks w, w + 4, ... of a tile; `src` is the tile's first block + 8 lane. The
+// loads are kept together, ahead of the tile's LDS reads (each sets M0, and the compiler
+// drains outstanding LDS reads before an M0 write).
+template <int KS>
+__device__ __forceinline__ void res_prefetch(const uint16_t *src, uint16_t *buf, int wave) {
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int s = 0; s < KS / 4; ++s)
+        __builtin_amdgcn_global_load_lds((global_void_t *)(src + (4 * s + wave) * 512),
+                                         (lds_void_t *)(buf + (4 * s + wave) * 512), 16, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// The column fragments are read from LDS in batches of B k-steps: batch j + 1 is issued right
+// after the first MFMA pair of batch j, so the one wait before each batch finds its reads
+// (issued a batch earlier) complete. The wait is a full LDS drain: the compiler counts the
+// in-flight global_load_lds of the next tile as outstanding LDS work and cannot wait for a
+// partial count. Scheduling group barriers pin the order.
+template <int KS>
+__device__ __forceinline__ void res_tile(const bf16x8_t *a0, const bf16x8_t *a1,
+                                         const uint16_t *buf, floatx16_t &acc0, floatx16_t &acc1) {
+    constexpr int B = KS % 8 == 0 ? 8 : 4;
+    static_assert(KS % B == 0, "k-steps per tile");
+    bf16x8_t b[2 * B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) b[i] = *(const bf16x8_t *)(buf + i * 512);
+    __builtin_amdgcn_sched_group_barrier(0x100, B, 0);
+#pragma unroll
+    for (int j = 0; j < KS / B; ++j) {
+        bf16x8_t *cur = b + (j & 1) * B, *nxt = b + ((j + 1) & 1) * B;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int s = j * B + i;
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur[i], a0[s], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur[i], a1[s], acc1, 0, 0, 0);
+            if (i == 0 && j + 1 < KS / B) {
+#pragma unroll
+                for (int q = 0; q < B; ++q) nxt[q] = *(const bf16x8_t *)(buf + (s + B + q) * 512);
+            }
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        if (j + 1 < KS / B) __builtin_amdgcn_sched_group_barrier(0x100, B, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 2 * B - 2, 0);
+    }
+}
+
+// Loads the wave's two row tiles (ap = first tile's block 0 + 8 lane). Where the fragments
+// live: the first tile's and eight of the second's in AGPRs (MFMA reads its B operand from
+// there directly), the rest in VGPRs beside the column-fragment batches. The empty asm
+// statements only constrain the register class.
+template <int KS>
+__device__ __forceinline__ void res_load_rows(const uint16_t *ap, bf16x8_t *a0, bf16x8_t *a1) {
+#pragma unroll
+    for (int s = 0; s < KS; ++s) a0[s] = *(const bf16x8_t *)(ap + s * 512);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) a1[s] = *(const bf16x8_t *)(ap + (KS + s) * 512);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) asm volatile("" : "+a"(a0[s]));
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        if (s < 8) asm volatile("" : "+a"(a1[s]));
+        else asm volatile("" : "+v"(a1[s]));
+    }
+}
+
+// Sweeps `ntiles` column tiles (cf = column fragments, tile t at cf + t * KS * 512) through
+// the double buffer at sm[0 .. 2 * KS * 512): init(t, acc0, acc1) before a tile's MFMAs,
+// epi(t, acc0, acc1) after (acc0: first row tile, acc1: second). The first barrier also
+// covers whatever the caller wrote to LDS before the call.
+template <int KS, class Init, class Epi>
+__device__ __forceinline__ void res_sweep(uint16_t *sm, const uint16_t *cf, int ntiles, int wave,
+                                          int lane, const bf16x8_t *a0, const bf16x8_t *a1,
+                                          Init init, Epi epi) {
+    auto src = [&](int t) { return cf + (int64_t)t * KS * 512 + lane * 8; };
+    floatx16_t pa, pb;
+    res_prefetch<KS>(src(0), sm, wave);
+    for (int t = 0; t < ntiles; ++t) {
+        __builtin_amdgcn_s_waitcnt(0);  // own loads of tile t landed
+        __syncthreads();                // everyone's, and tile t - 1 fully read
+        uint16_t *cur = sm + (t & 1) * KS * 512;
+        init(t, pa, pb);
+        if (t + 1 < ntiles) res_prefetch<KS>(src(t + 1), sm + ((t + 1) & 1) * KS * 512, wave);
+        res_tile<KS>(a0, a1, cur + lane * 8, pa, pb);
+        epi(t, pa, pb);
+    }
+}
+
+}  // namespace dkm

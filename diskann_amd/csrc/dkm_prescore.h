@@ -1,0 +1,37 @@
+// Candidate pre-scoring (BF16 MFMA approximate distances, exact FP32 rescore).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+// Largest sample of base rows the per-query threshold is read from (LDS of the selection).
+#define PS_SAMPLE_MAX 15360
+// Passing (row, query) pairs a filter workgroup stages in LDS before appending them.
+#define PS_STAGE 4096
+
+namespace dkm {
+bool ps_supported_dp(int dp);
+size_t ps_filter_lds(int dp, int nqp);
+hipError_t ps_dense(int dp, const uint16_t *xf, int64_t ns, const float *xn, const uint16_t *qf,
+                    int nq, int nqp, bool l2, float *out, int64_t ld, hipStream_t st);
+hipError_t ps_select(const float *dense, int64_t ld, int ns, int j0, int j1, int j2, float *tau,
+                     int nq, hipStream_t st);
+hipError_t ps_filter(int dp, const uint16_t *xf, int64_t n, const float *xn, const uint16_t *qf,
+                     int nq, int nqp, bool l2, const float *tau, uint32_t *count, uint64_t *lists,
+                     int64_t cap, hipStream_t st);
+hipError_t ps_norms(const float *x, int64_t n, int dim, const int64_t *rows, float *out,
+                    hipStream_t st);
+hipError_t ps_gather_rows(const float *x, int dim, const int64_t *rows, int64_t ns, float *out,
+                          hipStream_t st);
+hipError_t ps_sort_tmp_bytes(int64_t total, int segments, size_t *bytes);
+hipError_t ps_sort(void *tmp, size_t tmp_bytes, const uint64_t *in, uint64_t *out, int64_t total,
+                   int segments, const uint32_t *begin, const uint32_t *end, hipStream_t st);
+hipError_t ps_bounds(const uint32_t *count, int nq, int64_t cap, uint32_t *begin, uint32_t *end,
+                     hipStream_t st);
+hipError_t ps_take(const uint64_t *sorted, int nq, int64_t cap, int C, uint32_t *ids,
+                   hipStream_t st);
+hipError_t ps_final_keys(const float *d, const uint32_t *ids, int64_t total, uint64_t *keys,
+                         hipStream_t st);
+hipError_t ps_output(const uint64_t *sorted, int nq, int C, int k, uint32_t *ids, float *dists,
+                     hipStream_t st);
+}  // namespace dkm

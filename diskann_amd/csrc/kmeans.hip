@@ -12,19 +12,9 @@
 
 #include "dkm_device.h"
 #include "dkm_kmeans.h"
+#include "dkm_mfma.h"
 
 namespace dkm {
-
-typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
-typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16_t;
-
-// f32 -> bf16, round to nearest even (NaN kept quiet).
-__device__ inline uint16_t to_bf16(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
 
 // rows x dim f32 (row stride `stride`) -> rows x dp bf16, zero padded to dp columns.
 __global__ __launch_bounds__(256) void to_bf16_kernel(const float *x, int64_t rows, int dim,
@@ -225,67 +215,11 @@ __global__ __launch_bounds__(256) void assign_mfma_kernel(const uint16_t *xb, in
     }
 }
 
-// BF16 MFMA assignment with the rows resident in registers (dims up to 768). Rows and centres
-// are stored in the fragment layout (to_bf16_frag_kernel), so each operand load is one
-// contiguous 1 KB block. A workgroup of four waves owns 256 rows; wave w keeps the B-operand
-// fragments of two 32-row tiles for the whole padded dimension (KS k-steps of 16: lane l
-// holds row l % 32, dims 16 s + 8 (l / 32) .. + 7), so X is read from memory exactly once,
-// and every centre fragment read from LDS feeds two MFMAs. The centres stream through LDS in
-// tiles of 32 (KS blocks of 1 KB, block s being the A-operand fragment of k-step s), loaded
-// straight into LDS by global_load_lds (16 bytes a lane) and double-buffered so the next tile
-// loads while the current one multiplies. v_mfma_f32_32x32x16_bf16(centres, rows) leaves
-// accumulator r of lane l = x.c of row l % 32 and centre 8 (r / 4) + 4 (l / 32) + r % 4 of
-// the tile; the accumulators start at -c.c / 2, so they end at -score / 2, and each lane
+// BF16 MFMA assignment with the rows resident in registers (dims up to 768), on the
+// register-resident GEMM of dkm_mfma.h (rows = data rows, columns = centres). The
+// accumulators start at -c.c / 2, so they end at x.c - c.c / 2 = -score / 2, and each lane
 // tracks its row's largest value over its centres in increasing index order (strict >); the
 // two half-waves are merged at the end, ties to the smaller centre.
-typedef __attribute__((address_space(3))) void lds_void_t;
-typedef __attribute__((address_space(1))) void global_void_t;
-
-// Wave w loads blocks w, w + 4, ... of a tile; `src` is the tile's first block + 8 lane. The
-// loads are kept together, ahead of the tile's LDS reads (each sets M0, and the compiler
-// drains outstanding LDS reads before an M0 write).
-template <int KS>
-__device__ __forceinline__ void res_prefetch(const uint16_t *src, uint16_t *buf, int wave) {
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int s = 0; s < KS / 4; ++s)
-        __builtin_amdgcn_global_load_lds((global_void_t *)(src + (4 * s + wave) * 512),
-                                         (lds_void_t *)(buf + (4 * s + wave) * 512), 16, 0, 0);
-    __builtin_amdgcn_sched_barrier(0);
-}
-
-// The centre fragments are read from LDS in batches of B k-steps: batch j + 1 is issued right
-// after the first MFMA pair of batch j, so the one wait before each batch finds its reads
-// (issued a batch earlier) complete. The wait is a full LDS drain: the compiler counts the
-// in-flight global_load_lds of the next tile as outstanding LDS work and cannot wait for a
-// partial count. Scheduling group barriers pin the order.
-template <int KS>
-__device__ __forceinline__ void res_tile(const bf16x8_t *a0, const bf16x8_t *a1,
-                                         const uint16_t *buf, floatx16_t &acc0, floatx16_t &acc1) {
-    constexpr int B = KS % 8 == 0 ? 8 : 4;
-    static_assert(KS % B == 0, "k-steps per tile");
-    bf16x8_t b[2 * B];
-#pragma unroll
-    for (int i = 0; i < B; ++i) b[i] = *(const bf16x8_t *)(buf + i * 512);
-    __builtin_amdgcn_sched_group_barrier(0x100, B, 0);
-#pragma unroll
-    for (int j = 0; j < KS / B; ++j) {
-        bf16x8_t *cur = b + (j & 1) * B, *nxt = b + ((j + 1) & 1) * B;
-#pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int s = j * B + i;
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur[i], a0[s], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur[i], a1[s], acc1, 0, 0, 0);
-            if (i == 0 && j + 1 < KS / B) {
-#pragma unroll
-                for (int q = 0; q < B; ++q) nxt[q] = *(const bf16x8_t *)(buf + (s + B + q) * 512);
-            }
-        }
-        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-        if (j + 1 < KS / B) __builtin_amdgcn_sched_group_barrier(0x100, B, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 2 * B - 2, 0);
-    }
-}
 
 // Both row tiles' accumulators start at -c.c / 2 of the lane's 16 centres of the tile at c0.
 __device__ __forceinline__ void res_init(const float *cn, int c0, floatx16_t &acc0,
@@ -332,39 +266,18 @@ __global__ __launch_bounds__(256, 1) void assign_mfma_res_kernel(const uint16_t 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int half = lane >> 5, l32 = lane & 31;
     const int64_t rtile = (int64_t)blockIdx.x * 8 + 2 * wave;
-    const uint16_t *ap = xf + rtile * KS * 512 + lane * 8;
     bf16x8_t a0[KS], a1[KS];
-#pragma unroll
-    for (int s = 0; s < KS; ++s) a0[s] = *(const bf16x8_t *)(ap + s * 512);
-#pragma unroll
-    for (int s = 0; s < KS; ++s) a1[s] = *(const bf16x8_t *)(ap + (KS + s) * 512);
-    // Where the row fragments live: the first tile's and eight of the second's in AGPRs (MFMA
-    // reads its B operand from there directly), the rest in VGPRs beside the centre-fragment
-    // batches. The empty asm statements only constrain the register class.
-#pragma unroll
-    for (int s = 0; s < KS; ++s) asm volatile("" : "+a"(a0[s]));
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        if (s < 8) asm volatile("" : "+a"(a1[s]));
-        else asm volatile("" : "+v"(a1[s]));
-    }
+    res_load_rows<KS>(xf + rtile * KS * 512 + lane * 8, a0, a1);
     for (int c = tid; c < Kp; c += 256) cn_s[c] = c < K ? cnorm[c] : __builtin_inff();
     float bs0 = -__builtin_inff(), bs1 = -__builtin_inff();
     uint32_t bc0 = 0xFFFFFFFFu, bc1 = 0xFFFFFFFFu;
-    const int ntiles = Kp / 32;
-    auto src = [&](int t) { return cf + (int64_t)t * KS * 512 + lane * 8; };
-    floatx16_t pa, pb;
-    res_prefetch<KS>(src(0), sm, wave);
-    for (int t = 0; t < ntiles; ++t) {
-        __builtin_amdgcn_s_waitcnt(0);  // own loads of tile t landed
-        __syncthreads();                // everyone's, and tile t - 1 fully read
-        uint16_t *cur = sm + (t & 1) * KS * 512;
-        res_init(cn_s, 32 * t + 4 * half, pa, pb);
-        if (t + 1 < ntiles) res_prefetch<KS>(src(t + 1), sm + ((t + 1) & 1) * KS * 512, wave);
-        res_tile<KS>(a0, a1, cur + lane * 8, pa, pb);
-        res_epilogue(pa, 32 * t + 4 * half, bs0, bc0);
-        res_epilogue(pb, 32 * t + 4 * half, bs1, bc1);
-    }
+    res_sweep<KS>(
+        sm, cf, Kp / 32, wave, lane, a0, a1,
+        [&](int t, floatx16_t &pa, floatx16_t &pb) { res_init(cn_s, 32 * t + 4 * half, pa, pb); },
+        [&](int t, floatx16_t &pa, floatx16_t &pb) {
+            res_epilogue(pa, 32 * t + 4 * half, bs0, bc0);
+            res_epilogue(pb, 32 * t + 4 * half, bs1, bc1);
+        });
     const float bsv[2] = {bs0, bs1};
     const uint32_t bcv[2] = {bc0, bc1};
 #pragma unroll

@@ -1,0 +1,374 @@
+// Candidate pre-scoring for exhaustive top-k (the precision layer of the 1M x 768
+// inner-product configuration): the base rows and the queries are rounded to bf16 and
+// scored as an MFMA GEMM with FP32 accumulation; the best `C` candidates of each query by
+// that approximate score are then re-scored exactly in FP32 with the reference's distance
+// arithmetic (the same functions as Rerank, diskann-providers/src/model/graph/provider/
+// async_/inmem/full_precision.rs:357-404) and the top k of those are returned.
+//
+// Selecting the best C of n approximate scores per query without materialising n scores:
+//   1. the queries are scored against a fixed sample of the base rows (dense output) and a
+//      per-query threshold tau is read off the sample's order statistics (the distance the
+//      j-th best sample row has, j chosen so that about 1.5 C rows of the base score at or
+//      below it);
+//   2. the full GEMM appends (orderable distance << 32 | row) for every pair at or below its
+//      query's tau to that query's candidate list (atomic counter per query);
+//   3. queries whose list holds fewer than C entries (or more than its capacity) are redone
+//      with a looser (tighter) threshold; every list is sorted by (distance, row) and its
+//      first C rows are the candidates.
+// Approximate distance per (query q, row x): inner product / cosine-normalized -q.x,
+// squared L2 |x|^2 - 2 q.x (|q|^2 is common to all rows of a query).
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "dkm_device.h"
+#include "dkm_mfma.h"
+#include "dkm_prescore.h"
+
+namespace dkm {
+
+// Ascending order of the key = ascending order of the float (NaN not expected).
+__device__ __forceinline__ uint32_t orderable(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float from_orderable(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+// Approximate distance from the accumulated dot product (see the file comment).
+__device__ __forceinline__ float approx_dist(bool l2, float dot, float xn) {
+    return l2 ? xn - 2.0f * dot : -dot;
+}
+
+// Dense scores of the sample rows against a block of queries: out[q * ld + row] for
+// row < ns, q < nq. xf: sample rows (fragment layout, padded to 256), qf: queries (fragment
+// layout, padded to a multiple of 64), xn: row norms (L2 only).
+template <int KS>
+__global__ __launch_bounds__(256, 1) void ps_dense_kernel(const uint16_t *xf, int64_t ns,
+                                                          const float *xn, const uint16_t *qf,
+                                                          int nq, int nqp, bool l2, float *out,
+                                                          int64_t ld) {
+    extern __shared__ __align__(16) uint16_t sm[];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int64_t rtile = (int64_t)blockIdx.x * 8 + 2 * wave;
+    bf16x8_t a0[KS], a1[KS];
+    res_load_rows<KS>(xf + rtile * KS * 512 + lane * 8, a0, a1);
+    const int64_t r0 = 32 * rtile + l32, r1 = r0 + 32;
+    const float n0 = l2 && r0 < ns ? xn[r0] : 0.0f, n1 = l2 && r1 < ns ? xn[r1] : 0.0f;
+    res_sweep<KS>(
+        sm, qf, nqp / 32, wave, lane, a0, a1,
+        [&](int, floatx16_t &pa, floatx16_t &pb) {
+            pa = floatx16_t{};
+            pb = floatx16_t{};
+        },
+        [&](int t, floatx16_t &pa, floatx16_t &pb) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int q = 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
+                if (q < nq) {
+                    if (r0 < ns) out[(int64_t)q * ld + r0] = approx_dist(l2, pa[r], n0);
+                    if (r1 < ns) out[(int64_t)q * ld + r1] = approx_dist(l2, pb[r], n1);
+                }
+            }
+        });
+}
+
+// Threshold per query: the ranks j[0..2] smallest of the query's ns sample distances, by a
+// radix select over orderable keys held in LDS (ns <= kSampleMax).
+constexpr int kSampleMax = PS_SAMPLE_MAX;
+
+__global__ __launch_bounds__(256) void ps_select_kernel(const float *dense, int64_t ld, int ns,
+                                                        int j0, int j1, int j2, float *tau,
+                                                        int nq) {
+    __shared__ uint32_t keys[kSampleMax];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sel[2];
+    const int q = blockIdx.x, tid = threadIdx.x;
+    for (int i = tid; i < ns; i += 256) keys[i] = orderable(dense[(int64_t)q * ld + i]);
+    const int js[3] = {j0, j1, j2};
+    for (int w = 0; w < 3; ++w) {
+        uint32_t prefix = 0, mask = 0;
+        uint32_t want = (uint32_t)min(max(js[w], 0), ns - 1);
+        for (int shift = 24; shift >= 0; shift -= 8) {
+            hist[tid] = 0;
+            __syncthreads();
+            for (int i = tid; i < ns; i += 256)
+                if ((keys[i] & mask) == prefix) atomicAdd(&hist[(keys[i] >> shift) & 255u], 1u);
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t cum = 0, d = 0;
+                for (; d < 255; ++d) {
+                    if (cum + hist[d] > want) break;
+                    cum += hist[d];
+                }
+                sel[0] = d;
+                sel[1] = want - cum;
+            }
+            __syncthreads();
+            prefix |= sel[0] << shift;
+            mask |= 255u << shift;
+            want = sel[1];
+            __syncthreads();
+        }
+        if (tid == 0) tau[w * (int64_t)nq + q] = from_orderable(prefix);
+    }
+}
+
+// Full GEMM with the threshold filter: every (row, query) whose approximate distance is at
+// or below tau[query] appends (orderable distance << 32 | row) to the query's list
+// (capacity cap; the counter keeps counting past it so overflow is visible). Queries with
+// tau = -inf take no part (used when only some queries are redone). Passing pairs are
+// staged in LDS during the sweep (an LDS atomic per pair) and appended to the global lists
+// once the sweep is over, so no returning global atomic waits behind the in-flight tile
+// loads; a full stage spills straight to the global lists.
+constexpr int kStage = PS_STAGE;
+
+__device__ __forceinline__ void ps_append(uint32_t *count, uint64_t *lists, int64_t cap, int q,
+                                          uint64_t key) {
+    const uint32_t pos = atomicAdd(&count[q], 1u);
+    if (pos < cap) lists[(int64_t)q * cap + pos] = key;
+}
+
+template <int KS>
+__global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, int64_t n,
+                                                           const float *xn, const uint16_t *qf,
+                                                           int nq, int nqp, bool l2,
+                                                           const float *tau, uint32_t *count,
+                                                           uint64_t *lists, int64_t cap) {
+    extern __shared__ __align__(16) uint16_t sm[];
+    float *tau_s = (float *)(sm + 2 * KS * 512);
+    uint32_t *stage_n = (uint32_t *)(tau_s + nqp);
+    uint64_t *skey = (uint64_t *)(stage_n + 4);
+    uint16_t *sq = (uint16_t *)(skey + kStage);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int64_t rtile = (int64_t)blockIdx.x * 8 + 2 * wave;
+    bf16x8_t a0[KS], a1[KS];
+    res_load_rows<KS>(xf + rtile * KS * 512 + lane * 8, a0, a1);
+    for (int c = tid; c < nqp; c += 256) tau_s[c] = c < nq ? tau[c] : -__builtin_inff();
+    if (tid == 0) *stage_n = 0;
+    const int64_t r0 = 32 * rtile + l32, r1 = r0 + 32;
+    const float n0 = l2 && r0 < n ? xn[r0] : 0.0f, n1 = l2 && r1 < n ? xn[r1] : 0.0f;
+    const bool v0 = r0 < n, v1 = r1 < n;
+    auto push = [&](int q, uint64_t key) {
+        const uint32_t i = atomicAdd(stage_n, 1u);
+        if (i < (uint32_t)kStage) {
+            skey[i] = key;
+            sq[i] = (uint16_t)q;
+        } else {
+            ps_append(count, lists, cap, q, key);
+        }
+    };
+    res_sweep<KS>(
+        sm, qf, nqp / 32, wave, lane, a0, a1,
+        [&](int, floatx16_t &pa, floatx16_t &pb) {
+            pa = floatx16_t{};
+            pb = floatx16_t{};
+        },
+        [&](int t, floatx16_t &pa, floatx16_t &pb) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int qb = 32 * t + 8 * g + 4 * half;
+                const float4 t4 = *(const float4 *)(tau_s + qb);
+                const float tv[4] = {t4.x, t4.y, t4.z, t4.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int r = 4 * g + e;
+                    const float d0 = approx_dist(l2, pa[r], n0), d1 = approx_dist(l2, pb[r], n1);
+                    if (v0 && d0 <= tv[e]) push(qb + e, ((uint64_t)orderable(d0) << 32) | (uint32_t)r0);
+                    if (v1 && d1 <= tv[e]) push(qb + e, ((uint64_t)orderable(d1) << 32) | (uint32_t)r1);
+                }
+            }
+        });
+    __syncthreads();
+    const int staged = (int)min(*stage_n, (uint32_t)kStage);
+    for (int i = tid; i < staged; i += 256) ps_append(count, lists, cap, sq[i], skey[i]);
+}
+
+// Row norms in f32 (the L2 approximate distance's |x|^2; any fixed order serves).
+__global__ __launch_bounds__(256) void ps_norms_kernel(const float *x, int64_t n, int dim,
+                                                       const int64_t *rows, float *out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float *r = x + (rows ? rows[i] : i) * (int64_t)dim;
+    float s = 0.0f;
+    for (int d = 0; d < dim; ++d) s = fma_(r[d], r[d], s);
+    out[i] = s;
+}
+
+// Gathers sample rows into a contiguous matrix.
+__global__ __launch_bounds__(256) void ps_gather_rows_kernel(const float *x, int dim,
+                                                             const int64_t *rows, int64_t ns,
+                                                             float *out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= ns * dim) return;
+    const int64_t i = e / dim;
+    out[e] = x[rows[i] * dim + (e - i * dim)];
+}
+
+// Segment bounds of the candidate lists and of the rescored lists.
+__global__ __launch_bounds__(256) void ps_bounds_kernel(const uint32_t *count, int nq, int64_t cap,
+                                                        uint32_t *begin, uint32_t *end) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= nq) return;
+    begin[q] = (uint32_t)(q * cap);
+    end[q] = (uint32_t)(q * cap + min((int64_t)count[q], cap));
+}
+
+// First C rows of each sorted list -> candidate ids (count >= C is guaranteed by the host).
+__global__ __launch_bounds__(256) void ps_take_kernel(const uint64_t *sorted, int nq, int64_t cap,
+                                                      int C, uint32_t *ids) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)nq * C) return;
+    const int64_t q = e / C, j = e - q * C;
+    ids[e] = (uint32_t)(sorted[q * cap + j] & 0xFFFFFFFFull);
+}
+
+// (orderable exact distance << 32 | row) for the final selection.
+__global__ __launch_bounds__(256) void ps_final_keys_kernel(const float *d, const uint32_t *ids,
+                                                            int64_t total, uint64_t *keys) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e < total) keys[e] = ((uint64_t)orderable(d[e]) << 32) | ids[e];
+}
+
+__global__ __launch_bounds__(256) void ps_output_kernel(const uint64_t *sorted, int nq, int C,
+                                                        int k, uint32_t *ids, float *dists) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)nq * k) return;
+    const int64_t q = e / k, j = e - q * k;
+    const uint64_t key = sorted[q * C + j];
+    ids[e] = (uint32_t)(key & 0xFFFFFFFFull);
+    dists[e] = from_orderable((uint32_t)(key >> 32));
+}
+
+static unsigned blocks(int64_t n, int64_t per) { return (unsigned)((n + per - 1) / per); }
+
+template <int KS>
+static hipError_t dense_t(const uint16_t *xf, int64_t ns, const float *xn, const uint16_t *qf,
+                          int nq, int nqp, bool l2, float *out, int64_t ld, hipStream_t st) {
+    const size_t lds = (size_t)2 * KS * 1024;
+    hipError_t e = hipFuncSetAttribute((const void *)ps_dense_kernel<KS>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(ps_dense_kernel<KS>, dim3(blocks(ns, 256)), dim3(256), lds, st, xf, ns, xn,
+                       qf, nq, nqp, l2, out, ld);
+    return hipGetLastError();
+}
+
+template <int KS>
+static hipError_t filter_t(const uint16_t *xf, int64_t n, const float *xn, const uint16_t *qf,
+                           int nq, int nqp, bool l2, const float *tau, uint32_t *count,
+                           uint64_t *lists, int64_t cap, hipStream_t st) {
+    const size_t lds = ps_filter_lds(KS * 16, nqp);
+    hipError_t e = hipFuncSetAttribute((const void *)ps_filter_kernel<KS>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(ps_filter_kernel<KS>, dim3(blocks(n, 256)), dim3(256), lds, st, xf, n, xn,
+                       qf, nq, nqp, l2, tau, count, lists, cap);
+    return hipGetLastError();
+}
+
+#define PS_DISPATCH(KS_, CALL)                         \
+    switch (KS_) {                                     \
+    case 4: return CALL(4);                            \
+    case 8: return CALL(8);                            \
+    case 16: return CALL(16);                          \
+    case 24: return CALL(24);                          \
+    case 32: return CALL(32);                          \
+    case 48: return CALL(48);                          \
+    default: return hipErrorInvalidValue;              \
+    }
+
+size_t ps_filter_lds(int dp, int nqp) {
+    return (size_t)2 * dp * 64 + 4 * (size_t)nqp + 16 + (size_t)kStage * 10;
+}
+
+bool ps_supported_dp(int dp) {
+    const int ks = dp / 16;
+    return dp % 16 == 0 && (ks == 4 || ks == 8 || ks == 16 || ks == 24 || ks == 32 || ks == 48);
+}
+
+hipError_t ps_dense(int dp, const uint16_t *xf, int64_t ns, const float *xn, const uint16_t *qf,
+                    int nq, int nqp, bool l2, float *out, int64_t ld, hipStream_t st) {
+#define CALL(K) dense_t<K>(xf, ns, xn, qf, nq, nqp, l2, out, ld, st)
+    PS_DISPATCH(dp / 16, CALL)
+#undef CALL
+}
+
+hipError_t ps_filter(int dp, const uint16_t *xf, int64_t n, const float *xn, const uint16_t *qf,
+                     int nq, int nqp, bool l2, const float *tau, uint32_t *count, uint64_t *lists,
+                     int64_t cap, hipStream_t st) {
+#define CALL(K) filter_t<K>(xf, n, xn, qf, nq, nqp, l2, tau, count, lists, cap, st)
+    PS_DISPATCH(dp / 16, CALL)
+#undef CALL
+}
+
+hipError_t ps_select(const float *dense, int64_t ld, int ns, int j0, int j1, int j2, float *tau,
+                     int nq, hipStream_t st) {
+    if (ns > kSampleMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ps_select_kernel, dim3(nq), dim3(256), 0, st, dense, ld, ns, j0, j1, j2, tau,
+                       nq);
+    return hipGetLastError();
+}
+
+hipError_t ps_norms(const float *x, int64_t n, int dim, const int64_t *rows, float *out,
+                    hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(ps_norms_kernel, dim3(blocks(n, 256)), dim3(256), 0, st, x, n, dim, rows, out);
+    return hipGetLastError();
+}
+
+hipError_t ps_gather_rows(const float *x, int dim, const int64_t *rows, int64_t ns, float *out,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(ps_gather_rows_kernel, dim3(blocks(ns * dim, 256)), dim3(256), 0, st, x,
+                       dim, rows, ns, out);
+    return hipGetLastError();
+}
+
+hipError_t ps_sort_tmp_bytes(int64_t total, int segments, size_t *bytes) {
+    *bytes = 0;
+    return rocprim::segmented_radix_sort_keys(nullptr, *bytes, (uint64_t *)nullptr,
+                                              (uint64_t *)nullptr, (unsigned)total,
+                                              (unsigned)segments, (uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, 0, 64);
+}
+
+hipError_t ps_sort(void *tmp, size_t tmp_bytes, const uint64_t *in, uint64_t *out, int64_t total,
+                   int segments, const uint32_t *begin, const uint32_t *end, hipStream_t st) {
+    return rocprim::segmented_radix_sort_keys(tmp, tmp_bytes, in, out, (unsigned)total,
+                                              (unsigned)segments, begin, end, 0, 64, st);
+}
+
+hipError_t ps_bounds(const uint32_t *count, int nq, int64_t cap, uint32_t *begin, uint32_t *end,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(ps_bounds_kernel, dim3(blocks(nq, 256)), dim3(256), 0, st, count, nq, cap,
+                       begin, end);
+    return hipGetLastError();
+}
+
+hipError_t ps_take(const uint64_t *sorted, int nq, int64_t cap, int C, uint32_t *ids,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(ps_take_kernel, dim3(blocks((int64_t)nq * C, 256)), dim3(256), 0, st, sorted,
+                       nq, cap, C, ids);
+    return hipGetLastError();
+}
+
+hipError_t ps_final_keys(const float *d, const uint32_t *ids, int64_t total, uint64_t *keys,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(ps_final_keys_kernel, dim3(blocks(total, 256)), dim3(256), 0, st, d, ids,
+                       total, keys);
+    return hipGetLastError();
+}
+
+hipError_t ps_output(const uint64_t *sorted, int nq, int C, int k, uint32_t *ids, float *dists,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(ps_output_kernel, dim3(blocks((int64_t)nq * k, 256)), dim3(256), 0, st,
+                       sorted, nq, C, k, ids, dists);
+    return hipGetLastError();
+}
+
+}  // namespace dkm

@@ -298,6 +298,29 @@ dkm_status dkm_kmeans_assign_device(const float *d_x, int64_t n, int64_t dim, co
                                     int64_t K, int precision, uint32_t *d_assign,
                                     float *d_dist);
 
+/* -------------------------------------------------------------------------------- */
+/* Candidate pre-scoring (BF16 MFMA) with exact FP32 rescore                          */
+/* -------------------------------------------------------------------------------- */
+/* Exhaustive top-k over device-resident f32 rows (n x dim, dim <= 768) for inner product,
+ * L2 or cosine-normalized: every (query, row) pair is scored approximately as a BF16 MFMA
+ * GEMM with FP32 accumulation (-q.x, or |x|^2 - 2 q.x for L2); the `candidates` best rows
+ * of each query by that score are re-scored exactly with the reference's f32 distance
+ * functions (as Rerank, diskann-providers/src/model/graph/provider/async_/inmem/
+ * full_precision.rs:357-404, and dkm_distance_batch) and the k best of those are returned
+ * in increasing exact distance (ties by row). No reference counterpart: this is the
+ * precision layer that stands in for exact exhaustive FP32 scoring.
+ * dkm_prescore_create keeps d_x (which must stay valid) and builds the bf16 copy;
+ * dkm_prescore_search takes device queries / outputs (nq x k) and, if ms_stages is not
+ * NULL, reports the device time of four stages (approximate scoring with the candidate
+ * filter, candidate selection, exact rescore, final top-k). Synchronous. */
+typedef struct dkm_prescorer dkm_prescorer;
+dkm_status dkm_prescore_create(const float *d_x, int64_t n, int64_t dim, int metric,
+                               int precision, dkm_prescorer **out);
+dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_queries, int64_t nq, int64_t k,
+                               int64_t candidates, uint32_t *d_ids, float *d_dists,
+                               double *ms_stages);
+void dkm_prescore_destroy(dkm_prescorer *ps);
+
 /* Tuning knobs (autotuner surface; none changes results): visited-set slots in LDS per
  * query (a multiple of 64 in 256..32768, 0 = automatic) and the PQ distance mode (0 = lookup table in LDS,
  * 1 = direct recomputation from the pivots, the default). */

@@ -163,6 +163,88 @@ def serve_measure(dk, ix, qs, K, L, BW, batch_ids):
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
+def unit_mixture(n, dim, comps, rng, centers=None):
+    """N(0,1) component centres, members centre + 0.5 N(0,1), then L2-normalised rows."""
+    if centers is None:
+        centers = rng.standard_normal((comps, dim)).astype(np.float32)
+    x = np.empty((n, dim), np.float32)
+    for r0 in range(0, n, 100_000):
+        r1 = min(n, r0 + 100_000)
+        blk = centers[rng.integers(0, comps, r1 - r0)] + \
+            (0.5 * rng.standard_normal((r1 - r0, dim))).astype(np.float32)
+        blk /= np.linalg.norm(blk, axis=1, keepdims=True)
+        x[r0:r1] = blk
+    return x, centers
+
+
+def prescore_workload(args):
+    """Exhaustive top-k over 1M x 768 unit-norm rows (inner product) with BF16 MFMA
+    pre-scoring and exact FP32 rescore of the best 4,096 candidates per query, k = 10 and
+    100; recall and distance error on a sample of queries against FP64 brute force, and the
+    same for exhaustive FP32 scoring (numpy sgemm) as the plain-path reference."""
+    import diskann_amd as dk  # noqa: F401
+    from diskann_amd.prescore import PreScorer
+    N, D, NQ, C = args.n, 768, args.nq, 4096
+    rng = np.random.default_rng(0xD15C0003)
+    t0 = time.time()
+    x, centers = unit_mixture(N, D, 1024, rng)
+    qs, _ = unit_mixture(NQ, D, 1024, np.random.default_rng(0xD15C0013), centers)
+    log("data %.1fs" % (time.time() - t0))
+    ps = PreScorer(x, 1)  # inner product
+    res = {}
+    for k in (10, 100):
+        ps.search(qs[:256], k, C)  # warm-up
+        t1 = time.perf_counter()
+        ids, ds, ms = ps.search(qs, k, C)
+        wall = time.perf_counter() - t1
+        res[k] = (ids, ds, ms, wall)
+        log("k=%d: %.1f ms wall, stages %s" % (k, wall * 1e3, ["%.1f" % m for m in ms]))
+    # accuracy on a sample of queries
+    S = min(200, NQ)
+    qd = qs[:S].astype(np.float64)
+    ip64 = qd @ x.astype(np.float64).T
+    ip32 = qs[:S] @ x.T
+    acc = {}
+    for k in (10, 100):
+        ids, ds = res[k][0], res[k][1]
+        gt = np.argpartition(-ip64, k - 1, axis=1)[:, :k]
+        f32 = np.argpartition(-ip32, k - 1, axis=1)[:, :k]
+        hit = sum(len(set(ids[i].tolist()) & set(gt[i].tolist())) for i in range(S))
+        hit32 = sum(len(set(f32[i].tolist()) & set(gt[i].tolist())) for i in range(S))
+        d64 = -np.take_along_axis(ip64, ids[:S].astype(np.int64), axis=1)
+        err = np.abs(ds[:S].astype(np.float64) - d64) / np.maximum(np.abs(d64), 1e-12)
+        acc[k] = {"recall": hit / (S * k), "recall_fp32_exhaustive": hit32 / (S * k),
+                  "recall_delta": (hit - hit32) / (S * k),
+                  "dist_rel_err_max_fp64": float(err.max()),
+                  "dist_rel_err_mean_fp64": float(err.mean())}
+    ids, ds, ms, wall = res[10]
+    flops = 2.0 * N * NQ * D
+    score_ms = ms[0]
+    tflops = flops / (score_ms * 1e-3) / 1e12
+    out = {
+        "metric": "exhaustive top-k QPS, BF16 MFMA pre-scoring + exact FP32 rescore of 4096",
+        "value": NQ / wall, "unit": "queries/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {
+            "workload": "1M x 768 unit-norm (1,024-component mixture), inner product, %d "
+                        "queries, 4096 candidates rescored in FP32, k = 10 (k = 100 below)" % NQ,
+            "stage_ms_k10": {"score_filter": ms[0], "select": ms[1], "rescore": ms[2],
+                             "topk": ms[3]},
+            "qps_k100": NQ / res[100][3],
+            "stage_ms_k100": dict(zip(["score_filter", "select", "rescore", "topk"], res[100][2])),
+            "accuracy_sample_queries": S, "k10": acc[10], "k100": acc[100],
+            "rescore_bytes": float(NQ) * C * D * 4,
+            "rescore_gbs": float(NQ) * C * D * 4 / (ms[2] * 1e-3) / 1e9,
+        },
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "ps_filter_kernel<48> (+ sample scoring and selection)",
+                     "flops_per_launch": flops},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def kmeans_workload(args):
     """1M x 768 unit-norm vectors (1,024-component mixture), K = 1,024, 20 Lloyd iterations:
     assignment as a BF16 MFMA GEMM with FP32 accumulation, FP64 centroid update. Reports
@@ -176,14 +258,7 @@ def kmeans_workload(args):
     N, D, K, IT = args.n, 768, 1024, 20
     rng = np.random.default_rng(0xD15C0003)
     t0 = time.time()
-    centers = rng.standard_normal((1024, D)).astype(np.float32)
-    x = np.empty((N, D), np.float32)
-    for r0 in range(0, N, 100_000):
-        r1 = min(N, r0 + 100_000)
-        blk = centers[rng.integers(0, 1024, r1 - r0)] + \
-            (0.5 * rng.standard_normal((r1 - r0, D))).astype(np.float32)
-        blk /= np.linalg.norm(blk, axis=1, keepdims=True)
-        x[r0:r1] = blk
+    x, _ = unit_mixture(N, D, 1024, rng)
     log("data %.1fs" % (time.time() - t0))
     init = x[rng.choice(N, K, replace=False)].copy()
     dx = DeviceBuf(lib, x)
@@ -249,8 +324,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=1.5,
                     help="wall-clock budget of the CPU baseline sample")
     ap.add_argument("--workload", default="search1m",
-                    help="search1m (default: batched Hybrid search, 1M x 128) or kmeans768 "
-                         "(BF16 MFMA k-means, 1M x 768, K = 1024, 20 iterations)")
+                    help="search1m (default: batched Hybrid search, 1M x 128), kmeans768 "
+                         "(BF16 MFMA k-means, 1M x 768, K = 1024, 20 iterations) or "
+                         "prescore768 (exhaustive top-k with BF16 pre-scoring, 1M x 768)")
     ap.add_argument("--fixed", default="",
                     help="L,beam_width,visited_slots: skip the L sweep and the autotuner")
     ap.add_argument("--timed-only", action="store_true",
@@ -266,6 +342,8 @@ def main():
 
     if args.workload == "kmeans768":
         return kmeans_workload(args)
+    if args.workload == "prescore768":
+        return prescore_workload(args)
 
     import diskann_amd as dk
     from diskann_amd import quantization as qz

@@ -6,8 +6,8 @@
 
 // Largest sample of base rows the per-query threshold is read from (LDS of the selection).
 #define PS_SAMPLE_MAX 15360
-// Passing (row, query) pairs a filter workgroup stages in LDS before appending them.
-#define PS_STAGE 4096
+// Passing (row, query) pairs each filter thread stages in LDS before appending them.
+#define PS_SLOTS 16
 
 namespace dkm {
 bool ps_supported_dp(int dp);

@@ -118,14 +118,15 @@ __global__ __launch_bounds__(256) void ps_select_kernel(const float *dense, int6
     }
 }
 
-// Full GEMM with the threshold filter: every (row, query) whose approximate distance is at
-// or below tau[query] appends (orderable distance << 32 | row) to the query's list
+// Full GEMM with the threshold filter: every (row, query) whose approximate distance d is at
+// or below tau[query] appends (key << 32 | row) to the query's list, the key ordering the
+// query's entries like d
 // (capacity cap; the counter keeps counting past it so overflow is visible). Queries with
 // tau = -inf take no part (used when only some queries are redone). Passing pairs are
-// staged in LDS during the sweep (an LDS atomic per pair) and appended to the global lists
-// once the sweep is over, so no returning global atomic waits behind the in-flight tile
-// loads; a full stage spills straight to the global lists.
-constexpr int kStage = PS_STAGE;
+// staged in LDS during the sweep, in kSlots private slots per thread (a register counter,
+// no atomics), and appended to the global lists once the sweep is over, so no returning
+// atomic waits inside the tile loop; a thread whose slots are full appends directly.
+constexpr int kSlots = PS_SLOTS;
 
 __device__ __forceinline__ void ps_append(uint32_t *count, uint64_t *lists, int64_t cap, int q,
                                           uint64_t key) {
@@ -141,52 +142,56 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
                                                            uint64_t *lists, int64_t cap) {
     extern __shared__ __align__(16) uint16_t sm[];
     float *tau_s = (float *)(sm + 2 * KS * 512);
-    uint32_t *stage_n = (uint32_t *)(tau_s + nqp);
-    uint64_t *skey = (uint64_t *)(stage_n + 4);
-    uint16_t *sq = (uint16_t *)(skey + kStage);
+    uint64_t *skey = (uint64_t *)(tau_s + nqp);
+    uint16_t *sq = (uint16_t *)(skey + 256 * kSlots);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int half = lane >> 5, l32 = lane & 31;
     const int64_t rtile = (int64_t)blockIdx.x * 8 + 2 * wave;
     bf16x8_t a0[KS], a1[KS];
     res_load_rows<KS>(xf + rtile * KS * 512 + lane * 8, a0, a1);
     for (int c = tid; c < nqp; c += 256) tau_s[c] = c < nq ? tau[c] : -__builtin_inff();
-    if (tid == 0) *stage_n = 0;
     const int64_t r0 = 32 * rtile + l32, r1 = r0 + 32;
-    const float n0 = l2 && r0 < n ? xn[r0] : 0.0f, n1 = l2 && r1 < n ? xn[r1] : 0.0f;
+    // the accumulators start at tau (L2: tau / 2), so after the tile they hold tau - d
+    // (L2: (tau - d + |x|^2) / 2): a pair passes when that, less the row's |x|^2 / 2 (L2),
+    // is >= 0, and its negation orders a query's entries like d
+    const float h0 = l2 && r0 < n ? 0.5f * xn[r0] : 0.0f;
+    const float h1 = l2 && r1 < n ? 0.5f * xn[r1] : 0.0f;
     const bool v0 = r0 < n, v1 = r1 < n;
+    const float tscale = l2 ? 0.5f : 1.0f;
+    int used = 0;
     auto push = [&](int q, uint64_t key) {
-        const uint32_t i = atomicAdd(stage_n, 1u);
-        if (i < (uint32_t)kStage) {
-            skey[i] = key;
-            sq[i] = (uint16_t)q;
+        if (used < kSlots) {
+            skey[tid * kSlots + used] = key;
+            sq[tid * kSlots + used] = (uint16_t)q;
+            ++used;
         } else {
             ps_append(count, lists, cap, q, key);
         }
     };
     res_sweep<KS>(
         sm, qf, nqp / 32, wave, lane, a0, a1,
-        [&](int, floatx16_t &pa, floatx16_t &pb) {
-            pa = floatx16_t{};
-            pb = floatx16_t{};
-        },
         [&](int t, floatx16_t &pa, floatx16_t &pb) {
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                const int qb = 32 * t + 8 * g + 4 * half;
-                const float4 t4 = *(const float4 *)(tau_s + qb);
-                const float tv[4] = {t4.x, t4.y, t4.z, t4.w};
+                const float4 t4 = *(const float4 *)(tau_s + 32 * t + 8 * g + 4 * half);
+                pa[4 * g + 0] = tscale * t4.x;
+                pa[4 * g + 1] = tscale * t4.y;
+                pa[4 * g + 2] = tscale * t4.z;
+                pa[4 * g + 3] = tscale * t4.w;
+            }
+            pb = pa;
+        },
+        [&](int t, floatx16_t &pa, floatx16_t &pb) {
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int r = 4 * g + e;
-                    const float d0 = approx_dist(l2, pa[r], n0), d1 = approx_dist(l2, pb[r], n1);
-                    if (v0 && d0 <= tv[e]) push(qb + e, ((uint64_t)orderable(d0) << 32) | (uint32_t)r0);
-                    if (v1 && d1 <= tv[e]) push(qb + e, ((uint64_t)orderable(d1) << 32) | (uint32_t)r1);
-                }
+            for (int r = 0; r < 16; ++r) {
+                const int q = 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
+                const float u0 = pa[r] - h0, u1 = pb[r] - h1;
+                if (v0 && u0 >= 0.0f) push(q, ((uint64_t)orderable(-u0) << 32) | (uint32_t)r0);
+                if (v1 && u1 >= 0.0f) push(q, ((uint64_t)orderable(-u1) << 32) | (uint32_t)r1);
             }
         });
-    __syncthreads();
-    const int staged = (int)min(*stage_n, (uint32_t)kStage);
-    for (int i = tid; i < staged; i += 256) ps_append(count, lists, cap, sq[i], skey[i]);
+    for (int j = 0; j < used; ++j)
+        ps_append(count, lists, cap, sq[tid * kSlots + j], skey[tid * kSlots + j]);
 }
 
 // Row norms in f32 (the L2 approximate distance's |x|^2; any fixed order serves).
@@ -284,7 +289,7 @@ static hipError_t filter_t(const uint16_t *xf, int64_t n, const float *xn, const
     }
 
 size_t ps_filter_lds(int dp, int nqp) {
-    return (size_t)2 * dp * 64 + 4 * (size_t)nqp + 16 + (size_t)kStage * 10;
+    return (size_t)2 * dp * 64 + 4 * (size_t)nqp + (size_t)256 * kSlots * 10;
 }
 
 bool ps_supported_dp(int dp) {

@@ -1713,9 +1713,27 @@ dkm_status dkm_prescore_create(const float *d_x, int64_t n, int64_t dim, int met
     if ((e = dkm::kmeans_to_bf16_frag(d_x, n, (int)dim, dim, dp, npad, ps->xf.p, nullptr)) !=
         hipSuccess)
         return bad(e);
-    // the sample: rows floor(i * n / ns), i < ns
-    std::vector<int64_t> rows(ps->ns);
-    for (int64_t i = 0; i < ps->ns; ++i) rows[i] = (int64_t)((__int128)i * n / ps->ns);
+    // the sample: ns distinct rows drawn with a fixed seed (SplitMix64), in increasing order
+    std::vector<int64_t> rows;
+    if (ps->ns == n) {
+        rows.resize(n);
+        for (int64_t i = 0; i < n; ++i) rows[i] = i;
+    } else {
+        std::vector<uint8_t> taken(n, 0);
+        uint64_t st = 0x5DEECE66Dull;
+        while ((int64_t)rows.size() < ps->ns) {
+            uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            const int64_t r = (int64_t)(z % (uint64_t)n);
+            if (!taken[r]) {
+                taken[r] = 1;
+                rows.push_back(r);
+            }
+        }
+        std::sort(rows.begin(), rows.end());
+    }
     DevBuf<int64_t> drows;
     DevBuf<float> srows;
     if ((e = drows.ensure(ps->ns)) != hipSuccess) return bad(e);
@@ -1765,20 +1783,30 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
     const bool l2 = ps->metric == DKM_L2;
     const int64_t B = std::min<int64_t>(nq, kPsBatch);
     const int64_t Bp = (B + 63) / 64 * 64;
-    // candidate list capacity: the thresholds aim at ~1.5 C, retries at ~4.5 C / ~1.1 C
-    const int64_t cap = std::min<int64_t>(ps->n, 8 * C);
     if (dkm::ps_filter_lds(dp, (int)Bp) > 160 * 1024)
         return fail(DKM_ERR_INVALID_ARG, "pre-scoring tile does not fit LDS");
-    const double want = 1.5 * (double)C * (double)ps->ns / (double)ps->n;
-    const int j0 = (int)std::min<int64_t>(ps->ns - 1, std::max<int64_t>(0, (int64_t)std::ceil(want) - 1));
+    // Threshold ranks in the sample: j0 aims at ~1.5 C passing rows but is at least 15 (an
+    // order statistic of rank < 16 scatters too widely); retries use 3x (short lists) and
+    // 3/4x (overflow) of it. The list capacity holds 4x what rank j0 + 1 predicts and at
+    // least 8 C.
+    const double rows_per_sample = (double)ps->n / (double)ps->ns;
+    const double want = 1.5 * (double)C / rows_per_sample;
+    const int j0 = (int)std::min<int64_t>(ps->ns - 1,
+                                          std::max<int64_t>(15, (int64_t)std::ceil(want) - 1));
     const int j_loose = (int)std::min<int64_t>(ps->ns - 1, 3 * ((int64_t)j0 + 1) - 1);
     const int j_tight = (int)std::max<int64_t>(0, ((int64_t)j0 + 1) * 3 / 4 - 1);
+    const int64_t cap = std::min<int64_t>(
+        ps->n, std::max<int64_t>(8 * C, (int64_t)std::ceil(4.0 * (j0 + 1) * rows_per_sample)));
     const bool all_pass = 1.5 * (double)C >= (double)ps->n;  // tiny bases: every row is a candidate
     DevBuf<uint16_t> qf;
     DevBuf<float> dense, tau, dexact;
     DevBuf<uint32_t> cnt, bnd_b, bnd_e, cids;
     DevBuf<uint64_t> lists, sorted, fkeys, fsorted;
     DevBuf<uint8_t> tmp;
+    DevBuf<float> r_q, r_tau;  // redo launches
+    DevBuf<uint16_t> r_qf;
+    DevBuf<uint32_t> r_map;
+    DevBuf<int64_t> r_row;
     size_t tmp1 = 0, tmp2 = 0;
     HIP_TRY(dkm::ps_sort_tmp_bytes(B * cap, (int)B, &tmp1));
     HIP_TRY(dkm::ps_sort_tmp_bytes(B * C, (int)B, &tmp2));
@@ -1805,7 +1833,7 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
     } evg{ev};
     double acc_ms[4] = {0, 0, 0, 0};
     std::vector<uint32_t> hcnt(B);
-    std::vector<float> htau(3 * B), hretry(B);
+    std::vector<float> htau(3 * B);
     for (int64_t q0 = 0; q0 < nq; q0 += B) {
         const int b = (int)std::min<int64_t>(B, nq - q0);
         const int bp = (int)((b + 63) / 64 * 64);
@@ -1823,35 +1851,47 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
             HIP_TRY(hipMemcpy(tau.p, htau.data(), sizeof(float) * b, hipMemcpyHostToDevice));
         }
         HIP_TRY(hipMemset(cnt.p, 0, sizeof(uint32_t) * b));
-        HIP_TRY(dkm::ps_filter(dp, ps->xf.p, ps->n, ps->xn.p, qf.p, b, bp, l2, tau.p, cnt.p,
-                               lists.p, cap, nullptr));
-        // redo queries whose list is short (looser threshold, then everything) or overflowed
-        // (tighter threshold)
+        HIP_TRY(dkm::ps_filter(dp, ps->xf.p, ps->n, ps->xn.p, qf.p, b, bp, l2, tau.p, nullptr,
+                               cnt.p, lists.p, cap, nullptr));
+        // Redo the queries whose list is short (looser threshold, then every row) or overflowed
+        // (tighter threshold): only those queries, gathered into a small launch of their own.
         for (int round = 0;; ++round) {
             HIP_TRY(hipMemcpy(hcnt.data(), cnt.p, sizeof(uint32_t) * b, hipMemcpyDeviceToHost));
-            bool redo = false;
+            std::vector<uint32_t> ridx;
+            std::vector<int64_t> rrow;
+            std::vector<float> rtau;
             for (int i = 0; i < b; ++i) {
-                hretry[i] = -__builtin_inff();
                 if (hcnt[i] < (uint32_t)C) {
-                    hretry[i] = round == 0 ? htau[b + i] : __builtin_inff();
-                    hcnt[i] = 0;
-                    redo = true;
+                    rtau.push_back(round == 0 ? htau[b + i] : __builtin_inff());
                 } else if (hcnt[i] > (uint64_t)cap) {
-                    hretry[i] = round == 0 ? htau[2 * b + i] : -__builtin_inff();
                     if (round > 0)
                         return fail(DKM_ERR_CAPACITY, "query %lld: %u candidates exceed %lld",
                                     (long long)(q0 + i), hcnt[i], (long long)cap);
-                    hcnt[i] = 0;
-                    redo = true;
+                    rtau.push_back(htau[2 * b + i]);
+                } else {
+                    continue;
                 }
+                ridx.push_back((uint32_t)i);
+                rrow.push_back(i);
+                hcnt[i] = 0;
             }
-            if (!redo) break;
-            if (round == 2)
-                return fail(DKM_ERR_CAPACITY, "pre-scoring threshold did not converge");
+            if (ridx.empty()) break;
+            if (round == 2) return fail(DKM_ERR_CAPACITY, "pre-scoring threshold did not converge");
+            const int f = (int)ridx.size(), fp = (f + 63) / 64 * 64;
+            HIP_TRY(r_q.ensure((size_t)f * ps->dim));
+            HIP_TRY(r_qf.ensure((size_t)fp * dp));
+            HIP_TRY(r_map.ensure(f));
+            HIP_TRY(r_row.ensure(f));
+            HIP_TRY(r_tau.ensure(f));
             HIP_TRY(hipMemcpy(cnt.p, hcnt.data(), sizeof(uint32_t) * b, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(tau.p, hretry.data(), sizeof(float) * b, hipMemcpyHostToDevice));
-            HIP_TRY(dkm::ps_filter(dp, ps->xf.p, ps->n, ps->xn.p, qf.p, b, bp, l2, tau.p, cnt.p,
-                                   lists.p, cap, nullptr));
+            HIP_TRY(hipMemcpy(r_map.p, ridx.data(), sizeof(uint32_t) * f, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(r_row.p, rrow.data(), sizeof(int64_t) * f, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(r_tau.p, rtau.data(), sizeof(float) * f, hipMemcpyHostToDevice));
+            HIP_TRY(dkm::ps_gather_rows(q, (int)ps->dim, r_row.p, f, r_q.p, nullptr));
+            HIP_TRY(dkm::kmeans_to_bf16_frag(r_q.p, f, (int)ps->dim, ps->dim, dp, fp, r_qf.p,
+                                             nullptr));
+            HIP_TRY(dkm::ps_filter(dp, ps->xf.p, ps->n, ps->xn.p, r_qf.p, f, fp, l2, r_tau.p,
+                                   r_map.p, cnt.p, lists.p, cap, nullptr));
         }
         HIP_TRY(hipEventRecord(ev[1], nullptr));
         HIP_TRY(dkm::ps_bounds(cnt.p, b, cap, bnd_b.p, bnd_e.p, nullptr));

@@ -6,8 +6,9 @@
 
 // Largest sample of base rows the per-query threshold is read from (LDS of the selection).
 #define PS_SAMPLE_MAX 15360
-// Passing (row, query) pairs each filter thread stages in LDS before appending them.
-#define PS_SLOTS 16
+// Passing (row, query) pairs each filter wave stages in LDS before appending them
+// (a multiple of 64).
+#define PS_POOL 1536
 
 namespace dkm {
 bool ps_supported_dp(int dp);
@@ -17,7 +18,8 @@ hipError_t ps_dense(int dp, const uint16_t *xf, int64_t ns, const float *xn, con
 hipError_t ps_select(const float *dense, int64_t ld, int ns, int j0, int j1, int j2, float *tau,
                      int nq, hipStream_t st);
 hipError_t ps_filter(int dp, const uint16_t *xf, int64_t n, const float *xn, const uint16_t *qf,
-                     int nq, int nqp, bool l2, const float *tau, uint32_t *count, uint64_t *lists,
+                     int nq, int nqp, bool l2, const float *tau, const uint32_t *qmap,
+                     uint32_t *count, uint64_t *lists,
                      int64_t cap, hipStream_t st);
 hipError_t ps_norms(const float *x, int64_t n, int dim, const int64_t *rows, float *out,
                     hipStream_t st);

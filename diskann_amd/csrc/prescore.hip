@@ -119,14 +119,15 @@ __global__ __launch_bounds__(256) void ps_select_kernel(const float *dense, int6
 }
 
 // Full GEMM with the threshold filter: every (row, query) whose approximate distance d is at
-// or below tau[query] appends (key << 32 | row) to the query's list, the key ordering the
-// query's entries like d
-// (capacity cap; the counter keeps counting past it so overflow is visible). Queries with
-// tau = -inf take no part (used when only some queries are redone). Passing pairs are
-// staged in LDS during the sweep, in kSlots private slots per thread (a register counter,
-// no atomics), and appended to the global lists once the sweep is over, so no returning
-// atomic waits inside the tile loop; a thread whose slots are full appends directly.
-constexpr int kSlots = PS_SLOTS;
+// or below tau[query] appends (key << 32 | row) to the query's list (capacity cap; the
+// counter keeps counting past it so overflow is visible), the key ordering the query's
+// entries like d. Queries with tau = -inf take no part (used when only some queries are
+// redone). qmap (may be null) maps the launch's query q to its list / counter index. Passing
+// pairs are staged in a per-wave pool in LDS during the sweep (slots
+// handed out by ballot and lane rank, no atomics) and appended to the global lists once
+// the sweep is over, so no returning atomic waits inside the tile loop; pairs beyond a
+// full pool are appended directly.
+constexpr int kPool = PS_POOL;
 
 __device__ __forceinline__ void ps_append(uint32_t *count, uint64_t *lists, int64_t cap, int q,
                                           uint64_t key) {
@@ -138,14 +139,15 @@ template <int KS>
 __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, int64_t n,
                                                            const float *xn, const uint16_t *qf,
                                                            int nq, int nqp, bool l2,
-                                                           const float *tau, uint32_t *count,
-                                                           uint64_t *lists, int64_t cap) {
+                                                           const float *tau, const uint32_t *qmap,
+                                                           uint32_t *count, uint64_t *lists,
+                                                           int64_t cap) {
     extern __shared__ __align__(16) uint16_t sm[];
     float *tau_s = (float *)(sm + 2 * KS * 512);
-    uint64_t *skey = (uint64_t *)(tau_s + nqp);
-    uint16_t *sq = (uint16_t *)(skey + 256 * kSlots);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int half = lane >> 5, l32 = lane & 31;
+    uint64_t *skey = (uint64_t *)(tau_s + nqp) + wave * kPool;
+    uint16_t *sq = (uint16_t *)((uint64_t *)(tau_s + nqp) + 4 * kPool) + wave * kPool;
     const int64_t rtile = (int64_t)blockIdx.x * 8 + 2 * wave;
     bf16x8_t a0[KS], a1[KS];
     res_load_rows<KS>(xf + rtile * KS * 512 + lane * 8, a0, a1);
@@ -158,14 +160,22 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
     const float h1 = l2 && r1 < n ? 0.5f * xn[r1] : 0.0f;
     const bool v0 = r0 < n, v1 = r1 < n;
     const float tscale = l2 ? 0.5f : 1.0f;
-    int used = 0;
-    auto push = [&](int q, uint64_t key) {
-        if (used < kSlots) {
-            skey[tid * kSlots + used] = key;
-            sq[tid * kSlots + used] = (uint16_t)q;
-            ++used;
-        } else {
-            ps_append(count, lists, cap, q, key);
+    uint32_t filled = 0;  // the wave's pool entries (wave-uniform)
+    auto push = [&](bool pass, int q, uint64_t key) {
+        const uint64_t m = __ballot(pass);
+        if (m) {
+            if (pass) {
+                const uint32_t slot =
+                    filled + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                if (slot < (uint32_t)kPool) {
+                    skey[slot] = key;
+                    sq[slot] = (uint16_t)q;
+                } else {
+                    ps_append(count, lists, cap, qmap ? (int)qmap[q] : q, key);
+                }
+            }
+            filled += (uint32_t)__popcll(m);
         }
     };
     res_sweep<KS>(
@@ -186,12 +196,25 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
             for (int r = 0; r < 16; ++r) {
                 const int q = 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
                 const float u0 = pa[r] - h0, u1 = pb[r] - h1;
-                if (v0 && u0 >= 0.0f) push(q, ((uint64_t)orderable(-u0) << 32) | (uint32_t)r0);
-                if (v1 && u1 >= 0.0f) push(q, ((uint64_t)orderable(-u1) << 32) | (uint32_t)r1);
+                push(v0 && u0 >= 0.0f, q, ((uint64_t)orderable(-u0) << 32) | (uint32_t)r0);
+                push(v1 && u1 >= 0.0f, q, ((uint64_t)orderable(-u1) << 32) | (uint32_t)r1);
             }
         });
-    for (int j = 0; j < used; ++j)
-        ps_append(count, lists, cap, sq[tid * kSlots + j], skey[tid * kSlots + j]);
+    // the wave's pool: all counter increments first, then the stores (one round trip)
+    const uint32_t staged = min(filled, (uint32_t)kPool);
+    constexpr int kPer = kPool / 64;
+    uint32_t pos[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t i = (uint32_t)(j * 64 + lane);
+        if (i < staged) pos[j] = atomicAdd(&count[qmap ? qmap[sq[i]] : sq[i]], 1u);
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t i = (uint32_t)(j * 64 + lane);
+        if (i < staged && pos[j] < cap)
+            lists[(int64_t)(qmap ? qmap[sq[i]] : sq[i]) * cap + pos[j]] = skey[i];
+    }
 }
 
 // Row norms in f32 (the L2 approximate distance's |x|^2; any fixed order serves).
@@ -266,14 +289,14 @@ static hipError_t dense_t(const uint16_t *xf, int64_t ns, const float *xn, const
 
 template <int KS>
 static hipError_t filter_t(const uint16_t *xf, int64_t n, const float *xn, const uint16_t *qf,
-                           int nq, int nqp, bool l2, const float *tau, uint32_t *count,
-                           uint64_t *lists, int64_t cap, hipStream_t st) {
+                           int nq, int nqp, bool l2, const float *tau, const uint32_t *qmap,
+                           uint32_t *count, uint64_t *lists, int64_t cap, hipStream_t st) {
     const size_t lds = ps_filter_lds(KS * 16, nqp);
     hipError_t e = hipFuncSetAttribute((const void *)ps_filter_kernel<KS>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(ps_filter_kernel<KS>, dim3(blocks(n, 256)), dim3(256), lds, st, xf, n, xn,
-                       qf, nq, nqp, l2, tau, count, lists, cap);
+                       qf, nq, nqp, l2, tau, qmap, count, lists, cap);
     return hipGetLastError();
 }
 
@@ -289,7 +312,7 @@ static hipError_t filter_t(const uint16_t *xf, int64_t n, const float *xn, const
     }
 
 size_t ps_filter_lds(int dp, int nqp) {
-    return (size_t)2 * dp * 64 + 4 * (size_t)nqp + (size_t)256 * kSlots * 10;
+    return (size_t)2 * dp * 64 + 4 * (size_t)nqp + (size_t)4 * kPool * 10;
 }
 
 bool ps_supported_dp(int dp) {
@@ -305,9 +328,9 @@ hipError_t ps_dense(int dp, const uint16_t *xf, int64_t ns, const float *xn, con
 }
 
 hipError_t ps_filter(int dp, const uint16_t *xf, int64_t n, const float *xn, const uint16_t *qf,
-                     int nq, int nqp, bool l2, const float *tau, uint32_t *count, uint64_t *lists,
-                     int64_t cap, hipStream_t st) {
-#define CALL(K) filter_t<K>(xf, n, xn, qf, nq, nqp, l2, tau, count, lists, cap, st)
+                     int nq, int nqp, bool l2, const float *tau, const uint32_t *qmap,
+                     uint32_t *count, uint64_t *lists, int64_t cap, hipStream_t st) {
+#define CALL(K) filter_t<K>(xf, n, xn, qf, nq, nqp, l2, tau, qmap, count, lists, cap, st)
     PS_DISPATCH(dp / 16, CALL)
 #undef CALL
 }

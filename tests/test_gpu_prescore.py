@@ -83,6 +83,20 @@ def test_recall_on_clustered_unit_vectors():
     assert hits / (nq * k) >= 0.99
 
 
+@pytest.mark.parametrize("C", [8, 40])
+def test_small_candidate_counts(C):
+    # few candidates against many rows: the sample threshold is taken at a floor rank and
+    # the lists sized for it
+    rng = np.random.default_rng(C)
+    n, dim, nq, k = 40000, 64, 16, 5
+    x = rng.integers(-3, 4, (n, dim)).astype(np.float32)
+    qs = rng.integers(-3, 4, (nq, dim)).astype(np.float32)
+    ids, ds, _ = PreScorer(x, po.INNER_PRODUCT).search(qs, k, C)
+    for i in range(nq):
+        wi, wd = exact_topk(po.INNER_PRODUCT, x, qs[i], k)
+        assert np.array_equal(ids[i], wi) and ds[i].tobytes() == wd.tobytes(), i
+
+
 def test_many_queries_span_batches():
     # more queries than one filter launch holds (1024): the batches must not interfere
     rng = np.random.default_rng(3)

@@ -751,9 +751,44 @@ float dko_update_occlude_factor(int metric, float d_ik, float d_jk, float cur, f
 
 /* occlude_list, diskann/src/graph/index.rs:2570-2782. The pool must already be sorted
  * and truncated to max_occlusion. */
+/* Hybrid prune distance (HybridComputer, diskann-providers/src/model/graph/provider/
+ * async_/distances.rs:118-165): both full precision -> the metric; one full precision ->
+ * the direct PQ distance of that vector against the other's code; both codes -> code vs
+ * code (left = a). */
+static float hyb_dist(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                      int64_t dim, uint32_t a, int fa, uint32_t b, int fb) {
+    const float *va = vectors + (int64_t)a * vstride, *vb = vectors + (int64_t)b * vstride;
+    if (fa && fb) return dko_distance(cfg->metric, va, vb, dim);
+    const uint8_t *ca = cfg->codes + (int64_t)a * cfg->pq_M;
+    const uint8_t *cb = cfg->codes + (int64_t)b * cfg->pq_M;
+    if (fa) return dko_pq_direct(cfg->metric, cfg->pivots, dim, cfg->pq_offsets, cfg->pq_M, va, cb);
+    if (fb) return dko_pq_direct(cfg->metric, cfg->pivots, dim, cfg->pq_offsets, cfg->pq_M, vb, ca);
+    return dko_pq_qq(cfg->metric, cfg->pivots, dim, cfg->pq_offsets, cfg->pq_M, ca, cb);
+}
+
+static int in_set(const uint32_t *set, int64_t n, uint32_t id) {
+    for (int64_t i = 0; i < n; ++i)
+        if (set[i] == id) return 1;
+    return 0;
+}
+
+static int64_t occlude_h(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                         int64_t dim, const uint32_t *pool_ids, const float *pool_d,
+                         int64_t pool_len, uint32_t self_id, const uint8_t *isfp,
+                         uint32_t *out);
+
 int64_t dko_occlude_list(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
                          int64_t dim, const uint32_t *pool_ids, const float *pool_d,
                          int64_t pool_len, uint32_t self_id, uint32_t *out) {
+    return occlude_h(cfg, vectors, vstride, dim, pool_ids, pool_d, pool_len, self_id, NULL, out);
+}
+
+/* isfp (NULL: all full precision) marks the pool entries the Hybrid prune keeps in full
+ * precision. */
+static int64_t occlude_h(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                         int64_t dim, const uint32_t *pool_ids, const float *pool_d,
+                         int64_t pool_len, uint32_t self_id, const uint8_t *isfp,
+                         uint32_t *out) {
     if (pool_len == 0) return 0;
     float *occ = (float *)calloc((size_t)pool_len, sizeof(float));
     uint32_t *last = (uint32_t *)calloc((size_t)pool_len, sizeof(uint32_t));
@@ -784,11 +819,14 @@ int64_t dko_occlude_list(const dko_build_cfg *cfg, const float *vectors, int64_t
                     continue;
                 }
                 float d;
-                if (valid[rp])
+                if (!valid[rp])
+                    d = FLT_MAX;
+                else if (isfp)
+                    d = hyb_dist(cfg, vectors, vstride, dim, pool_ids[i], isfp[i], pool_ids[rp],
+                                 isfp[rp]);
+                else
                     d = dko_distance(cfg->metric, vi, vectors + (int64_t)pool_ids[rp] * vstride,
                                      dim);
-                else
-                    d = FLT_MAX;
                 of = dko_update_occlude_factor(cfg->metric, pool_d[i], d, of, cur_alpha);
                 if (of > cur_alpha) break;
             }
@@ -907,21 +945,40 @@ static int64_t prune_list_opt(const dko_build_cfg *cfg, const float *vectors, in
                               uint32_t *out, int force_saturate) {
     uint32_t *ids = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n + 1));
     float *ds = (float *)malloc(sizeof(float) * (size_t)(n + 1));
+    uint8_t *isfp = NULL;
+    uint32_t *fpset = NULL;
+    int64_t nfp = 0;
+    if (cfg->max_fp >= 0) {
+        /* fill(chain(once(location), list)): the first max_fp ids are full precision */
+        fpset = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n + 1));
+        if (nfp < cfg->max_fp) fpset[nfp++] = location;
+        for (int64_t i = 0; i < n && nfp < cfg->max_fp; ++i) fpset[nfp++] = list[i];
+        isfp = (uint8_t *)malloc((size_t)(n + 1));
+    }
+    const int loc_fp = fpset ? in_set(fpset, nfp, location) : 1;
     int64_t m = 0;
     const float *v = vectors + (int64_t)location * vstride;
     for (int64_t i = 0; i < n; ++i) {
         if (list[i] == location) continue;
         ids[m] = list[i];
-        ds[m] = dko_distance(cfg->metric, v, vectors + (int64_t)list[i] * vstride, dim);
+        if (fpset)
+            ds[m] = hyb_dist(cfg, vectors, vstride, dim, location, loc_fp, list[i],
+                             in_set(fpset, nfp, list[i]));
+        else
+            ds[m] = dko_distance(cfg->metric, v, vectors + (int64_t)list[i] * vstride, dim);
         m++;
     }
     sort_pool(ids, ds, m);
     if (m > cfg->max_occlusion) m = cfg->max_occlusion;
+    if (isfp)
+        for (int64_t i = 0; i < m; ++i) isfp[i] = (uint8_t)in_set(fpset, nfp, ids[i]);
     dko_build_cfg c2 = *cfg;
     if (force_saturate) c2.saturate = 2; /* forced: saturate regardless of alpha */
-    int64_t c = dko_occlude_list(&c2, vectors, vstride, dim, ids, ds, m, location, out);
+    int64_t c = occlude_h(&c2, vectors, vstride, dim, ids, ds, m, location, isfp, out);
     free(ids);
     free(ds);
+    free(isfp);
+    free(fpset);
     return c;
 }
 
@@ -993,17 +1050,40 @@ static int64_t search_and_prune(const dko_build_cfg *cfg, const float *vectors, 
     dko_stats st;
     search_core(&ix, 0, v, 1, &s, rid, rd, &rlen, cap, &st);
     if (rlen > cap) rlen = (uint32_t)cap;
+    /* Hybrid prune: fill(chain(once(self), chain(extras, record)).take(max_occlusion));
+     * the first max_fp of those ids are full precision */
+    uint32_t *fpset = NULL;
+    int64_t nfp = 0;
+    if (cfg->max_fp >= 0) {
+        int64_t lim = cfg->max_fp < cfg->max_occlusion ? cfg->max_fp : cfg->max_occlusion;
+        fpset = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(lim + 1));
+        if (nfp < lim) fpset[nfp++] = id;
+        for (int64_t e = 0; e < nextras && nfp < lim; ++e) fpset[nfp++] = extras[e];
+        for (uint32_t r = 0; r < rlen && nfp < lim; ++r) fpset[nfp++] = rid[r];
+    }
+    const int self_fp = fpset ? in_set(fpset, nfp, id) : 1;
     /* the in-memory providers' prune working set is a pass-through view: every id is
      * available, so all extras join the record before sorting */
     for (int64_t e = 0; e < nextras; ++e) {
         rid[rlen] = extras[e];
-        rd[rlen] = dko_distance(cfg->metric, v, vectors + (int64_t)extras[e] * vstride, dim);
+        if (fpset)
+            rd[rlen] = hyb_dist(cfg, vectors, vstride, dim, id, self_fp, extras[e],
+                                in_set(fpset, nfp, extras[e]));
+        else
+            rd[rlen] = dko_distance(cfg->metric, v, vectors + (int64_t)extras[e] * vstride, dim);
         rlen++;
     }
     sort_pool(rid, rd, rlen);
     int64_t plen = rlen;
     if (plen > cfg->max_occlusion) plen = cfg->max_occlusion;
-    int64_t c = dko_occlude_list(cfg, vectors, vstride, dim, rid, rd, plen, id, out);
+    uint8_t *isfp = NULL;
+    if (fpset) {
+        isfp = (uint8_t *)malloc((size_t)(plen + 1));
+        for (int64_t i = 0; i < plen; ++i) isfp[i] = (uint8_t)in_set(fpset, nfp, rid[i]);
+    }
+    int64_t c = occlude_h(cfg, vectors, vstride, dim, rid, rd, plen, id, isfp, out);
+    free(isfp);
+    free(fpset);
     free(rid);
     free(rd);
     scratch_free(&s);

@@ -116,6 +116,15 @@ typedef struct {
     int64_t max_occlusion;     /* 750 */
     int saturate;              /* saturate_after_prune */
     int64_t max_backedges;     /* insert(): number of back edges (default R_max) */
+    /* Hybrid prune (HybridPruneAccessor, diskann-providers/src/model/graph/provider/
+     * async_/inmem/product.rs:144-221): the first max_fp ids a prune fills are full
+     * precision, the others use their PQ codes (distances through the direct PQ
+     * functions). max_fp < 0: every id full precision (codes unused). */
+    int64_t max_fp;
+    const uint8_t *codes;      /* rows x pq_M */
+    const float *pivots;       /* K x dim */
+    const int64_t *pq_offsets; /* pq_M + 1 */
+    int64_t pq_M;
 } dko_build_cfg;
 
 /* occlude_list on a pool sorted by distance (diskann/src/graph/index.rs:2570-2782).

@@ -47,6 +47,8 @@ class BuildCfg(C.Structure):
         ("metric", C.c_int), ("R", C.c_int64), ("R_max", C.c_int64),
         ("L_build", C.c_int64), ("alpha", C.c_float), ("max_occlusion", C.c_int64),
         ("saturate", C.c_int), ("max_backedges", C.c_int64),
+        ("max_fp", C.c_int64), ("codes", C.c_void_p), ("pivots", C.c_void_p),
+        ("pq_offsets", C.c_void_p), ("pq_M", C.c_int64),
     ]
 
 
@@ -240,8 +242,10 @@ class OracleIndex:
 
 def build_graph(metric, vectors, R, R_max, L_build, alpha, starts, ids, batch=1,
                 intra_batch=-1, start_adj=None, max_occlusion=750, saturate=False,
-                max_backedges=None):
-    """Vamana build on the oracle. Returns (adj, deg) for all rows of `vectors`."""
+                max_backedges=None, max_fp=None, codes=None, pivots=None, pq_offsets=None):
+    """Vamana build on the oracle. Returns (adj, deg) for all rows of `vectors`.
+    max_fp (with codes / pivots / pq_offsets): Hybrid prune with that many full-precision
+    ids per prune; None: full precision throughout."""
     vectors = np.ascontiguousarray(vectors, dtype=np.float32)
     rows = vectors.shape[0]
     adj = np.zeros((rows, R_max), dtype=np.uint32)
@@ -251,8 +255,17 @@ def build_graph(metric, vectors, R, R_max, L_build, alpha, starts, ids, batch=1,
             adj[s, :len(lst)] = lst
             deg[s] = len(lst)
     g = Graph(rows, R_max, _p(adj), _p(deg))
+    keep = []
+    if max_fp is None:
+        hyb = (-1, None, None, None, 0)
+    else:
+        cd = np.ascontiguousarray(codes, dtype=np.uint8)
+        pv = np.ascontiguousarray(pivots, dtype=np.float32)
+        off = np.ascontiguousarray(pq_offsets, dtype=np.int64)
+        keep = [cd, pv, off]
+        hyb = (int(max_fp), _p(cd), _p(pv), _p(off), off.shape[0] - 1)
     cfg = BuildCfg(metric, R, R_max, L_build, alpha, max_occlusion, int(saturate),
-                   R_max if max_backedges is None else max_backedges)
+                   R_max if max_backedges is None else max_backedges, *hyb)
     starts = np.ascontiguousarray(starts, dtype=np.uint32)
     ids = np.ascontiguousarray(ids, dtype=np.uint32)
     lib().dko_build(C.byref(cfg), _p(vectors), vectors.shape[1], vectors.shape[1],
@@ -265,7 +278,7 @@ def occlude_list(metric, vectors, pool_ids, pool_d, self_id, R, alpha, saturate=
     vectors = np.ascontiguousarray(vectors, dtype=np.float32)
     pool_ids = np.ascontiguousarray(pool_ids, dtype=np.uint32)
     pool_d = np.ascontiguousarray(pool_d, dtype=np.float32)
-    cfg = BuildCfg(metric, R, R, 0, alpha, 750, int(saturate), R)
+    cfg = BuildCfg(metric, R, R, 0, alpha, 750, int(saturate), R, -1)
     out = np.zeros(max(R, 1) + pool_ids.shape[0], dtype=np.uint32)
     n = lib().dko_occlude_list(C.byref(cfg), _p(vectors), vectors.shape[1], vectors.shape[1],
                                _p(pool_ids), _p(pool_d), pool_ids.shape[0], self_id, _p(out))

@@ -18,7 +18,7 @@ EXPORTS = [
     "dkm_index_create", "dkm_index_destroy", "dkm_set_vectors", "dkm_set_start_points",
     "dkm_set_graph", "dkm_get_graph", "dkm_set_pq", "dkm_set_codes", "dkm_encode_index",
     "dkm_get_codes", "dkm_search_batch", "dkm_search_batch_device", "dkm_index_synchronize",
-    "dkm_profile", "dkm_profile_read", "dkm_set_tuning", "dkm_build", "dkm_medoid",
+    "dkm_profile", "dkm_profile_read", "dkm_set_tuning", "dkm_build", "dkm_set_build_hybrid", "dkm_medoid",
     "dkm_bruteforce", "dkm_search_batch_device_stats", "dkm_autotune",
     "dkm_server_start", "dkm_server_slots", "dkm_server_submit",
     "dkm_server_submit_batch", "dkm_server_poll", "dkm_server_pending", "dkm_server_stop",
@@ -81,6 +81,7 @@ def _declare(L):
         "dkm_profile_read": [P, C.POINTER(C.c_int64), C.POINTER(C.c_double)],
         "dkm_set_tuning": [P, I64, I],
         "dkm_build": [P, P, I64, I64, I64, U32, C.c_float, I64, I],
+        "dkm_set_build_hybrid": [P, I64],
         "dkm_medoid": [P, P, C.POINTER(C.c_int64)],
         "dkm_bruteforce": [P, P, I64, I64, P, P],
         "dkm_autotune": [P, I, P, I64, U32, U32, U32, C.POINTER(C.c_int64), C.POINTER(C.c_int)],

@@ -111,6 +111,7 @@ struct dkm_index {
     int64_t tune_hslots = 0;
     int tune_adc_mode = 1;              // PQ distances: 0 = LDS table, 1 = direct
     int pq_len = 0;                     // uniform PQ chunk width usable by the fast path
+    int64_t build_max_fp = -1;          // Hybrid prune: full-precision ids per prune (-1 = all)
     // staging for the host-buffer API
     DevBuf<float> s_q, s_d;
     DevBuf<uint32_t> s_k, s_L, s_ids, s_cnt, s_cmps, s_hops, s_status;
@@ -853,6 +854,13 @@ dkm_status dkm_search_batch(dkm_index *ix, int strategy, const float *queries, i
 // Graph construction (multi_insert batches)
 // ------------------------------------------------------------------------------------
 
+dkm_status dkm_set_build_hybrid(dkm_index *ix, int64_t max_fp_vecs_per_prune) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(ix->mu);
+    ix->build_max_fp = max_fp_vecs_per_prune < 0 ? -1 : max_fp_vecs_per_prune;
+    return DKM_OK;
+}
+
 dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batch,
                      int64_t intra_batch, uint32_t L_build, float alpha, int64_t max_occlusion,
                      int saturate) {
@@ -863,6 +871,8 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
     if (max_occlusion < 1 || max_occlusion > 4096) return fail(DKM_ERR_INVALID_ARG, "bad max_occlusion");
     if (ix->R > 256) return fail(DKM_ERR_INVALID_ARG, "pruned degree above 256 is not supported");
     if (!ix->starts_set) return fail(DKM_ERR_NOT_READY, "start points not set");
+    if (ix->build_max_fp >= 0 && (!ix->d_codes || !ix->codes_set))
+        return fail(DKM_ERR_NOT_READY, "the Hybrid prune needs the PQ codes of every row");
     for (int64_t i = 0; i < n; ++i)
         if (ids[i] >= ix->max_points) return fail(DKM_ERR_INDEX, "id %u is not a data row", ids[i]);
     if (n == 0) return DKM_OK;
@@ -927,6 +937,11 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
     bp.pend_cnt = pcnt.p;
     bp.status = bstat.p;
     bp.cache_accepted = (size_t)ix->R * ix->dim * sizeof(float) <= 48 * 1024;
+    bp.max_fp = ix->build_max_fp < 0 ? -1 : (int)std::min<int64_t>(ix->build_max_fp, 0x7FFFFFFF);
+    bp.codes = ix->d_codes;
+    bp.pivots = ix->d_pivots;
+    bp.pq_off = ix->d_offsets;
+    bp.pq_M = (int)ix->M;
     std::vector<uint32_t> h_pend, h_cnt;
     for (int64_t b0 = 0; b0 < n; b0 += B) {
         const int nb = (int)std::min<int64_t>(B, n - b0);

@@ -42,6 +42,8 @@ struct PruneLds {
     float *cand;      // candidate / location vector (dim)
     float *dist;      // per-half distances
     int32_t *misc;
+    uint8_t *in_fp;   // Hybrid prune: full-precision flag of in_id entries
+    uint8_t *pfp;     // ... and of the sorted pool
 };
 
 __host__ __device__ inline size_t a16(size_t x) { return (x + 15) & ~size_t(15); }
@@ -49,7 +51,8 @@ __host__ __device__ inline size_t a16(size_t x) { return (x + 15) & ~size_t(15);
 __host__ __device__ inline size_t prune_lds_bytes(const BuildParams &p) {
     size_t b = a16(sizeof(uint64_t) * p.sort_cap) + 2 * a16(sizeof(uint32_t) * p.sort_cap) +
                4 * a16(sizeof(uint32_t) * p.max_occlusion) + a16(sizeof(uint32_t) * p.R) +
-               a16(sizeof(float) * p.dim) + a16(sizeof(float) * kHalves) + 64;
+               a16(sizeof(float) * p.dim) + a16(sizeof(float) * kHalves) + 64 +
+               a16(p.sort_cap) + a16(p.max_occlusion);
     if (p.cache_accepted) b += a16(sizeof(float) * (size_t)p.R * p.dim);
     return b;
 }
@@ -73,6 +76,8 @@ __device__ inline PruneLds carve_prune(const BuildParams &p, char *base) {
     s.cand = (float *)take(sizeof(float) * p.dim);
     s.dist = (float *)take(sizeof(float) * kHalves);
     s.misc = (int32_t *)take(64);
+    s.in_fp = (uint8_t *)take(p.sort_cap);
+    s.pfp = (uint8_t *)take(p.max_occlusion);
     s.accv = p.cache_accepted ? (float *)take(sizeof(float) * (size_t)p.R * p.dim) : nullptr;
     return s;
 }
@@ -87,6 +92,34 @@ __device__ inline float metric_distance(int metric, const float *a, const float 
     if (metric == METRIC_L2) return half_distance<METRIC_L2>(a, b, dim, hp);
     if (metric == METRIC_IP) return half_distance<METRIC_IP>(a, b, dim, hp);
     return half_distance<METRIC_COSINE>(a, b, dim, hp);
+}
+
+// HybridComputer (diskann-providers/src/model/graph/provider/async_/distances.rs:118-165)
+// between rows a and b by one half-wave: both full precision -> the metric on the vectors
+// va, vb; one full precision -> the direct PQ distance of that vector against the other's
+// code; both quantized -> code against code (a on the left). Each group of eight lanes of
+// the half computes the PQ forms (the same value); the result is valid in hp == 0.
+__device__ inline float hybrid_distance(const BuildParams &p, uint32_t a, bool fa,
+                                        const float *va, uint32_t b, bool fb, const float *vb,
+                                        int hp) {
+    if (fa && fb) return metric_distance(p.metric, va, vb, p.dim, hp);
+    const uint8_t *ca = p.codes + (size_t)a * p.pq_M;
+    const uint8_t *cb = p.codes + (size_t)b * p.pq_M;
+    const int l = hp & 7;
+    if (fa) return pq_direct8_rt(p.metric, p.pivots, p.dim, p.pq_off, p.pq_M, va, nullptr, cb, l);
+    if (fb) return pq_direct8_rt(p.metric, p.pivots, p.dim, p.pq_off, p.pq_M, vb, nullptr, ca, l);
+    return pq_direct8_rt(p.metric, p.pivots, p.dim, p.pq_off, p.pq_M, nullptr, ca, cb, l);
+}
+
+// Full-precision membership (HybridPruneAccessor::fill keeps the first max_fp ids of the
+// fill iterator in a hash set, product.rs:184-191): `id`, whose own fill position is pos,
+// is full precision iff it is among fill(0) .. fill(nfp - 1).
+template <class Fill>
+__device__ inline uint8_t fp_member(uint32_t id, int pos, int nfp, Fill fill) {
+    if (pos < nfp) return 1;
+    for (int f = 0; f < nfp; ++f)
+        if (fill(f) == id) return 1;
+    return 0;
 }
 
 // Stable sort of s.in_id / s.in_d [0, n) by distance into s.pid / s.pd (first `keep`).
@@ -117,6 +150,7 @@ __device__ inline void sort_pool(const BuildParams &p, PruneLds &s, int n, int k
         uint32_t src = (uint32_t)(s.keys[r] & 0xFFFFFFFFu);
         s.pid[r] = s.in_id[src];
         s.pd[r] = s.in_d[src];
+        if (p.max_fp >= 0) s.pfp[r] = s.in_fp[src];
     }
     __syncthreads();
 }
@@ -163,7 +197,10 @@ __device__ inline int occlude(const BuildParams &p, PruneLds &s, int P, uint32_t
                 const float *av = !need ? s.cand
                                   : (s.accv ? s.accv + (size_t)slot * p.dim
                                             : p.vectors + (size_t)s.pid[rp] * p.vstride);
-                float d = metric_distance(p.metric, s.cand, av, p.dim, hp);
+                const float d = p.max_fp >= 0 && need
+                                    ? hybrid_distance(p, cid, s.pfp[i] != 0, s.cand, s.pid[rp],
+                                                      s.pfp[rp] != 0, av, hp)
+                                    : metric_distance(p.metric, s.cand, av, p.dim, hp);
                 if (slot < found && hp == 0) s.dist[half] = need ? d : 0.0f;
                 __syncthreads();
                 if (t == 0) {
@@ -278,12 +315,29 @@ __global__ __launch_bounds__(kPruneThreads) void prune_search_kernel(BuildParams
     }
     const float *self_v = p.vectors + (size_t)self_id * p.vstride;
     for (int e = t; e < p.dim; e += blockDim.x) s.cand[e] = self_v[e];
+    // Hybrid: fill order chain(once(self), chain(extras, record)).take(max_occlusion)
+    const bool hyb = p.max_fp >= 0;
+    int nfp = 0;
+    if (hyb) {
+        __syncthreads();  // record and extras ids visible to every thread
+        const int lim = p.max_fp < p.max_occlusion ? p.max_fp : p.max_occlusion;
+        nfp = 1 + want + nrec < lim ? 1 + want + nrec : lim;
+        auto fill = [&](int f) -> uint32_t {
+            return f == 0 ? self_id : (f <= want ? s.in_id[nrec + f - 1] : s.in_id[f - 1 - want]);
+        };
+        for (int i = t; i < n; i += blockDim.x)
+            s.in_fp[i] = fp_member(s.in_id[i], i < nrec ? 1 + want + i : 1 + i - nrec, nfp, fill);
+    }
     __syncthreads();
+    const bool self_fp = !hyb || nfp >= 1;
     for (int e0 = 0; e0 < want; e0 += kHalves) {
         int e = e0 + half;
         bool ok = e < want;
         uint32_t id = ok ? s.in_id[nrec + e] : self_id;
-        float d = metric_distance(p.metric, s.cand, p.vectors + (size_t)id * p.vstride, p.dim, hp);
+        const float *v = p.vectors + (size_t)id * p.vstride;
+        float d = hyb ? hybrid_distance(p, self_id, self_fp, s.cand, id,
+                                        ok ? s.in_fp[nrec + e] != 0 : self_fp, v, hp)
+                      : metric_distance(p.metric, s.cand, v, p.dim, hp);
         if (ok && hp == 0) s.in_d[nrec + e] = d;
     }
     __syncthreads();
@@ -308,12 +362,28 @@ __global__ __launch_bounds__(kPruneThreads) void prune_list_kernel(BuildParams p
     const int half = t >> 5, hp = t & 31;
     const uint32_t loc = p.batch_ids[b];
     int n = 0;
-    // drop self occurrences, keep order
     const uint32_t *lst = lists + (size_t)b * list_stride;
     int len = (int)list_len[b];
+    if (len > 8 * p.sort_cap) len = 8 * p.sort_cap;
+    // Hybrid: fill order chain(once(location), list); flags by list position in the sort
+    // keys (unused until the sort)
+    const bool hyb = p.max_fp >= 0;
+    uint8_t *lfp = (uint8_t *)s.keys;
+    int nfp = 0;
+    if (hyb) {
+        nfp = 1 + len < p.max_fp ? 1 + len : p.max_fp;
+        auto fill = [&](int f) -> uint32_t { return f == 0 ? loc : lst[f - 1]; };
+        for (int i = t; i < len; i += blockDim.x) lfp[i] = fp_member(lst[i], 1 + i, nfp, fill);
+        __syncthreads();
+    }
+    const bool loc_fp = !hyb || nfp >= 1;
+    // drop self occurrences, keep order
     if (t == 0) {
         for (int i = 0; i < len && n < p.sort_cap; ++i)
-            if (lst[i] != loc) s.in_id[n++] = lst[i];
+            if (lst[i] != loc) {
+                if (hyb) s.in_fp[n] = lfp[i];
+                s.in_id[n++] = lst[i];
+            }
         s.misc[4] = n;
     }
     const float *lv = p.vectors + (size_t)loc * p.vstride;
@@ -324,7 +394,10 @@ __global__ __launch_bounds__(kPruneThreads) void prune_list_kernel(BuildParams p
         int e = e0 + half;
         bool ok = e < n;
         uint32_t id = ok ? s.in_id[e] : loc;
-        float d = metric_distance(p.metric, s.cand, p.vectors + (size_t)id * p.vstride, p.dim, hp);
+        const float *v = p.vectors + (size_t)id * p.vstride;
+        float d = hyb ? hybrid_distance(p, loc, loc_fp, s.cand, id, ok ? s.in_fp[e] != 0 : loc_fp,
+                                        v, hp)
+                      : metric_distance(p.metric, s.cand, v, p.dim, hp);
         if (ok && hp == 0) s.in_d[e] = d;
     }
     __syncthreads();
@@ -403,11 +476,24 @@ __global__ __launch_bounds__(kPruneThreads) void backedge_kernel(BuildParams p,
         if (t == 0) row[p.gstride - 1] = (uint32_t)n;
         return;
     }
-    // robust_prune_list(target, merged list): distances from the target, drop self
+    // robust_prune_list(target, merged list): Hybrid fill order chain(once(target), list)
+    const bool hyb = p.max_fp >= 0;
+    int nfp = 0;
+    if (hyb) {
+        nfp = 1 + n < p.max_fp ? 1 + n : p.max_fp;
+        auto fill = [&](int f) -> uint32_t { return f == 0 ? target : s.in_id[f - 1]; };
+        for (int j = t; j < n; j += blockDim.x) s.in_fp[j] = fp_member(s.in_id[j], 1 + j, nfp, fill);
+        __syncthreads();
+    }
+    const bool tgt_fp = !hyb || nfp >= 1;
+    // distances from the target, drop self
     if (t == 0) {
         int m = 0;
         for (int j = 0; j < n; ++j)
-            if (s.in_id[j] != target) s.in_id[m++] = s.in_id[j];
+            if (s.in_id[j] != target) {
+                if (hyb) s.in_fp[m] = s.in_fp[j];
+                s.in_id[m++] = s.in_id[j];
+            }
         s.misc[4] = m;
     }
     const float *lv = p.vectors + (size_t)target * p.vstride;
@@ -418,7 +504,10 @@ __global__ __launch_bounds__(kPruneThreads) void backedge_kernel(BuildParams p,
         int e = e0 + half;
         bool ok = e < m;
         uint32_t id = ok ? s.in_id[e] : target;
-        float d = metric_distance(p.metric, s.cand, p.vectors + (size_t)id * p.vstride, p.dim, hp);
+        const float *v = p.vectors + (size_t)id * p.vstride;
+        float d = hyb ? hybrid_distance(p, target, tgt_fp, s.cand, id,
+                                        ok ? s.in_fp[e] != 0 : tgt_fp, v, hp)
+                      : metric_distance(p.metric, s.cand, v, p.dim, hp);
         if (ok && hp == 0) s.in_d[e] = d;
     }
     __syncthreads();

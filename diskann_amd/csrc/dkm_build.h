@@ -30,6 +30,14 @@ struct BuildParams {
     uint32_t *pend;        // batch x R
     uint32_t *pend_cnt;
     uint32_t *status;
+    // Hybrid prune (HybridPruneAccessor, diskann-providers/src/model/graph/provider/
+    // async_/inmem/product.rs:143-230): the first max_fp ids of a prune's fill order are
+    // full precision, every other id is its PQ code. max_fp < 0: all full precision.
+    int max_fp;
+    const uint8_t *codes;  // rows x pq_M
+    const float *pivots;   // K x dim
+    const int32_t *pq_off; // pq_M + 1 chunk offsets
+    int pq_M;
 };
 
 size_t build_prune_lds_bytes(const BuildParams &p);

@@ -248,5 +248,99 @@ __device__ inline float square_norm_ref(const float *x, int len) {
     return (a0 + a2) + (a1 + a3);
 }
 
+// Direct PQ distances (FixedChunkPQTable::{l2_distance, inner_product, cosine_distance,
+// qq_*}, diskann-providers/src/model/pq/fixed_chunk_pq_table.rs:32-56, 220-365) by a
+// group of eight lanes, lane l holding lane l of the f32x8 accumulator. Left operand: the
+// full-precision row lq, or (lq == nullptr) the pivots of lcode; right: the pivots of
+// rcode. For each chunk the lane computes its lane of the simd_op vector over the chunk's
+// slices (chunk-local element o into partial o % 32 for L2/IP with Strategy4x1, o % 16 for
+// cosine with Strategy2x4, then the masked remainder FMA) and adds it to the running
+// accumulator, which sum_tree (shuffles by 4, 2, 1 inside the group) reduces after the last
+// chunk. Result valid in lane 0 of the group. METRIC_COSNORM uses the cosine functions.
+__device__ __forceinline__ float down8(float v, int d) { return __shfl_down(v, d, 8); }
+
+__device__ __forceinline__ float tree8(float v) {
+    float u = v + down8(v, 4);
+    float w = u + down8(u, 2);
+    return w + down8(w, 1);
+}
+
+template <int METRIC>
+__device__ inline float pq_direct8(const float *pivots, int dim, const int32_t *offsets, int M,
+                                   const float *lq, const uint8_t *lcode, const uint8_t *rcode,
+                                   int l) {
+    constexpr bool COS = METRIC == METRIC_COSINE || METRIC == METRIC_COSNORM;
+    float acc = 0.0f, ax = 0.0f, ay = 0.0f, axy = 0.0f;
+    for (int c = 0; c < M; ++c) {
+        const int b = offsets[c], len = offsets[c + 1] - b;
+        const float *L = lq ? lq + b : pivots + (int64_t)lcode[c] * dim + b;
+        const float *R = pivots + (int64_t)rcode[c] * dim + b;
+        const int d8 = len & ~7, rem = len & 7;
+        if (COS) {
+            float nx[2] = {0.0f, 0.0f}, ny[2] = {0.0f, 0.0f}, xy[2] = {0.0f, 0.0f};
+            for (int v = 0; 8 * v < d8; ++v) {
+                const float a = L[8 * v + l], r = R[8 * v + l];
+                nx[v & 1] = fma_(a, a, nx[v & 1]);
+                ny[v & 1] = fma_(r, r, ny[v & 1]);
+                xy[v & 1] = fma_(a, r, xy[v & 1]);
+            }
+            float cx = nx[0] + nx[1], cy = ny[0] + ny[1], cxy = xy[0] + xy[1];
+            if (rem) {
+                const float a = l < rem ? L[d8 + l] : 0.0f, r = l < rem ? R[d8 + l] : 0.0f;
+                cx = fma_(a, a, cx);
+                cy = fma_(r, r, cy);
+                cxy = fma_(a, r, cxy);
+            }
+            ax = ax + cx;
+            ay = ay + cy;
+            axy = axy + cxy;
+        } else {
+            float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int v = 0; 8 * v < d8; ++v) {
+                const float a = L[8 * v + l], r = R[8 * v + l];
+                if (METRIC == METRIC_IP) {
+                    s[v & 3] = fma_(a, r, s[v & 3]);
+                } else {
+                    const float d = a - r;
+                    s[v & 3] = fma_(d, d, s[v & 3]);
+                }
+            }
+            float t = (s[0] + s[1]) + (s[2] + s[3]);
+            if (rem) {
+                const float a = l < rem ? L[d8 + l] : 0.0f, r = l < rem ? R[d8 + l] : 0.0f;
+                if (METRIC == METRIC_IP) {
+                    t = fma_(a, r, t);
+                } else {
+                    const float d = a - r;
+                    t = fma_(d, d, t);
+                }
+            }
+            acc = acc + t;
+        }
+    }
+    if (COS) {
+        const float normx = tree8(ax), normy = tree8(ay), prod = tree8(axy);
+        const float denom = __builtin_sqrtf(normx) * __builtin_sqrtf(normy);
+        float sim;
+        if (normx < 1.17549435e-38f || normy < 1.17549435e-38f) {
+            sim = 0.0f;
+        } else {
+            const float v = prod / denom;
+            sim = __builtin_fmaxf(-1.0f, __builtin_fminf(1.0f, v));
+        }
+        return 1.0f - sim;
+    }
+    const float sum = tree8(acc);
+    return METRIC == METRIC_IP ? -sum : sum;
+}
+
+__device__ inline float pq_direct8_rt(int metric, const float *pivots, int dim,
+                                      const int32_t *offsets, int M, const float *lq,
+                                      const uint8_t *lcode, const uint8_t *rcode, int l) {
+    if (metric == METRIC_L2) return pq_direct8<METRIC_L2>(pivots, dim, offsets, M, lq, lcode, rcode, l);
+    if (metric == METRIC_IP) return pq_direct8<METRIC_IP>(pivots, dim, offsets, M, lq, lcode, rcode, l);
+    return pq_direct8<METRIC_COSINE>(pivots, dim, offsets, M, lq, lcode, rcode, l);
+}
+
 
 }  // namespace dkm

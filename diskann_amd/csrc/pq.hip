@@ -70,14 +70,6 @@ hipError_t launch_distance_batch(int metric, int dim, const float *queries, int6
 // the masked remainder FMA), adds it to the running accumulator, and the accumulator is
 // reduced by sum_tree (shuffles by 4, 2, 1) after the last chunk.
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ float down8(float v, int d) { return __shfl_down(v, d, 8); }
-
-__device__ __forceinline__ float tree8(float v) {
-    float u = v + down8(v, 4);
-    float w = u + down8(u, 2);
-    return w + down8(w, 1);
-}
-
 template <int METRIC, bool QQ>
 __global__ __launch_bounds__(256) void pq_direct_kernel(const float *pivots, int dim,
                                                         const int32_t *offsets, int M,
@@ -93,71 +85,8 @@ __global__ __launch_bounds__(256) void pq_direct_kernel(const float *pivots, int
     const int64_t pr = pair < total ? pair : total - 1;  // all eight lanes stay in the shuffles
     const int64_t qi = pr / m;
     const uint8_t *code = codes + (int64_t)ids[pr] * M;
-    float acc = 0.0f, ax = 0.0f, ay = 0.0f, axy = 0.0f;
-    for (int c = 0; c < M; ++c) {
-        const int b = offsets[c], len = offsets[c + 1] - b;
-        const float *L = QQ ? pivots + (int64_t)query_codes[qi * M + c] * dim + b
-                            : queries + qi * dim + b;
-        const float *R = pivots + (int64_t)code[c] * dim + b;
-        const int d8 = len & ~7, rem = len & 7;
-        if (METRIC == METRIC_COSINE) {
-            float nx[2] = {0.0f, 0.0f}, ny[2] = {0.0f, 0.0f}, xy[2] = {0.0f, 0.0f};
-            for (int v = 0; 8 * v < d8; ++v) {
-                const float a = L[8 * v + l], r = R[8 * v + l];
-                nx[v & 1] = fma_(a, a, nx[v & 1]);
-                ny[v & 1] = fma_(r, r, ny[v & 1]);
-                xy[v & 1] = fma_(a, r, xy[v & 1]);
-            }
-            float cx = nx[0] + nx[1], cy = ny[0] + ny[1], cxy = xy[0] + xy[1];
-            if (rem) {
-                const float a = l < rem ? L[d8 + l] : 0.0f, r = l < rem ? R[d8 + l] : 0.0f;
-                cx = fma_(a, a, cx);
-                cy = fma_(r, r, cy);
-                cxy = fma_(a, r, cxy);
-            }
-            ax = ax + cx;
-            ay = ay + cy;
-            axy = axy + cxy;
-        } else {
-            float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (int v = 0; 8 * v < d8; ++v) {
-                const float a = L[8 * v + l], r = R[8 * v + l];
-                if (METRIC == METRIC_IP) {
-                    s[v & 3] = fma_(a, r, s[v & 3]);
-                } else {
-                    const float d = a - r;
-                    s[v & 3] = fma_(d, d, s[v & 3]);
-                }
-            }
-            float t = (s[0] + s[1]) + (s[2] + s[3]);
-            if (rem) {
-                const float a = l < rem ? L[d8 + l] : 0.0f, r = l < rem ? R[d8 + l] : 0.0f;
-                if (METRIC == METRIC_IP) {
-                    t = fma_(a, r, t);
-                } else {
-                    const float d = a - r;
-                    t = fma_(d, d, t);
-                }
-            }
-            acc = acc + t;
-        }
-    }
-    float res;
-    if (METRIC == METRIC_COSINE) {
-        const float normx = tree8(ax), normy = tree8(ay), prod = tree8(axy);
-        const float denom = __builtin_sqrtf(normx) * __builtin_sqrtf(normy);
-        float sim;
-        if (normx < 1.17549435e-38f || normy < 1.17549435e-38f) {
-            sim = 0.0f;
-        } else {
-            const float v = prod / denom;
-            sim = __builtin_fmaxf(-1.0f, __builtin_fminf(1.0f, v));
-        }
-        res = 1.0f - sim;
-    } else {
-        const float sum = tree8(acc);
-        res = METRIC == METRIC_IP ? -sum : sum;
-    }
+    const float res = pq_direct8<METRIC>(pivots, dim, offsets, M, QQ ? nullptr : queries + qi * dim,
+                                         QQ ? query_codes + qi * M : nullptr, code, l);
     if (pair < total && l == 0) out[pair] = res;
 }
 

@@ -160,6 +160,17 @@ class DiskANNIndex:
         check(self.lib.dkm_get_codes(self._h, _p(out), row0, n), self.lib)
         return out
 
+    def set_build_hybrid(self, enabled=True, max_fp_vecs_per_prune=None):
+        """Hybrid prune for build() (common::Hybrid::new(max_fp_vecs_per_prune)): each prune
+        keeps the first max_fp_vecs_per_prune ids of its fill order in full precision and
+        compares the others through their PQ codes; None = Hybrid::new(None). enabled=False
+        restores full precision everywhere. Needs set_pq and the codes of every row."""
+        m = -1 if not enabled else (2**31 - 1 if max_fp_vecs_per_prune is None
+                                    else int(max_fp_vecs_per_prune))
+        if enabled and m < 0:
+            raise ValueError("max_fp_vecs_per_prune must be >= 0")
+        check(self.lib.dkm_set_build_hybrid(self._h, m), self.lib)
+
     def build(self, ids=None, batch=1, intra_batch=-1, L_build=100, alpha=1.2,
               max_occlusion=750, saturate=False):
         """Insert rows `ids` (default: every data row, in order) with multi_insert

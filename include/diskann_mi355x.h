@@ -176,6 +176,16 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
                      int64_t intra_batch, uint32_t L_build, float alpha, int64_t max_occlusion,
                      int saturate);
 
+/* Hybrid prune for dkm_build (common::Hybrid, diskann-providers/src/model/graph/provider/
+ * async_/common.rs:385-405; HybridPruneAccessor, inmem/product.rs:143-230): each prune
+ * keeps the first max_fp_vecs_per_prune ids of its fill order (robust_prune_with:
+ * self, intra-batch extras, search record, at most max_occlusion ids; robust_prune_list:
+ * location, list) in full precision and compares every other id through its PQ code
+ * (direct distances, HybridComputer). Negative: no Hybrid, every vector full precision
+ * (the default). Hybrid::new(None) is any value at least the longest fill order. Needs the
+ * PQ schema and the codes of every row before dkm_build. */
+dkm_status dkm_set_build_hybrid(dkm_index *ix, int64_t max_fp_vecs_per_prune);
+
 /* compute_medoid over the data rows (diskann-utils/src/sampling/medoid.rs:15-47):
  * writes the medoid row's vector (dim floats) and its row id (-1 if none qualifies). */
 dkm_status dkm_medoid(dkm_index *ix, float *out_vector, int64_t *out_row);

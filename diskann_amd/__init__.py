@@ -8,11 +8,11 @@ from .errors import (ANNError, CapacityError, DimensionMismatchError, KnnSearchE
                      StorageError)
 from .index import (DiskANNIndex, FullPrecision, Hybrid, Knn, Metric, Quantized, SearchResult,
                     Server, Strategy)
-from . import prescore, quantization, storage
+from . import prescore, quantization, scalar, storage
 
 __all__ = [
     "ANNError", "CapacityError", "DimensionMismatchError", "KnnSearchError", "PQError",
     "StorageError", "storage", "prescore",
     "DiskANNIndex", "FullPrecision", "Hybrid", "Knn", "Metric", "Quantized", "SearchResult",
-    "Server", "Strategy", "quantization",
+    "Server", "Strategy", "quantization", "scalar",
 ]

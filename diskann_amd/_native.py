@@ -27,6 +27,8 @@ EXPORTS = [
     "dkm_pq_pivots_write", "dkm_pq_pivots_read_shape", "dkm_pq_pivots_read",
     "dkm_kmeans_device", "dkm_kmeans_assign_device", "dkm_pq_distance_batch",
     "dkm_prescore_create", "dkm_prescore_search", "dkm_prescore_destroy",
+    "dkm_sq_train", "dkm_sq_compress", "dkm_sq_distance_batch", "dkm_set_sq",
+    "dkm_encode_index_sq", "dkm_set_sq_codes", "dkm_get_sq_codes", "dkm_use_pq",
 ]
 
 _lib = None
@@ -73,6 +75,14 @@ def _declare(L):
         "dkm_set_pq": [P, P, I64, P, I64],
         "dkm_set_codes": [P, P, I64, I64],
         "dkm_encode_index": [P],
+        "dkm_sq_train": [P, I64, I64, C.c_double, P, P, P],
+        "dkm_sq_compress": [C.c_float, P, I64, P, I64, P],
+        "dkm_sq_distance_batch": [I, C.c_float, P, I64, P, I64, P, I64, P, I64, P],
+        "dkm_set_sq": [P, C.c_float, P, C.c_float],
+        "dkm_encode_index_sq": [P],
+        "dkm_set_sq_codes": [P, P, I64, I64],
+        "dkm_get_sq_codes": [P, P, I64, I64],
+        "dkm_use_pq": [P],
         "dkm_get_codes": [P, P, I64, I64],
         "dkm_search_batch": [P, I, P, I64, P, P, U32, I64, P, P, P, P, P],
         "dkm_search_batch_device": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P],

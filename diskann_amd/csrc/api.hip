@@ -20,6 +20,7 @@
 
 #include "../../include/diskann_mi355x.h"
 #include "dkm_build.h"
+#include "dkm_sq.h"
 #include "dkm_exact.h"
 #include "dkm_kernels.h"
 #include "dkm_kmeans.h"
@@ -112,6 +113,12 @@ struct dkm_index {
     int tune_adc_mode = 1;              // PQ distances: 0 = LDS table, 1 = direct
     int pq_len = 0;                     // uniform PQ chunk width usable by the fast path
     int64_t build_max_fp = -1;          // Hybrid prune: full-precision ids per prune (-1 = all)
+    // 8-bit scalar quantization; when sq_active the quantized strategies traverse with it
+    bool sq_active = false, sq_codes_set = false;
+    float sq_scale = 0.0f, sq_mean_norm = 0.0f, sq_ssn = 0.0f;
+    int64_t sq_cstride = 0;
+    DevBuf<float> d_sq_shift, d_sq_comp;
+    DevBuf<uint8_t> d_sq_codes;
     // staging for the host-buffer API
     DevBuf<float> s_q, s_d;
     DevBuf<uint32_t> s_k, s_L, s_ids, s_cnt, s_cmps, s_hops, s_status;
@@ -594,6 +601,230 @@ dkm_status dkm_get_codes(dkm_index *ix, uint8_t *codes, int64_t row0, int64_t n)
 }
 
 
+// ------------------------------------------------------------------------------------
+// 8-bit scalar quantization (diskann-quantization/src/scalar)
+// ------------------------------------------------------------------------------------
+
+}  // extern "C"
+
+// Device copy of a canonical code buffer (rows of [f32 compensation][dim codes]) into
+// padded code rows and a compensation array.
+static dkm_status sq_upload(const uint8_t *canon, int64_t n, int64_t dim, DevBuf<uint8_t> &codes,
+                            DevBuf<float> &comp) {
+    const int64_t cs = dkm::sq_code_stride(dim);
+    std::vector<uint8_t> rows((size_t)(n * cs), 0);
+    std::vector<float> cp((size_t)n);
+    for (int64_t r = 0; r < n; ++r) {
+        const uint8_t *src = canon + r * (dim + 4);
+        std::memcpy(&cp[(size_t)r], src, sizeof(float));
+        std::memcpy(&rows[(size_t)(r * cs)], src + 4, (size_t)dim);
+    }
+    HIP_TRY(codes.ensure(n * cs));
+    HIP_TRY(comp.ensure(n));
+    HIP_TRY(hipMemcpy(codes.p, rows.data(), rows.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(comp.p, cp.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    return DKM_OK;
+}
+
+static dkm_status sq_download(const uint8_t *d_codes, const float *d_comp, int64_t n, int64_t dim,
+                              uint8_t *canon) {
+    const int64_t cs = dkm::sq_code_stride(dim);
+    std::vector<uint8_t> rows((size_t)(n * cs));
+    std::vector<float> cp((size_t)n);
+    HIP_TRY(hipMemcpy(rows.data(), d_codes, rows.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cp.data(), d_comp, sizeof(float) * n, hipMemcpyDeviceToHost));
+    for (int64_t r = 0; r < n; ++r) {
+        uint8_t *dst = canon + r * (dim + 4);
+        std::memcpy(dst, &cp[(size_t)r], sizeof(float));
+        std::memcpy(dst + 4, &rows[(size_t)(r * cs)], (size_t)dim);
+    }
+    return DKM_OK;
+}
+
+static dkm_status sq_check(float scale, const float *shift, int64_t dim) {
+    if (!shift || dim <= 0) return fail(DKM_ERR_INVALID_ARG, "bad scalar quantizer");
+    if (!(scale > 0.0f) || !std::isfinite(scale))
+        return fail(DKM_ERR_INVALID_ARG, "scale must be positive and finite");
+    return DKM_OK;
+}
+
+extern "C" {
+
+dkm_status dkm_sq_train(const float *data, int64_t n, int64_t dim, double standard_deviations,
+                        float *out_scale, float *out_shift, float *out_mean_norm) {
+    if (!data || !out_scale || !out_shift || !out_mean_norm) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (n <= 0 || dim <= 0) return fail(DKM_ERR_INVALID_ARG, "training data cannot be empty");
+    if (!(standard_deviations > 0.0)) return fail(DKM_ERR_INVALID_ARG, "standard deviations must be positive");
+    HIP_TRY(hipSetDevice(g_device));
+    DevBuf<float> dx, dsh, dsc;
+    DevBuf<double> dm, dv, dn;
+    HIP_TRY(dx.ensure(n * dim));
+    HIP_TRY(dsh.ensure(dim));
+    HIP_TRY(dsc.ensure(2));
+    HIP_TRY(dm.ensure(dim));
+    HIP_TRY(dv.ensure(dim));
+    HIP_TRY(dn.ensure(n));
+    HIP_TRY(hipMemcpy(dx.p, data, sizeof(float) * n * dim, hipMemcpyHostToDevice));
+    HIP_TRY(dkm::launch_sq_train(dx.p, n, (int)dim, dim, standard_deviations, dm.p, dv.p, dn.p,
+                                 dsc.p, dsh.p, dsc.p + 1, nullptr));
+    float sc[2];
+    HIP_TRY(hipMemcpy(sc, dsc.p, sizeof(sc), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(out_shift, dsh.p, sizeof(float) * dim, hipMemcpyDeviceToHost));
+    *out_scale = sc[0];
+    *out_mean_norm = sc[1];
+    return DKM_OK;
+}
+
+dkm_status dkm_sq_compress(float scale, const float *shift, int64_t dim, const float *x,
+                           int64_t n, uint8_t *out) {
+    dkm_status st = sq_check(scale, shift, dim);
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!x || !out))) return fail(DKM_ERR_INVALID_ARG, "bad sizes");
+    if (n == 0) return DKM_OK;
+    HIP_TRY(hipSetDevice(g_device));
+    const int64_t cs = dkm::sq_code_stride(dim);
+    DevBuf<float> dx, dsh, dcp;
+    DevBuf<uint8_t> dc;
+    DevBuf<uint32_t> dnan;
+    HIP_TRY(dx.ensure(n * dim));
+    HIP_TRY(dsh.ensure(dim));
+    HIP_TRY(dcp.ensure(n));
+    HIP_TRY(dc.ensure(n * cs));
+    HIP_TRY(dnan.ensure(1));
+    HIP_TRY(hipMemcpy(dx.p, x, sizeof(float) * n * dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dsh.p, shift, sizeof(float) * dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(dnan.p, 0, sizeof(uint32_t)));
+    HIP_TRY(dkm::launch_sq_encode(dx.p, dim, n, (int)dim, scale, dsh.p, dc.p, cs, dcp.p, dnan.p,
+                                  nullptr));
+    st = sq_download(dc.p, dcp.p, n, dim, out);
+    if (st) return st;
+    uint32_t nan = 0;
+    HIP_TRY(hipMemcpy(&nan, dnan.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (nan) return fail(DKM_ERR_INVALID_ARG, "input contains NaN");
+    return DKM_OK;
+}
+
+dkm_status dkm_sq_distance_batch(int metric, float scale, const float *shift, int64_t dim,
+                                 const uint8_t *left, int64_t nl, const uint8_t *right,
+                                 int64_t nr, const uint32_t *ids, int64_t m, float *out) {
+    if (metric != DKM_L2 && metric != DKM_INNER_PRODUCT && metric != DKM_COSINE_NORMALIZED)
+        return fail(DKM_ERR_INVALID_ARG, "scalar quantization supports L2, inner product and "
+                                         "cosine-normalized (got metric %d)", metric);
+    dkm_status st = sq_check(scale, shift, dim);
+    if (st) return st;
+    if (nl < 0 || nr <= 0 || m < 0 || !left || !right || !out || (nl * m > 0 && !ids))
+        return fail(DKM_ERR_INVALID_ARG, "bad sizes");
+    if (nl == 0 || m == 0) return DKM_OK;
+    for (int64_t i = 0; i < nl * m; ++i)
+        if (ids[i] >= (uint64_t)nr)
+            return fail(DKM_ERR_INDEX, "id %u out of range (%lld rows)", ids[i], (long long)nr);
+    HIP_TRY(hipSetDevice(g_device));
+    DevBuf<uint8_t> dl, dr;
+    DevBuf<float> dlc, drc, dsh, dssn, dout;
+    DevBuf<uint32_t> di;
+    st = sq_upload(left, nl, dim, dl, dlc);
+    if (st) return st;
+    st = sq_upload(right, nr, dim, dr, drc);
+    if (st) return st;
+    HIP_TRY(dsh.ensure(dim));
+    HIP_TRY(dssn.ensure(1));
+    HIP_TRY(di.ensure(nl * m));
+    HIP_TRY(dout.ensure(nl * m));
+    HIP_TRY(hipMemcpy(dsh.p, shift, sizeof(float) * dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(di.p, ids, sizeof(uint32_t) * nl * m, hipMemcpyHostToDevice));
+    HIP_TRY(dkm::launch_sq_shift_norm(dsh.p, (int)dim, dssn.p, nullptr));
+    float ssn = 0.0f;
+    HIP_TRY(hipMemcpy(&ssn, dssn.p, sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(dkm::launch_sq_pairs(metric, dkm::sq_bss(scale), ssn, dl.p, dlc.p, dr.p, drc.p,
+                                 dkm::sq_code_stride(dim), di.p, nl, m, dout.p, nullptr));
+    HIP_TRY(hipMemcpy(out, dout.p, sizeof(float) * nl * m, hipMemcpyDeviceToHost));
+    return DKM_OK;
+}
+
+dkm_status dkm_set_sq(dkm_index *ix, float scale, const float *shift, float mean_norm) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    dkm_status st = sq_check(scale, shift, ix->dim);
+    if (st) return st;
+    if (ix->metric != DKM_L2 && ix->metric != DKM_INNER_PRODUCT && ix->metric != DKM_COSINE_NORMALIZED)
+        return fail(DKM_ERR_INVALID_ARG, "scalar quantization supports L2, inner product and "
+                                         "cosine-normalized");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    ix->sq_cstride = dkm::sq_code_stride(ix->dim);
+    HIP_TRY(ix->d_sq_shift.ensure(ix->dim));
+    HIP_TRY(ix->d_sq_comp.ensure(ix->rows));
+    HIP_TRY(ix->d_sq_codes.ensure(ix->rows * ix->sq_cstride + 16));
+    HIP_TRY(hipMemset(ix->d_sq_codes.p, 0, ix->rows * ix->sq_cstride + 16));
+    HIP_TRY(hipMemset(ix->d_sq_comp.p, 0, sizeof(float) * ix->rows));
+    HIP_TRY(hipMemcpy(ix->d_sq_shift.p, shift, sizeof(float) * ix->dim, hipMemcpyHostToDevice));
+    DevBuf<float> dssn;
+    HIP_TRY(dssn.ensure(1));
+    HIP_TRY(dkm::launch_sq_shift_norm(ix->d_sq_shift.p, (int)ix->dim, dssn.p, ix->stream));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    HIP_TRY(hipMemcpy(&ix->sq_ssn, dssn.p, sizeof(float), hipMemcpyDeviceToHost));
+    ix->sq_scale = scale;
+    ix->sq_mean_norm = mean_norm;
+    ix->sq_active = true;
+    ix->sq_codes_set = false;
+    return DKM_OK;
+}
+
+dkm_status dkm_encode_index_sq(dkm_index *ix) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    if (!ix->sq_active) return fail(DKM_ERR_NOT_READY, "set the scalar quantizer first");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    DevBuf<uint32_t> dnan;
+    HIP_TRY(dnan.ensure(1));
+    HIP_TRY(hipMemsetAsync(dnan.p, 0, sizeof(uint32_t), ix->stream));
+    HIP_TRY(dkm::launch_sq_encode(ix->d_vec, ix->vstride, ix->rows, (int)ix->dim, ix->sq_scale,
+                                  ix->d_sq_shift.p, ix->d_sq_codes.p, ix->sq_cstride,
+                                  ix->d_sq_comp.p, dnan.p, ix->stream));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    uint32_t nan = 0;
+    HIP_TRY(hipMemcpy(&nan, dnan.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (nan) return fail(DKM_ERR_INVALID_ARG, "input contains NaN");
+    ix->sq_codes_set = true;
+    return DKM_OK;
+}
+
+dkm_status dkm_set_sq_codes(dkm_index *ix, const uint8_t *codes, int64_t row0, int64_t n) {
+    if (!ix || (!codes && n > 0)) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (!ix->sq_active) return fail(DKM_ERR_NOT_READY, "set the scalar quantizer first");
+    if (row0 < 0 || n < 0 || row0 + n > ix->rows) return fail(DKM_ERR_INDEX, "rows out of range");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    if (n) {
+        DevBuf<uint8_t> c;
+        DevBuf<float> cp;
+        dkm_status st = sq_upload(codes, n, ix->dim, c, cp);
+        if (st) return st;
+        HIP_TRY(hipMemcpy(ix->d_sq_codes.p + row0 * ix->sq_cstride, c.p, n * ix->sq_cstride,
+                          hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(ix->d_sq_comp.p + row0, cp.p, sizeof(float) * n, hipMemcpyDeviceToDevice));
+    }
+    ix->sq_codes_set = true;
+    return DKM_OK;
+}
+
+dkm_status dkm_get_sq_codes(dkm_index *ix, uint8_t *codes, int64_t row0, int64_t n) {
+    if (!ix || !ix->sq_active) return fail(DKM_ERR_NOT_READY, "no scalar-quantized codes");
+    if (row0 < 0 || n < 0 || row0 + n > ix->rows) return fail(DKM_ERR_INDEX, "rows out of range");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    if (n) return sq_download(ix->d_sq_codes.p + row0 * ix->sq_cstride, ix->d_sq_comp.p + row0, n,
+                              ix->dim, codes);
+    return DKM_OK;
+}
+
+dkm_status dkm_use_pq(dkm_index *ix) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    std::lock_guard<std::mutex> g(ix->mu);
+    ix->sq_active = false;
+    return DKM_OK;
+}
+
 // Default LDS visited-set size: a small first level (most of the visited set lives in the
 // per-query HBM table); small LDS footprints keep more queries resident per CU, which
 // hides the gather latency better than a large LDS table does (measured on MI355X).
@@ -627,6 +858,19 @@ static dkm_status ensure_spill(dkm_index *ix, int64_t nq, uint64_t slots) {
     return DKM_OK;
 }
 
+// Scalar-quantized traversal fields of the search parameters.
+static void set_sq_params(const dkm_index *ix, bool sq, dkm::SearchParams &p) {
+    p.sq = sq;
+    p.sq_codes = ix->d_sq_codes.p;
+    p.sq_comp = ix->d_sq_comp.p;
+    p.sq_shift = ix->d_sq_shift.p;
+    p.sq_cstride = (int)ix->sq_cstride;
+    p.sq_scale = ix->sq_scale;
+    p.sq_bss = dkm::sq_bss(ix->sq_scale);
+    p.sq_ssn = ix->sq_ssn;
+    p.sq_mean_norm = ix->sq_mean_norm;
+}
+
 // Launch on the handle's stream; caller holds ix->mu.
 static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_queries, int64_t nq,
                                 const uint32_t *d_k, const uint32_t *d_L, uint32_t max_k,
@@ -640,7 +884,11 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     if (max_L > 4096) return fail(DKM_ERR_INVALID_ARG, "L above 4096 is not supported");
     if ((int64_t)max_k > k_stride) return fail(DKM_ERR_INVALID_ARG, "k exceeds output stride");
     if (!ix->starts_set) return fail(DKM_ERR_NOT_READY, "start points not set");
-    bool pq = strategy != DKM_STRATEGY_FULL;
+    const bool quant = strategy != DKM_STRATEGY_FULL;
+    const bool sq = quant && ix->sq_active;
+    const bool pq = quant && !sq;
+    if (sq && !ix->sq_codes_set)
+        return fail(DKM_ERR_NOT_READY, "scalar-quantized strategy needs the codes of every row");
     if (pq && (!ix->d_codes || !ix->codes_set))
         return fail(DKM_ERR_NOT_READY, "PQ strategy needs a PQ schema and codes");
     if (pq && ix->metric == DKM_COSINE)
@@ -668,6 +916,7 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     p.M = (int)ix->M;
     p.pivots = ix->d_pivots;
     p.offsets = ix->d_offsets;
+    set_sq_params(ix, sq, p);
     p.queries = d_queries;
     p.qstride = ix->dim;
     p.nq = (int)nq;
@@ -1214,7 +1463,11 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
     if (capacity < 1 || capacity > (int64_t(1) << 24) || slots < 0 || slots > (int64_t(1) << 20))
         return fail(DKM_ERR_INVALID_ARG, "bad capacity / slots");
     if (!ix->starts_set) return fail(DKM_ERR_NOT_READY, "start points not set");
-    const bool pq = strategy != DKM_STRATEGY_FULL;
+    const bool quant = strategy != DKM_STRATEGY_FULL;
+    const bool sq = quant && ix->sq_active;
+    const bool pq = quant && !sq;
+    if (sq && !ix->sq_codes_set)
+        return fail(DKM_ERR_NOT_READY, "scalar-quantized strategy needs the codes of every row");
     if (pq && (!ix->d_codes || !ix->codes_set))
         return fail(DKM_ERR_NOT_READY, "PQ strategy needs a PQ schema and codes");
     if (pq && ix->metric == DKM_COSINE) return fail(DKM_ERR_PQ, "cosine PQ traversal is not supported");
@@ -1239,6 +1492,7 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
     p.M = (int)ix->M;
     p.pivots = ix->d_pivots;
     p.offsets = ix->d_offsets;
+    set_sq_params(ix, sq, p);
     p.qstride = ix->dim;
     p.beam_width = beam_width;
     p.k_stride = max_k;

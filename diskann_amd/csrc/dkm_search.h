@@ -50,6 +50,13 @@ struct SearchParams {
     float *rec_d;
     uint32_t rec_cap;
     uint32_t *rec_len;
+    // 8-bit scalar quantization (traverse with it when sq is set; pq is then 0)
+    int sq;
+    const uint8_t *sq_codes;  // rows x sq_cstride, zero padded
+    const float *sq_comp;     // rows
+    const float *sq_shift;    // dim
+    int sq_cstride;           // multiple of 16
+    float sq_scale, sq_bss, sq_ssn, sq_mean_norm;
 };
 
 size_t search_lds_bytes(const SearchParams &p);

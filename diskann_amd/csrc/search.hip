@@ -18,6 +18,7 @@
 // hop's expansion list is ranked in parallel and merged with two binary searches.
 #include "dkm_device.h"
 #include "dkm_search.h"
+#include "dkm_sq.h"
 
 namespace dkm {
 
@@ -35,6 +36,8 @@ struct Lds {
     uint32_t *rows;  // beam_width x gstride adjacency rows staged per iteration; the
                      // merge reuses it for the new items' target positions
     int32_t *misc;
+    uint8_t *sqc;    // scalar-quantized query code (sq_cstride bytes)
+    uint32_t *sqm;   // [0] query compensation (f32 bits), [1] sum of squared query codes
 };
 
 // Expansion lists longer than this are ranked through a separate sorted buffer.
@@ -63,6 +66,7 @@ __host__ __device__ inline size_t lds_bytes(const SearchParams &p) {
     b += align16(sizeof(uint32_t) * p.beam_width);
     b += align16(sizeof(uint32_t) * rows_entries(p));
     b += 16;
+    if (p.sq) b += align16(p.sq_cstride) + 16;
     return b;
 }
 
@@ -93,6 +97,8 @@ __device__ inline Lds carve(const SearchParams &p, char *base) {
     s.beam = (uint32_t *)take(sizeof(uint32_t) * p.beam_width);
     s.rows = (uint32_t *)take(sizeof(uint32_t) * rows_entries(p));
     s.misc = (int32_t *)take(16);
+    s.sqc = p.sq ? (uint8_t *)take(p.sq_cstride) : nullptr;
+    s.sqm = p.sq ? (uint32_t *)take(16) : nullptr;
     return s;
 }
 
@@ -688,14 +694,112 @@ __device__ inline void adc_direct(const SearchParams &p, Lds &s, const uint32_t 
     }
 }
 
+// Scalar-quantized distances (SQStore's QueryComputer, diskann-providers/src/model/graph/
+// provider/async_/inmem/scalar.rs:299-320): a group of G lanes per candidate, lane t of
+// the group reads the 16-byte chunks t, t + G, ... of the candidate's code row; each lane
+// keeps U candidates' rows in flight. The integer sums are exact in any order; the
+// group's first lane applies the f32 finish.
+template <int METRIC, int G>
+__device__ inline void sq_distances(const SearchParams &p, Lds &s, const uint32_t *ids,
+                                    float *out, int n) {
+    constexpr int NG = kWave / G;
+    constexpr int U = 4;
+    const int lane = lane_id();
+    const int g = lane % G, grp = lane / G;
+    const int chunks = p.sq_cstride / 16;
+    const float qcomp = __uint_as_float(s.sqm[0]);
+    const uint32_t qq = s.sqm[1];
+    for (int c0 = 0; c0 < n; c0 += NG * U) {
+        uint32_t ab[U], bb[U], aa[U], id[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = c0 + grp + NG * u;
+            id[u] = ids[c < n ? c : 0];
+            ab[u] = bb[u] = aa[u] = 0;
+        }
+        for (int t = g; t < chunks; t += G) {
+            const uint4 a = *(const uint4 *)(s.sqc + 16 * t);
+            uint4 b[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                b[u] = *(const uint4 *)(p.sq_codes + (size_t)id[u] * p.sq_cstride + 16 * t);
+#pragma unroll
+            for (int u = 0; u < U; ++u) sq_dot16(a, b[u], ab[u], aa[u], bb[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int o = G / 2; o > 0; o >>= 1) {
+                ab[u] += __shfl_xor(ab[u], o, G);
+                bb[u] += __shfl_xor(bb[u], o, G);
+            }
+            const int c = c0 + grp + NG * u;
+            if (c < n && g == 0) {
+                const float comp = METRIC == METRIC_IP ? p.sq_comp[id[u]] : 0.0f;
+                out[c] = sq_finish(METRIC, p.sq_bss, p.sq_ssn, ab[u], qq + bb[u] - 2u * ab[u],
+                                   qcomp, comp);
+            }
+        }
+    }
+}
+
+// Compress the query (SQStore::query_computer, scalar.rs:227-253, allow_rescale): for
+// inner product the query is first scaled to the training mean norm; codes go to s.sqc
+// (zero padded), the compensation fma chain and the code square norm to s.sqm.
+template <int METRIC>
+__device__ inline void sq_query(const SearchParams &p, Lds &s) {
+    const int lane = lane_id();
+    float rs = 1.0f;
+    bool resc = false;
+    if (METRIC == METRIC_IP) {
+        const float v = half_distance<METRIC_IP>(s.q, s.q, p.dim, lane & 31);
+        const float nrm = __builtin_sqrtf(-__shfl(v, 0));
+        resc = nrm != 0.0f;
+        if (resc) rs = p.sq_mean_norm / nrm;
+    }
+    const float inv = 255.0f / p.sq_scale;
+    for (int e = lane; e < p.sq_cstride; e += kWave) {
+        uint8_t code = 0;
+        if (e < p.dim) {
+            const float f = resc ? s.q[e] * rs : s.q[e];
+            float c = (f - p.sq_shift[e]) * inv;
+            if (c < 0.0f) c = 0.0f;
+            else if (c > 255.0f) c = 255.0f;
+            c = __builtin_roundf(c);
+            code = c != c ? (uint8_t)0 : (uint8_t)c;
+        }
+        s.sqc[e] = code;
+    }
+    __syncthreads();
+    uint32_t qq = 0;
+    for (int t = lane; t < p.sq_cstride / 4; t += kWave) {
+        const uint32_t w = ((const uint32_t *)s.sqc)[t];
+        qq = __builtin_amdgcn_udot4(w, w, qq, false);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) qq += __shfl_xor(qq, o);
+    if (lane == 0) {
+        float dot = 0.0f;
+        if (METRIC == METRIC_IP)
+            for (int e = 0; e < p.dim; ++e) dot = fma_((float)s.sqc[e], p.sq_shift[e], dot);
+        s.sqm[0] = __float_as_uint(p.sq_scale * (1.0f / 255.0f) * dot);
+        s.sqm[1] = qq;
+    }
+    __syncthreads();
+}
+
+// MODE 0: full precision; 1: PQ with the LDS table; 2: PQ direct (PQLEN = chunk width);
+// 3: 8-bit scalar quantization (PQLEN = lanes per candidate).
 template <int METRIC, int MODE, int DIM, int PQLEN>
 __device__ inline void compute_distances(const SearchParams &p, Lds &s, int n) {
     if constexpr (MODE == 0) {
         fp_distances<METRIC, DIM>(p, s.q, s.nl_i, s.nl_d, n, s.starts[0]);
     } else if constexpr (MODE == 1) {
         adc_lut(p, s, s.nl_i, s.nl_d, n);
-    } else {
+    } else if constexpr (MODE == 2) {
         adc_direct<METRIC == METRIC_IP, PQLEN>(p, s, s.nl_i, s.nl_d, n);
+    } else {
+        sq_distances<METRIC, PQLEN>(p, s, s.nl_i, s.nl_d, n);
     }
     __syncthreads();
 }
@@ -716,7 +820,7 @@ struct QueryJob {
 
 template <int METRIC, int MODE, int DIM, int PQLEN>
 __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJob &job) {
-    constexpr bool PQ = MODE != 0;
+    constexpr bool PQ = MODE == 1 || MODE == 2;
     const int lane = threadIdx.x;
     const int L = job.L;
     const int k = job.k;
@@ -736,6 +840,7 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
         for (int i = lane; i <= p.M; i += kWave) s.offs[i] = p.offsets[i];
     }
     __syncthreads();
+    if constexpr (MODE == 3) sq_query<METRIC>(p, s);
     if (MODE == 1) {
         const bool ip = METRIC == METRIC_IP;
         for (int e = lane; e < p.M * 256; e += kWave) {
@@ -975,7 +1080,20 @@ static hipError_t launch_t(const SearchParams &p, hipStream_t stream, int *resid
     return hipGetLastError();
 }
 
+template <int METRIC>
+static hipError_t dispatch_sq(const SearchParams &p, hipStream_t stream, int *resident) {
+    const int chunks = p.sq_cstride / 16;
+    if (chunks <= 8) return launch_t<METRIC, 3, 0, 8>(p, stream, resident);
+    if (chunks <= 32) return launch_t<METRIC, 3, 0, 16>(p, stream, resident);
+    return launch_t<METRIC, 3, 0, 64>(p, stream, resident);
+}
+
 static hipError_t dispatch(const SearchParams &p, hipStream_t stream, int *resident) {
+    if (p.sq) {
+        if (p.metric == METRIC_IP) return dispatch_sq<METRIC_IP>(p, stream, resident);
+        if (p.metric == METRIC_COSNORM) return dispatch_sq<METRIC_COSNORM>(p, stream, resident);
+        return dispatch_sq<METRIC_L2>(p, stream, resident);
+    }
     if (p.pq) {
         const bool ip = p.metric == METRIC_IP;  // L2 and CosineNormalized use the L2 table
         if (p.adc_mode == 0) {

@@ -147,6 +147,29 @@ class DiskANNIndex:
                                   offsets.shape[0] - 1), self.lib)
         self.M = offsets.shape[0] - 1
 
+    def set_sq(self, quantizer):
+        """Scalar-quantized store (SQStore, diskann-providers/.../inmem/scalar.rs:83-253):
+        from now on the Hybrid / Quantized strategies traverse with 8-bit codes (call
+        encode_sq or set_sq_codes next); use_pq switches back to the PQ store."""
+        mn = 0.0 if quantizer.mean_norm is None else quantizer.mean_norm
+        check(self.lib.dkm_set_sq(self._h, quantizer.scale, _p(quantizer.shift), mn), self.lib)
+
+    def encode_sq(self):
+        check(self.lib.dkm_encode_index_sq(self._h), self.lib)
+
+    def set_sq_codes(self, compressed, row0=0):
+        c = np.ascontiguousarray(np.atleast_2d(compressed), dtype=np.uint8)
+        check(self.lib.dkm_set_sq_codes(self._h, _p(c), row0, c.shape[0]), self.lib)
+
+    def get_sq_codes(self, row0=0, n=None):
+        n = self.max_points + self.num_start - row0 if n is None else n
+        out = np.zeros((n, 4 + self.dim), dtype=np.uint8)
+        check(self.lib.dkm_get_sq_codes(self._h, _p(out), row0, n), self.lib)
+        return out
+
+    def use_pq(self):
+        check(self.lib.dkm_use_pq(self._h), self.lib)
+
     def set_codes(self, codes, row0=0):
         codes = np.ascontiguousarray(codes, dtype=np.uint8)
         check(self.lib.dkm_set_codes(self._h, _p(codes), row0, codes.shape[0]), self.lib)

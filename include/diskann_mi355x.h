@@ -176,6 +176,42 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
                      int64_t intra_batch, uint32_t L_build, float alpha, int64_t max_occlusion,
                      int saturate);
 
+/* 8-bit scalar quantization (diskann-quantization/src/scalar). A compressed vector uses
+ * the reference's canonical layout (meta/vector.rs:384-434): an f32 compensation, then one
+ * code byte per dimension, 4 + dim bytes per row.
+ * dkm_sq_train: ScalarQuantizationParameters::train (scalar/train.rs:75-88) over n x dim
+ *   host rows: scale = 2 * standard_deviations * max column std, shift = column mean -
+ *   scale / 2, mean_norm = average row norm (f64 statistics in row order).
+ * dkm_sq_compress: ScalarQuantizer::compress_into into MutCompensatedVectorRef
+ *   (scalar/quantizer.rs:380-432) for n host rows; DKM_ERR_INVALID_ARG when a row holds a
+ *   NaN (the codes are written regardless, as the reference does).
+ * dkm_sq_distance_batch: CompensatedSquaredL2 / CompensatedIP / CompensatedCosineNormalized
+ *   as distances (scalar/vectors.rs:182-470, the DistanceComputer of
+ *   diskann-providers/.../inmem/scalar.rs:260-297): out[i * m + j] = d(left[i],
+ *   right[ids[i * m + j]]); metric L2, inner product or cosine-normalized. */
+dkm_status dkm_sq_train(const float *data, int64_t n, int64_t dim, double standard_deviations,
+                        float *out_scale, float *out_shift, float *out_mean_norm);
+dkm_status dkm_sq_compress(float scale, const float *shift, int64_t dim, const float *x,
+                           int64_t n, uint8_t *out);
+dkm_status dkm_sq_distance_batch(int metric, float scale, const float *shift, int64_t dim,
+                                 const uint8_t *left, int64_t nl, const uint8_t *right,
+                                 int64_t nr, const uint32_t *ids, int64_t m, float *out);
+
+/* Scalar-quantized store of an index (SQStore, diskann-providers/src/model/graph/provider/
+ * async_/inmem/scalar.rs:83-253). After dkm_set_sq the quantized strategies traverse with
+ * 8-bit codes: DKM_STRATEGY_HYBRID = the reference's Quantized strategy over a
+ * full-precision provider (SQ traversal, FilterStartPoints -> Rerank, scalar.rs:601-637),
+ * DKM_STRATEGY_QUANTIZED = Quantized without a full-precision store (SQ distances
+ * returned, scalar.rs:640-672). The query is compressed per search, rescaled to mean_norm
+ * first for inner product (query_computer with allow_rescale). dkm_encode_index_sq
+ * compresses every stored row (SQStore::set_vector); dkm_set_sq_codes / dkm_get_sq_codes
+ * move canonical rows; dkm_use_pq switches the quantized strategies back to PQ. */
+dkm_status dkm_set_sq(dkm_index *ix, float scale, const float *shift, float mean_norm);
+dkm_status dkm_encode_index_sq(dkm_index *ix);
+dkm_status dkm_set_sq_codes(dkm_index *ix, const uint8_t *codes, int64_t row0, int64_t n);
+dkm_status dkm_get_sq_codes(dkm_index *ix, uint8_t *codes, int64_t row0, int64_t n);
+dkm_status dkm_use_pq(dkm_index *ix);
+
 /* Hybrid prune for dkm_build (common::Hybrid, diskann-providers/src/model/graph/provider/
  * async_/common.rs:385-405; HybridPruneAccessor, inmem/product.rs:143-230): each prune
  * keeps the first max_fp_vecs_per_prune ids of its fill order (robust_prune_with:

@@ -499,7 +499,9 @@ static void scratch_init(scratch *s, const dko_index *ix, int64_t L, int64_t bw)
     s->nb_cap = bw * ix->max_degree + 1;
     s->nb_id = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)s->nb_cap);
     s->nb_d = (float *)malloc(sizeof(float) * (size_t)s->nb_cap);
-    s->lut = ix->M > 0 ? (float *)malloc(sizeof(float) * (size_t)(ix->M * ix->K)) : NULL;
+    s->lut = ix->sq   ? (float *)malloc((size_t)ix->sq->dim + 2 * sizeof(float))
+             : ix->M > 0 ? (float *)malloc(sizeof(float) * (size_t)(ix->M * ix->K))
+                         : NULL;
 }
 static void scratch_free(scratch *s) {
     q_free(&s->q);
@@ -510,8 +512,126 @@ static void scratch_free(scratch *s) {
     free(s->lut);
 }
 
+/* ------------------------------------------------------------------------------------
+ * Scalar quantization, 8 bits
+ * ---------------------------------------------------------------------------------- */
+
+/* ScalarQuantizationParameters::train (diskann-quantization/src/scalar/train.rs:75-88)
+ * with compute_means_and_average_norm and compute_variances (src/utils.rs:109-197): f64
+ * column sums in row order, row norms as sequential f64 sums of squares, the norm sum in
+ * row order; range p = sqrt(max variance) * stdevs, scale = 2p, shift = mean - p. */
+void dko_sq_train(const float *x, int64_t n, int64_t dim, int64_t stride, double stdevs,
+                  float *scale, float *shift, float *mean_norm) {
+    double *mean = (double *)calloc((size_t)dim, sizeof(double));
+    double *var = (double *)calloc((size_t)dim, sizeof(double));
+    double norm_sum = 0.0;
+    for (int64_t r = 0; r < n; ++r) {
+        const float *row = x + r * stride;
+        double sq = 0.0;
+        for (int64_t j = 0; j < dim; ++j) {
+            const double v = (double)row[j];
+            mean[j] += v;
+            sq += v * v;
+        }
+        norm_sum += sqrt(sq);
+    }
+    for (int64_t j = 0; j < dim; ++j) mean[j] /= (double)n;
+    for (int64_t r = 0; r < n; ++r) {
+        const float *row = x + r * stride;
+        for (int64_t j = 0; j < dim; ++j) {
+            const double d = (double)row[j] - mean[j];
+            var[j] += d * d;
+        }
+    }
+    double mx = 0.0;
+    for (int64_t j = 0; j < dim; ++j) {
+        var[j] /= (double)n;
+        mx = fmax(mx, var[j]);
+    }
+    const double p = sqrt(mx) * stdevs;
+    *scale = (float)(2.0 * p);
+    for (int64_t j = 0; j < dim; ++j) shift[j] = (float)(mean[j] - p);
+    *mean_norm = (float)(norm_sum / (double)n);
+    free(mean);
+    free(var);
+}
+
+/* ScalarQuantizer::new: shift_square_norm = InnerProduct::evaluate(shift, shift), the
+ * mathematical value of the f32 inner product (quantizer.rs:116-127). */
+float dko_sq_shift_square_norm(const float *shift, int64_t dim) {
+    return -dko_distance(DKO_INNER_PRODUCT, shift, shift, dim);
+}
+
+/* ScalarQuantizer::compress + the compensation of CompressInto<MutCompensatedVectorRef>
+ * (quantizer.rs:190-240, 407-432): code = round(clamp((x - shift) * (255 / scale), 0,
+ * 255)) (round half away from zero), compensation = scale * (1/255) * dot with
+ * dot = fma(code, shift, dot) in dimension order. */
+int dko_sq_compress(float scale, const float *shift, int64_t dim, const float *x,
+                    uint8_t *code, float *comp) {
+    const float inv = 255.0f / scale;
+    float dot = 0.0f;
+    int nan = 0;
+    for (int64_t i = 0; i < dim; ++i) {
+        const float f = x[i];
+        nan |= f != f;
+        float c = (f - shift[i]) * inv;
+        if (c < 0.0f) c = 0.0f;
+        else if (c > 255.0f) c = 255.0f;
+        c = roundf(c);
+        dot = fmaf(c, shift[i], dot);
+        code[i] = c != c ? 0 : (uint8_t)c;
+    }
+    *comp = scale * (1.0f / 255.0f) * dot;
+    return nan;
+}
+
+/* ScalarQuantizer::rescale (quantizer.rs:300-313): x *= to_norm / |x| unless |x| = 0. */
+void dko_sq_rescale(float *x, int64_t dim, float to_norm) {
+    const float nsq = -dko_distance(DKO_INNER_PRODUCT, x, x, dim);
+    const float norm = sqrtf(nsq);
+    if (norm == 0.0f) return;
+    const float s = to_norm / norm;
+    for (int64_t i = 0; i < dim; ++i) x[i] *= s;
+}
+
+/* CompensatedSquaredL2 / CompensatedIP / CompensatedCosineNormalized
+ * (diskann-quantization/src/scalar/vectors.rs:182-470) on 8-bit codes: exact integer
+ * sums, then bs = (1/255)^2 in f32 and
+ *   L2: (bs * scale^2) * sum (a - b)^2
+ *   IP: -(fma(bs * scale^2, sum a b, shift_square_norm) + (comp_b + comp_a))
+ *   CosineNormalized: 1 - (1 - ((bs * scale^2) * sum (a - b)^2) / 2). */
+float dko_sq_distance(int metric, float scale, float shift_square_norm, const uint8_t *a,
+                      float comp_a, const uint8_t *b, float comp_b, int64_t dim) {
+    const float inv = 1.0f / 255.0f;
+    const float bs = inv * inv;
+    const float ss = scale * scale;
+    if (metric == DKO_INNER_PRODUCT) {
+        uint32_t ip = 0;
+        for (int64_t i = 0; i < dim; ++i) ip += (uint32_t)a[i] * (uint32_t)b[i];
+        const float r = fmaf(bs * ss, (float)ip, shift_square_norm) + (comp_b + comp_a);
+        return -r;
+    }
+    uint32_t l2 = 0;
+    for (int64_t i = 0; i < dim; ++i) {
+        const int32_t d = (int32_t)a[i] - (int32_t)b[i];
+        l2 += (uint32_t)(d * d);
+    }
+    const float v = bs * ss * (float)l2;
+    if (metric == DKO_COSINE_NORMALIZED) return 1.0f - (1.0f - v / 2.0f);
+    return v;
+}
+
 static inline float node_dist(const dko_index *ix, int pq, const float *query,
                               const float *lut, uint32_t id) {
+    if (pq && ix->sq) {
+        /* query code / compensation staged behind the LUT pointer by search_core */
+        const dko_sq *q = ix->sq;
+        const uint8_t *qc = (const uint8_t *)lut;
+        float qcomp;
+        memcpy(&qcomp, qc + q->dim, sizeof(float));
+        return dko_sq_distance(ix->metric, q->scale, q->shift_square_norm, qc, qcomp,
+                               q->codes + (int64_t)id * q->dim, q->comp[id], q->dim);
+    }
     if (pq) return dko_pq_adc(ix->codes + (int64_t)id * ix->M, lut, ix->M, ix->K);
     return dko_distance(ix->metric, query, ix->vectors + (int64_t)id * ix->vstride, ix->dim);
 }
@@ -527,7 +647,20 @@ static void search_core(const dko_index *ix, int pq, const float *query, int64_t
     q->cursor = 0;
     set_clear(&s->vis);
     uint32_t cmps = 0, hops = 0, nrec = 0;
-    if (pq) {
+    if (pq && ix->sq) {
+        /* SQStore::query_computer (diskann-providers/.../inmem/scalar.rs:227-253) with
+         * allow_rescale: the inner-product query is rescaled to the mean norm, then
+         * compressed; code and compensation go into the LUT buffer */
+        const dko_sq *q = ix->sq;
+        float *tmp = (float *)malloc(sizeof(float) * (size_t)q->dim);
+        memcpy(tmp, query, sizeof(float) * (size_t)q->dim);
+        if (ix->metric == DKO_INNER_PRODUCT) dko_sq_rescale(tmp, q->dim, q->mean_norm);
+        uint8_t *qc = (uint8_t *)s->lut;
+        float qcomp;
+        dko_sq_compress(q->scale, q->shift, q->dim, tmp, qc, &qcomp);
+        memcpy(qc + q->dim, &qcomp, sizeof(float));
+        free(tmp);
+    } else if (pq) {
         int lut_metric = ix->metric == DKO_INNER_PRODUCT ? DKO_INNER_PRODUCT : DKO_L2;
         dko_pq_lut(lut_metric, ix->pivots, ix->K, ix->dim, ix->offsets, ix->M, query, s->lut);
     }
@@ -634,7 +767,7 @@ int dko_search(const dko_index *ix, int mode, const float *query, int64_t k, int
                int64_t bw, uint32_t *out_ids, float *out_d, dko_stats *st) {
     int rc = dko_knn_validate(k, L, bw);
     if (rc) return rc;
-    if (mode != DKO_MODE_FULL && ix->M <= 0) return DKO_ERR_ARG;
+    if (mode != DKO_MODE_FULL && ix->M <= 0 && !ix->sq) return DKO_ERR_ARG;
     scratch s;
     scratch_init(&s, ix, L, bw);
     search_core(ix, mode != DKO_MODE_FULL, query, bw, &s, NULL, NULL, NULL, 0, st);

@@ -61,6 +61,27 @@ float dko_pq_qq(int metric, const float *pivots, int64_t dim, const int64_t *off
 int dko_pq_encode(const float *pivots, int64_t K, int64_t dim, const int64_t *offsets,
                   int64_t M, const float *x, int64_t n, uint8_t *codes);
 
+/* Scalar quantization, 8 bits (diskann-quantization/src/scalar) ------------------------ */
+typedef struct {
+    int64_t dim;
+    float scale;               /* dynamic range of the codes (train.rs:75-88) */
+    const float *shift;        /* [dim] per-dimension shift */
+    float shift_square_norm;   /* <shift, shift> (quantizer.rs:116-127) */
+    float mean_norm;           /* average row norm (query rescale for inner product) */
+    const uint8_t *codes;      /* [rows * dim] codes of the index rows */
+    const float *comp;         /* [rows] compensation (quantizer.rs:407-432) */
+} dko_sq;
+
+void dko_sq_train(const float *x, int64_t n, int64_t dim, int64_t stride, double stdevs,
+                  float *scale, float *shift, float *mean_norm);
+float dko_sq_shift_square_norm(const float *shift, int64_t dim);
+/* returns 1 when x holds a NaN (the codes are written regardless) */
+int dko_sq_compress(float scale, const float *shift, int64_t dim, const float *x,
+                    uint8_t *code, float *comp);
+void dko_sq_rescale(float *x, int64_t dim, float to_norm);
+float dko_sq_distance(int metric, float scale, float shift_square_norm, const uint8_t *a,
+                      float comp_a, const uint8_t *b, float comp_b, int64_t dim);
+
 /* Graph + search -------------------------------------------------------------------- */
 typedef struct {
     int metric;
@@ -77,6 +98,8 @@ typedef struct {
     const uint8_t *codes;      /* [rows * M] */
     const float *pivots;       /* [K * dim] */
     const int64_t *offsets;    /* [M+1] */
+    /* scalar quantization (optional): when set, the quantized modes traverse with it */
+    const dko_sq *sq;
 } dko_index;
 
 typedef struct {

@@ -34,7 +34,15 @@ class Index(C.Structure):
         ("vectors", C.c_void_p), ("max_degree", C.c_int64), ("adj", C.c_void_p),
         ("deg", C.c_void_p), ("nstart", C.c_int64), ("starts", C.c_void_p),
         ("M", C.c_int64), ("K", C.c_int64), ("codes", C.c_void_p),
-        ("pivots", C.c_void_p), ("offsets", C.c_void_p),
+        ("pivots", C.c_void_p), ("offsets", C.c_void_p), ("sq", C.c_void_p),
+    ]
+
+
+class SQ(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int64), ("scale", C.c_float), ("shift", C.c_void_p),
+        ("shift_square_norm", C.c_float), ("mean_norm", C.c_float), ("codes", C.c_void_p),
+        ("comp", C.c_void_p),
     ]
 
 
@@ -67,6 +75,14 @@ def _setup(L):
     L.dko_pq_lut.argtypes = [C.c_int, P, I64, I64, P, I64, P, P]
     L.dko_pq_adc.argtypes = [P, P, I64, I64]
     L.dko_pq_adc.restype = F
+    L.dko_sq_train.argtypes = [P, I64, I64, I64, C.c_double, P, P, P]
+    L.dko_sq_shift_square_norm.argtypes = [P, I64]
+    L.dko_sq_shift_square_norm.restype = F
+    L.dko_sq_compress.argtypes = [F, P, I64, P, P, P]
+    L.dko_sq_compress.restype = C.c_int
+    L.dko_sq_rescale.argtypes = [P, I64, F]
+    L.dko_sq_distance.argtypes = [C.c_int, F, F, P, F, P, F, I64]
+    L.dko_sq_distance.restype = F
     L.dko_pq_encode.argtypes = [P, I64, I64, P, I64, P, I64, P]
     L.dko_pq_direct.argtypes = [C.c_int, P, I64, P, I64, P, P]
     L.dko_pq_direct.restype = C.c_float
@@ -168,11 +184,56 @@ def pq_encode(pivots, offsets, x):
     return codes
 
 
+def sq_train(x, stdevs=2.0):
+    """ScalarQuantizationParameters::train: (scale, shift[dim], mean_norm)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n, dim = x.shape
+    shift = np.zeros(dim, np.float32)
+    scale, mn = C.c_float(), C.c_float()
+    lib().dko_sq_train(_p(x), n, dim, dim, float(stdevs), C.byref(scale), _p(shift),
+                       C.byref(mn))
+    return float(np.float32(scale.value)), shift, float(np.float32(mn.value))
+
+
+def sq_shift_square_norm(shift):
+    shift = np.ascontiguousarray(shift, dtype=np.float32)
+    return lib().dko_sq_shift_square_norm(_p(shift), shift.shape[0])
+
+
+def sq_compress(scale, shift, x):
+    """8-bit codes [n, dim] and compensations [n] of the rows of x; raises on NaN."""
+    shift = np.ascontiguousarray(shift, dtype=np.float32)
+    x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float32)
+    codes = np.zeros(x.shape, np.uint8)
+    comp = np.zeros(x.shape[0], np.float32)
+    for r in range(x.shape[0]):
+        c = C.c_float()
+        if lib().dko_sq_compress(scale, _p(shift), x.shape[1], _p(x[r]), _p(codes[r]),
+                                 C.byref(c)):
+            raise ValueError("input contains NaN (row %d)" % r)
+        comp[r] = c.value
+    return codes, comp
+
+
+def sq_rescale(x, to_norm):
+    x = np.array(x, dtype=np.float32)
+    lib().dko_sq_rescale(_p(x), x.shape[0], to_norm)
+    return x
+
+
+def sq_distance(metric, scale, shift_square_norm, a, comp_a, b, comp_b):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    b = np.ascontiguousarray(b, dtype=np.uint8)
+    return lib().dko_sq_distance(metric, scale, shift_square_norm, _p(a), comp_a, _p(b),
+                                 comp_b, a.shape[0])
+
+
 class OracleIndex:
-    """Holds numpy arrays alive behind an Index struct."""
+    """Holds numpy arrays alive behind an Index struct. sq = (scale, shift, mean_norm,
+    codes, comp) selects scalar-quantized traversal for the quantized modes."""
 
     def __init__(self, metric, vectors, adj, deg, starts, codes=None, pivots=None,
-                 offsets=None):
+                 offsets=None, sq=None):
         self.vectors = np.ascontiguousarray(vectors, dtype=np.float32)
         self.adj = np.ascontiguousarray(adj, dtype=np.uint32)
         self.deg = np.ascontiguousarray(deg, dtype=np.uint32)
@@ -196,6 +257,15 @@ class OracleIndex:
             s.codes = _p(self.codes)
             s.pivots = _p(self.pivots)
             s.offsets = _p(self.offsets)
+        if sq is not None:
+            scale, shift, mean_norm, sqc, comp = sq
+            self.sq_shift = np.ascontiguousarray(shift, dtype=np.float32)
+            self.sq_codes = np.ascontiguousarray(sqc, dtype=np.uint8)
+            self.sq_comp = np.ascontiguousarray(comp, dtype=np.float32)
+            self.sq = SQ(self.vectors.shape[1], scale, _p(self.sq_shift),
+                         sq_shift_square_norm(self.sq_shift), mean_norm,
+                         _p(self.sq_codes), _p(self.sq_comp))
+            s.sq = C.cast(C.pointer(self.sq), C.c_void_p)
         self.s = s
 
     def search(self, query, k, L, bw=1, mode=MODE_FULL):

@@ -325,8 +325,10 @@ def main():
                     help="wall-clock budget of the CPU baseline sample")
     ap.add_argument("--workload", default="search1m",
                     help="search1m (default: batched Hybrid search, 1M x 128), kmeans768 "
-                         "(BF16 MFMA k-means, 1M x 768, K = 1024, 20 iterations) or "
-                         "prescore768 (exhaustive top-k with BF16 pre-scoring, 1M x 768)")
+                         "(BF16 MFMA k-means, 1M x 768, K = 1024, 20 iterations), "
+                         "prescore768 (exhaustive top-k with BF16 pre-scoring, 1M x 768) or "
+                         "sq1m (batched search with 8-bit scalar-quantized traversal and "
+                         "FP32 rerank, same 1M x 128 index as search1m)")
     ap.add_argument("--fixed", default="",
                     help="L,beam_width,visited_slots: skip the L sweep and the autotuner")
     ap.add_argument("--timed-only", action="store_true",
@@ -344,6 +346,10 @@ def main():
         return kmeans_workload(args)
     if args.workload == "prescore768":
         return prescore_workload(args)
+    if args.workload not in ("search1m", "sq1m"):
+        print("unknown workload %s" % args.workload, file=sys.stderr)
+        sys.exit(2)
+    SQ = args.workload == "sq1m"
 
     import diskann_amd as dk
     from diskann_amd import quantization as qz
@@ -362,15 +368,21 @@ def main():
     row, med = ix.medoid()
     ix.set_start_points(med)
     t0 = time.time()
-    sample = x[np.random.default_rng(0xD15C2002).choice(N, 100_000, replace=False)]
-    pivots, offsets = qz.train_pq(sample, M, 256, 10, seed=0xD15C3002)
-    ix.set_pq(pivots, offsets)
-    ix.encode()
+    if SQ:
+        # ScalarQuantizationParameters::default() (2 standard deviations) on every row
+        sqz = dk.scalar.ScalarQuantizer.train(x, 2.0)
+        ix.set_sq(sqz)
+        ix.encode_sq()
+    else:
+        sample = x[np.random.default_rng(0xD15C2002).choice(N, 100_000, replace=False)]
+        pivots, offsets = qz.train_pq(sample, M, 256, 10, seed=0xD15C3002)
+        ix.set_pq(pivots, offsets)
+        ix.encode()
     t_pq = time.time() - t0
     t0 = time.time()
     ix.build(np.arange(N), batch=1024, intra_batch=0, L_build=L_BUILD, alpha=ALPHA)
     t_build = time.time() - t0
-    log("pq train+encode %.2fs, graph build %.2fs" % (t_pq, t_build))
+    log("%s train+encode %.2fs, graph build %.2fs" % ("sq" if SQ else "pq", t_pq, t_build))
     gt, _ = ix.bruteforce(qs, K)
 
     # resident inputs / outputs
@@ -418,8 +430,9 @@ def main():
             rec = recall_at_10(dids.get(np.uint32, (NQ, K)), gt)
             sweep[L] = rec
             log("L=%d recall@10=%.4f" % (L, rec))
-            if L >= 128 and rec >= 0.95:
+            if L_star is None and rec >= 0.95:
                 L_star = L
+            if L >= 128 and L_star is not None:   # L = 32..128 always reported
                 break
         if L_star is None:
             L_star = max(sweep, key=lambda l: sweep[l])
@@ -470,9 +483,14 @@ def main():
     cmps = dcm.get(np.uint32, NQ).astype(np.float64)
     hops = dhp.get(np.uint32, NQ).astype(np.float64)
     degs = ddeg.get(np.uint32, NQ).astype(np.float64)
-    # algorithmic bytes per query (PQ traversal + rerank):
-    #   cmps*(M + 4) + hops*4 + degsum*4 + K*D*4 (pivots for the LUT) + (L+1)*D*4 (rerank)
-    bytes_per_q = cmps * (M + 4) + hops * 4 + degs * 4 + 256 * D * 4 + (L_star + 1) * D * 4
+    if SQ:
+        # scalar-quantized traversal + rerank: cmps*D (code rows, L2 reads no compensation)
+        #   + hops*4 + degsum*4 + D*4 (query) + (L+1)*D*4 (rerank)
+        bytes_per_q = cmps * D + hops * 4 + degs * 4 + D * 4 + (L_star + 1) * D * 4
+    else:
+        # PQ traversal + rerank: cmps*(M + 4) + hops*4 + degsum*4 + K*D*4 (pivots for the
+        # LUT) + (L+1)*D*4 (rerank)
+        bytes_per_q = cmps * (M + 4) + hops * 4 + degs * 4 + 256 * D * 4 + (L_star + 1) * D * 4
     alg_bytes = float(bytes_per_q.sum())
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
 
@@ -513,9 +531,14 @@ def main():
     # CPU baseline: the oracle (C restatement of the reference search) on this host
     from oracle import pyoracle as po
     lists, counts = ix.get_graph()
-    codes = ix.get_codes()
     vec = np.vstack([x, med[None, :]])
-    oix = po.OracleIndex(po.L2, vec, lists, counts, [N], codes, pivots, offsets)
+    if SQ:
+        sqc, sqcomp = dk.scalar.split(ix.get_sq_codes())
+        oix = po.OracleIndex(po.L2, vec, lists, counts, [N],
+                             sq=(sqz.scale, sqz.shift, sqz.mean_norm, sqc, sqcomp))
+    else:
+        codes = ix.get_codes()
+        oix = po.OracleIndex(po.L2, vec, lists, counts, [N], codes, pivots, offsets)
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     probe = 200
     _, _, _, _, w, _ = oix.search_batch(qs[:probe], K, L_star, BW, po.MODE_HYBRID, threads)
@@ -528,15 +551,23 @@ def main():
     # HBM bytes per launch from the rocprofv3 counter runs (profiles/), when they were
     # taken at the settings chosen here
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_hybrid_1M.json")
+    tf = os.path.join(ROOT, "profiles", "traffic_sq_1M.json" if SQ else "traffic_hybrid_1M.json")
     if os.path.exists(tf):
         with open(tf) as f:
             tr = json.load(f)
         if (tr.get("L"), tr.get("beam_width"), tr.get("visited_lds_slots")) == (L_star, BW, HSLOTS):
             traffic = tr.get("hbm_bytes_per_launch")
 
+    workload = ("batched scalar-quantized search: 1M x 128 f32 GMM(256, sigma 0.5), 8-bit "
+                "scalar quantizer (2 std), Vamana R=64 alpha=1.2, 10k queries/launch, k=10, "
+                "SQ traversal + FP32 rerank" if SQ else
+                "batched PQ search: 1M x 128 f32 GMM(256, sigma 0.5), PQ 32x256, "
+                "Vamana R=64 alpha=1.2, 10k queries/launch, k=10, Hybrid "
+                "(ADC traversal + FP32 rerank)")
     out = {
-        "metric": "QPS at recall@10=0.95 + p99 latency, served batched vs plain per-query GPU",
+        "metric": ("QPS at recall@10=0.95, batched 8-bit scalar-quantized traversal + FP32 "
+                   "rerank" if SQ else
+                   "QPS at recall@10=0.95 + p99 latency, served batched vs plain per-query GPU"),
         "value": qps,
         "unit": "queries/s",
         "n_gpus": 1,
@@ -549,9 +580,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "batched PQ search: 1M x 128 f32 GMM(256, sigma 0.5), PQ 32x256, "
-                        "Vamana R=64 alpha=1.2, 10k queries/launch, k=10, Hybrid "
-                        "(ADC traversal + FP32 rerank)",
+            "workload": workload,
             "n": N, "dim": D, "queries_per_launch": NQ, "k": K, "L": L_star,
             "beam_width": BW, "visited_lds_slots": HSLOTS, "adc_mode": ADC,
             "L_build": L_BUILD, "build_batch": 1024,
@@ -567,13 +596,14 @@ def main():
             "served_over_plain": (served["saturation_qps"] / plain_qps
                                   if "saturation_qps" in served else None),
             "gpu_cpu_parity_on_sample": parity,
-            "build_seconds": t_build, "pq_train_encode_seconds": t_pq,
+            "build_seconds": t_build, "quantizer_train_encode_seconds": t_pq,
             "mean_cmps": float(cmps.mean()), "mean_hops": float(hops.mean()),
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "beam_search_kernel<L2,PQ>", "kernel_ms": kernel_ms,
+            "kernel": "beam_search_kernel<L2,SQ8>" if SQ else "beam_search_kernel<L2,PQ>",
+            "kernel_ms": kernel_ms,
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "cpu_baseline": {

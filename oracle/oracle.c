@@ -621,6 +621,408 @@ float dko_sq_distance(int metric, float scale, float shift_square_norm, const ui
     return v;
 }
 
+/* ------------------------------------------------------------------------------------
+ * Spherical quantization, 1 bit
+ * ---------------------------------------------------------------------------------- */
+
+/* f32 -> f16 round to nearest even (the meta conversion of DataMeta::new,
+ * diskann-quantization/src/spherical/vectors.rs:261-294) and back (exact). */
+uint16_t dko_f32_to_f16(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t ex = (x >> 23) & 0xFFu;
+    uint32_t mant = x & 0x7FFFFFu;
+    if (ex == 0xFFu) return (uint16_t)(sign | 0x7C00u | (mant ? 0x200u : 0u));
+    const int32_t e = (int32_t)ex - 127 + 15;
+    if (e >= 31) return (uint16_t)(sign | 0x7C00u);
+    if (e <= 0) {
+        if (e < -10) return (uint16_t)sign;
+        mant |= 0x800000u;
+        const uint32_t sh = (uint32_t)(14 - e);
+        uint32_t h = mant >> sh;
+        const uint32_t rem = mant & ((1u << sh) - 1u), hw = 1u << (sh - 1u);
+        if (rem > hw || (rem == hw && (h & 1u))) h++;
+        return (uint16_t)(sign | h);
+    }
+    uint32_t h = ((uint32_t)e << 10) | (mant >> 13);
+    const uint32_t rem = mant & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    return (uint16_t)(sign | h);
+}
+
+float dko_f16_to_f32(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t ex = (h >> 10) & 0x1Fu, mant = h & 0x3FFu;
+    uint32_t x;
+    if (ex == 0x1Fu) {
+        x = sign | 0x7F800000u | (mant << 13);
+    } else if (ex == 0) {
+        if (mant == 0) {
+            x = sign;
+        } else {
+            const float v = (float)mant * (1.0f / 16777216.0f);   /* mant * 2^-24, exact */
+            memcpy(&x, &v, 4);
+            x |= sign;
+        }
+    } else {
+        x = sign | ((ex + 112u) << 23) | (mant << 13);
+    }
+    float f;
+    memcpy(&f, &x, 4);
+    return f;
+}
+
+void dko_sph_train_stats(int metric, const float *x, int64_t n, int64_t dim, float *centroid,
+                         float *mean_norm) {
+    double *m = (double *)calloc((size_t)dim, sizeof(double));
+    double norm_sum = 0.0;
+    for (int64_t r = 0; r < n; ++r) {
+        const float *row = x + r * dim;
+        double sq = 0.0;
+        for (int64_t j = 0; j < dim; ++j) sq += (double)row[j] * (double)row[j];
+        const double norm = sqrt(sq);
+        if (metric == DKO_COSINE) {
+            /* compute_normalized_means (src/utils.rs:141-178) */
+            const double inv = norm == 0.0 ? 1.0 : 1.0 / norm;
+            for (int64_t j = 0; j < dim; ++j) m[j] += (double)row[j] * inv;
+        } else {
+            for (int64_t j = 0; j < dim; ++j) m[j] += (double)row[j];
+            norm_sum += norm;
+        }
+    }
+    for (int64_t j = 0; j < dim; ++j) centroid[j] = (float)(m[j] / (double)n);
+    *mean_norm = metric == DKO_COSINE ? 1.0f : (float)(norm_sum / (double)n);
+    free(m);
+}
+
+int64_t dko_sph_data_bytes(const dko_sph *q) { return (q->out_dim + 7) / 8 + 6; }
+
+/* hadamard_transform (diskann-quantization/src/algorithms/hadamard.rs) as the x86-64-v3
+ * build runs it: recursive halving with (l + r, l - r) butterflies; a 64-element block is
+ * done by micro_kernel_64 (eight 8-point transforms as sequential +-sums of the block's
+ * 8-element groups, then butterflies at strides 8, 16, 32); finally x *= 1/sqrt(len). */
+static void had_rec(float *x, int64_t len) {
+    if (len == 64) {
+        float d[8][8];
+        for (int j = 0; j < 8; ++j)
+            for (int k = 0; k < 8; ++k) {
+                float acc = 0.0f;
+                for (int m = 0; m < 8; ++m) {
+                    /* H8[m][k] = (-1)^popcount(m & k) */
+                    const float hv = (__builtin_popcount((unsigned)(m & k)) & 1) ? -1.0f : 1.0f;
+                    acc = fmaf(x[8 * j + m], hv, acc);
+                }
+                d[j][k] = acc;
+            }
+        for (int k = 0; k < 8; ++k) {
+            const float e0 = d[0][k] + d[1][k], e1 = d[0][k] - d[1][k];
+            const float e2 = d[2][k] + d[3][k], e3 = d[2][k] - d[3][k];
+            const float e4 = d[4][k] + d[5][k], e5 = d[4][k] - d[5][k];
+            const float e6 = d[6][k] + d[7][k], e7 = d[6][k] - d[7][k];
+            const float f0 = e0 + e2, f1 = e1 + e3, f2 = e0 - e2, f3 = e1 - e3;
+            const float f4 = e4 + e6, f5 = e5 + e7, f6 = e4 - e6, f7 = e5 - e7;
+            x[k] = f0 + f4;
+            x[8 + k] = f1 + f5;
+            x[16 + k] = f2 + f6;
+            x[24 + k] = f3 + f7;
+            x[32 + k] = f0 - f4;
+            x[40 + k] = f1 - f5;
+            x[48 + k] = f2 - f6;
+            x[56 + k] = f3 - f7;
+        }
+        return;
+    }
+    if (len == 2) {
+        const float l = x[0], r = x[1];
+        x[0] = l + r;
+        x[1] = l - r;
+        return;
+    }
+    const int64_t h = len / 2;
+    had_rec(x, h);
+    had_rec(x + h, h);
+    for (int64_t i = 0; i < h; ++i) {
+        const float a = x[i] + x[h + i], b = x[i] - x[h + i];
+        x[i] = a;
+        x[h + i] = b;
+    }
+}
+
+void dko_sph_hadamard(float *x, int64_t len) {
+    if (len <= 1) return;
+    had_rec(x, len);
+    const float m = 1.0f / sqrtf((float)len);
+    for (int64_t i = 0; i < len; ++i) x[i] *= m;
+}
+
+static float flip(float v, uint32_t sign) {
+    uint32_t u;
+    memcpy(&u, &v, 4);
+    u ^= sign;
+    memcpy(&v, &u, 4);
+    return v;
+}
+
+/* PaddingHadamard / DoubleHadamard / NullTransform ::transform_into
+ * (algorithms/transforms/{padding_hadamard.rs:204-272, double_hadamard.rs:177-230,
+ * null.rs:38-42}). */
+void dko_sph_transform(const dko_sph *q, const float *src, float *dst) {
+    if (q->kind == DKO_SPH_NULL) {
+        memcpy(dst, src, sizeof(float) * (size_t)q->dim);
+        return;
+    }
+    int64_t n;
+    float *tmp;
+    if (q->kind == DKO_SPH_PADDING_HADAMARD) {
+        n = q->padded_dim;
+        tmp = (float *)calloc((size_t)n, sizeof(float));
+        for (int64_t i = 0; i < q->dim; ++i) tmp[i] = flip(src[i], q->signs0[i]);
+        dko_sph_hadamard(tmp, n);
+    } else {
+        n = q->dim > q->out_dim ? q->dim : q->out_dim;
+        if (q->subsample) n = q->dim;  /* subsampling keeps the input dimension */
+        tmp = (float *)calloc((size_t)n, sizeof(float));
+        for (int64_t i = 0; i < q->dim; ++i) tmp[i] = flip(src[i], q->signs0[i]);
+        int64_t split = 1;
+        while (split * 2 <= n) split *= 2;
+        dko_sph_hadamard(tmp, split);
+        for (int64_t i = 0; i < n; ++i) tmp[i] = flip(tmp[i], q->signs1[i]);
+        dko_sph_hadamard(tmp + (n - split), split);
+    }
+    if (q->subsample) {
+        const float rescale = sqrtf((float)n / (float)q->nsub);
+        for (int64_t i = 0; i < q->nsub; ++i) dst[i] = tmp[q->subsample[i]] * rescale;
+    } else {
+        memcpy(dst, tmp, sizeof(float) * (size_t)n);
+    }
+    free(tmp);
+}
+
+static int sph_preserves_norms(const dko_sph *q) { return q->subsample == NULL; }
+
+/* SphericalQuantizer::rescale (spherical/quantizer.rs:322-326): v *= mean_norm / |v|. */
+void dko_sph_rescale(const dko_sph *q, float *x) {
+    const float norm = sqrtf(simd_4x1(1, x, x, q->dim));
+    const float m = q->mean_norm / norm;
+    for (int64_t i = 0; i < q->dim; ++i) x[i] *= m;
+}
+
+/* SphericalQuantizer::preprocess (spherical/quantizer.rs:333-377) then normalisation and
+ * the transform. Returns the transformed vector (out_dim, malloc'd) or NULL when the
+ * shifted norm is zero; *sn = shifted norm, *ms = metric term. */
+static float *sph_prepare(const dko_sph *q, const float *x, float *sn, float *ms, int *err) {
+    *err = 0;
+    float mul = q->pre_scale;
+    if (q->metric == DKO_COSINE) {
+        const float norm = sqrtf(simd_4x1(1, x, x, q->dim));
+        mul = norm == 0.0f ? 1.0f : 1.0f / norm;
+    }
+    float *sh = (float *)malloc(sizeof(float) * (size_t)q->dim);
+    for (int64_t i = 0; i < q->dim; ++i) sh[i] = mul * x[i] - q->shift[i];
+    const float norm = sqrtf(simd_4x1(1, sh, sh, q->dim));
+    if (!isfinite(norm)) {
+        *err = 1;
+        free(sh);
+        return NULL;
+    }
+    *sn = norm;
+    *ms = q->metric == DKO_L2 ? norm * norm : simd_4x1(1, sh, q->shift, q->dim);
+    if (norm == 0.0f) {
+        free(sh);
+        return NULL;
+    }
+    for (int64_t i = 0; i < q->dim; ++i) sh[i] /= norm;
+    float *t = (float *)malloc(sizeof(float) * (size_t)(q->out_dim + 1));
+    dko_sph_transform(q, sh, t);
+    free(sh);
+    return t;
+}
+
+static void put_u16(uint8_t *p, uint16_t v) {
+    p[0] = (uint8_t)(v & 0xFF);
+    p[1] = (uint8_t)(v >> 8);
+}
+static uint16_t get_u16(const uint8_t *p) { return (uint16_t)(p[0] | (p[1] << 8)); }
+
+/* CompressIntoWith<DataMut<1>> + FinishCompressing for 1 bit (spherical/quantizer.rs:
+ * 591-642, 779-857): bit = t > 0, quant_raw_ip = sequential sum of |t|, correction =
+ * 2 |t| |shifted| / quant_raw_ip; meta stored as f16, f16, u16. */
+int dko_sph_compress_data(const dko_sph *q, const float *x, uint8_t *out) {
+    const int64_t nb = (q->out_dim + 7) / 8;
+    memset(out, 0, (size_t)(nb + 6));
+    float sn = 0.0f, ms = 0.0f;
+    int err;
+    float *t = sph_prepare(q, x, &sn, &ms, &err);
+    if (err) return err;
+    if (!t) return 0;  /* the centroid itself: default meta, no bits */
+    const float tn = sph_preserves_norms(q) ? 1.0f : sqrtf(simd_4x1(1, t, t, q->out_dim));
+    float raw = 0.0f;
+    uint32_t bits = 0;
+    for (int64_t i = 0; i < q->out_dim; ++i) {
+        const int b = t[i] > 0.0f;
+        raw += fabsf(t[i]);
+        bits += (uint32_t)b;
+        if (b) out[i >> 3] |= (uint8_t)(1u << (i & 7));
+    }
+    free(t);
+    const float ipc = 2.0f * tn * sn / raw;
+    const uint16_t h0 = dko_f32_to_f16(ipc), h1 = dko_f32_to_f16(ms);
+    if ((h0 & 0x7C00u) == 0x7C00u || (h1 & 0x7C00u) == 0x7C00u || bits > 0xFFFFu) return 2;
+    put_u16(out + nb, h0);
+    put_u16(out + nb + 2, h1);
+    put_u16(out + nb + 4, (uint16_t)bits);
+    return 0;
+}
+
+/* CompressIntoWith<QueryMut<4, _>> (spherical/quantizer.rs:1101-1189): codes
+ * round((t - min) / scale) clamped to [0, 15] with scale = (max - min) / 15. */
+int dko_sph_compress_query4(const dko_sph *q, const float *x, uint8_t *codes, float *meta) {
+    memset(codes, 0, (size_t)q->out_dim);
+    meta[0] = meta[1] = meta[2] = meta[3] = 0.0f;
+    float sn = 0.0f, ms = 0.0f;
+    int err;
+    float *t = sph_prepare(q, x, &sn, &ms, &err);
+    if (err) return err;
+    if (!t) return 0;
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    for (int64_t i = 0; i < q->out_dim; ++i) {
+        mn = fminf(t[i], mn);
+        mx = fmaxf(t[i], mx);
+    }
+    const float scale = (mx - mn) / 15.0f;
+    float bs = 0.0f;
+    for (int64_t i = 0; i < q->out_dim; ++i) {
+        float c = roundf((t[i] - mn) / scale);
+        if (c < 0.0f) c = 0.0f;
+        else if (c > 15.0f) c = 15.0f;
+        bs += c;
+        codes[i] = (uint8_t)c;
+    }
+    free(t);
+    meta[0] = sn * scale;
+    meta[1] = bs;
+    meta[2] = mn / scale;
+    meta[3] = ms;
+    return 0;
+}
+
+/* CompressIntoWith<FullQueryMut> (spherical/quantizer.rs:702-777). */
+int dko_sph_compress_full(const dko_sph *q, const float *x, float *v, float *meta) {
+    memset(v, 0, sizeof(float) * (size_t)q->out_dim);
+    meta[0] = meta[1] = meta[2] = 0.0f;
+    float sn = 0.0f, ms = 0.0f;
+    int err;
+    float *t = sph_prepare(q, x, &sn, &ms, &err);
+    if (err) return err;
+    if (!t) return 0;
+    float sum = 0.0f;
+    for (int64_t i = 0; i < q->out_dim; ++i) {
+        v[i] = t[i];
+        sum += t[i];
+    }
+    free(t);
+    meta[0] = sum;
+    meta[1] = sn;
+    meta[2] = ms;
+    return 0;
+}
+
+static void data_meta(const dko_sph *q, const uint8_t *c, float *ipc, float *ms, float *bs) {
+    const int64_t nb = (q->out_dim + 7) / 8;
+    *ipc = dko_f16_to_f32(get_u16(c + nb));
+    *ms = dko_f16_to_f32(get_u16(c + nb + 2));
+    *bs = (float)get_u16(c + nb + 4);
+}
+
+static float sph_shift_sq(const dko_sph *q) { return simd_4x1(1, q->shift, q->shift, q->dim); }
+
+/* the metric's distance from the mathematical inner-product / L2 value */
+static float sph_finish(const dko_sph *q, float v) {
+    if (q->metric == DKO_L2) return v;
+    if (q->metric == DKO_COSINE) return 1.0f - v;
+    return -v;
+}
+
+/* CompensatedSquaredL2 / CompensatedIP / CompensatedCosine on two data codes
+ * (spherical/vectors.rs:493-535, 570-595, 730-759). */
+float dko_sph_dist_data(const dko_sph *q, const uint8_t *a, const uint8_t *b) {
+    const int64_t nb = (q->out_dim + 7) / 8;
+    uint32_t ip = 0;
+    for (int64_t i = 0; i < nb; ++i) ip += (uint32_t)__builtin_popcount((unsigned)(a[i] & b[i]));
+    float xi, xm, xb, yi, ym, yb;
+    data_meta(q, a, &xi, &xm, &xb);
+    data_meta(q, b, &yi, &ym, &yb);
+    const float dim = (float)q->out_dim;
+    const float k = xi * yi * ((float)ip - 0.5f * (xb + yb) + 0.25f * dim);
+    if (q->metric == DKO_L2) return xm + ym - 2.0f * k;
+    return sph_finish(q, xm + ym + k + sph_shift_sq(q));
+}
+
+/* QueryRef<4> x DataRef<1> (spherical/vectors.rs:597-641, 761-806). */
+float dko_sph_dist_query4(const dko_sph *q, const uint8_t *codes, const float *meta,
+                          const uint8_t *b) {
+    uint32_t ip = 0;
+    for (int64_t i = 0; i < q->out_dim; ++i)
+        if (b[i >> 3] & (1u << (i & 7))) ip += codes[i];
+    float yi, ym, yb;
+    data_meta(q, b, &yi, &ym, &yb);
+    const float dim = (float)q->out_dim;
+    const float cip = yi * meta[0] * ((float)ip - 0.5f * meta[1] + meta[2] * yb -
+                                      0.5f * meta[2] * dim);
+    if (q->metric == DKO_L2) return ym + meta[3] - 2.0f * cip;
+    return sph_finish(q, cip + ym + meta[3] + sph_shift_sq(q));
+}
+
+/* InnerProduct(&[f32], USlice<1>) as the x86-64-v3 build computes it
+ * (diskann-quantization/src/bits/distances.rs:2295-2436): two f32x8 accumulators
+ * alternating over the four 8-lane groups of each 32-element block, combined, then the
+ * 8-lane groups of the remainder, then sum_tree. */
+static float ip_f32_bits(const float *x, const uint8_t *bits, int64_t len) {
+    float s[8] = {0}, s0[8] = {0}, s1[8] = {0};
+    const int64_t blocks = len / 32;
+    if (blocks > 0) {
+        for (int64_t i = 0; i < blocks; ++i)
+            for (int v = 0; v < 4; ++v)
+                for (int l = 0; l < 8; ++l) {
+                    const int64_t e = 32 * i + 8 * v + l;
+                    const float bv = (bits[e >> 3] >> (e & 7)) & 1 ? 1.0f : 0.0f;
+                    float *acc = (v & 1) ? s1 : s0;
+                    acc[l] = fmaf(x[e], bv, acc[l]);
+                }
+        for (int l = 0; l < 8; ++l) s[l] = s0[l] + s1[l];
+    }
+    const int64_t rem = len % 32;
+    if (rem) {
+        const int64_t tail = len % 8 == 0 ? 8 : len % 8;
+        const int nv = rem <= 8 ? 1 : rem <= 16 ? 2 : rem <= 24 ? 3 : 4;
+        for (int v = 0; v < nv; ++v) {
+            const int64_t cnt = v == nv - 1 ? tail : 8;
+            for (int l = 0; l < 8; ++l) {
+                const int64_t e = 32 * blocks + 8 * v + l;
+                const float xv = l < cnt ? x[e] : 0.0f;
+                const float bv = l < cnt && ((bits[e >> 3] >> (e & 7)) & 1) ? 1.0f : 0.0f;
+                s[l] = fmaf(xv, bv, s[l]);
+            }
+        }
+    }
+    return sum_tree8(s);
+}
+
+/* FullQueryRef x DataRef<1> (spherical/vectors.rs:643-691, 808-850). */
+float dko_sph_dist_full(const dko_sph *q, const float *v, const float *meta, const uint8_t *b) {
+    const float s = ip_f32_bits(v, b, q->out_dim);
+    float yi, ym, yb;
+    data_meta(q, b, &yi, &ym, &yb);
+    (void)yb;
+    if (q->metric == DKO_L2) {
+        const float ip = s - meta[0] * 0.5f;
+        return meta[2] + ym - 2.0f * meta[1] * yi * ip;
+    }
+    const float ip = meta[1] * yi * (s - meta[0] * 0.5f);
+    return sph_finish(q, ip + meta[2] + ym + sph_shift_sq(q));
+}
+
 static inline float node_dist(const dko_index *ix, int pq, const float *query,
                               const float *lut, uint32_t id) {
     if (pq && ix->sq) {

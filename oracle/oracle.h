@@ -82,6 +82,47 @@ void dko_sq_rescale(float *x, int64_t dim, float to_norm);
 float dko_sq_distance(int metric, float scale, float shift_square_norm, const uint8_t *a,
                       float comp_a, const uint8_t *b, float comp_b, int64_t dim);
 
+/* Spherical quantization, 1 bit (diskann-quantization/src/spherical) ------------------- */
+enum { DKO_SPH_NULL = 0, DKO_SPH_PADDING_HADAMARD = 1, DKO_SPH_DOUBLE_HADAMARD = 2 };
+typedef struct {
+    int metric;                /* DKO_L2, DKO_INNER_PRODUCT or DKO_COSINE */
+    int64_t dim;               /* input dimension */
+    int64_t out_dim;           /* transform output dimension (bits per code) */
+    const float *shift;        /* [dim] centroid */
+    float mean_norm, pre_scale;
+    int kind;                  /* DKO_SPH_* */
+    const uint32_t *signs0;    /* [dim] f32 sign masks (padding Hadamard: its signs) */
+    const uint32_t *signs1;    /* double Hadamard: [max(dim, out_dim)] */
+    int64_t padded_dim;        /* padding Hadamard: power of two >= dim */
+    const uint32_t *subsample; /* optional strictly increasing indices, nsub = out_dim */
+    int64_t nsub;
+} dko_sph;
+
+/* bytes of one 1-bit data code: bits (little-endian within bytes), then f16 inner-product
+ * correction, f16 metric term, u16 bit sum (DataMut::from_canonical_back_mut) */
+/* SphericalQuantizer::train's deterministic part (spherical/quantizer.rs:225-320): the
+ * centroid (column means in row order; cosine: means of the unit-normalised rows) and the
+ * mean norm (1 for cosine). The transform's random signs are not reproduced. */
+void dko_sph_train_stats(int metric, const float *x, int64_t n, int64_t dim, float *centroid,
+                         float *mean_norm);
+int64_t dko_sph_data_bytes(const dko_sph *q);
+void dko_sph_hadamard(float *x, int64_t len);
+void dko_sph_transform(const dko_sph *q, const float *src, float *dst);
+void dko_sph_rescale(const dko_sph *q, float *x);
+/* 0 ok, 1 NaN / infinite input, 2 correction not representable in f16 */
+int dko_sph_compress_data(const dko_sph *q, const float *x, uint8_t *out);
+/* 4-bit query: out_dim codes (one per byte) + meta {ip correction, bit sum, offset,
+ * metric term} */
+int dko_sph_compress_query4(const dko_sph *q, const float *x, uint8_t *codes, float *meta);
+/* full-precision query: out_dim transformed values + meta {sum, shifted norm, metric term} */
+int dko_sph_compress_full(const dko_sph *q, const float *x, float *v, float *meta);
+float dko_sph_dist_data(const dko_sph *q, const uint8_t *a, const uint8_t *b);
+float dko_sph_dist_query4(const dko_sph *q, const uint8_t *codes, const float *meta,
+                          const uint8_t *b);
+float dko_sph_dist_full(const dko_sph *q, const float *v, const float *meta, const uint8_t *b);
+uint16_t dko_f32_to_f16(float f);
+float dko_f16_to_f32(uint16_t h);
+
 /* Graph + search -------------------------------------------------------------------- */
 typedef struct {
     int metric;

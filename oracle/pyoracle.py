@@ -83,6 +83,29 @@ def _setup(L):
     L.dko_sq_rescale.argtypes = [P, I64, F]
     L.dko_sq_distance.argtypes = [C.c_int, F, F, P, F, P, F, I64]
     L.dko_sq_distance.restype = F
+    S = C.POINTER(SphStruct)
+    L.dko_sph_train_stats.argtypes = [C.c_int, P, I64, I64, P, P]
+    L.dko_sph_data_bytes.argtypes = [S]
+    L.dko_sph_data_bytes.restype = I64
+    L.dko_sph_hadamard.argtypes = [P, I64]
+    L.dko_sph_transform.argtypes = [S, P, P]
+    L.dko_sph_rescale.argtypes = [S, P]
+    L.dko_sph_compress_data.argtypes = [S, P, P]
+    L.dko_sph_compress_data.restype = C.c_int
+    L.dko_sph_compress_query4.argtypes = [S, P, P, P]
+    L.dko_sph_compress_query4.restype = C.c_int
+    L.dko_sph_compress_full.argtypes = [S, P, P, P]
+    L.dko_sph_compress_full.restype = C.c_int
+    L.dko_sph_dist_data.argtypes = [S, P, P]
+    L.dko_sph_dist_data.restype = F
+    L.dko_sph_dist_query4.argtypes = [S, P, P, P]
+    L.dko_sph_dist_query4.restype = F
+    L.dko_sph_dist_full.argtypes = [S, P, P, P]
+    L.dko_sph_dist_full.restype = F
+    L.dko_f32_to_f16.argtypes = [F]
+    L.dko_f32_to_f16.restype = C.c_uint16
+    L.dko_f16_to_f32.argtypes = [C.c_uint16]
+    L.dko_f16_to_f32.restype = F
     L.dko_pq_encode.argtypes = [P, I64, I64, P, I64, P, I64, P]
     L.dko_pq_direct.argtypes = [C.c_int, P, I64, P, I64, P, P]
     L.dko_pq_direct.restype = C.c_float
@@ -226,6 +249,120 @@ def sq_distance(metric, scale, shift_square_norm, a, comp_a, b, comp_b):
     b = np.ascontiguousarray(b, dtype=np.uint8)
     return lib().dko_sq_distance(metric, scale, shift_square_norm, _p(a), comp_a, _p(b),
                                  comp_b, a.shape[0])
+
+
+class SphStruct(C.Structure):
+    _fields_ = [
+        ("metric", C.c_int), ("dim", C.c_int64), ("out_dim", C.c_int64), ("shift", C.c_void_p),
+        ("mean_norm", C.c_float), ("pre_scale", C.c_float), ("kind", C.c_int),
+        ("signs0", C.c_void_p), ("signs1", C.c_void_p), ("padded_dim", C.c_int64),
+        ("subsample", C.c_void_p), ("nsub", C.c_int64),
+    ]
+
+
+SPH_NULL, SPH_PADDING_HADAMARD, SPH_DOUBLE_HADAMARD = 0, 1, 2
+
+
+def _signs(b):
+    return np.ascontiguousarray(np.where(np.asarray(b, bool), 0x80000000, 0), dtype=np.uint32)
+
+
+class Spherical:
+    """1-bit spherical quantizer on the oracle. kind: SPH_*; signs0 / signs1 booleans
+    (True = flip); subsample: optional index array."""
+
+    def __init__(self, metric, shift, mean_norm, pre_scale=1.0, kind=SPH_DOUBLE_HADAMARD,
+                 signs0=None, signs1=None, padded_dim=0, subsample=None):
+        self.shift = np.ascontiguousarray(shift, dtype=np.float32)
+        dim = self.shift.shape[0]
+        self.signs0 = _signs(signs0 if signs0 is not None else np.zeros(dim, bool))
+        self.signs1 = _signs(signs1) if signs1 is not None else None
+        self.subsample = (None if subsample is None else
+                          np.ascontiguousarray(subsample, dtype=np.uint32))
+        if self.subsample is not None:
+            out = self.subsample.shape[0]
+        elif kind == SPH_PADDING_HADAMARD:
+            out = padded_dim
+        elif kind == SPH_DOUBLE_HADAMARD:
+            out = self.signs1.shape[0]
+        else:
+            out = dim
+        self.out_dim = out
+        self.metric = metric
+        self.s = SphStruct(metric, dim, out, _p(self.shift), mean_norm, pre_scale, kind,
+                           _p(self.signs0), _p(self.signs1), padded_dim, _p(self.subsample),
+                           0 if self.subsample is None else self.subsample.shape[0])
+        self.data_bytes = lib().dko_sph_data_bytes(C.byref(self.s))
+
+    def transform(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.zeros(self.out_dim, np.float32)
+        lib().dko_sph_transform(C.byref(self.s), _p(x), _p(out))
+        return out
+
+    def rescale(self, x):
+        x = np.array(x, dtype=np.float32)
+        lib().dko_sph_rescale(C.byref(self.s), _p(x))
+        return x
+
+    def compress(self, x):
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float32)
+        out = np.zeros((x.shape[0], self.data_bytes), np.uint8)
+        for r in range(x.shape[0]):
+            rc = lib().dko_sph_compress_data(C.byref(self.s), _p(x[r]), _p(out[r]))
+            if rc:
+                raise ValueError("spherical compression error %d (row %d)" % (rc, r))
+        return out
+
+    def compress_query4(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        codes = np.zeros(self.out_dim, np.uint8)
+        meta = np.zeros(4, np.float32)
+        rc = lib().dko_sph_compress_query4(C.byref(self.s), _p(x), _p(codes), _p(meta))
+        if rc:
+            raise ValueError("spherical compression error %d" % rc)
+        return codes, meta
+
+    def compress_full(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        v = np.zeros(self.out_dim, np.float32)
+        meta = np.zeros(3, np.float32)
+        rc = lib().dko_sph_compress_full(C.byref(self.s), _p(x), _p(v), _p(meta))
+        if rc:
+            raise ValueError("spherical compression error %d" % rc)
+        return v, meta
+
+    def dist_data(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.uint8)
+        b = np.ascontiguousarray(b, dtype=np.uint8)
+        return lib().dko_sph_dist_data(C.byref(self.s), _p(a), _p(b))
+
+    def dist_query4(self, codes, meta, b):
+        b = np.ascontiguousarray(b, dtype=np.uint8)
+        return lib().dko_sph_dist_query4(C.byref(self.s), _p(codes), _p(meta), _p(b))
+
+    def dist_full(self, v, meta, b):
+        b = np.ascontiguousarray(b, dtype=np.uint8)
+        return lib().dko_sph_dist_full(C.byref(self.s), _p(v), _p(meta), _p(b))
+
+
+def sph_train_stats(metric, x):
+    """(centroid, mean_norm) of SphericalQuantizer::train."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    c = np.zeros(x.shape[1], np.float32)
+    mn = C.c_float()
+    lib().dko_sph_train_stats(metric, _p(x), x.shape[0], x.shape[1], _p(c), C.byref(mn))
+    return c, float(np.float32(mn.value))
+
+
+def hadamard(x):
+    x = np.array(x, dtype=np.float32)
+    lib().dko_sph_hadamard(_p(x), x.shape[0])
+    return x
+
+
+def f32_to_f16_bits(v):
+    return lib().dko_f32_to_f16(float(v))
 
 
 class OracleIndex:

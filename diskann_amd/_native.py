@@ -29,6 +29,9 @@ EXPORTS = [
     "dkm_prescore_create", "dkm_prescore_search", "dkm_prescore_destroy",
     "dkm_sq_train", "dkm_sq_compress", "dkm_sq_distance_batch", "dkm_set_sq",
     "dkm_encode_index_sq", "dkm_set_sq_codes", "dkm_get_sq_codes", "dkm_use_pq",
+    "dkm_sph_create", "dkm_sph_destroy", "dkm_sph_dims", "dkm_sph_train_stats",
+    "dkm_sph_compress", "dkm_sph_distance_batch", "dkm_sph_query_distance_batch",
+    "dkm_set_spherical", "dkm_encode_index_sph", "dkm_set_sph_codes", "dkm_get_sph_codes",
 ]
 
 _lib = None
@@ -83,6 +86,17 @@ def _declare(L):
         "dkm_set_sq_codes": [P, P, I64, I64],
         "dkm_get_sq_codes": [P, P, I64, I64],
         "dkm_use_pq": [P],
+        "dkm_sph_create": [I, I64, P, C.c_float, C.c_float, I, P, P, I64, I64, P, I64,
+                           C.POINTER(C.c_void_p)],
+        "dkm_sph_dims": [P, C.POINTER(I64), C.POINTER(I64)],
+        "dkm_sph_train_stats": [I, P, I64, I64, P, P],
+        "dkm_sph_compress": [P, P, I64, P],
+        "dkm_sph_distance_batch": [P, P, I64, P, I64, P, I64, P],
+        "dkm_sph_query_distance_batch": [P, I, I, P, I64, P, I64, P, I64, P],
+        "dkm_set_spherical": [P, P, I],
+        "dkm_encode_index_sph": [P],
+        "dkm_set_sph_codes": [P, P, I64, I64],
+        "dkm_get_sph_codes": [P, P, I64, I64],
         "dkm_get_codes": [P, P, I64, I64],
         "dkm_search_batch": [P, I, P, I64, P, P, U32, I64, P, P, P, P, P],
         "dkm_search_batch_device": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P],
@@ -129,3 +143,5 @@ def _declare(L):
     L.dkm_index_destroy.restype = None
     L.dkm_prescore_destroy.argtypes = [P]
     L.dkm_prescore_destroy.restype = None
+    L.dkm_sph_destroy.argtypes = [P]
+    L.dkm_sph_destroy.restype = None

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -21,6 +22,7 @@
 #include "../../include/diskann_mi355x.h"
 #include "dkm_build.h"
 #include "dkm_sq.h"
+#include "dkm_sph.h"
 #include "dkm_exact.h"
 #include "dkm_kernels.h"
 #include "dkm_kmeans.h"
@@ -119,6 +121,14 @@ struct dkm_index {
     int64_t sq_cstride = 0;
     DevBuf<float> d_sq_shift, d_sq_comp;
     DevBuf<uint8_t> d_sq_codes;
+    // 1-bit spherical quantization; when sph_active the quantized strategies traverse with it
+    bool sph_active = false, sph_codes_set = false;
+    int sph_layout = 1;
+    dkm::SphParams sphq{};
+    DevBuf<float> d_sph_shift;
+    DevBuf<uint32_t> d_sph_signs0, d_sph_signs1, d_sph_sub;
+    DevBuf<uint8_t> d_sph_bits;
+    DevBuf<dkm::SphMeta> d_sph_meta;
     // staging for the host-buffer API
     DevBuf<float> s_q, s_d;
     DevBuf<uint32_t> s_k, s_L, s_ids, s_cnt, s_cmps, s_hops, s_status;
@@ -765,6 +775,7 @@ dkm_status dkm_set_sq(dkm_index *ix, float scale, const float *shift, float mean
     ix->sq_scale = scale;
     ix->sq_mean_norm = mean_norm;
     ix->sq_active = true;
+    ix->sph_active = false;
     ix->sq_codes_set = false;
     return DKM_OK;
 }
@@ -822,6 +833,394 @@ dkm_status dkm_use_pq(dkm_index *ix) {
     if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
     std::lock_guard<std::mutex> g(ix->mu);
     ix->sq_active = false;
+    ix->sph_active = false;
+    return DKM_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// 1-bit spherical quantization (diskann-quantization/src/spherical)
+// ------------------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct dkm_sph {
+    dkm::SphParams q{};                  // device pointers below
+    DevBuf<float> shift;
+    DevBuf<uint32_t> signs0, signs1, sub;
+};
+
+namespace {
+
+uint16_t host_f32_to_f16(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u, ex = (x >> 23) & 0xFFu;
+    uint32_t mant = x & 0x7FFFFFu;
+    if (ex == 0xFFu) return (uint16_t)(sign | 0x7C00u | (mant ? 0x200u : 0u));
+    const int32_t e = (int32_t)ex - 112;
+    if (e >= 31) return (uint16_t)(sign | 0x7C00u);
+    if (e <= 0) {
+        if (e < -10) return (uint16_t)sign;
+        mant |= 0x800000u;
+        const uint32_t sh = (uint32_t)(14 - e);
+        uint32_t h = mant >> sh;
+        const uint32_t rem = mant & ((1u << sh) - 1u), hw = 1u << (sh - 1u);
+        if (rem > hw || (rem == hw && (h & 1u))) h++;
+        return (uint16_t)(sign | h);
+    }
+    uint32_t h = ((uint32_t)e << 10) | (mant >> 13);
+    const uint32_t rem = mant & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    return (uint16_t)(sign | h);
+}
+
+float host_f16_to_f32(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, ex = (h >> 10) & 0x1Fu, mant = h & 0x3FFu;
+    uint32_t x;
+    if (ex == 0x1Fu) {
+        x = sign | 0x7F800000u | (mant << 13);
+    } else if (ex == 0) {
+        if (mant == 0) return sign ? -0.0f : 0.0f;
+        const float v = (float)mant * (1.0f / 16777216.0f);
+        return sign ? -v : v;
+    } else {
+        x = sign | ((ex + 112u) << 23) | (mant << 13);
+    }
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
+}
+
+// canonical rows (bits, f16, f16, u16) <-> device bit rows + SphMeta
+dkm_status sph_upload(const dkm::SphParams &q, const uint8_t *canon, int64_t n,
+                      DevBuf<uint8_t> &bits, DevBuf<dkm::SphMeta> &meta, int64_t cap = -1) {
+    const int64_t cb = dkm::sph_canon_bytes(q.out_dim), nb = (q.out_dim + 7) / 8;
+    std::vector<uint8_t> rows((size_t)(n * q.cstride), 0);
+    std::vector<dkm::SphMeta> ms((size_t)n);
+    for (int64_t r = 0; r < n; ++r) {
+        const uint8_t *c = canon + r * cb;
+        std::memcpy(&rows[(size_t)(r * q.cstride)], c, (size_t)nb);
+        if (q.out_dim % 8) rows[(size_t)(r * q.cstride + nb - 1)] &= (uint8_t)((1u << (q.out_dim % 8)) - 1u);
+        auto u16 = [&](int k) { return (uint16_t)(c[nb + 2 * k] | (c[nb + 2 * k + 1] << 8)); };
+        ms[(size_t)r] = {host_f16_to_f32(u16(0)), host_f16_to_f32(u16(1)), (float)u16(2), 0.0f};
+    }
+    HIP_TRY(bits.ensure(cap >= 0 ? cap * q.cstride : n * q.cstride));
+    HIP_TRY(meta.ensure(cap >= 0 ? cap : n));
+    HIP_TRY(hipMemcpy(bits.p, rows.data(), rows.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(meta.p, ms.data(), sizeof(dkm::SphMeta) * n, hipMemcpyHostToDevice));
+    return DKM_OK;
+}
+
+dkm_status sph_download(const dkm::SphParams &q, const uint8_t *bits, const dkm::SphMeta *meta,
+                        int64_t n, uint8_t *canon) {
+    const int64_t cb = dkm::sph_canon_bytes(q.out_dim), nb = (q.out_dim + 7) / 8;
+    std::vector<uint8_t> rows((size_t)(n * q.cstride));
+    std::vector<dkm::SphMeta> ms((size_t)n);
+    HIP_TRY(hipMemcpy(rows.data(), bits, rows.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ms.data(), meta, sizeof(dkm::SphMeta) * n, hipMemcpyDeviceToHost));
+    for (int64_t r = 0; r < n; ++r) {
+        uint8_t *c = canon + r * cb;
+        std::memcpy(c, &rows[(size_t)(r * q.cstride)], (size_t)nb);
+        const uint16_t h[3] = {host_f32_to_f16(ms[(size_t)r].ipc), host_f32_to_f16(ms[(size_t)r].ms),
+                               (uint16_t)ms[(size_t)r].bs};
+        for (int k = 0; k < 3; ++k) {
+            c[nb + 2 * k] = (uint8_t)(h[k] & 0xFFu);
+            c[nb + 2 * k + 1] = (uint8_t)(h[k] >> 8);
+        }
+    }
+    return DKM_OK;
+}
+
+int sph_metric(int m) {
+    if (m == DKM_L2) return dkm::METRIC_L2;
+    if (m == DKM_INNER_PRODUCT) return dkm::METRIC_IP;
+    if (m == DKM_COSINE) return dkm::METRIC_COSINE;
+    return -1;
+}
+
+dkm_status sph_errors(uint32_t e) {
+    if (e & dkm::SPH_ERR_NAN) return fail(DKM_ERR_INVALID_ARG, "input contains NaN or infinite values");
+    if (e & dkm::SPH_ERR_META) return fail(DKM_ERR_INVALID_ARG, "a correction term does not fit in f16");
+    return DKM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+dkm_status dkm_sph_create(int metric, int64_t dim, const float *centroid, float mean_norm,
+                          float pre_scale, int transform, const uint8_t *signs0,
+                          const uint8_t *signs1, int64_t n_signs1, int64_t padded_dim,
+                          const uint32_t *subsample, int64_t n_subsample, dkm_sph **out) {
+    if (!out || !centroid || dim <= 0) return fail(DKM_ERR_INVALID_ARG, "bad spherical quantizer");
+    *out = nullptr;
+    const int m = sph_metric(metric);
+    if (m < 0) return fail(DKM_ERR_INVALID_ARG, "spherical quantization supports L2, inner product and cosine");
+    if (!(mean_norm > 0.0f) || !(pre_scale > 0.0f)) return fail(DKM_ERR_INVALID_ARG, "mean_norm and pre_scale must be positive");
+    dkm::SphParams q{};
+    q.metric = m;
+    q.dim = (int)dim;
+    q.kind = transform;
+    if (transform == dkm::SPH_NULL) {
+        q.n = q.out_dim = (int)dim;
+        if (subsample) return fail(DKM_ERR_INVALID_ARG, "the null transform has no subsample");
+    } else if (transform == dkm::SPH_PADDING_HADAMARD) {
+        if (!signs0 || padded_dim < dim || (padded_dim & (padded_dim - 1)))
+            return fail(DKM_ERR_INVALID_ARG, "padding Hadamard needs dim signs and a power-of-two padded dim >= dim");
+        q.n = (int)padded_dim;
+        q.out_dim = subsample ? (int)n_subsample : (int)padded_dim;
+    } else if (transform == dkm::SPH_DOUBLE_HADAMARD) {
+        if (!signs0 || !signs1) return fail(DKM_ERR_INVALID_ARG, "double Hadamard needs both sign vectors");
+        q.out_dim = subsample ? (int)n_subsample : (int)n_signs1;
+        q.n = std::max<int>((int)dim, q.out_dim);
+        if (n_signs1 != q.n) return fail(DKM_ERR_INVALID_ARG, "double Hadamard: signs1 must have max(dim, output dim) entries");
+    } else {
+        return fail(DKM_ERR_INVALID_ARG, "unknown transform %d", transform);
+    }
+    if (subsample) {
+        if (n_subsample <= 0 || n_subsample >= q.n) return fail(DKM_ERR_INVALID_ARG, "bad subsample length");
+        for (int64_t i = 0; i < n_subsample; ++i)
+            if (subsample[i] >= (uint32_t)q.n || (i > 0 && subsample[i] <= subsample[i - 1]))
+                return fail(DKM_ERR_INVALID_ARG, "subsample indices must increase strictly and stay below %d", q.n);
+        q.sub_rescale = std::sqrt((float)q.n / (float)n_subsample);
+    }
+    if (q.n > 16384) return fail(DKM_ERR_CAPACITY, "transform length above 16384 is not supported");
+    q.split = 1;
+    while (q.split * 2 <= q.n) q.split *= 2;
+    q.mean_norm = mean_norm;
+    q.pre_scale = pre_scale;
+    q.cstride = dkm::sph_bit_stride(q.out_dim);
+    HIP_TRY(hipSetDevice(g_device));
+    auto h = std::make_unique<dkm_sph>();
+    HIP_TRY(h->shift.ensure(dim));
+    HIP_TRY(hipMemcpy(h->shift.p, centroid, sizeof(float) * dim, hipMemcpyHostToDevice));
+    if (signs0 && transform != dkm::SPH_NULL) {
+        std::vector<uint32_t> s0((size_t)dim);
+        for (int64_t i = 0; i < dim; ++i) s0[(size_t)i] = signs0[i] ? 0x80000000u : 0u;
+        HIP_TRY(h->signs0.ensure(dim));
+        HIP_TRY(hipMemcpy(h->signs0.p, s0.data(), sizeof(uint32_t) * dim, hipMemcpyHostToDevice));
+    }
+    if (transform == dkm::SPH_DOUBLE_HADAMARD) {
+        std::vector<uint32_t> s1((size_t)q.n);
+        for (int i = 0; i < q.n; ++i) s1[(size_t)i] = signs1[i] ? 0x80000000u : 0u;
+        HIP_TRY(h->signs1.ensure(q.n));
+        HIP_TRY(hipMemcpy(h->signs1.p, s1.data(), sizeof(uint32_t) * q.n, hipMemcpyHostToDevice));
+    }
+    if (subsample) {
+        HIP_TRY(h->sub.ensure(n_subsample));
+        HIP_TRY(hipMemcpy(h->sub.p, subsample, sizeof(uint32_t) * n_subsample, hipMemcpyHostToDevice));
+    }
+    q.shift = h->shift.p;
+    q.signs0 = h->signs0.p;
+    q.signs1 = h->signs1.p;
+    q.subsample = subsample ? h->sub.p : nullptr;
+    DevBuf<float> dss;
+    HIP_TRY(dss.ensure(1));
+    HIP_TRY(dkm::launch_sph_shift_sq(q.shift, q.dim, dss.p, nullptr));
+    HIP_TRY(hipMemcpy(&q.shift_sq, dss.p, sizeof(float), hipMemcpyDeviceToHost));
+    h->q = q;
+    *out = h.release();
+    return DKM_OK;
+}
+
+void dkm_sph_destroy(dkm_sph *q) {
+    if (!q) return;
+    (void)hipSetDevice(g_device);
+    delete q;
+}
+
+dkm_status dkm_sph_dims(const dkm_sph *q, int64_t *out_dim, int64_t *code_bytes) {
+    if (!q || !out_dim || !code_bytes) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    *out_dim = q->q.out_dim;
+    *code_bytes = dkm::sph_canon_bytes(q->q.out_dim);
+    return DKM_OK;
+}
+
+dkm_status dkm_sph_train_stats(int metric, const float *data, int64_t n, int64_t dim,
+                               float *out_centroid, float *out_mean_norm) {
+    const int m = sph_metric(metric);
+    if (m < 0) return fail(DKM_ERR_INVALID_ARG, "spherical quantization supports L2, inner product and cosine");
+    if (!data || !out_centroid || !out_mean_norm || n <= 0 || dim <= 0)
+        return fail(DKM_ERR_INVALID_ARG, "training data cannot be empty");
+    HIP_TRY(hipSetDevice(g_device));
+    DevBuf<float> dx, dc;
+    DevBuf<double> dm, dn;
+    HIP_TRY(dx.ensure(n * dim));
+    HIP_TRY(dc.ensure(dim + 1));
+    HIP_TRY(dm.ensure(dim));
+    HIP_TRY(dn.ensure(n));
+    HIP_TRY(hipMemcpy(dx.p, data, sizeof(float) * n * dim, hipMemcpyHostToDevice));
+    HIP_TRY(dkm::launch_sph_train_stats(m, dx.p, n, (int)dim, dm.p, dn.p, dc.p, dc.p + dim, nullptr));
+    HIP_TRY(hipMemcpy(out_centroid, dc.p, sizeof(float) * dim, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(out_mean_norm, dc.p + dim, sizeof(float), hipMemcpyDeviceToHost));
+    if (!(*out_mean_norm > 0.0f) || !std::isfinite(*out_mean_norm))
+        return fail(DKM_ERR_INVALID_ARG, "the mean norm is not positive and finite");
+    return DKM_OK;
+}
+
+dkm_status dkm_sph_compress(const dkm_sph *q, const float *x, int64_t n, uint8_t *out) {
+    if (!q || n < 0 || (n > 0 && (!x || !out))) return fail(DKM_ERR_INVALID_ARG, "bad arguments");
+    if (n == 0) return DKM_OK;
+    HIP_TRY(hipSetDevice(g_device));
+    const int64_t cb = dkm::sph_canon_bytes(q->q.out_dim);
+    DevBuf<float> dx;
+    DevBuf<uint8_t> dout;
+    DevBuf<uint32_t> derr;
+    HIP_TRY(dx.ensure(n * q->q.dim));
+    HIP_TRY(dout.ensure(n * cb));
+    HIP_TRY(derr.ensure(1));
+    HIP_TRY(hipMemcpy(dx.p, x, sizeof(float) * n * q->q.dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(derr.p, 0, sizeof(uint32_t)));
+    HIP_TRY(dkm::launch_sph_encode(q->q, dx.p, q->q.dim, n, nullptr, nullptr, dout.p, derr.p, nullptr));
+    HIP_TRY(hipMemcpy(out, dout.p, (size_t)(n * cb), hipMemcpyDeviceToHost));
+    uint32_t e = 0;
+    HIP_TRY(hipMemcpy(&e, derr.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return sph_errors(e);
+}
+
+dkm_status dkm_sph_distance_batch(const dkm_sph *q, const uint8_t *left, int64_t nl,
+                                  const uint8_t *right, int64_t nr, const uint32_t *ids, int64_t m,
+                                  float *out) {
+    if (!q || nl < 0 || nr <= 0 || m < 0 || !left || !right || !out || (nl * m > 0 && !ids))
+        return fail(DKM_ERR_INVALID_ARG, "bad sizes");
+    if (nl == 0 || m == 0) return DKM_OK;
+    for (int64_t i = 0; i < nl * m; ++i)
+        if (ids[i] >= (uint64_t)nr) return fail(DKM_ERR_INDEX, "id %u out of range (%lld rows)", ids[i], (long long)nr);
+    HIP_TRY(hipSetDevice(g_device));
+    DevBuf<uint8_t> lb, rb;
+    DevBuf<dkm::SphMeta> lm, rm;
+    DevBuf<uint32_t> di;
+    DevBuf<float> dout;
+    dkm_status st = sph_upload(q->q, left, nl, lb, lm);
+    if (st) return st;
+    if ((st = sph_upload(q->q, right, nr, rb, rm))) return st;
+    HIP_TRY(di.ensure(nl * m));
+    HIP_TRY(dout.ensure(nl * m));
+    HIP_TRY(hipMemcpy(di.p, ids, sizeof(uint32_t) * nl * m, hipMemcpyHostToDevice));
+    HIP_TRY(dkm::launch_sph_pairs(q->q, lb.p, lm.p, rb.p, rm.p, di.p, nl, m, dout.p, nullptr));
+    HIP_TRY(hipMemcpy(out, dout.p, sizeof(float) * nl * m, hipMemcpyDeviceToHost));
+    return DKM_OK;
+}
+
+dkm_status dkm_sph_query_distance_batch(const dkm_sph *q, int layout, int allow_rescale,
+                                        const float *queries, int64_t nq, const uint8_t *codes,
+                                        int64_t n, const uint32_t *ids, int64_t m, float *out) {
+    if (!q || nq < 0 || n <= 0 || m < 0 || !queries || !codes || !out || (nq * m > 0 && !ids))
+        return fail(DKM_ERR_INVALID_ARG, "bad sizes");
+    if (layout != dkm::SPH_QUERY_SAME_AS_DATA && layout != dkm::SPH_QUERY_FOUR_BIT &&
+        layout != dkm::SPH_QUERY_FULL)
+        return fail(DKM_ERR_INVALID_ARG, "unknown query layout %d", layout);
+    if (nq == 0 || m == 0) return DKM_OK;
+    for (int64_t i = 0; i < nq * m; ++i)
+        if (ids[i] >= (uint64_t)n) return fail(DKM_ERR_INDEX, "id %u out of range (%lld rows)", ids[i], (long long)n);
+    HIP_TRY(hipSetDevice(g_device));
+    DevBuf<uint8_t> b;
+    DevBuf<dkm::SphMeta> mt;
+    DevBuf<uint32_t> di, derr;
+    DevBuf<float> dq, dout;
+    dkm_status st = sph_upload(q->q, codes, n, b, mt);
+    if (st) return st;
+    HIP_TRY(dq.ensure(nq * q->q.dim));
+    HIP_TRY(di.ensure(nq * m));
+    HIP_TRY(dout.ensure(nq * m));
+    HIP_TRY(derr.ensure(1));
+    HIP_TRY(hipMemcpy(dq.p, queries, sizeof(float) * nq * q->q.dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(di.p, ids, sizeof(uint32_t) * nq * m, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(derr.p, 0, sizeof(uint32_t)));
+    HIP_TRY(dkm::launch_sph_query_pairs(q->q, layout, allow_rescale, dq.p, nq, b.p, mt.p, di.p, m,
+                                        dout.p, derr.p, nullptr));
+    HIP_TRY(hipMemcpy(out, dout.p, sizeof(float) * nq * m, hipMemcpyDeviceToHost));
+    uint32_t e = 0;
+    HIP_TRY(hipMemcpy(&e, derr.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return sph_errors(e);
+}
+
+dkm_status dkm_set_spherical(dkm_index *ix, const dkm_sph *q, int query_layout) {
+    if (!ix || !q) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (q->q.dim != ix->dim) return fail(DKM_ERR_DIM_MISMATCH, "quantizer dim %d != index dim %lld", q->q.dim, (long long)ix->dim);
+    if (sph_metric(ix->metric) != q->q.metric)
+        return fail(DKM_ERR_INVALID_ARG, "quantizer metric does not match the index metric");
+    if (query_layout != dkm::SPH_QUERY_SAME_AS_DATA && query_layout != dkm::SPH_QUERY_FOUR_BIT)
+        return fail(DKM_ERR_INVALID_ARG, "traversal supports the same-as-data and 4-bit query layouts");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    dkm::SphParams p = q->q;
+    HIP_TRY(ix->d_sph_shift.ensure(p.dim));
+    HIP_TRY(hipMemcpy(ix->d_sph_shift.p, p.shift, sizeof(float) * p.dim, hipMemcpyDeviceToDevice));
+    p.shift = ix->d_sph_shift.p;
+    if (p.signs0) {
+        HIP_TRY(ix->d_sph_signs0.ensure(p.dim));
+        HIP_TRY(hipMemcpy(ix->d_sph_signs0.p, p.signs0, sizeof(uint32_t) * p.dim, hipMemcpyDeviceToDevice));
+        p.signs0 = ix->d_sph_signs0.p;
+    }
+    if (p.signs1) {
+        HIP_TRY(ix->d_sph_signs1.ensure(p.n));
+        HIP_TRY(hipMemcpy(ix->d_sph_signs1.p, p.signs1, sizeof(uint32_t) * p.n, hipMemcpyDeviceToDevice));
+        p.signs1 = ix->d_sph_signs1.p;
+    }
+    if (p.subsample) {
+        HIP_TRY(ix->d_sph_sub.ensure(p.out_dim));
+        HIP_TRY(hipMemcpy(ix->d_sph_sub.p, p.subsample, sizeof(uint32_t) * p.out_dim, hipMemcpyDeviceToDevice));
+        p.subsample = ix->d_sph_sub.p;
+    }
+    HIP_TRY(ix->d_sph_bits.ensure(ix->rows * p.cstride + 16));
+    HIP_TRY(ix->d_sph_meta.ensure(ix->rows));
+    HIP_TRY(hipMemset(ix->d_sph_bits.p, 0, ix->rows * p.cstride + 16));
+    HIP_TRY(hipMemset(ix->d_sph_meta.p, 0, sizeof(dkm::SphMeta) * ix->rows));
+    ix->sphq = p;
+    ix->sph_layout = query_layout;
+    ix->sph_active = true;
+    ix->sph_codes_set = false;
+    ix->sq_active = false;
+    return DKM_OK;
+}
+
+dkm_status dkm_encode_index_sph(dkm_index *ix) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    if (!ix->sph_active) return fail(DKM_ERR_NOT_READY, "set the spherical quantizer first");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    DevBuf<uint32_t> derr;
+    HIP_TRY(derr.ensure(1));
+    HIP_TRY(hipMemsetAsync(derr.p, 0, sizeof(uint32_t), ix->stream));
+    HIP_TRY(dkm::launch_sph_encode(ix->sphq, ix->d_vec, ix->vstride, ix->rows, ix->d_sph_bits.p,
+                                   ix->d_sph_meta.p, nullptr, derr.p, ix->stream));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    uint32_t e = 0;
+    HIP_TRY(hipMemcpy(&e, derr.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    dkm_status st = sph_errors(e);
+    if (st) return st;
+    ix->sph_codes_set = true;
+    return DKM_OK;
+}
+
+dkm_status dkm_set_sph_codes(dkm_index *ix, const uint8_t *codes, int64_t row0, int64_t n) {
+    if (!ix || (!codes && n > 0)) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (!ix->sph_active) return fail(DKM_ERR_NOT_READY, "set the spherical quantizer first");
+    if (row0 < 0 || n < 0 || row0 + n > ix->rows) return fail(DKM_ERR_INDEX, "rows out of range");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    if (n) {
+        DevBuf<uint8_t> b;
+        DevBuf<dkm::SphMeta> mt;
+        dkm_status st = sph_upload(ix->sphq, codes, n, b, mt);
+        if (st) return st;
+        HIP_TRY(hipMemcpy(ix->d_sph_bits.p + row0 * ix->sphq.cstride, b.p, n * ix->sphq.cstride,
+                          hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(ix->d_sph_meta.p + row0, mt.p, sizeof(dkm::SphMeta) * n, hipMemcpyDeviceToDevice));
+    }
+    ix->sph_codes_set = true;
+    return DKM_OK;
+}
+
+dkm_status dkm_get_sph_codes(dkm_index *ix, uint8_t *codes, int64_t row0, int64_t n) {
+    if (!ix || !ix->sph_active) return fail(DKM_ERR_NOT_READY, "no spherical codes");
+    if (row0 < 0 || n < 0 || row0 + n > ix->rows) return fail(DKM_ERR_INDEX, "rows out of range");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    if (n) return sph_download(ix->sphq, ix->d_sph_bits.p + row0 * ix->sphq.cstride,
+                               ix->d_sph_meta.p + row0, n, codes);
     return DKM_OK;
 }
 
@@ -858,8 +1257,13 @@ static dkm_status ensure_spill(dkm_index *ix, int64_t nq, uint64_t slots) {
     return DKM_OK;
 }
 
-// Scalar-quantized traversal fields of the search parameters.
-static void set_sq_params(const dkm_index *ix, bool sq, dkm::SearchParams &p) {
+// Scalar-quantized and spherical traversal fields of the search parameters.
+static void set_sq_params(const dkm_index *ix, bool sq, dkm::SearchParams &p, bool sph = false) {
+    p.sph = sph;
+    p.sph_layout = ix->sph_layout;
+    p.sphq = ix->sphq;
+    p.sph_bits = ix->d_sph_bits.p;
+    p.sph_meta = ix->d_sph_meta.p;
     p.sq = sq;
     p.sq_codes = ix->d_sq_codes.p;
     p.sq_comp = ix->d_sq_comp.p;
@@ -885,8 +1289,11 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     if ((int64_t)max_k > k_stride) return fail(DKM_ERR_INVALID_ARG, "k exceeds output stride");
     if (!ix->starts_set) return fail(DKM_ERR_NOT_READY, "start points not set");
     const bool quant = strategy != DKM_STRATEGY_FULL;
+    const bool sph = quant && ix->sph_active;
     const bool sq = quant && ix->sq_active;
-    const bool pq = quant && !sq;
+    const bool pq = quant && !sq && !sph;
+    if (sph && !ix->sph_codes_set)
+        return fail(DKM_ERR_NOT_READY, "spherical strategy needs the codes of every row");
     if (sq && !ix->sq_codes_set)
         return fail(DKM_ERR_NOT_READY, "scalar-quantized strategy needs the codes of every row");
     if (pq && (!ix->d_codes || !ix->codes_set))
@@ -916,7 +1323,7 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     p.M = (int)ix->M;
     p.pivots = ix->d_pivots;
     p.offsets = ix->d_offsets;
-    set_sq_params(ix, sq, p);
+    set_sq_params(ix, sq, p, sph);
     p.queries = d_queries;
     p.qstride = ix->dim;
     p.nq = (int)nq;
@@ -1464,8 +1871,11 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
         return fail(DKM_ERR_INVALID_ARG, "bad capacity / slots");
     if (!ix->starts_set) return fail(DKM_ERR_NOT_READY, "start points not set");
     const bool quant = strategy != DKM_STRATEGY_FULL;
+    const bool sph = quant && ix->sph_active;
     const bool sq = quant && ix->sq_active;
-    const bool pq = quant && !sq;
+    const bool pq = quant && !sq && !sph;
+    if (sph && !ix->sph_codes_set)
+        return fail(DKM_ERR_NOT_READY, "spherical strategy needs the codes of every row");
     if (sq && !ix->sq_codes_set)
         return fail(DKM_ERR_NOT_READY, "scalar-quantized strategy needs the codes of every row");
     if (pq && (!ix->d_codes || !ix->codes_set))
@@ -1492,7 +1902,7 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
     p.M = (int)ix->M;
     p.pivots = ix->d_pivots;
     p.offsets = ix->d_offsets;
-    set_sq_params(ix, sq, p);
+    set_sq_params(ix, sq, p, sph);
     p.qstride = ix->dim;
     p.beam_width = beam_width;
     p.k_stride = max_k;

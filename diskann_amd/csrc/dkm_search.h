@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dkm_sph.h"
+
 namespace dkm {
 
 enum : uint32_t { ERR_VISITED_OVERFLOW = 1u, ERR_ITER_LIMIT = 2u };
@@ -57,6 +59,12 @@ struct SearchParams {
     const float *sq_shift;    // dim
     int sq_cstride;           // multiple of 16
     float sq_scale, sq_bss, sq_ssn, sq_mean_norm;
+    // 1-bit spherical quantization (traverse with it when sph is set; pq and sq are 0)
+    int sph;
+    int sph_layout;           // SPH_QUERY_SAME_AS_DATA or SPH_QUERY_FOUR_BIT
+    SphParams sphq;
+    const uint8_t *sph_bits;  // rows x sphq.cstride
+    const SphMeta *sph_meta;  // rows
 };
 
 size_t search_lds_bytes(const SearchParams &p);

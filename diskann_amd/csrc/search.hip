@@ -38,6 +38,9 @@ struct Lds {
     int32_t *misc;
     uint8_t *sqc;    // scalar-quantized query code (sq_cstride bytes)
     uint32_t *sqm;   // [0] query compensation (f32 bits), [1] sum of squared query codes
+    float *sphw;     // spherical: query preparation area (sph_query_lds_floats)
+    uint8_t *sphb;   // spherical: query bit row / 4 bit planes
+    float *sphm;     // spherical: query metadata (4 floats)
 };
 
 // Expansion lists longer than this are ranked through a separate sorted buffer.
@@ -67,6 +70,9 @@ __host__ __device__ inline size_t lds_bytes(const SearchParams &p) {
     b += align16(sizeof(uint32_t) * rows_entries(p));
     b += 16;
     if (p.sq) b += align16(p.sq_cstride) + 16;
+    if (p.sph)
+        b += align16(sizeof(float) * sph_query_lds_floats(p.sphq)) +
+             align16(sph_query_code_bytes(p.sphq)) + 16;
     return b;
 }
 
@@ -99,6 +105,9 @@ __device__ inline Lds carve(const SearchParams &p, char *base) {
     s.misc = (int32_t *)take(16);
     s.sqc = p.sq ? (uint8_t *)take(p.sq_cstride) : nullptr;
     s.sqm = p.sq ? (uint32_t *)take(16) : nullptr;
+    s.sphw = p.sph ? (float *)take(sizeof(float) * sph_query_lds_floats(p.sphq)) : nullptr;
+    s.sphb = p.sph ? (uint8_t *)take(sph_query_code_bytes(p.sphq)) : nullptr;
+    s.sphm = p.sph ? (float *)take(16) : nullptr;
     return s;
 }
 
@@ -788,8 +797,74 @@ __device__ inline void sq_query(const SearchParams &p, Lds &s) {
     __syncthreads();
 }
 
+// Spherical 1-bit distances (SphericalStore's query computer, diskann-providers/src/model/
+// graph/provider/async_/inmem/spherical.rs:360-380): a group of G lanes per candidate,
+// lane t of the group takes the 16-byte chunks t, t + G, ... of the candidate's bit row;
+// the bit-level inner product is an exact popcount sum (same-as-data: q & b; 4-bit:
+// sum_k 2^k popcount(plane_k & b)), finished with the reference's f32 formulas.
+template <int LAYOUT, int G>
+__device__ inline void sph_distances(const SearchParams &p, Lds &s, const uint32_t *ids,
+                                     float *out, int n) {
+    constexpr int NG = kWave / G;
+    constexpr int U = 2;
+    const SphParams &q = p.sphq;
+    const int lane = lane_id();
+    const int g = lane % G, grp = lane / G;
+    const int chunks = q.cstride / 16;
+    const float dim = (float)q.out_dim;
+    for (int c0 = 0; c0 < n; c0 += NG * U) {
+        uint32_t ip[U], id[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = c0 + grp + NG * u;
+            id[u] = ids[c < n ? c : 0];
+            ip[u] = 0;
+        }
+        for (int t = g; t < chunks; t += G) {
+            uint4 b[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                b[u] = *(const uint4 *)(p.sph_bits + (size_t)id[u] * q.cstride + 16 * t);
+            if constexpr (LAYOUT == SPH_QUERY_SAME_AS_DATA) {
+                const uint4 a = *(const uint4 *)(s.sphb + 16 * t);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    ip[u] += __builtin_popcount(a.x & b[u].x) + __builtin_popcount(a.y & b[u].y) +
+                             __builtin_popcount(a.z & b[u].z) + __builtin_popcount(a.w & b[u].w);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint4 a = *(const uint4 *)(s.sphb + k * q.cstride + 16 * t);
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        ip[u] += (uint32_t)(__builtin_popcount(a.x & b[u].x) +
+                                            __builtin_popcount(a.y & b[u].y) +
+                                            __builtin_popcount(a.z & b[u].z) +
+                                            __builtin_popcount(a.w & b[u].w)) << k;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int o = G / 2; o > 0; o >>= 1) ip[u] += __shfl_xor(ip[u], o, G);
+            const int c = c0 + grp + NG * u;
+            if (c < n && g == 0) {
+                const SphMeta y = p.sph_meta[id[u]];
+                if constexpr (LAYOUT == SPH_QUERY_SAME_AS_DATA) {
+                    const SphMeta x = {s.sphm[0], s.sphm[1], s.sphm[2], 0.0f};
+                    out[c] = sph_data_data(q.metric, q.shift_sq, dim, ip[u], x, y);
+                } else {
+                    out[c] = sph_query4_data(q.metric, q.shift_sq, dim, ip[u], s.sphm, y);
+                }
+            }
+        }
+    }
+}
+
 // MODE 0: full precision; 1: PQ with the LDS table; 2: PQ direct (PQLEN = chunk width);
-// 3: 8-bit scalar quantization (PQLEN = lanes per candidate).
+// 3: 8-bit scalar quantization (PQLEN = lanes per candidate); 4 / 5: 1-bit spherical with
+// the same-as-data / 4-bit query (PQLEN = lanes per candidate).
 template <int METRIC, int MODE, int DIM, int PQLEN>
 __device__ inline void compute_distances(const SearchParams &p, Lds &s, int n) {
     if constexpr (MODE == 0) {
@@ -798,8 +873,12 @@ __device__ inline void compute_distances(const SearchParams &p, Lds &s, int n) {
         adc_lut(p, s, s.nl_i, s.nl_d, n);
     } else if constexpr (MODE == 2) {
         adc_direct<METRIC == METRIC_IP, PQLEN>(p, s, s.nl_i, s.nl_d, n);
-    } else {
+    } else if constexpr (MODE == 3) {
         sq_distances<METRIC, PQLEN>(p, s, s.nl_i, s.nl_d, n);
+    } else if constexpr (MODE == 4) {
+        sph_distances<SPH_QUERY_SAME_AS_DATA, PQLEN>(p, s, s.nl_i, s.nl_d, n);
+    } else {
+        sph_distances<SPH_QUERY_FOUR_BIT, PQLEN>(p, s, s.nl_i, s.nl_d, n);
     }
     __syncthreads();
 }
@@ -841,6 +920,13 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
     }
     __syncthreads();
     if constexpr (MODE == 3) sq_query<METRIC>(p, s);
+    if constexpr (MODE == 4 || MODE == 5) {
+        // the reference's search accessor asks for the inner-product rescale
+        uint32_t e = 0;
+        sph_query_code(p.sphq, MODE == 4 ? SPH_QUERY_SAME_AS_DATA : SPH_QUERY_FOUR_BIT, true, s.q,
+                       s.sphw, s.sphb, s.sphm, e);
+        (void)e;
+    }
     if (MODE == 1) {
         const bool ip = METRIC == METRIC_IP;
         for (int e = lane; e < p.M * 256; e += kWave) {
@@ -1088,7 +1174,20 @@ static hipError_t dispatch_sq(const SearchParams &p, hipStream_t stream, int *re
     return launch_t<METRIC, 3, 0, 64>(p, stream, resident);
 }
 
+template <int MODE>
+static hipError_t dispatch_sph(const SearchParams &p, hipStream_t stream, int *resident) {
+    const int chunks = p.sphq.cstride / 16;
+    if (chunks <= 2) return launch_t<METRIC_L2, MODE, 0, 1>(p, stream, resident);
+    if (chunks <= 8) return launch_t<METRIC_L2, MODE, 0, 4>(p, stream, resident);
+    return launch_t<METRIC_L2, MODE, 0, 16>(p, stream, resident);
+}
+
 static hipError_t dispatch(const SearchParams &p, hipStream_t stream, int *resident) {
+    if (p.sph) {
+        // the metric is the quantizer's (run time); the FP32 rerank uses p.metric
+        if (p.sph_layout == SPH_QUERY_SAME_AS_DATA) return dispatch_sph<4>(p, stream, resident);
+        return dispatch_sph<5>(p, stream, resident);
+    }
     if (p.sq) {
         if (p.metric == METRIC_IP) return dispatch_sq<METRIC_IP>(p, stream, resident);
         if (p.metric == METRIC_COSNORM) return dispatch_sq<METRIC_COSNORM>(p, stream, resident);

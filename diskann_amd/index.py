@@ -167,6 +167,28 @@ class DiskANNIndex:
         check(self.lib.dkm_get_sq_codes(self._h, _p(out), row0, n), self.lib)
         return out
 
+    def set_spherical(self, quantizer, layout="four_bit_transposed"):
+        """1-bit spherical store (SphericalStore, diskann-providers/.../inmem/spherical.rs):
+        the Hybrid / Quantized strategies traverse with 1-bit codes and the query layout
+        `same_as_data` or `four_bit_transposed` (call encode_spherical or set_sph_codes
+        next)."""
+        from .spherical import LAYOUTS
+        check(self.lib.dkm_set_spherical(self._h, quantizer._h, LAYOUTS[layout]), self.lib)
+        self._sph_bytes = quantizer.code_bytes
+
+    def encode_spherical(self):
+        check(self.lib.dkm_encode_index_sph(self._h), self.lib)
+
+    def set_sph_codes(self, compressed, row0=0):
+        c = np.ascontiguousarray(np.atleast_2d(compressed), dtype=np.uint8)
+        check(self.lib.dkm_set_sph_codes(self._h, _p(c), row0, c.shape[0]), self.lib)
+
+    def get_sph_codes(self, row0=0, n=None):
+        n = self.max_points + self.num_start - row0 if n is None else n
+        out = np.zeros((n, self._sph_bytes), dtype=np.uint8)
+        check(self.lib.dkm_get_sph_codes(self._h, _p(out), row0, n), self.lib)
+        return out
+
     def use_pq(self):
         check(self.lib.dkm_use_pq(self._h), self.lib)
 

@@ -212,6 +212,58 @@ dkm_status dkm_set_sq_codes(dkm_index *ix, const uint8_t *codes, int64_t row0, i
 dkm_status dkm_get_sq_codes(dkm_index *ix, uint8_t *codes, int64_t row0, int64_t n);
 dkm_status dkm_use_pq(dkm_index *ix);
 
+/* 1-bit spherical quantization (diskann-quantization/src/spherical). A quantizer is the
+ * reference's SphericalQuantizer (quantizer.rs:36-78; serialized form
+ * schemas/spherical_v001.fbs): metric (DKM_L2, DKM_INNER_PRODUCT or DKM_COSINE), centroid
+ * (dim floats), mean_norm, pre_scale and a transform: 0 = NullTransform, 1 =
+ * PaddingHadamard (signs0: dim flags, padded_dim a power of two >= dim), 2 =
+ * DoubleHadamard (signs0: dim flags, signs1: max(dim, output dim) flags); flags are 1 for
+ * a flipped sign, as in the serialized bool vectors; subsample (optional, strictly
+ * increasing) selects the output coordinates. A compressed vector is the canonical row
+ * of the reference's Impl<1>::compress (iface.rs:1955-1966, DataMut::from_canonical_back):
+ * ceil(out_dim / 8) bytes of bits (little-endian within each byte), then f16 inner-product
+ * correction, f16 metric term and u16 bit count.
+ * dkm_sph_train_stats: the deterministic part of SphericalQuantizer::train
+ *   (quantizer.rs:225-320): centroid (column means in row order; cosine: of the
+ *   unit-normalised rows) and mean norm (1 for cosine); the transform's random signs
+ *   are the caller's.
+ * dkm_sph_compress: CompressIntoWith<DataMut<1>> (quantizer.rs:591-642, 779-857).
+ * dkm_sph_distance_batch: the quantizer's distance_computer on two compressed rows
+ *   (CompensatedSquaredL2 / CompensatedIP / CompensatedCosine, vectors.rs:476-897).
+ * dkm_sph_query_distance_batch: fused_query_computer (iface.rs:1898-1950) for layout 0 =
+ *   SameAsData, 1 = FourBitTransposed, 2 = FullPrecision, with allow_rescale. */
+typedef struct dkm_sph dkm_sph;
+dkm_status dkm_sph_create(int metric, int64_t dim, const float *centroid, float mean_norm,
+                          float pre_scale, int transform, const uint8_t *signs0,
+                          const uint8_t *signs1, int64_t n_signs1, int64_t padded_dim,
+                          const uint32_t *subsample, int64_t n_subsample, dkm_sph **out);
+void dkm_sph_destroy(dkm_sph *q);
+dkm_status dkm_sph_dims(const dkm_sph *q, int64_t *out_dim, int64_t *code_bytes);
+dkm_status dkm_sph_train_stats(int metric, const float *data, int64_t n, int64_t dim,
+                               float *out_centroid, float *out_mean_norm);
+dkm_status dkm_sph_compress(const dkm_sph *q, const float *x, int64_t n, uint8_t *out);
+dkm_status dkm_sph_distance_batch(const dkm_sph *q, const uint8_t *left, int64_t nl,
+                                  const uint8_t *right, int64_t nr, const uint32_t *ids,
+                                  int64_t m, float *out);
+dkm_status dkm_sph_query_distance_batch(const dkm_sph *q, int layout, int allow_rescale,
+                                        const float *queries, int64_t nq, const uint8_t *codes,
+                                        int64_t n, const uint32_t *ids, int64_t m, float *out);
+
+/* Spherical store of an index (SphericalStore, diskann-providers/src/model/graph/provider/
+ * async_/inmem/spherical.rs:83-240). After dkm_set_spherical the quantized strategies
+ * traverse with 1-bit codes and the given query layout (0 = SameAsData, 1 =
+ * FourBitTransposed), rescaling inner-product queries as the reference's search
+ * strategy does (Quantized::search, spherical.rs:505-531): DKM_STRATEGY_HYBRID = search
+ * over a full-precision provider (traversal, FilterStartPoints -> Rerank),
+ * DKM_STRATEGY_QUANTIZED = without one (1-bit distances returned). The quantizer's metric
+ * must be the index metric. dkm_encode_index_sph compresses every stored row;
+ * dkm_set_sph_codes / dkm_get_sph_codes move canonical rows; dkm_use_pq switches the
+ * quantized strategies back to PQ. */
+dkm_status dkm_set_spherical(dkm_index *ix, const dkm_sph *q, int query_layout);
+dkm_status dkm_encode_index_sph(dkm_index *ix);
+dkm_status dkm_set_sph_codes(dkm_index *ix, const uint8_t *codes, int64_t row0, int64_t n);
+dkm_status dkm_get_sph_codes(dkm_index *ix, uint8_t *codes, int64_t row0, int64_t n);
+
 /* Hybrid prune for dkm_build (common::Hybrid, diskann-providers/src/model/graph/provider/
  * async_/common.rs:385-405; HybridPruneAccessor, inmem/product.rs:143-230): each prune
  * keeps the first max_fp_vecs_per_prune ids of its fill order (robust_prune_with:

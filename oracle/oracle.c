@@ -499,7 +499,8 @@ static void scratch_init(scratch *s, const dko_index *ix, int64_t L, int64_t bw)
     s->nb_cap = bw * ix->max_degree + 1;
     s->nb_id = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)s->nb_cap);
     s->nb_d = (float *)malloc(sizeof(float) * (size_t)s->nb_cap);
-    s->lut = ix->sq   ? (float *)malloc((size_t)ix->sq->dim + 2 * sizeof(float))
+    s->lut = ix->sph  ? (float *)malloc((size_t)ix->sph->out_dim + 64)
+             : ix->sq ? (float *)malloc((size_t)ix->sq->dim + 2 * sizeof(float))
              : ix->M > 0 ? (float *)malloc(sizeof(float) * (size_t)(ix->M * ix->K))
                          : NULL;
 }
@@ -1025,6 +1026,14 @@ float dko_sph_dist_full(const dko_sph *q, const float *v, const float *meta, con
 
 static inline float node_dist(const dko_index *ix, int pq, const float *query,
                               const float *lut, uint32_t id) {
+    if (pq && ix->sph) {
+        /* query code staged behind the LUT pointer by search_core */
+        const dko_sph *q = ix->sph;
+        const int64_t cb = dko_sph_data_bytes(q);
+        const uint8_t *row = ix->sph_codes + (int64_t)id * cb;
+        if (ix->sph_layout == 0) return dko_sph_dist_data(q, (const uint8_t *)lut, row);
+        return dko_sph_dist_query4(q, (const uint8_t *)(lut + 4), lut, row);
+    }
     if (pq && ix->sq) {
         /* query code / compensation staged behind the LUT pointer by search_core */
         const dko_sq *q = ix->sq;
@@ -1049,7 +1058,20 @@ static void search_core(const dko_index *ix, int pq, const float *query, int64_t
     q->cursor = 0;
     set_clear(&s->vis);
     uint32_t cmps = 0, hops = 0, nrec = 0;
-    if (pq && ix->sq) {
+    if (pq && ix->sph) {
+        /* SphericalStore::query_computer with the search strategy's allow_rescale
+         * (diskann-providers/.../inmem/spherical.rs:211-232, 505-531): inner-product
+         * queries are rescaled to the mean norm, then compressed in the layout */
+        const dko_sph *q = ix->sph;
+        float *tmp = (float *)malloc(sizeof(float) * (size_t)q->dim);
+        memcpy(tmp, query, sizeof(float) * (size_t)q->dim);
+        if (q->metric == DKO_INNER_PRODUCT) dko_sph_rescale(q, tmp);
+        if (ix->sph_layout == 0)
+            dko_sph_compress_data(q, tmp, (uint8_t *)s->lut);
+        else
+            dko_sph_compress_query4(q, tmp, (uint8_t *)(s->lut + 4), s->lut);
+        free(tmp);
+    } else if (pq && ix->sq) {
         /* SQStore::query_computer (diskann-providers/.../inmem/scalar.rs:227-253) with
          * allow_rescale: the inner-product query is rescaled to the mean norm, then
          * compressed; code and compensation go into the LUT buffer */
@@ -1169,7 +1191,7 @@ int dko_search(const dko_index *ix, int mode, const float *query, int64_t k, int
                int64_t bw, uint32_t *out_ids, float *out_d, dko_stats *st) {
     int rc = dko_knn_validate(k, L, bw);
     if (rc) return rc;
-    if (mode != DKO_MODE_FULL && ix->M <= 0 && !ix->sq) return DKO_ERR_ARG;
+    if (mode != DKO_MODE_FULL && ix->M <= 0 && !ix->sq && !ix->sph) return DKO_ERR_ARG;
     scratch s;
     scratch_init(&s, ix, L, bw);
     search_core(ix, mode != DKO_MODE_FULL, query, bw, &s, NULL, NULL, NULL, 0, st);

@@ -141,6 +141,11 @@ typedef struct {
     const int64_t *offsets;    /* [M+1] */
     /* scalar quantization (optional): when set, the quantized modes traverse with it */
     const dko_sq *sq;
+    /* 1-bit spherical quantization (optional, takes precedence): canonical code rows and
+     * the traversal query layout (0 same-as-data, 1 4-bit) */
+    const dko_sph *sph;
+    const uint8_t *sph_codes;
+    int sph_layout;
 } dko_index;
 
 typedef struct {

@@ -35,6 +35,7 @@ class Index(C.Structure):
         ("deg", C.c_void_p), ("nstart", C.c_int64), ("starts", C.c_void_p),
         ("M", C.c_int64), ("K", C.c_int64), ("codes", C.c_void_p),
         ("pivots", C.c_void_p), ("offsets", C.c_void_p), ("sq", C.c_void_p),
+        ("sph", C.c_void_p), ("sph_codes", C.c_void_p), ("sph_layout", C.c_int),
     ]
 
 
@@ -370,7 +371,7 @@ class OracleIndex:
     codes, comp) selects scalar-quantized traversal for the quantized modes."""
 
     def __init__(self, metric, vectors, adj, deg, starts, codes=None, pivots=None,
-                 offsets=None, sq=None):
+                 offsets=None, sq=None, sph=None):
         self.vectors = np.ascontiguousarray(vectors, dtype=np.float32)
         self.adj = np.ascontiguousarray(adj, dtype=np.uint32)
         self.deg = np.ascontiguousarray(deg, dtype=np.uint32)
@@ -403,6 +404,13 @@ class OracleIndex:
                          sq_shift_square_norm(self.sq_shift), mean_norm,
                          _p(self.sq_codes), _p(self.sq_comp))
             s.sq = C.cast(C.pointer(self.sq), C.c_void_p)
+        if sph is not None:
+            quant, sph_codes, layout = sph   # Spherical, canonical rows, 0 / 1
+            self.sph_q = quant
+            self.sph_codes = np.ascontiguousarray(sph_codes, dtype=np.uint8)
+            s.sph = C.cast(C.pointer(quant.s), C.c_void_p)
+            s.sph_codes = _p(self.sph_codes)
+            s.sph_layout = layout
         self.s = s
 
     def search(self, query, k, L, bw=1, mode=MODE_FULL):

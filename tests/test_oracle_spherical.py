@@ -166,3 +166,50 @@ def test_distances_track_full_precision(metric):
     est = np.array([sq.dist_query4(cq, mq, codes[j]) for j in range(1, 300)])
     ex = np.array([po.distance(metric, q, x[j]) for j in range(1, 300)])
     assert np.corrcoef(est, ex)[0, 1] > 0.6
+
+
+def sph_graph_index(metric, layout, n=300, dim=40, seed=3):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n + 1, dim)).astype(np.float32)
+    if metric != po.L2:
+        x /= np.linalg.norm(x, axis=1, keepdims=True)
+    x[n] = x[:n].mean(axis=0)
+    adj, deg = po.build_graph(metric, x, R=16, R_max=20, L_build=40, alpha=1.2,
+                              starts=[n], ids=np.arange(n))
+    c, mn = po.sph_train_stats(metric, x[:n])
+    sq = po.Spherical(metric, c, mn, 1.0, po.SPH_DOUBLE_HADAMARD,
+                      rng.integers(0, 2, dim).astype(bool), rng.integers(0, 2, dim).astype(bool))
+    codes = sq.compress(x)
+    ix = po.OracleIndex(metric, x, adj, deg, [n], sph=(sq, codes, layout))
+    return ix, x, sq, codes
+
+
+@pytest.mark.parametrize("metric", [po.L2, po.INNER_PRODUCT, po.COSINE])
+@pytest.mark.parametrize("layout", [0, 1])
+def test_traversal_exhaustive_is_bruteforce(metric, layout):
+    """With L covering the graph, the quantized search returns the brute-force top-k of
+    the layout's 1-bit distances (inner-product queries rescaled as the search strategy
+    asks)."""
+    ix, x, sq, codes = sph_graph_index(metric, layout)
+    n = x.shape[0] - 1
+    seen, todo = {n}, [n]
+    while todo:
+        u = todo.pop()
+        for v in ix.adj[u, :ix.deg[u]]:
+            if int(v) not in seen:
+                seen.add(int(v))
+                todo.append(int(v))
+    reach = np.array(sorted(seen - {n}))
+    rng = np.random.default_rng(11)
+    for _ in range(4):
+        q = rng.standard_normal(x.shape[1]).astype(np.float32)
+        ids, ds, cm, hp = ix.search(q, 10, n + 1, mode=po.MODE_PQ_ONLY)
+        qq = sq.rescale(q) if metric == po.INNER_PRODUCT else q
+        if layout == 0:
+            qc = sq.compress(qq)[0]
+            bd = np.array([sq.dist_data(qc, codes[r]) for r in reach], np.float32)
+        else:
+            c4, m4 = sq.compress_query4(qq)
+            bd = np.array([sq.dist_query4(c4, m4, codes[r]) for r in reach], np.float32)
+        order = np.argsort(bd, kind="stable")[:10]
+        assert np.array_equal(np.sort(ds), np.sort(bd[order]))

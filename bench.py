@@ -328,7 +328,8 @@ def main():
                          "(BF16 MFMA k-means, 1M x 768, K = 1024, 20 iterations), "
                          "prescore768 (exhaustive top-k with BF16 pre-scoring, 1M x 768) or "
                          "sq1m (batched search with 8-bit scalar-quantized traversal and "
-                         "FP32 rerank, same 1M x 128 index as search1m)")
+                         "FP32 rerank, same 1M x 128 index as search1m) or sph1m (1-bit "
+                         "spherical traversal with 4-bit queries and FP32 rerank, same index)")
     ap.add_argument("--fixed", default="",
                     help="L,beam_width,visited_slots: skip the L sweep and the autotuner")
     ap.add_argument("--timed-only", action="store_true",
@@ -346,10 +347,11 @@ def main():
         return kmeans_workload(args)
     if args.workload == "prescore768":
         return prescore_workload(args)
-    if args.workload not in ("search1m", "sq1m"):
+    if args.workload not in ("search1m", "sq1m", "sph1m"):
         print("unknown workload %s" % args.workload, file=sys.stderr)
         sys.exit(2)
     SQ = args.workload == "sq1m"
+    SPH = args.workload == "sph1m"
 
     import diskann_amd as dk
     from diskann_amd import quantization as qz
@@ -357,12 +359,21 @@ def main():
 
     N, D, NQ, K = args.n, 128, args.nq, 10
     R, R_MAX, L_BUILD, ALPHA, M = 64, 83, 100, 1.2, 32
+    METRIC, NGT = dk.Metric.L2, NQ
     t0 = time.time()
-    x, centers = gmm(N, D, 256, 0.5, 0xD15C0002)
-    qs, _ = gmm(NQ, D, 256, 0.5, 0xD15C1002, centers)
+    if SPH:
+        # config-5 shape at a size the bench builds in about a minute: unit-norm 1536-d
+        # mixture (4096 components), cosine, R = 96 (R_max 124); recall on 2,000 queries
+        N = args.n if args.n != 1_000_000 else 200_000
+        D, R, R_MAX, METRIC, NGT = 1536, 96, 124, dk.Metric.COSINE, min(NQ, 2000)
+        x, centers = unit_mixture(N, D, 4096, np.random.default_rng(0xD15C0005))
+        qs, _ = unit_mixture(NQ, D, 4096, np.random.default_rng(0xD15C1005), centers)
+    else:
+        x, centers = gmm(N, D, 256, 0.5, 0xD15C0002)
+        qs, _ = gmm(NQ, D, 256, 0.5, 0xD15C1002, centers)
     log("data %.1fs" % (time.time() - t0))
 
-    ix = dk.DiskANNIndex(dk.Metric.L2, D, N, R, R_MAX)
+    ix = dk.DiskANNIndex(METRIC, D, N, R, R_MAX)
     lib = ix.lib
     ix.set_vectors(x)
     row, med = ix.medoid()
@@ -373,6 +384,12 @@ def main():
         sqz = dk.scalar.ScalarQuantizer.train(x, 2.0)
         ix.set_sq(sqz)
         ix.encode_sq()
+    elif SPH:
+        # 1 bit, PaddingHadamard to the same dimension (the reference's benchmark example)
+        sph = dk.spherical.SphericalQuantizer.train(x, "cosine", "padding_hadamard",
+                                                    seed=0xD15C4005)
+        ix.set_spherical(sph, "four_bit_transposed")
+        ix.encode_spherical()
     else:
         sample = x[np.random.default_rng(0xD15C2002).choice(N, 100_000, replace=False)]
         pivots, offsets = qz.train_pq(sample, M, 256, 10, seed=0xD15C3002)
@@ -382,8 +399,9 @@ def main():
     t0 = time.time()
     ix.build(np.arange(N), batch=1024, intra_batch=0, L_build=L_BUILD, alpha=ALPHA)
     t_build = time.time() - t0
-    log("%s train+encode %.2fs, graph build %.2fs" % ("sq" if SQ else "pq", t_pq, t_build))
-    gt, _ = ix.bruteforce(qs, K)
+    log("%s train+encode %.2fs, graph build %.2fs" % ("sq" if SQ else "sph" if SPH else "pq",
+                                                      t_pq, t_build))
+    gt, _ = ix.bruteforce(qs[:NGT], K)
 
     # resident inputs / outputs
     dq = DeviceBuf(lib, qs.astype(np.float32))
@@ -427,7 +445,7 @@ def main():
             set_L(L)
             launch(L)
             sync()
-            rec = recall_at_10(dids.get(np.uint32, (NQ, K)), gt)
+            rec = recall_at_10(dids.get(np.uint32, (NQ, K))[:NGT], gt)
             sweep[L] = rec
             log("L=%d recall@10=%.4f" % (L, rec))
             if L_star is None and rec >= 0.95:
@@ -445,7 +463,7 @@ def main():
             hs, am = ix.autotune(tune_q, K, L_star, bw, dk.Hybrid)
             launch(L_star, bw=bw)
             sync()
-            rec_bw = recall_at_10(dids.get(np.uint32, (NQ, K)), gt)
+            rec_bw = recall_at_10(dids.get(np.uint32, (NQ, K))[:NGT], gt)
             t0 = time.perf_counter()
             for _ in range(2):
                 launch(L_star, bw=bw)
@@ -479,7 +497,7 @@ def main():
     qps = NQ * args.steps / wall
     kernel_ms = kms.value / max(nlaunch.value, 1)
     ids = dids.get(np.uint32, (NQ, K))
-    rec = recall_at_10(ids, gt)
+    rec = recall_at_10(ids[:NGT], gt)
     cmps = dcm.get(np.uint32, NQ).astype(np.float64)
     hops = dhp.get(np.uint32, NQ).astype(np.float64)
     degs = ddeg.get(np.uint32, NQ).astype(np.float64)
@@ -487,6 +505,11 @@ def main():
         # scalar-quantized traversal + rerank: cmps*D (code rows, L2 reads no compensation)
         #   + hops*4 + degsum*4 + D*4 (query) + (L+1)*D*4 (rerank)
         bytes_per_q = cmps * D + hops * 4 + degs * 4 + D * 4 + (L_star + 1) * D * 4
+    elif SPH:
+        # 1-bit traversal + rerank: cmps*(D/8 bit row + 16 metadata) + hops*4 + degsum*4
+        #   + D*4 (query) + (L+1)*D*4 (rerank)
+        bytes_per_q = (cmps * (((sph.out_dim + 7) // 8 + 15) // 16 * 16 + 16) + hops * 4 +
+                       degs * 4 + D * 4 + (L_star + 1) * D * 4)
     else:
         # PQ traversal + rerank: cmps*(M + 4) + hops*4 + degsum*4 + K*D*4 (pivots for the
         # LUT) + (L+1)*D*4 (rerank)
@@ -536,6 +559,10 @@ def main():
         sqc, sqcomp = dk.scalar.split(ix.get_sq_codes())
         oix = po.OracleIndex(po.L2, vec, lists, counts, [N],
                              sq=(sqz.scale, sqz.shift, sqz.mean_norm, sqc, sqcomp))
+    elif SPH:
+        osph = po.Spherical(po.COSINE, sph.centroid, sph.mean_norm, 1.0, po.SPH_PADDING_HADAMARD,
+                            sph.signs0, None, sph.padded_dim)
+        oix = po.OracleIndex(po.COSINE, vec, lists, counts, [N], sph=(osph, ix.get_sph_codes(), 1))
     else:
         codes = ix.get_codes()
         oix = po.OracleIndex(po.L2, vec, lists, counts, [N], codes, pivots, offsets)
@@ -551,7 +578,8 @@ def main():
     # HBM bytes per launch from the rocprofv3 counter runs (profiles/), when they were
     # taken at the settings chosen here
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_sq_1M.json" if SQ else "traffic_hybrid_1M.json")
+    tf = os.path.join(ROOT, "profiles", "traffic_sq_1M.json" if SQ else
+                      "traffic_sph_1M.json" if SPH else "traffic_hybrid_1M.json")
     if os.path.exists(tf):
         with open(tf) as f:
             tr = json.load(f)
@@ -561,12 +589,18 @@ def main():
     workload = ("batched scalar-quantized search: 1M x 128 f32 GMM(256, sigma 0.5), 8-bit "
                 "scalar quantizer (2 std), Vamana R=64 alpha=1.2, 10k queries/launch, k=10, "
                 "SQ traversal + FP32 rerank" if SQ else
+                "batched 1-bit spherical search: %d x 1536 unit-norm mixture (4096 "
+                "components), cosine, 1-bit spherical quantizer (padding Hadamard to 2048), "
+                "4-bit transposed queries, Vamana R=96 alpha=1.2, 10k queries/launch, k=10, "
+                "FP32 rerank; recall on 2,000 queries" % N if SPH else
                 "batched PQ search: 1M x 128 f32 GMM(256, sigma 0.5), PQ 32x256, "
                 "Vamana R=64 alpha=1.2, 10k queries/launch, k=10, Hybrid "
                 "(ADC traversal + FP32 rerank)")
     out = {
         "metric": ("QPS at recall@10=0.95, batched 8-bit scalar-quantized traversal + FP32 "
                    "rerank" if SQ else
+                   "QPS at recall@10=0.95, batched 1-bit spherical traversal + FP32 rerank"
+                   if SPH else
                    "QPS at recall@10=0.95 + p99 latency, served batched vs plain per-query GPU"),
         "value": qps,
         "unit": "queries/s",
@@ -602,7 +636,8 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "beam_search_kernel<L2,SQ8>" if SQ else "beam_search_kernel<L2,PQ>",
+            "kernel": ("beam_search_kernel<L2,SQ8>" if SQ else "beam_search_kernel<SPH1,Q4>"
+                       if SPH else "beam_search_kernel<L2,PQ>"),
             "kernel_ms": kernel_ms,
             "algorithmic_bytes_per_launch": alg_bytes,
         },

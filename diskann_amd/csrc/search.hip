@@ -70,9 +70,8 @@ __host__ __device__ inline size_t lds_bytes(const SearchParams &p) {
     b += align16(sizeof(uint32_t) * rows_entries(p));
     b += 16;
     if (p.sq) b += align16(p.sq_cstride) + 16;
-    if (p.sph)
-        b += align16(sizeof(float) * sph_query_lds_floats(p.sphq)) +
-             align16(sph_query_code_bytes(p.sphq)) + 16;
+    if (p.sph)  // traversal layouts need at most the 4 bit planes
+        b += align16(sizeof(float) * sph_query_lds_floats(p.sphq)) + align16(4 * p.sphq.cstride) + 16;
     return b;
 }
 
@@ -106,7 +105,7 @@ __device__ inline Lds carve(const SearchParams &p, char *base) {
     s.sqc = p.sq ? (uint8_t *)take(p.sq_cstride) : nullptr;
     s.sqm = p.sq ? (uint32_t *)take(16) : nullptr;
     s.sphw = p.sph ? (float *)take(sizeof(float) * sph_query_lds_floats(p.sphq)) : nullptr;
-    s.sphb = p.sph ? (uint8_t *)take(sph_query_code_bytes(p.sphq)) : nullptr;
+    s.sphb = p.sph ? (uint8_t *)take(4 * p.sphq.cstride) : nullptr;
     s.sphm = p.sph ? (float *)take(16) : nullptr;
     return s;
 }

@@ -129,6 +129,7 @@ struct dkm_index {
     DevBuf<uint32_t> d_sph_signs0, d_sph_signs1, d_sph_sub;
     DevBuf<uint8_t> d_sph_bits;
     DevBuf<dkm::SphMeta> d_sph_meta;
+    DevBuf<uint8_t> d_sph_qrec;          // prepared traversal queries of a launch
     // staging for the host-buffer API
     DevBuf<float> s_q, s_d;
     DevBuf<uint32_t> s_k, s_L, s_ids, s_cnt, s_cmps, s_hops, s_status;
@@ -402,6 +403,8 @@ dkm_status dkm_index_create(int metric, int64_t dim, int64_t max_points, int64_t
     if (e == hipSuccess) e = hipMemset(ix->d_vec, 0, sizeof(float) * ix->rows * ix->vstride);
     if (e == hipSuccess) e = hipMalloc(&ix->d_graph, sizeof(uint32_t) * ix->rows * ix->gstride);
     if (e == hipSuccess) e = hipMemset(ix->d_graph, 0, sizeof(uint32_t) * ix->rows * ix->gstride);
+    // the index's own stream does not wait for the null stream: finish the clears here
+    if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMalloc(&ix->d_starts, sizeof(uint32_t) * num_start);
     if (e == hipSuccess) {
         std::vector<uint32_t> s(num_start);
@@ -561,7 +564,8 @@ dkm_status dkm_set_pq(dkm_index *ix, const float *pivots, int64_t K, const int64
     HIP_TRY(hipMalloc(&ix->d_pivots, sizeof(float) * K * ix->dim));
     HIP_TRY(hipMalloc(&ix->d_offsets, sizeof(int32_t) * (M + 1)));
     HIP_TRY(hipMalloc(&ix->d_codes, ix->rows * M + 16));
-    HIP_TRY(hipMemset(ix->d_codes, 0, ix->rows * M + 16));
+    HIP_TRY(hipMemsetAsync(ix->d_codes, 0, ix->rows * M + 16, ix->stream));
+    HIP_TRY(hipStreamSynchronize(ix->stream));  // cleared before any copy on another stream
     HIP_TRY(hipMemcpy(ix->d_pivots, pivots, sizeof(float) * K * ix->dim, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ix->d_offsets, o32.data(), sizeof(int32_t) * (M + 1), hipMemcpyHostToDevice));
     ix->codes_set = false;
@@ -764,8 +768,9 @@ dkm_status dkm_set_sq(dkm_index *ix, float scale, const float *shift, float mean
     HIP_TRY(ix->d_sq_shift.ensure(ix->dim));
     HIP_TRY(ix->d_sq_comp.ensure(ix->rows));
     HIP_TRY(ix->d_sq_codes.ensure(ix->rows * ix->sq_cstride + 16));
-    HIP_TRY(hipMemset(ix->d_sq_codes.p, 0, ix->rows * ix->sq_cstride + 16));
-    HIP_TRY(hipMemset(ix->d_sq_comp.p, 0, sizeof(float) * ix->rows));
+    HIP_TRY(hipMemsetAsync(ix->d_sq_codes.p, 0, ix->rows * ix->sq_cstride + 16, ix->stream));
+    HIP_TRY(hipMemsetAsync(ix->d_sq_comp.p, 0, sizeof(float) * ix->rows, ix->stream));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
     HIP_TRY(hipMemcpy(ix->d_sq_shift.p, shift, sizeof(float) * ix->dim, hipMemcpyHostToDevice));
     DevBuf<float> dssn;
     HIP_TRY(dssn.ensure(1));
@@ -1165,8 +1170,9 @@ dkm_status dkm_set_spherical(dkm_index *ix, const dkm_sph *q, int query_layout) 
     }
     HIP_TRY(ix->d_sph_bits.ensure(ix->rows * p.cstride + 16));
     HIP_TRY(ix->d_sph_meta.ensure(ix->rows));
-    HIP_TRY(hipMemset(ix->d_sph_bits.p, 0, ix->rows * p.cstride + 16));
-    HIP_TRY(hipMemset(ix->d_sph_meta.p, 0, sizeof(dkm::SphMeta) * ix->rows));
+    HIP_TRY(hipMemsetAsync(ix->d_sph_bits.p, 0, ix->rows * p.cstride + 16, ix->stream));
+    HIP_TRY(hipMemsetAsync(ix->d_sph_meta.p, 0, sizeof(dkm::SphMeta) * ix->rows, ix->stream));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
     ix->sphq = p;
     ix->sph_layout = query_layout;
     ix->sph_active = true;
@@ -1324,6 +1330,10 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     p.pivots = ix->d_pivots;
     p.offsets = ix->d_offsets;
     set_sq_params(ix, sq, p, sph);
+    if (sph) {
+        HIP_TRY(ix->d_sph_qrec.ensure(nq * dkm::sph_query_record_bytes(ix->sphq)));
+        p.sph_qrec = ix->d_sph_qrec.p;
+    }
     p.queries = d_queries;
     p.qstride = ix->dim;
     p.nq = (int)nq;
@@ -1728,6 +1738,7 @@ struct ServeLane {
     uint32_t *d_cmps = nullptr, *d_hops = nullptr, *d_status = nullptr;
     float *d_d = nullptr;
     uint32_t *d_spill = nullptr;
+    uint8_t *d_sphq = nullptr;  // spherical: prepared queries of a launch
 };
 
 double now_us() {
@@ -1792,7 +1803,7 @@ static void server_free(dkm_server *sv) {
     for (ServeLane &l : sv->lane) {
         if (l.stream) (void)hipStreamSynchronize(l.stream);
         void *dp[] = {l.d_q, l.d_k, l.d_L, l.d_ids, l.d_cnt, l.d_cmps, l.d_hops, l.d_status, l.d_d,
-                      l.d_spill};
+                      l.d_spill, l.d_sphq};
         for (void *q : dp)
             if (q) (void)hipFree(q);
         if (l.done) (void)hipEventDestroy(l.done);
@@ -1839,6 +1850,7 @@ static dkm_status server_pump(dkm_server *sv) {
         p.out_hops = l.d_hops;
         p.status = l.d_status;
         p.spill_table = l.d_spill;
+        p.sph_qrec = l.d_sphq;
         hipError_t e = dkm::launch_search(p, l.stream);
         if (e != hipSuccess) return fail(DKM_ERR_DEVICE, "serving launch: %s", hipGetErrorString(e));
         HIP_TRY(ring_copy(sv->res_ids, l.d_ids, seq0, n, sv->max_k, cap, false, l.stream));
@@ -1952,6 +1964,8 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
         if (e == hipSuccess) e = hipMalloc(&l.d_hops, sizeof(uint32_t) * P);
         if (e == hipSuccess) e = hipMalloc(&l.d_status, sizeof(uint32_t) * P);
         if (e == hipSuccess) e = hipMalloc(&l.d_spill, sizeof(uint32_t) * p.spill_slots * P);
+        if (e == hipSuccess && p.sph)
+            e = hipMalloc(&l.d_sphq, (size_t)dkm::sph_query_record_bytes(p.sphq) * P);
     }
     for (ServeLane &l : sv->lane)
         if (e == hipSuccess && l.stream) e = hipStreamSynchronize(l.stream);

@@ -65,6 +65,8 @@ struct SearchParams {
     SphParams sphq;
     const uint8_t *sph_bits;  // rows x sphq.cstride
     const SphMeta *sph_meta;  // rows
+    uint8_t *sph_qrec;        // nq prepared queries (sph_query_record_bytes each), written by
+                              // launch_search before the search kernel
 };
 
 size_t search_lds_bytes(const SearchParams &p);

@@ -314,6 +314,12 @@ __device__ inline void sph_query_code(const SphParams &q, int layout, bool resca
     __syncthreads();
 }
 
+// A prepared traversal query in HBM: the bit row (same-as-data) or the 4 bit planes,
+// then its 4 metadata floats.
+__host__ __device__ inline int sph_query_record_bytes(const SphParams &q) { return 4 * q.cstride + 16; }
+
+hipError_t launch_sph_query_prep(const SphParams &q, int layout, const float *queries,
+                                 int64_t qstride, int64_t nq, uint8_t *records, hipStream_t st);
 hipError_t launch_sph_shift_sq(const float *shift, int dim, float *out, hipStream_t st);
 hipError_t launch_sph_encode(const SphParams &q, const float *x, int64_t stride, int64_t n,
                              uint8_t *bits, SphMeta *meta, uint8_t *canon, uint32_t *err,

@@ -38,7 +38,6 @@ struct Lds {
     int32_t *misc;
     uint8_t *sqc;    // scalar-quantized query code (sq_cstride bytes)
     uint32_t *sqm;   // [0] query compensation (f32 bits), [1] sum of squared query codes
-    float *sphw;     // spherical: query preparation area (sph_query_lds_floats)
     uint8_t *sphb;   // spherical: query bit row / 4 bit planes
     float *sphm;     // spherical: query metadata (4 floats)
 };
@@ -70,8 +69,7 @@ __host__ __device__ inline size_t lds_bytes(const SearchParams &p) {
     b += align16(sizeof(uint32_t) * rows_entries(p));
     b += 16;
     if (p.sq) b += align16(p.sq_cstride) + 16;
-    if (p.sph)  // traversal layouts need at most the 4 bit planes
-        b += align16(sizeof(float) * sph_query_lds_floats(p.sphq)) + align16(4 * p.sphq.cstride) + 16;
+    if (p.sph) b += align16(4 * p.sphq.cstride) + 16;  // the prepared query record
     return b;
 }
 
@@ -104,7 +102,6 @@ __device__ inline Lds carve(const SearchParams &p, char *base) {
     s.misc = (int32_t *)take(16);
     s.sqc = p.sq ? (uint8_t *)take(p.sq_cstride) : nullptr;
     s.sqm = p.sq ? (uint32_t *)take(16) : nullptr;
-    s.sphw = p.sph ? (float *)take(sizeof(float) * sph_query_lds_floats(p.sphq)) : nullptr;
     s.sphb = p.sph ? (uint8_t *)take(4 * p.sphq.cstride) : nullptr;
     s.sphm = p.sph ? (float *)take(16) : nullptr;
     return s;
@@ -894,6 +891,7 @@ struct QueryJob {
     uint32_t *out_ids;      // k_stride entries
     float *out_d;
     uint32_t *out_count, *out_cmps, *out_hops, *rec_len, *out_degsum;
+    const uint8_t *sph_rec;  // spherical: this query's prepared record
 };
 
 template <int METRIC, int MODE, int DIM, int PQLEN>
@@ -920,11 +918,12 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
     __syncthreads();
     if constexpr (MODE == 3) sq_query<METRIC>(p, s);
     if constexpr (MODE == 4 || MODE == 5) {
-        // the reference's search accessor asks for the inner-product rescale
-        uint32_t e = 0;
-        sph_query_code(p.sphq, MODE == 4 ? SPH_QUERY_SAME_AS_DATA : SPH_QUERY_FOUR_BIT, true, s.q,
-                       s.sphw, s.sphb, s.sphm, e);
-        (void)e;
+        // the query record prepared by sph_query_prep_kernel (launch_search)
+        const uint8_t *rec = job.sph_rec;
+        const int nb = MODE == 4 ? p.sphq.cstride : 4 * p.sphq.cstride;
+        for (int i = lane; i < nb / 16; i += kWave) ((uint4 *)s.sphb)[i] = ((const uint4 *)rec)[i];
+        if (lane < 4) s.sphm[lane] = ((const float *)(rec + 4 * p.sphq.cstride))[lane];
+        __syncthreads();
     }
     if (MODE == 1) {
         const bool ip = METRIC == METRIC_IP;
@@ -1141,6 +1140,7 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     j.out_hops = p.out_hops ? p.out_hops + qidx : nullptr;
     j.rec_len = p.rec_len ? p.rec_len + qidx : nullptr;
     j.out_degsum = p.out_degsum ? p.out_degsum + qidx : nullptr;
+    j.sph_rec = p.sph_qrec ? p.sph_qrec + (size_t)qidx * sph_query_record_bytes(p.sphq) : nullptr;
     search_query<METRIC, MODE, DIM, PQLEN>(p, s, j);
 }
 
@@ -1220,6 +1220,12 @@ static hipError_t dispatch(const SearchParams &p, hipStream_t stream, int *resid
 size_t search_lds_bytes(const SearchParams &p) { return lds_bytes(p); }
 
 hipError_t launch_search(const SearchParams &p, hipStream_t stream) {
+    if (p.sph) {
+        if (!p.sph_qrec) return hipErrorInvalidValue;
+        hipError_t e = launch_sph_query_prep(p.sphq, p.sph_layout, p.queries, p.qstride, p.nq,
+                                             p.sph_qrec, stream);
+        if (e != hipSuccess) return e;
+    }
     return dispatch(p, stream, nullptr);
 }
 

@@ -160,6 +160,26 @@ __global__ __launch_bounds__(64) void sph_query_pairs_kernel(SphParams q, int la
     }
 }
 
+// Traversal queries compressed ahead of the search (one block per query, with the
+// inner-product rescale of the reference's search strategy): record q holds the layout's
+// bit row / planes and the 4 metadata floats.
+__global__ __launch_bounds__(64) void sph_query_prep_kernel(SphParams q, int layout,
+                                                            const float *queries, int64_t qstride,
+                                                            int64_t nq, uint8_t *records) {
+    extern __shared__ __align__(16) float lds[];
+    const int64_t qi = blockIdx.x;
+    if (qi >= nq) return;
+    uint8_t *qb = (uint8_t *)(lds + sph_query_lds_floats(q));
+    __shared__ float qm[4];
+    uint32_t e = 0;
+    sph_query_code(q, layout, true, queries + qi * qstride, lds, qb, qm, e);
+    uint8_t *rec = records + qi * sph_query_record_bytes(q);
+    const int nbytes = layout == SPH_QUERY_SAME_AS_DATA ? q.cstride : 4 * q.cstride;
+    for (int i = threadIdx.x; i < nbytes / 16; i += 64)
+        ((uint4 *)rec)[i] = ((const uint4 *)qb)[i];
+    if (threadIdx.x < 4) ((float *)(rec + 4 * q.cstride))[threadIdx.x] = qm[threadIdx.x];
+}
+
 // ---- training statistics ----------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void sph_row_norm_kernel(const float *x, int64_t n, int dim,
@@ -248,6 +268,18 @@ hipError_t launch_sph_query_pairs(const SphParams &q, int layout, int rescale, c
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sph_query_pairs_kernel, dim3((unsigned)nq), dim3(64), lds, st, q, layout,
                        rescale, queries, nq, bits, meta, ids, m, out, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_sph_query_prep(const SphParams &q, int layout, const float *queries,
+                                 int64_t qstride, int64_t nq, uint8_t *records, hipStream_t st) {
+    if (nq <= 0) return hipSuccess;
+    const size_t lds = sizeof(float) * (size_t)sph_query_lds_floats(q) + 4 * (size_t)q.cstride + 64;
+    hipError_t e = hipFuncSetAttribute((const void *)sph_query_prep_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sph_query_prep_kernel, dim3((unsigned)nq), dim3(64), lds, st, q, layout,
+                       queries, qstride, nq, records);
     return hipGetLastError();
 }
 

@@ -1024,6 +1024,113 @@ float dko_sph_dist_full(const dko_sph *q, const float *v, const float *meta, con
     return sph_finish(q, ip + meta[2] + ym + sph_shift_sq(q));
 }
 
+/* ------------------------------------------------------------------------------------
+ * MinMax quantization, 8 bits
+ * ---------------------------------------------------------------------------------- */
+
+int64_t dko_minmax_bytes(const dko_sph *t) { return 20 + t->out_dim; }
+
+/* MinMaxQuantizer::compress (diskann-quantization/src/minmax/quantizer.rs:45-160) for 8
+ * bits: range = the transformed vector's min / max widened by grid_scale about its
+ * middle; a = max(max - min, 1e-8) / 255; code = round(clamp((v - min) / a, 0, 255));
+ * |reconstruction|^2, code sum and loss are sequential f32 sums. */
+int dko_minmax_compress(const dko_sph *t, float grid_scale, const float *x, uint8_t *out,
+                        float *loss) {
+    const int64_t n = t->out_dim;
+    float *v = (float *)malloc(sizeof(float) * (size_t)(n + 1));
+    dko_sph_transform(t, x, v);
+    float mn = NAN, mx = NAN;
+    for (int64_t i = 0; i < n; ++i) {
+        mn = fminf(mn, v[i]);
+        mx = fmaxf(mx, v[i]);
+    }
+    const float width = (mx - mn) / 2.0f;
+    const float mid = mn + width;
+    const float lo = mid - width * grid_scale, hi = mid + width * grid_scale;
+    const float a = fmaxf(hi - lo, 1e-8f) / 255.0f;
+    float ns = 0.0f, cs = 0.0f, ls = 0.0f;
+    int nan = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        nan |= v[i] != v[i];
+        float c = (v[i] - lo) / a;
+        if (c < 0.0f) c = 0.0f;
+        else if (c > 255.0f) c = 255.0f;
+        c = roundf(c);
+        const float r = c * a + lo;
+        ns += r * r;
+        cs += c;
+        const float d = r - v[i];
+        ls += d * d;
+        out[20 + i] = c != c ? 0 : (uint8_t)c;
+    }
+    const uint32_t dim = (uint32_t)n;
+    const float meta[4] = {lo, a * cs, a, ns};
+    memcpy(out, &dim, 4);
+    memcpy(out + 4, meta, 16);
+    if (loss) *loss = ls;
+    free(v);
+    return nan;
+}
+
+/* CompressInto<FullQueryMut> (minmax/quantizer.rs:178-210): the transformed query with
+ * its sequential sum and square norm. */
+int dko_minmax_compress_full(const dko_sph *t, const float *x, float *v, float *meta) {
+    for (int64_t i = 0; i < t->dim; ++i)
+        if (x[i] != x[i]) return 1;
+    dko_sph_transform(t, x, v);
+    float ns = 0.0f, sum = 0.0f;
+    for (int64_t i = 0; i < t->out_dim; ++i) ns += v[i] * v[i];
+    for (int64_t i = 0; i < t->out_dim; ++i) sum += v[i];
+    meta[0] = sum;
+    meta[1] = ns;
+    return 0;
+}
+
+static void mm_meta(const uint8_t *c, uint32_t *dim, float m[4]) {
+    memcpy(dim, c, 4);
+    memcpy(m, c + 4, 16);  /* b, n, a, |r|^2 */
+}
+
+static float mm_finish(int metric, float v, float xns, float yns) {
+    if (metric == DKO_L2) return -2.0f * v + xns + yns;
+    if (metric == DKO_INNER_PRODUCT) return -v;
+    if (metric == DKO_COSINE) return 1.0f - v / (sqrtf(xns) * sqrtf(yns));
+    return 1.0f - v;
+}
+
+/* MinMaxIP / L2Squared / Cosine / CosineNormalized on two data rows
+ * (minmax/vectors.rs:206-229 kernel and the functors at 231-470). */
+float dko_minmax_dist_data(int metric, const uint8_t *a, const uint8_t *b) {
+    uint32_t dim, dim2;
+    float xm[4], ym[4];
+    mm_meta(a, &dim, xm);
+    mm_meta(b, &dim2, ym);
+    uint32_t raw = 0;
+    for (uint32_t i = 0; i < dim; ++i) raw += (uint32_t)a[20 + i] * (uint32_t)b[20 + i];
+    const float t0 = xm[2] * ym[2] * (float)raw;
+    const float t1x = xm[1] * ym[0];
+    const float t1y = ym[1] * xm[0];
+    const float t2 = xm[0] * ym[0] * (float)dim;
+    const float v = t0 + t1x + t1y + t2;
+    return mm_finish(metric, v, xm[3], ym[3]);
+}
+
+/* FullQueryRef x DataRef<8> (minmax/vectors.rs:271-470); the f32 x u8 product is the
+ * sequential sum x_i * code_i the x86-64 build runs for 8-bit codes
+ * (bits/distances.rs:2675-2700). */
+float dko_minmax_dist_full(int metric, const float *v, const float *meta, const uint8_t *b) {
+    uint32_t dim;
+    float ym[4];
+    mm_meta(b, &dim, ym);
+    float s = 0.0f;
+    for (uint32_t i = 0; i < dim; ++i) s += v[i] * (float)b[20 + i];
+    const float ip = s * ym[2] + meta[0] * ym[0];
+    if (metric == DKO_L2) return meta[1] + ym[3] - 2.0f * ip;
+    if (metric == DKO_INNER_PRODUCT) return -ip;
+    if (metric == DKO_COSINE) return 1.0f - ip / (sqrtf(meta[1]) * sqrtf(ym[3]));
+    return 1.0f - ip;
+}
+
 static inline float node_dist(const dko_index *ix, int pq, const float *query,
                               const float *lut, uint32_t id) {
     if (pq && ix->sph) {

@@ -123,6 +123,19 @@ float dko_sph_dist_full(const dko_sph *q, const float *v, const float *meta, con
 uint16_t dko_f32_to_f16(float f);
 float dko_f16_to_f32(uint16_t h);
 
+/* MinMax quantization, 8 bits (diskann-quantization/src/minmax) ------------------------
+ * The transform is a dko_sph's (metric, shift, mean_norm, pre_scale unused). A code row is
+ * the canonical front layout: u32 dim, f32 b (minimum), f32 n (a * sum of codes), f32 a
+ * (step), f32 |reconstruction|^2, then dim code bytes. */
+int64_t dko_minmax_bytes(const dko_sph *t);
+/* returns 1 when the transformed vector holds a NaN; *loss = squared reconstruction error */
+int dko_minmax_compress(const dko_sph *t, float grid_scale, const float *x, uint8_t *out,
+                        float *loss);
+/* full-precision query: out_dim transformed values + meta {sum, |v|^2} */
+int dko_minmax_compress_full(const dko_sph *t, const float *x, float *v, float *meta);
+float dko_minmax_dist_data(int metric, const uint8_t *a, const uint8_t *b);
+float dko_minmax_dist_full(int metric, const float *v, const float *meta, const uint8_t *b);
+
 /* Graph + search -------------------------------------------------------------------- */
 typedef struct {
     int metric;

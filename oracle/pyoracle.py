@@ -103,6 +103,16 @@ def _setup(L):
     L.dko_sph_dist_query4.restype = F
     L.dko_sph_dist_full.argtypes = [S, P, P, P]
     L.dko_sph_dist_full.restype = F
+    L.dko_minmax_bytes.argtypes = [S]
+    L.dko_minmax_bytes.restype = I64
+    L.dko_minmax_compress.argtypes = [S, F, P, P, P]
+    L.dko_minmax_compress.restype = C.c_int
+    L.dko_minmax_compress_full.argtypes = [S, P, P, P]
+    L.dko_minmax_compress_full.restype = C.c_int
+    L.dko_minmax_dist_data.argtypes = [C.c_int, P, P]
+    L.dko_minmax_dist_data.restype = F
+    L.dko_minmax_dist_full.argtypes = [C.c_int, P, P, P]
+    L.dko_minmax_dist_full.restype = F
     L.dko_f32_to_f16.argtypes = [F]
     L.dko_f32_to_f16.restype = C.c_uint16
     L.dko_f16_to_f32.argtypes = [C.c_uint16]
@@ -354,6 +364,46 @@ def sph_train_stats(metric, x):
     mn = C.c_float()
     lib().dko_sph_train_stats(metric, _p(x), x.shape[0], x.shape[1], _p(c), C.byref(mn))
     return c, float(np.float32(mn.value))
+
+
+class MinMax:
+    """8-bit MinMax quantizer on the oracle; the transform comes from a Spherical's."""
+
+    def __init__(self, transform, grid_scale=1.0):
+        self.t = transform
+        self.grid_scale = float(grid_scale)
+        self.bytes = lib().dko_minmax_bytes(C.byref(transform.s))
+
+    def compress(self, x):
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float32)
+        out = np.zeros((x.shape[0], self.bytes), np.uint8)
+        loss = np.zeros(x.shape[0], np.float32)
+        for r in range(x.shape[0]):
+            lo = C.c_float()
+            if lib().dko_minmax_compress(C.byref(self.t.s), self.grid_scale, _p(x[r]), _p(out[r]),
+                                         C.byref(lo)):
+                raise ValueError("input contains NaN (row %d)" % r)
+            loss[r] = lo.value
+        return out, loss
+
+    def compress_full(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        v = np.zeros(self.t.out_dim, np.float32)
+        m = np.zeros(2, np.float32)
+        if lib().dko_minmax_compress_full(C.byref(self.t.s), _p(x), _p(v), _p(m)):
+            raise ValueError("input contains NaN")
+        return v, m
+
+    @staticmethod
+    def dist_data(metric, a, b):
+        a = np.ascontiguousarray(a, dtype=np.uint8)
+        b = np.ascontiguousarray(b, dtype=np.uint8)
+        return lib().dko_minmax_dist_data(metric, _p(a), _p(b))
+
+    @staticmethod
+    def dist_full(metric, v, m, b):
+        b = np.ascontiguousarray(b, dtype=np.uint8)
+        return lib().dko_minmax_dist_full(metric, _p(v), _p(m), _p(b))
 
 
 def hadamard(x):

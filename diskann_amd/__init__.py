@@ -8,11 +8,11 @@ from .errors import (ANNError, CapacityError, DimensionMismatchError, KnnSearchE
                      StorageError)
 from .index import (DiskANNIndex, FullPrecision, Hybrid, Knn, Metric, Quantized, SearchResult,
                     Server, Strategy)
-from . import prescore, quantization, scalar, spherical, storage
+from . import minmax, prescore, quantization, scalar, spherical, storage
 
 __all__ = [
     "ANNError", "CapacityError", "DimensionMismatchError", "KnnSearchError", "PQError",
     "StorageError", "storage", "prescore",
     "DiskANNIndex", "FullPrecision", "Hybrid", "Knn", "Metric", "Quantized", "SearchResult",
-    "Server", "Strategy", "quantization", "scalar", "spherical",
+    "Server", "Strategy", "quantization", "scalar", "spherical", "minmax",
 ]

@@ -32,6 +32,8 @@ EXPORTS = [
     "dkm_sph_create", "dkm_sph_destroy", "dkm_sph_dims", "dkm_sph_train_stats",
     "dkm_sph_compress", "dkm_sph_distance_batch", "dkm_sph_query_distance_batch",
     "dkm_set_spherical", "dkm_encode_index_sph", "dkm_set_sph_codes", "dkm_get_sph_codes",
+    "dkm_minmax_create", "dkm_minmax_destroy", "dkm_minmax_dims", "dkm_minmax_compress",
+    "dkm_minmax_distance_batch", "dkm_minmax_query_distance_batch",
 ]
 
 _lib = None
@@ -97,6 +99,11 @@ def _declare(L):
         "dkm_encode_index_sph": [P],
         "dkm_set_sph_codes": [P, P, I64, I64],
         "dkm_get_sph_codes": [P, P, I64, I64],
+        "dkm_minmax_create": [I64, C.c_float, I, P, P, I64, I64, P, I64, C.POINTER(C.c_void_p)],
+        "dkm_minmax_dims": [P, C.POINTER(I64), C.POINTER(I64)],
+        "dkm_minmax_compress": [P, P, I64, P, P],
+        "dkm_minmax_distance_batch": [P, I, P, I64, P, I64, P, I64, P],
+        "dkm_minmax_query_distance_batch": [P, I, P, I64, P, I64, P, I64, P],
         "dkm_get_codes": [P, P, I64, I64],
         "dkm_search_batch": [P, I, P, I64, P, P, U32, I64, P, P, P, P, P],
         "dkm_search_batch_device": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P],
@@ -145,3 +152,5 @@ def _declare(L):
     L.dkm_prescore_destroy.restype = None
     L.dkm_sph_destroy.argtypes = [P]
     L.dkm_sph_destroy.restype = None
+    L.dkm_minmax_destroy.argtypes = [P]
+    L.dkm_minmax_destroy.restype = None

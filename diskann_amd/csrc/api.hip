@@ -1230,6 +1230,142 @@ dkm_status dkm_get_sph_codes(dkm_index *ix, uint8_t *codes, int64_t row0, int64_
     return DKM_OK;
 }
 
+// ------------------------------------------------------------------------------------
+// 8-bit MinMax quantization (diskann-quantization/src/minmax)
+// ------------------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct dkm_minmax {
+    dkm_sph *t = nullptr;   // the transform (centroid / metric unused)
+    float grid = 1.0f;
+};
+
+extern "C" {
+
+dkm_status dkm_minmax_create(int64_t dim, float grid_scale, int transform, const uint8_t *signs0,
+                             const uint8_t *signs1, int64_t n_signs1, int64_t padded_dim,
+                             const uint32_t *subsample, int64_t n_subsample, dkm_minmax **out) {
+    if (!out) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (!(grid_scale > 0.0f) || !std::isfinite(grid_scale)) return fail(DKM_ERR_INVALID_ARG, "grid_scale must be positive");
+    if (dim <= 0) return fail(DKM_ERR_INVALID_ARG, "dim must be positive");
+    std::vector<float> zero((size_t)dim, 0.0f);
+    auto h = std::make_unique<dkm_minmax>();
+    dkm_status st = dkm_sph_create(DKM_L2, dim, zero.data(), 1.0f, 1.0f, transform, signs0, signs1,
+                                   n_signs1, padded_dim, subsample, n_subsample, &h->t);
+    if (st) return st;
+    h->grid = grid_scale;
+    *out = h.release();
+    return DKM_OK;
+}
+
+void dkm_minmax_destroy(dkm_minmax *q) {
+    if (!q) return;
+    dkm_sph_destroy(q->t);
+    delete q;
+}
+
+dkm_status dkm_minmax_dims(const dkm_minmax *q, int64_t *out_dim, int64_t *code_bytes) {
+    if (!q || !out_dim || !code_bytes) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    *out_dim = q->t->q.out_dim;
+    *code_bytes = 20 + (int64_t)q->t->q.out_dim;
+    return DKM_OK;
+}
+
+dkm_status dkm_minmax_compress(const dkm_minmax *q, const float *x, int64_t n, uint8_t *out,
+                               float *out_loss) {
+    if (!q || n < 0 || (n > 0 && (!x || !out))) return fail(DKM_ERR_INVALID_ARG, "bad arguments");
+    if (n == 0) return DKM_OK;
+    HIP_TRY(hipSetDevice(g_device));
+    const dkm::SphParams &t = q->t->q;
+    const int64_t rb = 20 + (int64_t)t.out_dim;
+    DevBuf<float> dx, dl;
+    DevBuf<uint8_t> dout;
+    DevBuf<uint32_t> derr;
+    HIP_TRY(dx.ensure(n * t.dim));
+    HIP_TRY(dl.ensure(n));
+    HIP_TRY(dout.ensure(n * rb));
+    HIP_TRY(derr.ensure(1));
+    HIP_TRY(hipMemcpy(dx.p, x, sizeof(float) * n * t.dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(derr.p, 0, sizeof(uint32_t)));
+    HIP_TRY(dkm::launch_minmax_encode(t, q->grid, dx.p, t.dim, n, dout.p, dl.p, derr.p, nullptr));
+    HIP_TRY(hipMemcpy(out, dout.p, (size_t)(n * rb), hipMemcpyDeviceToHost));
+    if (out_loss) HIP_TRY(hipMemcpy(out_loss, dl.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    uint32_t e = 0;
+    HIP_TRY(hipMemcpy(&e, derr.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (e) return fail(DKM_ERR_INVALID_ARG, "input contains NaN");
+    return DKM_OK;
+}
+
+static int minmax_metric(int m) {
+    if (m == DKM_L2) return dkm::METRIC_L2;
+    if (m == DKM_INNER_PRODUCT) return dkm::METRIC_IP;
+    if (m == DKM_COSINE) return dkm::METRIC_COSINE;
+    if (m == DKM_COSINE_NORMALIZED) return dkm::METRIC_COSNORM;
+    return -1;
+}
+
+dkm_status dkm_minmax_distance_batch(const dkm_minmax *q, int metric, const uint8_t *left,
+                                     int64_t nl, const uint8_t *right, int64_t nr,
+                                     const uint32_t *ids, int64_t m, float *out) {
+    const int mm = minmax_metric(metric);
+    if (!q || mm < 0) return fail(DKM_ERR_INVALID_ARG, "bad quantizer or metric");
+    if (nl < 0 || nr <= 0 || m < 0 || !left || !right || !out || (nl * m > 0 && !ids))
+        return fail(DKM_ERR_INVALID_ARG, "bad sizes");
+    if (nl == 0 || m == 0) return DKM_OK;
+    for (int64_t i = 0; i < nl * m; ++i)
+        if (ids[i] >= (uint64_t)nr) return fail(DKM_ERR_INDEX, "id %u out of range (%lld rows)", ids[i], (long long)nr);
+    const int rb = 20 + q->t->q.out_dim;
+    HIP_TRY(hipSetDevice(g_device));
+    DevBuf<uint8_t> dl, dr;
+    DevBuf<uint32_t> di;
+    DevBuf<float> dout;
+    HIP_TRY(dl.ensure(nl * rb));
+    HIP_TRY(dr.ensure(nr * rb));
+    HIP_TRY(di.ensure(nl * m));
+    HIP_TRY(dout.ensure(nl * m));
+    HIP_TRY(hipMemcpy(dl.p, left, (size_t)(nl * rb), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dr.p, right, (size_t)(nr * rb), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(di.p, ids, sizeof(uint32_t) * nl * m, hipMemcpyHostToDevice));
+    HIP_TRY(dkm::launch_minmax_pairs(mm, dl.p, dr.p, rb, di.p, nl, m, dout.p, nullptr));
+    HIP_TRY(hipMemcpy(out, dout.p, sizeof(float) * nl * m, hipMemcpyDeviceToHost));
+    return DKM_OK;
+}
+
+dkm_status dkm_minmax_query_distance_batch(const dkm_minmax *q, int metric, const float *queries,
+                                           int64_t nq, const uint8_t *codes, int64_t n,
+                                           const uint32_t *ids, int64_t m, float *out) {
+    const int mm = minmax_metric(metric);
+    if (!q || mm < 0) return fail(DKM_ERR_INVALID_ARG, "bad quantizer or metric");
+    if (nq < 0 || n <= 0 || m < 0 || !queries || !codes || !out || (nq * m > 0 && !ids))
+        return fail(DKM_ERR_INVALID_ARG, "bad sizes");
+    if (nq == 0 || m == 0) return DKM_OK;
+    for (int64_t i = 0; i < nq * m; ++i)
+        if (ids[i] >= (uint64_t)n) return fail(DKM_ERR_INDEX, "id %u out of range (%lld rows)", ids[i], (long long)n);
+    const dkm::SphParams &t = q->t->q;
+    const int rb = 20 + t.out_dim;
+    HIP_TRY(hipSetDevice(g_device));
+    DevBuf<float> dq, dout;
+    DevBuf<uint8_t> dc;
+    DevBuf<uint32_t> di, derr;
+    HIP_TRY(dq.ensure(nq * t.dim));
+    HIP_TRY(dc.ensure(n * rb));
+    HIP_TRY(di.ensure(nq * m));
+    HIP_TRY(dout.ensure(nq * m));
+    HIP_TRY(derr.ensure(1));
+    HIP_TRY(hipMemcpy(dq.p, queries, sizeof(float) * nq * t.dim, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dc.p, codes, (size_t)(n * rb), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(di.p, ids, sizeof(uint32_t) * nq * m, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(derr.p, 0, sizeof(uint32_t)));
+    HIP_TRY(dkm::launch_minmax_query_pairs(mm, t, dq.p, nq, dc.p, rb, di.p, m, dout.p, derr.p, nullptr));
+    HIP_TRY(hipMemcpy(out, dout.p, sizeof(float) * nq * m, hipMemcpyDeviceToHost));
+    uint32_t e = 0;
+    HIP_TRY(hipMemcpy(&e, derr.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (e) return fail(DKM_ERR_INVALID_ARG, "query contains NaN");
+    return DKM_OK;
+}
+
 // Default LDS visited-set size: a small first level (most of the visited set lives in the
 // per-query HBM table); small LDS footprints keep more queries resident per CU, which
 // hides the gather latency better than a large LDS table does (measured on MI355X).

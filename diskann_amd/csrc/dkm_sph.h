@@ -92,6 +92,33 @@ __device__ __forceinline__ float wave_ip(const float *a, const float *b, int len
     return -__shfl(v, 0);
 }
 
+// The distance-preserving transform (NullTransform / PaddingHadamard / DoubleHadamard
+// ::transform_into, algorithms/transforms/) of the dim floats at the front of lds by one
+// wave; returns the out_dim transformed floats (front area or transform buffer).
+__device__ inline float *sph_apply_transform(const SphParams &q, float *lds) {
+    const int lane = threadIdx.x;
+    float *sh = lds;
+    float *buf = lds + sph_front(q);
+    if (q.kind == SPH_NULL) return sh;
+    for (int i = lane; i < q.n; i += 64) buf[i] = i < q.dim ? sph_flip(sh[i], q.signs0[i]) : 0.0f;
+    __syncthreads();
+    if (q.kind == SPH_PADDING_HADAMARD) {
+        wave_hadamard(buf, q.n);
+    } else {
+        wave_hadamard(buf, q.split);
+        for (int i = lane; i < q.n; i += 64) buf[i] = sph_flip(buf[i], q.signs1[i]);
+        __syncthreads();
+        wave_hadamard(buf + (q.n - q.split), q.split);
+    }
+    if (q.subsample) {
+        // gather into the front area (max(dim, out_dim) floats)
+        for (int i = lane; i < q.out_dim; i += 64) sh[i] = buf[q.subsample[i]] * q.sub_rescale;
+        __syncthreads();
+        return sh;
+    }
+    return buf;
+}
+
 // SphericalQuantizer::preprocess + normalisation + transform (spherical/quantizer.rs:
 // 333-377, 806-845) of x (dim floats, global or LDS) by one wave. lds: sph_lds_floats.
 // Returns the transformed vector (out_dim floats in LDS) or nullptr when the shifted norm
@@ -118,24 +145,7 @@ __device__ inline float *sph_prepare(const SphParams &q, const float *x, float *
     if (norm == 0.0f) return nullptr;
     for (int i = lane; i < q.dim; i += 64) sh[i] = sh[i] / norm;
     __syncthreads();
-    if (q.kind == SPH_NULL) return sh;
-    for (int i = lane; i < q.n; i += 64) buf[i] = i < q.dim ? sph_flip(sh[i], q.signs0[i]) : 0.0f;
-    __syncthreads();
-    if (q.kind == SPH_PADDING_HADAMARD) {
-        wave_hadamard(buf, q.n);
-    } else {
-        wave_hadamard(buf, q.split);
-        for (int i = lane; i < q.n; i += 64) buf[i] = sph_flip(buf[i], q.signs1[i]);
-        __syncthreads();
-        wave_hadamard(buf + (q.n - q.split), q.split);
-    }
-    if (q.subsample) {
-        // gather into the front area (max(dim, out_dim) floats)
-        for (int i = lane; i < q.out_dim; i += 64) sh[i] = buf[q.subsample[i]] * q.sub_rescale;
-        __syncthreads();
-        return sh;
-    }
-    return buf;
+    return sph_apply_transform(q, lds);
 }
 
 __device__ __forceinline__ uint16_t f16_bits(float v) {
@@ -320,6 +330,18 @@ __host__ __device__ inline int sph_query_record_bytes(const SphParams &q) { retu
 
 hipError_t launch_sph_query_prep(const SphParams &q, int layout, const float *queries,
                                  int64_t qstride, int64_t nq, uint8_t *records, hipStream_t st);
+// 8-bit MinMax (diskann-quantization/src/minmax) with a SphParams transform: canonical
+// rows of 20 + out_dim bytes (u32 dim, f32 b, f32 n, f32 a, f32 |r|^2, codes).
+hipError_t launch_minmax_encode(const SphParams &t, float grid, const float *x, int64_t stride,
+                                int64_t n, uint8_t *out, float *loss, uint32_t *err,
+                                hipStream_t st);
+hipError_t launch_minmax_pairs(int metric, const uint8_t *left, const uint8_t *right, int rowbytes,
+                               const uint32_t *ids, int64_t nl, int64_t m, float *out,
+                               hipStream_t st);
+hipError_t launch_minmax_query_pairs(int metric, const SphParams &t, const float *queries,
+                                     int64_t nq, const uint8_t *codes, int rowbytes,
+                                     const uint32_t *ids, int64_t m, float *out, uint32_t *err,
+                                     hipStream_t st);
 hipError_t launch_sph_shift_sq(const float *shift, int dim, float *out, hipStream_t st);
 hipError_t launch_sph_encode(const SphParams &q, const float *x, int64_t stride, int64_t n,
                              uint8_t *bits, SphMeta *meta, uint8_t *canon, uint32_t *err,

@@ -264,6 +264,33 @@ dkm_status dkm_encode_index_sph(dkm_index *ix);
 dkm_status dkm_set_sph_codes(dkm_index *ix, const uint8_t *codes, int64_t row0, int64_t n);
 dkm_status dkm_get_sph_codes(dkm_index *ix, uint8_t *codes, int64_t row0, int64_t n);
 
+/* 8-bit MinMax quantization (diskann-quantization/src/minmax): MinMaxQuantizer::new(
+ * transform, grid_scale) with the transform given as for dkm_sph_create (0 null, 1
+ * padding Hadamard, 2 double Hadamard). A code row is the canonical front layout of
+ * minmax::Data<8> (vectors.rs:40-115): u32 dim, f32 b, f32 n, f32 a, f32 |reconstruction|^2,
+ * then out_dim code bytes.
+ * dkm_minmax_compress: CompressInto<DataMutRef<8>> (minmax/quantizer.rs:45-170), with the
+ *   squared reconstruction loss per row (out_loss may be NULL).
+ * dkm_minmax_distance_batch: MinMaxL2Squared / MinMaxIP / MinMaxCosine /
+ *   MinMaxCosineNormalized on two code rows (vectors.rs:206-470), metric L2, inner
+ *   product, cosine or cosine-normalized.
+ * dkm_minmax_query_distance_batch: the same against full-precision queries
+ *   (CompressInto<FullQueryMut>, quantizer.rs:178-210). */
+typedef struct dkm_minmax dkm_minmax;
+dkm_status dkm_minmax_create(int64_t dim, float grid_scale, int transform, const uint8_t *signs0,
+                             const uint8_t *signs1, int64_t n_signs1, int64_t padded_dim,
+                             const uint32_t *subsample, int64_t n_subsample, dkm_minmax **out);
+void dkm_minmax_destroy(dkm_minmax *q);
+dkm_status dkm_minmax_dims(const dkm_minmax *q, int64_t *out_dim, int64_t *code_bytes);
+dkm_status dkm_minmax_compress(const dkm_minmax *q, const float *x, int64_t n, uint8_t *out,
+                               float *out_loss);
+dkm_status dkm_minmax_distance_batch(const dkm_minmax *q, int metric, const uint8_t *left,
+                                     int64_t nl, const uint8_t *right, int64_t nr,
+                                     const uint32_t *ids, int64_t m, float *out);
+dkm_status dkm_minmax_query_distance_batch(const dkm_minmax *q, int metric, const float *queries,
+                                           int64_t nq, const uint8_t *codes, int64_t n,
+                                           const uint32_t *ids, int64_t m, float *out);
+
 /* Hybrid prune for dkm_build (common::Hybrid, diskann-providers/src/model/graph/provider/
  * async_/common.rs:385-405; HybridPruneAccessor, inmem/product.rs:143-230): each prune
  * keeps the first max_fp_vecs_per_prune ids of its fill order (robust_prune_with:

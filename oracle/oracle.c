@@ -1339,6 +1339,8 @@ typedef struct {
     uint32_t *out_ids, *out_cmps, *out_hops;
     float *out_d;
     double *lat_us;
+    const uint32_t *ks, *Ls; /* per-query k and L (mixed batches), or NULL */
+    int64_t kstride;         /* output row stride of a mixed batch */
 } batch_arg;
 
 static double now_s(void) {
@@ -1350,32 +1352,38 @@ static double now_s(void) {
 static void *batch_worker(void *p) {
     batch_arg *a = (batch_arg *)p;
     scratch s;
-    scratch_init(&s, a->ix, a->L, a->bw);
+    if (!a->Ls) scratch_init(&s, a->ix, a->L, a->bw);
     for (int64_t i = a->lo; i < a->hi; ++i) {
         double t0 = now_s();
         dko_stats st;
         const float *query = a->queries + i * a->qstride;
+        /* a mixed batch sizes each query's scratch by its own L, as the reference's
+         * per-query scratch does (diskann/src/graph/search/scratch.rs:186-217) */
+        if (a->Ls) scratch_init(&s, a->ix, a->Ls[i], a->bw);
+        const int64_t k = a->ks ? (int64_t)a->ks[i] : a->k;
+        const int64_t stride = a->ks ? a->kstride : a->k;
         search_core(a->ix, a->mode != DKO_MODE_FULL, query, a->bw, &s, NULL, NULL, NULL, 0,
                     &st);
-        post_process(a->ix, a->mode, query, &s, a->k, a->out_ids + i * a->k,
-                     a->out_d + i * a->k, &st);
-        for (int64_t j = st.count; j < a->k; ++j) {
-            a->out_ids[i * a->k + j] = 0xFFFFFFFFu;
-            a->out_d[i * a->k + j] = INFINITY;
+        post_process(a->ix, a->mode, query, &s, k, a->out_ids + i * stride,
+                     a->out_d + i * stride, &st);
+        if (a->Ls) scratch_free(&s);
+        for (int64_t j = st.count; j < stride; ++j) {
+            a->out_ids[i * stride + j] = 0xFFFFFFFFu;
+            a->out_d[i * stride + j] = INFINITY;
         }
         if (a->out_cmps) a->out_cmps[i] = st.cmps;
         if (a->out_hops) a->out_hops[i] = st.hops;
         if (a->lat_us) a->lat_us[i] = (now_s() - t0) * 1e6;
     }
-    scratch_free(&s);
+    if (!a->Ls) scratch_free(&s);
     return NULL;
 }
 
-double dko_search_batch(const dko_index *ix, int mode, const float *queries, int64_t nq,
-                        int64_t qstride, int64_t k, int64_t L, int64_t bw, int nthreads,
+static double run_batch(const dko_index *ix, int mode, const float *queries, int64_t nq,
+                        int64_t qstride, int64_t k, int64_t L, const uint32_t *ks,
+                        const uint32_t *Ls, int64_t kstride, int64_t bw, int nthreads,
                         uint32_t *out_ids, float *out_d, uint32_t *out_cmps,
                         uint32_t *out_hops, double *lat_us) {
-    if (dko_knn_validate(k, L, bw)) return -1.0;
     if (nthreads < 1) nthreads = 1;
     if (nthreads > nq) nthreads = (int)(nq > 0 ? nq : 1);
     pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
@@ -1387,7 +1395,7 @@ double dko_search_batch(const dko_index *ix, int mode, const float *queries, int
         int64_t lo = t * base + (t < extra ? t : extra);
         int64_t hi = lo + base + (t < extra ? 1 : 0);
         batch_arg a = {ix, mode, queries, qstride, k, L, bw, lo, hi,
-                       out_ids, out_cmps, out_hops, out_d, lat_us};
+                       out_ids, out_cmps, out_hops, out_d, lat_us, ks, Ls, kstride};
         args[t] = a;
         pthread_create(&th[t], NULL, batch_worker, &args[t]);
     }
@@ -1396,6 +1404,26 @@ double dko_search_batch(const dko_index *ix, int mode, const float *queries, int
     free(th);
     free(args);
     return wall;
+}
+
+double dko_search_batch(const dko_index *ix, int mode, const float *queries, int64_t nq,
+                        int64_t qstride, int64_t k, int64_t L, int64_t bw, int nthreads,
+                        uint32_t *out_ids, float *out_d, uint32_t *out_cmps,
+                        uint32_t *out_hops, double *lat_us) {
+    if (dko_knn_validate(k, L, bw)) return -1.0;
+    return run_batch(ix, mode, queries, nq, qstride, k, L, NULL, NULL, k, bw, nthreads,
+                     out_ids, out_d, out_cmps, out_hops, lat_us);
+}
+
+double dko_search_batch_mixed(const dko_index *ix, int mode, const float *queries, int64_t nq,
+                              int64_t qstride, const uint32_t *ks, const uint32_t *Ls,
+                              int64_t kstride, int64_t bw, int nthreads, uint32_t *out_ids,
+                              float *out_d, uint32_t *out_cmps, uint32_t *out_hops,
+                              double *lat_us) {
+    for (int64_t i = 0; i < nq; ++i)
+        if (dko_knn_validate(ks[i], Ls[i], bw) || (int64_t)ks[i] > kstride) return -1.0;
+    return run_batch(ix, mode, queries, nq, qstride, 0, 0, ks, Ls, kstride, bw, nthreads,
+                     out_ids, out_d, out_cmps, out_hops, lat_us);
 }
 
 /* ------------------------------------------------------------------------------------ */

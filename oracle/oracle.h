@@ -187,6 +187,13 @@ double dko_search_batch(const dko_index *ix, int mode, const float *queries, int
                         int64_t qstride, int64_t k, int64_t L, int64_t bw, int nthreads,
                         uint32_t *out_ids, float *out_d, uint32_t *out_cmps,
                         uint32_t *out_hops, double *lat_us);
+/* Same with a k and an L per query (uneven requests in one batch); outputs have row
+ * stride kstride >= every k. Returns -1 when any (k, L) fails Knn validation. */
+double dko_search_batch_mixed(const dko_index *ix, int mode, const float *queries, int64_t nq,
+                              int64_t qstride, const uint32_t *ks, const uint32_t *Ls,
+                              int64_t kstride, int64_t bw, int nthreads, uint32_t *out_ids,
+                              float *out_d, uint32_t *out_cmps, uint32_t *out_hops,
+                              double *lat_us);
 
 /* Build (RobustPrune / insert / multi_insert) ---------------------------------------- */
 typedef struct {

@@ -131,6 +131,9 @@ def _setup(L):
     L.dko_search_batch.argtypes = [C.POINTER(Index), C.c_int, P, I64, I64, I64, I64, I64,
                                    C.c_int, P, P, P, P, P]
     L.dko_search_batch.restype = C.c_double
+    L.dko_search_batch_mixed.argtypes = [C.POINTER(Index), C.c_int, P, I64, I64, P, P, I64,
+                                         I64, C.c_int, P, P, P, P, P]
+    L.dko_search_batch_mixed.restype = C.c_double
     L.dko_occlude_list.argtypes = [C.POINTER(BuildCfg), P, I64, I64, P, P, I64, C.c_uint32, P]
     L.dko_occlude_list.restype = I64
     L.dko_update_occlude_factor.argtypes = [C.c_int, F, F, F, F]
@@ -503,6 +506,25 @@ class OracleIndex:
         if wall < 0:
             raise ValueError("knn validation error")
         return ids, ds, cm, hp, wall, lat
+
+    def search_batch_mixed(self, queries, ks, Ls, bw=1, mode=MODE_FULL, nthreads=1):
+        """One k and one L per query; rows of ids / ds have max(ks) entries (unused
+        entries are 0xFFFFFFFF / inf)."""
+        queries = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = queries.shape[0]
+        ks = np.ascontiguousarray(ks, dtype=np.uint32)
+        Ls = np.ascontiguousarray(Ls, dtype=np.uint32)
+        kmax = int(ks.max()) if nq else 1
+        ids = np.zeros((nq, kmax), dtype=np.uint32)
+        ds = np.zeros((nq, kmax), dtype=np.float32)
+        cm = np.zeros(nq, dtype=np.uint32)
+        hp = np.zeros(nq, dtype=np.uint32)
+        wall = lib().dko_search_batch_mixed(C.byref(self.s), mode, _p(queries), nq,
+                                            queries.shape[1], _p(ks), _p(Ls), kmax, bw,
+                                            nthreads, _p(ids), _p(ds), _p(cm), _p(hp), None)
+        if wall < 0:
+            raise ValueError("knn validation error")
+        return ids, ds, cm, hp, wall
 
 
 def build_graph(metric, vectors, R, R_max, L_build, alpha, starts, ids, batch=1,

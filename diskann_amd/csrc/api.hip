@@ -2499,15 +2499,45 @@ struct dkm_prescorer {
     int metric = 0;
     int64_t n = 0, dim = 0, ns = 0;
     int dp = 0;
+    bool fp8 = false;                // FP8 (E4M3) operands instead of bf16
+    float sx = 1.0f;                 // FP8: power-of-two scale of the rows
     const float *x = nullptr;        // caller's f32 rows (rescore)
-    DevBuf<uint16_t> xf, sf;         // bf16 fragment layout: rows, sample rows
+    DevBuf<uint16_t> xf, sf;         // fragment layout (bf16 or fp8): rows, sample rows
     DevBuf<float> xn, sn;            // f32 squared norms (L2 only)
     std::mutex mu;
 };
 
 namespace {
 constexpr int64_t kPsBatch = 1024;  // queries per filter launch (bounds the LDS stage load)
+
+// Power of two that brings the largest magnitude m to at most 448 (the largest finite E4M3)
+// while using as much of the range as possible; exact to undo.
+float fp8_scale(float m) {
+    if (!(m > 0.0f) || !std::isfinite(m)) return 1.0f;
+    const int e = std::max(-100, std::min(100, (int)std::floor(std::log2(448.0 / (double)m))));
+    return std::ldexp(1.0f, e);
 }
+
+hipError_t absmax_host(const float *x, int64_t rows, int dim, float *m) {
+    DevBuf<uint32_t> d;
+    hipError_t e = d.ensure(1);
+    if (e == hipSuccess) e = dkm::ps_absmax(x, rows, dim, dim, d.p, nullptr);
+    uint32_t bits = 0;
+    if (e == hipSuccess) e = hipMemcpy(&bits, d.p, sizeof(bits), hipMemcpyDeviceToHost);
+    float f;
+    std::memcpy(&f, &bits, sizeof(f));
+    *m = f;
+    return e;
+}
+
+// Rows in the operand layout of the prescorer's precision (scale: FP8 only).
+hipError_t ps_frag(bool fp8, const float *x, int64_t rows, int dim, int dp, int64_t rows_pad,
+                   float scale, uint16_t *out) {
+    if (fp8)
+        return dkm::ps_to_fp8_frag(x, rows, dim, dim, dp, rows_pad, scale, (uint8_t *)out, nullptr);
+    return dkm::kmeans_to_bf16_frag(x, rows, dim, dim, dp, rows_pad, out, nullptr);
+}
+}  // namespace
 
 dkm_status dkm_prescore_create(const float *d_x, int64_t n, int64_t dim, int metric,
                                int precision, dkm_prescorer **out) {
@@ -2517,10 +2547,12 @@ dkm_status dkm_prescore_create(const float *d_x, int64_t n, int64_t dim, int met
     if (metric != DKM_L2 && metric != DKM_INNER_PRODUCT && metric != DKM_COSINE_NORMALIZED)
         return fail(DKM_ERR_INVALID_ARG,
                     "pre-scoring supports L2, inner product and cosine-normalized (got %d)", metric);
-    if (precision != DKM_PRECISION_BF16)
+    if (precision != DKM_PRECISION_BF16 && precision != DKM_PRECISION_FP8)
         return fail(DKM_ERR_INVALID_ARG, "unknown pre-scoring precision %d", precision);
-    const int dp = dkm::kmeans_bf16_dp((int)dim);
-    if (!dkm::ps_supported_dp(dp))
+    const bool fp8 = precision == DKM_PRECISION_FP8;
+    const int dp = fp8 ? (dim <= 768 ? dkm::ps_fp8_dp((int)dim) : -1)
+                       : dkm::kmeans_bf16_dp((int)dim);
+    if (fp8 ? dp < 0 : !dkm::ps_supported_dp(dp))
         return fail(DKM_ERR_INVALID_ARG, "pre-scoring supports dims up to 768 (got %lld)",
                     (long long)dim);
     HIP_TRY(hipSetDevice(g_device));
@@ -2529,6 +2561,7 @@ dkm_status dkm_prescore_create(const float *d_x, int64_t n, int64_t dim, int met
     ps->n = n;
     ps->dim = dim;
     ps->dp = dp;
+    ps->fp8 = fp8;
     ps->x = d_x;
     ps->ns = std::min<int64_t>(n, PS_SAMPLE_MAX);
     const bool l2 = metric == DKM_L2;
@@ -2538,9 +2571,14 @@ dkm_status dkm_prescore_create(const float *d_x, int64_t n, int64_t dim, int met
     };
     hipError_t e;
     const int64_t npad = (n + 255) / 256 * 256, spad = (ps->ns + 255) / 256 * 256;
-    if ((e = ps->xf.ensure(npad * dp)) != hipSuccess) return bad(e);
-    if ((e = dkm::kmeans_to_bf16_frag(d_x, n, (int)dim, dim, dp, npad, ps->xf.p, nullptr)) !=
-        hipSuccess)
+    if (fp8) {
+        float m = 0.0f;
+        if ((e = absmax_host(d_x, n, (int)dim, &m)) != hipSuccess) return bad(e);
+        ps->sx = fp8_scale(m);
+    }
+    const int64_t per_row = fp8 ? dp / 2 : dp;  // 16-bit units per row
+    if ((e = ps->xf.ensure(npad * per_row)) != hipSuccess) return bad(e);
+    if ((e = ps_frag(fp8, d_x, n, (int)dim, dp, npad, ps->sx, ps->xf.p)) != hipSuccess)
         return bad(e);
     // the sample: ns distinct rows drawn with a fixed seed (SplitMix64), in increasing order
     std::vector<int64_t> rows;
@@ -2572,9 +2610,8 @@ dkm_status dkm_prescore_create(const float *d_x, int64_t n, int64_t dim, int met
         return bad(e);
     if ((e = dkm::ps_gather_rows(d_x, (int)dim, drows.p, ps->ns, srows.p, nullptr)) != hipSuccess)
         return bad(e);
-    if ((e = ps->sf.ensure(spad * dp)) != hipSuccess) return bad(e);
-    if ((e = dkm::kmeans_to_bf16_frag(srows.p, ps->ns, (int)dim, dim, dp, spad, ps->sf.p,
-                                      nullptr)) != hipSuccess)
+    if ((e = ps->sf.ensure(spad * per_row)) != hipSuccess) return bad(e);
+    if ((e = ps_frag(fp8, srows.p, ps->ns, (int)dim, dp, spad, ps->sx, ps->sf.p)) != hipSuccess)
         return bad(e);
     if (l2) {
         if ((e = ps->xn.ensure(n)) != hipSuccess) return bad(e);
@@ -2612,7 +2649,7 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
     const bool l2 = ps->metric == DKM_L2;
     const int64_t B = std::min<int64_t>(nq, kPsBatch);
     const int64_t Bp = (B + 63) / 64 * 64;
-    if (dkm::ps_filter_lds(dp, (int)Bp) > 160 * 1024)
+    if (dkm::ps_filter_lds(dp, ps->fp8, (int)Bp) > 160 * 1024)
         return fail(DKM_ERR_INVALID_ARG, "pre-scoring tile does not fit LDS");
     // Threshold ranks in the sample: j0 aims at ~1.5 C passing rows but is at least 15 (an
     // order statistic of rank < 16 scatters too widely); retries use 3x (short lists) and
@@ -2668,10 +2705,19 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
         const int bp = (int)((b + 63) / 64 * 64);
         const float *q = d_q + q0 * ps->dim;
         HIP_TRY(hipEventRecord(ev[0], nullptr));
-        HIP_TRY(dkm::kmeans_to_bf16_frag(q, b, (int)ps->dim, ps->dim, dp, bp, qf.p, nullptr));
+        // FP8: the batch's own power-of-two scale; the accumulators hold s times the dot
+        // products (s = row scale x query scale)
+        float sq = 1.0f;
+        if (ps->fp8) {
+            float m = 0.0f;
+            HIP_TRY(absmax_host(q, b, (int)ps->dim, &m));
+            sq = fp8_scale(m);
+        }
+        const float S = ps->fp8 ? ps->sx * sq : 1.0f;
+        HIP_TRY(ps_frag(ps->fp8, q, b, (int)ps->dim, dp, bp, sq, qf.p));
         if (!all_pass) {
-            HIP_TRY(dkm::ps_dense(dp, ps->sf.p, ps->ns, ps->sn.p, qf.p, b, bp, l2, dense.p, ps->ns,
-                                  nullptr));
+            HIP_TRY(dkm::ps_dense(dp, ps->fp8, ps->sf.p, ps->ns, ps->sn.p, qf.p, b, bp, l2,
+                                  1.0f / S, dense.p, ps->ns, nullptr));
             HIP_TRY(dkm::ps_select(dense.p, ps->ns, (int)ps->ns, j0, j_loose, j_tight, tau.p, b,
                                    nullptr));
             HIP_TRY(hipMemcpy(htau.data(), tau.p, sizeof(float) * 3 * b, hipMemcpyDeviceToHost));
@@ -2680,8 +2726,8 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
             HIP_TRY(hipMemcpy(tau.p, htau.data(), sizeof(float) * b, hipMemcpyHostToDevice));
         }
         HIP_TRY(hipMemset(cnt.p, 0, sizeof(uint32_t) * b));
-        HIP_TRY(dkm::ps_filter(dp, ps->xf.p, ps->n, ps->xn.p, qf.p, b, bp, l2, tau.p, nullptr,
-                               cnt.p, lists.p, cap, nullptr));
+        HIP_TRY(dkm::ps_filter(dp, ps->fp8, ps->xf.p, ps->n, ps->xn.p, qf.p, b, bp, l2, S, tau.p,
+                               nullptr, cnt.p, lists.p, cap, nullptr));
         // Redo the queries whose list is short (looser threshold, then every row) or overflowed
         // (tighter threshold): only those queries, gathered into a small launch of their own.
         for (int round = 0;; ++round) {
@@ -2717,10 +2763,9 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
             HIP_TRY(hipMemcpy(r_row.p, rrow.data(), sizeof(int64_t) * f, hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(r_tau.p, rtau.data(), sizeof(float) * f, hipMemcpyHostToDevice));
             HIP_TRY(dkm::ps_gather_rows(q, (int)ps->dim, r_row.p, f, r_q.p, nullptr));
-            HIP_TRY(dkm::kmeans_to_bf16_frag(r_q.p, f, (int)ps->dim, ps->dim, dp, fp, r_qf.p,
-                                             nullptr));
-            HIP_TRY(dkm::ps_filter(dp, ps->xf.p, ps->n, ps->xn.p, r_qf.p, f, fp, l2, r_tau.p,
-                                   r_map.p, cnt.p, lists.p, cap, nullptr));
+            HIP_TRY(ps_frag(ps->fp8, r_q.p, f, (int)ps->dim, dp, fp, sq, r_qf.p));
+            HIP_TRY(dkm::ps_filter(dp, ps->fp8, ps->xf.p, ps->n, ps->xn.p, r_qf.p, f, fp, l2, S,
+                                   r_tau.p, r_map.p, cnt.p, lists.p, cap, nullptr));
         }
         HIP_TRY(hipEventRecord(ev[1], nullptr));
         HIP_TRY(dkm::ps_bounds(cnt.p, b, cap, bnd_b.p, bnd_e.p, nullptr));

@@ -115,4 +115,115 @@ __device__ __forceinline__ void res_sweep(uint16_t *sm, const uint16_t *cf, int 
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// FP8 (E4M3, OCP) operands for v_mfma_scale_f32_32x32x64_f8f6f4 with unit scales.
+//
+// FP8 fragment layout: 16-byte chunk ((t * KB + b) * 64 + l) holds row 32 t + l % 32, dims
+// 32 b + 16 (l / 32) .. + 15, one byte each; KB = padded dimension / 32, so a 1 KB block
+// again holds 32 rows, and the same prefetch (res_prefetch<KB>) streams a column tile. One
+// MFMA k-step covers 64 dims: lane l's 32-byte operand is its chunk of block 2 s followed
+// by its chunk of block 2 s + 1. Row and column operands use this same assignment of dims
+// to (lane half, byte), so every k-step contracts the same 64 dims of both, whatever order
+// the matrix core sums them in. The accumulator layout is the one of the bf16 MFMA above.
+// ---------------------------------------------------------------------------------------
+typedef __attribute__((__vector_size__(8 * sizeof(int)))) int i32x8_t;
+typedef __attribute__((__vector_size__(4 * sizeof(int)))) int i32x4_t;
+
+__device__ __forceinline__ floatx16_t mfma_f8(const i32x8_t &col, const i32x8_t &row,
+                                              const floatx16_t &acc) {
+    // formats 0 / 0: E4M3 for both operands; E8M0 scales 127 = 2^0
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(col, row, acc, 0, 0, 0, 127, 0, 127);
+}
+
+__device__ __forceinline__ i32x8_t f8_join(const i32x4_t &lo, const i32x4_t &hi) {
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+// Column fragments of one tile from LDS (buf = tile + 8 lane, in 16-bit units), all KB / 2
+// k-steps against the two register-resident row tiles.
+template <int KB>
+__device__ __forceinline__ void f8_tile(const i32x8_t *a0, const i32x8_t *a1,
+                                        const uint16_t *buf, floatx16_t &acc0,
+                                        floatx16_t &acc1) {
+    static_assert(KB % 2 == 0, "k-steps of 64 dims");
+#pragma unroll
+    for (int s = 0; s < KB / 2; ++s) {
+        const i32x8_t b = f8_join(*(const i32x4_t *)(buf + (2 * s) * 512),
+                                  *(const i32x4_t *)(buf + (2 * s + 1) * 512));
+        acc0 = mfma_f8(b, a0[s], acc0);
+        acc1 = mfma_f8(b, a1[s], acc1);
+    }
+}
+
+// Operand policies for the pre-scoring GEMMs: kBlocks 1 KB blocks per 32-row tile, the
+// wave's two row tiles held in registers, and the sweep over the column tiles.
+template <int KS>
+struct Bf16Rows {
+    static constexpr int kBlocks = KS;
+    bf16x8_t a0[KS], a1[KS];
+    __device__ __forceinline__ void load(const uint16_t *ap) { res_load_rows<KS>(ap, a0, a1); }
+    template <class Init, class Epi>
+    __device__ __forceinline__ void sweep(uint16_t *sm, const uint16_t *cf, int ntiles, int wave,
+                                          int lane, Init init, Epi epi) {
+        res_sweep<KS>(sm, cf, ntiles, wave, lane, a0, a1, init, epi);
+    }
+};
+
+template <int KB>
+struct Fp8Rows {
+    static constexpr int kBlocks = KB;
+    static_assert(KB % 4 == 0, "a column tile is prefetched four blocks per wave");
+    i32x8_t a0[KB / 2], a1[KB / 2];
+    __device__ __forceinline__ void load(const uint16_t *ap) {
+#pragma unroll
+        for (int s = 0; s < KB / 2; ++s) {
+            a0[s] = f8_join(*(const i32x4_t *)(ap + (2 * s) * 512),
+                            *(const i32x4_t *)(ap + (2 * s + 1) * 512));
+            a1[s] = f8_join(*(const i32x4_t *)(ap + (KB + 2 * s) * 512),
+                            *(const i32x4_t *)(ap + (KB + 2 * s + 1) * 512));
+        }
+#pragma unroll
+        for (int s = 0; s < KB / 2; ++s) asm volatile("" : "+a"(a0[s]));
+    }
+    template <class Init, class Epi>
+    __device__ __forceinline__ void sweep(uint16_t *sm, const uint16_t *cf, int ntiles, int wave,
+                                          int lane, Init init, Epi epi) {
+        auto src = [&](int t) { return cf + (int64_t)t * KB * 512 + lane * 8; };
+        floatx16_t pa, pb;
+        res_prefetch<KB>(src(0), sm, wave);
+        for (int t = 0; t < ntiles; ++t) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            uint16_t *cur = sm + (t & 1) * KB * 512;
+            init(t, pa, pb);
+            if (t + 1 < ntiles) res_prefetch<KB>(src(t + 1), sm + ((t + 1) & 1) * KB * 512, wave);
+            f8_tile<KB>(a0, a1, cur + lane * 8, pa, pb);
+            epi(t, pa, pb);
+        }
+    }
+};
+
+// f32 -> FP8 E4M3 (OCP: bias 7, no infinities, 448 largest finite), round to nearest even;
+// magnitudes that round above 448 saturate to 448, NaN stays NaN.
+__device__ inline uint8_t to_e4m3(float f) {
+    const uint32_t u = __float_as_uint(f);
+    const uint8_t sign = (uint8_t)((u >> 24) & 0x80u);
+    const float a = fabsf(f);
+    if (a != a) return (uint8_t)(sign | 0x7Fu);
+    if (a >= 464.0f) return (uint8_t)(sign | 0x7Eu);
+    if (a < 0.015625f) {  // below 2^-6: subnormal steps of 2^-9 (8 rounds up to 2^-6)
+        return (uint8_t)(sign | (uint32_t)__builtin_rintf(a * 512.0f));
+    }
+    const uint32_t au = __float_as_uint(a);
+    int e = (int)(au >> 23) - 127;
+    uint32_t keep = (au >> 20) & 7u;
+    const uint32_t rest = au & 0xFFFFFu;
+    if (rest > 0x80000u || (rest == 0x80000u && (keep & 1u))) ++keep;
+    if (keep == 8u) {
+        keep = 0;
+        ++e;
+    }
+    return (uint8_t)(sign | (uint32_t)((e + 7) << 3) | keep);
+}
+
 }  // namespace dkm

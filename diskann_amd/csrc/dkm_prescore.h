@@ -1,4 +1,4 @@
-// Candidate pre-scoring (BF16 MFMA approximate distances, exact FP32 rescore).
+// Candidate pre-scoring (BF16 or FP8 MFMA approximate distances, exact FP32 rescore).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
@@ -12,15 +12,22 @@
 
 namespace dkm {
 bool ps_supported_dp(int dp);
-size_t ps_filter_lds(int dp, int nqp);
-hipError_t ps_dense(int dp, const uint16_t *xf, int64_t ns, const float *xn, const uint16_t *qf,
-                    int nq, int nqp, bool l2, float *out, int64_t ld, hipStream_t st);
+// Padded dimension of the FP8 operands (a multiple of 128 up to 768), -1 when dim > 768.
+int ps_fp8_dp(int dim);
+size_t ps_filter_lds(int dp, bool fp8, int nqp);
+hipError_t ps_dense(int dp, bool fp8, const uint16_t *xf, int64_t ns, const float *xn,
+                    const uint16_t *qf, int nq, int nqp, bool l2, float inv_s, float *out,
+                    int64_t ld, hipStream_t st);
 hipError_t ps_select(const float *dense, int64_t ld, int ns, int j0, int j1, int j2, float *tau,
                      int nq, hipStream_t st);
-hipError_t ps_filter(int dp, const uint16_t *xf, int64_t n, const float *xn, const uint16_t *qf,
-                     int nq, int nqp, bool l2, const float *tau, const uint32_t *qmap,
-                     uint32_t *count, uint64_t *lists,
-                     int64_t cap, hipStream_t st);
+hipError_t ps_filter(int dp, bool fp8, const uint16_t *xf, int64_t n, const float *xn,
+                     const uint16_t *qf, int nq, int nqp, bool l2, float s, const float *tau,
+                     const uint32_t *qmap, uint32_t *count, uint64_t *lists, int64_t cap,
+                     hipStream_t st);
+hipError_t ps_to_fp8_frag(const float *x, int64_t rows, int dim, int64_t stride, int dp,
+                          int64_t rows_pad, float scale, uint8_t *out, hipStream_t st);
+hipError_t ps_absmax(const float *x, int64_t rows, int dim, int64_t stride, uint32_t *out,
+                     hipStream_t st);
 hipError_t ps_norms(const float *x, int64_t n, int dim, const int64_t *rows, float *out,
                     hipStream_t st);
 hipError_t ps_gather_rows(const float *x, int dim, const int64_t *rows, int64_t ns, float *out,

@@ -44,23 +44,24 @@ __device__ __forceinline__ float approx_dist(bool l2, float dot, float xn) {
 }
 
 // Dense scores of the sample rows against a block of queries: out[q * ld + row] for
-// row < ns, q < nq. xf: sample rows (fragment layout, padded to 256), qf: queries (fragment
-// layout, padded to a multiple of 64), xn: row norms (L2 only).
-template <int KS>
+// row < ns, q < nq. xf: sample rows (fragment layout of the operand policy G, padded to
+// 256), qf: queries (same layout, padded to a multiple of 64), xn: row norms (L2 only).
+// inv_s undoes the power-of-two operand scales (1 for bf16).
+template <class G>
 __global__ __launch_bounds__(256, 1) void ps_dense_kernel(const uint16_t *xf, int64_t ns,
                                                           const float *xn, const uint16_t *qf,
-                                                          int nq, int nqp, bool l2, float *out,
-                                                          int64_t ld) {
+                                                          int nq, int nqp, bool l2, float inv_s,
+                                                          float *out, int64_t ld) {
     extern __shared__ __align__(16) uint16_t sm[];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int half = lane >> 5, l32 = lane & 31;
     const int64_t rtile = (int64_t)blockIdx.x * 8 + 2 * wave;
-    bf16x8_t a0[KS], a1[KS];
-    res_load_rows<KS>(xf + rtile * KS * 512 + lane * 8, a0, a1);
+    G g;
+    g.load(xf + rtile * G::kBlocks * 512 + lane * 8);
     const int64_t r0 = 32 * rtile + l32, r1 = r0 + 32;
     const float n0 = l2 && r0 < ns ? xn[r0] : 0.0f, n1 = l2 && r1 < ns ? xn[r1] : 0.0f;
-    res_sweep<KS>(
-        sm, qf, nqp / 32, wave, lane, a0, a1,
+    g.sweep(
+        sm, qf, nqp / 32, wave, lane,
         [&](int, floatx16_t &pa, floatx16_t &pb) {
             pa = floatx16_t{};
             pb = floatx16_t{};
@@ -70,8 +71,8 @@ __global__ __launch_bounds__(256, 1) void ps_dense_kernel(const uint16_t *xf, in
             for (int r = 0; r < 16; ++r) {
                 const int q = 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
                 if (q < nq) {
-                    if (r0 < ns) out[(int64_t)q * ld + r0] = approx_dist(l2, pa[r], n0);
-                    if (r1 < ns) out[(int64_t)q * ld + r1] = approx_dist(l2, pb[r], n1);
+                    if (r0 < ns) out[(int64_t)q * ld + r0] = approx_dist(l2, pa[r] * inv_s, n0);
+                    if (r1 < ns) out[(int64_t)q * ld + r1] = approx_dist(l2, pb[r] * inv_s, n1);
                 }
             }
         });
@@ -135,31 +136,33 @@ __device__ __forceinline__ void ps_append(uint32_t *count, uint64_t *lists, int6
     if (pos < cap) lists[(int64_t)q * cap + pos] = key;
 }
 
-template <int KS>
+// s: the product of the two operands' power-of-two scales (1 for bf16); the accumulators
+// then hold s times the dot products, and tau and the norm terms are scaled to match.
+template <class G>
 __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, int64_t n,
                                                            const float *xn, const uint16_t *qf,
-                                                           int nq, int nqp, bool l2,
+                                                           int nq, int nqp, bool l2, float s,
                                                            const float *tau, const uint32_t *qmap,
                                                            uint32_t *count, uint64_t *lists,
                                                            int64_t cap) {
     extern __shared__ __align__(16) uint16_t sm[];
-    float *tau_s = (float *)(sm + 2 * KS * 512);
+    float *tau_s = (float *)(sm + 2 * G::kBlocks * 512);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int half = lane >> 5, l32 = lane & 31;
     uint64_t *skey = (uint64_t *)(tau_s + nqp) + wave * kPool;
     uint16_t *sq = (uint16_t *)((uint64_t *)(tau_s + nqp) + 4 * kPool) + wave * kPool;
     const int64_t rtile = (int64_t)blockIdx.x * 8 + 2 * wave;
-    bf16x8_t a0[KS], a1[KS];
-    res_load_rows<KS>(xf + rtile * KS * 512 + lane * 8, a0, a1);
+    G g;
+    g.load(xf + rtile * G::kBlocks * 512 + lane * 8);
     for (int c = tid; c < nqp; c += 256) tau_s[c] = c < nq ? tau[c] : -__builtin_inff();
     const int64_t r0 = 32 * rtile + l32, r1 = r0 + 32;
     // the accumulators start at tau (L2: tau / 2), so after the tile they hold tau - d
     // (L2: (tau - d + |x|^2) / 2): a pair passes when that, less the row's |x|^2 / 2 (L2),
     // is >= 0, and its negation orders a query's entries like d
-    const float h0 = l2 && r0 < n ? 0.5f * xn[r0] : 0.0f;
-    const float h1 = l2 && r1 < n ? 0.5f * xn[r1] : 0.0f;
+    const float h0 = l2 && r0 < n ? 0.5f * s * xn[r0] : 0.0f;
+    const float h1 = l2 && r1 < n ? 0.5f * s * xn[r1] : 0.0f;
     const bool v0 = r0 < n, v1 = r1 < n;
-    const float tscale = l2 ? 0.5f : 1.0f;
+    const float tscale = (l2 ? 0.5f : 1.0f) * s;
     uint32_t filled = 0;  // the wave's pool entries (wave-uniform)
     auto push = [&](bool pass, int q, uint64_t key) {
         const uint64_t m = __ballot(pass);
@@ -178,16 +181,16 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
             filled += (uint32_t)__popcll(m);
         }
     };
-    res_sweep<KS>(
-        sm, qf, nqp / 32, wave, lane, a0, a1,
+    g.sweep(
+        sm, qf, nqp / 32, wave, lane,
         [&](int t, floatx16_t &pa, floatx16_t &pb) {
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const float4 t4 = *(const float4 *)(tau_s + 32 * t + 8 * g + 4 * half);
-                pa[4 * g + 0] = tscale * t4.x;
-                pa[4 * g + 1] = tscale * t4.y;
-                pa[4 * g + 2] = tscale * t4.z;
-                pa[4 * g + 3] = tscale * t4.w;
+            for (int j = 0; j < 4; ++j) {
+                const float4 t4 = *(const float4 *)(tau_s + 32 * t + 8 * j + 4 * half);
+                pa[4 * j + 0] = tscale * t4.x;
+                pa[4 * j + 1] = tscale * t4.y;
+                pa[4 * j + 2] = tscale * t4.z;
+                pa[4 * j + 3] = tscale * t4.w;
             }
             pb = pa;
         },
@@ -275,44 +278,57 @@ __global__ __launch_bounds__(256) void ps_output_kernel(const uint64_t *sorted, 
 
 static unsigned blocks(int64_t n, int64_t per) { return (unsigned)((n + per - 1) / per); }
 
-template <int KS>
+template <class G>
 static hipError_t dense_t(const uint16_t *xf, int64_t ns, const float *xn, const uint16_t *qf,
-                          int nq, int nqp, bool l2, float *out, int64_t ld, hipStream_t st) {
-    const size_t lds = (size_t)2 * KS * 1024;
-    hipError_t e = hipFuncSetAttribute((const void *)ps_dense_kernel<KS>,
+                          int nq, int nqp, bool l2, float inv_s, float *out, int64_t ld,
+                          hipStream_t st) {
+    const size_t lds = (size_t)2 * G::kBlocks * 1024;
+    hipError_t e = hipFuncSetAttribute((const void *)ps_dense_kernel<G>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(ps_dense_kernel<KS>, dim3(blocks(ns, 256)), dim3(256), lds, st, xf, ns, xn,
-                       qf, nq, nqp, l2, out, ld);
+    hipLaunchKernelGGL(ps_dense_kernel<G>, dim3(blocks(ns, 256)), dim3(256), lds, st, xf, ns, xn,
+                       qf, nq, nqp, l2, inv_s, out, ld);
     return hipGetLastError();
 }
 
-template <int KS>
+template <class G>
 static hipError_t filter_t(const uint16_t *xf, int64_t n, const float *xn, const uint16_t *qf,
-                           int nq, int nqp, bool l2, const float *tau, const uint32_t *qmap,
-                           uint32_t *count, uint64_t *lists, int64_t cap, hipStream_t st) {
-    const size_t lds = ps_filter_lds(KS * 16, nqp);
-    hipError_t e = hipFuncSetAttribute((const void *)ps_filter_kernel<KS>,
+                           int nq, int nqp, bool l2, float s, const float *tau,
+                           const uint32_t *qmap, uint32_t *count, uint64_t *lists, int64_t cap,
+                           hipStream_t st) {
+    const size_t lds = (size_t)2 * G::kBlocks * 1024 + 4 * (size_t)nqp + (size_t)4 * kPool * 10;
+    hipError_t e = hipFuncSetAttribute((const void *)ps_filter_kernel<G>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(ps_filter_kernel<KS>, dim3(blocks(n, 256)), dim3(256), lds, st, xf, n, xn,
-                       qf, nq, nqp, l2, tau, qmap, count, lists, cap);
+    hipLaunchKernelGGL(ps_filter_kernel<G>, dim3(blocks(n, 256)), dim3(256), lds, st, xf, n, xn,
+                       qf, nq, nqp, l2, s, tau, qmap, count, lists, cap);
     return hipGetLastError();
 }
 
-#define PS_DISPATCH(KS_, CALL)                         \
-    switch (KS_) {                                     \
-    case 4: return CALL(4);                            \
-    case 8: return CALL(8);                            \
-    case 16: return CALL(16);                          \
-    case 24: return CALL(24);                          \
-    case 32: return CALL(32);                          \
-    case 48: return CALL(48);                          \
+// bf16: KS = dp / 16 k-steps of 16 dims; fp8: KB = dp / 32 blocks (k-steps of 64 dims).
+#define PS_DISPATCH(FP8_, DP_, CALL)                   \
+    if (FP8_) {                                        \
+        switch ((DP_) / 32) {                          \
+        case 4: return CALL(Fp8Rows<4>);               \
+        case 8: return CALL(Fp8Rows<8>);               \
+        case 12: return CALL(Fp8Rows<12>);             \
+        case 16: return CALL(Fp8Rows<16>);             \
+        case 24: return CALL(Fp8Rows<24>);             \
+        default: return hipErrorInvalidValue;          \
+        }                                              \
+    }                                                  \
+    switch ((DP_) / 16) {                              \
+    case 4: return CALL(Bf16Rows<4>);                  \
+    case 8: return CALL(Bf16Rows<8>);                  \
+    case 16: return CALL(Bf16Rows<16>);                \
+    case 24: return CALL(Bf16Rows<24>);                \
+    case 32: return CALL(Bf16Rows<32>);                \
+    case 48: return CALL(Bf16Rows<48>);                \
     default: return hipErrorInvalidValue;              \
     }
 
-size_t ps_filter_lds(int dp, int nqp) {
-    return (size_t)2 * dp * 64 + 4 * (size_t)nqp + (size_t)4 * kPool * 10;
+size_t ps_filter_lds(int dp, bool fp8, int nqp) {
+    return (size_t)(fp8 ? 1 : 2) * dp * 64 + 4 * (size_t)nqp + (size_t)4 * kPool * 10;
 }
 
 bool ps_supported_dp(int dp) {
@@ -320,19 +336,87 @@ bool ps_supported_dp(int dp) {
     return dp % 16 == 0 && (ks == 4 || ks == 8 || ks == 16 || ks == 24 || ks == 32 || ks == 48);
 }
 
-hipError_t ps_dense(int dp, const uint16_t *xf, int64_t ns, const float *xn, const uint16_t *qf,
-                    int nq, int nqp, bool l2, float *out, int64_t ld, hipStream_t st) {
-#define CALL(K) dense_t<K>(xf, ns, xn, qf, nq, nqp, l2, out, ld, st)
-    PS_DISPATCH(dp / 16, CALL)
+int ps_fp8_dp(int dim) {
+    for (int dp : {128, 256, 384, 512, 768})
+        if (dp >= dim) return dp;
+    return -1;
+}
+
+hipError_t ps_dense(int dp, bool fp8, const uint16_t *xf, int64_t ns, const float *xn,
+                    const uint16_t *qf, int nq, int nqp, bool l2, float inv_s, float *out,
+                    int64_t ld, hipStream_t st) {
+#define CALL(G) dense_t<G>(xf, ns, xn, qf, nq, nqp, l2, inv_s, out, ld, st)
+    PS_DISPATCH(fp8, dp, CALL)
 #undef CALL
 }
 
-hipError_t ps_filter(int dp, const uint16_t *xf, int64_t n, const float *xn, const uint16_t *qf,
-                     int nq, int nqp, bool l2, const float *tau, const uint32_t *qmap,
-                     uint32_t *count, uint64_t *lists, int64_t cap, hipStream_t st) {
-#define CALL(K) filter_t<K>(xf, n, xn, qf, nq, nqp, l2, tau, qmap, count, lists, cap, st)
-    PS_DISPATCH(dp / 16, CALL)
+hipError_t ps_filter(int dp, bool fp8, const uint16_t *xf, int64_t n, const float *xn,
+                     const uint16_t *qf, int nq, int nqp, bool l2, float s, const float *tau,
+                     const uint32_t *qmap, uint32_t *count, uint64_t *lists, int64_t cap,
+                     hipStream_t st) {
+#define CALL(G) filter_t<G>(xf, n, xn, qf, nq, nqp, l2, s, tau, qmap, count, lists, cap, st)
+    PS_DISPATCH(fp8, dp, CALL)
 #undef CALL
+}
+
+// Rows (x, row stride `stride`) scaled by `scale` and rounded to E4M3 in the FP8 fragment
+// layout of dp dims; rows >= `rows` up to rows_pad and dims >= dim are zero. One thread per
+// 16-byte chunk.
+__global__ __launch_bounds__(256) void ps_fp8_frag_kernel(const float *x, int64_t rows, int dim,
+                                                          int64_t stride, int kb, int64_t chunks,
+                                                          float scale, uint4 *out) {
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < chunks;
+         c += (int64_t)gridDim.x * 256) {
+        const int64_t t = c / ((int64_t)kb * 64);
+        const int bl = (int)((c / 64) % kb), l = (int)(c % 64);
+        const int64_t row = 32 * t + (l & 31);
+        const int d0 = 32 * bl + 16 * (l >> 5);
+        uint32_t w[4] = {0, 0, 0, 0};
+        if (row < rows) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int d = d0 + j;
+                const uint32_t b = d < dim ? to_e4m3(x[row * stride + d] * scale) : 0u;
+                w[j >> 2] |= b << (8 * (j & 3));
+            }
+        }
+        out[c] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+hipError_t ps_to_fp8_frag(const float *x, int64_t rows, int dim, int64_t stride, int dp,
+                          int64_t rows_pad, float scale, uint8_t *out, hipStream_t st) {
+    if (dp % 128 != 0 || rows_pad % 32 != 0 || rows > rows_pad) return hipErrorInvalidValue;
+    const int64_t chunks = rows_pad * dp / 16;
+    if (chunks == 0) return hipSuccess;
+    const unsigned g = (unsigned)std::min<int64_t>((chunks + 255) / 256, 65536);
+    hipLaunchKernelGGL(ps_fp8_frag_kernel, dim3(g), dim3(256), 0, st, x, rows, dim, stride, dp / 32,
+                       chunks, scale, (uint4 *)out);
+    return hipGetLastError();
+}
+
+// Largest |x| over rows x dim as f32 bits (non-negative floats order like their bits);
+// *out must start at 0.
+__global__ __launch_bounds__(256) void ps_absmax_kernel(const float *x, int64_t rows, int dim,
+                                                        int64_t stride, uint32_t *out) {
+    float m = 0.0f;
+    const int64_t total = rows * dim;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * 256) {
+        const int64_t r = e / dim;
+        m = fmaxf(m, fabsf(x[r * stride + (e - r * dim)]));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+hipError_t ps_absmax(const float *x, int64_t rows, int dim, int64_t stride, uint32_t *out,
+                     hipStream_t st) {
+    hipError_t e = hipMemsetAsync(out, 0, sizeof(uint32_t), st);
+    if (e != hipSuccess || rows == 0) return e;
+    const unsigned g = (unsigned)std::min<int64_t>((rows * dim + 255) / 256, 4096);
+    hipLaunchKernelGGL(ps_absmax_kernel, dim3(g), dim3(256), 0, st, x, rows, dim, stride, out);
+    return hipGetLastError();
 }
 
 hipError_t ps_select(const float *dense, int64_t ld, int ns, int j0, int j1, int j2, float *tau,

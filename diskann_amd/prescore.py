@@ -1,20 +1,22 @@
 """Candidate pre-scoring with exact FP32 rescore (dkm_prescore_*): exhaustive top-k where
-every (query, row) pair is scored approximately in BF16 on the matrix cores and only the
-best `candidates` rows of each query are scored exactly with the reference's f32 distance
-functions (the arithmetic of Rerank, full_precision.rs:357-404)."""
+every (query, row) pair is scored approximately in BF16 or FP8 (E4M3, power-of-two scaled
+operands) on the matrix cores and only the best `candidates` rows of each query are scored
+exactly with the reference's f32 distance functions (the arithmetic of Rerank,
+full_precision.rs:357-404)."""
 import ctypes as C
 
 import numpy as np
 
 from . import _native
 from .errors import check
-from .quantization import BF16, _Dev
+from .quantization import BF16, FP8, _Dev  # noqa: F401
 
 
 class PreScorer:
-    """Holds the rows (device copy) and their bf16 copy.
+    """Holds the rows (device copy) and their bf16 or fp8 copy.
 
-    `metric`: Metric.L2, Metric.INNER_PRODUCT or Metric.COSINE_NORMALIZED."""
+    `metric`: Metric.L2, Metric.INNER_PRODUCT or Metric.COSINE_NORMALIZED; `precision`:
+    quantization.BF16 or quantization.FP8."""
 
     def __init__(self, rows, metric, precision=BF16):
         self.lib = _native.load()

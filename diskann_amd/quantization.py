@@ -100,6 +100,7 @@ def distance_batch(metric, queries, base, ids):
 
 F32 = 0   # k-means assignment in the reference's FP32 arithmetic
 BF16 = 1  # k-means assignment as a BF16 MFMA GEMM with FP32 accumulation
+FP8 = 2   # candidate pre-scoring as an FP8 (E4M3) MFMA GEMM with FP32 accumulation
 
 
 class _Dev:

@@ -413,7 +413,7 @@ dkm_status dkm_pq_pivots_read(const char *path, float *pivots, int64_t K, int64_
  * FP32), ties to the smaller centre index. d_centroids holds the K initial centres on
  * entry and the final ones on return; ms_* receive the summed device time of each step
  * (may be NULL). Synchronous. */
-typedef enum { DKM_PRECISION_F32 = 0, DKM_PRECISION_BF16 = 1 } dkm_precision;
+typedef enum { DKM_PRECISION_F32 = 0, DKM_PRECISION_BF16 = 1, DKM_PRECISION_FP8 = 2 } dkm_precision;
 dkm_status dkm_kmeans_device(const float *d_x, int64_t n, int64_t dim, int64_t K, int64_t iters,
                              int precision, float *d_centroids, uint32_t *d_assign,
                              double *ms_assign, double *ms_update);
@@ -424,17 +424,21 @@ dkm_status dkm_kmeans_assign_device(const float *d_x, int64_t n, int64_t dim, co
                                     float *d_dist);
 
 /* -------------------------------------------------------------------------------- */
-/* Candidate pre-scoring (BF16 MFMA) with exact FP32 rescore                          */
+/* Candidate pre-scoring (BF16 / FP8 MFMA) with exact FP32 rescore                   */
 /* -------------------------------------------------------------------------------- */
 /* Exhaustive top-k over device-resident f32 rows (n x dim, dim <= 768) for inner product,
- * L2 or cosine-normalized: every (query, row) pair is scored approximately as a BF16 MFMA
- * GEMM with FP32 accumulation (-q.x, or |x|^2 - 2 q.x for L2); the `candidates` best rows
+ * L2 or cosine-normalized: every (query, row) pair is scored approximately as an MFMA GEMM
+ * with FP32 accumulation (-q.x, or |x|^2 - 2 q.x for L2) on bf16 operands
+ * (DKM_PRECISION_BF16) or on FP8 E4M3 operands (DKM_PRECISION_FP8: the rows and each
+ * batch of queries are multiplied by a power of two that brings their largest magnitude to
+ * at most 448, rounded to nearest even, and the scales are divided out of the scores);
+ * the `candidates` best rows
  * of each query by that score are re-scored exactly with the reference's f32 distance
  * functions (as Rerank, diskann-providers/src/model/graph/provider/async_/inmem/
  * full_precision.rs:357-404, and dkm_distance_batch) and the k best of those are returned
  * in increasing exact distance (ties by row). No reference counterpart: this is the
  * precision layer that stands in for exact exhaustive FP32 scoring.
- * dkm_prescore_create keeps d_x (which must stay valid) and builds the bf16 copy;
+ * dkm_prescore_create keeps d_x (which must stay valid) and builds the bf16 / fp8 copy;
  * dkm_prescore_search takes device queries / outputs (nq x k) and, if ms_stages is not
  * NULL, reports the device time of four stages (approximate scoring with the candidate
  * filter, candidate selection, exact rescore, final top-k). Synchronous. */

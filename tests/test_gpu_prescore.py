@@ -1,18 +1,20 @@
-"""Candidate pre-scoring (dkm_prescore_*): BF16 MFMA approximate scores with a threshold
-filter, exact FP32 rescore of the best C, top k.
+"""Candidate pre-scoring (dkm_prescore_*): BF16 or FP8 (E4M3) MFMA approximate scores with a
+threshold filter, exact FP32 rescore of the best C, top k.
 
 * Every returned distance equals the oracle's f32 distance (the reference arithmetic)
   bit for bit, and results are ordered by (distance, row).
-* On small-integer data the bf16 scores are exact, so the best C by approximate score are
-  the best C by exact distance and the output must equal the oracle's exhaustive top k —
-  including when the threshold filter (and its retries) does the selection.
+* On small-integer data the bf16 and fp8 scores are exact (E4M3 holds the integers up to
+  16, and the power-of-two operand scales are exact), so the best C by approximate score
+  are the best C by exact distance and the output must equal the oracle's exhaustive top
+  k — including when the threshold filter (and its retries) does the selection, and for
+  integers times a power of two far outside the E4M3 range (the operand scaling).
 * With C >= n every row is rescored: the output is the exhaustive FP32 top k on any data.
 * On real-valued clustered data recall@k against FP64 brute force stays >= 0.99."""
 import numpy as np
 import pytest
 
 from diskann_amd.errors import ANNError
-from diskann_amd.prescore import PreScorer
+from diskann_amd.prescore import BF16, FP8, PreScorer
 from oracle import pyoracle as po
 from test_gpu_search import log
 
@@ -34,15 +36,19 @@ def check_rows(metric, x, qs, ids, ds):
         assert key == sorted(key)
 
 
+PRECISIONS = [pytest.param(BF16, id="bf16"), pytest.param(FP8, id="fp8")]
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("metric,dim", [(po.INNER_PRODUCT, 96), (po.L2, 37), (po.L2, 200),
                                         (po.INNER_PRODUCT, 768)])
-def test_integer_data_equals_exhaustive(metric, dim):
-    log("prescore integer metric=%d dim=%d" % (metric, dim))
+def test_integer_data_equals_exhaustive(metric, dim, precision):
+    log("prescore integer metric=%d dim=%d precision=%d" % (metric, dim, precision))
     rng = np.random.default_rng(dim + metric)
     n, nq, k, C = 6000, 12, 10, 300
     x = rng.integers(-3, 4, (n, dim)).astype(np.float32)
     qs = rng.integers(-3, 4, (nq, dim)).astype(np.float32)
-    ps = PreScorer(x, metric)
+    ps = PreScorer(x, metric, precision)
     ids, ds, ms = ps.search(qs, k, C)
     check_rows(metric, x, qs, ids, ds)
     for i in range(nq):
@@ -52,8 +58,9 @@ def test_integer_data_equals_exhaustive(metric, dim):
     assert len(ms) == 4 and all(m >= 0 for m in ms)
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("metric", [po.L2, po.INNER_PRODUCT, po.COSINE_NORMALIZED])
-def test_all_rows_rescored_equals_exhaustive(metric):
+def test_all_rows_rescored_equals_exhaustive(metric, precision):
     rng = np.random.default_rng(5)
     n, dim, nq, k = 900, 128, 8, 20
     x = rng.standard_normal((n, dim)).astype(np.float32)
@@ -61,13 +68,14 @@ def test_all_rows_rescored_equals_exhaustive(metric):
     if metric == po.COSINE_NORMALIZED:
         x /= np.linalg.norm(x, axis=1, keepdims=True)
         qs /= np.linalg.norm(qs, axis=1, keepdims=True)
-    ids, ds, _ = PreScorer(x, metric).search(qs, k, candidates=n)
+    ids, ds, _ = PreScorer(x, metric, precision).search(qs, k, candidates=n)
     for i in range(nq):
         wi, wd = exact_topk(metric, x, qs[i], k)
         assert np.array_equal(ids[i], wi) and ds[i].tobytes() == wd.tobytes(), i
 
 
-def test_recall_on_clustered_unit_vectors():
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_recall_on_clustered_unit_vectors(precision):
     rng = np.random.default_rng(11)
     n, dim, nq, k, C = 20000, 768, 40, 10, 400
     cen = rng.standard_normal((64, dim)).astype(np.float32)
@@ -75,7 +83,7 @@ def test_recall_on_clustered_unit_vectors():
     x /= np.linalg.norm(x, axis=1, keepdims=True)
     qs = cen[rng.integers(0, 64, nq)] + 0.5 * rng.standard_normal((nq, dim)).astype(np.float32)
     qs /= np.linalg.norm(qs, axis=1, keepdims=True)
-    ids, ds, _ = PreScorer(x, po.INNER_PRODUCT).search(qs, k, C)
+    ids, ds, _ = PreScorer(x, po.INNER_PRODUCT, precision).search(qs, k, C)
     check_rows(po.INNER_PRODUCT, x, qs, ids, ds)
     ip = qs.astype(np.float64) @ x.astype(np.float64).T
     gt = np.argsort(-ip, axis=1, kind="stable")[:, :k]
@@ -83,29 +91,47 @@ def test_recall_on_clustered_unit_vectors():
     assert hits / (nq * k) >= 0.99
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("C", [8, 40])
-def test_small_candidate_counts(C):
+def test_small_candidate_counts(C, precision):
     # few candidates against many rows: the sample threshold is taken at a floor rank and
     # the lists sized for it
     rng = np.random.default_rng(C)
     n, dim, nq, k = 40000, 64, 16, 5
     x = rng.integers(-3, 4, (n, dim)).astype(np.float32)
     qs = rng.integers(-3, 4, (nq, dim)).astype(np.float32)
-    ids, ds, _ = PreScorer(x, po.INNER_PRODUCT).search(qs, k, C)
+    ids, ds, _ = PreScorer(x, po.INNER_PRODUCT, precision).search(qs, k, C)
     for i in range(nq):
         wi, wd = exact_topk(po.INNER_PRODUCT, x, qs[i], k)
         assert np.array_equal(ids[i], wi) and ds[i].tobytes() == wd.tobytes(), i
 
 
-def test_many_queries_span_batches():
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_many_queries_span_batches(precision):
     # more queries than one filter launch holds (1024): the batches must not interfere
     rng = np.random.default_rng(3)
     n, dim, nq, k, C = 3000, 64, 1500, 5, 100
     x = rng.integers(-2, 3, (n, dim)).astype(np.float32)
     qs = rng.integers(-2, 3, (nq, dim)).astype(np.float32)
-    ids, ds, _ = PreScorer(x, po.L2).search(qs, k, C)
+    ids, ds, _ = PreScorer(x, po.L2, precision).search(qs, k, C)
     for i in (0, 1023, 1024, 1499):
         wi, wd = exact_topk(po.L2, x, qs[i], k)
+        assert np.array_equal(ids[i], wi) and ds[i].tobytes() == wd.tobytes(), i
+
+
+@pytest.mark.parametrize("metric", [po.L2, po.INNER_PRODUCT])
+def test_fp8_operand_scaling(metric):
+    # integers times powers of two far outside E4M3's range (rows up to 3 * 2^12, queries
+    # down to 2^-9 for inner product): the per-matrix power-of-two scales bring them into
+    # range exactly, and the f32 distances of such values are exact as well
+    rng = np.random.default_rng(17 + metric)
+    n, dim, nq, k, C = 5000, 128, 10, 10, 200
+    x = (rng.integers(-3, 4, (n, dim)) * 4096.0).astype(np.float32)
+    qscale = 4096.0 if metric == po.L2 else 1.0 / 512.0
+    qs = (rng.integers(-3, 4, (nq, dim)) * qscale).astype(np.float32)
+    ids, ds, _ = PreScorer(x, metric, FP8).search(qs, k, C)
+    for i in range(nq):
+        wi, wd = exact_topk(metric, x, qs[i], k)
         assert np.array_equal(ids[i], wi) and ds[i].tobytes() == wd.tobytes(), i
 
 
@@ -115,6 +141,10 @@ def test_errors():
         PreScorer(x, po.COSINE)
     with pytest.raises(ANNError):
         PreScorer(np.zeros((10, 800), np.float32), po.L2)
+    with pytest.raises(ANNError):
+        PreScorer(np.zeros((10, 800), np.float32), po.L2, FP8)
+    with pytest.raises(ANNError):
+        PreScorer(x, po.L2, 7)
     ps = PreScorer(x, po.L2)
     with pytest.raises(ANNError):
         ps.search(np.zeros((1, 16), np.float32), 0)

@@ -161,6 +161,7 @@ def serve_measure(dk, ix, qs, K, L, BW, batch_ids):
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+MFMA_FP8_PEAK_TFLOPS = 5000.0
 
 
 def unit_mixture(n, dim, comps, rng, centers=None):
@@ -183,14 +184,15 @@ def prescore_workload(args):
     100; recall and distance error on a sample of queries against FP64 brute force, and the
     same for exhaustive FP32 scoring (numpy sgemm) as the plain-path reference."""
     import diskann_amd as dk  # noqa: F401
-    from diskann_amd.prescore import PreScorer
+    from diskann_amd.prescore import BF16, FP8, PreScorer
     N, D, NQ, C = args.n, 768, args.nq, 4096
+    FP8_RUN = args.precision == "fp8"
     rng = np.random.default_rng(0xD15C0003)
     t0 = time.time()
     x, centers = unit_mixture(N, D, 1024, rng)
     qs, _ = unit_mixture(NQ, D, 1024, np.random.default_rng(0xD15C0013), centers)
     log("data %.1fs" % (time.time() - t0))
-    ps = PreScorer(x, 1)  # inner product
+    ps = PreScorer(x, 1, FP8 if FP8_RUN else BF16)  # inner product
     res = {}
     for k in (10, 100):
         ps.search(qs[:256], k, C)  # warm-up
@@ -221,14 +223,18 @@ def prescore_workload(args):
     flops = 2.0 * N * NQ * D
     score_ms = ms[0]
     tflops = flops / (score_ms * 1e-3) / 1e12
+    prec = "fp8" if FP8_RUN else "bf16"
+    peak = MFMA_FP8_PEAK_TFLOPS if FP8_RUN else MFMA_BF16_PEAK_TFLOPS
     out = {
-        "metric": "exhaustive top-k QPS, BF16 MFMA pre-scoring + exact FP32 rescore of 4096",
+        "metric": "exhaustive top-k QPS, %s MFMA pre-scoring + exact FP32 rescore of 4096"
+                  % prec.upper(),
         "value": NQ / wall, "unit": "queries/s", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "vs_baseline": None, "dtype": prec, "data": "synthetic",
         "config": {
             "workload": "1M x 768 unit-norm (1,024-component mixture), inner product, %d "
-                        "queries, 4096 candidates rescored in FP32, k = 10 (k = 100 below)" % NQ,
+                        "queries, %s pre-scoring, 4096 candidates rescored in FP32, k = 10 "
+                        "(k = 100 below)" % (NQ, prec),
             "stage_ms_k10": {"score_filter": ms[0], "select": ms[1], "rescore": ms[2],
                              "topk": ms[3]},
             "qps_k100": NQ / res[100][3],
@@ -237,9 +243,10 @@ def prescore_workload(args):
             "rescore_bytes": float(NQ) * C * D * 4,
             "rescore_gbs": float(NQ) * C * D * 4 / (ms[2] * 1e-3) / 1e9,
         },
-        "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "ps_filter_kernel<48> (+ sample scoring and selection)",
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": peak,
+                     "unit": "TFLOP/s", "frac": tflops / peak, "traffic": None,
+                     "kernel": "ps_filter_kernel<%s> (+ sample scoring and selection)" %
+                               ("Fp8Rows<24>" if FP8_RUN else "Bf16Rows<48>"),
                      "flops_per_launch": flops},
     }
     print(json.dumps(out), flush=True)
@@ -314,6 +321,277 @@ def kmeans_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def served_workload(args):
+    """Served search on 10M x 96 f32 vectors from a 1,024-component Gaussian mixture
+    (sigma 0.3, centres N(0,1)), L2, Vamana R 64 (max degree 83) alpha 1.2 L_build 100, PQ
+    48 x 256 (10 Lloyd iterations on a 100k sample). Each request draws k from {10, 100}
+    and L uniformly from [32, 256]; L < k is raised to k (the reference refuses L < k,
+    diskann/src/graph/search/knn_search.rs:101-106). Reports batched QPS over a 10k-request
+    mix, served saturation QPS and p50/p99 latency under Poisson arrivals at 20k-200k
+    requests/s, the plain per-query GPU path and the CPU baseline on the same mix."""
+    import diskann_amd as dk
+    from diskann_amd import quantization as qz
+    from diskann_amd.errors import check
+    N = args.n if args.n != 1_000_000 else 10_000_000
+    D, R, R_MAX, L_BUILD, ALPHA, M = 96, 64, 83, 100, 1.2, 48
+    NQ, KMAX, LMAX, NGT = args.nq, 100, 256, min(args.nq, 2000)
+    t0 = time.time()
+    rng = np.random.default_rng(0xD15C0004)
+    centers = rng.standard_normal((1024, D)).astype(np.float32)
+    x = np.empty((N, D), np.float32)
+    for r0 in range(0, N, 1_000_000):
+        r1 = min(N, r0 + 1_000_000)
+        x[r0:r1] = centers[rng.integers(0, 1024, r1 - r0)]
+        x[r0:r1] += (0.3 * rng.standard_normal((r1 - r0, D))).astype(np.float32)
+    qs, _ = gmm(NQ, D, 1024, 0.3, 0xD15C1004, centers)
+    rq = np.random.default_rng(0xD15C5004)
+    ks = rq.choice(np.array([10, 100], np.uint32), NQ).astype(np.uint32)
+    Ls = rq.integers(32, LMAX + 1, NQ).astype(np.uint32)
+    Ls = np.maximum(Ls, ks)
+    log("data %.1fs" % (time.time() - t0))
+
+    ix = dk.DiskANNIndex(dk.Metric.L2, D, N, R, R_MAX)
+    lib = ix.lib
+    ix.set_vectors(x)
+    _, med = ix.medoid()
+    ix.set_start_points(med)
+    t0 = time.time()
+    sample = x[np.random.default_rng(0xD15C2004).choice(N, 100_000, replace=False)]
+    pivots, offsets = qz.train_pq(sample, M, 256, 10, seed=0xD15C3004)
+    ix.set_pq(pivots, offsets)
+    ix.encode()
+    t_pq = time.time() - t0
+    log("pq train+encode %.1fs" % t_pq)
+    t0 = time.time()
+    step = 1 << 20      # a multiple of the batch: the same batches as one call
+    for r0 in range(0, N, step):
+        ix.build(np.arange(r0, min(N, r0 + step)), batch=1024, intra_batch=0,
+                 L_build=L_BUILD, alpha=ALPHA)
+        log("graph build: %d rows, %.1fs" % (min(N, r0 + step), time.time() - t0))
+    t_build = time.time() - t0
+    t0 = time.time()
+    gt, _ = ix.bruteforce(qs[:NGT], 10)
+    log("FP64 brute force of %d queries %.1fs" % (NGT, time.time() - t0))
+
+    dq = DeviceBuf(lib, qs)
+    dkk = DeviceBuf(lib, ks)
+    dL = DeviceBuf(lib, Ls)
+    dids = DeviceBuf(lib, nbytes=4 * NQ * KMAX)
+    dd = DeviceBuf(lib, nbytes=4 * NQ * KMAX)
+    dcnt = DeviceBuf(lib, nbytes=4 * NQ)
+    dcm = DeviceBuf(lib, nbytes=4 * NQ)
+    dhp = DeviceBuf(lib, nbytes=4 * NQ)
+    ddeg = DeviceBuf(lib, nbytes=4 * NQ)
+    h = ix._h
+    cur = {"strat": int(dk.Hybrid), "bw": 1}
+
+    def launch(nq=NQ, qoff=0):
+        check(lib.dkm_search_batch_device_stats(
+            h, cur["strat"], C.c_void_p(dq.ptr.value + 4 * D * qoff), nq,
+            C.c_void_p(dkk.ptr.value + 4 * qoff), C.c_void_p(dL.ptr.value + 4 * qoff), KMAX,
+            LMAX, cur["bw"], KMAX, dids.ptr, dd.ptr, dcnt.ptr, dcm.ptr, dhp.ptr, ddeg.ptr), lib)
+
+    def sync():
+        check(lib.dkm_index_synchronize(h), lib)
+
+    def timed(reps=2):
+        launch()
+        sync()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            launch()
+        sync()
+        return (time.perf_counter() - t1) / reps
+
+    # strategy and beam width: Hybrid (PQ traversal + FP32 rerank) unless only full
+    # precision traversal reaches recall@10 0.95 on the mix; beam width and the
+    # result-preserving knobs autotuned
+    best = None
+    by_strategy = {}
+    for name, st in (("hybrid", dk.Hybrid), ("full_precision", dk.FullPrecision)):
+        cur["strat"] = int(st)
+        for bw in (1, 2, 4):
+            cur["bw"] = bw
+            hs, am = ix.autotune(qs, 10, 144, bw, st)
+            dt = timed()
+            rec = recall_at_10(dids.get(np.uint32, (NQ, KMAX))[:NGT, :10], gt)
+            by_strategy["%s_bw%d" % (name, bw)] = {"recall_at_10": rec, "qps": NQ / dt}
+            log("%s bw=%d visited_slots=%d adc_mode=%d recall=%.4f qps=%.0f" %
+                (name, bw, hs, am, rec, NQ / dt))
+            if rec >= 0.95 and (best is None or dt < best[0]):
+                best = (dt, name, int(st), bw, hs, am)
+        if best is not None:
+            break
+    if best is None:
+        raise RuntimeError("no strategy reached recall@10 0.95 on the request mix")
+    _, SNAME, cur["strat"], cur["bw"], HSLOTS, ADC = best
+    ix.set_tuning(HSLOTS, ADC)
+    BW = cur["bw"]
+
+    for _ in range(args.warmup):
+        launch()
+    sync()
+    check(lib.dkm_profile(h, 1), lib)
+    step_ms = []
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        launch()
+        sync()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+    wall = time.perf_counter() - t_all
+    nlaunch, kms = C.c_int64(), C.c_double()
+    check(lib.dkm_profile_read(h, C.byref(nlaunch), C.byref(kms)), lib)
+    check(lib.dkm_profile(h, 0), lib)
+    qps = NQ * args.steps / wall
+    kernel_ms = kms.value / max(nlaunch.value, 1)
+    ids = dids.get(np.uint32, (NQ, KMAX))
+    rec = recall_at_10(ids[:NGT, :10], gt)
+    cmps = dcm.get(np.uint32, NQ).astype(np.float64)
+    hops = dhp.get(np.uint32, NQ).astype(np.float64)
+    degs = ddeg.get(np.uint32, NQ).astype(np.float64)
+    if SNAME == "hybrid":
+        # cmps*(M + 4) + hops*4 + degsum*4 + K*D*4 (pivots for the LUT) + (L+1)*D*4 (rerank)
+        bytes_per_q = cmps * (M + 4) + hops * 4 + degs * 4 + 256 * D * 4 + \
+            (Ls.astype(np.float64) + 1) * D * 4
+    else:
+        bytes_per_q = cmps * D * 4 + hops * 4 + degs * 4 + D * 4
+    alg_bytes = float(bytes_per_q.sum())
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    log("batched: %.0f q/s, kernel %.2f ms, recall %.4f, %.0f GB/s" % (qps, kernel_ms, rec,
+                                                                    achieved))
+    if args.timed_only:
+        print(json.dumps({"timed_only": True, "queries_per_s": qps, "kernel_ms": kernel_ms,
+                          "strategy": SNAME, "beam_width": BW, "visited_lds_slots": HSLOTS,
+                          "recall_at_10": rec, "algorithmic_bytes_per_launch": alg_bytes,
+                          "achieved_gbs": achieved}), flush=True)
+        profile_copy(lib, args.profile_copy_gib)
+        return
+
+    # plain per-query GPU path: one request per launch, synchronised
+    n_plain = 300
+    lat = []
+    tp = time.perf_counter()
+    for i in range(n_plain):
+        ts = time.perf_counter()
+        launch(nq=1, qoff=i)
+        sync()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    plain_qps = n_plain / (time.perf_counter() - tp)
+    log("plain per-query: %.0f q/s" % plain_qps)
+
+    # served: continuous batching, saturation then Poisson arrivals
+    served = {}
+    strat_obj = dk.Hybrid if SNAME == "hybrid" else dk.FullPrecision
+    with dk.Server(ix, strat_obj, max_k=KMAX, max_L=LMAX, beam_width=BW, capacity=1 << 16,
+                   slots=0) as sv:
+        served["slots"] = sv.slots
+        tags = np.arange(NQ, dtype=np.uint64)
+        arr = np.zeros(NQ)
+        latv = np.zeros(NQ)
+        t0 = time.perf_counter()
+        assert sv.submit_batch(qs[:2000], ks[:2000], Ls[:2000], tags[:2000]) == 2000
+        serve_drain(sv, 2000, arr, t0, latv, None)
+        served_ids = np.zeros((NQ, KMAX), np.uint32)
+        t0 = time.perf_counter()
+        assert sv.submit_batch(qs, ks, Ls, tags) == NQ
+        serve_drain(sv, NQ, arr, t0, latv, served_ids)
+        sat = NQ / (time.perf_counter() - t0)
+        served["saturation_qps"] = sat
+        same = all(np.array_equal(served_ids[i, :ks[i]], ids[i, :ks[i]]) for i in range(NQ))
+        served["identical_to_batched"] = bool(same)
+        log("served saturation %.0f q/s, identical to batched: %s" % (sat, same))
+        rngp = np.random.default_rng(0xD15C6004)
+        loads = []
+        for rate in (20_000.0, 50_000.0, 100_000.0, 200_000.0):
+            if rate > 0.9 * sat:
+                continue
+            nreq = int(max(20_000, rate * 0.5))
+            arrival = np.cumsum(rngp.exponential(1e3 / rate, nreq))
+            latp = np.full(nreq, np.nan)
+            qidx = np.arange(nreq) % NQ
+            t0 = time.perf_counter()
+            i = done = 0
+            while i < nreq:
+                now = (time.perf_counter() - t0) * 1e3
+                j = int(np.searchsorted(arrival, now, side="right"))
+                if j > i:
+                    i += sv.submit_batch(qs[qidx[i:j]], ks[qidx[i:j]], Ls[qidx[i:j]],
+                                         np.arange(i, j, dtype=np.uint64))
+                tg, _, _, _, _, _, _ = sv.poll(4096)
+                if len(tg):
+                    tgi = tg.astype(np.int64)
+                    latp[tgi] = (time.perf_counter() - t0) * 1e3 - arrival[tgi]
+                    done += len(tg)
+            done += serve_drain(sv, nreq - done, arrival, t0, latp, None)
+            w = time.perf_counter() - t0
+            ld = {"offered_qps": rate, "achieved_qps": nreq / w,
+                  "p50_ms": float(np.percentile(latp, 50)),
+                  "p99_ms": float(np.percentile(latp, 99))}
+            loads.append(ld)
+            log("poisson %s" % json.dumps(ld))
+        served["poisson"] = loads
+
+    # CPU baseline: the oracle's C search on the same index and request mix
+    from oracle import pyoracle as po
+    lists, counts = ix.get_graph()
+    vec = np.vstack([x, med[None, :]])
+    codes = ix.get_codes()
+    oix = po.OracleIndex(po.L2, vec, lists, counts, [N], codes, pivots, offsets)
+    mode = po.MODE_HYBRID if SNAME == "hybrid" else po.MODE_FULL
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    probe = 200
+    _, _, _, _, w = oix.search_batch_mixed(qs[:probe], ks[:probe], Ls[:probe], BW, mode, threads)
+    ncpu = int(min(NQ, max(probe, probe * args.cpu_seconds / max(w, 1e-6))))
+    cids, _, ccm, _, cwall = oix.search_batch_mixed(qs[:ncpu], ks[:ncpu], Ls[:ncpu], BW, mode,
+                                                    threads)
+    cpu_qps = ncpu / cwall
+    parity = bool(np.array_equal(ccm, cmps[:ncpu].astype(np.uint32)) and
+                  all(np.array_equal(cids[i, :ks[i]], ids[i, :ks[i]]) for i in range(ncpu)))
+    log("cpu %.0f q/s on %d threads, GPU == CPU: %s" % (cpu_qps, threads, parity))
+
+    p99_served = [ld["p99_ms"] for ld in served["poisson"]]
+    out = {
+        "metric": "QPS at recall@10=0.95 + p99 latency, served batched vs plain per-query GPU",
+        "value": sat, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": float(np.mean(step_ms)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "served search: %d x 96 f32 GMM(1024, sigma 0.3), L2, Vamana R=64 "
+                        "alpha=1.2, PQ 48x256, requests k in {10, 100}, L ~ U[32, 256] "
+                        "(L < k raised to k), %s traversal" % (N, SNAME),
+            "n": N, "dim": D, "requests_in_mix": NQ, "strategy": SNAME, "beam_width": BW,
+            "visited_lds_slots": HSLOTS, "adc_mode": ADC, "recall_at_10": rec,
+            "recall_queries": NGT, "by_strategy": by_strategy,
+            "batched_qps": qps, "batched_p50_ms": float(np.percentile(step_ms, 50)),
+            "batched_p99_ms": float(np.percentile(step_ms, 99)),
+            "plain_per_query_qps": plain_qps,
+            "plain_p50_ms": float(np.percentile(lat, 50)),
+            "plain_p99_ms": float(np.percentile(lat, 99)),
+            "served": served, "served_over_plain": sat / plain_qps,
+            "served_over_cpu": sat / cpu_qps,
+            "served_p99_ms_by_load": p99_served,
+            "gpu_cpu_parity_on_sample": parity, "build_seconds": t_build,
+            "pq_train_encode_seconds": t_pq,
+            "mean_cmps": float(cmps.mean()), "mean_hops": float(hops.mean()),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "beam_search_kernel<L2,PQ>" if SNAME == "hybrid" else
+                      "beam_search_kernel<L2,FP>",
+            "kernel_ms": kernel_ms, "algorithmic_bytes_per_launch": alg_bytes,
+        },
+        "cpu_baseline": {
+            "value": cpu_qps, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": "%d requests of the mix, same index, %s beam %d, static partition over "
+                      "%d threads (oracle/oracle.c)" % (ncpu, SNAME, BW, threads),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,7 +607,11 @@ def main():
                          "prescore768 (exhaustive top-k with BF16 pre-scoring, 1M x 768) or "
                          "sq1m (batched search with 8-bit scalar-quantized traversal and "
                          "FP32 rerank, same 1M x 128 index as search1m) or sph1m (1-bit "
-                         "spherical traversal with 4-bit queries and FP32 rerank, same index)")
+                         "spherical traversal with 4-bit queries and FP32 rerank, same index) or "
+                         "served10m (served search on 10M x 96 with PQ 48 x 256 and a mix of "
+                         "k in {10, 100}, L in [32, 256])")
+    ap.add_argument("--precision", default="bf16", choices=("bf16", "fp8"),
+                    help="pre-scoring operand type of the prescore768 workload")
     ap.add_argument("--fixed", default="",
                     help="L,beam_width,visited_slots: skip the L sweep and the autotuner")
     ap.add_argument("--timed-only", action="store_true",
@@ -347,6 +629,8 @@ def main():
         return kmeans_workload(args)
     if args.workload == "prescore768":
         return prescore_workload(args)
+    if args.workload == "served10m":
+        return served_workload(args)
     if args.workload not in ("search1m", "sq1m", "sph1m"):
         print("unknown workload %s" % args.workload, file=sys.stderr)
         sys.exit(2)

@@ -47,6 +47,9 @@ def load():
         raise RuntimeError(
             "libdiskann_mi355x.so is not built; run `python -m diskann_amd.build` (no CPU "
             "fallback exists)")
+    # the serving lanes (eight streams) run concurrently only with a hardware queue each;
+    # HIP reads this once, when the runtime starts, and its default is 4
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "12")
     lib = C.CDLL(LIB_PATH)
     _declare(lib)
     _lib = lib

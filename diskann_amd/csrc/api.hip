@@ -1860,7 +1860,11 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
 
 namespace {
 
-constexpr int kServeStreams = 4;
+// Serving lanes. Launches on different lanes overlap on the device only when each lane's
+// stream has a hardware queue of its own: HIP gives a process GPU_MAX_HW_QUEUES queues
+// (default 4) and streams beyond that share them, serialising their launches. The lanes
+// plus the index stream and the null stream need GPU_MAX_HW_QUEUES >= kServeStreams + 2.
+constexpr int kServeStreams = 8;
 
 // One in-flight launch: requests [seq0, seq0 + n) of the ring.
 struct ServeLane {
@@ -1869,10 +1873,12 @@ struct ServeLane {
     bool busy = false;
     uint64_t seq0 = 0;
     int64_t n = 0;
-    float *d_q = nullptr;  // device buffers sized for one launch
-    uint32_t *d_k = nullptr, *d_L = nullptr, *d_ids = nullptr, *d_cnt = nullptr;
-    uint32_t *d_cmps = nullptr, *d_hops = nullptr, *d_status = nullptr;
-    float *d_d = nullptr;
+    // A launch's inputs (queries n x dim, k n, L n) and outputs (ids n x max_k, distances
+    // n x max_k, counts, cmps, hops, status: n each) are packed back to back for its n in
+    // pinned host buffers sized for the largest launch; the kernel reads the inputs and
+    // writes the outputs there directly (each query reads its inputs once and stores its
+    // outputs once), so a launch issues no copies.
+    uint32_t *h_in = nullptr, *h_out = nullptr;
     uint32_t *d_spill = nullptr;
     uint8_t *d_sphq = nullptr;  // spherical: prepared queries of a launch
 };
@@ -1883,29 +1889,17 @@ double now_us() {
         .count();
 }
 
+// Pinned host memory (zeroed); coherent = fine-grained, for buffers the device reads and
+// writes directly while the host reuses them between launches.
 template <typename T>
-hipError_t pinned(T **host, size_t count) {
+hipError_t pinned(T **host, size_t count, bool coherent = false) {
     void *h = nullptr;
-    hipError_t e = hipHostMalloc(&h, sizeof(T) * std::max<size_t>(count, 1), hipHostMallocDefault);
+    hipError_t e = hipHostMalloc(&h, sizeof(T) * std::max<size_t>(count, 1),
+                                 coherent ? hipHostMallocCoherent : hipHostMallocDefault);
     if (e != hipSuccess) return e;
     memset(h, 0, sizeof(T) * std::max<size_t>(count, 1));
     *host = (T *)h;
     return hipSuccess;
-}
-
-// Copies ring entries [seq0, seq0 + n) (at most two contiguous pieces) host -> device or
-// device -> host; `elems` values of `T` per entry.
-template <typename T>
-hipError_t ring_copy(T *ring, T *dev, uint64_t seq0, int64_t n, int64_t elems,
-                            int64_t capacity, bool to_device, hipStream_t st) {
-    const int64_t a = (int64_t)(seq0 % (uint64_t)capacity);
-    const int64_t n1 = std::min(n, capacity - a);
-    const size_t b1 = sizeof(T) * elems * n1, b2 = sizeof(T) * elems * (n - n1);
-    hipError_t e = to_device ? hipMemcpyAsync(dev, ring + a * elems, b1, hipMemcpyHostToDevice, st)
-                             : hipMemcpyAsync(ring + a * elems, dev, b1, hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess || n == n1) return e;
-    return to_device ? hipMemcpyAsync(dev + n1 * elems, ring, b2, hipMemcpyHostToDevice, st)
-                     : hipMemcpyAsync(ring, dev + n1 * elems, b2, hipMemcpyDeviceToHost, st);
 }
 
 }  // namespace
@@ -1938,10 +1932,12 @@ static void server_free(dkm_server *sv) {
     if (!sv) return;
     for (ServeLane &l : sv->lane) {
         if (l.stream) (void)hipStreamSynchronize(l.stream);
-        void *dp[] = {l.d_q, l.d_k, l.d_L, l.d_ids, l.d_cnt, l.d_cmps, l.d_hops, l.d_status, l.d_d,
-                      l.d_spill, l.d_sphq};
+        void *dp[] = {l.d_spill, l.d_sphq};
         for (void *q : dp)
             if (q) (void)hipFree(q);
+        void *hp[] = {l.h_in, l.h_out};
+        for (void *q : hp)
+            if (q) (void)hipHostFree(q);
         if (l.done) (void)hipEventDestroy(l.done);
         if (l.stream) (void)hipStreamDestroy(l.stream);
     }
@@ -1960,6 +1956,18 @@ static dkm_status server_pump(dkm_server *sv) {
         hipError_t q = hipEventQuery(l.done);
         if (q == hipErrorNotReady) continue;
         if (q != hipSuccess) return fail(DKM_ERR_DEVICE, "serving launch: %s", hipGetErrorString(q));
+        // unpack the launch's outputs into the ring entries
+        const int64_t n = l.n, mk = sv->max_k;
+        const uint32_t *o = l.h_out;
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t a = (int64_t)((l.seq0 + (uint64_t)i) % (uint64_t)sv->capacity);
+            memcpy(sv->res_ids + a * mk, o + i * mk, sizeof(uint32_t) * mk);
+            memcpy(sv->res_d + a * mk, o + (n + i) * mk, sizeof(float) * mk);
+            sv->res_cnt[a] = o[2 * n * mk + i];
+            sv->res_cmps[a] = o[2 * n * mk + n + i];
+            sv->res_hops[a] = o[2 * n * mk + 2 * n + i];
+            sv->res_status[a] = o[2 * n * mk + 3 * n + i];
+        }
         for (int64_t i = 0; i < l.n; ++i) sv->finished.push_back(l.seq0 + (uint64_t)i);
         sv->inflight -= l.n;
         l.busy = false;
@@ -1970,31 +1978,31 @@ static dkm_status server_pump(dkm_server *sv) {
         const int64_t n = std::min<int64_t>(
             {(int64_t)(sv->tail - sv->launched), sv->per_launch, sv->slots - sv->inflight});
         const uint64_t seq0 = sv->launched;
-        const int64_t dim = sv->ix->dim, cap = sv->capacity;
-        HIP_TRY(ring_copy(sv->queries, l.d_q, seq0, n, dim, cap, true, l.stream));
-        HIP_TRY(ring_copy(sv->req_k, l.d_k, seq0, n, 1, cap, true, l.stream));
-        HIP_TRY(ring_copy(sv->req_L, l.d_L, seq0, n, 1, cap, true, l.stream));
+        const int64_t dim = sv->ix->dim, cap = sv->capacity, mk = sv->max_k;
+        // pack the requests' queries, k and L into the lane's pinned input stage
+        float *hq = (float *)l.h_in;
+        uint32_t *hk = l.h_in + n * dim, *hL = hk + n;
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t a = (int64_t)((seq0 + (uint64_t)i) % (uint64_t)cap);
+            memcpy(hq + i * dim, sv->queries + a * dim, sizeof(float) * dim);
+            hk[i] = sv->req_k[a];
+            hL[i] = sv->req_L[a];
+        }
         dkm::SearchParams p = sv->base;
-        p.queries = l.d_q;
+        p.queries = (const float *)l.h_in;
         p.nq = (int)n;
-        p.k = l.d_k;
-        p.L = l.d_L;
-        p.out_ids = l.d_ids;
-        p.out_d = l.d_d;
-        p.out_count = l.d_cnt;
-        p.out_cmps = l.d_cmps;
-        p.out_hops = l.d_hops;
-        p.status = l.d_status;
+        p.k = l.h_in + n * dim;
+        p.L = l.h_in + n * dim + n;
+        p.out_ids = l.h_out;
+        p.out_d = (float *)(l.h_out + n * mk);
+        p.out_count = l.h_out + 2 * n * mk;
+        p.out_cmps = p.out_count + n;
+        p.out_hops = p.out_count + 2 * n;
+        p.status = p.out_count + 3 * n;
         p.spill_table = l.d_spill;
         p.sph_qrec = l.d_sphq;
         hipError_t e = dkm::launch_search(p, l.stream);
         if (e != hipSuccess) return fail(DKM_ERR_DEVICE, "serving launch: %s", hipGetErrorString(e));
-        HIP_TRY(ring_copy(sv->res_ids, l.d_ids, seq0, n, sv->max_k, cap, false, l.stream));
-        HIP_TRY(ring_copy(sv->res_d, l.d_d, seq0, n, sv->max_k, cap, false, l.stream));
-        HIP_TRY(ring_copy(sv->res_cnt, l.d_cnt, seq0, n, 1, cap, false, l.stream));
-        HIP_TRY(ring_copy(sv->res_cmps, l.d_cmps, seq0, n, 1, cap, false, l.stream));
-        HIP_TRY(ring_copy(sv->res_hops, l.d_hops, seq0, n, 1, cap, false, l.stream));
-        HIP_TRY(ring_copy(sv->res_status, l.d_status, seq0, n, 1, cap, false, l.stream));
         HIP_TRY(hipEventRecord(l.done, l.stream));
         l.busy = true;
         l.seq0 = seq0;
@@ -2090,15 +2098,9 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
     for (ServeLane &l : sv->lane) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&l.done, hipEventDisableTiming);
-        if (e == hipSuccess) e = hipMalloc(&l.d_q, sizeof(float) * P * ix->dim);
-        if (e == hipSuccess) e = hipMalloc(&l.d_k, sizeof(uint32_t) * P);
-        if (e == hipSuccess) e = hipMalloc(&l.d_L, sizeof(uint32_t) * P);
-        if (e == hipSuccess) e = hipMalloc(&l.d_ids, sizeof(uint32_t) * P * max_k);
-        if (e == hipSuccess) e = hipMalloc(&l.d_d, sizeof(float) * P * max_k);
-        if (e == hipSuccess) e = hipMalloc(&l.d_cnt, sizeof(uint32_t) * P);
-        if (e == hipSuccess) e = hipMalloc(&l.d_cmps, sizeof(uint32_t) * P);
-        if (e == hipSuccess) e = hipMalloc(&l.d_hops, sizeof(uint32_t) * P);
-        if (e == hipSuccess) e = hipMalloc(&l.d_status, sizeof(uint32_t) * P);
+        const size_t in_words = (size_t)P * (ix->dim + 2), out_words = (size_t)P * (2 * max_k + 4);
+        if (e == hipSuccess) e = pinned(&l.h_in, in_words, true);
+        if (e == hipSuccess) e = pinned(&l.h_out, out_words, true);
         if (e == hipSuccess) e = hipMalloc(&l.d_spill, sizeof(uint32_t) * p.spill_slots * P);
         if (e == hipSuccess && p.sph)
             e = hipMalloc(&l.d_sphq, (size_t)dkm::sph_query_record_bytes(p.sphq) * P);

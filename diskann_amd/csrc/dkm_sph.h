@@ -127,7 +127,6 @@ __device__ inline float *sph_prepare(const SphParams &q, const float *x, float *
                                      float &ms, uint32_t &err) {
     const int lane = threadIdx.x;
     float *sh = lds;
-    float *buf = lds + sph_front(q);
     float mul = q.pre_scale;
     if (q.metric == METRIC_COSINE) {
         const float norm = __builtin_sqrtf(wave_ip(x, x, q.dim));

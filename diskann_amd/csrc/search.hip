@@ -1125,11 +1125,14 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     const int qidx = blockIdx.x;
     if (qidx >= p.nq) return;
     Lds s = carve(p, smem);
+    // the query's status bits gather in LDS and are stored once at the end (plain stores
+    // only, so the outputs may live in host memory)
+    __shared__ uint32_t status;
     QueryJob j;
     j.query = p.queries + (size_t)qidx * p.qstride;
     j.L = (int)p.L[qidx];
     j.k = (int)p.k[qidx];
-    j.err = p.status + qidx;
+    j.err = &status;
     j.spill = p.spill_table ? p.spill_table + (size_t)qidx * p.spill_slots : nullptr;
     j.rec_ids = p.rec_ids ? p.rec_ids + (size_t)qidx * p.rec_cap : nullptr;
     j.rec_d = p.rec_d ? p.rec_d + (size_t)qidx * p.rec_cap : nullptr;
@@ -1142,6 +1145,8 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     j.out_degsum = p.out_degsum ? p.out_degsum + qidx : nullptr;
     j.sph_rec = p.sph_qrec ? p.sph_qrec + (size_t)qidx * sph_query_record_bytes(p.sphq) : nullptr;
     search_query<METRIC, MODE, DIM, PQLEN>(p, s, j);
+    __syncthreads();
+    if (threadIdx.x == 0) p.status[qidx] = status;
 }
 
 // resident == nullptr: launch one block per query; otherwise report in *resident how many

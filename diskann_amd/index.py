@@ -299,7 +299,7 @@ class Server:
     """Continuous batching over one index.
 
     Requests go into a ring of `capacity` entries in pinned host memory; every submit and
-    poll launches the requests waiting at that moment as one batch on one of four
+    poll launches the requests waiting at that moment as one batch on one of eight
     streams, while at most `slots` requests are in flight (0: twice what the device holds
     at once), so requests arriving while others run overlap with them on the device.
     Each result equals what `search_batch` returns for that query alone (same k, L, beam

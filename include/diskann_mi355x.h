@@ -328,11 +328,14 @@ dkm_status dkm_search_batch_device_stats(dkm_index *ix, int strategy, const floa
  * diskann/src/graph/index.rs:2034-2070); a server is the device-side form of that loop.
  * Requests go into a ring of `capacity` entries in pinned host memory. Every submit and
  * poll call collects finished launches and launches the requests waiting at that moment
- * as one batch (uneven k and L in one launch) on one of four streams, as long as at most
+ * as one batch (uneven k and L in one launch) on one of eight streams, as long as at most
  * `slots` requests are in flight: requests that arrive while others run start in the
  * next launch and overlap with them on the device, filling compute units as earlier
- * queries finish. Admission is by device memory: each stream owns the visited-set
- * regions (HBM) of slots / 4 requests, allocated at start; submits beyond the free ring
+ * queries finish. The streams overlap on the device only with a hardware queue each: run
+ * the process with GPU_MAX_HW_QUEUES >= 10 (HIP's default, 4, makes the streams share two
+ * queues and serialise; the Python package sets 12 unless the variable is set).
+ * Admission is by device memory: each stream owns the visited-set
+ * regions (HBM) of slots / 8 requests, allocated at start; submits beyond the free ring
  * entries are refused with DKM_ERR_CAPACITY (back-pressure). Requests need
  * k <= max_k and k <= L <= max_L (L < k is rejected as Knn::new does,
  * diskann/src/graph/search/knn_search.rs:101-106). Each result equals

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -1879,6 +1880,10 @@ struct ServeLane {
     // writes the outputs there directly (each query reads its inputs once and stores its
     // outputs once), so a launch issues no copies.
     uint32_t *h_in = nullptr, *h_out = nullptr;
+    // per request of the launch: set by the kernel once that request's outputs are written
+    // (so it is returned before the rest of its launch ends), and taken by the host
+    uint32_t *h_done = nullptr;
+    std::vector<uint8_t> taken;
     uint32_t *d_spill = nullptr;
     uint8_t *d_sphq = nullptr;  // spherical: prepared queries of a launch
 };
@@ -1935,7 +1940,7 @@ static void server_free(dkm_server *sv) {
         void *dp[] = {l.d_spill, l.d_sphq};
         for (void *q : dp)
             if (q) (void)hipFree(q);
-        void *hp[] = {l.h_in, l.h_out};
+        void *hp[] = {l.h_in, l.h_out, l.h_done};
         for (void *q : hp)
             if (q) (void)hipHostFree(q);
         if (l.done) (void)hipEventDestroy(l.done);
@@ -1954,12 +1959,17 @@ static dkm_status server_pump(dkm_server *sv) {
     for (ServeLane &l : sv->lane) {
         if (!l.busy) continue;
         hipError_t q = hipEventQuery(l.done);
-        if (q == hipErrorNotReady) continue;
-        if (q != hipSuccess) return fail(DKM_ERR_DEVICE, "serving launch: %s", hipGetErrorString(q));
-        // unpack the launch's outputs into the ring entries
+        if (q != hipSuccess && q != hipErrorNotReady)
+            return fail(DKM_ERR_DEVICE, "serving launch: %s", hipGetErrorString(q));
+        const bool all = q == hipSuccess;
+        // requests whose search has finished go to the ring now (their outputs were written
+        // before their flag); the lane itself is free once the whole launch has ended
         const int64_t n = l.n, mk = sv->max_k;
         const uint32_t *o = l.h_out;
+        const volatile uint32_t *fl = l.h_done;
         for (int64_t i = 0; i < n; ++i) {
+            if (l.taken[i] || !(all || fl[i])) continue;
+            std::atomic_thread_fence(std::memory_order_acquire);
             const int64_t a = (int64_t)((l.seq0 + (uint64_t)i) % (uint64_t)sv->capacity);
             memcpy(sv->res_ids + a * mk, o + i * mk, sizeof(uint32_t) * mk);
             memcpy(sv->res_d + a * mk, o + (n + i) * mk, sizeof(float) * mk);
@@ -1967,8 +1977,10 @@ static dkm_status server_pump(dkm_server *sv) {
             sv->res_cmps[a] = o[2 * n * mk + n + i];
             sv->res_hops[a] = o[2 * n * mk + 2 * n + i];
             sv->res_status[a] = o[2 * n * mk + 3 * n + i];
+            sv->finished.push_back(l.seq0 + (uint64_t)i);
+            l.taken[i] = 1;
         }
-        for (int64_t i = 0; i < l.n; ++i) sv->finished.push_back(l.seq0 + (uint64_t)i);
+        if (!all) continue;
         sv->inflight -= l.n;
         l.busy = false;
     }
@@ -1988,7 +2000,10 @@ static dkm_status server_pump(dkm_server *sv) {
             hk[i] = sv->req_k[a];
             hL[i] = sv->req_L[a];
         }
+        for (int64_t i = 0; i < n; ++i) l.h_done[i] = 0;
+        std::fill(l.taken.begin(), l.taken.begin() + n, (uint8_t)0);
         dkm::SearchParams p = sv->base;
+        p.done = l.h_done;
         p.queries = (const float *)l.h_in;
         p.nq = (int)n;
         p.k = l.h_in + n * dim;
@@ -2101,6 +2116,8 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
         const size_t in_words = (size_t)P * (ix->dim + 2), out_words = (size_t)P * (2 * max_k + 4);
         if (e == hipSuccess) e = pinned(&l.h_in, in_words, true);
         if (e == hipSuccess) e = pinned(&l.h_out, out_words, true);
+        if (e == hipSuccess) e = pinned(&l.h_done, P, true);
+        l.taken.assign(P, 0);
         if (e == hipSuccess) e = hipMalloc(&l.d_spill, sizeof(uint32_t) * p.spill_slots * P);
         if (e == hipSuccess && p.sph)
             e = hipMalloc(&l.d_sphq, (size_t)dkm::sph_query_record_bytes(p.sphq) * P);

@@ -41,6 +41,8 @@ struct SearchParams {
     uint32_t *out_hops;    // optional
     uint32_t *out_degsum;  // optional: sum of the stored counts of expanded lists
     uint32_t *status;      // per query error bits
+    uint32_t *done;        // optional: per query flag set to 1 once its outputs are written,
+                           // after a system-scope fence (outputs in host memory)
     int qcap;              // max L + nstart in the launch
     int nlcap;             // expansion list capacity
     uint32_t hslots;       // LDS visited slots (power of two)

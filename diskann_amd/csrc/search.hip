@@ -1147,6 +1147,12 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     search_query<METRIC, MODE, DIM, PQLEN>(p, s, j);
     __syncthreads();
     if (threadIdx.x == 0) p.status[qidx] = status;
+    if (p.done) {
+        // every lane's output stores reach memory before the flag does
+        __threadfence_system();
+        __syncthreads();
+        if (threadIdx.x == 0) *(volatile uint32_t *)&p.done[qidx] = 1u;
+    }
 }
 
 // resident == nullptr: launch one block per query; otherwise report in *resident how many

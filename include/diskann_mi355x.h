@@ -339,7 +339,8 @@ dkm_status dkm_search_batch_device_stats(dkm_index *ix, int strategy, const floa
  * entries are refused with DKM_ERR_CAPACITY (back-pressure). Requests need
  * k <= max_k and k <= L <= max_L (L < k is rejected as Knn::new does,
  * diskann/src/graph/search/knn_search.rs:101-106). Each result equals
- * dkm_search_batch for the same query, k, L, beam width and strategy. slots = 0 picks
+ * dkm_search_batch for the same query, k, L, beam width and strategy, and is returned by
+ * poll as soon as its own search ends (not when the rest of its launch does). slots = 0 picks
  * twice the number of queries the device holds at once. Progress needs the caller to
  * keep calling submit or poll. The index must not be modified while a server runs.
  * Thread-safe per server. */

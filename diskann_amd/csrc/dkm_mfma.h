@@ -160,6 +160,8 @@ __device__ __forceinline__ void f8_tile(const i32x8_t *a0, const i32x8_t *a1,
 template <int KS>
 struct Bf16Rows {
     static constexpr int kBlocks = KS;
+    // the 768-dim row fragments fill the register file: no room for the mask epilogue
+    static constexpr bool kMaskEpilogue = KS < 48;
     bf16x8_t a0[KS], a1[KS];
     __device__ __forceinline__ void load(const uint16_t *ap) { res_load_rows<KS>(ap, a0, a1); }
     template <class Init, class Epi>
@@ -172,6 +174,7 @@ struct Bf16Rows {
 template <int KB>
 struct Fp8Rows {
     static constexpr int kBlocks = KB;
+    static constexpr bool kMaskEpilogue = true;
     static_assert(KB % 4 == 0, "a column tile is prefetched four blocks per wave");
     i32x8_t a0[KB / 2], a1[KB / 2];
     __device__ __forceinline__ void load(const uint16_t *ap) {

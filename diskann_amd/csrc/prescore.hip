@@ -123,11 +123,14 @@ __global__ __launch_bounds__(256) void ps_select_kernel(const float *dense, int6
 // or below tau[query] appends (key << 32 | row) to the query's list (capacity cap; the
 // counter keeps counting past it so overflow is visible), the key ordering the query's
 // entries like d. Queries with tau = -inf take no part (used when only some queries are
-// redone). qmap (may be null) maps the launch's query q to its list / counter index. Passing
-// pairs are staged in a per-wave pool in LDS during the sweep (slots
-// handed out by ballot and lane rank, no atomics) and appended to the global lists once
-// the sweep is over, so no returning atomic waits inside the tile loop; pairs beyond a
-// full pool are appended directly.
+// redone). qmap (may be null) maps the launch's query q to its list / counter index. After a
+// tile, each lane builds the 16-bit masks of its passing accumulators without branches; a
+// wave-uniform loop then takes one set bit per lane per round (a pass rate under 1 % makes
+// that one or two rounds), selects that accumulator with a binary tree of selects (static
+// register indices only) and stages the pair in the lane's own LDS slots (no atomics, no
+// ballots). The staged pairs are appended to the global lists once the sweep is over, so
+// no returning atomic waits inside the tile loop; pairs beyond a lane's slots are appended
+// directly.
 constexpr int kPool = PS_POOL;
 
 __device__ __forceinline__ void ps_append(uint32_t *count, uint64_t *lists, int64_t cap, int q,
@@ -163,7 +166,51 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
     const float h1 = l2 && r1 < n ? 0.5f * s * xn[r1] : 0.0f;
     const bool v0 = r0 < n, v1 = r1 < n;
     const float tscale = (l2 ? 0.5f : 1.0f) * s;
-    uint32_t filled = 0;  // the wave's pool entries (wave-uniform)
+    constexpr int kPer = kPool / 64;  // LDS slots per lane: entry j of lane l at j * 64 + l
+    uint32_t nst = 0;                  // mask epilogue: this lane's staged entries
+    uint32_t filled = 0;               // ballot epilogue: the wave's pool entries (uniform)
+    // passing accumulators of one row tile (acc, row r, its h, valid) after tile t
+    auto scan = [&](const floatx16_t &acc, float h, bool valid, int64_t row, int t) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) m |= (valid && acc[r] - h >= 0.0f) ? (1u << r) : 0u;
+        while (__ballot(m != 0)) {
+            if (m) {
+                const int r = __builtin_ctz(m);
+                m &= m - 1;
+                // binary tree of bitwise selects under opaque masks: a select tree the
+                // compiler can see through is folded into a dynamic element index, which
+                // it lowers through scratch memory
+                uint32_t msk[4];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    msk[b] = ((r >> b) & 1) ? ~0u : 0u;
+                    asm volatile("" : "+v"(msk[b]));
+                }
+                auto sel = [](uint32_t m, uint32_t x1, uint32_t x0) { return (x1 & m) | (x0 & ~m); };
+                uint32_t a8[8], a4[4], a2[2];
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    a8[i] = sel(msk[0], __float_as_uint(acc[2 * i + 1]), __float_as_uint(acc[2 * i]));
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a4[i] = sel(msk[1], a8[2 * i + 1], a8[2 * i]);
+#pragma unroll
+                for (int i = 0; i < 2; ++i) a2[i] = sel(msk[2], a4[2 * i + 1], a4[2 * i]);
+                const float u = __uint_as_float(sel(msk[3], a2[1], a2[0])) - h;
+                const int q = 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
+                const uint64_t key = ((uint64_t)orderable(-u) << 32) | (uint32_t)row;
+                if (nst < (uint32_t)kPer) {
+                    skey[nst * 64 + lane] = key;
+                    sq[nst * 64 + lane] = (uint16_t)q;
+                    ++nst;
+                } else {
+                    ps_append(count, lists, cap, qmap ? (int)qmap[q] : q, key);
+                }
+            }
+        }
+    };
+    // ballot epilogue (kernels whose row fragments leave no registers for the masks): one
+    // ballot per accumulator, slots of the wave's pool handed out by lane rank
     auto push = [&](bool pass, int q, uint64_t key) {
         const uint64_t m = __ballot(pass);
         if (m) {
@@ -195,27 +242,35 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
             pb = pa;
         },
         [&](int t, floatx16_t &pa, floatx16_t &pb) {
+            if constexpr (G::kMaskEpilogue) {
+                scan(pa, h0, v0, r0, t);
+                scan(pb, h1, v1, r1, t);
+            } else {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int q = 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
-                const float u0 = pa[r] - h0, u1 = pb[r] - h1;
-                push(v0 && u0 >= 0.0f, q, ((uint64_t)orderable(-u0) << 32) | (uint32_t)r0);
-                push(v1 && u1 >= 0.0f, q, ((uint64_t)orderable(-u1) << 32) | (uint32_t)r1);
+                for (int r = 0; r < 16; ++r) {
+                    const int q = 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
+                    const float u0 = pa[r] - h0, u1 = pb[r] - h1;
+                    push(v0 && u0 >= 0.0f, q, ((uint64_t)orderable(-u0) << 32) | (uint32_t)r0);
+                    push(v1 && u1 >= 0.0f, q, ((uint64_t)orderable(-u1) << 32) | (uint32_t)r1);
+                }
             }
         });
-    // the wave's pool: all counter increments first, then the stores (one round trip)
+    // the staged pairs: all counter increments first, then the stores (one round trip); in
+    // both layouts entry j of this lane sits at j * 64 + lane
     const uint32_t staged = min(filled, (uint32_t)kPool);
-    constexpr int kPer = kPool / 64;
+    auto held = [&](int j) {
+        return G::kMaskEpilogue ? (uint32_t)j < nst : (uint32_t)(j * 64 + lane) < staged;
+    };
     uint32_t pos[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const uint32_t i = (uint32_t)(j * 64 + lane);
-        if (i < staged) pos[j] = atomicAdd(&count[qmap ? qmap[sq[i]] : sq[i]], 1u);
+        if (held(j)) pos[j] = atomicAdd(&count[qmap ? qmap[sq[i]] : sq[i]], 1u);
     }
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const uint32_t i = (uint32_t)(j * 64 + lane);
-        if (i < staged && pos[j] < cap)
+        if (held(j) && pos[j] < cap)
             lists[(int64_t)(qmap ? qmap[sq[i]] : sq[i]) * cap + pos[j]] = skey[i];
     }
 }

@@ -7,12 +7,12 @@ interfaces on top of it.
 from .errors import (ANNError, CapacityError, DimensionMismatchError, KnnSearchError, PQError,
                      StorageError)
 from .index import (DiskANNIndex, FullPrecision, Hybrid, Knn, Metric, Quantized, SearchResult,
-                    Server, Strategy)
+                    SearchGraph, Server, Strategy)
 from . import minmax, prescore, quantization, scalar, spherical, storage
 
 __all__ = [
     "ANNError", "CapacityError", "DimensionMismatchError", "KnnSearchError", "PQError",
     "StorageError", "storage", "prescore",
     "DiskANNIndex", "FullPrecision", "Hybrid", "Knn", "Metric", "Quantized", "SearchResult",
-    "Server", "Strategy", "quantization", "scalar", "spherical", "minmax",
+    "SearchGraph", "Server", "Strategy", "quantization", "scalar", "spherical", "minmax",
 ]

@@ -20,6 +20,7 @@ EXPORTS = [
     "dkm_get_codes", "dkm_search_batch", "dkm_search_batch_device", "dkm_index_synchronize",
     "dkm_profile", "dkm_profile_read", "dkm_set_tuning", "dkm_build", "dkm_set_build_hybrid", "dkm_medoid",
     "dkm_bruteforce", "dkm_search_batch_device_stats", "dkm_autotune",
+    "dkm_search_graph_create", "dkm_search_graph_launch", "dkm_search_graph_destroy",
     "dkm_server_start", "dkm_server_slots", "dkm_server_submit",
     "dkm_server_submit_batch", "dkm_server_poll", "dkm_server_pending", "dkm_server_stop",
     "dkm_index_shape", "dkm_bin_write", "dkm_bin_read_shape", "dkm_bin_read",
@@ -144,6 +145,9 @@ def _declare(L):
         "dkm_prescore_search": [P, P, I64, I64, I64, P, P, C.POINTER(C.c_double)],
         "dkm_search_batch_device_stats": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P,
                                           P],
+        "dkm_search_graph_create": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P, P,
+                                    C.POINTER(C.c_void_p)],
+        "dkm_search_graph_launch": [P],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -151,6 +155,8 @@ def _declare(L):
         f.restype = C.c_int
     L.dkm_index_destroy.argtypes = [P]
     L.dkm_index_destroy.restype = None
+    L.dkm_search_graph_destroy.argtypes = [P]
+    L.dkm_search_graph_destroy.restype = None
     L.dkm_prescore_destroy.argtypes = [P]
     L.dkm_prescore_destroy.restype = None
     L.dkm_sph_destroy.argtypes = [P]

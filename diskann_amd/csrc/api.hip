@@ -1418,12 +1418,22 @@ static void set_sq_params(const dkm_index *ix, bool sq, dkm::SearchParams &p, bo
     p.sq_mean_norm = ix->sq_mean_norm;
 }
 
-// Launch on the handle's stream; caller holds ix->mu.
-static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_queries, int64_t nq,
+// Per-launch device buffers a caller owns (a captured graph) instead of the index's.
+struct OwnBufs {
+    uint32_t *spill = nullptr;  // nq regions of region_slots(max_L) slots
+    uint64_t spill_slots = 0;
+    uint8_t *sph_qrec = nullptr;  // spherical: nq prepared query records
+};
+
+// Validates a batched search and builds its launch parameters (nq >= 1); the visited-set
+// regions and spherical query records are the index's (grown as needed) unless `own`
+// supplies them. Caller holds ix->mu.
+static dkm_status search_params(dkm_index *ix, int strategy, const float *d_queries, int64_t nq,
                                 const uint32_t *d_k, const uint32_t *d_L, uint32_t max_k,
                                 uint32_t max_L, uint32_t bw, int64_t k_stride, uint32_t *d_ids,
                                 float *d_d, uint32_t *d_cnt, uint32_t *d_cmps, uint32_t *d_hops,
-                                uint32_t *d_status, uint32_t *d_degsum = nullptr) {
+                                uint32_t *d_status, uint32_t *d_degsum, const OwnBufs *own,
+                                dkm::SearchParams &p) {
     if (strategy < 0 || strategy > 2) return fail(DKM_ERR_INVALID_ARG, "unknown strategy %d", strategy);
     if (bw == 0) return fail(DKM_ERR_BEAM_WIDTH_ZERO, "beam width must be >= 1");
     if (bw > 64) return fail(DKM_ERR_INVALID_ARG, "beam width above 64 is not supported");
@@ -1443,12 +1453,12 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
         return fail(DKM_ERR_NOT_READY, "PQ strategy needs a PQ schema and codes");
     if (pq && ix->metric == DKM_COSINE)
         return fail(DKM_ERR_PQ, "cosine PQ traversal (direct distances) is not supported");
-    if (nq == 0) return DKM_OK;
-    {
+    if (nq == 0) return DKM_OK;  // validated; nothing to launch
+    if (!own) {
         dkm_status st = ensure_spill(ix, nq, region_slots(max_L));
         if (st) return st;
     }
-    dkm::SearchParams p{};
+    memset(&p, 0, sizeof(p));  // padding too: captured parameters are compared bytewise
     p.metric = ix->metric;
     p.pq = pq;
     p.rerank = strategy == DKM_STRATEGY_HYBRID;
@@ -1467,7 +1477,9 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     p.pivots = ix->d_pivots;
     p.offsets = ix->d_offsets;
     set_sq_params(ix, sq, p, sph);
-    if (sph) {
+    if (sph && own) {
+        p.sph_qrec = own->sph_qrec;
+    } else if (sph) {
         HIP_TRY(ix->d_sph_qrec.ensure(nq * dkm::sph_query_record_bytes(ix->sphq)));
         p.sph_qrec = ix->d_sph_qrec.p;
     }
@@ -1488,12 +1500,26 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     p.qcap = (int)(max_L + ix->num_start);
     p.nlcap = (int)(bw * ix->R_max + 64);
     p.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots : default_visited_slots(max_L);
-    p.spill_table = ix->d_spill;
-    p.spill_slots = ix->spill_slots;
+    p.spill_table = own ? own->spill : ix->d_spill;
+    p.spill_slots = own ? own->spill_slots : ix->spill_slots;
     p.max_iters = (uint32_t)std::min<int64_t>(ix->rows + 16, 0x7FFFFFFF);
     size_t lds = dkm::search_lds_bytes(p);
     if (lds > 160 * 1024)
         return fail(DKM_ERR_CAPACITY, "per-query LDS need %zu bytes exceeds 160 KiB", lds);
+    return DKM_OK;
+}
+
+// Launch on the handle's stream; caller holds ix->mu.
+static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_queries, int64_t nq,
+                                const uint32_t *d_k, const uint32_t *d_L, uint32_t max_k,
+                                uint32_t max_L, uint32_t bw, int64_t k_stride, uint32_t *d_ids,
+                                float *d_d, uint32_t *d_cnt, uint32_t *d_cmps, uint32_t *d_hops,
+                                uint32_t *d_status, uint32_t *d_degsum = nullptr) {
+    dkm::SearchParams p;
+    dkm_status st = search_params(ix, strategy, d_queries, nq, d_k, d_L, max_k, max_L, bw,
+                                  k_stride, d_ids, d_d, d_cnt, d_cmps, d_hops, d_status, d_degsum,
+                                  nullptr, p);
+    if (st || nq == 0) return st;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ix->profile) {
         HIP_TRY(hipEventCreate(&e0));
@@ -1538,6 +1564,131 @@ dkm_status dkm_search_batch_device_stats(dkm_index *ix, int strategy, const floa
     return launch_locked(ix, strategy, d_queries, nq, d_k, d_L, max_k, max_L, beam_width, k_stride,
                          d_out_ids, d_out_dists, d_out_counts, d_out_cmps, d_out_hops,
                          ix->s_status.p, d_out_degsum);
+}
+
+}  // extern "C"
+
+struct dkm_search_graph {
+    dkm_index *ix = nullptr;
+    int strategy = 0;
+    const float *queries = nullptr;
+    int64_t nq = 0, k_stride = 0;
+    const uint32_t *k = nullptr, *L = nullptr;
+    uint32_t max_k = 0, max_L = 0, bw = 0;
+    uint32_t *ids = nullptr, *cnt = nullptr, *cmps = nullptr, *hops = nullptr, *degsum = nullptr;
+    float *dists = nullptr;
+    DevBuf<uint32_t> spill, status;
+    DevBuf<uint8_t> qrec;
+    OwnBufs own;
+    dkm::SearchParams p{};  // what was captured
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+void dkm_search_graph_destroy(dkm_search_graph *g) {
+    if (!g) return;
+    (void)hipSetDevice(g->ix->device);
+    (void)hipStreamSynchronize(g->ix->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+}
+
+dkm_status dkm_search_graph_create(dkm_index *ix, int strategy, const float *d_queries,
+                                   int64_t nq, const uint32_t *d_k, const uint32_t *d_L,
+                                   uint32_t max_k, uint32_t max_L, uint32_t beam_width,
+                                   int64_t k_stride, uint32_t *d_out_ids, float *d_out_dists,
+                                   uint32_t *d_out_counts, uint32_t *d_out_cmps,
+                                   uint32_t *d_out_hops, uint32_t *d_out_degsum,
+                                   dkm_search_graph **out) {
+    if (!ix || !out) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (nq <= 0) return fail(DKM_ERR_INVALID_ARG, "a search graph needs at least one query");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    auto *g = new dkm_search_graph();
+    g->ix = ix;
+    g->strategy = strategy;
+    g->queries = d_queries;
+    g->nq = nq;
+    g->k = d_k;
+    g->L = d_L;
+    g->max_k = max_k;
+    g->max_L = max_L == 0 ? 1 : max_L;
+    g->bw = beam_width;
+    g->k_stride = k_stride;
+    g->ids = d_out_ids;
+    g->dists = d_out_dists;
+    g->cnt = d_out_counts;
+    g->cmps = d_out_cmps;
+    g->hops = d_out_hops;
+    g->degsum = d_out_degsum;
+    auto bad = [&](dkm_status st) {
+        delete g;
+        return st;
+    };
+    hipError_t e;
+    // the graph's own visited-set regions, status words and query records: later launches
+    // through the index may reallocate the index's, never these
+    g->own.spill_slots = region_slots(g->max_L);
+    if ((e = g->spill.ensure((int64_t)(g->own.spill_slots * (uint64_t)nq))) != hipSuccess ||
+        (e = g->status.ensure(nq)) != hipSuccess)
+        return bad(fail(e == hipErrorOutOfMemory ? DKM_ERR_OUT_OF_MEMORY : DKM_ERR_DEVICE,
+                        "search graph buffers: %s", hipGetErrorString(e)));
+    g->own.spill = g->spill.p;
+    if (ix->sph_active) {
+        if ((e = g->qrec.ensure(nq * dkm::sph_query_record_bytes(ix->sphq))) != hipSuccess)
+            return bad(fail(DKM_ERR_DEVICE, "search graph buffers: %s", hipGetErrorString(e)));
+        g->own.sph_qrec = g->qrec.p;
+    }
+    dkm_status st = search_params(ix, strategy, d_queries, nq, d_k, d_L, max_k, max_L,
+                                  beam_width, k_stride, d_out_ids, d_out_dists, d_out_counts,
+                                  d_out_cmps, d_out_hops, g->status.p, d_out_degsum, &g->own, g->p);
+    if (st) return bad(st);
+    // capture the launch (kernels only: no memset or copy nodes) on the index's stream
+    if ((e = hipStreamSynchronize(ix->stream)) != hipSuccess ||
+        (e = hipStreamBeginCapture(ix->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return bad(fail(DKM_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e)));
+    hipError_t le = dkm::launch_search(g->p, ix->stream);
+    e = hipStreamEndCapture(ix->stream, &g->graph);
+    if (le != hipSuccess || e != hipSuccess)
+        return bad(fail(DKM_ERR_DEVICE, "graph capture: %s",
+                        hipGetErrorString(le != hipSuccess ? le : e)));
+    if ((e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0)) != hipSuccess)
+        return bad(fail(DKM_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(e)));
+    *out = g;
+    return DKM_OK;
+}
+
+dkm_status dkm_search_graph_launch(dkm_search_graph *g) {
+    if (!g) return fail(DKM_ERR_INVALID_ARG, "null graph");
+    dkm_index *ix = g->ix;
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    // the captured launch is still the one the index would make: same buffers, tuning and
+    // quantizer (a changed index needs a new graph)
+    dkm::SearchParams now;
+    dkm_status st = search_params(ix, g->strategy, g->queries, g->nq, g->k, g->L, g->max_k,
+                                  g->max_L, g->bw, g->k_stride, g->ids, g->dists, g->cnt, g->cmps,
+                                  g->hops, g->status.p, g->degsum, &g->own, now);
+    if (st) return st;
+    if (memcmp(&now, &g->p, sizeof(now)) != 0)
+        return fail(DKM_ERR_NOT_READY,
+                    "the index changed since the search graph was captured; create a new one");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ix->profile) {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, ix->stream));
+    }
+    HIP_TRY(hipGraphLaunch(g->exec, ix->stream));
+    if (ix->profile) {
+        HIP_TRY(hipEventRecord(e1, ix->stream));
+        ix->events.push_back({e0, e1});
+    }
+    return DKM_OK;
 }
 
 dkm_status dkm_medoid(dkm_index *ix, float *out_vector, int64_t *out_row) {

@@ -13,6 +13,7 @@ import numpy as np
 
 from . import _native
 from .errors import (BEAM_WIDTH_ZERO, K_ZERO, L_LESS_THAN_K, L_ZERO, KnnSearchError, check)
+from .quantization import _Dev
 
 
 class Metric(enum.IntEnum):
@@ -293,6 +294,75 @@ class DiskANNIndex:
         r = self.search_batch(query, knn.k, knn.L, knn.beam_width, strategy)
         n = int(r.counts[0])
         return r.ids[0, :n], r.distances[0, :n], int(r.cmps[0]), int(r.hops[0])
+
+
+class SearchGraph:
+    """A batched search captured in a HIP graph (dkm_search_graph_*): the queries (n x dim
+    float32), k and L of each query are copied into device buffers owned by the graph;
+    `set_queries` refreshes them and `run` replays the captured launch and returns the same
+    SearchResult as DiskANNIndex.search_batch for those queries."""
+
+    def __init__(self, index, queries, k, L, beam_width=1, strategy=Strategy.FULL_PRECISION,
+                 max_k=None, max_L=None):
+        self.index, self.lib = index, index.lib
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim != 2 or q.shape[1] != index.dim:
+            raise ValueError("queries must be (n, %d)" % index.dim)
+        self.nq = q.shape[0]
+        ks = np.ascontiguousarray(np.broadcast_to(np.asarray(k, dtype=np.uint32), (self.nq,)))
+        Ls = np.ascontiguousarray(np.broadcast_to(np.asarray(L, dtype=np.uint32), (self.nq,)))
+        for kk, ll in set(zip(ks.tolist(), Ls.tolist())):
+            Knn(kk, ll, beam_width)  # Knn::new validation (the device path does none)
+        self.max_k = (int(ks.max()) if self.nq else 1) if max_k is None else int(max_k)
+        self.max_L = (int(Ls.max()) if self.nq else 1) if max_L is None else int(max_L)
+        if self.nq and (self.max_k < int(ks.max()) or self.max_L < int(Ls.max())):
+            raise ValueError("max_k / max_L below a query's k / L")
+        self._dq = _Dev(self.lib, q)
+        self._dk = _Dev(self.lib, ks)
+        self._dL = _Dev(self.lib, Ls)
+        n = self.nq
+        self._ids = _Dev(self.lib, nbytes=4 * n * self.max_k)
+        self._d = _Dev(self.lib, nbytes=4 * n * self.max_k)
+        self._cnt = _Dev(self.lib, nbytes=4 * n)
+        self._cm = _Dev(self.lib, nbytes=4 * n)
+        self._hp = _Dev(self.lib, nbytes=4 * n)
+        h = C.c_void_p()
+        check(self.lib.dkm_search_graph_create(
+            index._h, int(strategy), self._dq.ptr, n, self._dk.ptr, self._dL.ptr, self.max_k,
+            self.max_L, int(beam_width), self.max_k, self._ids.ptr, self._d.ptr, self._cnt.ptr,
+            self._cm.ptr, self._hp.ptr, None, C.byref(h)), self.lib)
+        self._h = h
+
+    def set_queries(self, queries):
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.shape != (self.nq, self.index.dim):
+            raise ValueError("queries must be (%d, %d)" % (self.nq, self.index.dim))
+        self._dq.put(q)
+
+    def run(self):
+        check(self.lib.dkm_search_graph_launch(self._h), self.lib)
+        check(self.lib.dkm_index_synchronize(self.index._h), self.lib)
+        n, mk = self.nq, self.max_k
+        return SearchResult(self._ids.get(np.uint32, (n, mk)), self._d.get(np.float32, (n, mk)),
+                            self._cnt.get(np.uint32, n), self._cm.get(np.uint32, n),
+                            self._hp.get(np.uint32, n))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            h, self._h = self._h, None
+            self.lib.dkm_search_graph_destroy(h)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

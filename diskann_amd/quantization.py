@@ -117,6 +117,12 @@ class _Dev:
         check(self.lib.dkm_memcpy_d2h(C.c_void_p(out.ctypes.data), self.ptr, out.nbytes), self.lib)
         return out
 
+    def put(self, arr):
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes != self.nbytes:
+            raise ValueError("expected %d bytes, got %d" % (self.nbytes, arr.nbytes))
+        check(self.lib.dkm_memcpy_h2d(self.ptr, C.c_void_p(arr.ctypes.data), arr.nbytes), self.lib)
+
     def __del__(self):
         try:
             self.lib.dkm_device_free(self.ptr)

@@ -162,6 +162,26 @@ dkm_status dkm_search_batch_device(dkm_index *ix, int strategy, const float *d_q
                                    uint32_t *d_out_hops);
 dkm_status dkm_index_synchronize(dkm_index *ix);
 
+/* HIP graph of one batched search launch: dkm_search_graph_create validates and captures
+ * the launch dkm_search_batch_device_stats would make with these arguments (the kernels
+ * only: no copies or memsets) into a graph with its own visited-set regions and status
+ * words; dkm_search_graph_launch replays it on the handle's stream, reading the same query
+ * and k / L buffers and writing the same outputs (refresh their contents between replays;
+ * d_out_degsum may be NULL). A replay is refused with DKM_ERR_NOT_READY when the index
+ * would now launch differently (tuning, quantizer or buffers changed since the capture).
+ * Results equal dkm_search_batch_device's. No reference counterpart: the reference has no
+ * device launches to capture; this serves callers that repeat fixed-shape batches. */
+typedef struct dkm_search_graph dkm_search_graph;
+dkm_status dkm_search_graph_create(dkm_index *ix, int strategy, const float *d_queries,
+                                   int64_t nq, const uint32_t *d_k, const uint32_t *d_L,
+                                   uint32_t max_k, uint32_t max_L, uint32_t beam_width,
+                                   int64_t k_stride, uint32_t *d_out_ids, float *d_out_dists,
+                                   uint32_t *d_out_counts, uint32_t *d_out_cmps,
+                                   uint32_t *d_out_hops, uint32_t *d_out_degsum,
+                                   dkm_search_graph **out);
+dkm_status dkm_search_graph_launch(dkm_search_graph *g);
+void dkm_search_graph_destroy(dkm_search_graph *g);
+
 /* Kernel timing with HIP events on the handle's stream: enable, then read the count and
  * total milliseconds of the search kernel launches since the last reset. */
 dkm_status dkm_profile(dkm_index *ix, int enable);

@@ -131,6 +131,7 @@ struct dkm_index {
     DevBuf<uint8_t> d_sph_bits;
     DevBuf<dkm::SphMeta> d_sph_meta;
     DevBuf<uint8_t> d_sph_qrec;          // prepared traversal queries of a launch
+    DevBuf<uint32_t> d_order;            // longest-L-first order of a launch (+ bins)
     // staging for the host-buffer API
     DevBuf<float> s_q, s_d;
     DevBuf<uint32_t> s_k, s_L, s_ids, s_cnt, s_cmps, s_hops, s_status;
@@ -1423,7 +1424,13 @@ struct OwnBufs {
     uint32_t *spill = nullptr;  // nq regions of region_slots(max_L) slots
     uint64_t spill_slots = 0;
     uint8_t *sph_qrec = nullptr;  // spherical: nq prepared query records
+    uint32_t *order = nullptr;    // launches of kOrderMin queries or more: order + bins
 };
+
+// Launches of at least this many queries run them longest L first (below it the device
+// holds all of them at once and the order does not matter).
+constexpr int64_t kOrderMin = 2048;
+static size_t order_words(int64_t nq) { return (size_t)nq + dkm::kOrderBins + 1; }
 
 // Validates a batched search and builds its launch parameters (nq >= 1); the visited-set
 // regions and spherical query records are the index's (grown as needed) unless `own`
@@ -1502,6 +1509,14 @@ static dkm_status search_params(dkm_index *ix, int strategy, const float *d_quer
     p.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots : default_visited_slots(max_L);
     p.spill_table = own ? own->spill : ix->d_spill;
     p.spill_slots = own ? own->spill_slots : ix->spill_slots;
+    if (nq >= kOrderMin) {
+        if (own) {
+            p.order = own->order;
+        } else {
+            HIP_TRY(ix->d_order.ensure((int64_t)order_words(nq)));
+            p.order = ix->d_order.p;
+        }
+    }
     p.max_iters = (uint32_t)std::min<int64_t>(ix->rows + 16, 0x7FFFFFFF);
     size_t lds = dkm::search_lds_bytes(p);
     if (lds > 160 * 1024)
@@ -1577,7 +1592,7 @@ struct dkm_search_graph {
     uint32_t max_k = 0, max_L = 0, bw = 0;
     uint32_t *ids = nullptr, *cnt = nullptr, *cmps = nullptr, *hops = nullptr, *degsum = nullptr;
     float *dists = nullptr;
-    DevBuf<uint32_t> spill, status;
+    DevBuf<uint32_t> spill, status, order;
     DevBuf<uint8_t> qrec;
     OwnBufs own;
     dkm::SearchParams p{};  // what was captured
@@ -1638,6 +1653,11 @@ dkm_status dkm_search_graph_create(dkm_index *ix, int strategy, const float *d_q
         return bad(fail(e == hipErrorOutOfMemory ? DKM_ERR_OUT_OF_MEMORY : DKM_ERR_DEVICE,
                         "search graph buffers: %s", hipGetErrorString(e)));
     g->own.spill = g->spill.p;
+    if (nq >= kOrderMin) {
+        if ((e = g->order.ensure((int64_t)order_words(nq))) != hipSuccess)
+            return bad(fail(DKM_ERR_DEVICE, "search graph buffers: %s", hipGetErrorString(e)));
+        g->own.order = g->order.p;
+    }
     if (ix->sph_active) {
         if ((e = g->qrec.ensure(nq * dkm::sph_query_record_bytes(ix->sphq))) != hipSuccess)
             return bad(fail(DKM_ERR_DEVICE, "search graph buffers: %s", hipGetErrorString(e)));

@@ -9,6 +9,9 @@ namespace dkm {
 
 enum : uint32_t { ERR_VISITED_OVERFLOW = 1u, ERR_ITER_LIMIT = 2u };
 
+// L values 0..4096 counted by the longest-L-first ordering of a launch.
+constexpr int kOrderBins = 4097;
+
 struct SearchParams {
     int metric;            // dkm_metric
     int pq;                // traverse with PQ ADC distances
@@ -41,6 +44,8 @@ struct SearchParams {
     uint32_t *out_hops;    // optional
     uint32_t *out_degsum;  // optional: sum of the stored counts of expanded lists
     uint32_t *status;      // per query error bits
+    uint32_t *order;       // optional: nq entries + (kOrderBins + 1) bins; the launch then
+                           // runs the queries longest L first (block b takes query order[b])
     uint32_t *done;        // optional: per query flag set to 1 once its outputs are written,
                            // after a system-scope fence (outputs in host memory)
     int qcap;              // max L + nstart in the launch

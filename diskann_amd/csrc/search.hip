@@ -1122,8 +1122,8 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
 template <int METRIC, int MODE, int DIM, int PQLEN>
 __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     extern __shared__ __align__(16) char smem[];
-    const int qidx = blockIdx.x;
-    if (qidx >= p.nq) return;
+    if ((int)blockIdx.x >= p.nq) return;
+    const int qidx = p.order ? (int)p.order[blockIdx.x] : (int)blockIdx.x;
     Lds s = carve(p, smem);
     // the query's status bits gather in LDS and are stored once at the end (plain stores
     // only, so the outputs may live in host memory)
@@ -1230,7 +1230,67 @@ static hipError_t dispatch(const SearchParams &p, hipStream_t stream, int *resid
 
 size_t search_lds_bytes(const SearchParams &p) { return lds_bytes(p); }
 
+// Longest-L-first order of a launch's queries (a counting sort over L, ties in any order:
+// each query's result does not depend on the order): blocks then start the longest
+// searches first and the short ones fill in behind them, which shortens the launch's tail.
+// order = nq entries followed by kOrderBins + 1 bins.
+__global__ __launch_bounds__(256) void order_clear_kernel(uint32_t *bins) {
+    for (int i = threadIdx.x + blockIdx.x * 256; i <= kOrderBins; i += gridDim.x * 256) bins[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void order_count_kernel(const uint32_t *L, int nq, uint32_t *bins) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < nq) atomicAdd(&bins[min(L[i], (uint32_t)(kOrderBins - 1))], 1u);
+}
+
+// bins[l] becomes the first position of L = l in decreasing L order (one block).
+__global__ __launch_bounds__(1024) void order_scan_kernel(uint32_t *bins) {
+    __shared__ uint32_t part[1024];
+    constexpr int per = (kOrderBins + 1023) / 1024;
+    const int t = threadIdx.x;
+    // thread t owns L values [kOrderBins - 1 - per (t + 1) + 1, kOrderBins - 1 - per t]
+    uint32_t sum = 0;
+    for (int j = 0; j < per; ++j) {
+        const int l = kOrderBins - 1 - (per * t + j);
+        if (l >= 0) sum += bins[l];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint32_t v = t >= o ? part[t - o] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;  // exclusive prefix of the larger L values
+    for (int j = 0; j < per; ++j) {
+        const int l = kOrderBins - 1 - (per * t + j);
+        if (l >= 0) {
+            const uint32_t c = bins[l];
+            bins[l] = run;
+            run += c;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void order_scatter_kernel(const uint32_t *L, int nq,
+                                                            uint32_t *bins, uint32_t *order) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < nq) order[atomicAdd(&bins[min(L[i], (uint32_t)(kOrderBins - 1))], 1u)] = (uint32_t)i;
+}
+
 hipError_t launch_search(const SearchParams &p, hipStream_t stream) {
+    if (p.order) {
+        uint32_t *bins = p.order + p.nq;
+        const unsigned g = (unsigned)((p.nq + 255) / 256);
+        hipLaunchKernelGGL(order_clear_kernel, dim3(17), dim3(256), 0, stream, bins);
+        hipLaunchKernelGGL(order_count_kernel, dim3(g), dim3(256), 0, stream, p.L, p.nq, bins);
+        hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(1024), 0, stream, bins);
+        hipLaunchKernelGGL(order_scatter_kernel, dim3(g), dim3(256), 0, stream, p.L, p.nq, bins,
+                           p.order);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     if (p.sph) {
         if (!p.sph_qrec) return hipErrorInvalidValue;
         hipError_t e = launch_sph_query_prep(p.sphq, p.sph_layout, p.queries, p.qstride, p.nq,

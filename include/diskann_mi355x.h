@@ -153,7 +153,9 @@ dkm_status dkm_search_batch(dkm_index *ix, int strategy, const float *queries, i
                             uint32_t *out_counts, uint32_t *out_cmps, uint32_t *out_hops);
 
 /* Same, with every buffer in device memory (dkm_device_alloc); asynchronous on the
- * handle's stream. Validation of k/L is the caller's duty (see dkm_search_batch). */
+ * handle's stream. Validation of k/L is the caller's duty (see dkm_search_batch).
+ * Launches of 2048 queries or more run them longest L first (a device sort of the
+ * launch's L values); no query's result depends on that order. */
 dkm_status dkm_search_batch_device(dkm_index *ix, int strategy, const float *d_queries,
                                    int64_t nq, const uint32_t *d_k, const uint32_t *d_L,
                                    uint32_t max_k, uint32_t max_L, uint32_t beam_width,

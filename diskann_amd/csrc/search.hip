@@ -1238,13 +1238,36 @@ __global__ __launch_bounds__(256) void order_clear_kernel(uint32_t *bins) {
     for (int i = threadIdx.x + blockIdx.x * 256; i <= kOrderBins; i += gridDim.x * 256) bins[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void order_count_kernel(const uint32_t *L, int nq, uint32_t *bins) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < nq) atomicAdd(&bins[min(L[i], (uint32_t)(kOrderBins - 1))], 1u);
+// Wave-aggregated bin increment: the lanes holding the same key make one atomic between
+// them (a batch of one L value would otherwise serialise on a single counter); returns the
+// lane's slot = the bin's old value + the lane's rank among those lanes.
+__device__ inline uint32_t order_slot(uint32_t *bins, uint32_t key, bool active) {
+    const int lane = threadIdx.x & 63;
+    uint32_t slot = 0;
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lk = __shfl(key, leader);
+        const uint64_t same = __ballot(active && key == lk) & pending;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&bins[lk], (uint32_t)__popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull)
+            slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    return slot;
 }
 
-// bins[l] becomes the first position of L = l in decreasing L order (one block).
-__global__ __launch_bounds__(1024) void order_scan_kernel(uint32_t *bins) {
+__global__ __launch_bounds__(256) void order_count_kernel(const uint32_t *L, int nq, uint32_t *bins) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t key = i < nq ? min(L[i], (uint32_t)(kOrderBins - 1)) : 0u;
+    (void)order_slot(bins, key, i < nq);
+}
+
+// bins[l] becomes the first position of L = l in decreasing L order (one block);
+// bins[kOrderBins] = 1 when every query has the same L (the launch then keeps its order).
+__global__ __launch_bounds__(1024) void order_scan_kernel(uint32_t *bins, int nq) {
     __shared__ uint32_t part[1024];
     constexpr int per = (kOrderBins + 1023) / 1024;
     const int t = threadIdx.x;
@@ -1252,7 +1275,10 @@ __global__ __launch_bounds__(1024) void order_scan_kernel(uint32_t *bins) {
     uint32_t sum = 0;
     for (int j = 0; j < per; ++j) {
         const int l = kOrderBins - 1 - (per * t + j);
-        if (l >= 0) sum += bins[l];
+        if (l >= 0) {
+            sum += bins[l];
+            if (bins[l] == (uint32_t)nq) bins[kOrderBins] = 1u;  // one writer at most
+        }
     }
     part[t] = sum;
     __syncthreads();
@@ -1276,7 +1302,13 @@ __global__ __launch_bounds__(1024) void order_scan_kernel(uint32_t *bins) {
 __global__ __launch_bounds__(256) void order_scatter_kernel(const uint32_t *L, int nq,
                                                             uint32_t *bins, uint32_t *order) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < nq) order[atomicAdd(&bins[min(L[i], (uint32_t)(kOrderBins - 1))], 1u)] = (uint32_t)i;
+    if (bins[kOrderBins]) {  // a single L value: keep the launch's own order
+        if (i < nq) order[i] = (uint32_t)i;
+        return;
+    }
+    const uint32_t key = i < nq ? min(L[i], (uint32_t)(kOrderBins - 1)) : 0u;
+    const uint32_t pos = order_slot(bins, key, i < nq);
+    if (i < nq) order[pos] = (uint32_t)i;
 }
 
 hipError_t launch_search(const SearchParams &p, hipStream_t stream) {
@@ -1285,7 +1317,7 @@ hipError_t launch_search(const SearchParams &p, hipStream_t stream) {
         const unsigned g = (unsigned)((p.nq + 255) / 256);
         hipLaunchKernelGGL(order_clear_kernel, dim3(17), dim3(256), 0, stream, bins);
         hipLaunchKernelGGL(order_count_kernel, dim3(g), dim3(256), 0, stream, p.L, p.nq, bins);
-        hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(1024), 0, stream, bins);
+        hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(1024), 0, stream, bins, p.nq);
         hipLaunchKernelGGL(order_scatter_kernel, dim3(g), dim3(256), 0, stream, p.L, p.nq, bins,
                            p.order);
         hipError_t e = hipGetLastError();

@@ -616,8 +616,9 @@ def served_workload(args):
     if os.path.exists(tf) and N == 10_000_000 and SNAME == "hybrid":
         with open(tf) as f:
             tr = json.load(f)
-        if (tr.get("beam_width"), tr.get("visited_lds_slots")) == (BW, HSLOTS):
-            traffic = tr.get("hbm_bytes_per_launch")
+        for run in tr.get("runs", []):
+            if (run.get("beam_width"), run.get("visited_lds_slots")) == (BW, HSLOTS):
+                traffic = run.get("hbm_bytes_per_launch")
     p99_served = [ld["p99_ms"] for ld in served["poisson"]]
     out = {
         "metric": "QPS at recall@10=0.95 + p99 latency, served batched vs plain per-query GPU",

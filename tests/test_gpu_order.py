@@ -28,9 +28,11 @@ def check_same(big, parts, n):
         assert big.distances[i, :c].tobytes() == parts.distances[i, :c].tobytes(), i
 
 
+@pytest.mark.parametrize("uniform", [False, True])
 @pytest.mark.parametrize("strategy", [dk.FullPrecision, dk.Hybrid])
-def test_large_mixed_batch_equals_small_launches(strategy):
-    log("ordered launch strategy=%d" % strategy)
+def test_large_mixed_batch_equals_small_launches(strategy, uniform):
+    # uniform: every query has the same L (the launch keeps its own order)
+    log("ordered launch strategy=%d uniform=%d" % (strategy, uniform))
     vec, adj, deg, qs = dataset(po.L2)
     ix = make_index(po.L2, vec, adj, deg)
     piv, off = pq_setup(vec, 32)
@@ -41,6 +43,8 @@ def test_large_mixed_batch_equals_small_launches(strategy):
     q = (qs[rng.integers(0, qs.shape[0], n)] +
          0.3 * rng.standard_normal((n, vec.shape[1]))).astype(np.float32)
     ks, Ls = mixed(n, 4)
+    if uniform:
+        Ls[:] = 64
     big = ix.search_batch(q, ks, Ls, 2, strategy)
     ids = np.zeros_like(big.ids)
     ds = np.zeros_like(big.distances)

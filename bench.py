@@ -6,10 +6,13 @@ built on the GPU, PQ 48 x 256 trained by k-means (10 iterations on a 100k sample
 10k requests, each with k from {10, 100} and L uniform in [32, 256] (L < k raised to k),
 searched with the Hybrid strategy (ADC traversal + FP32 rerank of the final list).
 
-One "step" = one launch searching the whole 10k-request mix, with queries and outputs
-resident in HBM; `value` is that batched throughput. Also reported: recall@10 against exact
-FP64 brute force, the served path (continuous batching: saturation QPS and p50/p99 latency
-under Poisson arrivals at 20k-200k requests/s, each result identical to the batched one),
+`value` is the served path's throughput (continuous batching over the resident index):
+the highest offered Poisson load, from 20k requests/s up, whose p99 latency (arrival to
+result) stays under 5 ms, with p50/p99 beside it; a "step" is 10k requests at that rate.
+Also reported: recall@10 against exact FP64 brute force (ties at the 10th distance
+counted), the batched launch over the whole 10k-request mix replayed from a HIP graph
+(queries and outputs resident in HBM), served saturation QPS (each served result identical
+to the batched one),
 the plain per-query GPU path (one request per launch), the search kernel's achieved
 algorithmic HBM bandwidth against MI355X's 8 TB/s, and the CPU baseline (the oracle's C
 restatement of the reference algorithm on this host's cores, same index and mix).
@@ -73,11 +76,23 @@ class DeviceBuf:
         self.lib.dkm_device_free(self.ptr)
 
 
-def recall_at_10(ids, gt):
-    hit = 0
-    for i in range(ids.shape[0]):
-        hit += len(set(ids[i, :10].tolist()) & set(gt[i, :10].tolist()))
-    return hit / (10.0 * ids.shape[0])
+def recall_at_10(ids, gt, gt_dists=None):
+    """10-recall@10 as diskann-benchmark-core scores it (recall.rs:146-300): ground-truth
+    ids tied with the 10th ground-truth distance count as hits when the distances are
+    given."""
+    from diskann_amd.recall import knn_recall
+    return knn_recall(gt, ids[:, :10], 10, 10, gt_dists)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def profile_copy(lib, gib):
@@ -379,7 +394,7 @@ def served_workload(args):
         log("graph build: %d rows, %.1fs" % (min(N, r0 + step), time.time() - t0))
     t_build = time.time() - t0
     t0 = time.time()
-    gt, _ = ix.bruteforce(qs[:NGT], 10)
+    gt, gtd = ix.bruteforce(qs[:NGT], 32)
     log("FP64 brute force of %d queries %.1fs" % (NGT, time.time() - t0))
 
     dq = DeviceBuf(lib, qs)
@@ -428,7 +443,7 @@ def served_workload(args):
             cur["bw"] = bw
             hs, am = ix.autotune(qs, 10, 144, bw, st)
             dt = timed()
-            rec = recall_at_10(dids.get(np.uint32, (NQ, KMAX))[:NGT, :10], gt)
+            rec = recall_at_10(dids.get(np.uint32, (NQ, KMAX))[:NGT, :10], gt, gtd)
             by_strategy["%s_bw%d" % (name, bw)] = {"recall_at_10": rec, "qps": NQ / dt}
             log("%s bw=%d visited_slots=%d adc_mode=%d recall=%.4f qps=%.0f" %
                 (name, bw, hs, am, rec, NQ / dt))
@@ -480,7 +495,8 @@ def served_workload(args):
     qps = NQ * args.steps / wall
     kernel_ms = kms.value / max(nlaunch.value, 1)
     ids = dids.get(np.uint32, (NQ, KMAX))
-    rec = recall_at_10(ids[:NGT, :10], gt)
+    rec = recall_at_10(ids[:NGT, :10], gt, gtd)
+    check(lib.dkm_search_status(h, None), lib)  # no query of the timed launches failed
     cmps = dcm.get(np.uint32, NQ).astype(np.float64)
     hops = dhp.get(np.uint32, NQ).astype(np.float64)
     degs = ddeg.get(np.uint32, NQ).astype(np.float64)
@@ -545,11 +561,12 @@ def served_workload(args):
             same = all(np.array_equal(served_ids[i, :ks[i]], ref_ids[i, :ks[i]])
                        for i in range(NQ))
             out["identical_to_batched"] = bool(same)
-            out["recall_at_10"] = recall_at_10(served_ids[:NGT, :10], gt)
+            out["recall_at_10"] = recall_at_10(served_ids[:NGT, :10], gt, gtd)
             log("served bw=%d saturation %.0f q/s, identical to batched: %s" % (bw, sat, same))
             rngp = np.random.default_rng(0xD15C6004)
             loads = []
-            for rate in (20_000.0, 50_000.0, 100_000.0, 200_000.0):
+            for rate in (20_000.0, 50_000.0, 100_000.0, 200_000.0, 300_000.0, 400_000.0,
+                         500_000.0, 600_000.0, 800_000.0, 1_000_000.0):
                 if rate > 0.9 * sat:
                     continue
                 nreq = int(max(20_000, rate * 0.5))
@@ -586,9 +603,18 @@ def served_workload(args):
     log("plain per-query: %s" % {b: round(v[0]) for b, v in plains.items()})
     serveds = [served_at(bw) for bw in bws]
 
-    def worst_p99(sv_):
-        return max([ld["p99_ms"] for ld in sv_["poisson"]] or [float("inf")])
-    served = min(serveds, key=worst_p99)
+    # the headline: the highest offered Poisson load whose p99 stays under 5 ms, over the
+    # measured beam widths (ties: lower p99)
+    def best_load(sv_):
+        ok = [ld for ld in sv_["poisson"] if ld["p99_ms"] < 5.0]
+        return max(ok, key=lambda ld: (ld["offered_qps"], -ld["p99_ms"])) if ok else None
+    cands = [(best_load(sv_), sv_) for sv_ in serveds]
+    cands = [c for c in cands if c[0] is not None]
+    if cands:
+        head, served = max(cands, key=lambda c: (c[0]["offered_qps"], -c[0]["p99_ms"]))
+    else:
+        served = serveds[0]
+        head = min(served["poisson"], key=lambda ld: ld["p99_ms"])
     sat = served["saturation_qps"]
 
     # CPU baseline: the oracle's C search on the same index and request mix
@@ -612,6 +638,7 @@ def served_workload(args):
     # HBM bytes per launch from the rocprofv3 counter runs (profiles/), when they were
     # taken at the settings chosen here
     traffic = None
+    traffic_source = None
     tf = os.path.join(ROOT, "profiles", "traffic_served_10M.json")
     if os.path.exists(tf) and N == 10_000_000 and SNAME == "hybrid":
         with open(tf) as f:
@@ -619,21 +646,31 @@ def served_workload(args):
         for run in tr.get("runs", []):
             if (run.get("beam_width"), run.get("visited_lds_slots")) == (BW, HSLOTS):
                 traffic = run.get("hbm_bytes_per_launch")
+                traffic_source = ("profiles/traffic_served_10M.json: rocprofv3 --pmc FETCH_SIZE / "
+                                  "WRITE_SIZE runs of this workload at the same beam width and "
+                                  "visited slots (not measured in this run)")
     p99_served = [ld["p99_ms"] for ld in served["poisson"]]
     out = {
         "metric": "QPS at recall@10=0.95 + p99 latency, served batched vs plain per-query GPU",
-        "value": qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": float(np.mean(step_ms)),
+        "value": head["achieved_qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * NQ / head["achieved_qps"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {
             "workload": "served search: %d x 96 f32 GMM(1024, sigma 0.3), L2, Vamana R=64 "
                         "alpha=1.2, PQ 48x256, requests k in {10, 100}, L ~ U[32, 256] "
                         "(L < k raised to k), %s traversal" % (N, SNAME),
+            "value_is": "served (continuous batching) QPS at the highest offered Poisson load "
+                        "whose p99 latency is under 5 ms; a step is %d requests at that rate" % NQ,
+            "served_p50_ms": head["p50_ms"], "served_p99_ms": head["p99_ms"],
+            "served_offered_qps": head["offered_qps"], "served_beam_width": served["beam_width"],
             "n": N, "dim": D, "requests_in_mix": NQ, "strategy": SNAME, "beam_width": BW,
             "visited_lds_slots": HSLOTS, "adc_mode": ADC, "recall_at_10": rec,
             "recall_queries": NGT, "by_strategy": by_strategy,
+            "recall": "10-recall@10 vs FP64 brute force, ties at the 10th distance counted "
+                      "(recall.rs:146-300)",
             "batched_qps": qps, "batched_launch": "HIP graph replay",
+            "batched_ms_per_launch": float(np.mean(step_ms)),
             "graph_replay_identical_to_eager": graph_same,
             "batched_p50_ms": float(np.percentile(step_ms, 50)),
             "batched_p99_ms": float(np.percentile(step_ms, 99)),
@@ -651,13 +688,14 @@ def served_workload(args):
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
             "kernel": "beam_search_kernel<L2,PQ>" if SNAME == "hybrid" else
                       "beam_search_kernel<L2,FP>",
             "kernel_ms": kernel_ms, "algorithmic_bytes_per_launch": alg_bytes,
         },
         "cpu_baseline": {
             "value": cpu_qps, "unit": "queries/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": "%d requests of the mix, same index, %s beam %d, static partition over "
                       "%d threads (oracle/oracle.c)" % (ncpu, SNAME, BW, threads),
         },
@@ -993,7 +1031,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_source,
             "kernel": ("beam_search_kernel<L2,SQ8>" if SQ else "beam_search_kernel<SPH1,Q4>"
                        if SPH else "beam_search_kernel<L2,PQ>"),
             "kernel_ms": kernel_ms,
@@ -1001,6 +1039,7 @@ def main():
         },
         "cpu_baseline": {
             "value": cpu_qps, "unit": "queries/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": "%d of the 10k queries, same index, Hybrid L=%d beam %d, static "
                       "partition over %d threads (oracle/oracle.c)" % (ncpu, L_star, BW, threads),
         },

@@ -15,12 +15,14 @@ EXPORTS = [
     "dkm_memcpy_h2d", "dkm_memcpy_d2h", "dkm_device_copy", "dkm_synchronize",
     "dkm_distance_batch",
     "dkm_pq_chunk_offsets", "dkm_pq_lut", "dkm_pq_encode", "dkm_pq_train",
+    "dkm_pq_train_draws",
     "dkm_index_create", "dkm_index_destroy", "dkm_set_vectors", "dkm_set_start_points",
     "dkm_set_graph", "dkm_get_graph", "dkm_set_pq", "dkm_set_codes", "dkm_encode_index",
     "dkm_get_codes", "dkm_search_batch", "dkm_search_batch_device", "dkm_index_synchronize",
     "dkm_profile", "dkm_profile_read", "dkm_set_tuning", "dkm_build", "dkm_set_build_hybrid", "dkm_medoid",
     "dkm_bruteforce", "dkm_search_batch_device_stats", "dkm_autotune",
     "dkm_search_graph_create", "dkm_search_graph_launch", "dkm_search_graph_destroy",
+    "dkm_search_status", "dkm_search_graph_status", "dkm_set_visited_capacity",
     "dkm_server_start", "dkm_server_slots", "dkm_server_submit",
     "dkm_server_submit_batch", "dkm_server_poll", "dkm_server_pending", "dkm_server_stop",
     "dkm_index_shape", "dkm_bin_write", "dkm_bin_read_shape", "dkm_bin_read",
@@ -75,6 +77,7 @@ def _declare(L):
         "dkm_pq_lut": [I, P, I64, I64, P, I64, P, I64, P],
         "dkm_pq_encode": [P, I64, I64, P, I64, P, I64, P],
         "dkm_pq_train": [P, I64, I64, I64, I64, I64, C.c_uint64, P, P],
+        "dkm_pq_train_draws": [P, I64, I64, I64, I64, I64, P, P, P],
         "dkm_pq_distance_batch": [I, P, I64, I64, P, I64, P, P, I64, P, I64, P, I64, P],
         "dkm_index_create": [I, I64, I64, I64, I64, I64, C.POINTER(C.c_void_p)],
         "dkm_set_vectors": [P, P, I64, I64],
@@ -148,6 +151,9 @@ def _declare(L):
         "dkm_search_graph_create": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P, P,
                                     C.POINTER(C.c_void_p)],
         "dkm_search_graph_launch": [P],
+        "dkm_search_graph_status": [P, P],
+        "dkm_search_status": [P, P],
+        "dkm_set_visited_capacity": [P, I64],
     }
     for name, args in sig.items():
         f = getattr(L, name)

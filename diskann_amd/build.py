@@ -5,6 +5,7 @@
 Flags: -ffp-contract=off keeps every reduction in the reference's exact order (only
 explicit fmaf fuses); packed-fp32 instructions are disabled; f32 denormals are kept.
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -37,16 +38,18 @@ def flags():
 
 def build(verbose=False):
     cc = hipcc()
-    objs = []
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
-    for src in SOURCES:
-        obj = os.path.join(bdir, src.replace(".hip", ".o"))
-        cmd = [cc] + flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
+    objs = [os.path.join(bdir, src.replace(".hip", ".o")) for src in SOURCES]
+    cmds = [[cc] + flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
+            for src, obj in zip(SOURCES, objs)]
+    if verbose:
+        for cmd in cmds:
             print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
-        objs.append(obj)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        for f in [pool.submit(subprocess.check_call, cmd) for cmd in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     cmd = [cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:

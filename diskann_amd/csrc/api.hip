@@ -113,6 +113,7 @@ struct dkm_index {
     int64_t spill_regions = 0;          // allocated regions (one per query of a launch)
     uint64_t spill_slots = 0;           // slots per region (power of two)
     int64_t tune_hslots = 0;
+    uint64_t tune_spill = 0;            // HBM visited slots per query (0 = by max L)
     int tune_adc_mode = 1;              // PQ distances: 0 = LDS table, 1 = direct
     int pq_len = 0;                     // uniform PQ chunk width usable by the fast path
     int64_t build_max_fp = -1;          // Hybrid prune: full-precision ids per prune (-1 = all)
@@ -135,6 +136,7 @@ struct dkm_index {
     // staging for the host-buffer API
     DevBuf<float> s_q, s_d;
     DevBuf<uint32_t> s_k, s_L, s_ids, s_cnt, s_cmps, s_hops, s_status;
+    int64_t status_nq = 0;  // queries of the latest device-buffer launch (dkm_search_status)
     // profiling
     bool profile = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -360,19 +362,43 @@ dkm_status dkm_pq_encode(const float *pivots, int64_t K, int64_t dim, const int6
     return DKM_OK;
 }
 
-dkm_status dkm_pq_train(const float *data, int64_t n, int64_t dim, int64_t M, int64_t K,
-                        int64_t iters, uint64_t seed, float *pivots, int64_t *offsets) {
-    if (n <= 0 || K <= 0 || K > 256) return fail(DKM_ERR_KMEANS, "bad k-means sizes");
-    if (n < K) return fail(DKM_ERR_KMEANS, "need at least K=%lld training rows (got %lld)",
-                           (long long)K, (long long)n);
+dkm_status dkm_pq_train_draws(const float *data, int64_t n, int64_t dim, int64_t M, int64_t K,
+                              int64_t iters, const double *draws, float *pivots,
+                              int64_t *offsets) {
+    if (!data || !draws || !pivots || !offsets) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (n <= 0) return fail(DKM_ERR_PQ, "PQ training needs at least one row");
+    if (K <= 0 || K > 256) return fail(DKM_ERR_PQ, "K must be in [1, 256] (got %lld)", (long long)K);
+    if (iters < 0) return fail(DKM_ERR_INVALID_ARG, "negative Lloyd iteration count");
     dkm_status st = dkm_pq_chunk_offsets(dim, M, offsets);
     if (st) return st;
+    int64_t maxds = 0;
+    for (int64_t m = 0; m < M; ++m) maxds = std::max(maxds, offsets[m + 1] - offsets[m]);
+    if (maxds > 64)
+        return fail(DKM_ERR_PQ, "chunk width %lld above 64 is not supported", (long long)maxds);
+    for (int64_t i = 0; i < M * K; ++i)
+        if (!(draws[i] >= 0.0 && draws[i] < 1.0))
+            return fail(DKM_ERR_INVALID_ARG, "draw %lld is not in [0, 1)", (long long)i);
     HIP_TRY(hipSetDevice(g_device));
     std::vector<int32_t> o32(offsets, offsets + M + 1);
+    std::vector<uint32_t> cs(2 * (size_t)M, 0u);
     hipError_t e = dkm::pq_train_device(data, n, (int)dim, o32.data(), (int)M, (int)K, (int)iters,
-                                        seed, pivots);
+                                        draws, pivots, cs.data());
+    for (int64_t m = 0; m < M; ++m)
+        if (cs[2 * m] == 3 || (e == hipErrorInvalidValue && cs[2 * m + 1] == 0))
+            return fail(DKM_ERR_PQ, "pq training failed on chunk %lld of %lld: k-means++ %s",
+                        (long long)m, (long long)M,
+                        cs[2 * m] == 3 ? "saw an infinite or NaN value" : "picked no centre");
     if (e != hipSuccess) return fail(DKM_ERR_DEVICE, "PQ training: %s", hipGetErrorString(e));
     return DKM_OK;
+}
+
+dkm_status dkm_pq_train(const float *data, int64_t n, int64_t dim, int64_t M, int64_t K,
+                        int64_t iters, uint64_t seed, float *pivots, int64_t *offsets) {
+    if (M <= 0 || K <= 0 || K > 256) return fail(DKM_ERR_PQ, "bad PQ training sizes");
+    std::vector<double> draws((size_t)(M * K));
+    for (int64_t m = 0; m < M; ++m)
+        for (int64_t c = 0; c < K; ++c) draws[(size_t)(m * K + c)] = dkm::pq_train_draw(seed, (int)m, (uint64_t)c);
+    return dkm_pq_train_draws(data, n, dim, M, K, iters, draws.data(), pivots, offsets);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1462,7 +1488,7 @@ static dkm_status search_params(dkm_index *ix, int strategy, const float *d_quer
         return fail(DKM_ERR_PQ, "cosine PQ traversal (direct distances) is not supported");
     if (nq == 0) return DKM_OK;  // validated; nothing to launch
     if (!own) {
-        dkm_status st = ensure_spill(ix, nq, region_slots(max_L));
+        dkm_status st = ensure_spill(ix, nq, std::max<uint64_t>(region_slots(max_L), ix->tune_spill));
         if (st) return st;
     }
     memset(&p, 0, sizeof(p));  // padding too: captured parameters are compared bytewise
@@ -1508,7 +1534,9 @@ static dkm_status search_params(dkm_index *ix, int strategy, const float *d_quer
     p.nlcap = (int)(bw * ix->R_max + 64);
     p.hslots = ix->tune_hslots > 0 ? (uint32_t)ix->tune_hslots : default_visited_slots(max_L);
     p.spill_table = own ? own->spill : ix->d_spill;
-    p.spill_slots = own ? own->spill_slots : ix->spill_slots;
+    // a capacity override (dkm_set_visited_capacity) lays the regions out at its own stride,
+    // inside an allocation sized for at least that many slots per query
+    p.spill_slots = own ? own->spill_slots : (ix->tune_spill ? ix->tune_spill : ix->spill_slots);
     if (nq >= kOrderMin) {
         if (own) {
             p.order = own->order;
@@ -1550,6 +1578,25 @@ static dkm_status launch_locked(dkm_index *ix, int strategy, const float *d_quer
     return DKM_OK;
 }
 
+// Maps the per-query status words of a launch to the first failing query's error
+// (the checks dkm_search_batch makes after its copy-back). *first = -1 when none failed.
+static dkm_status status_words(const uint32_t *status, int64_t nq, int64_t *first) {
+    if (first) *first = -1;
+    for (int64_t i = 0; i < nq; ++i) {
+        if (!status[i]) continue;
+        if (first) *first = i;
+        if (status[i] & dkm::ERR_BAD_REQUEST)
+            return fail(DKM_ERR_INVALID_ARG,
+                        "query %lld: k / L outside Knn::new's rules or the launch's max_k / max_L",
+                        (long long)i);
+        if (status[i] & dkm::ERR_VISITED_OVERFLOW)
+            return fail(DKM_ERR_CAPACITY, "query %lld: visited set overflowed its LDS and spill capacity",
+                        (long long)i);
+        return fail(DKM_ERR_INDEX, "query %lld: traversal exceeded the iteration bound", (long long)i);
+    }
+    return DKM_OK;
+}
+
 dkm_status dkm_search_batch_device(dkm_index *ix, int strategy, const float *d_queries,
                                    int64_t nq, const uint32_t *d_k, const uint32_t *d_L,
                                    uint32_t max_k, uint32_t max_L, uint32_t beam_width,
@@ -1560,6 +1607,7 @@ dkm_status dkm_search_batch_device(dkm_index *ix, int strategy, const float *d_q
     std::lock_guard<std::mutex> g(ix->mu);
     HIP_TRY(hipSetDevice(ix->device));
     HIP_TRY(ix->s_status.ensure(std::max<int64_t>(nq, 1)));
+    ix->status_nq = nq;
     return launch_locked(ix, strategy, d_queries, nq, d_k, d_L, max_k, max_L, beam_width, k_stride,
                          d_out_ids, d_out_dists, d_out_counts, d_out_cmps, d_out_hops,
                          ix->s_status.p);
@@ -1576,9 +1624,23 @@ dkm_status dkm_search_batch_device_stats(dkm_index *ix, int strategy, const floa
     std::lock_guard<std::mutex> g(ix->mu);
     HIP_TRY(hipSetDevice(ix->device));
     HIP_TRY(ix->s_status.ensure(std::max<int64_t>(nq, 1)));
+    ix->status_nq = nq;
     return launch_locked(ix, strategy, d_queries, nq, d_k, d_L, max_k, max_L, beam_width, k_stride,
                          d_out_ids, d_out_dists, d_out_counts, d_out_cmps, d_out_hops,
                          ix->s_status.p, d_out_degsum);
+}
+
+dkm_status dkm_search_status(dkm_index *ix, int64_t *first_failed) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    if (first_failed) *first_failed = -1;
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    if (ix->status_nq <= 0) return DKM_OK;
+    std::vector<uint32_t> status((size_t)ix->status_nq);
+    HIP_TRY(hipMemcpy(status.data(), ix->s_status.p, sizeof(uint32_t) * status.size(),
+                      hipMemcpyDeviceToHost));
+    return status_words(status.data(), ix->status_nq, first_failed);
 }
 
 }  // extern "C"
@@ -1647,7 +1709,7 @@ dkm_status dkm_search_graph_create(dkm_index *ix, int strategy, const float *d_q
     hipError_t e;
     // the graph's own visited-set regions, status words and query records: later launches
     // through the index may reallocate the index's, never these
-    g->own.spill_slots = region_slots(g->max_L);
+    g->own.spill_slots = ix->tune_spill ? ix->tune_spill : region_slots(g->max_L);
     if ((e = g->spill.ensure((int64_t)(g->own.spill_slots * (uint64_t)nq))) != hipSuccess ||
         (e = g->status.ensure(nq)) != hipSuccess)
         return bad(fail(e == hipErrorOutOfMemory ? DKM_ERR_OUT_OF_MEMORY : DKM_ERR_DEVICE,
@@ -1709,6 +1771,19 @@ dkm_status dkm_search_graph_launch(dkm_search_graph *g) {
         ix->events.push_back({e0, e1});
     }
     return DKM_OK;
+}
+
+dkm_status dkm_search_graph_status(dkm_search_graph *g, int64_t *first_failed) {
+    if (!g) return fail(DKM_ERR_INVALID_ARG, "null graph");
+    if (first_failed) *first_failed = -1;
+    dkm_index *ix = g->ix;
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    HIP_TRY(hipStreamSynchronize(ix->stream));
+    std::vector<uint32_t> status((size_t)g->nq);
+    HIP_TRY(hipMemcpy(status.data(), g->status.p, sizeof(uint32_t) * status.size(),
+                      hipMemcpyDeviceToHost));
+    return status_words(status.data(), g->nq, first_failed);
 }
 
 dkm_status dkm_medoid(dkm_index *ix, float *out_vector, int64_t *out_row) {
@@ -1813,14 +1888,8 @@ dkm_status dkm_search_batch(dkm_index *ix, int strategy, const float *queries, i
         HIP_TRY(hipMemcpyAsync(out_hops, ix->s_hops.p, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, ix->stream));
     HIP_TRY(hipMemcpyAsync(status.data(), ix->s_status.p, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost, ix->stream));
     HIP_TRY(hipStreamSynchronize(ix->stream));
-    for (int64_t i = 0; i < nq; ++i) {
-        if (status[i] & dkm::ERR_VISITED_OVERFLOW)
-            return fail(DKM_ERR_CAPACITY, "query %lld: visited set overflowed its LDS and spill capacity",
-                        (long long)i);
-        if (status[i] & dkm::ERR_ITER_LIMIT)
-            return fail(DKM_ERR_INDEX, "query %lld: traversal exceeded the iteration bound", (long long)i);
-    }
-    return DKM_OK;
+    ix->status_nq = 0;  // the staging status words now belong to this host call
+    return status_words(status.data(), nq, nullptr);
 }
 
 
@@ -2389,6 +2458,9 @@ dkm_status dkm_server_poll(dkm_server *sv, int64_t max_results, uint64_t *tags, 
         if (status) {
             // the failing request is consumed; the n results before it are valid
             *n_out = n;
+            if (status & dkm::ERR_BAD_REQUEST)
+                return fail(DKM_ERR_INVALID_ARG, "request %llu: k / L outside the server's limits",
+                            (unsigned long long)seq);
             if (status & dkm::ERR_VISITED_OVERFLOW)
                 return fail(DKM_ERR_CAPACITY, "request %llu: visited set overflowed",
                             (unsigned long long)seq);
@@ -2681,6 +2753,14 @@ dkm_status dkm_set_tuning(dkm_index *ix, int64_t visited_lds_slots, int adc_mode
     return DKM_OK;
 }
 
+dkm_status dkm_set_visited_capacity(dkm_index *ix, int64_t slots) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    if (slots < 0 || (slots > 0 && (slots < 256 || slots > (int64_t(1) << 24) || (slots & (slots - 1)))))
+        return fail(DKM_ERR_INVALID_ARG, "visited capacity must be 0 or a power of two in [256, 2^24]");
+    std::lock_guard<std::mutex> g(ix->mu);
+    ix->tune_spill = (uint64_t)slots;
+    return DKM_OK;
+}
 
 // -------------------------------------------------------------------------------------
 // Candidate pre-scoring (BF16 MFMA approximate scores, exact FP32 rescore of the best C)

@@ -91,6 +91,7 @@ __device__ inline float metric_distance(int metric, const float *a, const float 
                                         int hp) {
     if (metric == METRIC_L2) return half_distance<METRIC_L2>(a, b, dim, hp);
     if (metric == METRIC_IP) return half_distance<METRIC_IP>(a, b, dim, hp);
+    if (metric == METRIC_COSNORM) return half_distance<METRIC_COSNORM>(a, b, dim, hp);
     return half_distance<METRIC_COSINE>(a, b, dim, hp);
 }
 

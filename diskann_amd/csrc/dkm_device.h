@@ -139,7 +139,9 @@ __device__ __forceinline__ float half_cosine(const float *q, const float *x, int
     return 1.0f - sim;
 }
 
-// Full-precision distance by one half-wave, with the reference's PostOps.
+// Full-precision distance by one half-wave, with the reference's PostOps. f32
+// CosineNormalized has its own functor (inner-product loop, 1 - x); only the integer
+// element types redirect it to Cosine (distance_provider.rs:274-299).
 template <int METRIC>
 __device__ __forceinline__ float half_distance(const float *q, const float *x, int dim, int p) {
     if (METRIC == METRIC_L2) {
@@ -150,6 +152,11 @@ __device__ __forceinline__ float half_distance(const float *q, const float *x, i
         if (dim == 768) return -half_simd_4x1_fixed<true, 768>(q, x, p);
         if (dim == 128) return -half_simd_4x1_fixed<true, 128>(q, x, p);
         return -half_simd_4x1<true>(q, x, dim, p);
+    } else if (METRIC == METRIC_COSNORM) {
+        // CosineNormalized for f32: the inner-product loop with post-op 1 - x
+        // (diskann-vector/src/distance/implementations.rs:330-332, 368-402)
+        if (dim == 768) return 1.0f - half_simd_4x1_fixed<true, 768>(q, x, p);
+        return 1.0f - half_simd_4x1<true>(q, x, dim, p);
     } else {
         return half_cosine(q, x, dim, p);
     }

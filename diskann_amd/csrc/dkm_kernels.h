@@ -27,8 +27,12 @@ hipError_t launch_pq_encode(const float *pivots, int K, int dim, const int32_t *
                             const float *x, int64_t x_stride, int64_t n, uint8_t *codes,
                             int64_t code_stride, uint32_t *bad_row, hipStream_t stream);
 
-// Full PQ training on the device (host data in, host pivots out).
+// Full PQ training on the device (host data in, host pivots out). draws: M x K uniforms in
+// [0, 1) for k-means++; chunk_status: 2 x M words (failure reason, centres picked).
 hipError_t pq_train_device(const float *data, int64_t n, int dim, const int32_t *offsets, int M,
-                           int K, int iters, uint64_t seed, float *pivots);
+                           int K, int iters, const double *draws, float *pivots,
+                           uint32_t *chunk_status);
+// Draw `ctr` of chunk `chunk` of the library's k-means++ stream for `seed`.
+double pq_train_draw(uint64_t seed, int chunk, uint64_t ctr);
 
 }  // namespace dkm

@@ -7,7 +7,10 @@
 
 namespace dkm {
 
-enum : uint32_t { ERR_VISITED_OVERFLOW = 1u, ERR_ITER_LIMIT = 2u };
+// Per-query status bits. ERR_BAD_REQUEST: the query's k / L break Knn::new's rules
+// (k, L >= 1, k <= L; knn_search.rs:101-106) or the launch's capacity (L <= max_L,
+// k <= k_stride); the query is not searched and returns no results.
+enum : uint32_t { ERR_VISITED_OVERFLOW = 1u, ERR_ITER_LIMIT = 2u, ERR_BAD_REQUEST = 4u };
 
 // L values 0..4096 counted by the longest-L-first ordering of a launch.
 constexpr int kOrderBins = 4097;

@@ -110,6 +110,7 @@ __global__ __launch_bounds__(256) void bruteforce_kernel(int metric, const float
                     yy += (double)xr[e] * (double)xr[e];
                 }
                 if (metric == METRIC_IP) d = -xy;
+                else if (metric == METRIC_COSNORM) d = 1.0 - xy;
                 else d = (qq == 0.0 || yy == 0.0) ? 1.0 : 1.0 - xy / (sqrt(qq) * sqrt(yy));
             }
         }

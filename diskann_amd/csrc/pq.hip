@@ -53,6 +53,10 @@ hipError_t launch_distance_batch(int metric, int dim, const float *queries, int6
         hipLaunchKernelGGL(distance_batch_kernel<METRIC_IP>, grid, dim3(256), 0, stream, dim,
                            queries, nq, base, base_stride, ids, m, out);
         break;
+    case METRIC_COSNORM:
+        hipLaunchKernelGGL(distance_batch_kernel<METRIC_COSNORM>, grid, dim3(256), 0, stream, dim,
+                           queries, nq, base, base_stride, ids, m, out);
+        break;
     default:
         hipLaunchKernelGGL(distance_batch_kernel<METRIC_COSINE>, grid, dim3(256), 0, stream, dim,
                            queries, nq, base, base_stride, ids, m, out);
@@ -204,139 +208,212 @@ hipError_t launch_pq_encode(const float *pivots, int K, int dim, const int32_t *
 // PQ training
 // ---------------------------------------------------------------------------------
 
-__device__ __host__ inline uint64_t splitmix64(uint64_t x) {
+static uint64_t splitmix64(uint64_t x) {
     x += 0x9e3779b97f4a7c15ull;
     x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
     x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
     return x ^ (x >> 31);
 }
-// Uniform double in [0, 1) for draw `ctr` of chunk `chunk`.
-__device__ __host__ inline double draw01(uint64_t seed, int chunk, uint64_t ctr) {
+
+// Uniform double in [0, 1) (53-bit grid) for draw `ctr` of chunk `chunk`: the library's
+// own counter-based SplitMix64 stream in place of the reference's rand/ChaCha12.
+double pq_train_draw(uint64_t seed, int chunk, uint64_t ctr) {
     uint64_t s = splitmix64(seed ^ splitmix64(0x5851f42d4c957f2dull * (uint64_t)(chunk + 1)));
     uint64_t v = splitmix64(s + ctr);
     return (double)(v >> 11) * (1.0 / 9007199254740992.0);
 }
 
-// norms[m*n + r] = square_norm(x[r, chunk m])
-__global__ void chunk_norms_kernel(const float *x, int64_t n, int dim, const int32_t *offsets,
-                                   float *norms) {
+// Per-chunk copy of the training rows: xc[(m * n + r) * maxds + d] = x[r, off_m + d], and
+// norms[m * n + r] = square_norm of that slice (diskann-quantization/src/algorithms/kmeans/
+// common.rs:9-65); the reference takes the same strided view per chunk (train.rs:155-166).
+__global__ __launch_bounds__(256) void chunk_rows_kernel(const float *x, int64_t n, int dim,
+                                                         const int32_t *offsets, int maxds,
+                                                         float *xc, float *norms) {
     const int m = blockIdx.y;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    const int o0 = offsets[m], o1 = offsets[m + 1];
-    norms[(int64_t)m * n + r] = square_norm_ref(x + r * dim + o0, o1 - o0);
+    const int o0 = offsets[m], ds = offsets[m + 1] - o0;
+    float *dst = xc + ((int64_t)m * n + r) * maxds;
+    for (int d = 0; d < maxds; ++d) dst[d] = d < ds ? x[r * dim + o0 + d] : 0.0f;
+    norms[(int64_t)m * n + r] = square_norm_ref(dst, ds);
 }
 
-// k-means++ seeding, one 256-thread block per chunk; rows split in 256 contiguous
-// segments (thread t owns rows [t*n/256, (t+1)*n/256)).
-__global__ __launch_bounds__(256) void kmeanspp_kernel(const float *x, int64_t n, int dim,
+// rand 0.9 UniformFloat::<f64>::new(0.0, s): an empty range (!(0 < s), NaN included) or a
+// non-finite scale fail; otherwise the scale s is lowered one ulp at a time while
+// scale * (1 - 2^-52) + 0 >= s, and a draw u in [0, 1) maps to u * scale + 0.
+// Returns 0 ok, 1 empty range, 2 non-finite.
+__device__ inline int uniform_scale(double s, double *scale) {
+    if (!(0.0 < s)) return 1;
+    double sc = s - 0.0;
+    if (!__builtin_isfinite(sc)) return 2;
+    const double max_rand = 1.0 - 2.220446049250313e-16;
+    while (sc * max_rand + 0.0 >= s) sc = __longlong_as_double(__double_as_longlong(sc) - 1);
+    *scale = sc;
+    return 0;
+}
+
+constexpr int kKppTile = 4096;
+
+// k-means++ seeding (kmeans_plusplus_into_inner + update_distances, diskann-quantization/
+// src/algorithms/kmeans/plusplus.rs:238-498, micro-kernel :84-231), one 256-thread block per
+// chunk, draws[m * K + c] standing in for the reference's RNG:
+//  - first centre: row floor(draws[0] * n);
+//  - each step: distance = (norm_x + norm_c) + dot * -2 with dot a sequential fma over the
+//    chunk's dims, min_d lowered where distance < min_d; one thread per 16-row block forms
+//    sum_{j<8} ((f64) min_d[j] + (f64) min_d[j + 8]) and one lane adds the blocks in order
+//    (the micro-kernel's rolling f64 sum);
+//  - threshold = draws[c] * scale(0, s); one lane walks the rows in order with an f64
+//    running sum (rows staged through LDS) and picks the first row with running >= threshold,
+//    min_d > 0 and not yet picked.
+// status[2m] = 0 ok / 1 DatasetTooSmall / 2 InsufficientDiversity / 3 SawInfinity,
+// status[2m + 1] = centres picked. Unpicked centres stay zero.
+__global__ __launch_bounds__(256) void kmeanspp_kernel(const float *xc, int64_t n,
                                                        const int32_t *offsets, const float *norms,
-                                                       int K, uint64_t seed, float *mind_all,
-                                                       uint8_t *picked_all, float *centers,
-                                                       int maxds, uint32_t *status) {
+                                                       int K, const double *draws, float *mind_all,
+                                                       uint8_t *picked_all, double *bsum_all,
+                                                       float *centers, int maxds, uint32_t *status) {
     const int m = blockIdx.x;
-    const int o0 = offsets[m], o1 = offsets[m + 1], ds = o1 - o0;
+    const int ds = offsets[m + 1] - offsets[m];
     const int t = threadIdx.x;
+    const int64_t nblk = (n + 15) / 16;
+    const float *xm = xc + (int64_t)m * n * maxds;
     float *mind = mind_all + (int64_t)m * n;
     uint8_t *picked = picked_all + (int64_t)m * n;
+    double *bsum = bsum_all + (int64_t)m * nblk;
     const float *nrm = norms + (int64_t)m * n;
     float *cent = centers + (int64_t)m * K * maxds;
-    __shared__ double segsum[256];
-    __shared__ double segoff[256];
+    const double *u = draws + (int64_t)m * K;
     __shared__ float cur[64];
     __shared__ float cur_norm;
-    __shared__ uint32_t best;
-    __shared__ int stop;
-    const int64_t lo = n * t / 256, hi = n * (t + 1) / 256;
-    for (int64_t r = lo; r < hi; ++r) {
+    __shared__ double thr;
+    __shared__ int64_t found;
+    __shared__ int state;
+    __shared__ int selected;
+    __shared__ float tmind[kKppTile];
+    __shared__ uint8_t tpick[kKppTile];
+    for (int64_t r = t; r < n; r += 256) {
         mind[r] = __builtin_inff();
         picked[r] = 0;
     }
     for (int e = t; e < K * maxds; e += 256) cent[e] = 0.0f;
     __syncthreads();
     if (t == 0) {
-        uint64_t i0 = (uint64_t)(draw01(seed, m, 0) * (double)n);
-        if (i0 >= (uint64_t)n) i0 = n - 1;
+        int64_t i0 = (int64_t)(u[0] * (double)n);
+        if (i0 >= n) i0 = n - 1;
+        if (i0 < 0) i0 = 0;
         picked[i0] = 1;
         for (int d = 0; d < ds; ++d) {
-            cur[d] = x[i0 * dim + o0 + d];
+            cur[d] = xm[i0 * maxds + d];
             cent[d] = cur[d];
         }
         cur_norm = nrm[i0];
-        stop = 0;
+        state = 0;
+        selected = 1;
     }
     __syncthreads();
-    const int steps = (int)(K < n ? K : n);
+    const int steps = (int)((int64_t)K < n ? (int64_t)K : n);
     for (int c = 1; c < steps; ++c) {
-        double acc = 0.0;
-        for (int64_t r = lo; r < hi; ++r) {
-            const float *xr = x + r * dim + o0;
-            float dot = 0.0f;
-            for (int d = 0; d < ds; ++d) dot = fma_(xr[d], cur[d], dot);
-            float dist = (nrm[r] + cur_norm) + dot * -2.0f;
-            float md = mind[r];
-            if (dist < md) md = dist;
-            mind[r] = md;
-            acc += (double)md;
+        for (int64_t b = t; b < nblk; b += 256) {
+            float dv[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int64_t r = 16 * b + j;
+                float md = 0.0f;
+                if (r < n) {
+                    const float *xr = xm + r * maxds;
+                    float dot = 0.0f;
+                    for (int d = 0; d < ds; ++d) dot = fma_(xr[d], cur[d], dot);
+                    const float dist = (nrm[r] + cur_norm) + dot * -2.0f;
+                    md = mind[r];
+                    if (dist < md) md = dist;
+                    mind[r] = md;
+                }
+                dv[j] = md;
+            }
+            double bs = 0.0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bs += (double)dv[j] + (double)dv[j + 8];
+            bsum[b] = bs;
         }
-        segsum[t] = acc;
-        if (t == 0) best = 0xFFFFFFFFu;
         __syncthreads();
         if (t == 0) {
             double s = 0.0;
-            for (int i = 0; i < 256; ++i) {
-                segoff[i] = s;
-                s += segsum[i];
+            int64_t b = 0;
+            for (; b + 8 <= nblk; b += 8) {
+                double v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = bsum[b + j];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) s = s + v[j];
             }
-            segsum[0] = s;  // total (segoff already holds the exclusive offsets)
-            if (!(s < __builtin_inf())) {
-                stop = 2;  // non-finite total: SawInfinity
-            } else if (!(s > 0.0)) {
-                stop = 1;  // empty range: no further center (recoverable)
-            }
+            for (; b < nblk; ++b) s = s + bsum[b];
+            double scale = 0.0;
+            const int us = uniform_scale(s, &scale);
+            if (us == 2) state = 3;       // SawInfinity
+            else if (us == 1) state = 2;  // empty range: no pick, InsufficientDiversity
+            else thr = u[c] * scale + 0.0;
+            found = -1;
         }
         __syncthreads();
-        if (stop) break;
-        const double threshold = draw01(seed, m, (uint64_t)c) * segsum[0];
-        {
-            double run = segoff[t];
-            for (int64_t r = lo; r < hi; ++r) {
-                float md = mind[r];
-                run += (double)md;
-                if (run >= threshold && md > 0.0f && !picked[r]) {
-                    atomicMin(&best, (uint32_t)r);
-                    break;
+        if (state) break;
+        double rolling = 0.0;  // lane 0's running sum across tiles
+        for (int64_t base = 0; base < n; base += kKppTile) {
+            const int len = (int)((n - base) < kKppTile ? (n - base) : kKppTile);
+            for (int i = t; i < len; i += 256) {
+                tmind[i] = mind[base + i];
+                tpick[i] = picked[base + i];
+            }
+            __syncthreads();
+            if (t == 0) {
+                const double th = thr;
+                for (int i = 0; i < len; ++i) {
+                    const float md = tmind[i];
+                    rolling += (double)md;
+                    if (rolling >= th && md > 0.0f && !tpick[i]) {
+                        found = base + i;
+                        break;
+                    }
                 }
             }
+            __syncthreads();
+            if (found >= 0) break;
         }
-        __syncthreads();
         if (t == 0) {
-            if (best == 0xFFFFFFFFu) {
-                stop = 1;  // insufficient diversity (recoverable)
+            if (found < 0) {
+                state = 2;
             } else {
-                picked[best] = 1;
+                picked[found] = 1;
                 for (int d = 0; d < ds; ++d) {
-                    cur[d] = x[(int64_t)best * dim + o0 + d];
+                    cur[d] = xm[found * maxds + d];
                     cent[c * maxds + d] = cur[d];
                 }
-                cur_norm = nrm[best];
+                cur_norm = nrm[found];
+                selected = c + 1;
             }
         }
         __syncthreads();
-        if (stop) break;
+        if (state) break;
     }
-    if (t == 0 && stop == 2) atomicOr(status, 1u);
+    if (t == 0) {
+        const int reason = state ? state : (selected != K ? 1 : 0);
+        status[2 * m] = (uint32_t)reason;
+        status[2 * m + 1] = (uint32_t)selected;
+    }
 }
 
-// Lloyd assignment (distances_in_place): thread per row, block = 256 rows x one chunk.
-__global__ __launch_bounds__(256) void lloyd_assign_kernel(const float *x, int64_t n, int dim,
+// Lloyd assignment (distances_in_place, lloyds.rs:28-328): one thread per (row, chunk);
+// distance = ((|c|^2 - x.c) - x.c) + |x|^2 with x.c a sequential fma over the chunk's dims,
+// the first strictly smaller distance in centre order wins. DS > 0: the chunk width is
+// DS (row held in registers); 0: any width.
+template <int DS>
+__global__ __launch_bounds__(256) void lloyd_assign_kernel(const float *xc, int64_t n,
                                                            const int32_t *offsets,
                                                            const float *norms,
                                                            const float *centers, int K, int maxds,
                                                            uint32_t *assign) {
     extern __shared__ float sm[];
     const int m = blockIdx.y;
-    const int o0 = offsets[m], o1 = offsets[m + 1], ds = o1 - o0;
+    const int ds = DS > 0 ? DS : offsets[m + 1] - offsets[m];
     float *cs = sm;               // K * ds
     float *cn = sm + K * ds;      // K
     const float *cent = centers + (int64_t)m * K * maxds;
@@ -349,64 +426,76 @@ __global__ __launch_bounds__(256) void lloyd_assign_kernel(const float *x, int64
     __syncthreads();
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (r >= n) return;
-    const float *xr = x + r * dim + o0;
-    float xv[64];
-    for (int d = 0; d < ds; ++d) xv[d] = xr[d];
+    const float *xr = xc + ((int64_t)m * n + r) * maxds;
     const float xn = norms[(int64_t)m * n + r];
     float best = __builtin_inff();
     uint32_t arg = 0xFFFFFFFFu;
-    for (int c = 0; c < K; ++c) {
-        float dot = 0.0f;
-        for (int d = 0; d < ds; ++d) dot = fma_(cs[c * ds + d], xv[d], dot);
-        float dist = ((cn[c] - dot) - dot) + xn;
-        if (dist < best) {
-            best = dist;
-            arg = (uint32_t)c;
+    if constexpr (DS > 0) {
+        float xv[DS];
+#pragma unroll
+        for (int d = 0; d < DS; ++d) xv[d] = xr[d];
+        for (int c = 0; c < K; ++c) {
+            float dot = 0.0f;
+#pragma unroll
+            for (int d = 0; d < DS; ++d) dot = fma_(cs[c * DS + d], xv[d], dot);
+            const float dist = ((cn[c] - dot) - dot) + xn;
+            if (dist < best) {
+                best = dist;
+                arg = (uint32_t)c;
+            }
+        }
+    } else {
+        for (int c = 0; c < K; ++c) {
+            float dot = 0.0f;
+            for (int d = 0; d < ds; ++d) dot = fma_(cs[c * ds + d], xr[d], dot);
+            const float dist = ((cn[c] - dot) - dot) + xn;
+            if (dist < best) {
+                best = dist;
+                arg = (uint32_t)c;
+            }
         }
     }
     assign[(int64_t)m * n + r] = arg;
 }
 
-// Lloyd update (update_centroids): one block per chunk, thread c owns center c and walks
-// every row in order, so each f64 sum is accumulated in row order exactly as the
-// reference does; empty clusters become the zero vector.
-__global__ __launch_bounds__(256) void lloyd_update_kernel(const float *x, int64_t n, int dim,
+// Lloyd update (update_centroids, lloyds.rs:345-366): one thread per (chunk, centre, dim)
+// walks every row in order through LDS tiles and adds the rows assigned to its centre in
+// f64, so each sum has the reference's row order; centre = (f32)(sum / max(count, 1))
+// (empty clusters become zero). Grid: (M, ceil(K * maxds / 256)).
+__global__ __launch_bounds__(256) void lloyd_update_kernel(const float *xc, int64_t n,
                                                            const int32_t *offsets,
                                                            const uint32_t *assign, int K,
-                                                           int maxds, int tile,
-                                                           float *centers) {
-    extern __shared__ double smd[];
+                                                           int maxds, int tile, float *centers) {
+    extern __shared__ float sm[];
     const int m = blockIdx.x;
-    const int o0 = offsets[m], o1 = offsets[m + 1], ds = o1 - o0;
-    double *sums = smd;                                   // 256 * ds
-    uint32_t *ta = (uint32_t *)(smd + 256 * ds);          // tile
-    float *tx = (float *)(ta + tile);                     // tile * ds
-    const int c = threadIdx.x;
-    for (int d = 0; d < ds; ++d) sums[c * ds + d] = 0.0;
+    const int ds = offsets[m + 1] - offsets[m];
+    uint32_t *ta = (uint32_t *)sm;       // tile
+    float *tx = sm + tile;               // tile * maxds
+    const int idx = blockIdx.y * 256 + threadIdx.x;
+    const int c = idx / maxds, d = idx - c * maxds;
+    const bool mine = c < K && d < ds;
+    double sum = 0.0;
     uint32_t cnt = 0;
     const uint32_t *am = assign + (int64_t)m * n;
+    const float *xm = xc + (int64_t)m * n * maxds;
     for (int64_t base = 0; base < n; base += tile) {
         const int len = (int)((n - base) < tile ? (n - base) : tile);
         __syncthreads();
-        for (int i = threadIdx.x; i < len; i += blockDim.x) ta[i] = am[base + i];
-        for (int e = threadIdx.x; e < len * ds; e += blockDim.x) {
-            int i = e / ds, d = e - i * ds;
-            tx[e] = x[(base + i) * dim + o0 + d];
-        }
+        for (int i = threadIdx.x; i < len; i += 256) ta[i] = am[base + i];
+        for (int e = threadIdx.x; e < len * maxds; e += 256) tx[e] = xm[base * maxds + e];
         __syncthreads();
-        if (c < K) {
+        if (mine) {
             for (int i = 0; i < len; ++i) {
                 if (ta[i] == (uint32_t)c) {
-                    cnt++;
-                    for (int d = 0; d < ds; ++d) sums[c * ds + d] += (double)tx[i * ds + d];
+                    ++cnt;
+                    sum += (double)tx[i * maxds + d];
                 }
             }
         }
     }
-    if (c < K) {
-        double den = (double)(cnt > 1 ? cnt : 1);
-        float *cent = centers + (int64_t)m * K * maxds + c * maxds;
-        for (int d = 0; d < ds; ++d) cent[d] = (float)(sums[c * ds + d] / den);
+    if (mine) {
+        const double den = (double)(cnt > 1u ? cnt : 1u);
+        centers[((int64_t)m * K + c) * maxds + d] = (float)(sum / den);
     }
 }
 
@@ -420,53 +509,71 @@ __global__ __launch_bounds__(256) void lloyd_update_kernel(const float *x, int64
     } while (0)
 
 hipError_t pq_train_device(const float *data, int64_t n, int dim, const int32_t *offsets, int M,
-                           int K, int iters, uint64_t seed, float *pivots) {
+                           int K, int iters, const double *draws, float *pivots,
+                           uint32_t *chunk_status) {
     hipError_t err = hipSuccess;
     int maxds = 0;
     for (int m = 0; m < M; ++m) maxds = offsets[m + 1] - offsets[m] > maxds ? offsets[m + 1] - offsets[m] : maxds;
-    if (maxds > 64 || K > 256) return hipErrorInvalidValue;
-    float *dx = nullptr, *dn = nullptr, *dmind = nullptr, *dcent = nullptr;
+    if (maxds > 64 || maxds < 1 || K > 256 || K < 1 || n < 1) return hipErrorInvalidValue;
+    const int64_t nblk = (n + 15) / 16;
+    float *dx = nullptr, *dxc = nullptr, *dn = nullptr, *dmind = nullptr, *dcent = nullptr;
+    double *dbsum = nullptr, *ddraw = nullptr;
     uint8_t *dpick = nullptr;
     uint32_t *dassign = nullptr, *dstatus = nullptr;
     int32_t *doff = nullptr;
-    uint32_t status = 0;
     std::vector<float> cent;
-    const int tile = 1024;
-    size_t upd_lds = sizeof(double) * 256 * maxds + (sizeof(uint32_t) + sizeof(float) * maxds) * tile;
-    size_t asg_lds = sizeof(float) * (size_t)K * (maxds + 1);
+    bool uniform = true;  // every chunk has width maxds
+    const int tile = (int)(((65536 / (4 + 4 * maxds)) / 256) * 256);
+    const size_t upd_lds = (sizeof(uint32_t) + sizeof(float) * maxds) * (size_t)tile;
+    const size_t asg_lds = sizeof(float) * (size_t)K * (maxds + 1);
+    const unsigned upd_y = (unsigned)((K * maxds + 255) / 256);
     TRAIN_TRY(hipMalloc(&dx, sizeof(float) * n * dim));
+    TRAIN_TRY(hipMalloc(&dxc, sizeof(float) * n * M * maxds));
     TRAIN_TRY(hipMalloc(&dn, sizeof(float) * n * M));
     TRAIN_TRY(hipMalloc(&dmind, sizeof(float) * n * M));
     TRAIN_TRY(hipMalloc(&dpick, (size_t)n * M));
+    TRAIN_TRY(hipMalloc(&dbsum, sizeof(double) * nblk * M));
+    TRAIN_TRY(hipMalloc(&ddraw, sizeof(double) * (size_t)M * K));
     TRAIN_TRY(hipMalloc(&dcent, sizeof(float) * (size_t)M * K * maxds));
     TRAIN_TRY(hipMalloc(&dassign, sizeof(uint32_t) * n * M));
-    TRAIN_TRY(hipMalloc(&dstatus, sizeof(uint32_t)));
+    TRAIN_TRY(hipMalloc(&dstatus, sizeof(uint32_t) * 2 * M));
     TRAIN_TRY(hipMalloc(&doff, sizeof(int32_t) * (M + 1)));
     TRAIN_TRY(hipMemcpy(dx, data, sizeof(float) * n * dim, hipMemcpyHostToDevice));
     TRAIN_TRY(hipMemcpy(doff, offsets, sizeof(int32_t) * (M + 1), hipMemcpyHostToDevice));
-    TRAIN_TRY(hipMemcpy(dstatus, &status, sizeof(uint32_t), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(chunk_norms_kernel, dim3((unsigned)((n + 255) / 256), M), dim3(256), 0, 0,
-                       dx, n, dim, doff, dn);
+    TRAIN_TRY(hipMemcpy(ddraw, draws, sizeof(double) * (size_t)M * K, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(chunk_rows_kernel, dim3((unsigned)((n + 255) / 256), M), dim3(256), 0, 0,
+                       dx, n, dim, doff, maxds, dxc, dn);
     TRAIN_TRY(hipGetLastError());
-    hipLaunchKernelGGL(kmeanspp_kernel, dim3(M), dim3(256), 0, 0, dx, n, dim, doff, dn, K, seed,
-                       dmind, dpick, dcent, maxds, dstatus);
+    hipLaunchKernelGGL(kmeanspp_kernel, dim3(M), dim3(256), 0, 0, dxc, n, doff, dn, K, ddraw,
+                       dmind, dpick, dbsum, dcent, maxds, dstatus);
     TRAIN_TRY(hipGetLastError());
+    TRAIN_TRY(hipMemcpy(chunk_status, dstatus, sizeof(uint32_t) * 2 * M, hipMemcpyDeviceToHost));
+    for (int m = 0; m < M; ++m) {
+        // SawInfinity, or nothing picked, is not recoverable (train.rs:180-196)
+        if (chunk_status[2 * m] == 3 || chunk_status[2 * m + 1] == 0) {
+            err = hipErrorInvalidValue;
+            goto done;
+        }
+    }
     TRAIN_TRY(hipFuncSetAttribute((const void *)lloyd_update_kernel,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)upd_lds));
+    for (int m = 0; m < M; ++m) uniform = uniform && offsets[m + 1] - offsets[m] == maxds;
     for (int it = 0; it < iters; ++it) {
-        hipLaunchKernelGGL(lloyd_assign_kernel, dim3((unsigned)((n + 255) / 256), M), dim3(256),
-                           asg_lds, 0, dx, n, dim, doff, dn, dcent, K, maxds, dassign);
+        const dim3 ag((unsigned)((n + 255) / 256), M);
+        switch (uniform && maxds <= 8 ? maxds : 0) {
+        case 1: hipLaunchKernelGGL(lloyd_assign_kernel<1>, ag, dim3(256), asg_lds, 0, dxc, n, doff, dn, dcent, K, maxds, dassign); break;
+        case 2: hipLaunchKernelGGL(lloyd_assign_kernel<2>, ag, dim3(256), asg_lds, 0, dxc, n, doff, dn, dcent, K, maxds, dassign); break;
+        case 3: hipLaunchKernelGGL(lloyd_assign_kernel<3>, ag, dim3(256), asg_lds, 0, dxc, n, doff, dn, dcent, K, maxds, dassign); break;
+        case 4: hipLaunchKernelGGL(lloyd_assign_kernel<4>, ag, dim3(256), asg_lds, 0, dxc, n, doff, dn, dcent, K, maxds, dassign); break;
+        case 8: hipLaunchKernelGGL(lloyd_assign_kernel<8>, ag, dim3(256), asg_lds, 0, dxc, n, doff, dn, dcent, K, maxds, dassign); break;
+        default: hipLaunchKernelGGL(lloyd_assign_kernel<0>, ag, dim3(256), asg_lds, 0, dxc, n, doff, dn, dcent, K, maxds, dassign); break;
+        }
         TRAIN_TRY(hipGetLastError());
-        hipLaunchKernelGGL(lloyd_update_kernel, dim3(M), dim3(256), upd_lds, 0, dx, n, dim, doff,
-                           dassign, K, maxds, tile, dcent);
+        hipLaunchKernelGGL(lloyd_update_kernel, dim3(M, upd_y), dim3(256), upd_lds, 0, dxc, n,
+                           doff, dassign, K, maxds, tile, dcent);
         TRAIN_TRY(hipGetLastError());
     }
     TRAIN_TRY(hipDeviceSynchronize());
-    TRAIN_TRY(hipMemcpy(&status, dstatus, sizeof(uint32_t), hipMemcpyDeviceToHost));
-    if (status) {
-        err = hipErrorInvalidValue;
-        goto done;
-    }
     cent.resize((size_t)M * K * maxds);
     TRAIN_TRY(hipMemcpy(cent.data(), dcent, sizeof(float) * cent.size(), hipMemcpyDeviceToHost));
     for (int m = 0; m < M; ++m)
@@ -475,9 +582,12 @@ hipError_t pq_train_device(const float *data, int64_t n, int dim, const int32_t 
                 pivots[(size_t)c * dim + offsets[m] + d] = cent[((size_t)m * K + c) * maxds + d];
 done:
     if (dx) (void)hipFree(dx);
+    if (dxc) (void)hipFree(dxc);
     if (dn) (void)hipFree(dn);
     if (dmind) (void)hipFree(dmind);
     if (dpick) (void)hipFree(dpick);
+    if (dbsum) (void)hipFree(dbsum);
+    if (ddraw) (void)hipFree(ddraw);
     if (dcent) (void)hipFree(dcent);
     if (dassign) (void)hipFree(dassign);
     if (dstatus) (void)hipFree(dstatus);

@@ -527,7 +527,7 @@ __device__ inline void fp_distances(const SearchParams &p, const float *q, const
     if constexpr (METRIC != METRIC_COSINE && DIM > 0 && DIM % 32 == 0 && DIM <= 128) {
         constexpr int NI = DIM / 32;
         constexpr int CPB = 8;
-        constexpr bool IP = METRIC == METRIC_IP;
+        constexpr bool IP = METRIC == METRIC_IP || METRIC == METRIC_COSNORM;
         float qv[NI];
 #pragma unroll
         for (int i = 0; i < NI; ++i) qv[i] = q[hp + 32 * i];
@@ -554,7 +554,8 @@ __device__ inline void fp_distances(const SearchParams &p, const float *q, const
                     }
                 }
                 float d = finish32<IP>(acc, q, q, DIM, hp);  // DIM % 8 == 0: no remainder
-                if (IP) d = -d;
+                if (METRIC == METRIC_IP) d = -d;
+                else if (METRIC == METRIC_COSNORM) d = 1.0f - d;
                 const int c = c0 + 2 * j + half;
                 if (c < n && hp == 0) out[c] = d;
             }
@@ -902,6 +903,19 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
     const int k = job.k;
     const int cap = L + p.nstart;
     uint32_t *err = job.err;
+    // per-query k / L come from device memory: a request outside Knn::new's rules or the
+    // launch's LDS queue (sized for max_L) and output stride is refused, not searched
+    if (L < 1 || k < 1 || k > L || cap > p.qcap || (int64_t)k > p.k_stride) {
+        if (lane == 0) {
+            *err = ERR_BAD_REQUEST;
+            *job.out_count = 0u;
+            if (job.out_cmps) *job.out_cmps = 0u;
+            if (job.out_hops) *job.out_hops = 0u;
+            if (job.rec_len) *job.rec_len = 0u;
+            if (job.out_degsum) *job.out_degsum = 0u;
+        }
+        return;
+    }
 
     // ---- init --------------------------------------------------------------------
     const float *qg = job.query;
@@ -1075,6 +1089,7 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
         __syncthreads();
         if (p.metric == METRIC_L2) fp_distances<METRIC_L2, DIM>(p, s.q, qi, qd, n, s.starts[0]);
         else if (p.metric == METRIC_IP) fp_distances<METRIC_IP, DIM>(p, s.q, qi, qd, n, s.starts[0]);
+        else if (p.metric == METRIC_COSNORM) fp_distances<METRIC_COSNORM, DIM>(p, s.q, qi, qd, n, s.starts[0]);
         else fp_distances<METRIC_COSINE, 0>(p, s.q, qi, qd, n, s.starts[0]);
         __syncthreads();
         // stable selection of the k smallest (distance, position) keys: k rounds of a
@@ -1224,6 +1239,7 @@ static hipError_t dispatch(const SearchParams &p, hipStream_t stream, int *resid
     case METRIC_IP:
         if (p.dim == 128) return launch_t<METRIC_IP, 0, 128, 0>(p, stream, resident);
         return launch_t<METRIC_IP, 0, 0, 0>(p, stream, resident);
+    case METRIC_COSNORM: return launch_t<METRIC_COSNORM, 0, 0, 0>(p, stream, resident);
     default: return launch_t<METRIC_COSINE, 0, 0, 0>(p, stream, resident);
     }
 }

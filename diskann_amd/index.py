@@ -341,7 +341,7 @@ class SearchGraph:
 
     def run(self):
         check(self.lib.dkm_search_graph_launch(self._h), self.lib)
-        check(self.lib.dkm_index_synchronize(self.index._h), self.lib)
+        check(self.lib.dkm_search_graph_status(self._h, None), self.lib)
         n, mk = self.nq, self.max_k
         return SearchResult(self._ids.get(np.uint32, (n, mk)), self._d.get(np.float32, (n, mk)),
                             self._cnt.get(np.uint32, n), self._cm.get(np.uint32, n),

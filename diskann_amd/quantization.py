@@ -37,6 +37,40 @@ def train_pq(data, num_chunks, num_centers=256, lloyd_iters=5, seed=0):
     return pivots, offsets
 
 
+def train_pq_draws(data, num_chunks, draws, num_centers=256, lloyd_iters=5):
+    """PQ training with the caller's k-means++ draws (num_chunks x num_centers uniforms in
+    [0, 1)); returns (pivots[num_centers, dim], offsets[num_chunks + 1])."""
+    lib = _native.load()
+    x = np.ascontiguousarray(data, dtype=np.float32)
+    n, dim = x.shape
+    u = np.ascontiguousarray(draws, dtype=np.float64).reshape(num_chunks, num_centers)
+    pivots = np.zeros((num_centers, dim), dtype=np.float32)
+    offsets = np.zeros(num_chunks + 1, dtype=np.int64)
+    check(lib.dkm_pq_train_draws(_p(x), n, dim, num_chunks, num_centers, lloyd_iters, _p(u),
+                                 _p(pivots), _p(offsets)), lib)
+    return pivots, offsets
+
+
+def train_draws(seed, num_chunks, num_centers=256):
+    """The draws dkm_pq_train takes from its SplitMix64 stream for `seed` (chunk m, draw c:
+    splitmix64(splitmix64(seed ^ splitmix64(0x5851f42d4c957f2d * (m + 1))) + c) >> 11,
+    scaled by 2^-53)."""
+    mask = (1 << 64) - 1
+
+    def sm(x):
+        x = (x + 0x9e3779b97f4a7c15) & mask
+        x = ((x ^ (x >> 30)) * 0xbf58476d1ce4e5b9) & mask
+        x = ((x ^ (x >> 27)) * 0x94d049bb133111eb) & mask
+        return x ^ (x >> 31)
+
+    out = np.zeros((num_chunks, num_centers), np.float64)
+    for m in range(num_chunks):
+        s = sm((seed & mask) ^ sm((0x5851f42d4c957f2d * (m + 1)) & mask))
+        for c in range(num_centers):
+            out[m, c] = (sm((s + c) & mask) >> 11) * (1.0 / 9007199254740992.0)
+    return out
+
+
 def pq_encode(pivots, offsets, x):
     lib = _native.load()
     pivots = np.ascontiguousarray(pivots, dtype=np.float32)

@@ -108,10 +108,23 @@ dkm_status dkm_pq_distance_batch(int metric, const float *pivots, int64_t K, int
                                  int64_t n, const uint32_t *ids, int64_t m, float *out);
 dkm_status dkm_pq_encode(const float *pivots, int64_t K, int64_t dim, const int64_t *offsets,
                          int64_t M, const float *x, int64_t n, uint8_t *codes);
-/* PQ training: per-chunk k-means++ seeding (own RNG, seeded by `seed`) then `iters`
- * Lloyd iterations. pivots: K x dim (reference layout), offsets: M+1. */
+/* PQ training, LightPQTrainingParameters::train (diskann-quantization/src/product/
+ * train.rs:109-231) as generate_pq_pivots_from_membuf calls it
+ * (diskann-providers/src/model/pq/pq_construction.rs:163-253): chunk offsets by
+ * ChunkOffsets::partition, then per chunk k-means++ seeding (plusplus.rs:381-498;
+ * DatasetTooSmall / InsufficientDiversity are suppressed and the unpicked centres stay
+ * zero, SawInfinity fails with DKM_ERR_PQ) and `iters` Lloyd iterations (lloyds.rs:372-426).
+ * pivots: K x dim (reference layout), offsets: M+1. Host buffers.
+ * dkm_pq_train_draws takes the k-means++ random draws from the caller: draws[m * K + c] in
+ * [0, 1) for chunk m; the first centre is row floor(draws[m * K] * n) and step c's threshold
+ * is draws[m * K + c] times the scale of rand's UniformFloat over [0, total). dkm_pq_train
+ * fills the draws from the library's SplitMix64 stream seeded by `seed` (the reference
+ * draws from rand / ChaCha12, which this library does not reproduce). */
 dkm_status dkm_pq_train(const float *data, int64_t n, int64_t dim, int64_t M, int64_t K,
                         int64_t iters, uint64_t seed, float *pivots, int64_t *offsets);
+dkm_status dkm_pq_train_draws(const float *data, int64_t n, int64_t dim, int64_t M, int64_t K,
+                              int64_t iters, const double *draws, float *pivots,
+                              int64_t *offsets);
 
 /* -------------------------------------------------------------------------------- */
 /* Index (DefaultProvider + DiskANNIndex)                                           */
@@ -153,8 +166,11 @@ dkm_status dkm_search_batch(dkm_index *ix, int strategy, const float *queries, i
                             uint32_t *out_counts, uint32_t *out_cmps, uint32_t *out_hops);
 
 /* Same, with every buffer in device memory (dkm_device_alloc); asynchronous on the
- * handle's stream. Validation of k/L is the caller's duty (see dkm_search_batch).
- * Launches of 2048 queries or more run them longest L first (a device sort of the
+ * handle's stream. max_k / max_L bound the launch (max_k <= k_stride, max_L <= 4096); each
+ * query's own k / L are read on the device and a query breaking Knn::new's rules (k, L >= 1,
+ * k <= L) or the bounds (L <= max_L, k <= k_stride) is not searched: its count is 0 and
+ * dkm_search_status reports DKM_ERR_INVALID_ARG for it; no other query's outputs are
+ * touched. Launches of 2048 queries or more run them longest L first (a device sort of the
  * launch's L values); no query's result depends on that order. */
 dkm_status dkm_search_batch_device(dkm_index *ix, int strategy, const float *d_queries,
                                    int64_t nq, const uint32_t *d_k, const uint32_t *d_L,
@@ -163,6 +179,12 @@ dkm_status dkm_search_batch_device(dkm_index *ix, int strategy, const float *d_q
                                    uint32_t *d_out_counts, uint32_t *d_out_cmps,
                                    uint32_t *d_out_hops);
 dkm_status dkm_index_synchronize(dkm_index *ix);
+/* Waits for the handle's stream and checks the per-query status of the most recent
+ * dkm_search_batch_device(_stats) launch: DKM_OK, or the first failing query's error as
+ * dkm_search_batch reports it (DKM_ERR_INVALID_ARG for a refused k / L, DKM_ERR_CAPACITY for
+ * a visited-set overflow, DKM_ERR_INDEX for the iteration bound), with its position in
+ * *first_failed (-1 when none; may be NULL). Outputs of a failed query are not valid. */
+dkm_status dkm_search_status(dkm_index *ix, int64_t *first_failed);
 
 /* HIP graph of one batched search launch: dkm_search_graph_create validates and captures
  * the launch dkm_search_batch_device_stats would make with these arguments (the kernels
@@ -182,6 +204,8 @@ dkm_status dkm_search_graph_create(dkm_index *ix, int strategy, const float *d_q
                                    uint32_t *d_out_hops, uint32_t *d_out_degsum,
                                    dkm_search_graph **out);
 dkm_status dkm_search_graph_launch(dkm_search_graph *g);
+/* dkm_search_status for the graph's most recent replay. */
+dkm_status dkm_search_graph_status(dkm_search_graph *g, int64_t *first_failed);
 void dkm_search_graph_destroy(dkm_search_graph *g);
 
 /* Kernel timing with HIP events on the handle's stream: enable, then read the count and
@@ -480,6 +504,10 @@ void dkm_prescore_destroy(dkm_prescorer *ps);
  * query (a multiple of 64 in 256..32768, 0 = automatic) and the PQ distance mode (0 = lookup table in LDS,
  * 1 = direct recomputation from the pivots, the default). */
 dkm_status dkm_set_tuning(dkm_index *ix, int64_t visited_lds_slots, int adc_mode);
+/* Capacity of a query's HBM visited-set region for the batched and graph searches created
+ * after the call (a power of two in [256, 2^24]; 0 = automatic: at least 16384 and 16 x the
+ * launch's max L). A query that visits more ids than fit reports DKM_ERR_CAPACITY. */
+dkm_status dkm_set_visited_capacity(dkm_index *ix, int64_t slots);
 
 /* Autotuner: times the search of `nq` device-resident queries (k, L, beam_width fixed)
  * over the PQ distance modes and a few visited-set sizes derived from the measured

@@ -241,14 +241,17 @@ float dko_pq_qq(int metric, const float *pivots, int64_t dim, const int64_t *off
 }
 
 /* PostOps: diskann-vector/src/distance/implementations.rs:215-405. For f32 data the
- * DistanceProvider maps CosineNormalized to the full Cosine functor
- * (diskann-vector/src/distance/distance_provider.rs:288-291). */
+ * DistanceProvider builds the CosineNormalized functor (distance_provider.rs:266-270):
+ * the inner-product loop (architecture_hook!(CosineNormalized, simd::IP),
+ * implementations.rs:400) with post-op cosine_transformation(x) = 1 - x (:330-332,
+ * :368-380). Only the integer element types redirect CosineNormalized to Cosine
+ * (distance_provider.rs:274-299). */
 float dko_distance(int metric, const float *q, const float *x, int64_t dim) {
     switch (metric) {
     case DKO_L2: return simd_4x1(0, q, x, dim);
     case DKO_INNER_PRODUCT: return -simd_4x1(1, q, x, dim);
-    case DKO_COSINE:
-    case DKO_COSINE_NORMALIZED: return cosine_v3(q, x, dim);
+    case DKO_COSINE_NORMALIZED: return 1.0f - simd_4x1(1, q, x, dim);
+    case DKO_COSINE: return cosine_v3(q, x, dim);
     default: return NAN;
     }
 }
@@ -1977,6 +1980,170 @@ void dko_lloyd_update(const float *x, int64_t n, int64_t dim, const uint32_t *ne
     free(cnt);
 }
 
+/* rand 0.9 UniformFloat::<f64>::new(0.0, s) (third-party, rand 0.9.x in Cargo.lock): an
+ * empty range (!(0 < s), NaN included) or a non-finite scale fail; otherwise scale = s,
+ * lowered one ulp at a time while scale * (1 - 2^-52) + 0 >= s; a draw maps to
+ * value0_1 * scale + 0. Returns 0 ok, 1 empty range, 2 non-finite. */
+static int uniform_f64_scale(double s, double *scale) {
+    if (!(0.0 < s)) return 1;
+    double sc = s - 0.0;
+    if (!isfinite(sc)) return 2;
+    const double max_rand = 1.0 - 2.220446049250313e-16;
+    while (sc * max_rand + 0.0 >= s) sc = nextafter(sc, 0.0);
+    *scale = sc;
+    return 0;
+}
+
+/* update_distances (plusplus.rs:238-318, MicroKernel for BlockTransposed<f32, 16> :84-231):
+ * dot = sequential fma(x_d, c_d, dot) over the chunk, distance = (norm_x + norm_c) +
+ * dot * -2, min_d = distance if distance < min_d; the f64 total runs over blocks of 16
+ * rows, each block adding sum_{j<8} ((f64) min_d[16b + j] + (f64) min_d[16b + 8 + j]) in
+ * order j (rows past n count 0). */
+static double kpp_update(const float *x, int64_t n, int64_t stride, int64_t ds,
+                         const float *norms, const float *c, float cnorm, float *mind) {
+    for (int64_t r = 0; r < n; ++r) {
+        const float *xr = x + r * stride;
+        float dot = 0.0f;
+        for (int64_t d = 0; d < ds; ++d) dot = fmaf(xr[d], c[d], dot);
+        const float dist = (norms[r] + cnorm) + dot * -2.0f;
+        if (dist < mind[r]) mind[r] = dist;
+    }
+    double s = 0.0;
+    for (int64_t b = 0; b < n; b += 16) {
+        double bs = 0.0;
+        for (int64_t j = 0; j < 8; ++j) {
+            const double d0 = b + j < n ? (double)mind[b + j] : 0.0;
+            const double d1 = b + 8 + j < n ? (double)mind[b + 8 + j] : 0.0;
+            bs += d0 + d1;
+        }
+        s = s + bs;
+    }
+    return s;
+}
+
+/* kmeans_plusplus_into_inner, plusplus.rs:381-498. */
+int dko_kmeanspp(const float *x, int64_t n, int64_t stride, int64_t ds, int64_t K,
+                 const double *u, float *centers, int64_t *selected) {
+    memset(centers, 0, sizeof(float) * (size_t)(K * ds));
+    *selected = 0;
+    if (n <= 0) return K == 0 ? 0 : 1;
+    if (K == 0) return 0;
+    float *norms = (float *)malloc(sizeof(float) * (size_t)n);
+    float *mind = (float *)malloc(sizeof(float) * (size_t)n);
+    uint8_t *picked = (uint8_t *)calloc((size_t)n, 1);
+    for (int64_t r = 0; r < n; ++r) {
+        norms[r] = dko_square_norm(x + r * stride, ds);
+        mind[r] = INFINITY;
+    }
+    int64_t i0 = (int64_t)(u[0] * (double)n);
+    if (i0 >= n) i0 = n - 1;
+    memcpy(centers, x + i0 * stride, sizeof(float) * (size_t)ds);
+    picked[i0] = 1;
+    float prev_norm = norms[i0];
+    int64_t sel = 1;
+    int reason = 0;
+    const int64_t steps = K < n ? K : n;
+    for (int64_t cur = 1; cur < steps; ++cur) {
+        const double s = kpp_update(x, n, stride, ds, norms, centers + (cur - 1) * ds, prev_norm,
+                                    mind);
+        double scale = 0.0;
+        const int us = uniform_f64_scale(s, &scale);
+        if (us == 2) {
+            reason = 3;  /* SawInfinity */
+            break;
+        }
+        if (us == 0) {
+            const double threshold = u[cur] * scale + 0.0;
+            double rolling = 0.0;
+            for (int64_t i = 0; i < n; ++i) {
+                rolling += (double)mind[i];
+                if (rolling >= threshold && mind[i] > 0.0f && !picked[i]) {
+                    memcpy(centers + cur * ds, x + i * stride, sizeof(float) * (size_t)ds);
+                    picked[i] = 1;
+                    prev_norm = norms[i];
+                    sel = cur + 1;
+                    break;
+                }
+            }
+        }
+        if (sel != cur + 1) {
+            reason = 2;  /* InsufficientDiversity */
+            break;
+        }
+    }
+    if (reason == 0 && sel != K) reason = 1;  /* DatasetTooSmall */
+    *selected = sel;
+    free(norms);
+    free(mind);
+    free(picked);
+    return reason;
+}
+
+/* Train driver, train.rs:109-231 (per chunk: strided view, norms, k-means++ with
+ * recoverable errors suppressed, lloyds_inner) and lloyds.rs:372-426 (centre norms
+ * recomputed between iterations, not after the last). Chunks are independent (the
+ * reference maps them in parallel with rayon); here up to 16 threads take chunks in turn. */
+typedef struct {
+    const float *x;
+    int64_t n, dim, M, K, iters;
+    const int64_t *offsets;
+    const double *u;
+    float *pivots;
+    int64_t next;
+    int failed;  /* 1 + the first failing chunk, 0 if none */
+    pthread_mutex_t mu;
+} pqt_job;
+
+static int pq_train_chunk(pqt_job *j, int64_t m) {
+    const int64_t n = j->n, K = j->K, o0 = j->offsets[m], ds = j->offsets[m + 1] - j->offsets[m];
+    float *xc = (float *)malloc(sizeof(float) * (size_t)(n * ds + 1));
+    float *c = (float *)malloc(sizeof(float) * (size_t)(K * ds + 1));
+    uint32_t *nearest = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n + 1));
+    for (int64_t r = 0; r < n; ++r)
+        memcpy(xc + r * ds, j->x + r * j->dim + o0, sizeof(float) * (size_t)ds);
+    int64_t sel = 0;
+    const int reason = dko_kmeanspp(xc, n, ds, ds, K, j->u + m * K, c, &sel);
+    const int bad = reason == 3 || (reason != 0 && sel == 0);
+    if (!bad) {
+        for (int64_t it = 0; it < j->iters; ++it) {
+            (void)dko_lloyd_assign(xc, n, ds, c, K, nearest);
+            dko_lloyd_update(xc, n, ds, nearest, K, c);
+        }
+        for (int64_t k = 0; k < K; ++k)
+            memcpy(j->pivots + k * j->dim + o0, c + k * ds, sizeof(float) * (size_t)ds);
+    }
+    free(xc);
+    free(c);
+    free(nearest);
+    return bad;
+}
+
+static void *pq_train_worker(void *p) {
+    pqt_job *j = (pqt_job *)p;
+    for (;;) {
+        pthread_mutex_lock(&j->mu);
+        const int64_t m = j->next++;
+        pthread_mutex_unlock(&j->mu);
+        if (m >= j->M) break;
+        if (pq_train_chunk(j, m)) {
+            pthread_mutex_lock(&j->mu);
+            if (j->failed == 0 || 1 + m < j->failed) j->failed = (int)(1 + m);
+            pthread_mutex_unlock(&j->mu);
+        }
+    }
+    return NULL;
+}
+
+int dko_pq_train(const float *x, int64_t n, int64_t dim, const int64_t *offsets, int64_t M,
+                 int64_t K, int64_t iters, const double *u, float *pivots) {
+    pqt_job j = {x, n, dim, M, K, iters, offsets, u, pivots, 0, 0, PTHREAD_MUTEX_INITIALIZER};
+    int nt = M < 16 ? (int)M : 16;
+    pthread_t th[16];
+    for (int t = 0; t < nt; ++t) pthread_create(&th[t], NULL, pq_train_worker, &j);
+    for (int t = 0; t < nt; ++t) pthread_join(th[t], NULL);
+    return j.failed;
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* Brute force FP64 top-k                                                                */
 /* ------------------------------------------------------------------------------------ */
@@ -2007,6 +2174,7 @@ static double dist_f64(int metric, const float *a, const float *b, int64_t dim) 
         yy += (double)b[j] * (double)b[j];
     }
     if (metric == DKO_INNER_PRODUCT) return -xy;
+    if (metric == DKO_COSINE_NORMALIZED) return 1.0 - xy;
     if (xx == 0 || yy == 0) return 1.0;
     return 1.0 - xy / (sqrt(xx) * sqrt(yy));
 }

@@ -24,8 +24,16 @@
  *                             grid_insert golden (post-build searches) where the
  *                             reference's unstable sort ties do not matter.
  *   dko_lloyd_*               pinned: closed form on separated clusters, monotone residual.
- *   dko_kmeanspp              parity unpinned (reference uses rand 0.9.4 / ChaCha12,
- *                             not reproducible without the Rust crates).
+ *   dko_distance (cos.-norm.) pinned: bitwise 1 - (pinned inner-product sum), FP64 bound,
+ *                             closed forms.
+ *   dko_kmeanspp              given the uniform draws: pinned by n = K distinct rows (every
+ *                             row picked), a numpy f64 cumulative-sum search for the pick,
+ *                             the block-pair order of the f64 total on crafted inputs, and
+ *                             the all-zero / infinite / too-small error cases. The draws
+ *                             themselves (rand 0.9.4 / ChaCha12 in the reference) are parity
+ *                             unpinned: the library uses its own SplitMix64 stream.
+ *   dko_pq_train              composition of the pinned pieces (square_norm, k-means++,
+ *                             Lloyd); pinned against a numpy restatement of the driver.
  */
 #ifndef DKO_ORACLE_H
 #define DKO_ORACLE_H
@@ -248,6 +256,22 @@ float dko_lloyd_assign(const float *x, int64_t n, int64_t dim, const float *c, i
                        uint32_t *nearest);
 void dko_lloyd_update(const float *x, int64_t n, int64_t dim, const uint32_t *nearest,
                       int64_t K, float *c);
+
+/* k-means++ seeding (diskann-quantization/src/algorithms/kmeans/plusplus.rs:238-498) over
+ * n rows of ds floats (row stride `stride`) into centers (K x ds, zero-filled first), with
+ * the caller's uniform draws u[0..K) in [0, 1) standing in for the reference's RNG: the
+ * first centre is row floor(u[0] n), step c draws its threshold as u[c] times the scale of
+ * rand's UniformFloat over [0, s). Returns 0 (ok) or the failure reason: 1
+ * DatasetTooSmall, 2 InsufficientDiversity, 3 SawInfinity; *selected = centres picked. */
+int dko_kmeanspp(const float *x, int64_t n, int64_t stride, int64_t ds, int64_t K,
+                 const double *u, float *centers, int64_t *selected);
+/* LightPQTrainingParameters::train (diskann-quantization/src/product/train.rs:109-231) on
+ * n x dim rows with chunk offsets (M + 1): per chunk k-means++ (draws u + m*K), recoverable
+ * failures suppressed, then `iters` Lloyd iterations (lloyds_inner, lloyds.rs:372-426).
+ * pivots: K x dim. Returns 0, or 1 + the chunk whose initialisation failed
+ * unrecoverably. */
+int dko_pq_train(const float *x, int64_t n, int64_t dim, const int64_t *offsets, int64_t M,
+                 int64_t K, int64_t iters, const double *u, float *pivots);
 
 /* Exact FP64 brute-force top-k (recall oracle). */
 void dko_bruteforce_f64(int metric, const float *base, int64_t n, int64_t stride,

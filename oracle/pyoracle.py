@@ -145,6 +145,10 @@ def _setup(L):
     L.dko_lloyd_assign.argtypes = [P, I64, I64, P, I64, P]
     L.dko_lloyd_assign.restype = F
     L.dko_lloyd_update.argtypes = [P, I64, I64, P, I64, P]
+    L.dko_kmeanspp.argtypes = [P, I64, I64, I64, I64, P, P, P]
+    L.dko_kmeanspp.restype = C.c_int
+    L.dko_pq_train.argtypes = [P, I64, I64, P, I64, I64, I64, P, P]
+    L.dko_pq_train.restype = C.c_int
     L.dko_bruteforce_f64.argtypes = [C.c_int, P, I64, I64, I64, P, I64, I64, C.c_int, P, P]
 
 
@@ -597,6 +601,36 @@ def lloyd_update(x, nearest, K):
     c = np.zeros((K, x.shape[1]), dtype=np.float32)
     lib().dko_lloyd_update(_p(x), x.shape[0], x.shape[1], _p(nearest), K, _p(c))
     return c
+
+
+KPP_OK, KPP_DATASET_TOO_SMALL, KPP_INSUFFICIENT_DIVERSITY, KPP_SAW_INFINITY = 0, 1, 2, 3
+
+
+def kmeanspp(x, K, u):
+    """k-means++ seeding with the uniform draws u (K doubles in [0, 1)); returns
+    (centers K x ds, selected, reason)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    assert u.shape[0] >= max(K, 1)
+    c = np.zeros((K, x.shape[1]), dtype=np.float32)
+    sel = C.c_int64(0)
+    r = lib().dko_kmeanspp(_p(x), x.shape[0], x.shape[1], x.shape[1], K, _p(u), _p(c),
+                           C.byref(sel))
+    return c, sel.value, r
+
+
+def pq_train(x, offsets, K, iters, u):
+    """PQ pivots (K x dim) from the train driver with draws u (M x K doubles); raises on
+    an unrecoverable k-means++ failure."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    M = off.shape[0] - 1
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(M, K)
+    piv = np.zeros((K, x.shape[1]), dtype=np.float32)
+    r = lib().dko_pq_train(_p(x), x.shape[0], x.shape[1], _p(off), M, K, iters, _p(u), _p(piv))
+    if r:
+        raise RuntimeError("k-means++ failed on chunk %d" % (r - 1))
+    return piv
 
 
 def bruteforce_f64(metric, base, queries, k, nthreads=8):

@@ -48,3 +48,22 @@ def test_knn_validation_matches_reference():
 def test_metric_numbering_matches_reference_repr_c():
     assert [int(m) for m in (dk.Metric.COSINE, dk.Metric.INNER_PRODUCT, dk.Metric.L2,
                              dk.Metric.COSINE_NORMALIZED)] == [0, 1, 2, 3]
+
+
+def test_library_chunk_offsets_match_partition_kat():
+    # ChunkOffsets::partition KAT (diskann-quantization/src/views.rs:528-548): host code of
+    # the library, no device needed
+    import json
+    from diskann_amd import quantization as qz
+    kats = json.load(open(os.path.join(ROOT, "tests", "golden", "kats.json")))
+    for dim, m, exp in kats["partition"]:
+        assert list(qz.chunk_offsets(dim, m)) == exp
+
+
+def test_pq_train_draw_stream_is_documented():
+    # the Python restatement of the library's SplitMix64 draw stream; pinned on the GPU
+    # against dkm_pq_train's pivots (tests/test_gpu_pq_train.py)
+    from diskann_amd import quantization as qz
+    u = qz.train_draws(1, 3, 5)
+    assert u.shape == (3, 5) and ((u >= 0) & (u < 1)).all()
+    assert len(set(u.ravel().tolist())) == 15

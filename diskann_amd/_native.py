@@ -23,6 +23,7 @@ EXPORTS = [
     "dkm_bruteforce", "dkm_search_batch_device_stats", "dkm_autotune",
     "dkm_search_graph_create", "dkm_search_graph_launch", "dkm_search_graph_destroy",
     "dkm_search_status", "dkm_search_graph_status", "dkm_set_visited_capacity",
+    "dkm_search_batch_device_noalloc",
     "dkm_server_start", "dkm_server_slots", "dkm_server_submit",
     "dkm_server_submit_batch", "dkm_server_poll", "dkm_server_pending", "dkm_server_stop",
     "dkm_index_shape", "dkm_bin_write", "dkm_bin_read_shape", "dkm_bin_read",
@@ -153,6 +154,8 @@ def _declare(L):
         "dkm_search_graph_launch": [P],
         "dkm_search_graph_status": [P, P],
         "dkm_search_status": [P, P],
+        "dkm_search_batch_device_noalloc": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P,
+                                            P, P],
         "dkm_set_visited_capacity": [P, I64],
     }
     for name, args in sig.items():

@@ -1645,6 +1645,36 @@ dkm_status dkm_search_status(dkm_index *ix, int64_t *first_failed) {
 
 }  // extern "C"
 
+// ------------------------------------------------------------------------------------
+// Allocation-free batched search: the launch's scratch (status words when the caller
+// gives none, visited-set regions, the longest-L-first order, spherical query records)
+// lives in this file-scope device array. Each GPU holds its own copy, at its own address:
+// host code uses the address hipGetSymbolAddress returns for the current device, never the
+// array's name (a host-side stand-in).
+// ------------------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static uint8_t *g_scratch_ptr[64] = {};
+
+// The scratch address on `device` (which must be current); looked up once per device.
+static dkm_status scratch_ptr(int device, uint8_t **out) {
+    if (device < 0 || device >= 64) return fail(DKM_ERR_INVALID_ARG, "device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(DKM_ERR_DEVICE, "scratch symbol address: %s", hipGetErrorString(e));
+        g_scratch_ptr[device] = (uint8_t *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return DKM_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
 struct dkm_search_graph {
     dkm_index *ix = nullptr;
     int strategy = 0;
@@ -1784,6 +1814,49 @@ dkm_status dkm_search_graph_status(dkm_search_graph *g, int64_t *first_failed) {
     HIP_TRY(hipMemcpy(status.data(), g->status.p, sizeof(uint32_t) * status.size(),
                       hipMemcpyDeviceToHost));
     return status_words(status.data(), g->nq, first_failed);
+}
+
+dkm_status dkm_search_batch_device_noalloc(dkm_index *ix, int strategy, const float *d_queries,
+                                           int64_t nq, const uint32_t *d_k, const uint32_t *d_L,
+                                           uint32_t max_k, uint32_t max_L, uint32_t beam_width,
+                                           int64_t k_stride, uint32_t *d_out_ids,
+                                           float *d_out_dists, uint32_t *d_out_counts,
+                                           uint32_t *d_out_cmps, uint32_t *d_out_hops,
+                                           uint32_t *d_out_status) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null index");
+    if (nq < 0) return fail(DKM_ERR_INVALID_ARG, "negative query count");
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    uint8_t *base = nullptr;
+    dkm_status st = scratch_ptr(ix->device, &base);
+    if (st) return st;
+    // scratch layout: [status words][visited regions][order + bins][spherical records]
+    const uint64_t slots = ix->tune_spill ? ix->tune_spill : region_slots(max_L == 0 ? 1 : max_L);
+    const size_t status_b = d_out_status ? 0 : align256(sizeof(uint32_t) * (size_t)nq);
+    const size_t spill_b = align256(sizeof(uint32_t) * slots * (size_t)nq);
+    const size_t order_b = nq >= kOrderMin ? align256(sizeof(uint32_t) * order_words(nq)) : 0;
+    const size_t sph_b =
+        ix->sph_active ? align256((size_t)nq * dkm::sph_query_record_bytes(ix->sphq)) : 0;
+    const size_t need = status_b + spill_b + order_b + sph_b;
+    if (need > kScratchBytes)
+        return fail(DKM_ERR_CAPACITY,
+                    "%lld queries at max L %u need %zu bytes of scratch; the allocation-free "
+                    "search holds %zu (use fewer queries per call)",
+                    (long long)nq, max_L, need, kScratchBytes);
+    OwnBufs own;
+    uint32_t *status = d_out_status ? d_out_status : (uint32_t *)base;
+    own.spill = (uint32_t *)(base + status_b);
+    own.spill_slots = slots;
+    own.order = order_b ? (uint32_t *)(base + status_b + spill_b) : nullptr;
+    own.sph_qrec = sph_b ? base + status_b + spill_b + order_b : nullptr;
+    dkm::SearchParams p;
+    st = search_params(ix, strategy, d_queries, nq, d_k, d_L, max_k, max_L, beam_width, k_stride,
+                       d_out_ids, d_out_dists, d_out_counts, d_out_cmps, d_out_hops, status,
+                       nullptr, &own, p);
+    if (st || nq == 0) return st;
+    hipError_t e = dkm::launch_search(p, ix->stream);
+    if (e != hipSuccess) return fail(DKM_ERR_DEVICE, "search launch: %s", hipGetErrorString(e));
+    return DKM_OK;
 }
 
 dkm_status dkm_medoid(dkm_index *ix, float *out_vector, int64_t *out_row) {

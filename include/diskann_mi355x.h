@@ -186,6 +186,25 @@ dkm_status dkm_index_synchronize(dkm_index *ix);
  * *first_failed (-1 when none; may be NULL). Outputs of a failed query are not valid. */
 dkm_status dkm_search_status(dkm_index *ix, int64_t *first_failed);
 
+/* Allocation-free variant of dkm_search_batch_device (same arguments, results and per-query
+ * k / L checks) for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No reference counterpart.
+ * The launch's scratch (visited-set regions, the longest-L-first order, spherical query
+ * records, and the status words when d_out_status is NULL) comes from a fixed 64 MiB device
+ * array the library holds on each GPU until the process ends; the call makes no device
+ * allocation. A launch needing more (about nq x 64 KiB at max_L <= 1024) is refused with
+ * DKM_ERR_CAPACITY. d_out_status (nq words, may be NULL) receives each query's status:
+ * 0 = ok, 1 = visited-set overflow, 2 = iteration bound, 4 = k / L refused. Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (any handle, any stream) race,
+ * so the caller serialises them (one stream, or synchronise between calls). */
+dkm_status dkm_search_batch_device_noalloc(dkm_index *ix, int strategy, const float *d_queries,
+                                           int64_t nq, const uint32_t *d_k, const uint32_t *d_L,
+                                           uint32_t max_k, uint32_t max_L, uint32_t beam_width,
+                                           int64_t k_stride, uint32_t *d_out_ids,
+                                           float *d_out_dists, uint32_t *d_out_counts,
+                                           uint32_t *d_out_cmps, uint32_t *d_out_hops,
+                                           uint32_t *d_out_status);
+
 /* HIP graph of one batched search launch: dkm_search_graph_create validates and captures
  * the launch dkm_search_batch_device_stats would make with these arguments (the kernels
  * only: no copies or memsets) into a graph with its own visited-set regions and status

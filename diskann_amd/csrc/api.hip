@@ -2166,6 +2166,174 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
     return DKM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Prune parameters shared by dkm_prune_batch and dkm_prune_range (the index's Hybrid
+// prune setting applies as in dkm_build).
+dkm_status prune_setup(dkm_index *ix, float alpha, int64_t max_occlusion, int saturate,
+                       int64_t list_stride, dkm::BuildParams &bp) {
+    if (!(alpha >= 1.0f)) return fail(DKM_ERR_INVALID_ARG, "alpha must be >= 1");
+    if (max_occlusion < 1 || max_occlusion > 4096) return fail(DKM_ERR_INVALID_ARG, "bad max_occlusion");
+    if (ix->R > 256) return fail(DKM_ERR_INVALID_ARG, "pruned degree above 256 is not supported");
+    if (list_stride < 1 || list_stride > 4096)
+        return fail(DKM_ERR_INVALID_ARG, "candidate lists must hold 1 to 4096 entries");
+    if (ix->build_max_fp >= 0 && (!ix->d_codes || !ix->codes_set))
+        return fail(DKM_ERR_NOT_READY, "the Hybrid prune needs the PQ codes of every row");
+    int64_t cap = 2;
+    while (cap < list_stride) cap <<= 1;
+    bp = dkm::BuildParams{};
+    bp.metric = ix->metric;
+    bp.dim = (int)ix->dim;
+    bp.vstride = ix->vstride;
+    bp.vectors = ix->d_vec;
+    bp.graph = ix->d_graph;
+    bp.gstride = (int)ix->gstride;
+    bp.R = (int)ix->R;
+    bp.R_max = (int)ix->R_max;
+    bp.alpha = alpha;
+    bp.max_occlusion = (int)max_occlusion;
+    bp.saturate = saturate;
+    bp.sort_cap = (int)cap;
+    bp.cache_accepted = (size_t)ix->R * ix->dim * sizeof(float) <= 48 * 1024;
+    bp.max_fp = ix->build_max_fp < 0 ? -1 : (int)std::min<int64_t>(ix->build_max_fp, 0x7FFFFFFF);
+    bp.codes = ix->d_codes;
+    bp.pivots = ix->d_pivots;
+    bp.pq_off = ix->d_offsets;
+    bp.pq_M = (int)ix->M;
+    if (dkm::build_prune_lds_bytes(bp) > 160 * 1024)
+        return fail(DKM_ERR_CAPACITY, "a candidate pool of %lld ids does not fit in LDS",
+                    (long long)list_stride);
+    return DKM_OK;
+}
+
+// Locations per prune launch (one block each).
+constexpr int64_t kPruneChunk = 4096;
+
+}  // namespace
+
+extern "C" {
+
+dkm_status dkm_prune_batch(dkm_index *ix, const uint32_t *locations, int64_t n,
+                           const uint32_t *lists, const uint32_t *list_len, int64_t list_stride,
+                           float alpha, int64_t max_occlusion, int saturate, int force_saturate,
+                           uint32_t *out_lists, uint32_t *out_counts, int64_t out_stride) {
+    if (!ix || (n > 0 && (!locations || !lists || !list_len || !out_lists || !out_counts)))
+        return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (n < 0) return fail(DKM_ERR_INVALID_ARG, "negative count");
+    if (out_stride < ix->R) return fail(DKM_ERR_INVALID_ARG, "output rows hold fewer than R ids");
+    dkm::BuildParams bp;
+    dkm_status st = prune_setup(ix, alpha, max_occlusion, saturate, list_stride, bp);
+    if (st) return st;
+    for (int64_t i = 0; i < n; ++i) {
+        if (locations[i] >= ix->rows)
+            return fail(DKM_ERR_INDEX, "location %u out of range (%lld rows)", locations[i],
+                        (long long)ix->rows);
+        if (list_len[i] > list_stride)
+            return fail(DKM_ERR_INVALID_ARG, "list %lld is longer than the stride", (long long)i);
+        for (uint32_t j = 0; j < list_len[i]; ++j)
+            if (lists[i * list_stride + j] >= ix->rows)
+                return fail(DKM_ERR_INDEX, "candidate %u out of range (%lld rows)",
+                            lists[i * list_stride + j], (long long)ix->rows);
+    }
+    if (n == 0) return DKM_OK;
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    const int64_t B = std::min<int64_t>(n, kPruneChunk);
+    DevBuf<uint32_t> loc, dl, dn, pend, pcnt;
+    HIP_TRY(loc.ensure(B));
+    HIP_TRY(dl.ensure((size_t)B * list_stride));
+    HIP_TRY(dn.ensure(B));
+    HIP_TRY(pend.ensure((size_t)B * ix->R));
+    HIP_TRY(pcnt.ensure(B));
+    bp.pend = pend.p;
+    bp.pend_cnt = pcnt.p;
+    std::vector<uint32_t> hp((size_t)B * ix->R), hc(B);
+    for (int64_t b0 = 0; b0 < n; b0 += B) {
+        const int nb = (int)std::min<int64_t>(B, n - b0);
+        HIP_TRY(hipMemcpyAsync(loc.p, locations + b0, sizeof(uint32_t) * nb, hipMemcpyHostToDevice, ix->stream));
+        HIP_TRY(hipMemcpyAsync(dl.p, lists + b0 * list_stride, sizeof(uint32_t) * nb * list_stride,
+                               hipMemcpyHostToDevice, ix->stream));
+        HIP_TRY(hipMemcpyAsync(dn.p, list_len + b0, sizeof(uint32_t) * nb, hipMemcpyHostToDevice, ix->stream));
+        bp.batch = nb;
+        bp.batch_ids = loc.p;
+        HIP_TRY(dkm::launch_prune_lists(bp, dl.p, dn.p, (int)list_stride, force_saturate ? 1 : 0,
+                                        ix->stream));
+        HIP_TRY(hipMemcpyAsync(hp.data(), pend.p, sizeof(uint32_t) * nb * ix->R, hipMemcpyDeviceToHost, ix->stream));
+        HIP_TRY(hipMemcpyAsync(hc.data(), pcnt.p, sizeof(uint32_t) * nb, hipMemcpyDeviceToHost, ix->stream));
+        HIP_TRY(hipStreamSynchronize(ix->stream));
+        for (int b = 0; b < nb; ++b) {
+            // an empty list leaves nothing to prune (robust_prune_list's early exit)
+            const uint32_t c = list_len[b0 + b] == 0 ? 0u : hc[b];
+            out_counts[b0 + b] = c;
+            std::memcpy(out_lists + (b0 + b) * out_stride, hp.data() + (size_t)b * ix->R,
+                        sizeof(uint32_t) * c);
+        }
+    }
+    return DKM_OK;
+}
+
+dkm_status dkm_prune_range(dkm_index *ix, const uint32_t *ids, int64_t n, float alpha,
+                           int64_t max_occlusion, int saturate) {
+    if (!ix || (n > 0 && !ids)) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (n < 0) return fail(DKM_ERR_INVALID_ARG, "negative count");
+    // an adjacency row holds at most gstride - 1 ids
+    const int64_t stride = ix->gstride - 1;
+    dkm::BuildParams bp;
+    dkm_status st = prune_setup(ix, alpha, max_occlusion, saturate, stride, bp);
+    if (st) return st;
+    for (int64_t i = 0; i < n; ++i)
+        if (ids[i] >= ix->rows)
+            return fail(DKM_ERR_INDEX, "id %u out of range (%lld rows)", ids[i], (long long)ix->rows);
+    if (n == 0) return DKM_OK;
+    std::lock_guard<std::mutex> g(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device));
+    // the current lists of every id (rows of gstride u32, count in the last slot)
+    const int64_t B = std::min<int64_t>(n, kPruneChunk);
+    DevBuf<uint32_t> loc, rows, dl, dn, pend, pcnt;
+    HIP_TRY(loc.ensure(B));
+    HIP_TRY(rows.ensure((size_t)B * ix->gstride));
+    HIP_TRY(dl.ensure((size_t)B * stride));
+    HIP_TRY(dn.ensure(B));
+    HIP_TRY(pend.ensure((size_t)B * ix->R));
+    HIP_TRY(pcnt.ensure(B));
+    bp.pend = pend.p;
+    bp.pend_cnt = pcnt.p;
+    std::vector<uint32_t> hrow((size_t)B * ix->gstride), hl((size_t)B * stride), hn(B), hid(B);
+    for (int64_t b0 = 0; b0 < n; b0 += B) {
+        const int nb = (int)std::min<int64_t>(B, n - b0);
+        for (int b = 0; b < nb; ++b)
+            HIP_TRY(hipMemcpyAsync(hrow.data() + (size_t)b * ix->gstride,
+                                   ix->d_graph + (size_t)ids[b0 + b] * ix->gstride,
+                                   sizeof(uint32_t) * ix->gstride, hipMemcpyDeviceToHost, ix->stream));
+        HIP_TRY(hipStreamSynchronize(ix->stream));
+        // prune_range skips lists within the pruned degree
+        int m = 0;
+        for (int b = 0; b < nb; ++b) {
+            const uint32_t *r = hrow.data() + (size_t)b * ix->gstride;
+            const uint32_t c = r[ix->gstride - 1];
+            if ((int64_t)c <= ix->R) continue;
+            if ((int64_t)c > stride) return fail(DKM_ERR_INDEX, "row %u holds a bad count", ids[b0 + b]);
+            hid[m] = ids[b0 + b];
+            hn[m] = c;
+            std::memcpy(hl.data() + (size_t)m * stride, r, sizeof(uint32_t) * c);
+            ++m;
+        }
+        if (m == 0) continue;
+        HIP_TRY(hipMemcpyAsync(loc.p, hid.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, ix->stream));
+        HIP_TRY(hipMemcpyAsync(dl.p, hl.data(), sizeof(uint32_t) * m * stride, hipMemcpyHostToDevice, ix->stream));
+        HIP_TRY(hipMemcpyAsync(dn.p, hn.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, ix->stream));
+        bp.batch = m;
+        bp.batch_ids = loc.p;
+        HIP_TRY(dkm::launch_prune_lists(bp, dl.p, dn.p, (int)stride, 0, ix->stream));
+        // every pruned list is written back after all of the chunk's prunes read the graph
+        HIP_TRY(dkm::launch_set_lists(bp, ix->stream));
+        HIP_TRY(hipStreamSynchronize(ix->stream));
+    }
+    return DKM_OK;
+}
+
 // ------------------------------------------------------------------------------------
 // Serving: continuous batching over the batch search kernel
 // ------------------------------------------------------------------------------------

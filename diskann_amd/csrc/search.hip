@@ -880,8 +880,8 @@ __device__ inline void compute_distances(const SearchParams &p, Lds &s, int n) {
     __syncthreads();
 }
 
-// One query's search with the wave's LDS state; used by the batched kernel (one block
-// per query) and by the persistent serving kernel (a block loops over requests).
+// One query's search with the wave's LDS state; the batched kernel runs one per block
+// (the serving lanes launch the same kernel on their batches).
 struct QueryJob {
     const float *query;
     int L, k;

@@ -228,6 +228,33 @@ class DiskANNIndex:
                                  int(L_build), float(alpha), int(max_occlusion), int(saturate)),
               self.lib)
 
+    def prune_batch(self, locations, lists, alpha=1.2, max_occlusion=750, saturate=False,
+                    force_saturate=False):
+        """robust_prune_list of each candidate list around its location (row id): returns
+        a list of pruned id arrays. `lists` is a sequence of id arrays (one per location)."""
+        loc = np.ascontiguousarray(locations, dtype=np.uint32).reshape(-1)
+        if len(lists) != loc.shape[0]:
+            raise ValueError("one candidate list per location")
+        lens = np.array([len(x) for x in lists], dtype=np.uint32)
+        stride = max(1, int(lens.max()) if lens.size else 1)
+        buf = np.zeros((loc.shape[0], stride), dtype=np.uint32)
+        for i, x in enumerate(lists):
+            buf[i, :len(x)] = x
+        out = np.zeros((loc.shape[0], max(self.R, 1)), dtype=np.uint32)
+        cnt = np.zeros(loc.shape[0], dtype=np.uint32)
+        check(self.lib.dkm_prune_batch(self._h, _p(loc), loc.shape[0], _p(buf), _p(lens), stride,
+                                       float(alpha), int(max_occlusion), int(saturate),
+                                       int(force_saturate), _p(out), _p(cnt), out.shape[1]),
+              self.lib)
+        return [out[i, :cnt[i]].copy() for i in range(loc.shape[0])]
+
+    def prune_range(self, ids=None, alpha=1.2, max_occlusion=750, saturate=False):
+        """DiskANNIndex::prune_range: prune every list of `ids` longer than R in place."""
+        ids = np.arange(self.max_points, dtype=np.uint32) if ids is None else \
+            np.ascontiguousarray(ids, dtype=np.uint32)
+        check(self.lib.dkm_prune_range(self._h, _p(ids), ids.shape[0], float(alpha),
+                                       int(max_occlusion), int(saturate)), self.lib)
+
     def medoid(self):
         """(row, vector) of compute_medoid over the data rows."""
         v = np.zeros(self.dim, dtype=np.float32)

@@ -241,6 +241,30 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
                      int64_t intra_batch, uint32_t L_build, float alpha, int64_t max_occlusion,
                      int saturate);
 
+/* RobustPrune of caller-given candidate lists: the PruneStrategy / PruneAccessor seam
+ * (diskann/src/graph/glue.rs:839-966) with robust_prune_list (diskann/src/graph/index.rs:
+ * 2402-2450) and occlude_list (:2570-2782). For each location i (a row id), the candidates
+ * lists[i * list_stride, + list_len[i]) (ids, self occurrences dropped, order kept) get
+ * their distances to the location's vector, the pool is sorted (distance, then list
+ * position) and cut to max_occlusion, and occlude_list with alpha picks at most R ids,
+ * saturating when `saturate` and alpha > 1 or when force_saturate
+ * (prune::Options::force_saturate). The index's Hybrid prune setting
+ * (dkm_set_build_hybrid) applies: fill order location, then the list. The pruned lists go
+ * to out_lists (n rows of out_stride >= R ids) with their lengths in out_counts; an empty
+ * list prunes to nothing. Host buffers; the graph is not read or changed. */
+dkm_status dkm_prune_batch(dkm_index *ix, const uint32_t *locations, int64_t n,
+                           const uint32_t *lists, const uint32_t *list_len, int64_t list_stride,
+                           float alpha, int64_t max_occlusion, int saturate, int force_saturate,
+                           uint32_t *out_lists, uint32_t *out_counts, int64_t out_stride);
+
+/* DiskANNIndex::prune_range (diskann/src/graph/index.rs:2788-2830): every id whose
+ * adjacency list is longer than R is pruned with robust_prune_list over its own list
+ * (no forced saturation) and its list replaced; shorter lists are left alone. A prune reads
+ * only vectors, so the ids are pruned together and the result equals the reference's
+ * sequential loop. */
+dkm_status dkm_prune_range(dkm_index *ix, const uint32_t *ids, int64_t n, float alpha,
+                           int64_t max_occlusion, int saturate);
+
 /* 8-bit scalar quantization (diskann-quantization/src/scalar). A compressed vector uses
  * the reference's canonical layout (meta/vector.rs:384-434): an f32 compensation, then one
  * code byte per dimension, 4 + dim bytes per row.

@@ -1677,6 +1677,13 @@ static int64_t prune_list_opt(const dko_build_cfg *cfg, const float *vectors, in
     return c;
 }
 
+int64_t dko_prune_list(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                       int64_t dim, uint32_t location, const uint32_t *list, int64_t n,
+                       int force_saturate, uint32_t *out) {
+    if (n == 0) return 0; /* robust_prune_list's early exit */
+    return prune_list_opt(cfg, vectors, vstride, dim, location, list, n, out, force_saturate);
+}
+
 /* add_edge_and_prune, index.rs:2269-2346: append unique targets; prune when the list
  * exceeds the max degree with slack. */
 static void add_edge_and_prune(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,

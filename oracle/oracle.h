@@ -233,6 +233,15 @@ int64_t dko_occlude_list(const dko_build_cfg *cfg, const float *vectors, int64_t
 
 float dko_update_occlude_factor(int metric, float d_ik, float d_jk, float cur, float alpha);
 
+/* robust_prune_list (diskann/src/graph/index.rs:2402-2450): distances from `location` to
+ * every id of `list` other than itself (list order), sorted pool cut to max_occlusion,
+ * occlude_list; cfg->max_fp >= 0 selects the Hybrid prune (fill order location, then the
+ * list). force_saturate: prune::Options::force_saturate. Returns the number of ids
+ * written to out (room for cfg->R). */
+int64_t dko_prune_list(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                       int64_t dim, uint32_t location, const uint32_t *list, int64_t n,
+                       int force_saturate, uint32_t *out);
+
 /* Mutable graph used by the build. */
 typedef struct {
     int64_t rows, max_degree;

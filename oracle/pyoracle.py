@@ -136,6 +136,8 @@ def _setup(L):
     L.dko_search_batch_mixed.restype = C.c_double
     L.dko_occlude_list.argtypes = [C.POINTER(BuildCfg), P, I64, I64, P, P, I64, C.c_uint32, P]
     L.dko_occlude_list.restype = I64
+    L.dko_prune_list.argtypes = [C.POINTER(BuildCfg), P, I64, I64, C.c_uint32, P, I64, C.c_int, P]
+    L.dko_prune_list.restype = I64
     L.dko_update_occlude_factor.argtypes = [C.c_int, F, F, F, F]
     L.dko_update_occlude_factor.restype = F
     L.dko_build.argtypes = [C.POINTER(BuildCfg), P, I64, I64, C.POINTER(Graph), P, I64, P,
@@ -573,6 +575,28 @@ def occlude_list(metric, vectors, pool_ids, pool_d, self_id, R, alpha, saturate=
     out = np.zeros(max(R, 1) + pool_ids.shape[0], dtype=np.uint32)
     n = lib().dko_occlude_list(C.byref(cfg), _p(vectors), vectors.shape[1], vectors.shape[1],
                                _p(pool_ids), _p(pool_d), pool_ids.shape[0], self_id, _p(out))
+    return out[:n]
+
+
+def prune_list(metric, vectors, location, lst, R, alpha, max_occlusion=750, saturate=False,
+               force_saturate=False, max_fp=None, codes=None, pivots=None, pq_offsets=None):
+    """robust_prune_list of `lst` around row `location`; returns the pruned ids."""
+    vectors = np.ascontiguousarray(vectors, dtype=np.float32)
+    lst = np.ascontiguousarray(lst, dtype=np.uint32)
+    keep = []
+    if max_fp is None:
+        hyb = (-1, None, None, None, 0)
+    else:
+        cd = np.ascontiguousarray(codes, dtype=np.uint8)
+        pv = np.ascontiguousarray(pivots, dtype=np.float32)
+        off = np.ascontiguousarray(pq_offsets, dtype=np.int64)
+        keep = [cd, pv, off]
+        hyb = (int(max_fp), _p(cd), _p(pv), _p(off), off.shape[0] - 1)
+    cfg = BuildCfg(metric, R, R, 0, alpha, max_occlusion, int(saturate), R, *hyb)
+    out = np.zeros(max(R, 1), dtype=np.uint32)
+    n = lib().dko_prune_list(C.byref(cfg), _p(vectors), vectors.shape[1], vectors.shape[1],
+                             int(location), _p(lst), lst.shape[0], int(force_saturate), _p(out))
+    del keep
     return out[:n]
 
 

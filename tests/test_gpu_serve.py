@@ -1,6 +1,7 @@
-"""GPU serving loop (dkm_server_*): a request served through the persistent kernel must
-return exactly what a batched search of that query alone returns (ids, f32 distances,
-cmps, hops), whatever else is in flight, with mixed k and L per request."""
+"""GPU serving loop (dkm_server_*): a request served by the continuous-batching lanes
+(batched search launches on their own streams) must return exactly what a batched search
+of that query alone returns (ids, f32 distances, cmps, hops), whatever else is in flight,
+with mixed k and L per request."""
 import time
 
 import numpy as np

@@ -60,3 +60,30 @@ def test_grid_insert_golden(name):
         if dim == 1:
             assert [int(i) for i in ids] == [int(e[0]) for e in s["results"]], name
             assert (cm, hp) == (s["comparisons"], s["hops"]), name
+
+
+def test_prune_list_on_a_line_keeps_the_nearest_on_each_side():
+    # location 0 at the origin, candidates on one line: for any candidate beyond the
+    # nearest one on its side, that nearest one occludes it by a factor d_ik / d_jk > alpha
+    # (occlude_list, diskann/src/graph/index.rs:2570-2782), so only the two nearest remain;
+    # saturation then appends the rest in pool (distance) order
+    xs = np.array([0.0, 3.0, -2.0, 1.0, 5.0, -1.0, 2.0, -4.0], np.float32)
+    vec = np.zeros((len(xs), 4), np.float32)
+    vec[:, 0] = xs
+    lst = np.arange(1, len(xs), dtype=np.uint32)
+    got = po.prune_list(po.L2, vec, 0, lst, 6, 1.2)
+    assert list(got) == [3, 5]                        # x = 1, then x = -1 (list order on ties)
+    sat = po.prune_list(po.L2, vec, 0, lst, 6, 1.2, saturate=True)
+    assert list(sat) == [3, 5, 2, 6, 1, 7]            # then x = -2, 2, 3, -4 (distance, list order)
+    assert list(po.prune_list(po.L2, vec, 0, np.array([0], np.uint32), 6, 1.2)) == []
+    assert list(po.prune_list(po.L2, vec, 0, np.array([], np.uint32), 6, 1.2)) == []
+
+
+def test_prune_list_equals_occlude_list_on_the_sorted_pool():
+    rng = np.random.default_rng(4)
+    vec = rng.standard_normal((300, 16)).astype(np.float32)
+    lst = rng.choice(np.arange(1, 300), 120, replace=False).astype(np.uint32)
+    d = np.array([po.distance(po.L2, vec[0], vec[i]) for i in lst], np.float32)
+    order = np.argsort(d, kind="stable")
+    want = po.occlude_list(po.L2, vec, lst[order], d[order], 0, 12, 1.2)
+    assert np.array_equal(po.prune_list(po.L2, vec, 0, lst, 12, 1.2), want)

@@ -224,7 +224,7 @@ __device__ inline void visit_group(const SearchParams &p, Lds &s, const uint32_t
         st[e] = act[e] ? 0 : 2;
     }
     // round: a load per pending id; an id found empty is claimed with a compare-and-swap
-    // in the same round (dups, the common case, cost one load)
+    // in the same round (ids already visited cost one load)
     for (uint64_t round = 0; round < GH; ++round) {
         bool pending = false;
 #pragma unroll
@@ -895,7 +895,9 @@ struct QueryJob {
     const uint8_t *sph_rec;  // spherical: this query's prepared record
 };
 
-template <int METRIC, int MODE, int DIM, int PQLEN>
+// RM: the metric of the FP32 rerank (the index metric; a template so that a kernel holds
+// only the rerank arithmetic it runs).
+template <int METRIC, int MODE, int DIM, int PQLEN, int RM>
 __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJob &job) {
     constexpr bool PQ = MODE == 1 || MODE == 2;
     const int lane = threadIdx.x;
@@ -980,6 +982,13 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
     }
     cmps = p.nstart;
 
+    // Beam width 1: the next hop's beam node is known once this hop's new items are
+    // ranked (the best new item or the old queue's next unvisited entry, whichever the
+    // merge puts first), so its adjacency row is loaded into registers (lanes j and
+    // j + 64 of a row of at most 128 slots) before the merge and used if the guess holds.
+    const bool prefetch = p.beam_width == 1 && p.gstride <= 2 * kWave;
+    uint32_t pre_id = kEmpty, pre0 = 0u, pre1 = 0u;
+
     // ---- main loop -----------------------------------------------------------------
     for (uint32_t iter = 0;; ++iter) {
         if (iter > p.max_iters) {
@@ -1020,9 +1029,14 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
 
         // expand: stage the beam nodes' adjacency rows (ids + count slot) with one round
         // of loads, then filter the neighbours through the visited set
-        for (int b = 0; b < nbeam; ++b) {
-            const uint32_t *row = p.graph + (size_t)s.beam[b] * p.gstride;
-            for (int j = lane; j < p.gstride; j += kWave) s.rows[b * p.gstride + j] = row[j];
+        if (prefetch && nbeam == 1 && s.beam[0] == pre_id) {
+            if (lane < p.gstride) s.rows[lane] = pre0;
+            if (lane + kWave < p.gstride) s.rows[lane + kWave] = pre1;
+        } else {
+            for (int b = 0; b < nbeam; ++b) {
+                const uint32_t *row = p.graph + (size_t)s.beam[b] * p.gstride;
+                for (int j = lane; j < p.gstride; j += kWave) s.rows[b * p.gstride + j] = row[j];
+            }
         }
         __syncthreads();
         // the exact LDS table keeps room for one more hop's ids; past that the HBM table
@@ -1038,6 +1052,18 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
         compute_distances<METRIC, MODE, DIM, PQLEN>(p, s, nn);
         int nvalid = rank_new(s, nn);
         __syncthreads();
+        if (prefetch) {
+            // new items go before old ones of equal distance
+            const bool have_old = next < size;
+            const float d_old = have_old ? qd[next] : 0.0f;
+            pre_id = nvalid > 0 && (!have_old || s.sn_d[0] <= d_old) ? s.sn_i[0]
+                     : have_old ? qi[next] : kEmpty;
+            if (pre_id != kEmpty) {
+                const uint32_t *row = p.graph + (size_t)pre_id * p.gstride;
+                pre0 = lane < p.gstride ? row[lane] : 0u;
+                pre1 = lane + kWave < p.gstride ? row[lane + kWave] : 0u;
+            }
+        }
         size = merge_queue(s, size, nvalid, cap, qd, qi, next, &cursor);
         cmps += (uint32_t)nn;
         hops += (uint32_t)nbeam;
@@ -1087,39 +1113,41 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
             n += __builtin_popcountll(m);
         }
         __syncthreads();
-        if (p.metric == METRIC_L2) fp_distances<METRIC_L2, DIM>(p, s.q, qi, qd, n, s.starts[0]);
-        else if (p.metric == METRIC_IP) fp_distances<METRIC_IP, DIM>(p, s.q, qi, qd, n, s.starts[0]);
-        else if (p.metric == METRIC_COSNORM) fp_distances<METRIC_COSNORM, DIM>(p, s.q, qi, qd, n, s.starts[0]);
-        else fp_distances<METRIC_COSINE, 0>(p, s.q, qi, qd, n, s.starts[0]);
+        fp_distances<RM, RM == METRIC_COSINE ? 0 : DIM>(p, s.q, qi, qd, n, s.starts[0]);
         __syncthreads();
-        // stable selection of the k smallest (distance, position) keys: k rounds of a
-        // wave-wide arg-min; a chosen entry is marked through bit 31 of its id
+        // stable order of the (distance, position) keys: every candidate's rank is the
+        // number of keys below its own (distances as order-preserving u32 with -0 == +0,
+        // NaN above +inf), and the ranks below k are the output positions
         const int kk = n < k ? n : k;
-        for (int r = 0; r < kk; ++r) {
-            unsigned long long bk = ~0ull;
-            for (int i = lane; i < n; i += kWave) {
-                if (!(qi[i] & kVisitedBit)) {
-                    const float d = qd[i];
-                    uint32_t u = __float_as_uint(d == 0.0f ? 0.0f : d);
-                    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-                    const unsigned long long key = ((unsigned long long)u << 32) | (uint32_t)i;
-                    bk = key < bk ? key : bk;
-                }
+        auto okey = [](float d) -> uint32_t {
+            const uint32_t u = __float_as_uint(d == 0.0f ? 0.0f : d);
+            return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+        };
+        for (int b0 = 0; b0 < n; b0 += 4 * kWave) {
+            uint32_t ku[4];
+            int rk[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = b0 + lane + kWave * e;
+                ku[e] = i < n ? okey(qd[i]) : 0xFFFFFFFFu;
+                rk[e] = 0;
+            }
+            for (int j = 0; j < n; ++j) {
+                const uint32_t kj = okey(qd[j]);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    rk[e] += (kj < ku[e]) || (kj == ku[e] && j < b0 + lane + kWave * e);
             }
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const unsigned long long o = __shfl_xor(bk, off);
-                bk = o < bk ? o : bk;
+            for (int e = 0; e < 4; ++e) {
+                const int i = b0 + lane + kWave * e;
+                if (i < n && rk[e] < kk) {
+                    oids[rk[e]] = qi[i];
+                    ods[rk[e]] = qd[i];
+                }
             }
-            const int win = (int)(bk & 0xFFFFFFFFull);
-            if (lane == 0) {
-                oids[r] = qi[win];
-                ods[r] = qd[win];
-                qi[win] |= kVisitedBit;
-            }
-            __syncthreads();
         }
-        count = n < k ? n : k;
+        count = kk;
     }
     for (int i = count + lane; i < k; i += kWave) {
         oids[i] = kEmpty;
@@ -1134,8 +1162,10 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
     }
 }
 
-template <int METRIC, int MODE, int DIM, int PQLEN>
-__global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
+// At most 128 VGPRs: four waves (queries) per SIMD.
+template <int METRIC, int MODE, int DIM, int PQLEN, int RM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
+beam_search_kernel(SearchParams p) {
     extern __shared__ __align__(16) char smem[];
     if ((int)blockIdx.x >= p.nq) return;
     const int qidx = p.order ? (int)p.order[blockIdx.x] : (int)blockIdx.x;
@@ -1159,7 +1189,7 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
     j.rec_len = p.rec_len ? p.rec_len + qidx : nullptr;
     j.out_degsum = p.out_degsum ? p.out_degsum + qidx : nullptr;
     j.sph_rec = p.sph_qrec ? p.sph_qrec + (size_t)qidx * sph_query_record_bytes(p.sphq) : nullptr;
-    search_query<METRIC, MODE, DIM, PQLEN>(p, s, j);
+    search_query<METRIC, MODE, DIM, PQLEN, RM>(p, s, j);
     __syncthreads();
     if (threadIdx.x == 0) p.status[qidx] = status;
     if (p.done) {
@@ -1172,10 +1202,10 @@ __global__ __launch_bounds__(64) void beam_search_kernel(SearchParams p) {
 
 // resident == nullptr: launch one block per query; otherwise report in *resident how many
 // blocks of this configuration the current device holds at once.
-template <int METRIC, int MODE, int DIM, int PQLEN>
-static hipError_t launch_t(const SearchParams &p, hipStream_t stream, int *resident) {
+template <int METRIC, int MODE, int DIM, int PQLEN, int RM>
+static hipError_t launch_k(const SearchParams &p, hipStream_t stream, int *resident) {
     size_t lds = lds_bytes(p);
-    auto kern = beam_search_kernel<METRIC, MODE, DIM, PQLEN>;
+    auto kern = beam_search_kernel<METRIC, MODE, DIM, PQLEN, RM>;
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -1189,6 +1219,27 @@ static hipError_t launch_t(const SearchParams &p, hipStream_t stream, int *resid
     }
     hipLaunchKernelGGL(kern, dim3((unsigned)p.nq), dim3(64), lds, stream, p);
     return hipGetLastError();
+}
+
+// Traversal with full precision or SQ codes reranks (if at all) with the traversal metric;
+// PQ and spherical traversal rerank with the index metric. The rerank of a dimension-
+// specialized variant is specialized for L2 / inner product only.
+template <int METRIC, int MODE, int DIM, int PQLEN>
+static hipError_t launch_t(const SearchParams &p, hipStream_t stream, int *resident) {
+    if constexpr (MODE == 0 || MODE == 3) {
+        return launch_k<METRIC, MODE, DIM, PQLEN, METRIC>(p, stream, resident);
+    } else if constexpr (METRIC == METRIC_IP && MODE != 4 && MODE != 5) {
+        return launch_k<METRIC, MODE, DIM, PQLEN, METRIC_IP>(p, stream, resident);
+    } else {
+        switch (p.metric) {
+        case METRIC_L2: return launch_k<METRIC, MODE, DIM, PQLEN, METRIC_L2>(p, stream, resident);
+        case METRIC_IP:
+            if constexpr (DIM == 0) return launch_k<METRIC, MODE, 0, PQLEN, METRIC_IP>(p, stream, resident);
+            else return launch_k<METRIC, MODE, DIM, PQLEN, METRIC_IP>(p, stream, resident);
+        case METRIC_COSNORM: return launch_k<METRIC, MODE, 0, PQLEN, METRIC_COSNORM>(p, stream, resident);
+        default: return launch_k<METRIC, MODE, 0, PQLEN, METRIC_COSINE>(p, stream, resident);
+        }
+    }
 }
 
 template <int METRIC>

@@ -8,6 +8,8 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdiskann_mi355x.so")
+# hardware queues the serving lanes need: one per lane (16) plus the index and null streams
+SERVE_HW_QUEUES = 20
 
 # Every symbol declared in include/diskann_mi355x.h.
 EXPORTS = [
@@ -18,7 +20,7 @@ EXPORTS = [
     "dkm_pq_train_draws",
     "dkm_index_create", "dkm_index_destroy", "dkm_set_vectors", "dkm_set_start_points",
     "dkm_set_graph", "dkm_get_graph", "dkm_set_pq", "dkm_set_codes", "dkm_encode_index",
-    "dkm_get_codes", "dkm_search_batch", "dkm_search_batch_device", "dkm_index_synchronize",
+    "dkm_get_codes", "dkm_search_batch", "dkm_disk_search_batch", "dkm_search_batch_device", "dkm_index_synchronize",
     "dkm_profile", "dkm_profile_read", "dkm_set_tuning", "dkm_build", "dkm_prune_batch", "dkm_prune_range", "dkm_set_build_hybrid", "dkm_medoid",
     "dkm_bruteforce", "dkm_search_batch_device_stats", "dkm_autotune",
     "dkm_search_graph_create", "dkm_search_graph_launch", "dkm_search_graph_destroy",
@@ -51,9 +53,11 @@ def load():
         raise RuntimeError(
             "libdiskann_mi355x.so is not built; run `python -m diskann_amd.build` (no CPU "
             "fallback exists)")
-    # the serving lanes (eight streams) run concurrently only with a hardware queue each;
-    # HIP reads this once, when the runtime starts, and its default is 4
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", "12")
+    # the serving lanes (sixteen streams) run concurrently only with a hardware queue
+    # each; HIP reads this once, when the runtime starts, and its default is 4 (streams
+    # beyond the queues share them and their launches serialise)
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < SERVE_HW_QUEUES:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(SERVE_HW_QUEUES)
     lib = C.CDLL(LIB_PATH)
     _declare(lib)
     _lib = lib
@@ -114,6 +118,7 @@ def _declare(L):
         "dkm_minmax_query_distance_batch": [P, I, P, I64, P, I64, P, I64, P],
         "dkm_get_codes": [P, P, I64, I64],
         "dkm_search_batch": [P, I, P, I64, P, P, U32, I64, P, P, P, P, P],
+        "dkm_disk_search_batch": [P, P, I64, P, P, U32, U32, C.c_uint64, I64, P, P, P, P, P],
         "dkm_search_batch_device": [P, I, P, I64, P, P, U32, U32, U32, I64, P, P, P, P, P],
         "dkm_index_synchronize": [P],
         "dkm_profile": [P, I],

@@ -77,6 +77,13 @@ struct SearchParams {
     const SphMeta *sph_meta;  // rows
     uint8_t *sph_qrec;        // nq prepared queries (sph_query_record_bytes each), written by
                               // launch_search before the search kernel
+    // Disk-index search (the DiskAccessor of diskann-disk) when tlut is set: PQ distances
+    // from the transposed table's arithmetic, the start point is a data row kept in the
+    // results (keep_starts), and at most io_limit beam nodes per query are expanded
+    // (loaded); io_limit is not read otherwise.
+    int tlut;
+    int keep_starts;
+    uint32_t io_limit;
 };
 
 size_t search_lds_bytes(const SearchParams &p);

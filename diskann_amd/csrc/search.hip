@@ -596,6 +596,24 @@ __device__ inline void adc_lut(const SearchParams &p, Lds &s, const uint32_t *id
     }
 }
 
+// One chunk term of the transposed pivot table (TransposedTable::process_into,
+// diskann-quantization/src/product/tables/transposed/table.rs:285-313 with the kernels of
+// pivots.rs:785-820,915-948): a fold over the chunk's dimensions from 0, L2
+// acc = fma(p - q, p - q, acc), inner product acc = fma(-q, p, acc).
+template <bool IP>
+__device__ __forceinline__ float transposed_term(const float *q, const float *pv, int len) {
+    float acc = 0.0f;
+    for (int d = 0; d < len; ++d) {
+        if (IP) {
+            acc = __builtin_fmaf(-q[d], pv[d], acc);
+        } else {
+            const float t = pv[d] - q[d];
+            acc = __builtin_fmaf(t, t, acc);
+        }
+    }
+    return acc;
+}
+
 // Direct ADC without a table: every chunk term is recomputed from the pivots (held in
 // L2) with exactly the arithmetic that fills the table, so the result is bit-identical to
 // the table path while the query keeps no 32-48 KiB table in LDS. A group of G lanes
@@ -604,7 +622,7 @@ __device__ inline void adc_lut(const SearchParams &p, Lds &s, const uint32_t *id
 template <typename T>
 __device__ inline T load_as(const uint8_t *p) { return *(const T *)p; }
 
-template <bool IP, int LEN, int M, int G>
+template <bool IP, int LEN, int M, int G, bool TL>
 __device__ inline void adc_direct_group(const SearchParams &p, Lds &s, const uint32_t *ids,
                                         float *out, int n) {
     constexpr int MPER = M / G;
@@ -650,8 +668,12 @@ __device__ inline void adc_direct_group(const SearchParams &p, Lds &s, const uin
 #pragma unroll
                 for (int l = 0; l < LEN; ++l) pc[l] = pv[l];
             }
-            const float v = scalar_simd_4x1<IP>(s.q + m * LEN, pc, LEN);
-            ent[q] = IP ? -v : v;
+            if (TL) {
+                ent[q] = transposed_term<IP>(s.q + m * LEN, pc, LEN);
+            } else {
+                const float v = scalar_simd_4x1<IP>(s.q + m * LEN, pc, LEN);
+                ent[q] = IP ? -v : v;
+            }
         }
         float acc = 0.0f;
 #pragma unroll
@@ -667,22 +689,22 @@ __device__ inline void adc_direct_group(const SearchParams &p, Lds &s, const uin
     }
 }
 
-template <bool IP, int LEN>
+template <bool IP, int LEN, bool TL>
 __device__ inline void adc_direct(const SearchParams &p, Lds &s, const uint32_t *ids, float *out,
                                   int n) {
     if constexpr (LEN == 4) {
         if (p.M == 32) {
-            if (n <= 8) return adc_direct_group<IP, 4, 32, 8>(p, s, ids, out, n);
-            if (n <= 16) return adc_direct_group<IP, 4, 32, 4>(p, s, ids, out, n);
-            if (n <= 32) return adc_direct_group<IP, 4, 32, 2>(p, s, ids, out, n);
-            return adc_direct_group<IP, 4, 32, 1>(p, s, ids, out, n);
+            if (n <= 8) return adc_direct_group<IP, 4, 32, 8, TL>(p, s, ids, out, n);
+            if (n <= 16) return adc_direct_group<IP, 4, 32, 4, TL>(p, s, ids, out, n);
+            if (n <= 32) return adc_direct_group<IP, 4, 32, 2, TL>(p, s, ids, out, n);
+            return adc_direct_group<IP, 4, 32, 1, TL>(p, s, ids, out, n);
         }
     } else if constexpr (LEN == 2) {
         if (p.M == 48) {
-            if (n <= 8) return adc_direct_group<IP, 2, 48, 8>(p, s, ids, out, n);
-            if (n <= 16) return adc_direct_group<IP, 2, 48, 4>(p, s, ids, out, n);
-            if (n <= 32) return adc_direct_group<IP, 2, 48, 2>(p, s, ids, out, n);
-            return adc_direct_group<IP, 2, 48, 1>(p, s, ids, out, n);
+            if (n <= 8) return adc_direct_group<IP, 2, 48, 8, TL>(p, s, ids, out, n);
+            if (n <= 16) return adc_direct_group<IP, 2, 48, 4, TL>(p, s, ids, out, n);
+            if (n <= 32) return adc_direct_group<IP, 2, 48, 2, TL>(p, s, ids, out, n);
+            return adc_direct_group<IP, 2, 48, 1, TL>(p, s, ids, out, n);
         }
     }
     const int lane = lane_id();
@@ -693,8 +715,12 @@ __device__ inline void adc_direct(const SearchParams &p, Lds &s, const uint32_t 
         for (int m = 0; m < M; ++m) {
             const int o0 = s.offs[m], o1 = s.offs[m + 1];
             const float *pv = p.pivots + (size_t)row[m] * p.dim + o0;
-            float v = scalar_simd_4x1<IP>(s.q + o0, pv, o1 - o0);
-            acc += IP ? -v : v;
+            if (TL) {
+                acc += transposed_term<IP>(s.q + o0, pv, o1 - o0);
+            } else {
+                float v = scalar_simd_4x1<IP>(s.q + o0, pv, o1 - o0);
+                acc += IP ? -v : v;
+            }
         }
         out[c] = acc;
     }
@@ -861,7 +887,8 @@ __device__ inline void sph_distances(const SearchParams &p, Lds &s, const uint32
 
 // MODE 0: full precision; 1: PQ with the LDS table; 2: PQ direct (PQLEN = chunk width);
 // 3: 8-bit scalar quantization (PQLEN = lanes per candidate); 4 / 5: 1-bit spherical with
-// the same-as-data / 4-bit query (PQLEN = lanes per candidate).
+// the same-as-data / 4-bit query (PQLEN = lanes per candidate); 6: PQ direct with the
+// transposed table's arithmetic (disk-index search; MODE 1 builds either table).
 template <int METRIC, int MODE, int DIM, int PQLEN>
 __device__ inline void compute_distances(const SearchParams &p, Lds &s, int n) {
     if constexpr (MODE == 0) {
@@ -869,7 +896,9 @@ __device__ inline void compute_distances(const SearchParams &p, Lds &s, int n) {
     } else if constexpr (MODE == 1) {
         adc_lut(p, s, s.nl_i, s.nl_d, n);
     } else if constexpr (MODE == 2) {
-        adc_direct<METRIC == METRIC_IP, PQLEN>(p, s, s.nl_i, s.nl_d, n);
+        adc_direct<METRIC == METRIC_IP, PQLEN, false>(p, s, s.nl_i, s.nl_d, n);
+    } else if constexpr (MODE == 6) {
+        adc_direct<METRIC == METRIC_IP, PQLEN, true>(p, s, s.nl_i, s.nl_d, n);
     } else if constexpr (MODE == 3) {
         sq_distances<METRIC, PQLEN>(p, s, s.nl_i, s.nl_d, n);
     } else if constexpr (MODE == 4) {
@@ -899,7 +928,7 @@ struct QueryJob {
 // only the rerank arithmetic it runs).
 template <int METRIC, int MODE, int DIM, int PQLEN, int RM>
 __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJob &job) {
-    constexpr bool PQ = MODE == 1 || MODE == 2;
+    constexpr bool PQ = MODE == 1 || MODE == 2 || MODE == 6;
     const int lane = threadIdx.x;
     const int L = job.L;
     const int k = job.k;
@@ -947,8 +976,14 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
             int m = e >> 8, c = e & 255;
             int o0 = s.offs[m], o1 = s.offs[m + 1];
             const float *pv = p.pivots + (size_t)c * p.dim + o0;
-            float v = ip ? -scalar_simd_4x1<true>(s.q + o0, pv, o1 - o0)
-                         : scalar_simd_4x1<false>(s.q + o0, pv, o1 - o0);
+            float v;
+            if (p.tlut) {
+                v = ip ? transposed_term<true>(s.q + o0, pv, o1 - o0)
+                       : transposed_term<false>(s.q + o0, pv, o1 - o0);
+            } else {
+                v = ip ? -scalar_simd_4x1<true>(s.q + o0, pv, o1 - o0)
+                       : scalar_simd_4x1<false>(s.q + o0, pv, o1 - o0);
+            }
             s.lut[m * 256 + c] = v;
         }
         __syncthreads();
@@ -957,7 +992,7 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
     float *qd = s.qa_d;
     uint32_t *qi = s.qa_i;
     int size = 0;
-    uint32_t cmps = 0, hops = 0, nrec = 0, degsum = 0;
+    uint32_t cmps = 0, hops = 0, nrec = 0, degsum = 0, ios = 0;
     int lds_count = 0;
     const int spill_at = (int)(p.hslots - (p.hslots >> 3)) - (p.nlcap - kWave);
     uint32_t *gtab = nullptr;
@@ -1026,14 +1061,22 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
         __syncthreads();
         if (nbeam == 0) break;
         nrec += nbeam;
+        // disk-index search: only the first (io_limit - loaded) beam nodes are loaded and
+        // expanded; the rest stay taken (visited, counted as hops) without expansion
+        int nload = nbeam;
+        if (p.tlut) {
+            const uint32_t left = p.io_limit - ios;
+            nload = (uint32_t)nbeam < left ? nbeam : (int)left;
+            ios += (uint32_t)nload;
+        }
 
         // expand: stage the beam nodes' adjacency rows (ids + count slot) with one round
         // of loads, then filter the neighbours through the visited set
-        if (prefetch && nbeam == 1 && s.beam[0] == pre_id) {
+        if (prefetch && nload == 1 && s.beam[0] == pre_id) {
             if (lane < p.gstride) s.rows[lane] = pre0;
             if (lane + kWave < p.gstride) s.rows[lane + kWave] = pre1;
         } else {
-            for (int b = 0; b < nbeam; ++b) {
+            for (int b = 0; b < nload; ++b) {
                 const uint32_t *row = p.graph + (size_t)s.beam[b] * p.gstride;
                 for (int j = lane; j < p.gstride; j += kWave) s.rows[b * p.gstride + j] = row[j];
             }
@@ -1046,7 +1089,7 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
             gtab = job.spill;
             flush_visited(p, s, gtab, err);
         }
-        const int nn = expand_hop(p, s, nbeam, spill, gtab, err, degsum);
+        const int nn = expand_hop(p, s, nload, spill, gtab, err, degsum);
         if (!spill) lds_count += nn;
         __syncthreads();
         compute_distances<METRIC, MODE, DIM, PQLEN>(p, s, nn);
@@ -1082,7 +1125,8 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
             if (i < size) {
                 id = qi[i] & ~kVisitedBit;
                 ok = true;
-                for (int t = 0; t < p.nstart; ++t) ok = ok && (s.starts[t] != id);
+                if (!p.keep_starts)
+                    for (int t = 0; t < p.nstart; ++t) ok = ok && (s.starts[t] != id);
             }
             uint64_t m = ballot(ok);
             int pos = count + prefix_count(m);
@@ -1106,7 +1150,8 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
             if (i < size) {
                 id = qi[i] & ~kVisitedBit;
                 ok = true;
-                for (int t = 0; t < p.nstart; ++t) ok = ok && (s.starts[t] != id);
+                if (!p.keep_starts)
+                    for (int t = 0; t < p.nstart; ++t) ok = ok && (s.starts[t] != id);
             }
             uint64_t m = ballot(ok);
             if (ok) qi[n + prefix_count(m)] = id;
@@ -1276,6 +1321,13 @@ static hipError_t dispatch(const SearchParams &p, hipStream_t stream, int *resid
             if (p.dim == 128) return launch_t<METRIC_L2, 1, 128, 0>(p, stream, resident);
             if (p.dim == 96) return launch_t<METRIC_L2, 1, 96, 0>(p, stream, resident);
             return launch_t<METRIC_L2, 1, 0, 0>(p, stream, resident);
+        }
+        if (p.tlut) {
+            // disk-index search with direct ADC (the transposed table's arithmetic)
+            if (ip) return launch_t<METRIC_IP, 6, 0, 0>(p, stream, resident);
+            if (p.dim == 128 && p.pq_len == 4) return launch_t<METRIC_L2, 6, 128, 4>(p, stream, resident);
+            if (p.dim == 96 && p.pq_len == 2) return launch_t<METRIC_L2, 6, 96, 2>(p, stream, resident);
+            return launch_t<METRIC_L2, 6, 0, 0>(p, stream, resident);
         }
         if (ip) return launch_t<METRIC_IP, 2, 0, 0>(p, stream, resident);
         if (p.dim == 128 && p.pq_len == 4) return launch_t<METRIC_L2, 2, 128, 4>(p, stream, resident);

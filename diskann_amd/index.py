@@ -316,6 +316,30 @@ class DiskANNIndex:
                                         _p(cmps), _p(hops)), self.lib)
         return SearchResult(ids, ds, cnt, cmps, hops)
 
+    def disk_search_batch(self, queries, k, L, beam_width=1, medoid=0, io_limit=None):
+        """Disk-index search (DiskIndexSearcher: PQ traversal from the data row `medoid` with
+        the transposed table's arithmetic, at most io_limit expanded nodes per query, every
+        queue entry reranked at full precision). k and L as in search_batch."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim == 1:
+            q = q[None, :]
+        if q.shape[1] != self.dim:
+            raise ValueError("query dim %d != index dim %d" % (q.shape[1], self.dim))
+        nq = q.shape[0]
+        ks = np.ascontiguousarray(np.broadcast_to(np.asarray(k, dtype=np.uint32), (nq,)))
+        Ls = np.ascontiguousarray(np.broadcast_to(np.asarray(L, dtype=np.uint32), (nq,)))
+        kstride = int(ks.max()) if nq else 1
+        ids = np.zeros((nq, kstride), dtype=np.uint32)
+        ds = np.zeros((nq, kstride), dtype=np.float32)
+        cnt = np.zeros(nq, dtype=np.uint32)
+        cmps = np.zeros(nq, dtype=np.uint32)
+        hops = np.zeros(nq, dtype=np.uint32)
+        lim = (1 << 64) - 1 if io_limit is None else int(io_limit)
+        check(self.lib.dkm_disk_search_batch(self._h, _p(q), nq, _p(ks), _p(Ls), int(beam_width),
+                                             int(medoid), lim, kstride, _p(ids), _p(ds), _p(cnt),
+                                             _p(cmps), _p(hops)), self.lib)
+        return SearchResult(ids, ds, cnt, cmps, hops)
+
     def search(self, knn, strategy, query):
         """Single-query form of DiskANNIndex::search(Knn, strategy, query)."""
         r = self.search_batch(query, knn.k, knn.L, knn.beam_width, strategy)

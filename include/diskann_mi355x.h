@@ -165,6 +165,28 @@ dkm_status dkm_search_batch(dkm_index *ix, int strategy, const float *queries, i
                             int64_t k_stride, uint32_t *out_ids, float *out_dists,
                             uint32_t *out_counts, uint32_t *out_cmps, uint32_t *out_hops);
 
+/* Disk-index search (DiskIndexSearcher: DiskAccessor, diskann-disk/src/search/provider/
+ * disk_provider.rs:666-721, with RerankAndFilter, :323-388) over the index's rows as the
+ * disk layout's node records (vector + adjacency list) with the PQ codes in HBM:
+ *   - the single start point is the data row `medoid` (the graph header's medoid), with
+ *     its PQ distance;
+ *   - PQ distances use the transposed pivot table (quantizer_preprocess.rs:17-64,
+ *     TransposedTable::process_into: per pivot a fold of fma over the chunk's dimensions;
+ *     cosine and cosine-normalized indexes use the L2 table) summed in chunk order
+ *     (pq_dist_lookup, fixed_chunk_pq_table.rs:469-598);
+ *   - expand_beam loads at most io_limit beam nodes per query in total (search_io_limit);
+ *     beam nodes past the budget are taken (hops) but not expanded;
+ *   - every queue entry, the start point included, is reranked with its full-precision
+ *     distance and sorted (ties keep queue order; the reference's sort_unstable leaves
+ *     them unspecified); the first k are returned.
+ * The per-query loads (the reference's io count) are min(hops, io_limit). Needs the PQ
+ * schema and codes; host buffers as dkm_search_batch. */
+dkm_status dkm_disk_search_batch(dkm_index *ix, const float *queries, int64_t nq,
+                                 const uint32_t *k, const uint32_t *L, uint32_t beam_width,
+                                 uint32_t medoid, uint64_t io_limit, int64_t k_stride,
+                                 uint32_t *out_ids, float *out_dists, uint32_t *out_counts,
+                                 uint32_t *out_cmps, uint32_t *out_hops);
+
 /* Same, with every buffer in device memory (dkm_device_alloc); asynchronous on the
  * handle's stream. max_k / max_L bound the launch (max_k <= k_stride, max_L <= 4096); each
  * query's own k / L are read on the device and a query breaking Knn::new's rules (k, L >= 1,

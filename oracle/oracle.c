@@ -314,6 +314,33 @@ void dko_pq_lut(int metric, const float *pivots, int64_t K, int64_t dim,
     }
 }
 
+/* TransposedTable::process_into, diskann-quantization/src/product/tables/transposed/
+ * table.rs:285-313 with the compute kernels of pivots.rs:785-820 (a fold over the chunk's
+ * dimensions from a zero accumulator) and :915-948 (SquaredL2: d = p - q, acc = fma(d, d,
+ * acc); InnerProduct: acc = fma(-q, p, acc)). The disk index's quantizer_preprocess
+ * (diskann-disk/src/search/pq/quantizer_preprocess.rs:32-50) uses the L2 kernel for L2,
+ * cosine and cosine-normalized indexes. */
+void dko_pq_lut_transposed(int metric, const float *pivots, int64_t K, int64_t dim,
+                           const int64_t *offsets, int64_t M, const float *query, float *lut) {
+    const int ip = metric == DKO_INNER_PRODUCT;
+    for (int64_t m = 0; m < M; ++m) {
+        const int64_t s = offsets[m], e = offsets[m + 1];
+        for (int64_t c = 0; c < K; ++c) {
+            const float *p = pivots + c * dim;
+            float acc = 0.0f;
+            for (int64_t d = s; d < e; ++d) {
+                if (ip) {
+                    acc = fmaf(-query[d], p[d], acc);
+                } else {
+                    const float t = p[d] - query[d];
+                    acc = fmaf(t, t, acc);
+                }
+            }
+            lut[m * K + c] = acc;
+        }
+    }
+}
+
 /* pq_dist_lookup_single, fixed_chunk_pq_table.rs:79-95: sequential sum in chunk order. */
 float dko_pq_adc(const uint8_t *code, const float *lut, int64_t M, int64_t K) {
     float acc = 0.0f;
@@ -1295,6 +1322,101 @@ static void post_process(const dko_index *ix, int mode, const float *query, scra
     st->count = (uint32_t)c;
     free(ids);
     free(ds);
+}
+
+/* Disk-index search: DiskANNIndex::search_internal (diskann/src/graph/index.rs:1938-2005)
+ * with the DiskAccessor (diskann-disk/src/search/provider/disk_provider.rs:666-721) and
+ * RerankAndFilter (:323-388):
+ *   - the starting point is the graph header's medoid, here ix->starts[0] (a data row),
+ *     at its PQ distance;
+ *   - PQ distances: dko_pq_lut_transposed, then pq_dist_lookup (fixed_chunk_pq_table.rs:
+ *     469-598: 0 + LUT[m][code_m] in chunk order, as dko_pq_adc);
+ *   - expand_beam loads the first (io_limit - io_count) beam ids (ensure_loaded counts
+ *     them) and expands only those; the loop ends when no unvisited node remains or
+ *     terminate_early (io_count > io_limit);
+ *   - RerankAndFilter (AcceptAll): every queue entry (the first min(search_l, size), the
+ *     start point included) gets its full-precision distance (the cached one of a loaded
+ *     node is the same value), sorted by distance; sort_unstable leaves ties unspecified
+ *     and this restatement keeps queue order (stable sort); the first k are returned. */
+int dko_disk_search(const dko_index *ix, const float *query, int64_t k, int64_t L, int64_t bw,
+                    uint64_t io_limit, uint32_t *out_ids, float *out_d, dko_stats *st) {
+    int rc = dko_knn_validate(k, L, bw);
+    if (rc) return rc;
+    if (ix->M <= 0 || ix->nstart != 1) return DKO_ERR_ARG;
+    scratch s;
+    scratch_init(&s, ix, L, bw);
+    pqueue *q = &s.q;
+    q->size = 0;
+    q->cursor = 0;
+    set_clear(&s.vis);
+    const int lut_metric = ix->metric == DKO_INNER_PRODUCT ? DKO_INNER_PRODUCT : DKO_L2;
+    dko_pq_lut_transposed(lut_metric, ix->pivots, ix->K, ix->dim, ix->offsets, ix->M, query,
+                          s.lut);
+    uint32_t cmps = 0, hops = 0;
+    uint64_t ios = 0;
+    const uint32_t medoid = ix->starts[0];
+    set_insert(&s.vis, medoid);
+    q_insert(q, medoid, dko_pq_adc(ix->codes + (int64_t)medoid * ix->M, s.lut, ix->M, ix->K));
+    cmps += 1;
+    while (q_has_notvisited(q) && !(ios > io_limit)) {
+        int64_t nbeam = 0;
+        while (nbeam < bw) {
+            int64_t c = q_closest_notvisited(q);
+            if (c < 0) break;
+            s.beam[nbeam++] = q->id[c];
+        }
+        const uint64_t left = io_limit - ios;
+        const int64_t nload = (uint64_t)nbeam < left ? nbeam : (int64_t)left;
+        ios += (uint64_t)nload;
+        int64_t nn = 0;
+        for (int64_t b = 0; b < nload; ++b) {
+            const uint32_t node = s.beam[b];
+            const uint32_t *adj = ix->adj + (int64_t)node * ix->max_degree;
+            for (uint32_t j = 0; j < ix->deg[node]; ++j) {
+                const uint32_t nbr = adj[j];
+                if (!set_insert(&s.vis, nbr)) continue;
+                s.nb_id[nn] = nbr;
+                s.nb_d[nn] = dko_pq_adc(ix->codes + (int64_t)nbr * ix->M, s.lut, ix->M, ix->K);
+                nn++;
+            }
+        }
+        for (int64_t j = 0; j < nn; ++j) q_insert(q, s.nb_id[j], s.nb_d[j]);
+        cmps += (uint32_t)nn;
+        hops += (uint32_t)nbeam;
+    }
+    st->cmps = cmps;
+    st->hops = hops;
+    /* RerankAndFilter */
+    const int64_t lim = q->search_l < q->size ? q->search_l : q->size;
+    uint32_t *ids = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(lim + 1));
+    float *ds = (float *)malloc(sizeof(float) * (size_t)(lim + 1));
+    for (int64_t i = 0; i < lim; ++i) {
+        ids[i] = q->id[i];
+        ds[i] = dko_distance(ix->metric, query, ix->vectors + (int64_t)q->id[i] * ix->vstride,
+                             ix->dim);
+    }
+    for (int64_t i = 1; i < lim; ++i) {
+        uint32_t id = ids[i];
+        float d = ds[i];
+        int64_t j = i - 1;
+        while (j >= 0 && ds[j] > d) {
+            ids[j + 1] = ids[j];
+            ds[j + 1] = ds[j];
+            j--;
+        }
+        ids[j + 1] = id;
+        ds[j + 1] = d;
+    }
+    const int64_t c = lim < k ? lim : k;
+    for (int64_t i = 0; i < c; ++i) {
+        out_ids[i] = ids[i];
+        out_d[i] = ds[i];
+    }
+    st->count = (uint32_t)c;
+    free(ids);
+    free(ds);
+    scratch_free(&s);
+    return 0;
 }
 
 int dko_search(const dko_index *ix, int mode, const float *query, int64_t k, int64_t L,

@@ -224,6 +224,15 @@ typedef struct {
     int64_t pq_M;
 } dko_build_cfg;
 
+/* TransposedTable::process_into LUT (the disk index's PQ table): lut[m*K + c] is a fold
+ * of fma over chunk m's dimensions (L2: (p - q)^2, inner product: -q * p). */
+void dko_pq_lut_transposed(int metric, const float *pivots, int64_t K, int64_t dim,
+                           const int64_t *offsets, int64_t M, const float *query, float *lut);
+/* Disk-index search (DiskAccessor + RerankAndFilter) from the data row ix->starts[0]
+ * (nstart must be 1) with at most io_limit expanded nodes. */
+int dko_disk_search(const dko_index *ix, const float *query, int64_t k, int64_t L, int64_t bw,
+                    uint64_t io_limit, uint32_t *out_ids, float *out_d, dko_stats *st);
+
 /* occlude_list on a pool sorted by distance (diskann/src/graph/index.rs:2570-2782).
  * pool_ids/pool_d: candidates; `self_id` excluded. Distances between candidates are
  * computed with the metric on `vectors`. Returns the number of neighbors written. */

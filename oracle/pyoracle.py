@@ -74,6 +74,9 @@ def _setup(L):
     L.dko_square_norm.restype = F
     L.dko_chunk_offsets.argtypes = [I64, I64, P]
     L.dko_pq_lut.argtypes = [C.c_int, P, I64, I64, P, I64, P, P]
+    L.dko_pq_lut_transposed.argtypes = [C.c_int, P, I64, I64, P, I64, P, P]
+    L.dko_disk_search.argtypes = [C.POINTER(Index), P, I64, I64, I64, C.c_uint64, P, P,
+                                  C.POINTER(Stats)]
     L.dko_pq_adc.argtypes = [P, P, I64, I64]
     L.dko_pq_adc.restype = F
     L.dko_sq_train.argtypes = [P, I64, I64, I64, C.c_double, P, P, P]
@@ -184,6 +187,17 @@ def pq_lut(metric, pivots, offsets, query):
     lut = np.zeros(M * K, dtype=np.float32)
     lib().dko_pq_lut(metric, _p(pivots), K, dim, _p(offsets), M, _p(query), _p(lut))
     return lut.reshape(M, K)
+
+
+def pq_lut_transposed(metric, pivots, offsets, query):
+    pivots = np.ascontiguousarray(pivots, dtype=np.float32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    query = np.ascontiguousarray(query, dtype=np.float32)
+    M = offsets.shape[0] - 1
+    lut = np.zeros(M * pivots.shape[0], dtype=np.float32)
+    lib().dko_pq_lut_transposed(metric, _p(pivots), pivots.shape[0], pivots.shape[1],
+                                _p(offsets), M, _p(query), _p(lut))
+    return lut
 
 
 def pq_adc(code, lut):
@@ -479,6 +493,20 @@ class OracleIndex:
         st = Stats()
         rc = lib().dko_search(C.byref(self.s), mode, _p(query), k, L, bw, _p(ids), _p(ds),
                               C.byref(st))
+        if rc:
+            raise ValueError("knn validation error %d" % rc)
+        n = st.count
+        return ids[:n], ds[:n], st.cmps, st.hops
+
+    def search_disk(self, query, k, L, bw=1, io_limit=None):
+        """Disk-index search from the single start row (the medoid)."""
+        query = np.ascontiguousarray(query, dtype=np.float32)
+        ids = np.zeros(k, dtype=np.uint32)
+        ds = np.zeros(k, dtype=np.float32)
+        st = Stats()
+        lim = (1 << 64) - 1 if io_limit is None else int(io_limit)
+        rc = lib().dko_disk_search(C.byref(self.s), _p(query), k, L, bw, lim, _p(ids), _p(ds),
+                                   C.byref(st))
         if rc:
             raise ValueError("knn validation error %d" % rc)
         n = st.count

@@ -22,7 +22,7 @@ EXPORTS = [
     "dkm_set_graph", "dkm_get_graph", "dkm_set_pq", "dkm_set_codes", "dkm_encode_index",
     "dkm_get_codes", "dkm_search_batch", "dkm_disk_search_batch", "dkm_search_batch_device", "dkm_index_synchronize",
     "dkm_profile", "dkm_profile_read", "dkm_set_tuning", "dkm_build", "dkm_prune_batch", "dkm_prune_range", "dkm_set_build_hybrid", "dkm_medoid",
-    "dkm_bruteforce", "dkm_search_batch_device_stats", "dkm_autotune",
+    "dkm_bruteforce", "dkm_closest_centers", "dkm_search_batch_device_stats", "dkm_autotune",
     "dkm_search_graph_create", "dkm_search_graph_launch", "dkm_search_graph_destroy",
     "dkm_search_status", "dkm_search_graph_status", "dkm_set_visited_capacity",
     "dkm_search_batch_device_noalloc",
@@ -130,6 +130,7 @@ def _declare(L):
         "dkm_set_build_hybrid": [P, I64],
         "dkm_medoid": [P, P, C.POINTER(C.c_int64)],
         "dkm_bruteforce": [P, P, I64, I64, P, P],
+        "dkm_closest_centers": [P, I64, I64, P, I64, I64, P],
         "dkm_autotune": [P, I, P, I64, U32, U32, U32, C.POINTER(C.c_int64), C.POINTER(C.c_int)],
         "dkm_server_start": [P, I, U32, U32, U32, I64, I64, C.POINTER(C.c_void_p)],
         "dkm_server_slots": [P, C.POINTER(C.c_int64)],

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdiskann_mi355x.so")
 SOURCES = ["api.hip", "search.hip", "pq.hip", "build.hip", "exact.hip", "storage.hip", "kmeans.hip",
-           "prescore.hip", "sq.hip", "sph.hip", "minmax.hip"]
+           "prescore.hip", "sq.hip", "sph.hip", "minmax.hip", "partition.hip"]
 ARCH = os.environ.get("DKM_OFFLOAD_ARCH", "gfx950")
 
 

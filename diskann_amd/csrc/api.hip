@@ -27,6 +27,7 @@
 #include "dkm_exact.h"
 #include "dkm_kernels.h"
 #include "dkm_kmeans.h"
+#include "dkm_partition.h"
 #include "dkm_prescore.h"
 #include "dkm_search.h"
 
@@ -1886,6 +1887,32 @@ dkm_status dkm_medoid(dkm_index *ix, float *out_vector, int64_t *out_row) {
     *out_row = (int64_t)(best & 0xFFFFFFFFull);
     HIP_TRY(hipMemcpy(out_vector, ix->d_vec + *out_row * ix->vstride, sizeof(float) * ix->dim,
                       hipMemcpyDeviceToHost));
+    return DKM_OK;
+}
+
+dkm_status dkm_closest_centers(const float *data, int64_t n, int64_t dim, const float *centers,
+                               int64_t K, int64_t k, uint32_t *out) {
+    if (n < 0 || dim < 1 || K < 1) return fail(DKM_ERR_INVALID_ARG, "bad sizes");
+    if (k < 1 || k > K || k > dkm::kMaxClosest)
+        return fail(DKM_ERR_INDEX, "k (%lld) must be in [1, min(num_centers, %d)]", (long long)k,
+                    dkm::kMaxClosest);
+    if (n > 0 && (!data || !centers || !out)) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return DKM_OK;
+    DevBuf<float> dx, dc, dn;
+    DevBuf<uint32_t> dout;
+    const int64_t chunk = std::min<int64_t>(n, std::max<int64_t>(1, (int64_t(1) << 30) / (4 * dim)));
+    HIP_TRY(dx.ensure((size_t)chunk * dim));
+    HIP_TRY(dc.ensure((size_t)K * dim));
+    HIP_TRY(dn.ensure(K));
+    HIP_TRY(dout.ensure((size_t)chunk * k));
+    HIP_TRY(hipMemcpy(dc.p, centers, sizeof(float) * K * dim, hipMemcpyHostToDevice));
+    for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+        const int64_t m = std::min(chunk, n - r0);
+        HIP_TRY(hipMemcpy(dx.p, data + r0 * dim, sizeof(float) * m * dim, hipMemcpyHostToDevice));
+        HIP_TRY(dkm::launch_closest_centers(dx.p, m, (int)dim, dc.p, (int)K, (int)k, dn.p, dout.p,
+                                            nullptr));
+        HIP_TRY(hipMemcpy(out + r0 * k, dout.p, sizeof(uint32_t) * m * k, hipMemcpyDeviceToHost));
+    }
     return DKM_OK;
 }
 

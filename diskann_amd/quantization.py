@@ -193,3 +193,17 @@ def kmeans_assign(data, centres, precision=F32):
     check(lib.dkm_kmeans_assign_device(dx.ptr, n, dim, dc.ptr, c.shape[0], int(precision), da.ptr,
                                        dd.ptr), lib)
     return da.get(np.uint32, n), dd.get(np.float32, n)
+
+
+def closest_centers(data, centers, k=1):
+    """compute_closest_centers of the partitioned build: for each row of `data`, the ids of
+    its k closest `centers` (ascending squared L2 in the |x|^2 + |c|^2 - 2<x,c> form)."""
+    lib = _native.load()
+    x = np.ascontiguousarray(data, dtype=np.float32)
+    c = np.ascontiguousarray(centers, dtype=np.float32)
+    if x.ndim != 2 or c.ndim != 2 or x.shape[1] != c.shape[1]:
+        raise ValueError("data and centers must be matrices of the same dimension")
+    out = np.zeros((x.shape[0], int(k)), dtype=np.uint32)
+    check(lib.dkm_closest_centers(_p(x), x.shape[0], x.shape[1], _p(c), c.shape[0], int(k),
+                                  _p(out)), lib)
+    return out

@@ -402,6 +402,17 @@ dkm_status dkm_minmax_query_distance_batch(const dkm_minmax *q, int metric, cons
                                            int64_t nq, const uint8_t *codes, int64_t n,
                                            const uint32_t *ids, int64_t m, float *out);
 
+/* compute_closest_centers (diskann-disk/src/utils/math_util.rs:246-385) of the partitioned
+ * disk-index build: for each of the n rows of data (n x dim), the k closest of the K
+ * centres (K x dim) by d = (|x|^2 + |c|^2) + (-2 <x, c>) with FastL2NormSquared norms
+ * (diskann-vector/src/norm.rs:45-56); out[i*k + t] is the t-th closest (ascending d; equal d:
+ * lower centre index first; k = 1 is the first index with the strictly smallest d, as
+ * compute_closest_centers_in_block's loop). The reference's dot products come from faer's
+ * sgemm, whose summation order is not pinned; here they are fma chains in dimension order.
+ * 1 <= k <= min(K, 16). Host buffers. */
+dkm_status dkm_closest_centers(const float *data, int64_t n, int64_t dim, const float *centers,
+                               int64_t K, int64_t k, uint32_t *out);
+
 /* Hybrid prune for dkm_build (common::Hybrid, diskann-providers/src/model/graph/provider/
  * async_/common.rs:385-405; HybridPruneAccessor, inmem/product.rs:143-230): each prune
  * keeps the first max_fp_vecs_per_prune ids of its fill order (robust_prune_with:

@@ -341,6 +341,55 @@ void dko_pq_lut_transposed(int metric, const float *pivots, int64_t K, int64_t d
     }
 }
 
+/* compute_closest_centers, diskann-disk/src/utils/math_util.rs:246-385, with
+ * compute_closest_centers_in_block :133-238: norms by FastL2NormSquared (the inner
+ * product's simd order, diskann-vector/src/norm.rs:45-56), the three sgemm steps
+ * d = (|x|^2 + |c|^2) + (-2 <x, c>), then k = 1: the first index with the strictly smallest
+ * d (the in-block loop from f32::MAX); k > 1: ascending d by the PivotContainer heap, whose
+ * order among equal d is unspecified (here: lower index first). The dot product inside
+ * faer's sgemm (third-party, its blocking order is not pinned) is an fma chain in dimension
+ * order here. */
+void dko_closest_centers(const float *x, int64_t n, int64_t dim, const float *c, int64_t K,
+                         int64_t k, uint32_t *out) {
+    float *cn = (float *)malloc(sizeof(float) * (size_t)K);
+    float *d = (float *)malloc(sizeof(float) * (size_t)K);
+    int64_t *ord = (int64_t *)malloc(sizeof(int64_t) * (size_t)K);
+    for (int64_t j = 0; j < K; ++j) cn[j] = simd_4x1(1, c + j * dim, c + j * dim, dim);
+    for (int64_t i = 0; i < n; ++i) {
+        const float *xi = x + i * dim;
+        const float xn = simd_4x1(1, xi, xi, dim);
+        for (int64_t j = 0; j < K; ++j) {
+            float dot = 0.0f;
+            for (int64_t e = 0; e < dim; ++e) dot = fmaf(xi[e], c[j * dim + e], dot);
+            d[j] = (xn + cn[j]) + (-2.0f * dot);
+        }
+        if (k == 1) {
+            float best = FLT_MAX;
+            int64_t bi = 0;
+            for (int64_t j = 0; j < K; ++j)
+                if (d[j] < best) {
+                    best = d[j];
+                    bi = j;
+                }
+            out[i] = (uint32_t)bi;
+            continue;
+        }
+        /* stable insertion sort of the indices by d */
+        for (int64_t j = 0; j < K; ++j) {
+            int64_t t = j;
+            while (t > 0 && d[ord[t - 1]] > d[j]) {
+                ord[t] = ord[t - 1];
+                t--;
+            }
+            ord[t] = j;
+        }
+        for (int64_t t = 0; t < k; ++t) out[i * k + t] = (uint32_t)ord[t];
+    }
+    free(cn);
+    free(d);
+    free(ord);
+}
+
 /* pq_dist_lookup_single, fixed_chunk_pq_table.rs:79-95: sequential sum in chunk order. */
 float dko_pq_adc(const uint8_t *code, const float *lut, int64_t M, int64_t K) {
     float acc = 0.0f;

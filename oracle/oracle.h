@@ -224,6 +224,10 @@ typedef struct {
     int64_t pq_M;
 } dko_build_cfg;
 
+/* compute_closest_centers (partitioned build): out[i*k + t] = the t-th closest centre of
+ * row i by (|x|^2 + |c|^2) + (-2 <x, c>); ties by lower index. */
+void dko_closest_centers(const float *x, int64_t n, int64_t dim, const float *c, int64_t K,
+                         int64_t k, uint32_t *out);
 /* TransposedTable::process_into LUT (the disk index's PQ table): lut[m*K + c] is a fold
  * of fma over chunk m's dimensions (L2: (p - q)^2, inner product: -q * p). */
 void dko_pq_lut_transposed(int metric, const float *pivots, int64_t K, int64_t dim,

@@ -75,6 +75,7 @@ def _setup(L):
     L.dko_chunk_offsets.argtypes = [I64, I64, P]
     L.dko_pq_lut.argtypes = [C.c_int, P, I64, I64, P, I64, P, P]
     L.dko_pq_lut_transposed.argtypes = [C.c_int, P, I64, I64, P, I64, P, P]
+    L.dko_closest_centers.argtypes = [P, I64, I64, P, I64, I64, P]
     L.dko_disk_search.argtypes = [C.POINTER(Index), P, I64, I64, I64, C.c_uint64, P, P,
                                   C.POINTER(Stats)]
     L.dko_pq_adc.argtypes = [P, P, I64, I64]
@@ -187,6 +188,14 @@ def pq_lut(metric, pivots, offsets, query):
     lut = np.zeros(M * K, dtype=np.float32)
     lib().dko_pq_lut(metric, _p(pivots), K, dim, _p(offsets), M, _p(query), _p(lut))
     return lut.reshape(M, K)
+
+
+def closest_centers(x, c, k=1):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    c = np.ascontiguousarray(c, dtype=np.float32)
+    out = np.zeros((x.shape[0], k), dtype=np.uint32)
+    lib().dko_closest_centers(_p(x), x.shape[0], x.shape[1], _p(c), c.shape[0], k, _p(out))
+    return out
 
 
 def pq_lut_transposed(metric, pivots, offsets, query):

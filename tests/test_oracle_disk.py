@@ -131,3 +131,32 @@ def test_disk_search_io_budget():
         assert cm_b <= free[2] and hp_b >= b
     with pytest.raises(ValueError):
         disk.search_disk(qs[0], 10, 5, 1)             # L < k
+
+
+def test_closest_centers_exact_on_integer_data():
+    # small integers: every product, sum and norm is exact in f32, so the form
+    # (|x|^2 + |c|^2) - 2<x,c> is the exact squared distance and the result is the
+    # brute-force order (ties by lower centre index), for k = 1 and k > 1
+    rng = np.random.default_rng(8)
+    x = rng.integers(-6, 7, (500, 20)).astype(np.float32)
+    c = rng.integers(-6, 7, (40, 20)).astype(np.float32)
+    c[7] = c[3]                      # duplicate centres: the lower index first
+    exact = ((x[:, None, :].astype(np.int64) - c[None, :, :].astype(np.int64)) ** 2).sum(-1)
+    order = np.argsort(exact, axis=1, kind="stable")
+    assert np.array_equal(po.closest_centers(x, c, 1)[:, 0], order[:, 0])
+    assert np.array_equal(po.closest_centers(x, c, 5), order[:, :5])
+
+
+def test_closest_centers_float_data_within_rounding():
+    # float data: the k = 1 answer is the FP64 argmin wherever the FP64 margin to the
+    # runner-up exceeds the f32 error of the three-term form
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((400, 64)).astype(np.float32)
+    c = rng.standard_normal((32, 64)).astype(np.float32)
+    got = po.closest_centers(x, c, 1)[:, 0]
+    d64 = ((x[:, None, :].astype(np.float64) - c[None, :, :]) ** 2).sum(-1)
+    srt = np.sort(d64, axis=1)
+    bound = 1e-5 * (np.abs(x).sum(1) ** 2 + (np.abs(c).sum(1) ** 2).max())
+    clear = (srt[:, 1] - srt[:, 0]) > bound
+    assert clear.mean() > 0.9
+    assert np.array_equal(got[clear], np.argmin(d64, axis=1)[clear])

@@ -105,6 +105,7 @@ struct dkm_index {
     // PQ
     int64_t M = 0, K = 0;
     float *d_pivots = nullptr;
+    float *d_pivots_t = nullptr;  // chunk-major copy (uniform 2- or 4-dim chunks): (m * 256 + c) * w
     int32_t *d_offsets = nullptr;
     uint8_t *d_codes = nullptr;
     bool codes_set = false;
@@ -458,8 +459,8 @@ void dkm_index_destroy(dkm_index *ix) {
         (void)hipEventDestroy(ev.first);
         (void)hipEventDestroy(ev.second);
     }
-    void *ptrs[] = {ix->d_vec, ix->d_graph, ix->d_starts, ix->d_pivots, ix->d_offsets,
-                    ix->d_codes, ix->d_spill};
+    void *ptrs[] = {ix->d_vec, ix->d_graph, ix->d_starts, ix->d_pivots, ix->d_pivots_t,
+                    ix->d_offsets, ix->d_codes, ix->d_spill};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     ix->s_q.release();
@@ -575,9 +576,11 @@ dkm_status dkm_set_pq(dkm_index *ix, const float *pivots, int64_t K, const int64
     std::lock_guard<std::mutex> g(ix->mu);
     HIP_TRY(hipSetDevice(ix->device));
     if (ix->d_pivots) (void)hipFree(ix->d_pivots);
+    if (ix->d_pivots_t) (void)hipFree(ix->d_pivots_t);
     if (ix->d_offsets) (void)hipFree(ix->d_offsets);
     if (ix->d_codes) (void)hipFree(ix->d_codes);
     ix->d_pivots = nullptr;
+    ix->d_pivots_t = nullptr;
     ix->d_offsets = nullptr;
     ix->d_codes = nullptr;
     ix->M = M;
@@ -598,6 +601,18 @@ dkm_status dkm_set_pq(dkm_index *ix, const float *pivots, int64_t K, const int64
     HIP_TRY(hipStreamSynchronize(ix->stream));  // cleared before any copy on another stream
     HIP_TRY(hipMemcpy(ix->d_pivots, pivots, sizeof(float) * K * ix->dim, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ix->d_offsets, o32.data(), sizeof(int32_t) * (M + 1), hipMemcpyHostToDevice));
+    if (ix->pq_len) {
+        // chunk-major copy for the traversal's direct ADC: the lanes of one load read one
+        // chunk's 256 x w block instead of 256 rows of the pivot table
+        const int64_t w = ix->pq_len;
+        std::vector<float> t((size_t)K * ix->dim);
+        for (int64_t m = 0; m < M; ++m)
+            for (int64_t c = 0; c < K; ++c)
+                for (int64_t d = 0; d < w; ++d)
+                    t[(size_t)(m * K + c) * w + d] = pivots[c * ix->dim + m * w + d];
+        HIP_TRY(hipMalloc(&ix->d_pivots_t, sizeof(float) * K * ix->dim));
+        HIP_TRY(hipMemcpy(ix->d_pivots_t, t.data(), sizeof(float) * K * ix->dim, hipMemcpyHostToDevice));
+    }
     ix->codes_set = false;
     return DKM_OK;
 }
@@ -1511,6 +1526,7 @@ static dkm_status search_params(dkm_index *ix, int strategy, const float *d_quer
     p.code_stride = (int)ix->M;
     p.M = (int)ix->M;
     p.pivots = ix->d_pivots;
+    p.pivots_t = ix->d_pivots_t;
     p.offsets = ix->d_offsets;
     set_sq_params(ix, sq, p, sph);
     if (sph && own) {
@@ -2654,6 +2670,7 @@ dkm_status dkm_server_start(dkm_index *ix, int strategy, uint32_t max_k, uint32_
     p.code_stride = (int)ix->M;
     p.M = (int)ix->M;
     p.pivots = ix->d_pivots;
+    p.pivots_t = ix->d_pivots_t;
     p.offsets = ix->d_offsets;
     set_sq_params(ix, sq, p, sph);
     p.qstride = ix->dim;

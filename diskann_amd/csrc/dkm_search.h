@@ -32,6 +32,8 @@ struct SearchParams {
     int code_stride;
     int M;
     const float *pivots;   // 256 x dim
+    const float *pivots_t; // uniform chunks of pq_len dims: chunk-major copy, entry (m, c)
+                           // at (m * 256 + c) * pq_len
     const int32_t *offsets;
     const float *queries;
     int64_t qstride;

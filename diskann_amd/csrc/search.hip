@@ -618,7 +618,11 @@ __device__ __forceinline__ float transposed_term(const float *q, const float *pv
 // L2) with exactly the arithmetic that fills the table, so the result is bit-identical to
 // the table path while the query keeps no 32-48 KiB table in LDS. A group of G lanes
 // shares one candidate: each lane computes M/G consecutive chunk terms, then the running
-// sum is handed from lane to lane so the terms are added in chunk order.
+// sum is handed from lane to lane so the terms are added in chunk order. The pivots are
+// read from the chunk-major copy: the lanes of one load instruction work on the same chunk
+// (G = 1) or on G chunks, so they touch at most 16-32 cache lines of that chunk's
+// 256 x LEN block rather than one line per lane; these scattered loads are what bounds
+// the traversal's distance step.
 template <typename T>
 __device__ inline T load_as(const uint8_t *p) { return *(const T *)p; }
 
@@ -626,15 +630,35 @@ template <bool IP, int LEN, int M, int G, bool TL>
 __device__ inline void adc_direct_group(const SearchParams &p, Lds &s, const uint32_t *ids,
                                         float *out, int n) {
     constexpr int MPER = M / G;
-    const int lane = lane_id();
+    uint32_t opaque = 0;
+    asm volatile("" : "+v"(opaque));
+    const int lane = lane_id() + (int)opaque;
     const int g = lane % G;
     for (int c0 = 0; c0 < n; c0 += kWave / G) {
         const int c = c0 + lane / G;
         const bool ok = c < n;
         const uint32_t id = ids[ok ? c : 0];
         const uint8_t *row = p.codes + (size_t)id * p.code_stride + g * MPER;
+        // widest loads the row part's alignment allows (code rows are M bytes at stride M,
+        // M a multiple of 16): each load instruction touches one line per lane
         uint8_t code[MPER];
-        if constexpr (MPER % 4 == 0) {
+        if constexpr (MPER % 16 == 0) {
+#pragma unroll
+            for (int q = 0; q < MPER; q += 16) {
+                const uint4 v = load_as<uint4>(row + q);
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int b = 0; b < 16; ++b) code[q + b] = (uint8_t)(w[b >> 2] >> (8 * (b & 3)));
+            }
+        } else if constexpr (MPER % 8 == 0) {
+#pragma unroll
+            for (int q = 0; q < MPER; q += 8) {
+                const uint2 v = load_as<uint2>(row + q);
+                const uint32_t w[2] = {v.x, v.y};
+#pragma unroll
+                for (int b = 0; b < 8; ++b) code[q + b] = (uint8_t)(w[b >> 2] >> (8 * (b & 3)));
+            }
+        } else if constexpr (MPER % 4 == 0) {
 #pragma unroll
             for (int q = 0; q < MPER; q += 4) {
                 uint32_t w = load_as<uint32_t>(row + q);
@@ -656,7 +680,7 @@ __device__ inline void adc_direct_group(const SearchParams &p, Lds &s, const uin
 #pragma unroll
         for (int q = 0; q < MPER; ++q) {
             const int m = g * MPER + q;
-            const float *pv = p.pivots + (size_t)code[q] * p.dim + m * LEN;
+            const float *pv = p.pivots_t + ((size_t)m * 256 + code[q]) * LEN;
             float pc[LEN];
             if constexpr (LEN == 4) {
                 float4 t4 = *(const float4 *)pv;
@@ -1207,9 +1231,15 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
     }
 }
 
-// At most 128 VGPRs: four waves (queries) per SIMD.
+// Waves (queries) per SIMD the register budget is set for: five (at most 96 VGPRs) for
+// direct-ADC PQ traversal, whose hops wait on L2 pivot loads and gain from more resident
+// queries (with 512 LDS visited slots the LDS holds 20 such queries per CU), four (128
+// VGPRs) for the other modes.
+template <int MODE>
+constexpr int kSearchWavesPerSimd = MODE == 2 ? 5 : 4;
+
 template <int METRIC, int MODE, int DIM, int PQLEN, int RM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSearchWavesPerSimd<MODE>, 8))) void
 beam_search_kernel(SearchParams p) {
     extern __shared__ __align__(16) char smem[];
     if ((int)blockIdx.x >= p.nq) return;

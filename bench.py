@@ -34,10 +34,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The serving loop runs its eight launch streams concurrently only when each has a
-# hardware queue of its own (HIP's default of 4 per process makes them share two); this
-# must be set before the HIP runtime starts.
-os.environ["GPU_MAX_HW_QUEUES"] = "12"
+# The serving loop runs its sixteen launch streams concurrently only when each has a
+# hardware queue of its own (HIP's default of 4 per process makes them share); with the
+# index stream and the null stream that is 18 queues. This must be set before the HIP
+# runtime starts.
+os.environ["GPU_MAX_HW_QUEUES"] = "20"
 
 HBM_PEAK_GBS = 8000.0
 

@@ -975,13 +975,17 @@ def main():
     # HBM bytes per launch from the rocprofv3 counter runs (profiles/), when they were
     # taken at the settings chosen here
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_sq_1M.json" if SQ else
-                      "traffic_sph_1M.json" if SPH else "traffic_hybrid_1M.json")
+    traffic_source = None
+    tname = ("traffic_sq_1M.json" if SQ else "traffic_sph_1M.json" if SPH else
+             "traffic_hybrid_1M.json")
+    tf = os.path.join(ROOT, "profiles", tname)
     if os.path.exists(tf):
         with open(tf) as f:
             tr = json.load(f)
         if (tr.get("L"), tr.get("beam_width"), tr.get("visited_lds_slots")) == (L_star, BW, HSLOTS):
             traffic = tr.get("hbm_bytes_per_launch")
+            traffic_source = ("profiles/%s: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this "
+                              "workload at the same L, beam width and visited slots" % tname)
 
     workload = ("batched scalar-quantized search: 1M x 128 f32 GMM(256, sigma 0.5), 8-bit "
                 "scalar quantizer (2 std), Vamana R=64 alpha=1.2, 10k queries/launch, k=10, "

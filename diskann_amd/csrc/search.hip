@@ -1232,14 +1232,15 @@ __device__ inline void search_query(const SearchParams &p, Lds &s, const QueryJo
 }
 
 // Waves (queries) per SIMD the register budget is set for: five (at most 96 VGPRs) for
-// direct-ADC PQ traversal, whose hops wait on L2 pivot loads and gain from more resident
-// queries (with 512 LDS visited slots the LDS holds 20 such queries per CU), four (128
-// VGPRs) for the other modes.
-template <int MODE>
-constexpr int kSearchWavesPerSimd = MODE == 2 ? 5 : 4;
+// direct-ADC PQ traversal over 2-dimensional chunks, whose hops wait on L2 pivot loads
+// and gain from more resident queries (with 512 LDS visited slots the LDS holds 20 such
+// queries per CU; 10M x 96, PQ 48: 6.7 -> 5.8 ms per 10k launch on a 2M proxy), four (128
+// VGPRs) for the other modes (PQ 32 x 4 at L 512 was 3 % slower with five).
+template <int MODE, int PQLEN>
+constexpr int kSearchWavesPerSimd = MODE == 2 && PQLEN == 2 ? 5 : 4;
 
 template <int METRIC, int MODE, int DIM, int PQLEN, int RM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSearchWavesPerSimd<MODE>, 8))) void
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kSearchWavesPerSimd<MODE, PQLEN>, 8))) void
 beam_search_kernel(SearchParams p) {
     extern __shared__ __align__(16) char smem[];
     if ((int)blockIdx.x >= p.nq) return;

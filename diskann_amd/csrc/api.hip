@@ -119,6 +119,7 @@ struct dkm_index {
     int tune_adc_mode = 1;              // PQ distances: 0 = LDS table, 1 = direct
     int pq_len = 0;                     // uniform PQ chunk width usable by the fast path
     int64_t build_max_fp = -1;          // Hybrid prune: full-precision ids per prune (-1 = all)
+    bool build_sq = false;              // SQ Quantized build strategy (dkm_set_build_sq)
     // 8-bit scalar quantization; when sq_active the quantized strategies traverse with it
     bool sq_active = false, sq_codes_set = false;
     float sq_scale = 0.0f, sq_mean_norm = 0.0f, sq_ssn = 0.0f;
@@ -2092,6 +2093,30 @@ dkm_status dkm_set_build_hybrid(dkm_index *ix, int64_t max_fp_vecs_per_prune) {
     return DKM_OK;
 }
 
+dkm_status dkm_set_build_sq(dkm_index *ix, int enable) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(ix->mu);
+    ix->build_sq = enable != 0;
+    return DKM_OK;
+}
+
+// The build strategy's checks and its SQ distance parameters (prunes and the insert search).
+static dkm_status build_strategy(const dkm_index *ix, dkm::BuildParams &bp) {
+    if (ix->build_max_fp >= 0 && (!ix->d_codes || !ix->codes_set))
+        return fail(DKM_ERR_NOT_READY, "the Hybrid prune needs the PQ codes of every row");
+    if (ix->build_sq && ix->build_max_fp >= 0)
+        return fail(DKM_ERR_INVALID_ARG, "the SQ build and the Hybrid prune exclude each other");
+    if (ix->build_sq && (!ix->sq_active || !ix->sq_codes_set))
+        return fail(DKM_ERR_NOT_READY, "the SQ build needs the scalar quantizer and the codes of every row");
+    bp.sq = ix->build_sq;
+    bp.sq_codes = ix->d_sq_codes.p;
+    bp.sq_comp = ix->d_sq_comp.p;
+    bp.sq_cstride = (int)ix->sq_cstride;
+    bp.sq_bss = dkm::sq_bss(ix->sq_scale);
+    bp.sq_ssn = ix->sq_ssn;
+    return DKM_OK;
+}
+
 dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batch,
                      int64_t intra_batch, uint32_t L_build, float alpha, int64_t max_occlusion,
                      int saturate) {
@@ -2102,8 +2127,11 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
     if (max_occlusion < 1 || max_occlusion > 4096) return fail(DKM_ERR_INVALID_ARG, "bad max_occlusion");
     if (ix->R > 256) return fail(DKM_ERR_INVALID_ARG, "pruned degree above 256 is not supported");
     if (!ix->starts_set) return fail(DKM_ERR_NOT_READY, "start points not set");
-    if (ix->build_max_fp >= 0 && (!ix->d_codes || !ix->codes_set))
-        return fail(DKM_ERR_NOT_READY, "the Hybrid prune needs the PQ codes of every row");
+    {
+        dkm::BuildParams probe{};
+        dkm_status st = build_strategy(ix, probe);
+        if (st) return st;
+    }
     for (int64_t i = 0; i < n; ++i)
         if (ids[i] >= ix->max_points) return fail(DKM_ERR_INDEX, "id %u is not a data row", ids[i]);
     if (n == 0) return DKM_OK;
@@ -2173,6 +2201,7 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
     bp.pivots = ix->d_pivots;
     bp.pq_off = ix->d_offsets;
     bp.pq_M = (int)ix->M;
+    (void)build_strategy(ix, bp);  // checked on entry
     std::vector<uint32_t> h_pend, h_cnt;
     for (int64_t b0 = 0; b0 < n; b0 += B) {
         const int nb = (int)std::min<int64_t>(B, n - b0);
@@ -2187,6 +2216,8 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
         sp.metric = ix->metric;
         sp.pq = 0;
         sp.rerank = 0;
+        // the SQ build's insert search is the Quantized search (SQ traversal, no rerank)
+        if (ix->build_sq) set_sq_params(ix, true, sp);
         sp.dim = (int)ix->dim;
         sp.vstride = ix->vstride;
         sp.vectors = ix->d_vec;
@@ -2294,8 +2325,11 @@ dkm_status prune_setup(dkm_index *ix, float alpha, int64_t max_occlusion, int sa
     if (ix->R > 256) return fail(DKM_ERR_INVALID_ARG, "pruned degree above 256 is not supported");
     if (list_stride < 1 || list_stride > 4096)
         return fail(DKM_ERR_INVALID_ARG, "candidate lists must hold 1 to 4096 entries");
-    if (ix->build_max_fp >= 0 && (!ix->d_codes || !ix->codes_set))
-        return fail(DKM_ERR_NOT_READY, "the Hybrid prune needs the PQ codes of every row");
+    {
+        dkm::BuildParams probe{};
+        dkm_status st = build_strategy(ix, probe);
+        if (st) return st;
+    }
     int64_t cap = 2;
     while (cap < list_stride) cap <<= 1;
     bp = dkm::BuildParams{};
@@ -2317,6 +2351,7 @@ dkm_status prune_setup(dkm_index *ix, float alpha, int64_t max_occlusion, int sa
     bp.pivots = ix->d_pivots;
     bp.pq_off = ix->d_offsets;
     bp.pq_M = (int)ix->M;
+    (void)build_strategy(ix, bp);  // checked on entry
     if (dkm::build_prune_lds_bytes(bp) > 160 * 1024)
         return fail(DKM_ERR_CAPACITY, "a candidate pool of %lld ids does not fit in LDS",
                     (long long)list_stride);

@@ -17,6 +17,7 @@
 
 #include "dkm_build.h"
 #include "dkm_device.h"
+#include "dkm_sq.h"
 
 namespace dkm {
 
@@ -93,6 +94,33 @@ __device__ inline float metric_distance(int metric, const float *a, const float 
     if (metric == METRIC_IP) return half_distance<METRIC_IP>(a, b, dim, hp);
     if (metric == METRIC_COSNORM) return half_distance<METRIC_COSNORM>(a, b, dim, hp);
     return half_distance<METRIC_COSINE>(a, b, dim, hp);
+}
+
+// The SQ build's distance between rows a and b (PruneAccessor's DistanceComputer over two
+// stored codes, scalar.rs:370-446): the integer sums of the code rows by one half-wave
+// (exact in any order) and the f32 finish of the compensated metric; valid in every lane.
+__device__ inline float sq_pair(const BuildParams &p, uint32_t a, uint32_t b, int hp) {
+    const uint4 *ra = (const uint4 *)(p.sq_codes + (size_t)a * p.sq_cstride);
+    const uint4 *rb = (const uint4 *)(p.sq_codes + (size_t)b * p.sq_cstride);
+    uint32_t ab = 0, aa = 0, bb = 0;
+    for (int c = hp; c < p.sq_cstride / 16; c += 32) sq_dot16(ra[c], rb[c], ab, aa, bb);
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) {
+        ab += __shfl_xor(ab, off, 32);
+        aa += __shfl_xor(aa, off, 32);
+        bb += __shfl_xor(bb, off, 32);
+    }
+    // sum (a - b)^2 = aa + bb - 2 ab, exact: each sum is at most dim * 255^2
+    return sq_finish(p.metric, p.sq_bss, p.sq_ssn, ab, aa + bb - 2u * ab, p.sq_comp[a],
+                     p.sq_comp[b]);
+}
+
+// A prune distance between rows a (vector va, staged) and b (vector vb) by one half-wave:
+// the metric on the vectors, or the SQ build's code distance. Valid in hp == 0.
+__device__ inline float row_distance(const BuildParams &p, uint32_t a, const float *va,
+                                     uint32_t b, const float *vb, int hp) {
+    if (p.sq) return sq_pair(p, a, b, hp);
+    return metric_distance(p.metric, va, vb, p.dim, hp);
 }
 
 // HybridComputer (diskann-providers/src/model/graph/provider/async_/distances.rs:118-165)
@@ -201,7 +229,7 @@ __device__ inline int occlude(const BuildParams &p, PruneLds &s, int P, uint32_t
                 const float d = p.max_fp >= 0 && need
                                     ? hybrid_distance(p, cid, s.pfp[i] != 0, s.cand, s.pid[rp],
                                                       s.pfp[rp] != 0, av, hp)
-                                    : metric_distance(p.metric, s.cand, av, p.dim, hp);
+                                    : row_distance(p, cid, s.cand, need ? s.pid[rp] : cid, av, hp);
                 if (slot < found && hp == 0) s.dist[half] = need ? d : 0.0f;
                 __syncthreads();
                 if (t == 0) {
@@ -338,7 +366,7 @@ __global__ __launch_bounds__(kPruneThreads) void prune_search_kernel(BuildParams
         const float *v = p.vectors + (size_t)id * p.vstride;
         float d = hyb ? hybrid_distance(p, self_id, self_fp, s.cand, id,
                                         ok ? s.in_fp[nrec + e] != 0 : self_fp, v, hp)
-                      : metric_distance(p.metric, s.cand, v, p.dim, hp);
+                      : row_distance(p, self_id, s.cand, id, v, hp);
         if (ok && hp == 0) s.in_d[nrec + e] = d;
     }
     __syncthreads();
@@ -398,7 +426,7 @@ __global__ __launch_bounds__(kPruneThreads) void prune_list_kernel(BuildParams p
         const float *v = p.vectors + (size_t)id * p.vstride;
         float d = hyb ? hybrid_distance(p, loc, loc_fp, s.cand, id, ok ? s.in_fp[e] != 0 : loc_fp,
                                         v, hp)
-                      : metric_distance(p.metric, s.cand, v, p.dim, hp);
+                      : row_distance(p, loc, s.cand, id, v, hp);
         if (ok && hp == 0) s.in_d[e] = d;
     }
     __syncthreads();
@@ -508,7 +536,7 @@ __global__ __launch_bounds__(kPruneThreads) void backedge_kernel(BuildParams p,
         const float *v = p.vectors + (size_t)id * p.vstride;
         float d = hyb ? hybrid_distance(p, target, tgt_fp, s.cand, id,
                                         ok ? s.in_fp[e] != 0 : tgt_fp, v, hp)
-                      : metric_distance(p.metric, s.cand, v, p.dim, hp);
+                      : row_distance(p, target, s.cand, id, v, hp);
         if (ok && hp == 0) s.in_d[e] = d;
     }
     __syncthreads();

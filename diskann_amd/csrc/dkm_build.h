@@ -38,6 +38,14 @@ struct BuildParams {
     const float *pivots;   // K x dim
     const int32_t *pq_off; // pq_M + 1 chunk offsets
     int pq_M;
+    // 8-bit scalar-quantized build (the Quantized strategy over an SQStore, diskann-providers/
+    // src/model/graph/provider/async_/inmem/scalar.rs:370-446, 676-720): every prune distance
+    // is the compensated distance between the two rows' stored codes. 0: full precision.
+    int sq;
+    const uint8_t *sq_codes;  // rows x sq_cstride, zero padded
+    const float *sq_comp;     // rows
+    int sq_cstride;           // multiple of 16
+    float sq_bss, sq_ssn;     // (1/255)^2 scale^2, <shift, shift>
 };
 
 size_t build_prune_lds_bytes(const BuildParams &p);

@@ -217,6 +217,14 @@ class DiskANNIndex:
             raise ValueError("max_fp_vecs_per_prune must be >= 0")
         check(self.lib.dkm_set_build_hybrid(self._h, m), self.lib)
 
+    def set_build_sq(self, enabled=True):
+        """Scalar-quantized build (the Quantized strategy over an SQStore,
+        diskann-providers/src/model/graph/provider/async_/inmem/scalar.rs:676-720): the
+        insert search traverses with the SQ query computer and every prune distance is the
+        compensated distance between stored codes. Needs set_sq and encode_sq (codes of
+        every row); the reference runs it with build(batch=1)."""
+        check(self.lib.dkm_set_build_sq(self._h, int(bool(enabled))), self.lib)
+
     def build(self, ids=None, batch=1, intra_batch=-1, L_build=100, alpha=1.2,
               max_occlusion=750, saturate=False):
         """Insert rows `ids` (default: every data row, in order) with multi_insert

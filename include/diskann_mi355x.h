@@ -423,6 +423,17 @@ dkm_status dkm_closest_centers(const float *data, int64_t n, int64_t dim, const 
  * PQ schema and the codes of every row before dkm_build. */
 dkm_status dkm_set_build_hybrid(dkm_index *ix, int64_t max_fp_vecs_per_prune);
 
+/* Scalar-quantized build: the Quantized strategy over an SQStore (diskann-providers/src/
+ * model/graph/provider/async_/inmem/scalar.rs:676-720). dkm_build's insert search traverses
+ * with the SQ query computer, as the Quantized search does (insert_search_accessor =
+ * search_accessor, diskann/src/graph/glue.rs:933-940; inner-product queries rescaled to the
+ * mean norm), and every prune distance of dkm_build, dkm_prune_batch and dkm_prune_range is
+ * the compensated distance between the two rows' stored codes (PruneAccessor,
+ * scalar.rs:370-446). The reference runs it with single inserts (batch 1); batches use the
+ * same distances. Needs dkm_set_sq and the SQ codes of every row, start rows included
+ * (dkm_encode_index_sq); excludes the Hybrid prune. enable = 0: full precision. */
+dkm_status dkm_set_build_sq(dkm_index *ix, int enable);
+
 /* compute_medoid over the data rows (diskann-utils/src/sampling/medoid.rs:15-47):
  * writes the medoid row's vector (dim floats) and its row id (-1 if none qualifies). */
 dkm_status dkm_medoid(dkm_index *ix, float *out_vector, int64_t *out_row);

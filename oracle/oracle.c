@@ -1632,6 +1632,21 @@ static float hyb_dist(const dko_build_cfg *cfg, const float *vectors, int64_t vs
     return dko_pq_qq(cfg->metric, cfg->pivots, dim, cfg->pq_offsets, cfg->pq_M, ca, cb);
 }
 
+/* Distance between rows a and b in a prune: the metric on the vectors, or, for the SQ
+ * build, the compensated distance between their stored codes (scalar.rs:370-446 with the
+ * SQStore distance computer, scalar/vectors.rs:182-470). */
+static float pair_dist(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
+                       int64_t dim, uint32_t a, uint32_t b) {
+    if (cfg->sq) {
+        const dko_sq *q = cfg->sq;
+        return dko_sq_distance(cfg->metric, q->scale, q->shift_square_norm,
+                               q->codes + (int64_t)a * q->dim, q->comp[a],
+                               q->codes + (int64_t)b * q->dim, q->comp[b], q->dim);
+    }
+    return dko_distance(cfg->metric, vectors + (int64_t)a * vstride,
+                        vectors + (int64_t)b * vstride, dim);
+}
+
 static int in_set(const uint32_t *set, int64_t n, uint32_t id) {
     for (int64_t i = 0; i < n; ++i)
         if (set[i] == id) return 1;
@@ -1676,7 +1691,6 @@ static int64_t occlude_h(const dko_build_cfg *cfg, const float *vectors, int64_t
                 occ[i] = FLT_MAX;
                 continue;
             }
-            const float *vi = vectors + (int64_t)pool_ids[i] * vstride;
             while ((int64_t)lc != found) {
                 uint32_t rp = nbr[lc];
                 lc += 1;
@@ -1691,8 +1705,7 @@ static int64_t occlude_h(const dko_build_cfg *cfg, const float *vectors, int64_t
                     d = hyb_dist(cfg, vectors, vstride, dim, pool_ids[i], isfp[i], pool_ids[rp],
                                  isfp[rp]);
                 else
-                    d = dko_distance(cfg->metric, vi, vectors + (int64_t)pool_ids[rp] * vstride,
-                                     dim);
+                    d = pair_dist(cfg, vectors, vstride, dim, pool_ids[i], pool_ids[rp]);
                 of = dko_update_occlude_factor(cfg->metric, pool_d[i], d, of, cur_alpha);
                 if (of > cur_alpha) break;
             }
@@ -1823,7 +1836,6 @@ static int64_t prune_list_opt(const dko_build_cfg *cfg, const float *vectors, in
     }
     const int loc_fp = fpset ? in_set(fpset, nfp, location) : 1;
     int64_t m = 0;
-    const float *v = vectors + (int64_t)location * vstride;
     for (int64_t i = 0; i < n; ++i) {
         if (list[i] == location) continue;
         ids[m] = list[i];
@@ -1831,7 +1843,7 @@ static int64_t prune_list_opt(const dko_build_cfg *cfg, const float *vectors, in
             ds[m] = hyb_dist(cfg, vectors, vstride, dim, location, loc_fp, list[i],
                              in_set(fpset, nfp, list[i]));
         else
-            ds[m] = dko_distance(cfg->metric, v, vectors + (int64_t)list[i] * vstride, dim);
+            ds[m] = pair_dist(cfg, vectors, vstride, dim, location, list[i]);
         m++;
     }
     sort_pool(ids, ds, m);
@@ -1912,6 +1924,7 @@ static int64_t search_and_prune(const dko_build_cfg *cfg, const float *vectors, 
     ix.deg = g->deg;
     ix.nstart = nstart;
     ix.starts = starts;
+    ix.sq = cfg->sq;  /* SQ build: the insert search is the Quantized search */
     const float *v = vectors + (int64_t)id * vstride;
     int64_t L = cfg->L_build;
     scratch s;
@@ -1921,7 +1934,7 @@ static int64_t search_and_prune(const dko_build_cfg *cfg, const float *vectors, 
     float *rd = (float *)malloc(sizeof(float) * (size_t)(cap + nextras));
     uint32_t rlen = 0;
     dko_stats st;
-    search_core(&ix, 0, v, 1, &s, rid, rd, &rlen, cap, &st);
+    search_core(&ix, cfg->sq != NULL, v, 1, &s, rid, rd, &rlen, cap, &st);
     if (rlen > cap) rlen = (uint32_t)cap;
     /* Hybrid prune: fill(chain(once(self), chain(extras, record)).take(max_occlusion));
      * the first max_fp of those ids are full precision */
@@ -1943,7 +1956,7 @@ static int64_t search_and_prune(const dko_build_cfg *cfg, const float *vectors, 
             rd[rlen] = hyb_dist(cfg, vectors, vstride, dim, id, self_fp, extras[e],
                                 in_set(fpset, nfp, extras[e]));
         else
-            rd[rlen] = dko_distance(cfg->metric, v, vectors + (int64_t)extras[e] * vstride, dim);
+            rd[rlen] = pair_dist(cfg, vectors, vstride, dim, id, extras[e]);
         rlen++;
     }
     sort_pool(rid, rd, rlen);

@@ -222,6 +222,14 @@ typedef struct {
     const float *pivots;       /* K x dim */
     const int64_t *pq_offsets; /* pq_M + 1 */
     int64_t pq_M;
+    /* 8-bit scalar-quantized build (the Quantized strategy over an SQStore,
+     * diskann-providers/src/model/graph/provider/async_/inmem/scalar.rs:676-720): the
+     * insert search traverses with the SQ query computer (the Quantized search accessor,
+     * insert_search_accessor = search_accessor, glue.rs:933-940) and every prune distance
+     * is the compensated distance between two stored codes (PruneAccessor, scalar.rs:
+     * 370-446). NULL: full-precision (or Hybrid) build. Codes of every row, the start
+     * points included, must be present. */
+    const dko_sq *sq;
 } dko_build_cfg;
 
 /* compute_closest_centers (partitioned build): out[i*k + t] = the t-th closest centre of

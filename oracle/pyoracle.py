@@ -57,7 +57,7 @@ class BuildCfg(C.Structure):
         ("L_build", C.c_int64), ("alpha", C.c_float), ("max_occlusion", C.c_int64),
         ("saturate", C.c_int), ("max_backedges", C.c_int64),
         ("max_fp", C.c_int64), ("codes", C.c_void_p), ("pivots", C.c_void_p),
-        ("pq_offsets", C.c_void_p), ("pq_M", C.c_int64),
+        ("pq_offsets", C.c_void_p), ("pq_M", C.c_int64), ("sq", C.c_void_p),
     ]
 
 
@@ -570,12 +570,26 @@ class OracleIndex:
         return ids, ds, cm, hp, wall
 
 
+def _sq_struct(sq, dim, keep):
+    """dko_sq for sq = (scale, shift, mean_norm, codes (rows x dim), compensations)."""
+    scale, shift, mean_norm, sqc, comp = sq
+    sh = np.ascontiguousarray(shift, dtype=np.float32)
+    cd = np.ascontiguousarray(sqc, dtype=np.uint8)
+    cp = np.ascontiguousarray(comp, dtype=np.float32)
+    st = SQ(dim, scale, _p(sh), sq_shift_square_norm(sh), mean_norm, _p(cd), _p(cp))
+    keep += [sh, cd, cp, st]
+    return C.cast(C.pointer(st), C.c_void_p)
+
+
 def build_graph(metric, vectors, R, R_max, L_build, alpha, starts, ids, batch=1,
                 intra_batch=-1, start_adj=None, max_occlusion=750, saturate=False,
-                max_backedges=None, max_fp=None, codes=None, pivots=None, pq_offsets=None):
+                max_backedges=None, max_fp=None, codes=None, pivots=None, pq_offsets=None,
+                sq=None):
     """Vamana build on the oracle. Returns (adj, deg) for all rows of `vectors`.
     max_fp (with codes / pivots / pq_offsets): Hybrid prune with that many full-precision
-    ids per prune; None: full precision throughout."""
+    ids per prune; None: full precision throughout. sq = (scale, shift, mean_norm, codes,
+    compensations) of every row: the SQ Quantized build (SQ insert search, prune distances
+    between stored codes)."""
     vectors = np.ascontiguousarray(vectors, dtype=np.float32)
     rows = vectors.shape[0]
     adj = np.zeros((rows, R_max), dtype=np.uint32)
@@ -595,12 +609,14 @@ def build_graph(metric, vectors, R, R_max, L_build, alpha, starts, ids, batch=1,
         keep = [cd, pv, off]
         hyb = (int(max_fp), _p(cd), _p(pv), _p(off), off.shape[0] - 1)
     cfg = BuildCfg(metric, R, R_max, L_build, alpha, max_occlusion, int(saturate),
-                   R_max if max_backedges is None else max_backedges, *hyb)
+                   R_max if max_backedges is None else max_backedges, *hyb,
+                   _sq_struct(sq, vectors.shape[1], keep) if sq is not None else None)
     starts = np.ascontiguousarray(starts, dtype=np.uint32)
     ids = np.ascontiguousarray(ids, dtype=np.uint32)
     lib().dko_build(C.byref(cfg), _p(vectors), vectors.shape[1], vectors.shape[1],
                     C.byref(g), _p(starts), starts.shape[0], _p(ids), ids.shape[0], batch,
                     intra_batch)
+    del keep
     return adj, deg
 
 
@@ -616,8 +632,10 @@ def occlude_list(metric, vectors, pool_ids, pool_d, self_id, R, alpha, saturate=
 
 
 def prune_list(metric, vectors, location, lst, R, alpha, max_occlusion=750, saturate=False,
-               force_saturate=False, max_fp=None, codes=None, pivots=None, pq_offsets=None):
-    """robust_prune_list of `lst` around row `location`; returns the pruned ids."""
+               force_saturate=False, max_fp=None, codes=None, pivots=None, pq_offsets=None,
+               sq=None):
+    """robust_prune_list of `lst` around row `location`; returns the pruned ids (sq as in
+    build_graph: distances between stored SQ codes)."""
     vectors = np.ascontiguousarray(vectors, dtype=np.float32)
     lst = np.ascontiguousarray(lst, dtype=np.uint32)
     keep = []
@@ -629,7 +647,8 @@ def prune_list(metric, vectors, location, lst, R, alpha, max_occlusion=750, satu
         off = np.ascontiguousarray(pq_offsets, dtype=np.int64)
         keep = [cd, pv, off]
         hyb = (int(max_fp), _p(cd), _p(pv), _p(off), off.shape[0] - 1)
-    cfg = BuildCfg(metric, R, R, 0, alpha, max_occlusion, int(saturate), R, *hyb)
+    cfg = BuildCfg(metric, R, R, 0, alpha, max_occlusion, int(saturate), R, *hyb,
+                   _sq_struct(sq, vectors.shape[1], keep) if sq is not None else None)
     out = np.zeros(max(R, 1), dtype=np.uint32)
     n = lib().dko_prune_list(C.byref(cfg), _p(vectors), vectors.shape[1], vectors.shape[1],
                              int(location), _p(lst), lst.shape[0], int(force_saturate), _p(out))

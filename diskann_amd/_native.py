@@ -21,7 +21,7 @@ EXPORTS = [
     "dkm_index_create", "dkm_index_destroy", "dkm_set_vectors", "dkm_set_start_points",
     "dkm_set_graph", "dkm_get_graph", "dkm_set_pq", "dkm_set_codes", "dkm_encode_index",
     "dkm_get_codes", "dkm_search_batch", "dkm_disk_search_batch", "dkm_search_batch_device", "dkm_index_synchronize",
-    "dkm_profile", "dkm_profile_read", "dkm_set_tuning", "dkm_build", "dkm_prune_batch", "dkm_prune_range", "dkm_set_build_hybrid", "dkm_set_build_sq", "dkm_medoid",
+    "dkm_profile", "dkm_profile_read", "dkm_set_tuning", "dkm_build", "dkm_prune_batch", "dkm_prune_range", "dkm_set_build_hybrid", "dkm_set_build_sq", "dkm_set_build_spherical", "dkm_medoid",
     "dkm_bruteforce", "dkm_closest_centers", "dkm_search_batch_device_stats", "dkm_autotune",
     "dkm_search_graph_create", "dkm_search_graph_launch", "dkm_search_graph_destroy",
     "dkm_search_status", "dkm_search_graph_status", "dkm_set_visited_capacity",
@@ -129,6 +129,7 @@ def _declare(L):
         "dkm_prune_range": [P, P, I64, C.c_float, I64, I],
         "dkm_set_build_hybrid": [P, I64],
         "dkm_set_build_sq": [P, I],
+        "dkm_set_build_spherical": [P, I],
         "dkm_medoid": [P, P, C.POINTER(C.c_int64)],
         "dkm_bruteforce": [P, P, I64, I64, P, P],
         "dkm_closest_centers": [P, I64, I64, P, I64, I64, P],

@@ -120,6 +120,7 @@ struct dkm_index {
     int pq_len = 0;                     // uniform PQ chunk width usable by the fast path
     int64_t build_max_fp = -1;          // Hybrid prune: full-precision ids per prune (-1 = all)
     bool build_sq = false;              // SQ Quantized build strategy (dkm_set_build_sq)
+    bool build_sph = false;             // spherical build strategy (dkm_set_build_spherical)
     // 8-bit scalar quantization; when sq_active the quantized strategies traverse with it
     bool sq_active = false, sq_codes_set = false;
     float sq_scale = 0.0f, sq_mean_norm = 0.0f, sq_ssn = 0.0f;
@@ -1449,6 +1450,7 @@ static dkm_status ensure_spill(dkm_index *ix, int64_t nq, uint64_t slots) {
 static void set_sq_params(const dkm_index *ix, bool sq, dkm::SearchParams &p, bool sph = false) {
     p.sph = sph;
     p.sph_layout = ix->sph_layout;
+    p.sph_rescale = 1;
     p.sphq = ix->sphq;
     p.sph_bits = ix->d_sph_bits.p;
     p.sph_meta = ix->d_sph_meta.p;
@@ -2100,14 +2102,32 @@ dkm_status dkm_set_build_sq(dkm_index *ix, int enable) {
     return DKM_OK;
 }
 
+dkm_status dkm_set_build_spherical(dkm_index *ix, int enable) {
+    if (!ix) return fail(DKM_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(ix->mu);
+    ix->build_sph = enable != 0;
+    return DKM_OK;
+}
+
 // The build strategy's checks and its SQ distance parameters (prunes and the insert search).
 static dkm_status build_strategy(const dkm_index *ix, dkm::BuildParams &bp) {
     if (ix->build_max_fp >= 0 && (!ix->d_codes || !ix->codes_set))
         return fail(DKM_ERR_NOT_READY, "the Hybrid prune needs the PQ codes of every row");
-    if (ix->build_sq && ix->build_max_fp >= 0)
-        return fail(DKM_ERR_INVALID_ARG, "the SQ build and the Hybrid prune exclude each other");
+    if ((int)ix->build_sq + (int)ix->build_sph + (int)(ix->build_max_fp >= 0) > 1)
+        return fail(DKM_ERR_INVALID_ARG,
+                    "the SQ build, the spherical build and the Hybrid prune exclude each other");
     if (ix->build_sq && (!ix->sq_active || !ix->sq_codes_set))
         return fail(DKM_ERR_NOT_READY, "the SQ build needs the scalar quantizer and the codes of every row");
+    if (ix->build_sph && (!ix->sph_active || !ix->sph_codes_set))
+        return fail(DKM_ERR_NOT_READY,
+                    "the spherical build needs the spherical quantizer and the codes of every row");
+    bp.sph = ix->build_sph;
+    bp.sph_bits = ix->d_sph_bits.p;
+    bp.sph_meta = ix->d_sph_meta.p;
+    bp.sph_cstride = ix->sphq.cstride;
+    bp.sph_metric = ix->sphq.metric;
+    bp.sph_shift_sq = ix->sphq.shift_sq;
+    bp.sph_dim = (float)ix->sphq.out_dim;
     bp.sq = ix->build_sq;
     bp.sq_codes = ix->d_sq_codes.p;
     bp.sq_comp = ix->d_sq_comp.p;
@@ -2216,8 +2236,17 @@ dkm_status dkm_build(dkm_index *ix, const uint32_t *ids, int64_t n, int64_t batc
         sp.metric = ix->metric;
         sp.pq = 0;
         sp.rerank = 0;
-        // the SQ build's insert search is the Quantized search (SQ traversal, no rerank)
+        // the SQ build's insert search is the Quantized search (SQ traversal, no rerank); the
+        // spherical build's traverses with SameAsData queries and no inner-product rescale
+        // (Quantized::build(), spherical.rs:515-533)
         if (ix->build_sq) set_sq_params(ix, true, sp);
+        if (ix->build_sph) {
+            set_sq_params(ix, false, sp, true);
+            sp.sph_layout = dkm::SPH_QUERY_SAME_AS_DATA;
+            sp.sph_rescale = 0;
+            HIP_TRY(ix->d_sph_qrec.ensure((int64_t)nb * dkm::sph_query_record_bytes(ix->sphq)));
+            sp.sph_qrec = ix->d_sph_qrec.p;
+        }
         sp.dim = (int)ix->dim;
         sp.vstride = ix->vstride;
         sp.vectors = ix->d_vec;

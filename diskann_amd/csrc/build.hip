@@ -115,10 +115,29 @@ __device__ inline float sq_pair(const BuildParams &p, uint32_t a, uint32_t b, in
                      p.sq_comp[b]);
 }
 
+// The spherical build's distance between rows a and b (the store's data x data distance
+// computer, spherical.rs:293-350): the 1-bit inner product of the code rows by one
+// half-wave (an exact popcount sum) and the compensated metric's f32 finish.
+__device__ inline float sph_pair(const BuildParams &p, uint32_t a, uint32_t b, int hp) {
+    const uint4 *ra = (const uint4 *)(p.sph_bits + (size_t)a * p.sph_cstride);
+    const uint4 *rb = (const uint4 *)(p.sph_bits + (size_t)b * p.sph_cstride);
+    uint32_t ip = 0;
+    for (int c = hp; c < p.sph_cstride / 16; c += 32) {
+        const uint4 x = ra[c], y = rb[c];
+        ip += __builtin_popcount(x.x & y.x) + __builtin_popcount(x.y & y.y) +
+              __builtin_popcount(x.z & y.z) + __builtin_popcount(x.w & y.w);
+    }
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) ip += __shfl_xor(ip, off, 32);
+    return sph_data_data(p.sph_metric, p.sph_shift_sq, p.sph_dim, ip, p.sph_meta[a],
+                         p.sph_meta[b]);
+}
+
 // A prune distance between rows a (vector va, staged) and b (vector vb) by one half-wave:
-// the metric on the vectors, or the SQ build's code distance. Valid in hp == 0.
+// the metric on the vectors, or the SQ / spherical build's code distance. Valid in hp == 0.
 __device__ inline float row_distance(const BuildParams &p, uint32_t a, const float *va,
                                      uint32_t b, const float *vb, int hp) {
+    if (p.sph) return sph_pair(p, a, b, hp);
     if (p.sq) return sq_pair(p, a, b, hp);
     return metric_distance(p.metric, va, vb, p.dim, hp);
 }

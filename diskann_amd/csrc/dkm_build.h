@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dkm_sph.h"
+
 namespace dkm {
 
 enum : uint32_t { BUILD_ERR_RECORD = 1u, BUILD_ERR_POOL = 2u };
@@ -46,6 +48,15 @@ struct BuildParams {
     const float *sq_comp;     // rows
     int sq_cstride;           // multiple of 16
     float sq_bss, sq_ssn;     // (1/255)^2 scale^2, <shift, shift>
+    // 1-bit spherical build (Quantized::build() over a SphericalStore, diskann-providers/src/
+    // model/graph/provider/async_/inmem/spherical.rs:293-350, 605-650): every prune distance
+    // is the data x data distance of the two rows' codes. 0: not spherical.
+    int sph;
+    const uint8_t *sph_bits;  // rows x sph_cstride
+    const SphMeta *sph_meta;  // rows
+    int sph_cstride;
+    int sph_metric;           // the quantizer's metric
+    float sph_shift_sq, sph_dim;  // <shift, shift>, output dimension
 };
 
 size_t build_prune_lds_bytes(const BuildParams &p);

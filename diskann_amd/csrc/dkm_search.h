@@ -74,6 +74,8 @@ struct SearchParams {
     // 1-bit spherical quantization (traverse with it when sph is set; pq and sq are 0)
     int sph;
     int sph_layout;           // SPH_QUERY_SAME_AS_DATA or SPH_QUERY_FOUR_BIT
+    int sph_rescale;          // inner-product queries rescaled to the mean norm (the search
+                              // strategies; the build strategy does not)
     SphParams sphq;
     const uint8_t *sph_bits;  // rows x sphq.cstride
     const SphMeta *sph_meta;  // rows

@@ -327,7 +327,7 @@ __device__ inline void sph_query_code(const SphParams &q, int layout, bool resca
 // then its 4 metadata floats.
 __host__ __device__ inline int sph_query_record_bytes(const SphParams &q) { return 4 * q.cstride + 16; }
 
-hipError_t launch_sph_query_prep(const SphParams &q, int layout, const float *queries,
+hipError_t launch_sph_query_prep(const SphParams &q, int layout, int rescale, const float *queries,
                                  int64_t qstride, int64_t nq, uint8_t *records, hipStream_t st);
 // 8-bit MinMax (diskann-quantization/src/minmax) with a SphParams transform: canonical
 // rows of 20 + out_dim bytes (u32 dim, f32 b, f32 n, f32 a, f32 |r|^2, codes).

@@ -1475,7 +1475,8 @@ hipError_t launch_search(const SearchParams &p, hipStream_t stream) {
     }
     if (p.sph) {
         if (!p.sph_qrec) return hipErrorInvalidValue;
-        hipError_t e = launch_sph_query_prep(p.sphq, p.sph_layout, p.queries, p.qstride, p.nq,
+        hipError_t e = launch_sph_query_prep(p.sphq, p.sph_layout, p.sph_rescale, p.queries,
+                                             p.qstride, p.nq,
                                              p.sph_qrec, stream);
         if (e != hipSuccess) return e;
     }

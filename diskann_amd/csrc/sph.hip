@@ -163,7 +163,7 @@ __global__ __launch_bounds__(64) void sph_query_pairs_kernel(SphParams q, int la
 // Traversal queries compressed ahead of the search (one block per query, with the
 // inner-product rescale of the reference's search strategy): record q holds the layout's
 // bit row / planes and the 4 metadata floats.
-__global__ __launch_bounds__(64) void sph_query_prep_kernel(SphParams q, int layout,
+__global__ __launch_bounds__(64) void sph_query_prep_kernel(SphParams q, int layout, int rescale,
                                                             const float *queries, int64_t qstride,
                                                             int64_t nq, uint8_t *records) {
     extern __shared__ __align__(16) float lds[];
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(64) void sph_query_prep_kernel(SphParams q, int lay
     uint8_t *qb = (uint8_t *)(lds + sph_query_lds_floats(q));
     __shared__ float qm[4];
     uint32_t e = 0;
-    sph_query_code(q, layout, true, queries + qi * qstride, lds, qb, qm, e);
+    sph_query_code(q, layout, rescale != 0, queries + qi * qstride, lds, qb, qm, e);
     uint8_t *rec = records + qi * sph_query_record_bytes(q);
     const int nbytes = layout == SPH_QUERY_SAME_AS_DATA ? q.cstride : 4 * q.cstride;
     for (int i = threadIdx.x; i < nbytes / 16; i += 64)
@@ -271,7 +271,7 @@ hipError_t launch_sph_query_pairs(const SphParams &q, int layout, int rescale, c
     return hipGetLastError();
 }
 
-hipError_t launch_sph_query_prep(const SphParams &q, int layout, const float *queries,
+hipError_t launch_sph_query_prep(const SphParams &q, int layout, int rescale, const float *queries,
                                  int64_t qstride, int64_t nq, uint8_t *records, hipStream_t st) {
     if (nq <= 0) return hipSuccess;
     const size_t lds = sizeof(float) * (size_t)sph_query_lds_floats(q) + 4 * (size_t)q.cstride + 64;
@@ -279,7 +279,7 @@ hipError_t launch_sph_query_prep(const SphParams &q, int layout, const float *qu
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sph_query_prep_kernel, dim3((unsigned)nq), dim3(64), lds, st, q, layout,
-                       queries, qstride, nq, records);
+                       rescale, queries, qstride, nq, records);
     return hipGetLastError();
 }
 

@@ -225,6 +225,14 @@ class DiskANNIndex:
         every row); the reference runs it with build(batch=1)."""
         check(self.lib.dkm_set_build_sq(self._h, int(bool(enabled))), self.lib)
 
+    def set_build_spherical(self, enabled=True):
+        """Spherical build (Quantized::build() over a SphericalStore,
+        diskann-providers/src/model/graph/provider/async_/inmem/spherical.rs:515-533,
+        605-650): the insert search traverses with SameAsData queries and no inner-product
+        rescale, every prune distance is the data x data distance of stored codes. Needs
+        set_spherical and encode_spherical."""
+        check(self.lib.dkm_set_build_spherical(self._h, int(bool(enabled))), self.lib)
+
     def build(self, ids=None, batch=1, intra_batch=-1, L_build=100, alpha=1.2,
               max_occlusion=750, saturate=False):
         """Insert rows `ids` (default: every data row, in order) with multi_insert

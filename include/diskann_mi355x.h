@@ -434,6 +434,16 @@ dkm_status dkm_set_build_hybrid(dkm_index *ix, int64_t max_fp_vecs_per_prune);
  * (dkm_encode_index_sq); excludes the Hybrid prune. enable = 0: full precision. */
 dkm_status dkm_set_build_sq(dkm_index *ix, int enable);
 
+/* Spherical build: Quantized::build() over a SphericalStore (diskann-providers/src/model/
+ * graph/provider/async_/inmem/spherical.rs:515-533, 605-650). dkm_build's insert search
+ * traverses with the spherical codes, the query compressed in the data layout (SameAsData)
+ * without the inner-product rescale (is_search = false), and every prune distance of
+ * dkm_build, dkm_prune_batch and dkm_prune_range is the data x data distance of the two
+ * rows' codes (PruneAccessor, spherical.rs:293-350). Needs dkm_set_spherical and the codes of
+ * every row, start rows included (dkm_encode_index_sph); excludes the SQ build and the Hybrid
+ * prune. enable = 0: full precision. */
+dkm_status dkm_set_build_spherical(dkm_index *ix, int enable);
+
 /* compute_medoid over the data rows (diskann-utils/src/sampling/medoid.rs:15-47):
  * writes the medoid row's vector (dim floats) and its row id (-1 if none qualifies). */
 dkm_status dkm_medoid(dkm_index *ix, float *out_vector, int64_t *out_row);

@@ -1251,7 +1251,7 @@ static void search_core(const dko_index *ix, int pq, const float *query, int64_t
         const dko_sph *q = ix->sph;
         float *tmp = (float *)malloc(sizeof(float) * (size_t)q->dim);
         memcpy(tmp, query, sizeof(float) * (size_t)q->dim);
-        if (q->metric == DKO_INNER_PRODUCT) dko_sph_rescale(q, tmp);
+        if (q->metric == DKO_INNER_PRODUCT && !ix->sph_norescale) dko_sph_rescale(q, tmp);
         if (ix->sph_layout == 0)
             dko_sph_compress_data(q, tmp, (uint8_t *)s->lut);
         else
@@ -1637,6 +1637,11 @@ static float hyb_dist(const dko_build_cfg *cfg, const float *vectors, int64_t vs
  * SQStore distance computer, scalar/vectors.rs:182-470). */
 static float pair_dist(const dko_build_cfg *cfg, const float *vectors, int64_t vstride,
                        int64_t dim, uint32_t a, uint32_t b) {
+    if (cfg->sph) {
+        const int64_t cb = dko_sph_data_bytes(cfg->sph);
+        return dko_sph_dist_data(cfg->sph, cfg->sph_codes + (int64_t)a * cb,
+                                 cfg->sph_codes + (int64_t)b * cb);
+    }
     if (cfg->sq) {
         const dko_sq *q = cfg->sq;
         return dko_sq_distance(cfg->metric, q->scale, q->shift_square_norm,
@@ -1925,6 +1930,10 @@ static int64_t search_and_prune(const dko_build_cfg *cfg, const float *vectors, 
     ix.nstart = nstart;
     ix.starts = starts;
     ix.sq = cfg->sq;  /* SQ build: the insert search is the Quantized search */
+    ix.sph = cfg->sph; /* spherical build: SameAsData queries, no rescale */
+    ix.sph_codes = cfg->sph_codes;
+    ix.sph_layout = 0;
+    ix.sph_norescale = 1;
     const float *v = vectors + (int64_t)id * vstride;
     int64_t L = cfg->L_build;
     scratch s;
@@ -1934,7 +1943,7 @@ static int64_t search_and_prune(const dko_build_cfg *cfg, const float *vectors, 
     float *rd = (float *)malloc(sizeof(float) * (size_t)(cap + nextras));
     uint32_t rlen = 0;
     dko_stats st;
-    search_core(&ix, cfg->sq != NULL, v, 1, &s, rid, rd, &rlen, cap, &st);
+    search_core(&ix, cfg->sq != NULL || cfg->sph != NULL, v, 1, &s, rid, rd, &rlen, cap, &st);
     if (rlen > cap) rlen = (uint32_t)cap;
     /* Hybrid prune: fill(chain(once(self), chain(extras, record)).take(max_occlusion));
      * the first max_fp of those ids are full precision */

@@ -167,6 +167,9 @@ typedef struct {
     const dko_sph *sph;
     const uint8_t *sph_codes;
     int sph_layout;
+    /* spherical traversal without the inner-product query rescale (the build strategy,
+     * Quantized::build, is_search = false: spherical.rs:515-533) */
+    int sph_norescale;
 } dko_index;
 
 typedef struct {
@@ -230,6 +233,13 @@ typedef struct {
      * 370-446). NULL: full-precision (or Hybrid) build. Codes of every row, the start
      * points included, must be present. */
     const dko_sq *sq;
+    /* 1-bit spherical build (Quantized::build() over a SphericalStore,
+     * diskann-providers/src/model/graph/provider/async_/inmem/spherical.rs:293-350, 515-533,
+     * 605-650): the insert search traverses with SameAsData queries and no inner-product
+     * rescale, every prune distance is the data x data distance of the two rows' canonical
+     * codes (sph_codes, rows x dko_sph_data_bytes). NULL: not spherical. */
+    const dko_sph *sph;
+    const uint8_t *sph_codes;
 } dko_build_cfg;
 
 /* compute_closest_centers (partitioned build): out[i*k + t] = the t-th closest centre of

@@ -36,6 +36,7 @@ class Index(C.Structure):
         ("M", C.c_int64), ("K", C.c_int64), ("codes", C.c_void_p),
         ("pivots", C.c_void_p), ("offsets", C.c_void_p), ("sq", C.c_void_p),
         ("sph", C.c_void_p), ("sph_codes", C.c_void_p), ("sph_layout", C.c_int),
+        ("sph_norescale", C.c_int),
     ]
 
 
@@ -58,6 +59,7 @@ class BuildCfg(C.Structure):
         ("saturate", C.c_int), ("max_backedges", C.c_int64),
         ("max_fp", C.c_int64), ("codes", C.c_void_p), ("pivots", C.c_void_p),
         ("pq_offsets", C.c_void_p), ("pq_M", C.c_int64), ("sq", C.c_void_p),
+        ("sph", C.c_void_p), ("sph_codes", C.c_void_p),
     ]
 
 
@@ -487,7 +489,9 @@ class OracleIndex:
                          _p(self.sq_codes), _p(self.sq_comp))
             s.sq = C.cast(C.pointer(self.sq), C.c_void_p)
         if sph is not None:
-            quant, sph_codes, layout = sph   # Spherical, canonical rows, 0 / 1
+            # Spherical, canonical rows, layout 0 / 1 [, no inner-product query rescale]
+            quant, sph_codes, layout = sph[:3]
+            s.sph_norescale = int(len(sph) > 3 and bool(sph[3]))
             self.sph_q = quant
             self.sph_codes = np.ascontiguousarray(sph_codes, dtype=np.uint8)
             s.sph = C.cast(C.pointer(quant.s), C.c_void_p)
@@ -581,15 +585,26 @@ def _sq_struct(sq, dim, keep):
     return C.cast(C.pointer(st), C.c_void_p)
 
 
+def _sph_args(sph, keep):
+    """(dko_sph pointer, canonical code rows pointer) for sph = (Spherical, codes)."""
+    if sph is None:
+        return None, None
+    quant, codes = sph
+    cd = np.ascontiguousarray(codes, dtype=np.uint8)
+    keep += [quant, cd]
+    return C.cast(C.pointer(quant.s), C.c_void_p), _p(cd)
+
+
 def build_graph(metric, vectors, R, R_max, L_build, alpha, starts, ids, batch=1,
                 intra_batch=-1, start_adj=None, max_occlusion=750, saturate=False,
                 max_backedges=None, max_fp=None, codes=None, pivots=None, pq_offsets=None,
-                sq=None):
+                sq=None, sph=None):
     """Vamana build on the oracle. Returns (adj, deg) for all rows of `vectors`.
     max_fp (with codes / pivots / pq_offsets): Hybrid prune with that many full-precision
     ids per prune; None: full precision throughout. sq = (scale, shift, mean_norm, codes,
     compensations) of every row: the SQ Quantized build (SQ insert search, prune distances
-    between stored codes)."""
+    between stored codes). sph = (Spherical, canonical code rows of every row): the
+    spherical Quantized::build() strategy."""
     vectors = np.ascontiguousarray(vectors, dtype=np.float32)
     rows = vectors.shape[0]
     adj = np.zeros((rows, R_max), dtype=np.uint32)
@@ -610,7 +625,8 @@ def build_graph(metric, vectors, R, R_max, L_build, alpha, starts, ids, batch=1,
         hyb = (int(max_fp), _p(cd), _p(pv), _p(off), off.shape[0] - 1)
     cfg = BuildCfg(metric, R, R_max, L_build, alpha, max_occlusion, int(saturate),
                    R_max if max_backedges is None else max_backedges, *hyb,
-                   _sq_struct(sq, vectors.shape[1], keep) if sq is not None else None)
+                   _sq_struct(sq, vectors.shape[1], keep) if sq is not None else None,
+                   *_sph_args(sph, keep))
     starts = np.ascontiguousarray(starts, dtype=np.uint32)
     ids = np.ascontiguousarray(ids, dtype=np.uint32)
     lib().dko_build(C.byref(cfg), _p(vectors), vectors.shape[1], vectors.shape[1],
@@ -633,7 +649,7 @@ def occlude_list(metric, vectors, pool_ids, pool_d, self_id, R, alpha, saturate=
 
 def prune_list(metric, vectors, location, lst, R, alpha, max_occlusion=750, saturate=False,
                force_saturate=False, max_fp=None, codes=None, pivots=None, pq_offsets=None,
-               sq=None):
+               sq=None, sph=None):
     """robust_prune_list of `lst` around row `location`; returns the pruned ids (sq as in
     build_graph: distances between stored SQ codes)."""
     vectors = np.ascontiguousarray(vectors, dtype=np.float32)
@@ -648,7 +664,8 @@ def prune_list(metric, vectors, location, lst, R, alpha, max_occlusion=750, satu
         keep = [cd, pv, off]
         hyb = (int(max_fp), _p(cd), _p(pv), _p(off), off.shape[0] - 1)
     cfg = BuildCfg(metric, R, R, 0, alpha, max_occlusion, int(saturate), R, *hyb,
-                   _sq_struct(sq, vectors.shape[1], keep) if sq is not None else None)
+                   _sq_struct(sq, vectors.shape[1], keep) if sq is not None else None,
+                   *_sph_args(sph, keep))
     out = np.zeros(max(R, 1), dtype=np.uint32)
     n = lib().dko_prune_list(C.byref(cfg), _p(vectors), vectors.shape[1], vectors.shape[1],
                              int(location), _p(lst), lst.shape[0], int(force_saturate), _p(out))

@@ -567,7 +567,7 @@ def served_workload(args):
             rngp = np.random.default_rng(0xD15C6004)
             loads = []
             for rate in (20_000.0, 50_000.0, 100_000.0, 200_000.0, 300_000.0, 400_000.0,
-                         500_000.0, 600_000.0, 800_000.0, 1_000_000.0):
+                         450_000.0, 500_000.0, 600_000.0, 800_000.0, 1_000_000.0):
                 if rate > 0.9 * sat:
                     continue
                 nreq = int(max(20_000, rate * 0.5))

@@ -152,6 +152,8 @@ __device__ __forceinline__ void f8_tile(const i32x8_t *a0, const i32x8_t *a1,
                                   *(const i32x4_t *)(buf + (2 * s + 1) * 512));
         acc0 = mfma_f8(b, a0[s], acc0);
         acc1 = mfma_f8(b, a1[s], acc1);
+        // keep the two row tiles' chains interleaved (each column fragment feeds both)
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
     }
 }
 
@@ -201,6 +203,9 @@ struct Fp8Rows {
             init(t, pa, pb);
             if (t + 1 < ntiles) res_prefetch<KB>(src(t + 1), sm + ((t + 1) & 1) * KB * 512, wave);
             f8_tile<KB>(a0, a1, cur + lane * 8, pa, pb);
+            // both row tiles' results are complete here: without this the second chain is
+            // sunk past the first tile's epilogue (its loop), serialising the two
+            asm volatile("" : "+a"(pa), "+a"(pb));
             epi(t, pa, pb);
         }
     }

@@ -171,9 +171,11 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
     uint32_t filled = 0;               // ballot epilogue: the wave's pool entries (uniform)
     // passing accumulators of one row tile (acc, row r, its h, valid) after tile t
     auto scan = [&](const floatx16_t &acc, float h, bool valid, int64_t row, int t) {
+        // acc - h >= 0 exactly when acc >= h (the difference of two distinct floats is never
+        // zero), so the mask needs no subtraction; the key below takes the difference
         uint32_t m = 0;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) m |= (valid && acc[r] - h >= 0.0f) ? (1u << r) : 0u;
+        for (int r = 0; r < 16; ++r) m |= (valid && acc[r] >= h) ? (1u << r) : 0u;
         while (__ballot(m != 0)) {
             if (m) {
                 const int r = __builtin_ctz(m);

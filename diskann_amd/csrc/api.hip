@@ -3372,7 +3372,7 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
     HIP_TRY(qf.ensure(Bp * dp));
     HIP_TRY(dense.ensure(B * ps->ns));
     HIP_TRY(tau.ensure(3 * B));
-    HIP_TRY(cnt.ensure(B));
+    HIP_TRY(cnt.ensure(B * PS_COUNT_STRIDE));  // one cache line per query's counter
     HIP_TRY(bnd_b.ensure(B));
     HIP_TRY(bnd_e.ensure(B));
     HIP_TRY(lists.ensure(B * cap));
@@ -3391,7 +3391,7 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
         }
     } evg{ev};
     double acc_ms[4] = {0, 0, 0, 0};
-    std::vector<uint32_t> hcnt(B);
+    std::vector<uint32_t> hcnt(B * PS_COUNT_STRIDE);  // query i's count at i * PS_COUNT_STRIDE
     std::vector<float> htau(3 * B);
     for (int64_t q0 = 0; q0 < nq; q0 += B) {
         const int b = (int)std::min<int64_t>(B, nq - q0);
@@ -3418,30 +3418,33 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
             for (int i = 0; i < b; ++i) htau[i] = __builtin_inff();
             HIP_TRY(hipMemcpy(tau.p, htau.data(), sizeof(float) * b, hipMemcpyHostToDevice));
         }
-        HIP_TRY(hipMemset(cnt.p, 0, sizeof(uint32_t) * b));
+        HIP_TRY(hipMemset(cnt.p, 0, sizeof(uint32_t) * b * PS_COUNT_STRIDE));
         HIP_TRY(dkm::ps_filter(dp, ps->fp8, ps->xf.p, ps->n, ps->xn.p, qf.p, b, bp, l2, S, tau.p,
                                nullptr, cnt.p, lists.p, cap, nullptr));
         // Redo the queries whose list is short (looser threshold, then every row) or overflowed
         // (tighter threshold): only those queries, gathered into a small launch of their own.
         for (int round = 0;; ++round) {
-            HIP_TRY(hipMemcpy(hcnt.data(), cnt.p, sizeof(uint32_t) * b, hipMemcpyDeviceToHost));
+            // the padded counters whole (a strided copy moves one 4-byte row at a time)
+            HIP_TRY(hipMemcpy(hcnt.data(), cnt.p, sizeof(uint32_t) * b * PS_COUNT_STRIDE,
+                              hipMemcpyDeviceToHost));
             std::vector<uint32_t> ridx;
             std::vector<int64_t> rrow;
             std::vector<float> rtau;
             for (int i = 0; i < b; ++i) {
-                if (hcnt[i] < (uint32_t)C) {
+                uint32_t &ci = hcnt[(size_t)i * PS_COUNT_STRIDE];
+                if (ci < (uint32_t)C) {
                     rtau.push_back(round == 0 ? htau[b + i] : __builtin_inff());
-                } else if (hcnt[i] > (uint64_t)cap) {
+                } else if (ci > (uint64_t)cap) {
                     if (round > 0)
                         return fail(DKM_ERR_CAPACITY, "query %lld: %u candidates exceed %lld",
-                                    (long long)(q0 + i), hcnt[i], (long long)cap);
+                                    (long long)(q0 + i), ci, (long long)cap);
                     rtau.push_back(htau[2 * b + i]);
                 } else {
                     continue;
                 }
                 ridx.push_back((uint32_t)i);
                 rrow.push_back(i);
-                hcnt[i] = 0;
+                ci = 0;
             }
             if (ridx.empty()) break;
             if (round == 2) return fail(DKM_ERR_CAPACITY, "pre-scoring threshold did not converge");
@@ -3451,7 +3454,8 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
             HIP_TRY(r_map.ensure(f));
             HIP_TRY(r_row.ensure(f));
             HIP_TRY(r_tau.ensure(f));
-            HIP_TRY(hipMemcpy(cnt.p, hcnt.data(), sizeof(uint32_t) * b, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(cnt.p, hcnt.data(), sizeof(uint32_t) * b * PS_COUNT_STRIDE,
+                              hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(r_map.p, ridx.data(), sizeof(uint32_t) * f, hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(r_row.p, rrow.data(), sizeof(int64_t) * f, hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(r_tau.p, rtau.data(), sizeof(float) * f, hipMemcpyHostToDevice));

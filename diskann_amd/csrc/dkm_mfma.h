@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace dkm {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
@@ -157,6 +159,33 @@ __device__ __forceinline__ void f8_tile(const i32x8_t *a0, const i32x8_t *a1,
     }
 }
 
+// f(std::integral_constant<int, i>) for i = I .. N - 1, so f can index registers statically.
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// f8_tile with independent work threaded between the k-steps: work(integral_constant<s>)
+// follows k-step s's two MFMAs in program order, so its VALU and LDS instructions issue
+// while the matrix core runs (each 32x32x64 MFMA keeps it busy for 64 cycles).
+template <int KB, class Work>
+__device__ __forceinline__ void f8_tile_with(const i32x8_t *a0, const i32x8_t *a1,
+                                             const uint16_t *buf, floatx16_t &acc0,
+                                             floatx16_t &acc1, Work &&work) {
+    static_assert(KB % 2 == 0, "k-steps of 64 dims");
+    static_for<0, KB / 2>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const i32x8_t b = f8_join(*(const i32x4_t *)(buf + (2 * s) * 512),
+                                  *(const i32x4_t *)(buf + (2 * s + 1) * 512));
+        acc0 = mfma_f8(b, a0[s], acc0);
+        acc1 = mfma_f8(b, a1[s], acc1);
+        work(sc);
+    });
+}
+
 // Operand policies for the pre-scoring GEMMs: kBlocks 1 KB blocks per 32-row tile, the
 // wave's two row tiles held in registers, and the sweep over the column tiles.
 template <int KS>
@@ -164,6 +193,7 @@ struct Bf16Rows {
     static constexpr int kBlocks = KS;
     // the 768-dim row fragments fill the register file: no room for the mask epilogue
     static constexpr bool kMaskEpilogue = KS < 48;
+    static constexpr bool kPipelined = false;
     bf16x8_t a0[KS], a1[KS];
     __device__ __forceinline__ void load(const uint16_t *ap) { res_load_rows<KS>(ap, a0, a1); }
     template <class Init, class Epi>
@@ -177,6 +207,8 @@ template <int KB>
 struct Fp8Rows {
     static constexpr int kBlocks = KB;
     static constexpr bool kMaskEpilogue = true;
+    // the filter threads a tile's threshold work between the next tile's MFMAs
+    static constexpr bool kPipelined = true;
     static_assert(KB % 4 == 0, "a column tile is prefetched four blocks per wave");
     i32x8_t a0[KB / 2], a1[KB / 2];
     __device__ __forceinline__ void load(const uint16_t *ap) {

@@ -9,6 +9,9 @@
 // Passing (row, query) pairs each filter wave stages in LDS before appending them
 // (a multiple of 64).
 #define PS_POOL 1536
+// Query q's list counter sits at count[q * PS_COUNT_STRIDE]: one 128-byte line per query, so
+// the filter's atomics on different queries never contend for a line.
+#define PS_COUNT_STRIDE 32
 
 namespace dkm {
 bool ps_supported_dp(int dp);

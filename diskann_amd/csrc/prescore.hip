@@ -132,10 +132,13 @@ __global__ __launch_bounds__(256) void ps_select_kernel(const float *dense, int6
 // no returning atomic waits inside the tile loop; pairs beyond a lane's slots are appended
 // directly.
 constexpr int kPool = PS_POOL;
+// a wave's LDS pool holds kPool staged pairs, plus one sink slot per lane in the pipelined
+// (FP8) sweep for the stores of a branch-free staging round that stage nothing
+constexpr int pool_alloc(bool pipelined) { return kPool + (pipelined ? 64 : 0); }
 
 __device__ __forceinline__ void ps_append(uint32_t *count, uint64_t *lists, int64_t cap, int q,
                                           uint64_t key) {
-    const uint32_t pos = atomicAdd(&count[q], 1u);
+    const uint32_t pos = atomicAdd(&count[(int64_t)q * PS_COUNT_STRIDE], 1u);
     if (pos < cap) lists[(int64_t)q * cap + pos] = key;
 }
 
@@ -152,8 +155,9 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
     float *tau_s = (float *)(sm + 2 * G::kBlocks * 512);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int half = lane >> 5, l32 = lane & 31;
-    uint64_t *skey = (uint64_t *)(tau_s + nqp) + wave * kPool;
-    uint16_t *sq = (uint16_t *)((uint64_t *)(tau_s + nqp) + 4 * kPool) + wave * kPool;
+    constexpr int kPoolAlloc = pool_alloc(G::kPipelined);
+    uint64_t *skey = (uint64_t *)(tau_s + nqp) + wave * kPoolAlloc;
+    uint16_t *sq = (uint16_t *)((uint64_t *)(tau_s + nqp) + 4 * kPoolAlloc) + wave * kPoolAlloc;
     const int64_t rtile = (int64_t)blockIdx.x * 8 + 2 * wave;
     G g;
     g.load(xf + rtile * G::kBlocks * 512 + lane * 8);
@@ -169,38 +173,41 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
     constexpr int kPer = kPool / 64;  // LDS slots per lane: entry j of lane l at j * 64 + l
     uint32_t nst = 0;                  // mask epilogue: this lane's staged entries
     uint32_t filled = 0;               // ballot epilogue: the wave's pool entries (uniform)
-    // passing accumulators of one row tile (acc, row r, its h, valid) after tile t
-    auto scan = [&](const floatx16_t &acc, float h, bool valid, int64_t row, int t) {
-        // acc - h >= 0 exactly when acc >= h (the difference of two distinct floats is never
-        // zero), so the mask needs no subtraction; the key below takes the difference
-        uint32_t m = 0;
+    // accumulator r of acc, r lane-varying: a binary tree of bitwise selects under opaque
+    // masks (a select tree the compiler can see through is folded into a dynamic element
+    // index, which it lowers through scratch memory)
+    auto pick = [](const floatx16_t &acc, int r) {
+        uint32_t msk[4];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) m |= (valid && acc[r] >= h) ? (1u << r) : 0u;
+        for (int b = 0; b < 4; ++b) {
+            msk[b] = ((r >> b) & 1) ? ~0u : 0u;
+            asm volatile("" : "+v"(msk[b]));
+        }
+        auto sel = [](uint32_t m, uint32_t x1, uint32_t x0) { return (x1 & m) | (x0 & ~m); };
+        uint32_t a8[8], a4[4], a2[2];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            a8[i] = sel(msk[0], __float_as_uint(acc[2 * i + 1]), __float_as_uint(acc[2 * i]));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a4[i] = sel(msk[1], a8[2 * i + 1], a8[2 * i]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a2[i] = sel(msk[2], a4[2 * i + 1], a4[2 * i]);
+        return __uint_as_float(sel(msk[3], a2[1], a2[0]));
+    };
+    // the pair (query q, key) of accumulator r of a row tile after tile t
+    auto query_of = [&](int t, int r) { return 32 * t + 8 * (r >> 2) + 4 * half + (r & 3); };
+    auto key_of = [](float u, int64_t row) {
+        return ((uint64_t)orderable(-u) << 32) | (uint32_t)row;
+    };
+    // every set bit of m (passing accumulators of acc after tile t): a wave-uniform loop
+    // takes one bit per lane per round and stages the pair in the lane's own LDS slots
+    auto drain = [&](const floatx16_t &acc, uint32_t m, float h, int64_t row, int t) {
         while (__ballot(m != 0)) {
             if (m) {
                 const int r = __builtin_ctz(m);
                 m &= m - 1;
-                // binary tree of bitwise selects under opaque masks: a select tree the
-                // compiler can see through is folded into a dynamic element index, which
-                // it lowers through scratch memory
-                uint32_t msk[4];
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    msk[b] = ((r >> b) & 1) ? ~0u : 0u;
-                    asm volatile("" : "+v"(msk[b]));
-                }
-                auto sel = [](uint32_t m, uint32_t x1, uint32_t x0) { return (x1 & m) | (x0 & ~m); };
-                uint32_t a8[8], a4[4], a2[2];
-#pragma unroll
-                for (int i = 0; i < 8; ++i)
-                    a8[i] = sel(msk[0], __float_as_uint(acc[2 * i + 1]), __float_as_uint(acc[2 * i]));
-#pragma unroll
-                for (int i = 0; i < 4; ++i) a4[i] = sel(msk[1], a8[2 * i + 1], a8[2 * i]);
-#pragma unroll
-                for (int i = 0; i < 2; ++i) a2[i] = sel(msk[2], a4[2 * i + 1], a4[2 * i]);
-                const float u = __uint_as_float(sel(msk[3], a2[1], a2[0])) - h;
-                const int q = 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
-                const uint64_t key = ((uint64_t)orderable(-u) << 32) | (uint32_t)row;
+                const int q = query_of(t, r);
+                const uint64_t key = key_of(pick(acc, r) - h, row);
                 if (nst < (uint32_t)kPer) {
                     skey[nst * 64 + lane] = key;
                     sq[nst * 64 + lane] = (uint16_t)q;
@@ -210,6 +217,25 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
                 }
             }
         }
+    };
+    // acc - h >= 0 exactly when acc >= h (h is finite, and the difference of two distinct
+    // floats is never zero), so the mask needs no subtraction; the key takes the difference
+    auto scan = [&](const floatx16_t &acc, float h, bool valid, int64_t row, int t) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) m |= (valid && acc[r] >= h) ? (1u << r) : 0u;
+        drain(acc, m, h, row, t);
+    };
+    // one branch-free round of drain: the lowest set bit of m is staged when the lane has
+    // a free slot; otherwise the stores go to the lane's sink slot (kPer) and m keeps the bit
+    auto stage_one = [&](const floatx16_t &acc, uint32_t &m, float h, int64_t row, int t) {
+        const bool take = m != 0 && nst < (uint32_t)kPer;
+        const int r = __builtin_ctz(m | 0x10000u) & 15;
+        const uint32_t slot = (take ? nst : (uint32_t)kPer) * 64 + lane;
+        skey[slot] = key_of(pick(acc, r) - h, row);
+        sq[slot] = (uint16_t)query_of(t, r);
+        nst += take ? 1u : 0u;
+        m = take ? m & (m - 1) : m;
     };
     // ballot epilogue (kernels whose row fragments leave no registers for the masks): one
     // ballot per accumulator, slots of the wave's pool handed out by lane rank
@@ -230,33 +256,106 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
             filled += (uint32_t)__popcll(m);
         }
     };
-    g.sweep(
-        sm, qf, nqp / 32, wave, lane,
-        [&](int t, floatx16_t &pa, floatx16_t &pb) {
+    auto init = [&](int t, floatx16_t &pa, floatx16_t &pb) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float4 t4 = *(const float4 *)(tau_s + 32 * t + 8 * j + 4 * half);
-                pa[4 * j + 0] = tscale * t4.x;
-                pa[4 * j + 1] = tscale * t4.y;
-                pa[4 * j + 2] = tscale * t4.z;
-                pa[4 * j + 3] = tscale * t4.w;
+        for (int j = 0; j < 4; ++j) {
+            const float4 t4 = *(const float4 *)(tau_s + 32 * t + 8 * j + 4 * half);
+            pa[4 * j + 0] = tscale * t4.x;
+            pa[4 * j + 1] = tscale * t4.y;
+            pa[4 * j + 2] = tscale * t4.z;
+            pa[4 * j + 3] = tscale * t4.w;
+        }
+        pb = pa;
+    };
+    if constexpr (G::kPipelined) {
+        // software-pipelined sweep: tile t's MFMAs run into one accumulator pair while tile
+        // t - 1's threshold masks and two staging rounds per row tile, held in the other
+        // pair, are threaded between them; bits left over (a lane with more passes than
+        // rounds, or a full pool) go through drain after the tile
+        constexpr int KB = G::kBlocks, S = KB / 2, SM = (S + 1) / 2;
+        const int ntiles = nqp / 32;
+        const uint32_t vm0 = v0 ? 0xFFFFu : 0u, vm1 = v1 ? 0xFFFFu : 0u;
+        auto src = [&](int t) { return qf + (int64_t)t * KB * 512 + lane * 8; };
+        // tile t's column fragments landed (everyone's), its accumulators started and tile
+        // t + 1's prefetch issued into the buffer tile t - 1 used
+        auto begin = [&](int t, floatx16_t &x0, floatx16_t &x1) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            init(t, x0, x1);
+            if (t + 1 < ntiles) res_prefetch<KB>(src(t + 1), sm + ((t + 1) & 1) * KB * 512, wave);
+            return (const uint16_t *)(sm + (t & 1) * KB * 512 + lane * 8);
+        };
+        auto overlap = [&](int t, floatx16_t &x0, floatx16_t &x1, const floatx16_t &p0,
+                           const floatx16_t &p1) {
+            const uint16_t *buf = begin(t, x0, x1);
+            uint32_t m0 = 0, m1 = 0;
+            f8_tile_with<KB>(g.a0, g.a1, buf, x0, x1, [&](auto sc) {
+                constexpr int s = decltype(sc)::value;
+                if constexpr (s < SM) {
+                    // accumulators [32 s / SM, 32 (s + 1) / SM) of the two row tiles
+                    static_for<32 * s / SM, 32 * (s + 1) / SM>([&](auto ec) {
+                        constexpr int e = decltype(ec)::value, r = e & 15;
+                        if constexpr (e < 16) m0 |= p0[r] >= h0 ? (1u << r) : 0u;
+                        else m1 |= p1[r] >= h1 ? (1u << r) : 0u;
+                    });
+                    if constexpr (s == SM - 1) {
+                        m0 &= vm0;
+                        m1 &= vm1;
+                    }
+                } else {
+                    // staging rounds c = 0 .. 3 (row tile c & 1) spread over steps SM .. S - 1
+                    static_for<0, 4>([&](auto cc) {
+                        constexpr int c = decltype(cc)::value;
+                        if constexpr (SM + c * (S - SM) / 4 == s) {
+                            if constexpr ((c & 1) == 0) stage_one(p0, m0, h0, r0, t - 1);
+                            else stage_one(p1, m1, h1, r1, t - 1);
+                        }
+                    });
+                }
+            });
+            // the new chains complete before anything below reads them
+            asm volatile("" : "+a"(x0), "+a"(x1));
+            if (__ballot((m0 | m1) != 0)) {
+                drain(p0, m0, h0, r0, t - 1);
+                drain(p1, m1, h1, r1, t - 1);
             }
-            pb = pa;
-        },
-        [&](int t, floatx16_t &pa, floatx16_t &pb) {
+        };
+        floatx16_t pa, pb, qa, qb;
+        res_prefetch<KB>(src(0), sm, wave);
+        {
+            const uint16_t *buf = begin(0, pa, pb);
+            f8_tile<KB>(g.a0, g.a1, buf, pa, pb);
+            asm volatile("" : "+a"(pa), "+a"(pb));
+        }
+        int t = 1;
+        for (; t + 1 < ntiles; t += 2) {
+            overlap(t, qa, qb, pa, pb);
+            overlap(t + 1, pa, pb, qa, qb);
+        }
+        if (t < ntiles) {
+            overlap(t, qa, qb, pa, pb);
+            scan(qa, h0, v0, r0, t);
+            scan(qb, h1, v1, r1, t);
+        } else {
+            scan(pa, h0, v0, r0, t - 1);
+            scan(pb, h1, v1, r1, t - 1);
+        }
+    } else {
+        g.sweep(sm, qf, nqp / 32, wave, lane, init, [&](int t, floatx16_t &pa, floatx16_t &pb) {
             if constexpr (G::kMaskEpilogue) {
                 scan(pa, h0, v0, r0, t);
                 scan(pb, h1, v1, r1, t);
             } else {
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const int q = 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
+                    const int q = query_of(t, r);
                     const float u0 = pa[r] - h0, u1 = pb[r] - h1;
-                    push(v0 && u0 >= 0.0f, q, ((uint64_t)orderable(-u0) << 32) | (uint32_t)r0);
-                    push(v1 && u1 >= 0.0f, q, ((uint64_t)orderable(-u1) << 32) | (uint32_t)r1);
+                    push(v0 && u0 >= 0.0f, q, key_of(u0, r0));
+                    push(v1 && u1 >= 0.0f, q, key_of(u1, r1));
                 }
             }
         });
+    }
     // the staged pairs: all counter increments first, then the stores (one round trip); in
     // both layouts entry j of this lane sits at j * 64 + lane
     const uint32_t staged = min(filled, (uint32_t)kPool);
@@ -267,7 +366,8 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const uint32_t i = (uint32_t)(j * 64 + lane);
-        if (held(j)) pos[j] = atomicAdd(&count[qmap ? qmap[sq[i]] : sq[i]], 1u);
+        if (held(j))
+            pos[j] = atomicAdd(&count[(int64_t)(qmap ? qmap[sq[i]] : sq[i]) * PS_COUNT_STRIDE], 1u);
     }
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
@@ -304,7 +404,7 @@ __global__ __launch_bounds__(256) void ps_bounds_kernel(const uint32_t *count, i
     const int q = blockIdx.x * 256 + threadIdx.x;
     if (q >= nq) return;
     begin[q] = (uint32_t)(q * cap);
-    end[q] = (uint32_t)(q * cap + min((int64_t)count[q], cap));
+    end[q] = (uint32_t)(q * cap + min((int64_t)count[(int64_t)q * PS_COUNT_STRIDE], cap));
 }
 
 // First C rows of each sorted list -> candidate ids (count >= C is guaranteed by the host).
@@ -353,7 +453,8 @@ static hipError_t filter_t(const uint16_t *xf, int64_t n, const float *xn, const
                            int nq, int nqp, bool l2, float s, const float *tau,
                            const uint32_t *qmap, uint32_t *count, uint64_t *lists, int64_t cap,
                            hipStream_t st) {
-    const size_t lds = (size_t)2 * G::kBlocks * 1024 + 4 * (size_t)nqp + (size_t)4 * kPool * 10;
+    const size_t lds = (size_t)2 * G::kBlocks * 1024 + 4 * (size_t)nqp +
+                       (size_t)4 * pool_alloc(G::kPipelined) * 10;
     hipError_t e = hipFuncSetAttribute((const void *)ps_filter_kernel<G>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -385,7 +486,7 @@ static hipError_t filter_t(const uint16_t *xf, int64_t n, const float *xn, const
     }
 
 size_t ps_filter_lds(int dp, bool fp8, int nqp) {
-    return (size_t)(fp8 ? 1 : 2) * dp * 64 + 4 * (size_t)nqp + (size_t)4 * kPool * 10;
+    return (size_t)(fp8 ? 1 : 2) * dp * 64 + 4 * (size_t)nqp + (size_t)4 * pool_alloc(fp8) * 10;
 }
 
 bool ps_supported_dp(int dp) {

@@ -107,10 +107,13 @@ def test_small_candidate_counts(C, precision):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
-def test_many_queries_span_batches(precision):
-    # more queries than one filter launch holds (1024): the batches must not interfere
+@pytest.mark.parametrize("dim", [64, 768])
+def test_many_queries_span_batches(dim, precision):
+    # more queries than one filter launch holds (1024): the batches must not interfere; at
+    # 768 dims a full launch has the largest LDS footprint (column tiles, thresholds, pool)
+    log("prescore batches dim=%d precision=%d" % (dim, precision))
     rng = np.random.default_rng(3)
-    n, dim, nq, k, C = 3000, 64, 1500, 5, 100
+    n, nq, k, C = 3000, 1500, 5, 100
     x = rng.integers(-2, 3, (n, dim)).astype(np.float32)
     qs = rng.integers(-2, 3, (nq, dim)).astype(np.float32)
     ids, ds, _ = PreScorer(x, po.L2, precision).search(qs, k, C)

@@ -356,24 +356,38 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
             }
         });
     }
-    // the staged pairs: all counter increments first, then the stores (one round trip); in
-    // both layouts entry j of this lane sits at j * 64 + lane
+    // the staged pairs, counted per query in LDS first (the thresholds are no longer needed):
+    // one global atomic per (workgroup, query) reserves the query's range, and each pair
+    // lands at that base plus its rank among the workgroup's pairs of the query. In both
+    // layouts entry j of this lane sits at j * 64 + lane.
     const uint32_t staged = min(filled, (uint32_t)kPool);
     auto held = [&](int j) {
         return G::kMaskEpilogue ? (uint32_t)j < nst : (uint32_t)(j * 64 + lane) < staged;
     };
-    uint32_t pos[kPer];
+    uint32_t *qcnt = (uint32_t *)tau_s;
+    __syncthreads();
+    for (int c = tid; c < nqp; c += 256) qcnt[c] = 0;
+    __syncthreads();
+    uint32_t rank[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const uint32_t i = (uint32_t)(j * 64 + lane);
-        if (held(j))
-            pos[j] = atomicAdd(&count[(int64_t)(qmap ? qmap[sq[i]] : sq[i]) * PS_COUNT_STRIDE], 1u);
+        if (held(j)) rank[j] = atomicAdd(&qcnt[sq[i]], 1u);
     }
+    __syncthreads();
+    for (int c = tid; c < nqp; c += 256) {
+        const uint32_t k = qcnt[c];  // nonzero only for real queries (padding has tau -inf)
+        if (k) qcnt[c] = atomicAdd(&count[(int64_t)(qmap ? qmap[c] : c) * PS_COUNT_STRIDE], k);
+    }
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const uint32_t i = (uint32_t)(j * 64 + lane);
-        if (held(j) && pos[j] < cap)
-            lists[(int64_t)(qmap ? qmap[sq[i]] : sq[i]) * cap + pos[j]] = skey[i];
+        if (held(j)) {
+            const int q = sq[i];
+            const uint32_t pos = qcnt[q] + rank[j];
+            if (pos < cap) lists[(int64_t)(qmap ? qmap[q] : q) * cap + pos] = skey[i];
+        }
     }
 }
 

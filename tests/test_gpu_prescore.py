@@ -41,8 +41,11 @@ PRECISIONS = [pytest.param(BF16, id="bf16"), pytest.param(FP8, id="fp8")]
 
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("metric,dim", [(po.INNER_PRODUCT, 96), (po.L2, 37), (po.L2, 200),
+                                        (po.L2, 320), (po.INNER_PRODUCT, 500),
                                         (po.INNER_PRODUCT, 768)])
 def test_integer_data_equals_exhaustive(metric, dim, precision):
+    # padded dims 128 / 256 / 384 / 512 / 768: every operand width of both precisions (for
+    # FP8 each width is its own schedule of threshold work between the MFMAs)
     log("prescore integer metric=%d dim=%d precision=%d" % (metric, dim, precision))
     rng = np.random.default_rng(dim + metric)
     n, nq, k, C = 6000, 12, 10, 300

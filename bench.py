@@ -275,6 +275,8 @@ def prescore_workload(args):
                      "flops_per_launch": flops},
     }
     print(json.dumps(out), flush=True)
+    from diskann_amd import _native
+    profile_copy(_native.load(), args.profile_copy_gib)
 
 
 def kmeans_workload(args):

@@ -3332,9 +3332,13 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
     if (!ps || !d_q || !d_ids || !d_dists) return fail(DKM_ERR_INVALID_ARG, "null argument");
     if (k <= 0) return fail(DKM_ERR_K_ZERO, "k must be positive");
     if (nq < 0) return fail(DKM_ERR_INVALID_ARG, "bad query count");
-    const int64_t C = std::min<int64_t>(std::max<int64_t>(candidates, k), ps->n);
+    const int64_t C_req = std::min<int64_t>(std::max<int64_t>(candidates, k), ps->n);
     if (k > ps->n) return fail(DKM_ERR_INVALID_ARG, "k = %lld exceeds %lld rows", (long long)k,
                                (long long)ps->n);
+    // Tiny bases (1.5 C >= n): the filter passes every row with the same (infinite) key, so
+    // the first C of a list would be the first C rows by id; every row is rescored instead.
+    const bool all_pass = 1.5 * (double)C_req >= (double)ps->n;
+    const int64_t C = all_pass ? ps->n : C_req;
     if (nq == 0) return DKM_OK;
     std::lock_guard<std::mutex> g(ps->mu);
     HIP_TRY(hipSetDevice(g_device));
@@ -3356,7 +3360,6 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
     const int j_tight = (int)std::max<int64_t>(0, ((int64_t)j0 + 1) * 3 / 4 - 1);
     const int64_t cap = std::min<int64_t>(
         ps->n, std::max<int64_t>(8 * C, (int64_t)std::ceil(4.0 * (j0 + 1) * rows_per_sample)));
-    const bool all_pass = 1.5 * (double)C >= (double)ps->n;  // tiny bases: every row is a candidate
     DevBuf<uint16_t> qf;
     DevBuf<float> dense, tau, dexact;
     DevBuf<uint32_t> cnt, bnd_b, bnd_e, cids;

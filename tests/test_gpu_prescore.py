@@ -95,6 +95,22 @@ def test_recall_on_clustered_unit_vectors(precision):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
+def test_small_base_between_c_and_one_and_a_half_c(precision):
+    # C < n <= 1.5 C: every row passes the filter (with one common key), so the candidates
+    # must be every row, not the first C by id; the output is the exhaustive FP32 top k
+    log("prescore small base precision=%d" % precision)
+    rng = np.random.default_rng(11)
+    n, dim, nq, k, C = 1000, 64, 8, 10, 800
+    x = rng.standard_normal((n, dim)).astype(np.float32)
+    qs = rng.standard_normal((nq, dim)).astype(np.float32)
+    ids, ds, _ = PreScorer(x, po.L2, precision).search(qs, k, C)
+    check_rows(po.L2, x, qs, ids, ds)
+    for i in range(nq):
+        wi, wd = exact_topk(po.L2, x, qs[i], k)
+        assert np.array_equal(ids[i], wi) and ds[i].tobytes() == wd.tobytes(), i
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("C", [8, 40])
 def test_small_candidate_counts(C, precision):
     # few candidates against many rows: the sample threshold is taken at a floor rank and

@@ -3197,6 +3197,16 @@ struct dkm_prescorer {
     const float *x = nullptr;        // caller's f32 rows (rescore)
     DevBuf<uint16_t> xf, sf;         // fragment layout (bf16 or fp8): rows, sample rows
     DevBuf<float> xn, sn;            // f32 squared norms (L2 only)
+    // Work buffers of dkm_prescore_search, grown on demand and kept with the handle so a
+    // search allocates nothing once they fit (searches hold mu).
+    struct Work {
+        DevBuf<uint16_t> qf, r_qf;
+        DevBuf<float> dense, tau, dexact, r_q, r_tau;
+        DevBuf<uint32_t> cnt, bnd_b, bnd_e, cids, r_map, amax;
+        DevBuf<uint64_t> lists, sorted, fkeys, fsorted;
+        DevBuf<uint8_t> tmp;
+        DevBuf<int64_t> r_row;
+    } w;
     std::mutex mu;
 };
 
@@ -3211,8 +3221,7 @@ float fp8_scale(float m) {
     return std::ldexp(1.0f, e);
 }
 
-hipError_t absmax_host(const float *x, int64_t rows, int dim, float *m) {
-    DevBuf<uint32_t> d;
+hipError_t absmax_host(const float *x, int64_t rows, int dim, DevBuf<uint32_t> &d, float *m) {
     hipError_t e = d.ensure(1);
     if (e == hipSuccess) e = dkm::ps_absmax(x, rows, dim, dim, d.p, nullptr);
     uint32_t bits = 0;
@@ -3266,7 +3275,8 @@ dkm_status dkm_prescore_create(const float *d_x, int64_t n, int64_t dim, int met
     const int64_t npad = (n + 255) / 256 * 256, spad = (ps->ns + 255) / 256 * 256;
     if (fp8) {
         float m = 0.0f;
-        if ((e = absmax_host(d_x, n, (int)dim, &m)) != hipSuccess) return bad(e);
+        DevBuf<uint32_t> amax;
+        if ((e = absmax_host(d_x, n, (int)dim, amax, &m)) != hipSuccess) return bad(e);
         ps->sx = fp8_scale(m);
     }
     const int64_t per_row = fp8 ? dp / 2 : dp;  // 16-bit units per row
@@ -3360,15 +3370,16 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
     const int j_tight = (int)std::max<int64_t>(0, ((int64_t)j0 + 1) * 3 / 4 - 1);
     const int64_t cap = std::min<int64_t>(
         ps->n, std::max<int64_t>(8 * C, (int64_t)std::ceil(4.0 * (j0 + 1) * rows_per_sample)));
-    DevBuf<uint16_t> qf;
-    DevBuf<float> dense, tau, dexact;
-    DevBuf<uint32_t> cnt, bnd_b, bnd_e, cids;
-    DevBuf<uint64_t> lists, sorted, fkeys, fsorted;
-    DevBuf<uint8_t> tmp;
-    DevBuf<float> r_q, r_tau;  // redo launches
-    DevBuf<uint16_t> r_qf;
-    DevBuf<uint32_t> r_map;
-    DevBuf<int64_t> r_row;
+    auto &w = ps->w;
+    auto &qf = w.qf;
+    auto &dense = w.dense, &tau = w.tau, &dexact = w.dexact;
+    auto &cnt = w.cnt, &bnd_b = w.bnd_b, &bnd_e = w.bnd_e, &cids = w.cids;
+    auto &lists = w.lists, &sorted = w.sorted, &fkeys = w.fkeys, &fsorted = w.fsorted;
+    auto &tmp = w.tmp;
+    auto &r_q = w.r_q, &r_tau = w.r_tau;  // redo launches
+    auto &r_qf = w.r_qf;
+    auto &r_map = w.r_map;
+    auto &r_row = w.r_row;
     size_t tmp1 = 0, tmp2 = 0;
     HIP_TRY(dkm::ps_sort_tmp_bytes(B * cap, (int)B, &tmp1));
     HIP_TRY(dkm::ps_sort_tmp_bytes(B * C, (int)B, &tmp2));
@@ -3406,7 +3417,7 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
         float sq = 1.0f;
         if (ps->fp8) {
             float m = 0.0f;
-            HIP_TRY(absmax_host(q, b, (int)ps->dim, &m));
+            HIP_TRY(absmax_host(q, b, (int)ps->dim, ps->w.amax, &m));
             sq = fp8_scale(m);
         }
         const float S = ps->fp8 ? ps->sx * sq : 1.0f;

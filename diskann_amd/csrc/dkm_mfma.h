@@ -35,11 +35,13 @@ __device__ inline uint16_t to_bf16(float f) {
 // drains outstanding LDS reads before an M0 write).
 template <int KS>
 __device__ __forceinline__ void res_prefetch(const uint16_t *src, uint16_t *buf, int wave) {
+    // the wave index is uniform: as a scalar the LDS addresses (M0) need no readfirstlane
+    const int w = __builtin_amdgcn_readfirstlane(wave);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int s = 0; s < KS / 4; ++s)
-        __builtin_amdgcn_global_load_lds((global_void_t *)(src + (4 * s + wave) * 512),
-                                         (lds_void_t *)(buf + (4 * s + wave) * 512), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((global_void_t *)(src + (4 * s + w) * 512),
+                                         (lds_void_t *)(buf + (4 * s + w) * 512), 16, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
 }
 

@@ -588,7 +588,11 @@ dkm_status dkm_kmeans_assign_device(const float *d_x, int64_t n, int64_t dim, co
  * dkm_prescore_create keeps d_x (which must stay valid) and builds the bf16 / fp8 copy;
  * dkm_prescore_search takes device queries / outputs (nq x k) and, if ms_stages is not
  * NULL, reports the device time of four stages (approximate scoring with the candidate
- * filter, candidate selection, exact rescore, final top-k). Synchronous. */
+ * filter, candidate selection, exact rescore, final top-k). Synchronous; searches on one
+ * handle are serialised. The handle keeps its search work buffers (candidate lists of up
+ * to 1,024 queries x max(8 C, ...) keys, sort space, rescore arrays: ~600 MB at C = 4096)
+ * from the first search until dkm_prescore_destroy, so later searches allocate nothing.
+ * When 1.5 C >= n every row is rescored. */
 typedef struct dkm_prescorer dkm_prescorer;
 dkm_status dkm_prescore_create(const float *d_x, int64_t n, int64_t dim, int metric,
                                int precision, dkm_prescorer **out);

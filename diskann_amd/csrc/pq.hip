@@ -523,7 +523,10 @@ hipError_t pq_train_device(const float *data, int64_t n, int dim, const int32_t 
     int32_t *doff = nullptr;
     std::vector<float> cent;
     bool uniform = true;  // every chunk has width maxds
-    const int tile = (int)(((65536 / (4 + 4 * maxds)) / 256) * 256);
+    // Lloyd update row tile: (4 + 4 maxds) bytes a row within 64 KiB of LDS, a multiple of 64
+    // rows and never 0 (192 rows at the widest chunk, maxds = 64)
+    const int tile_fit = (65536 / (4 + 4 * maxds)) / 64 * 64;
+    const int tile = tile_fit > 64 ? tile_fit : 64;
     const size_t upd_lds = (sizeof(uint32_t) + sizeof(float) * maxds) * (size_t)tile;
     const size_t asg_lds = sizeof(float) * (size_t)K * (maxds + 1);
     const unsigned upd_y = (unsigned)((K * maxds + 255) / 256);
@@ -557,6 +560,9 @@ hipError_t pq_train_device(const float *data, int64_t n, int dim, const int32_t 
     }
     TRAIN_TRY(hipFuncSetAttribute((const void *)lloyd_update_kernel,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)upd_lds));
+    // the generic assignment stages K x (maxds + 1) floats: past 64 KiB at K 256, maxds 64
+    TRAIN_TRY(hipFuncSetAttribute((const void *)lloyd_assign_kernel<0>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)asg_lds));
     for (int m = 0; m < M; ++m) uniform = uniform && offsets[m + 1] - offsets[m] == maxds;
     for (int it = 0; it < iters; ++it) {
         const dim3 ag((unsigned)((n + 255) / 256), M);

@@ -30,6 +30,7 @@ def gmm(n, dim, comps, sigma, seed):
     (3000, 96, 12, 64, 3),          # dsub 8, K < 256
     (2000, 200, 4, 100, 2),         # dsub 50: generic assignment path
     (1500, 1536, 48, 256, 2),       # dsub 32 (config-5 dimension)
+    (1500, 128, 2, 256, 2),         # dsub 64: the widest chunk (Lloyd update row tile 192)
 ])
 def test_pq_train_matches_oracle(n, dim, M, K, iters):
     log("pq train n=%d dim=%d M=%d K=%d iters=%d" % (n, dim, M, K, iters))

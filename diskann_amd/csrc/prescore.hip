@@ -161,6 +161,10 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
     const int64_t rtile = (int64_t)blockIdx.x * 8 + 2 * wave;
     G g;
     g.load(xf + rtile * G::kBlocks * 512 + lane * 8);
+    // pipelined sweep: the first column tile's DMA goes out with the row loads, ahead of
+    // the threshold fill (whose LDS stores wait for every load issued before them)
+    if constexpr (G::kPipelined)
+        res_prefetch<G::kBlocks>(qf + lane * 8, sm, wave);
     for (int c = tid; c < nqp; c += 256) tau_s[c] = c < nq ? tau[c] : -__builtin_inff();
     const int64_t r0 = 32 * rtile + l32, r1 = r0 + 32;
     // the accumulators start at tau (L2: tau / 2), so after the tile they hold tau - d
@@ -321,7 +325,6 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
             }
         };
         floatx16_t pa, pb, qa, qb;
-        res_prefetch<KB>(src(0), sm, wave);
         {
             const uint16_t *buf = begin(0, pa, pb);
             f8_tile<KB>(g.a0, g.a1, buf, pa, pb);

@@ -100,14 +100,15 @@ __device__ __forceinline__ void res_load_rows(const uint16_t *ap, bf16x8_t *a0, 
 // Sweeps `ntiles` column tiles (cf = column fragments, tile t at cf + t * KS * 512) through
 // the double buffer at sm[0 .. 2 * KS * 512): init(t, acc0, acc1) before a tile's MFMAs,
 // epi(t, acc0, acc1) after (acc0: first row tile, acc1: second). The first barrier also
-// covers whatever the caller wrote to LDS before the call.
-template <int KS, class Init, class Epi>
+// covers whatever the caller wrote to LDS before the call. ISSUED: the caller already
+// issued tile 0's DMA (res_prefetch<KS>(cf + 8 lane, sm, wave)), e.g. with its row loads.
+template <int KS, bool ISSUED = false, class Init, class Epi>
 __device__ __forceinline__ void res_sweep(uint16_t *sm, const uint16_t *cf, int ntiles, int wave,
                                           int lane, const bf16x8_t *a0, const bf16x8_t *a1,
                                           Init init, Epi epi) {
     auto src = [&](int t) { return cf + (int64_t)t * KS * 512 + lane * 8; };
     floatx16_t pa, pb;
-    res_prefetch<KS>(src(0), sm, wave);
+    if constexpr (!ISSUED) res_prefetch<KS>(src(0), sm, wave);
     for (int t = 0; t < ntiles; ++t) {
         __builtin_amdgcn_s_waitcnt(0);  // own loads of tile t landed
         __syncthreads();                // everyone's, and tile t - 1 fully read
@@ -198,10 +199,10 @@ struct Bf16Rows {
     static constexpr bool kPipelined = false;
     bf16x8_t a0[KS], a1[KS];
     __device__ __forceinline__ void load(const uint16_t *ap) { res_load_rows<KS>(ap, a0, a1); }
-    template <class Init, class Epi>
+    template <bool ISSUED = false, class Init, class Epi>
     __device__ __forceinline__ void sweep(uint16_t *sm, const uint16_t *cf, int ntiles, int wave,
                                           int lane, Init init, Epi epi) {
-        res_sweep<KS>(sm, cf, ntiles, wave, lane, a0, a1, init, epi);
+        res_sweep<KS, ISSUED>(sm, cf, ntiles, wave, lane, a0, a1, init, epi);
     }
 };
 
@@ -224,12 +225,12 @@ struct Fp8Rows {
 #pragma unroll
         for (int s = 0; s < KB / 2; ++s) asm volatile("" : "+a"(a0[s]));
     }
-    template <class Init, class Epi>
+    template <bool ISSUED = false, class Init, class Epi>
     __device__ __forceinline__ void sweep(uint16_t *sm, const uint16_t *cf, int ntiles, int wave,
                                           int lane, Init init, Epi epi) {
         auto src = [&](int t) { return cf + (int64_t)t * KB * 512 + lane * 8; };
         floatx16_t pa, pb;
-        res_prefetch<KB>(src(0), sm, wave);
+        if constexpr (!ISSUED) res_prefetch<KB>(src(0), sm, wave);
         for (int t = 0; t < ntiles; ++t) {
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();

@@ -268,10 +268,13 @@ __global__ __launch_bounds__(256, 1) void assign_mfma_res_kernel(const uint16_t 
     const int64_t rtile = (int64_t)blockIdx.x * 8 + 2 * wave;
     bf16x8_t a0[KS], a1[KS];
     res_load_rows<KS>(xf + rtile * KS * 512 + lane * 8, a0, a1);
+    // the first centre tile's DMA goes out with the row loads, ahead of the norm fill (whose
+    // LDS stores wait for every load issued before them)
+    res_prefetch<KS>(cf + lane * 8, sm, wave);
     for (int c = tid; c < Kp; c += 256) cn_s[c] = c < K ? cnorm[c] : __builtin_inff();
     float bs0 = -__builtin_inff(), bs1 = -__builtin_inff();
     uint32_t bc0 = 0xFFFFFFFFu, bc1 = 0xFFFFFFFFu;
-    res_sweep<KS>(
+    res_sweep<KS, true>(
         sm, cf, Kp / 32, wave, lane, a0, a1,
         [&](int t, floatx16_t &pa, floatx16_t &pb) { res_init(cn_s, 32 * t + 4 * half, pa, pb); },
         [&](int t, floatx16_t &pa, floatx16_t &pb) {

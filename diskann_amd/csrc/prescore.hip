@@ -161,10 +161,9 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
     const int64_t rtile = (int64_t)blockIdx.x * 8 + 2 * wave;
     G g;
     g.load(xf + rtile * G::kBlocks * 512 + lane * 8);
-    // pipelined sweep: the first column tile's DMA goes out with the row loads, ahead of
-    // the threshold fill (whose LDS stores wait for every load issued before them)
-    if constexpr (G::kPipelined)
-        res_prefetch<G::kBlocks>(qf + lane * 8, sm, wave);
+    // the first column tile's DMA goes out with the row loads, ahead of the threshold fill
+    // (whose LDS stores wait for every load issued before them)
+    res_prefetch<G::kBlocks>(qf + lane * 8, sm, wave);
     for (int c = tid; c < nqp; c += 256) tau_s[c] = c < nq ? tau[c] : -__builtin_inff();
     const int64_t r0 = 32 * rtile + l32, r1 = r0 + 32;
     // the accumulators start at tau (L2: tau / 2), so after the tile they hold tau - d
@@ -344,7 +343,8 @@ __global__ __launch_bounds__(256, 1) void ps_filter_kernel(const uint16_t *xf, i
             scan(pb, h1, v1, r1, t - 1);
         }
     } else {
-        g.sweep(sm, qf, nqp / 32, wave, lane, init, [&](int t, floatx16_t &pa, floatx16_t &pb) {
+        g.template sweep<true>(sm, qf, nqp / 32, wave, lane, init,
+                               [&](int t, floatx16_t &pa, floatx16_t &pb) {
             if constexpr (G::kMaskEpilogue) {
                 scan(pa, h0, v0, r0, t);
                 scan(pb, h1, v1, r1, t);

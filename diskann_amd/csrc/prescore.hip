@@ -90,32 +90,48 @@ __global__ __launch_bounds__(256) void ps_select_kernel(const float *dense, int6
     __shared__ uint32_t sel[2];
     const int q = blockIdx.x, tid = threadIdx.x;
     for (int i = tid; i < ns; i += 256) keys[i] = orderable(dense[(int64_t)q * ld + i]);
+    // the three ranks descend byte by byte together; a rank whose prefix equals the previous
+    // rank's reuses that level's histogram (the ranks are close, so the heavily contended
+    // top-byte histogram is usually built once instead of three times)
     const int js[3] = {j0, j1, j2};
-    for (int w = 0; w < 3; ++w) {
-        uint32_t prefix = 0, mask = 0;
-        uint32_t want = (uint32_t)min(max(js[w], 0), ns - 1);
-        for (int shift = 24; shift >= 0; shift -= 8) {
-            hist[tid] = 0;
-            __syncthreads();
-            for (int i = tid; i < ns; i += 256)
-                if ((keys[i] & mask) == prefix) atomicAdd(&hist[(keys[i] >> shift) & 255u], 1u);
-            __syncthreads();
+    uint32_t prefix[3] = {0, 0, 0}, want[3];
+#pragma unroll
+    for (int w = 0; w < 3; ++w) want[w] = (uint32_t)min(max(js[w], 0), ns - 1);
+    uint32_t mask = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        bool have = false;   // hist holds this level's histogram of the keys matching `built`
+        uint32_t built = 0;  // (uniform: every thread derives it from the same broadcasts)
+#pragma unroll
+        for (int w = 0; w < 3; ++w) {
+            if (!have || prefix[w] != built) {
+                hist[tid] = 0;
+                __syncthreads();
+                for (int i = tid; i < ns; i += 256)
+                    if ((keys[i] & mask) == prefix[w])
+                        atomicAdd(&hist[(keys[i] >> shift) & 255u], 1u);
+                __syncthreads();
+                built = prefix[w];
+                have = true;
+            }
             if (tid == 0) {
                 uint32_t cum = 0, d = 0;
                 for (; d < 255; ++d) {
-                    if (cum + hist[d] > want) break;
+                    if (cum + hist[d] > want[w]) break;
                     cum += hist[d];
                 }
                 sel[0] = d;
-                sel[1] = want - cum;
+                sel[1] = want[w] - cum;
             }
             __syncthreads();
-            prefix |= sel[0] << shift;
-            mask |= 255u << shift;
-            want = sel[1];
-            __syncthreads();
+            prefix[w] |= sel[0] << shift;
+            want[w] = sel[1];
+            __syncthreads();  // sel and hist are read before they are written again
         }
-        if (tid == 0) tau[w * (int64_t)nq + q] = from_orderable(prefix);
+        mask |= 255u << shift;
+    }
+    if (tid == 0) {
+#pragma unroll
+        for (int w = 0; w < 3; ++w) tau[w * (int64_t)nq + q] = from_orderable(prefix[w]);
     }
 }
 

@@ -88,7 +88,8 @@ __global__ __launch_bounds__(256) void ps_select_kernel(const float *dense, int6
     __shared__ uint32_t keys[kSampleMax];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t sel[2];
-    const int q = blockIdx.x, tid = threadIdx.x;
+    __shared__ uint32_t wsum[4];
+    const int q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < ns; i += 256) keys[i] = orderable(dense[(int64_t)q * ld + i]);
     // the three ranks descend byte by byte together; a rank whose prefix equals the previous
     // rank's reuses that level's histogram (the ranks are close, so the heavily contended
@@ -113,14 +114,22 @@ __global__ __launch_bounds__(256) void ps_select_kernel(const float *dense, int6
                 built = prefix[w];
                 have = true;
             }
-            if (tid == 0) {
-                uint32_t cum = 0, d = 0;
-                for (; d < 255; ++d) {
-                    if (cum + hist[d] > want[w]) break;
-                    cum += hist[d];
-                }
-                sel[0] = d;
-                sel[1] = want[w] - cum;
+            // the first bin d whose running count passes want (bin 255 if none does): thread
+            // t owns bin t and takes it when want lies in [count before t, count through t)
+            const uint32_t h = hist[tid];
+            uint32_t incl = h;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t v = __shfl_up(incl, o);
+                if (lane >= o) incl += v;
+            }
+            if (lane == 63) wsum[wave] = incl;
+            __syncthreads();
+            for (int k = 0; k < wave; ++k) incl += wsum[k];
+            const uint32_t excl = incl - h;
+            if ((excl <= want[w] && want[w] < incl) || (tid == 255 && incl <= want[w])) {
+                sel[0] = (uint32_t)tid;
+                sel[1] = want[w] - excl;
             }
             __syncthreads();
             prefix[w] |= sel[0] << shift;

@@ -3203,7 +3203,7 @@ struct dkm_prescorer {
         DevBuf<uint16_t> qf, r_qf;
         DevBuf<float> dense, tau, dexact, r_q, r_tau;
         DevBuf<uint32_t> cnt, bnd_b, bnd_e, cids, r_map, amax;
-        DevBuf<uint64_t> lists, sorted, fkeys, fsorted;
+        DevBuf<uint64_t> lists, fkeys, fsorted;
         DevBuf<uint8_t> tmp;
         DevBuf<int64_t> r_row;
     } w;
@@ -3374,14 +3374,13 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
     auto &qf = w.qf;
     auto &dense = w.dense, &tau = w.tau, &dexact = w.dexact;
     auto &cnt = w.cnt, &bnd_b = w.bnd_b, &bnd_e = w.bnd_e, &cids = w.cids;
-    auto &lists = w.lists, &sorted = w.sorted, &fkeys = w.fkeys, &fsorted = w.fsorted;
+    auto &lists = w.lists, &fkeys = w.fkeys, &fsorted = w.fsorted;
     auto &tmp = w.tmp;
     auto &r_q = w.r_q, &r_tau = w.r_tau;  // redo launches
     auto &r_qf = w.r_qf;
     auto &r_map = w.r_map;
     auto &r_row = w.r_row;
-    size_t tmp1 = 0, tmp2 = 0;
-    HIP_TRY(dkm::ps_sort_tmp_bytes(B * cap, (int)B, &tmp1));
+    size_t tmp2 = 0;
     HIP_TRY(dkm::ps_sort_tmp_bytes(B * C, (int)B, &tmp2));
     HIP_TRY(qf.ensure(Bp * dp));
     HIP_TRY(dense.ensure(B * ps->ns));
@@ -3390,12 +3389,11 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
     HIP_TRY(bnd_b.ensure(B));
     HIP_TRY(bnd_e.ensure(B));
     HIP_TRY(lists.ensure(B * cap));
-    HIP_TRY(sorted.ensure(B * cap));
     HIP_TRY(cids.ensure(B * C));
     HIP_TRY(dexact.ensure(B * C));
     HIP_TRY(fkeys.ensure(B * C));
     HIP_TRY(fsorted.ensure(B * C));
-    HIP_TRY(tmp.ensure(std::max<size_t>(std::max(tmp1, tmp2), 1)));
+    HIP_TRY(tmp.ensure(std::max<size_t>(tmp2, 1)));
     hipEvent_t ev[5];
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
     struct EvGuard {
@@ -3479,10 +3477,7 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
                                    r_tau.p, r_map.p, cnt.p, lists.p, cap, nullptr));
         }
         HIP_TRY(hipEventRecord(ev[1], nullptr));
-        HIP_TRY(dkm::ps_bounds(cnt.p, b, cap, bnd_b.p, bnd_e.p, nullptr));
-        HIP_TRY(dkm::ps_sort(tmp.p, tmp1, lists.p, sorted.p, (int64_t)b * cap, b, bnd_b.p,
-                             bnd_e.p, nullptr));
-        HIP_TRY(dkm::ps_take(sorted.p, b, cap, (int)C, cids.p, nullptr));
+        HIP_TRY(dkm::ps_take_smallest(lists.p, cnt.p, b, cap, (int)C, cids.p, nullptr));
         HIP_TRY(hipEventRecord(ev[2], nullptr));
         HIP_TRY(dkm::launch_distance_batch(ps->metric, (int)ps->dim, q, b, ps->x, ps->dim, cids.p,
                                            C, dexact.p, nullptr));

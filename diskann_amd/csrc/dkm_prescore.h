@@ -38,10 +38,10 @@ hipError_t ps_gather_rows(const float *x, int dim, const int64_t *rows, int64_t 
 hipError_t ps_sort_tmp_bytes(int64_t total, int segments, size_t *bytes);
 hipError_t ps_sort(void *tmp, size_t tmp_bytes, const uint64_t *in, uint64_t *out, int64_t total,
                    int segments, const uint32_t *begin, const uint32_t *end, hipStream_t st);
-hipError_t ps_bounds(const uint32_t *count, int nq, int64_t cap, uint32_t *begin, uint32_t *end,
-                     hipStream_t st);
-hipError_t ps_take(const uint64_t *sorted, int nq, int64_t cap, int C, uint32_t *ids,
-                   hipStream_t st);
+// Rows of the C smallest keys of each query's list (count[q * PS_COUNT_STRIDE] entries at
+// lists + q * cap, count >= C), in no particular order.
+hipError_t ps_take_smallest(const uint64_t *lists, const uint32_t *count, int nq, int64_t cap,
+                            int C, uint32_t *ids, hipStream_t st);
 hipError_t ps_final_keys(const float *d, const uint32_t *ids, int64_t total, uint64_t *keys,
                          hipStream_t st);
 hipError_t ps_output(const uint64_t *sorted, int nq, int C, int k, uint32_t *ids, float *dists,

@@ -440,22 +440,68 @@ __global__ __launch_bounds__(256) void ps_gather_rows_kernel(const float *x, int
     out[e] = x[rows[i] * dim + (e - i * dim)];
 }
 
-// Segment bounds of the candidate lists and of the rescored lists.
-__global__ __launch_bounds__(256) void ps_bounds_kernel(const uint32_t *count, int nq, int64_t cap,
-                                                        uint32_t *begin, uint32_t *end) {
-    const int q = blockIdx.x * 256 + threadIdx.x;
-    if (q >= nq) return;
-    begin[q] = (uint32_t)(q * cap);
-    end[q] = (uint32_t)(q * cap + min((int64_t)count[(int64_t)q * PS_COUNT_STRIDE], cap));
-}
-
-// First C rows of each sorted list -> candidate ids (count >= C is guaranteed by the host).
-__global__ __launch_bounds__(256) void ps_take_kernel(const uint64_t *sorted, int nq, int64_t cap,
-                                                      int C, uint32_t *ids) {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= (int64_t)nq * C) return;
-    const int64_t q = e / C, j = e - q * C;
-    ids[e] = (uint32_t)(sorted[q * cap + j] & 0xFFFFFFFFull);
+// The rows of the C smallest keys of each query's list, in no particular order (the rescore
+// that follows is re-sorted by exact distance): keys are distinct (the row is in the low 32
+// bits), so an 8-bit radix select over the 64-bit keys finds the C-th smallest and exactly
+// C keys lie at or below it -- the same set as the first C of the sorted list. One
+// workgroup per query; the host guarantees count >= C (lists of exactly C are taken whole).
+__global__ __launch_bounds__(256) void ps_take_smallest_kernel(const uint64_t *lists,
+                                                               const uint32_t *count, int64_t cap,
+                                                               int C, uint32_t *ids) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t sel[2];
+    __shared__ uint32_t fill;
+    const int q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t *l = lists + (int64_t)q * cap;
+    const int64_t nl = min((int64_t)count[(int64_t)q * PS_COUNT_STRIDE], cap);
+    uint32_t *out = ids + (int64_t)q * C;
+    uint64_t kth = ~0ull;
+    if (nl > C) {
+        uint64_t prefix = 0, mask = 0;
+        uint32_t want = (uint32_t)(C - 1);
+        for (int shift = 56; shift >= 0; shift -= 8) {
+            hist[tid] = 0;
+            __syncthreads();
+            for (int64_t i = tid; i < nl; i += 256) {
+                const uint64_t k = l[i];
+                if ((k & mask) == prefix) atomicAdd(&hist[(uint32_t)(k >> shift) & 255u], 1u);
+            }
+            __syncthreads();
+            // the bin holding rank `want` (as in ps_select_kernel: want is always below the
+            // count of keys matching the prefix)
+            const uint32_t h = hist[tid];
+            uint32_t incl = h;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t v = __shfl_up(incl, o);
+                if (lane >= o) incl += v;
+            }
+            if (lane == 63) wsum[wave] = incl;
+            __syncthreads();
+            for (int k = 0; k < wave; ++k) incl += wsum[k];
+            const uint32_t excl = incl - h;
+            if ((excl <= want && want < incl) || (tid == 255 && incl <= want)) {
+                sel[0] = (uint32_t)tid;
+                sel[1] = want - excl;
+            }
+            __syncthreads();
+            prefix |= (uint64_t)sel[0] << shift;
+            mask |= 255ull << shift;
+            want = sel[1];
+            __syncthreads();
+        }
+        kth = prefix;
+    }
+    if (tid == 0) fill = 0;
+    __syncthreads();
+    for (int64_t i = tid; i < nl; i += 256) {
+        const uint64_t k = l[i];
+        if (k <= kth) {
+            const uint32_t pos = atomicAdd(&fill, 1u);
+            if (pos < (uint32_t)C) out[pos] = (uint32_t)(k & 0xFFFFFFFFull);
+        }
+    }
 }
 
 // (orderable exact distance << 32 | row) for the final selection.
@@ -655,17 +701,10 @@ hipError_t ps_sort(void *tmp, size_t tmp_bytes, const uint64_t *in, uint64_t *ou
                                               (unsigned)segments, begin, end, 0, 64, st);
 }
 
-hipError_t ps_bounds(const uint32_t *count, int nq, int64_t cap, uint32_t *begin, uint32_t *end,
-                     hipStream_t st) {
-    hipLaunchKernelGGL(ps_bounds_kernel, dim3(blocks(nq, 256)), dim3(256), 0, st, count, nq, cap,
-                       begin, end);
-    return hipGetLastError();
-}
-
-hipError_t ps_take(const uint64_t *sorted, int nq, int64_t cap, int C, uint32_t *ids,
-                   hipStream_t st) {
-    hipLaunchKernelGGL(ps_take_kernel, dim3(blocks((int64_t)nq * C, 256)), dim3(256), 0, st, sorted,
-                       nq, cap, C, ids);
+hipError_t ps_take_smallest(const uint64_t *lists, const uint32_t *count, int nq, int64_t cap,
+                            int C, uint32_t *ids, hipStream_t st) {
+    hipLaunchKernelGGL(ps_take_smallest_kernel, dim3(nq), dim3(256), 0, st, lists, count, cap, C,
+                       ids);
     return hipGetLastError();
 }
 

@@ -590,7 +590,8 @@ dkm_status dkm_kmeans_assign_device(const float *d_x, int64_t n, int64_t dim, co
  * NULL, reports the device time of four stages (approximate scoring with the candidate
  * filter, candidate selection, exact rescore, final top-k). Synchronous; searches on one
  * handle are serialised. The handle keeps its search work buffers (candidate lists of up
- * to 1,024 queries x max(8 C, ...) keys, sort space, rescore arrays: ~600 MB at C = 4096)
+ * to 1,024 queries x max(8 C, ...) keys, sample scores, rescore arrays, sort space: about
+ * 0.5 GB at C = 4096)
  * from the first search until dkm_prescore_destroy, so later searches allocate nothing.
  * When 1.5 C >= n every row is rescored. */
 typedef struct dkm_prescorer dkm_prescorer;

@@ -3483,17 +3483,22 @@ dkm_status dkm_prescore_search(dkm_prescorer *ps, const float *d_q, int64_t nq, 
                                            C, dexact.p, nullptr));
         HIP_TRY(hipEventRecord(ev[3], nullptr));
         HIP_TRY(dkm::ps_final_keys(dexact.p, cids.p, (int64_t)b * C, fkeys.p, nullptr));
-        std::vector<uint32_t> fb(b), fe(b);
-        for (int i = 0; i < b; ++i) {
-            fb[i] = (uint32_t)(i * C);
-            fe[i] = (uint32_t)((i + 1) * C);
+        if (dkm::ps_topk_supported((int)k)) {
+            HIP_TRY(dkm::ps_topk(fkeys.p, b, (int)C, (int)k, d_ids + q0 * k, d_dists + q0 * k,
+                                 nullptr));
+        } else {
+            std::vector<uint32_t> fb(b), fe(b);
+            for (int i = 0; i < b; ++i) {
+                fb[i] = (uint32_t)(i * C);
+                fe[i] = (uint32_t)((i + 1) * C);
+            }
+            HIP_TRY(hipMemcpy(bnd_b.p, fb.data(), sizeof(uint32_t) * b, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(bnd_e.p, fe.data(), sizeof(uint32_t) * b, hipMemcpyHostToDevice));
+            HIP_TRY(dkm::ps_sort(tmp.p, tmp2, fkeys.p, fsorted.p, (int64_t)b * C, b, bnd_b.p,
+                                 bnd_e.p, nullptr));
+            HIP_TRY(dkm::ps_output(fsorted.p, b, (int)C, (int)k, d_ids + q0 * k,
+                                   d_dists + q0 * k, nullptr));
         }
-        HIP_TRY(hipMemcpy(bnd_b.p, fb.data(), sizeof(uint32_t) * b, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(bnd_e.p, fe.data(), sizeof(uint32_t) * b, hipMemcpyHostToDevice));
-        HIP_TRY(dkm::ps_sort(tmp.p, tmp2, fkeys.p, fsorted.p, (int64_t)b * C, b, bnd_b.p, bnd_e.p,
-                             nullptr));
-        HIP_TRY(dkm::ps_output(fsorted.p, b, (int)C, (int)k, d_ids + q0 * k, d_dists + q0 * k,
-                               nullptr));
         HIP_TRY(hipEventRecord(ev[4], nullptr));
         HIP_TRY(hipEventSynchronize(ev[4]));
         for (int i = 0; i < 4; ++i) {

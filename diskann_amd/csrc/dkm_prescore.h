@@ -46,4 +46,9 @@ hipError_t ps_final_keys(const float *d, const uint32_t *ids, int64_t total, uin
                          hipStream_t st);
 hipError_t ps_output(const uint64_t *sorted, int nq, int C, int k, uint32_t *ids, float *dists,
                      hipStream_t st);
+// Final top k (k <= 1024) of each query's C rescored keys, sorted: the same output as a
+// segmented sort of the keys followed by ps_output.
+bool ps_topk_supported(int k);
+hipError_t ps_topk(const uint64_t *keys, int nq, int C, int k, uint32_t *ids, float *dists,
+                   hipStream_t st);
 }  // namespace dkm

@@ -440,11 +440,51 @@ __global__ __launch_bounds__(256) void ps_gather_rows_kernel(const float *x, int
     out[e] = x[rows[i] * dim + (e - i * dim)];
 }
 
+// The key of rank `want` (0-based, ascending) among the nl distinct keys at l, by an 8-bit
+// radix select over the 64 bits; called by all 256 threads of the block, which all get it.
+// want < nl is required (it keeps want below the count of keys matching each prefix).
+__device__ uint64_t block_kth_key(const uint64_t *l, int64_t nl, uint32_t want, uint32_t *hist,
+                                  uint32_t *wsum, uint32_t *sel) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint64_t prefix = 0, mask = 0;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        hist[tid] = 0;
+        __syncthreads();
+        for (int64_t i = tid; i < nl; i += 256) {
+            const uint64_t k = l[i];
+            if ((k & mask) == prefix) atomicAdd(&hist[(uint32_t)(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        // the bin holding rank want: thread t owns bin t (as in ps_select_kernel)
+        const uint32_t h = hist[tid];
+        uint32_t incl = h;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        for (int k = 0; k < wave; ++k) incl += wsum[k];
+        const uint32_t excl = incl - h;
+        if ((excl <= want && want < incl) || (tid == 255 && incl <= want)) {
+            sel[0] = (uint32_t)tid;
+            sel[1] = want - excl;
+        }
+        __syncthreads();
+        prefix |= (uint64_t)sel[0] << shift;
+        mask |= 255ull << shift;
+        want = sel[1];
+        __syncthreads();
+    }
+    return prefix;
+}
+
 // The rows of the C smallest keys of each query's list, in no particular order (the rescore
 // that follows is re-sorted by exact distance): keys are distinct (the row is in the low 32
-// bits), so an 8-bit radix select over the 64-bit keys finds the C-th smallest and exactly
-// C keys lie at or below it -- the same set as the first C of the sorted list. One
-// workgroup per query; the host guarantees count >= C (lists of exactly C are taken whole).
+// bits), so exactly C keys lie at or below the C-th smallest -- the same set as the first C
+// of the sorted list. One workgroup per query; the host guarantees count >= C (lists of
+// exactly C are taken whole).
 __global__ __launch_bounds__(256) void ps_take_smallest_kernel(const uint64_t *lists,
                                                                const uint32_t *count, int64_t cap,
                                                                int C, uint32_t *ids) {
@@ -452,47 +492,11 @@ __global__ __launch_bounds__(256) void ps_take_smallest_kernel(const uint64_t *l
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t sel[2];
     __shared__ uint32_t fill;
-    const int q = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int q = blockIdx.x, tid = threadIdx.x;
     const uint64_t *l = lists + (int64_t)q * cap;
     const int64_t nl = min((int64_t)count[(int64_t)q * PS_COUNT_STRIDE], cap);
     uint32_t *out = ids + (int64_t)q * C;
-    uint64_t kth = ~0ull;
-    if (nl > C) {
-        uint64_t prefix = 0, mask = 0;
-        uint32_t want = (uint32_t)(C - 1);
-        for (int shift = 56; shift >= 0; shift -= 8) {
-            hist[tid] = 0;
-            __syncthreads();
-            for (int64_t i = tid; i < nl; i += 256) {
-                const uint64_t k = l[i];
-                if ((k & mask) == prefix) atomicAdd(&hist[(uint32_t)(k >> shift) & 255u], 1u);
-            }
-            __syncthreads();
-            // the bin holding rank `want` (as in ps_select_kernel: want is always below the
-            // count of keys matching the prefix)
-            const uint32_t h = hist[tid];
-            uint32_t incl = h;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t v = __shfl_up(incl, o);
-                if (lane >= o) incl += v;
-            }
-            if (lane == 63) wsum[wave] = incl;
-            __syncthreads();
-            for (int k = 0; k < wave; ++k) incl += wsum[k];
-            const uint32_t excl = incl - h;
-            if ((excl <= want && want < incl) || (tid == 255 && incl <= want)) {
-                sel[0] = (uint32_t)tid;
-                sel[1] = want - excl;
-            }
-            __syncthreads();
-            prefix |= (uint64_t)sel[0] << shift;
-            mask |= 255ull << shift;
-            want = sel[1];
-            __syncthreads();
-        }
-        kth = prefix;
-    }
+    const uint64_t kth = nl > C ? block_kth_key(l, nl, (uint32_t)(C - 1), hist, wsum, sel) : ~0ull;
     if (tid == 0) fill = 0;
     __syncthreads();
     for (int64_t i = tid; i < nl; i += 256) {
@@ -501,6 +505,54 @@ __global__ __launch_bounds__(256) void ps_take_smallest_kernel(const uint64_t *l
             const uint32_t pos = atomicAdd(&fill, 1u);
             if (pos < (uint32_t)C) out[pos] = (uint32_t)(k & 0xFFFFFFFFull);
         }
+    }
+}
+
+// Final top k of each query's C rescored keys (orderable exact distance << 32 | row, distinct):
+// the k smallest by radix select, then sorted in LDS (bitonic over the next power of two,
+// padded with ~0) and written as (row, distance) -- the first k of the sorted keys. k <= 1024.
+constexpr int kTopkMax = 1024;
+__global__ __launch_bounds__(256) void ps_topk_kernel(const uint64_t *keys, int C, int k,
+                                                      uint32_t *ids, float *dists) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t sel[2];
+    __shared__ uint32_t fill;
+    __shared__ uint64_t srt[kTopkMax];
+    const int q = blockIdx.x, tid = threadIdx.x;
+    const uint64_t *l = keys + (int64_t)q * C;
+    const uint64_t kth = k < C ? block_kth_key(l, C, (uint32_t)(k - 1), hist, wsum, sel) : ~0ull;
+    int P = 1;
+    while (P < k) P <<= 1;
+    for (int i = tid; i < P; i += 256) srt[i] = ~0ull;
+    if (tid == 0) fill = 0;
+    __syncthreads();
+    for (int i = tid; i < C; i += 256) {
+        const uint64_t key = l[i];
+        if (key <= kth) {
+            const uint32_t pos = atomicAdd(&fill, 1u);
+            if (pos < (uint32_t)k) srt[pos] = key;
+        }
+    }
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = tid; i < P / 2; i += 256) {
+                const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const uint64_t a = srt[lo], b = srt[hi];
+                if ((a > b) == up) {
+                    srt[lo] = b;
+                    srt[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int j = tid; j < k; j += 256) {
+        const uint64_t key = srt[j];
+        ids[(int64_t)q * k + j] = (uint32_t)(key & 0xFFFFFFFFull);
+        dists[(int64_t)q * k + j] = from_orderable((uint32_t)(key >> 32));
     }
 }
 
@@ -712,6 +764,15 @@ hipError_t ps_final_keys(const float *d, const uint32_t *ids, int64_t total, uin
                          hipStream_t st) {
     hipLaunchKernelGGL(ps_final_keys_kernel, dim3(blocks(total, 256)), dim3(256), 0, st, d, ids,
                        total, keys);
+    return hipGetLastError();
+}
+
+bool ps_topk_supported(int k) { return k >= 1 && k <= kTopkMax; }
+
+hipError_t ps_topk(const uint64_t *keys, int nq, int C, int k, uint32_t *ids, float *dists,
+                   hipStream_t st) {
+    if (!ps_topk_supported(k) || k > C) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ps_topk_kernel, dim3(nq), dim3(256), 0, st, keys, C, k, ids, dists);
     return hipGetLastError();
 }
 

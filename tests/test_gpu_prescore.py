@@ -111,6 +111,22 @@ def test_small_base_between_c_and_one_and_a_half_c(precision):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("k", [1, 1024, 1100])
+def test_large_k_equals_exhaustive(k, precision):
+    # the final top k is sorted in LDS up to k = 1024 and by a segmented sort beyond
+    log("prescore k=%d precision=%d" % (k, precision))
+    rng = np.random.default_rng(k)
+    n, dim, nq, C = 6000, 32, 3, 1500
+    x = rng.integers(-3, 4, (n, dim)).astype(np.float32)
+    qs = rng.integers(-3, 4, (nq, dim)).astype(np.float32)
+    ids, ds, _ = PreScorer(x, po.L2, precision).search(qs, k, C)
+    check_rows(po.L2, x, qs, ids, ds)
+    for i in range(nq):
+        wi, wd = exact_topk(po.L2, x, qs[i], k)
+        assert np.array_equal(ids[i], wi) and ds[i].tobytes() == wd.tobytes(), i
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("C", [8, 40])
 def test_small_candidate_counts(C, precision):
     # few candidates against many rows: the sample threshold is taken at a floor rank and
